@@ -1,0 +1,217 @@
+/*
+ * diffmpm_amd — C ABI of the MI355X (gfx950) explicit MPM time step.
+ *
+ * The reference (geoelements/diffmpm) has no FFI: its step is a set of Python
+ * methods on JAX pytrees.  Each entry point below replaces one of those methods
+ * (cited as reference file:line) or a fused phase of the step.  Everything is
+ * plain C: raw device pointers, sizes and a hipStream_t passed as void*.
+ *
+ * Memory layout (owned by the caller, e.g. torch tensors):
+ *   particle block  T pbuf[DMPM_P_ROWS][ld]   field row r of particle p is
+ *                                             pbuf[r*ld + p]  (SoA)
+ *   element ids     int32 eid[n]
+ *   node block      T nbuf[DMPM_N_ROWS][nld]  field row r of node i is
+ *                                             nbuf[r*nld + i]
+ * T is float (DMPM_F32) or double (DMPM_F64), fixed per plan.  Vector fields
+ * occupy 2 consecutive rows (x, y); 1D meshes use only the first.  Tensor
+ * fields occupy 6 rows in Voigt order [xx, yy, zz, xy, yz, xz] (xy is the
+ * engineering shear).
+ *
+ * Node numbering follows the reference (element.py:667-680): node
+ * n = j*(nx+1) + i sits at (fl(i*hx), fl(j*hy)); cell e = J*nx + I has nodes
+ * [ll, ll+1, ll+nx+2, ll+nx+1] with ll = J*(nx+1) + I (element.py:692-723).
+ *
+ * Return codes: 0 on success, DMPM_EINVAL for an invalid argument,
+ * DMPM_EHIP for a HIP runtime error.  dmpm_last_error() gives the message
+ * of the last failure on the calling host thread.
+ *
+ * Threading: a plan is not re-entrant; different plans are independent.
+ * Entry points only enqueue work on the given stream (no host synchronisation
+ * and no allocation) except dmpm_plan_create / dmpm_plan_set_* /
+ * dmpm_plan_destroy.
+ */
+#ifndef DIFFMPM_AMD_H
+#define DIFFMPM_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DMPM_OK 0
+#define DMPM_EINVAL (-1)
+#define DMPM_EHIP (-2)
+
+enum { DMPM_F32 = 0, DMPM_F64 = 1 };
+enum { DMPM_LINEAR1D = 1, DMPM_QUAD4 = 2 };
+enum { DMPM_LINEAR_ELASTIC = 0, DMPM_SIMPLE_MATERIAL = 1 };
+enum { DMPM_USF = 0, DMPM_USL = 1 };
+enum { DMPM_FN_UNIT = 0, DMPM_FN_LINEAR = 1 };
+
+/* Particle block rows (particle.py:53-71). */
+enum {
+  DMPM_P_X = 0,           /* loc (2) */
+  DMPM_P_V = 2,           /* velocity (2) */
+  DMPM_P_MASS = 4,
+  DMPM_P_VOLUME = 5,
+  DMPM_P_DENSITY = 6,
+  DMPM_P_SIZE = 7,        /* size (2) */
+  DMPM_P_MOMENTUM = 9,    /* (2) */
+  DMPM_P_ACCEL = 11,      /* (2) never written by the step (as in the reference) */
+  DMPM_P_STRAIN = 13,     /* (6) */
+  DMPM_P_STRESS = 19,     /* (6) */
+  DMPM_P_STRAIN_RATE = 25,/* (6) */
+  DMPM_P_DSTRAIN = 31,    /* (6) */
+  DMPM_P_FEXT = 37,       /* (2) */
+  DMPM_P_TRACTION = 39,   /* (2) */
+  DMPM_P_XI = 41,         /* reference_loc (2) */
+  DMPM_P_DVOLSTRAIN = 43, /* dvolumetric_strain */
+  DMPM_P_VOLSTRAIN_C = 44,/* volumetric_strain_centroid */
+  DMPM_P_YOUNGS = 45,     /* per-particle Young's modulus (extension) */
+  DMPM_P_ROWS = 46
+};
+
+/* Node block rows (node.py:64-71). */
+enum {
+  DMPM_N_LOC = 0,   /* (2) */
+  DMPM_N_V = 2,     /* (2) */
+  DMPM_N_A = 4,     /* (2) */
+  DMPM_N_MASS = 6,
+  DMPM_N_P = 7,     /* momentum (2) */
+  DMPM_N_FINT = 9,  /* (2) */
+  DMPM_N_FEXT = 11, /* (2) */
+  DMPM_N_FDAMP = 13,/* (2) */
+  DMPM_N_ROWS = 15
+};
+
+typedef struct dmpm_plan dmpm_plan;
+
+typedef struct dmpm_plan_desc {
+  int32_t element;        /* DMPM_LINEAR1D or DMPM_QUAD4 */
+  int32_t dtype;          /* DMPM_F32 or DMPM_F64 */
+  int32_t nx, ny;         /* cells per axis (ny = 1 for 1D) */
+  double hx, hy;          /* element lengths; node coords are fl(i*h) in T */
+  int64_t nparticles;     /* particles in the set */
+  int64_t ld;             /* row stride of the particle block, >= nparticles */
+  int64_t nld;            /* row stride of the node block, >= node count */
+  int32_t material;       /* DMPM_LINEAR_ELASTIC or DMPM_SIMPLE_MATERIAL */
+  int32_t per_particle_E; /* 1: E_p from row DMPM_P_YOUNGS (LinearElastic only) */
+  /* LinearElastic: De coefficients (a1, a2, G) computed by the caller in
+   * double as material.py:60-111 does; with per_particle_E they are the
+   * coefficients per unit E.  SimpleMaterial: de[0] = E (material.py:130). */
+  double de[3];
+} dmpm_plan_desc;
+
+/* ---- plan lifetime -------------------------------------------------------- */
+/* Allocates the sort workspace for one particle set on one mesh.  No
+ * reference counterpart (the reference has no persistent workspace). */
+int dmpm_plan_create(const dmpm_plan_desc* desc, dmpm_plan** out);
+int dmpm_plan_destroy(dmpm_plan* plan);
+const char* dmpm_last_error(void);
+/* ABI version (major*100 + minor) and the row counts above, for bindings. */
+int dmpm_abi_version(void);
+
+/* Time functions (functions.py:16-41): kinds[k] is DMPM_FN_UNIT or
+ * DMPM_FN_LINEAR; a Linear function uses xs/fs[offsets[k] .. +counts[k]). */
+int dmpm_plan_set_functions(dmpm_plan* plan, int32_t nfn, const int32_t* kinds,
+                            const int32_t* offsets, const int32_t* counts,
+                            const double* xs, const double* fs);
+/* Velocity constraints, applied in list order (element.py:331-334,
+ * constraint.py:29-44): entry k sets dof dirs[k] of node node_ids[k]. */
+int dmpm_plan_set_constraints(dmpm_plan* plan, int32_t n, const int32_t* node_ids,
+                              const int32_t* dirs, const double* velocities);
+/* Concentrated nodal forces, in list order (element.py:270-284): entry k adds
+ * f_{fn[k]}(t) * forces[k] to f_ext[node_ids[k], dirs[k]]. */
+int dmpm_plan_set_nodal_forces(dmpm_plan* plan, int32_t n, const int32_t* node_ids,
+                               const int32_t* dirs, const int32_t* fn,
+                               const double* forces);
+/* Particle tractions, in list order (mesh.py:60-77, particle.py:329-343):
+ * entry k adds f_{fn[k]}(t) * tractions[k] * V / size[dir] to traction[pids[k], dirs[k]]. */
+int dmpm_plan_set_tractions(dmpm_plan* plan, int32_t n, const int32_t* pids,
+                            const int32_t* dirs, const int32_t* fn,
+                            const double* tractions);
+/* Declares that the element ids changed outside the library (forces a re-sort). */
+int dmpm_mark_unsorted(dmpm_plan* plan, void* stream);
+
+/* ---- per-method entry points (drop-in for the reference methods) ---------- */
+/* element.py:517-541 (1D), 820-840 (Q4): element ids from loc. */
+int dmpm_set_particle_element_ids(dmpm_plan* plan, void* pbuf, int32_t* eid, void* stream);
+/* particle.py:180-206: reference_loc from loc and element ids. */
+int dmpm_update_natural_coords(dmpm_plan* plan, void* pbuf, const int32_t* eid, void* stream);
+/* element.py:543-546 (1D, intended h), 903-909 (Q4): element volume (one
+ * value, in the working precision).  Host-only; needs no device. */
+int dmpm_element_volume(int32_t element, int32_t dtype, double hx, double hy,
+                        double* out_volume);
+/* particle.py:158-178: V_p = V_e / count(e), size = V^(1/ndim), m = V * rho. */
+int dmpm_compute_particle_volume(dmpm_plan* plan, void* pbuf, const int32_t* eid,
+                                 double element_volume, void* stream);
+/* element.py:122-152. */
+int dmpm_compute_nodal_mass(dmpm_plan* plan, const void* pbuf, const int32_t* eid,
+                            void* nbuf, void* stream);
+/* element.py:154-189. */
+int dmpm_compute_nodal_momentum(dmpm_plan* plan, const void* pbuf, const int32_t* eid,
+                                void* nbuf, void* stream);
+/* element.py:191-202. */
+int dmpm_compute_velocity(dmpm_plan* plan, void* nbuf, void* stream);
+/* element.py:331-334. */
+int dmpm_apply_boundary_constraints(dmpm_plan* plan, void* nbuf, void* stream);
+/* element.py:204-234. */
+int dmpm_compute_external_force(dmpm_plan* plan, const void* pbuf, const int32_t* eid,
+                                void* nbuf, void* stream);
+/* element.py:236-268. */
+int dmpm_compute_body_force(dmpm_plan* plan, const void* pbuf, const int32_t* eid,
+                            void* nbuf, const double* gravity, void* stream);
+/* mesh.py:60-77 first half: zero_traction + assign_traction at time t. */
+int dmpm_assign_tractions(dmpm_plan* plan, void* pbuf, double t, void* stream);
+/* element.py:286-303. */
+int dmpm_apply_particle_traction_forces(dmpm_plan* plan, const void* pbuf,
+                                        const int32_t* eid, void* nbuf, void* stream);
+/* element.py:270-284. */
+int dmpm_apply_concentrated_nodal_forces(dmpm_plan* plan, void* nbuf, double t, void* stream);
+/* element.py:548-600 (1D), 842-901 (Q4). */
+int dmpm_compute_internal_force(dmpm_plan* plan, const void* pbuf, const int32_t* eid,
+                                void* nbuf, void* stream);
+/* element.py:305-329. */
+int dmpm_update_nodal_acceleration_velocity(dmpm_plan* plan, void* nbuf, double dt,
+                                            void* stream);
+/* particle.py:208-247. */
+int dmpm_update_position_velocity(dmpm_plan* plan, void* pbuf, const int32_t* eid,
+                                  const void* nbuf, double dt, int32_t velocity_update,
+                                  void* stream);
+/* particle.py:249-313. */
+int dmpm_compute_strain(dmpm_plan* plan, void* pbuf, const int32_t* eid, const void* nbuf,
+                        double dt, void* stream);
+/* particle.py:324-327. */
+int dmpm_update_volume(dmpm_plan* plan, void* pbuf, void* stream);
+/* particle.py:315-322 -> material.py:113-116 / 130-131. */
+int dmpm_compute_stress(dmpm_plan* plan, void* pbuf, void* stream);
+
+/* ---- fused phases (scheme.py:22-67) and the whole step (solver.py:179-185) -- */
+/* scheme.py:22-29: element ids, natural coords, nodal mass/momentum/velocity,
+ * constraints. */
+int dmpm_phase_nodal_kinematics(dmpm_plan* plan, void* pbuf, int32_t* eid, void* nbuf,
+                                void* stream);
+/* scheme.py:31-35: strain, volume, stress. */
+int dmpm_phase_stress_strain(dmpm_plan* plan, void* pbuf, const int32_t* eid,
+                             const void* nbuf, double dt, void* stream);
+/* scheme.py:37-56: external, body, traction, nodal and internal forces at
+ * t = step*dt (computed in T as the reference's traced step loop does). */
+int dmpm_phase_forces(dmpm_plan* plan, void* pbuf, const int32_t* eid, void* nbuf,
+                      const double* gravity, int64_t step, double dt, void* stream);
+/* scheme.py:58-67: nodal acceleration/velocity update and G2P. */
+int dmpm_phase_particle_kinematics(dmpm_plan* plan, void* pbuf, const int32_t* eid,
+                                   void* nbuf, double dt, int32_t velocity_update,
+                                   void* stream);
+/* nsteps explicit steps starting at step index step0 (solver.py:179-213),
+ * USF or USL.  flags bit 0: also materialise the per-step temporaries
+ * (strain_rate, dstrain, dvolumetric_strain, reference_loc, momentum). */
+int dmpm_run(dmpm_plan* plan, void* pbuf, int32_t* eid, void* nbuf, int32_t scheme,
+             int32_t velocity_update, const double* gravity, double dt, int64_t step0,
+             int64_t nsteps, int32_t flags, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DIFFMPM_AMD_H */

@@ -1,0 +1,41 @@
+"""CPU restatement of diffmpm's explicit MPM step — TEST INFRASTRUCTURE ONLY.
+
+This package is the oracle that the HIP path of ``diffmpm_amd`` is checked
+against.  Only ``tests/``, ``__graft_entry__.smoke()`` and the
+``cpu_baseline`` leg of ``bench.py`` may import it, and only as the checker
+(or, for ``cpu_baseline``, as the timed CPU column).  The product package
+``diffmpm_amd`` never imports it and has no CPU fallback.
+
+``oracle.mpm`` restates, in NumPy, the arithmetic of the reference's
+``diffmpm/element.py``, ``particle.py``, ``node.py``, ``material.py``,
+``constraint.py``, ``functions.py``, ``mesh.py``, ``scheme.py`` and
+``solver.py``; each function cites the reference lines it follows.  It is
+templated on dtype: float32 mirrors the reference (JAX with x64 disabled),
+float64 is the high-precision oracle.  P2G scatters use ``np.add.at``, which
+accumulates in particle-index order like the reference's serial
+``lax.fori_loop`` scatter loops.
+
+Pinning (how each function is checked, see ``tests/test_oracle_*.py``):
+
+* every unit-test known-answer value of the reference
+  (``tests/test_element.py``, ``tests/test_material.py``,
+  ``tests/test_particle.py``) restated as fixtures in
+  ``tests/golden/reference_kats.json``;
+* the three end-to-end goldens of ``benchmarks/2d/*/test_benchmark.py``
+  restated in ``tests/golden/reference_goldens.json``;
+* closed forms and invariants: partition of unity, gradient sums, P2G mass and
+  momentum conservation, exact strain rates for linear velocity fields,
+  De algebra, ``np.interp`` semantics, brute-force element search on tiny
+  meshes, FP32 vs FP64 agreement.
+
+Parity unpinned (no reference test covers them and the reference cannot be
+imported here — JAX is absent):
+
+* the whole Linear1D path (``TestLinear1D`` in the reference is empty), checked
+  only by closed forms (linear-field strain rate, 1D mass/momentum
+  conservation) and its documented quirks;
+* element search for points outside the mesh (wrap / element-0 / out-of-range
+  ids), checked by brute force against the reference's set definition only;
+* the exact FP32 rounding of XLA fusions (FMA contraction, LU inverse), pinned
+  to tolerance only.
+"""
