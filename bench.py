@@ -1,0 +1,139 @@
+"""Throughput benchmark of the explicit MPM step on one MI355X.
+
+Workload: 2D Quadrilateral4Node 1024x1024 cells on [0,1]^2, 4 particles per
+cell at jittered (+-0.25 h, seed 0) Gauss points: 4,194,304 particles,
+LinearElastic E=1e6, nu=0.3, rho=1000, gravity (0,-9.81), bottom nodes fixed,
+dt=1e-5, USF, FP32.  W warm-up steps, then K timed steps bracketed by
+synchronisation; the step inputs are resident in HBM.
+
+Prints one JSON line with particle-steps/s, the roofline of the dominant
+kernel (HIP-event durations measured inside the timed region) and the CPU
+baseline (the oracle, a NumPy restatement of the reference step, timed on a
+bounded strip of the same workload on this host).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+HBM_PEAK_GBS = 8000.0
+# Algorithmic bytes per particle of each timed kernel class (FP32 2D
+# LinearElastic): the persistent particle fields the kernel must read and
+# write; grid arrays, sort keys and recomputable temporaries excluded.
+KERNEL_BYTES = {
+    # strain/volume/stress update: read eps(3), sigma(4), V, rho, eps_v,
+    # xi(2, read back from the locate kernel), write eps(3), sigma(4), V, rho, eps_v
+    "stress": 4 * (3 + 4 + 3 + 2) + 4 * (3 + 4 + 3),
+    # P2G of mass and momentum: read xi(2), m, v(2)
+    "p2g_kin": 4 * (2 + 1 + 2),
+    # P2G of forces: read xi(2), m, V, sigma(3)
+    "p2g_force": 4 * (2 + 1 + 1 + 3),
+    # G2P: read xi(2), x(2), v(2); write x(2), v(2)
+    "g2p": 4 * (2 + 2 + 2) + 4 * (2 + 2),
+    # locate: read x(2); write xi(2)
+    "locate": 4 * 2 + 4 * 2,
+}
+STEP_BYTES = 116  # per particle-step, whole step (FP32 2D LinearElastic)
+
+
+def cpu_baseline(rows=32, steps=8):
+    """Oracle (NumPy restatement of the reference step, one core) on a
+    1024 x `rows`-cell strip of the same workload."""
+    import numpy as np
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    from oracle import mpm as om
+    from diffmpm_amd.workloads import gauss_points
+    try:
+        from threadpoolctl import threadpool_limits
+        lim = threadpool_limits(1)
+    except Exception:
+        lim = None
+    n = 1024
+    h = 1.0 / n
+    pts = gauss_points(n, rows, h, 0.25, 0)
+    cons = [(np.arange(n + 1), om.Constraint(0, 0.0)), (np.arange(n + 1), om.Constraint(1, 0.0))]
+    el = om.Elements("quad4", (n, rows + 1), (h, h), cons, [], np.float32)
+    ps = om.Particles(pts, om.LinearElastic({"density": 1000.0, "youngs_modulus": 1e6,
+                                              "poisson_ratio": 0.3}), dtype=np.float32)
+    sim = om.MPMExplicit(om.Mesh(el, [ps]), 1e-5, "usf", False, 1)
+    g = np.array([0.0, -9.81])
+    sim.step(0, g)  # warm-up
+    t0 = time.perf_counter()
+    for i in range(1, steps + 1):
+        sim.step(i, g)
+    el_s = time.perf_counter() - t0
+    if lim is not None:
+        lim.unregister() if hasattr(lim, "unregister") else None
+    return {"value": len(pts) * steps / el_s, "unit": "particle-steps/s", "cores": 1,
+            "kind": "port",
+            "sample": f"oracle (NumPy FP32 restatement of the reference step, 1 thread) on a "
+                      f"1024x{rows}-cell strip ({len(pts)} particles) of the same workload, "
+                      f"{steps} steps after 1 warm-up ({el_s:.1f} s)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--n", type=int, default=1024)
+    ap.add_argument("--kernel", default="stress", help="kernel class timed for the roofline")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: this build has no multi-GPU path "
+              f"(--gpus {args.gpus} requested)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from diffmpm_amd import workloads
+    from diffmpm_amd._plan import plan_for, timing_enable, timing_read
+
+    sim, gravity = workloads.block_2d(args.n)
+    ps = sim.mesh.particles[0]
+    plan = plan_for(sim.mesh.elements, ps)
+    N = len(ps)
+    print(f"# workload built: {N} particles, {args.n}x{args.n} cells", file=sys.stderr, flush=True)
+    plan.run("usf", False, gravity, sim.dt, 0, args.warmup, keep_temporaries=False)
+    torch.cuda.synchronize()
+    print("# warm-up done", file=sys.stderr, flush=True)
+    timing_enable(plan, args.kernel, 64 * args.steps)
+    stream = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record(stream)
+    plan.run("usf", False, gravity, sim.dt, args.warmup, args.steps, keep_temporaries=False)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1)
+    kms, klaunch = timing_read(plan)
+    timing_enable(plan, None, 0)
+    assert torch.isfinite(ps.loc).all() and torch.isfinite(ps.stress).all()
+    value = N * args.steps / (ms / 1e3)
+    kbytes = KERNEL_BYTES.get(args.kernel)
+    kavg = kms / max(klaunch, 1)
+    achieved = (kbytes * N / (kavg / 1e3) / 1e9) if kbytes else None
+    line = {
+        "metric": "particle-steps/s fwd at 4.2M ptcl FP32",
+        "value": value, "unit": "particle-steps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms / args.steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"Q4 {args.n}x{args.n} cells, 4 ppc jittered, "
+                               f"{N} particles, USF, LinearElastic, gravity, dt=1e-5",
+                   "particles": N, "scheme": "usf"},
+        "step_effective_GBs": STEP_BYTES * N * args.steps / (ms / 1e3) / 1e9,
+        "roofline": {"bound": "hbm", "kernel": args.kernel, "achieved": achieved,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
+                     "traffic": None, "avg_launch_us": kavg * 1e3,
+                     "bytes_per_particle": kbytes, "launches": klaunch},
+    }
+    if not args.no_cpu_baseline:
+        print("# cpu baseline", file=sys.stderr, flush=True)
+        line["cpu_baseline"] = cpu_baseline()
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -28,6 +28,10 @@ P_FEXT, P_TRACTION, P_XI, P_DVOLSTRAIN, P_VOLSTRAIN_C, P_YOUNGS, P_ROWS = 37, 39
 # Node block rows
 N_LOC, N_V, N_A, N_MASS, N_P, N_FINT, N_FEXT, N_FDAMP, N_ROWS = 0, 2, 4, 6, 7, 9, 11, 13, 15
 
+# Kernel classes (dmpm_timing_enable)
+KC = dict(locate=0, sort=1, p2g_kin=2, stress=3, p2g_force=4, grid=5, g2p=6, small=7,
+          tile_kin=8, tile_force=9, tile_g2p=10, adj=11, batch=12)
+
 ROW_ALIGN = 64  # row stride multiple (256 B in fp32) so every row starts aligned
 
 
@@ -78,6 +82,8 @@ SIGNATURES = {
     "dmpm_phase_stress_strain": [_P, _P, _P, _P, c_double, _P],
     "dmpm_phase_forces": [_P, _P, _P, _P, _DP, c_int64, c_double, _P],
     "dmpm_phase_particle_kinematics": [_P, _P, _P, _P, c_double, c_int32, _P],
+    "dmpm_timing_enable": [_P, c_int32, c_int32],
+    "dmpm_timing_read": [_P, _DP, POINTER(c_int64)],
     "dmpm_run": [_P, _P, _P, _P, c_int32, c_int32, _DP, c_double, c_int64, c_int64, c_int32, _P],
 }
 

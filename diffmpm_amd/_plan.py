@@ -239,3 +239,18 @@ def plan_for(elements, particles):
         p = Plan(elements, particles)
         particles._plans[key] = p
     return p
+
+
+def timing_enable(plan, kernel_class, max_launches):
+    """Records HIP events around launches of one kernel class (see
+    ``_lib.KC``); ``kernel_class=None`` disables."""
+    kc = -1 if kernel_class is None else _lib.KC[kernel_class]
+    call("dmpm_timing_enable", plan.h, kc, int(max_launches))
+
+
+def timing_read(plan):
+    """(total milliseconds, launches) of the recorded kernel class."""
+    from ctypes import c_double, c_int64
+    ms, n = c_double(), c_int64()
+    call("dmpm_timing_read", plan.h, byref(ms), byref(n))
+    return ms.value, n.value

@@ -46,6 +46,14 @@ int launch_check(const char* what) { return hip_check(hipGetLastError(), what); 
     if (rc__ != DMPM_OK) return rc__; \
   } while (0)
 
+struct ProfScope {
+  dmpm_plan* P;
+  int cls;
+  hipStream_t s;
+  ProfScope(dmpm_plan* p, int c, hipStream_t st) : P(p), cls(c), s(st) { prof_begin(P, cls, s); }
+  ~ProfScope() { prof_end(P, cls, s); }
+};
+
 static inline unsigned nblocks(int64_t n, int b = kBlock) {
   return (unsigned)((n + b - 1) / b);
 }
@@ -160,13 +168,23 @@ __global__ void k_sort_scatter(const int* __restrict__ key, int n, const int* __
 }
 
 // Restores ascending particle index inside each cell (makes the order stable
-// and independent of atomic arrival order).  The outside bucket is unused.
+// and independent of atomic arrival order).  Cells of up to kSmallCell
+// particles are insertion-sorted by one thread; larger cells are queued for
+// k_sort_fix_big.  The outside bucket is never used and stays unsorted.
+constexpr int kSmallCell = 64;
+constexpr int kBigCellMax = 8192;  // LDS capacity of the rank sort
+
 __global__ void k_sort_fix(const int* __restrict__ cs, int ncells, int* __restrict__ perm,
-                           const int* __restrict__ flags) {
+                           int* __restrict__ flags, int* __restrict__ big) {
   if (flags[0] == 0) return;
   int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= ncells) return;
   int b = cs[c], e = cs[c + 1];
+  if (e - b > kSmallCell) {
+    int k = atomicAdd(&flags[3], 1);
+    big[k] = c;
+    return;
+  }
   for (int i = b + 1; i < e; ++i) {
     int v = perm[i];
     int j = i - 1;
@@ -178,12 +196,44 @@ __global__ void k_sort_fix(const int* __restrict__ cs, int ncells, int* __restri
   }
 }
 
+// One workgroup per queued cell: each element's slot is its rank among the
+// cell's (distinct) particle indices, O(k^2 / 256) per cell.  Cells beyond
+// kBigCellMax particles keep the atomic arrival order (sums stay correct but
+// their order is no longer reproducible); flags[2] records that this happened.
+__global__ void k_sort_fix_big(const int* __restrict__ cs, int* __restrict__ perm,
+                               int* __restrict__ flags, const int* __restrict__ big) {
+  if (flags[0] == 0) return;
+  __shared__ int sh[kBigCellMax];
+  const int nbig = flags[3];
+  for (int q = blockIdx.x; q < nbig; q += gridDim.x) {
+    const int c = big[q];
+    const int b = cs[c], n = cs[c + 1] - b;
+    if (n > kBigCellMax) {
+      if (threadIdx.x == 0) flags[2] = 1;
+      continue;
+    }
+    for (int i = threadIdx.x; i < n; i += blockDim.x) sh[i] = perm[b + i];
+    __syncthreads();
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+      const int v = sh[i];
+      int r = 0;
+      for (int j = 0; j < n; ++j) r += sh[j] < v;
+      perm[b + r] = v;
+    }
+    __syncthreads();
+  }
+}
+
 __global__ void k_sort_done(int* flags) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) flags[0] = 0;
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    flags[0] = 0;
+    flags[3] = 0;
+  }
 }
 
 namespace dmpm {
 int enqueue_sort(dmpm_plan* P, hipStream_t s) {
+  ProfScope ps_(P, KC_SORT, s);
   const int n = (int)P->n;
   const int nb = P->ncells + 1;
   if (n > 0) {
@@ -194,7 +244,9 @@ int enqueue_sort(dmpm_plan* P, hipStream_t s) {
     k_scan_sums<<<1, kBlock, 0, s>>>(P->bsum, nbs, P->flags);
     k_scan_add<<<nblocks(nb), kBlock, 0, s>>>(P->cs, nb, n, P->bsum, P->flags);
     k_sort_scatter<<<nblocks(n), kBlock, 0, s>>>(P->key, n, P->cs, P->fill, P->perm, P->flags);
-    k_sort_fix<<<nblocks(P->ncells), kBlock, 0, s>>>(P->cs, P->ncells, P->perm, P->flags);
+    k_sort_fix<<<nblocks(P->ncells), kBlock, 0, s>>>(P->cs, P->ncells, P->perm, P->flags,
+                                                     P->big);
+    k_sort_fix_big<<<256, kBlock, 0, s>>>(P->cs, P->perm, P->flags, P->big);
   }
   k_sort_done<<<1, 64, 0, s>>>(P->flags);
   return launch_check("sort");
@@ -771,7 +823,7 @@ static void free_ptr(void* p) {
 
 void plan_free(dmpm_plan* P) {
   if (!P) return;
-  void* ptrs[] = {P->key, P->perm, P->counts, P->fill, P->cs, P->bsum, P->flags,
+  void* ptrs[] = {P->key, P->perm, P->counts, P->fill, P->cs, P->bsum, P->flags, P->big,
                   P->con_node, P->con_dir, P->con_val, P->nf_id, P->nf_dir, P->nf_off,
                   P->nf_fn, P->nf_val, P->tr_id, P->tr_dir, P->tr_off, P->tr_fn, P->tr_val,
                   P->fn_kind, P->fn_off, P->fn_cnt, P->fn_x, P->fn_f};
@@ -864,7 +916,10 @@ int dmpm_plan_create(const dmpm_plan_desc* desc, dmpm_plan** out) {
   al(&P->cs, nb + 1);
   al(&P->bsum, nbs);
   al(&P->flags, 4);
+  al(&P->big, P->ncells);
   if (rc == DMPM_OK) rc = hip_check(hipMemset(P->key, 0xff, n * sizeof(int)), "hipMemset");
+  if (rc == DMPM_OK) rc = hip_check(hipMemset(P->cs, 0, (nb + 1) * sizeof(int)), "hipMemset");
+  if (rc == DMPM_OK) rc = hip_check(hipMemset(P->perm, 0, n * sizeof(int)), "hipMemset");
   int f0[4] = {1, 0, 0, 0};
   if (rc == DMPM_OK)
     rc = hip_check(hipMemcpy(P->flags, f0, sizeof(f0), hipMemcpyHostToDevice), "hipMemcpy");
@@ -877,7 +932,35 @@ int dmpm_plan_create(const dmpm_plan_desc* desc, dmpm_plan** out) {
 }
 
 int dmpm_plan_destroy(dmpm_plan* plan) {
+  if (plan) {
+    for (auto e : plan->prof_ev) (void)hipEventDestroy(e);
+  }
   plan_free(plan);
+  return DMPM_OK;
+}
+
+int dmpm_timing_enable(dmpm_plan* P, int32_t kernel_class, int32_t max_launches) {
+  (void)max_launches;
+  if (!P) return fail(DMPM_EINVAL, "null plan");
+  for (int k = 0; k < kProfRing; ++k)
+    if (!P->prof_ev.empty()) prof_drain_slot(P, k);
+  for (auto e : P->prof_ev) (void)hipEventDestroy(e);
+  P->prof_ev.clear();
+  P->prof_n = 0;
+  P->prof_ms = 0;
+  P->prof_class = kernel_class;
+  if (kernel_class < 0) return DMPM_OK;
+  P->prof_ev.resize(2 * kProfRing);
+  for (auto& e : P->prof_ev) DMPM_TRY(hip_check(hipEventCreate(&e), "hipEventCreate"));
+  return DMPM_OK;
+}
+
+int dmpm_timing_read(dmpm_plan* P, double* total_ms, int64_t* launches) {
+  if (!P || !total_ms || !launches) return fail(DMPM_EINVAL, "null argument");
+  if (!P->prof_ev.empty())
+    for (int k = 0; k < kProfRing; ++k) prof_drain_slot(P, k);
+  *total_ms = P->prof_ms;
+  *launches = P->prof_n;
   return DMPM_OK;
 }
 
@@ -1003,6 +1086,7 @@ namespace dmpm {
 
 template <typename T, int DIM> struct Ops {
   static int locate(dmpm_plan* P, void* pbuf, int32_t* eid, int write_xi, hipStream_t s) {
+    ProfScope ps_(P, KC_LOCATE, s);
     if (P->n > 0)
       k_locate<T, DIM><<<nblocks(P->n), kBlock, 0, s>>>(make_geo<T>(P), (T*)pbuf, P->d.ld,
                                                          (int)P->n, eid, P->key, P->flags,
@@ -1023,12 +1107,14 @@ template <typename T, int DIM> struct Ops {
     return launch_check("particle_volume");
   }
   static int kin(dmpm_plan* P, const void* pbuf, void* nbuf, int mode, hipStream_t s) {
+    ProfScope ps_(P, KC_P2G_KIN, s);
     k_p2g_kin<T, DIM><<<nblocks(P->nnodes), kBlock, 0, s>>>(
         make_geo<T>(P), (const T*)pbuf, P->d.ld, P->perm, P->cs, (T*)nbuf, P->d.nld, mode);
     return launch_check("p2g_kin");
   }
   static int force(dmpm_plan* P, const void* pbuf, void* nbuf, int mode, const double* grav,
                    hipStream_t s) {
+    ProfScope ps_(P, KC_P2G_FORCE, s);
     T gx = grav ? T(grav[0]) : T(0);
     T gy = (grav && DIM == 2) ? T(grav[1]) : T(0);
     k_p2g_force<T, DIM><<<nblocks(P->nnodes), kBlock, 0, s>>>(
@@ -1059,6 +1145,7 @@ template <typename T, int DIM> struct Ops {
     return launch_check("tractions");
   }
   static int grid_update(dmpm_plan* P, void* nbuf, double dt, hipStream_t s) {
+    ProfScope ps_(P, KC_GRID, s);
     k_grid_update<T, DIM><<<nblocks(P->nnodes), kBlock, 0, s>>>((T*)nbuf, P->d.nld, P->nnodes,
                                                                 T(dt));
     DMPM_TRY(launch_check("grid_update"));
@@ -1066,6 +1153,7 @@ template <typename T, int DIM> struct Ops {
   }
   static int stress(dmpm_plan* P, void* pbuf, const int32_t* eid, const void* nbuf, double dt,
                     int mode, hipStream_t s) {
+    ProfScope ps_(P, KC_STRESS, s);
     if (P->n > 0)
       k_stress<T, DIM><<<nblocks(P->n), kBlock, 0, s>>>(make_geo<T>(P), make_mat(P), (T*)pbuf,
                                                         P->d.ld, (int)P->n, eid,
@@ -1074,6 +1162,7 @@ template <typename T, int DIM> struct Ops {
   }
   static int g2p(dmpm_plan* P, void* pbuf, const int32_t* eid, const void* nbuf, double dt,
                  int vu, int write_mom, hipStream_t s) {
+    ProfScope ps_(P, KC_G2P, s);
     if (P->n > 0)
       k_g2p<T, DIM><<<nblocks(P->n), kBlock, 0, s>>>(make_geo<T>(P), (T*)pbuf, P->d.ld,
                                                      (int)P->n, eid, (const T*)nbuf, P->d.nld,

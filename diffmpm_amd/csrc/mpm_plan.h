@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include <string>
+#include <vector>
 
 #include "../../include/diffmpm_amd.h"
 
@@ -21,7 +22,8 @@ struct dmpm_plan {
   int* fill = nullptr;    // [ncells+1]
   int* cs = nullptr;      // [ncells+2] first slot of each cell (+ total)
   int* bsum = nullptr;    // scan block totals
-  int* flags = nullptr;   // [0] sort dirty
+  int* flags = nullptr;   // [0] sort dirty, [2] unstable big cell seen, [3] big-cell count
+  int* big = nullptr;     // [ncells] cells queued for the big-cell sort
   // velocity constraints, last value per (node, dir)
   int ncon = 0;
   int* con_node = nullptr;
@@ -40,6 +42,13 @@ struct dmpm_plan {
   int nfn = 0;
   int *fn_kind = nullptr, *fn_off = nullptr, *fn_cnt = nullptr;
   double *fn_x = nullptr, *fn_f = nullptr;
+  // per-kernel-class timing (dmpm_timing_enable): event pairs recorded around
+  // every launch of class prof_class on its stream
+  int prof_class = -1;
+  int prof_n = 0;
+  double prof_ms = 0;
+  int prof_pending[4] = {0, 0, 0, 0};
+  std::vector<hipEvent_t> prof_ev;
   // workspace of the fused tile kernels and the adjoint (allocated lazily)
   void* extra[16] = {nullptr};
   size_t extra_bytes[16] = {0};
@@ -60,6 +69,33 @@ int hip_check(hipError_t e, const char* what);
 int launch_check(const char* what);
 int enqueue_sort(dmpm_plan* P, hipStream_t s);
 int ensure_sorted(dmpm_plan* P, const int32_t* eid, hipStream_t s);
-// Returns extra workspace slot `slot` with at least `bytes` bytes.
-void* plan_workspace(dmpm_plan* P, int slot, size_t bytes);
+// Kernel classes for dmpm_timing_enable.
+enum { KC_LOCATE = 0, KC_SORT = 1, KC_P2G_KIN = 2, KC_STRESS = 3, KC_P2G_FORCE = 4,
+       KC_GRID = 5, KC_G2P = 6, KC_SMALL = 7, KC_TILE_KIN = 8, KC_TILE_FORCE = 9,
+       KC_TILE_G2P = 10, KC_ADJ = 11, KC_BATCH = 12 };
+// A ring of kProfRing event pairs: before a pair is reused its elapsed time is
+// read (host waits for a launch kProfRing launches back), so at most kProfRing
+// timed launches are outstanding.
+constexpr int kProfRing = 4;
+inline void prof_drain_slot(dmpm_plan* P, int slot) {
+  if (!P->prof_pending[slot]) return;
+  (void)hipEventSynchronize(P->prof_ev[2 * slot + 1]);
+  float ms = 0;
+  if (hipEventElapsedTime(&ms, P->prof_ev[2 * slot], P->prof_ev[2 * slot + 1]) == hipSuccess)
+    P->prof_ms += ms;
+  P->prof_pending[slot] = 0;
+}
+inline void prof_begin(dmpm_plan* P, int cls, hipStream_t s) {
+  if (P->prof_class != cls || P->prof_ev.empty()) return;
+  int slot = P->prof_n % kProfRing;
+  prof_drain_slot(P, slot);
+  (void)hipEventRecord(P->prof_ev[2 * slot], s);
+}
+inline void prof_end(dmpm_plan* P, int cls, hipStream_t s) {
+  if (P->prof_class != cls || P->prof_ev.empty()) return;
+  int slot = P->prof_n % kProfRing;
+  (void)hipEventRecord(P->prof_ev[2 * slot + 1], s);
+  P->prof_pending[slot] = 1;
+  P->prof_n++;
+}
 }  // namespace dmpm

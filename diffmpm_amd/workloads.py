@@ -1,0 +1,58 @@
+"""Seeded synthetic workloads of the benchmark configurations.
+
+``block_2d`` builds the 2D uniform-grid cases: Quadrilateral4Node mesh of
+n x n cells on [0,1]^2, 4 particles per cell at the 2x2 Gauss points of the
+lower ``rows`` cell rows (optionally jittered uniformly by +-jitter*h per
+coordinate, then clamped into [0, 1) so no particle leaves the mesh),
+LinearElastic, gravity (0, -9.81), the bottom node row fixed in x and y and
+the left and right node columns fixed in x (rollers).  Without the rollers
+the side particles leave the mesh within a few steps under the lateral
+expansion of the compressed block; the reference's element search
+(element.py:820-840) then assigns them element 0 or wraps them into the next
+cell row, and the state diverges.
+"""
+import math
+
+import numpy as np
+import torch
+
+
+def gauss_points(nx, rows, h, jitter=0.0, seed=0):
+    g = (1.0 - 1.0 / math.sqrt(3.0)) / 2.0
+    offs = np.array([[g, g], [1 - g, g], [g, 1 - g], [1 - g, 1 - g]], dtype=np.float64)
+    jj, ii = np.meshgrid(np.arange(rows, dtype=np.float64), np.arange(nx, dtype=np.float64),
+                         indexing="ij")
+    base = np.stack([ii.reshape(-1), jj.reshape(-1)], axis=1)
+    pts = (base[:, None, :] + offs[None, :, :]).reshape(-1, 2)
+    if jitter:
+        rng = np.random.Generator(np.random.PCG64(seed))
+        pts += rng.uniform(-jitter, jitter, size=pts.shape)
+    pts *= h
+    hi = np.nextafter(np.float32(1.0), np.float32(0.0))
+    np.clip(pts, 0.0, hi, out=pts)
+    return pts
+
+
+def block_2d(n=1024, rows=None, jitter=0.25, seed=0, E=1e6, nu=0.3, density=1000.0,
+             dt=1e-5, scheme="usf", velocity_update=False, dtype=torch.float32,
+             device="cuda", keep_temporaries=False, fix_bottom=(0, 1), side_rollers=True):
+    """Returns (MPMExplicit, gravity) for the uniform 2D block case.  The
+    4.2M-particle throughput configuration is ``block_2d(1024)``."""
+    import diffmpm_amd as d
+    rows = n if rows is None else rows
+    h = 1.0 / n
+    pts = gauss_points(n, rows, h, jitter, seed)
+    bottom = torch.arange(n + 1)
+    cons = [(bottom, d.Constraint(k, 0.0)) for k in fix_bottom]
+    if side_rollers:
+        j = torch.arange(n + 1)
+        cons.append((torch.cat([j * (n + 1), j * (n + 1) + n]), d.Constraint(0, 0.0)))
+    el = d.Quadrilateral4Node((n, n), n * n, (h, h), cons, device=device, dtype=dtype)
+    mat = d.LinearElastic({"density": density, "youngs_modulus": E, "poisson_ratio": nu})
+    np_dt = np.float64 if dtype == torch.float64 else np.float32
+    loc = torch.from_numpy(pts.astype(np_dt)).reshape(-1, 1, 2)
+    ps = d.Particles(loc, mat, torch.zeros(loc.shape[0], dtype=torch.int32), device=device,
+                     dtype=dtype)
+    mesh = d.Mesh2D({"particles": [ps], "elements": el, "particle_surface_traction": []})
+    sim = d.MPMExplicit(mesh, dt, scheme, velocity_update, 1, keep_temporaries=keep_temporaries)
+    return sim, torch.tensor([0.0, -9.81])

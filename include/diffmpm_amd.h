@@ -210,6 +210,22 @@ int dmpm_run(dmpm_plan* plan, void* pbuf, int32_t* eid, void* nbuf, int32_t sche
              int32_t velocity_update, const double* gravity, double dt, int64_t step0,
              int64_t nsteps, int32_t flags, void* stream);
 
+/* ---- measurement (no reference counterpart) ------------------------------- */
+/* Kernel classes for dmpm_timing_enable. */
+enum {
+  DMPM_KC_LOCATE = 0, DMPM_KC_SORT = 1, DMPM_KC_P2G_KIN = 2, DMPM_KC_STRESS = 3,
+  DMPM_KC_P2G_FORCE = 4, DMPM_KC_GRID = 5, DMPM_KC_G2P = 6, DMPM_KC_SMALL = 7,
+  DMPM_KC_TILE_KIN = 8, DMPM_KC_TILE_FORCE = 9, DMPM_KC_TILE_G2P = 10,
+  DMPM_KC_ADJ = 11, DMPM_KC_BATCH = 12
+};
+/* Times every launch of one kernel class with HIP event pairs recorded on the
+ * launch stream (kernel_class < 0 disables).  A ring of 4 pairs is reused,
+ * so the host waits for the timed launch 4 launches back before recording a
+ * new pair.  max_launches is ignored (kept for ABI stability). */
+int dmpm_timing_enable(dmpm_plan* plan, int32_t kernel_class, int32_t max_launches);
+/* Waits for the recorded events; returns the summed duration and count. */
+int dmpm_timing_read(dmpm_plan* plan, double* total_ms, int64_t* launches);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,5 +1,10 @@
-"""Builders shared by the tests: the reference's end-to-end golden cases and
-seeded synthetic meshes, for both the oracle and the HIP package."""
+"""Case builders shared by the tests: one spec dict drives both the oracle
+(CPU restatement) and the HIP package, so both see identical inputs.
+
+Spec keys: element ('quad4' | 'linear1d'), nelements, el_len, particles
+(list of coordinates), init_velocity (scalar, per-dimension or per-particle),
+material, constraints, nodal_forces, tractions, gravity, scheme, dt, nsteps,
+velocity_update, optional youngs (per-particle E)."""
 import json
 import math
 import os
@@ -13,6 +18,8 @@ def load_goldens():
     with open(os.path.join(GOLDEN_DIR, "reference_goldens.json")) as f:
         d = json.load(f)
     d.pop("_source", None)
+    for v in d.values():
+        v.setdefault("element", "quad4")
     return d
 
 
@@ -21,41 +28,203 @@ def load_kats():
         return json.load(f)
 
 
-def oracle_function(spec):
-    from oracle import mpm as om
-    if spec is None or spec["type"] == "Unit":
-        return om.Unit(-1)
-    return om.Linear(0, spec["xvalues"], spec["fxvalues"])
-
-
-def oracle_material(spec):
-    from oracle import mpm as om
-    spec = dict(spec)
-    kind = spec.pop("type")
-    return getattr(om, kind)(spec)
-
-
-def build_oracle_golden(case, dtype=np.float32):
-    """Restates the TOML front-end (io.py:52-170) for one golden case."""
-    from oracle import mpm as om
-    cons = [(np.asarray(c["nodes"]), om.Constraint(c["dir"], c["velocity"]))
-            for c in case["constraints"]]
-    cnfs = [om.NodalForce(np.asarray(f["nodes"]), oracle_function(f.get("function")),
-                          f["dir"], f["force"]) for f in case["nodal_forces"]]
-    el = om.Elements("quad4", case["nelements"], case["el_len"], cons, cnfs, dtype)
-    ps = om.Particles(np.asarray(case["particles"]), oracle_material(case["material"]),
-                      dtype=dtype)
-    ps.velocity[:] = np.asarray(case["init_velocity"], dtype=np.float64).astype(dtype)
-    trs = [om.ParticleTraction(t["pset"], np.asarray(t["pids"]),
-                               oracle_function(t.get("function")), t["dir"], t["traction"])
-           for t in case["tractions"]]
-    mesh = om.Mesh(el, [ps], trs)
-    return om.MPMExplicit(mesh, case["dt"], case["scheme"], case["velocity_update"],
-                          case["nsteps"])
-
-
 def check_value(stress, chk):
     """The reference's assertion: round(reduce(stress[:, comp]) - value, d) == 0."""
-    col = np.asarray(stress)[:, chk["component"]].astype(np.float64)
-    got = col.max() if chk["reduce"] == "max" else col.min()
+    col = np.asarray(stress, dtype=np.float64).reshape(len(stress), -1)[:, chk["component"]]
+    got = float(col.max() if chk["reduce"] == "max" else col.min())
     return got, round(got - chk["value"], chk["decimals"]) == 0.0
+
+
+def _velocity(spec, n, ndim):
+    v = np.asarray(spec.get("init_velocity", 0.0), dtype=np.float64)
+    return np.broadcast_to(v.reshape(-1, ndim) if v.size == n * ndim else v, (n, ndim))
+
+
+# ---------------------------------------------------------------- oracle side
+def oracle_function(fspec):
+    from oracle import mpm as om
+    if fspec is None or fspec["type"] == "Unit":
+        return om.Unit(-1)
+    return om.Linear(0, fspec["xvalues"], fspec["fxvalues"])
+
+
+def oracle_material(mspec):
+    from oracle import mpm as om
+    m = dict(mspec)
+    return getattr(om, m.pop("type"))(m)
+
+
+def build_oracle(spec, dtype=np.float32):
+    """Restates the reference's setup (io.py:52-170 for the TOML cases)."""
+    from oracle import mpm as om
+    cons = [(np.asarray(c["nodes"]), om.Constraint(c["dir"], c["velocity"]))
+            for c in spec["constraints"]]
+    cnfs = [om.NodalForce(np.asarray(f["nodes"]), oracle_function(f.get("function")),
+                          f["dir"], f["force"]) for f in spec["nodal_forces"]]
+    el = om.Elements(spec["element"], spec["nelements"], spec["el_len"], cons, cnfs, dtype)
+    pts = np.asarray(spec["particles"], dtype=np.float64).reshape(len(spec["particles"]), -1)
+    ps = om.Particles(pts, oracle_material(spec["material"]), dtype=dtype)
+    ps.velocity[:] = _velocity(spec, len(pts), pts.shape[1]).astype(dtype)
+    if spec.get("youngs") is not None:
+        ps.youngs_modulus = np.asarray(spec["youngs"], dtype=dtype)
+    trs = [om.ParticleTraction(t["pset"], np.asarray(t["pids"]),
+                               oracle_function(t.get("function")), t["dir"], t["traction"])
+           for t in spec["tractions"]]
+    mesh = om.Mesh(el, [ps], trs)
+    return om.MPMExplicit(mesh, spec["dt"], spec["scheme"], spec["velocity_update"],
+                          spec["nsteps"])
+
+
+build_oracle_golden = build_oracle
+
+
+# ------------------------------------------------------------------ GPU side
+def gpu_function(fspec):
+    import diffmpm_amd as d
+    if fspec is None or fspec["type"] == "Unit":
+        return d.Unit(-1)
+    return d.Linear(0, fspec["xvalues"], fspec["fxvalues"])
+
+
+def gpu_material(mspec):
+    import diffmpm_amd as d
+    m = dict(mspec)
+    return getattr(d, m.pop("type"))(m)
+
+
+def build_gpu(spec, dtype=None, keep_temporaries=True):
+    import torch
+    import diffmpm_amd as d
+    dtype = dtype or torch.float32
+    dev = "cuda"
+    cons = [(torch.tensor(c["nodes"]), d.Constraint(c["dir"], c["velocity"]))
+            for c in spec["constraints"]]
+    cnfs = [d.NodalForce(torch.tensor(f["nodes"]), gpu_function(f.get("function")),
+                         f["dir"], f["force"]) for f in spec["nodal_forces"]]
+    if spec["element"] == "quad4":
+        nx, ny = spec["nelements"]
+        el = d.Quadrilateral4Node((nx, ny), nx * ny, spec["el_len"], cons,
+                                  concentrated_nodal_forces=cnfs, device=dev, dtype=dtype)
+        ndim, Mesh = 2, d.Mesh2D
+    else:
+        el = d.Linear1D(spec["nelements"], spec["nelements"], spec["el_len"], cons,
+                        concentrated_nodal_forces=cnfs, device=dev, dtype=dtype)
+        ndim, Mesh = 1, d.Mesh1D
+    pts = np.asarray(spec["particles"], dtype=np.float64).reshape(len(spec["particles"]), 1, ndim)
+    np_dt = np.float64 if dtype == torch.float64 else np.float32
+    ps = d.Particles(torch.from_numpy(pts.astype(np_dt)), gpu_material(spec["material"]),
+                     torch.zeros(len(pts), dtype=torch.int32), device=dev, dtype=dtype)
+    ps.velocity = torch.from_numpy(
+        _velocity(spec, len(pts), ndim).astype(np_dt).reshape(len(pts), 1, ndim))
+    if spec.get("youngs") is not None:
+        ps.youngs_modulus = torch.as_tensor(np.asarray(spec["youngs"], dtype=np_dt))
+    trs = [d.ParticleTraction(t["pset"], torch.tensor(t["pids"]),
+                              gpu_function(t.get("function")), t["dir"], t["traction"])
+           for t in spec["tractions"]]
+    mesh = Mesh({"particles": [ps], "elements": el, "particle_surface_traction": trs})
+    return d.MPMExplicit(mesh, spec["dt"], spec["scheme"], spec["velocity_update"],
+                         spec["nsteps"], keep_temporaries=keep_temporaries)
+
+
+# -------------------------------------------------------- state comparison
+PARTICLE_FIELDS = ["loc", "velocity", "momentum", "stress", "strain", "strain_rate", "dstrain",
+                   "volume", "density", "mass", "reference_loc", "dvolumetric_strain",
+                   "volumetric_strain_centroid", "size", "traction"]
+NODE_FIELDS = ["mass", "velocity", "acceleration", "momentum", "f_int", "f_ext"]
+
+
+def gpu_particle_field(ps, name):
+    v = getattr(ps, name).detach().double().cpu().numpy()
+    return v.reshape(v.shape[0], -1) if v.ndim > 1 else v
+
+
+def oracle_particle_field(ps, name):
+    v = np.asarray(getattr(ps, name), dtype=np.float64)
+    return v.reshape(v.shape[0], -1)
+
+
+def gpu_node_field(nodes, name):
+    v = getattr(nodes, name).detach().double().cpu().numpy()
+    return v.reshape(v.shape[0], -1)
+
+
+def oracle_node_field(nodes, name):
+    v = np.asarray(getattr(nodes, name), dtype=np.float64)
+    return v.reshape(v.shape[0], -1)
+
+
+def normwise(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    den = np.linalg.norm(b)
+    num = np.linalg.norm(a - b)
+    if den == 0:
+        return num
+    return num / den
+
+
+def load_oracle_state_into_gpu(osim, gsim):
+    """Copies the oracle's full particle state and the persistent nodal
+    velocity into the GPU objects (for per-step, re-synchronised parity)."""
+    import torch
+    ops, gps = osim.mesh.particles[0], gsim.mesh.particles[0]
+    for name in PARTICLE_FIELDS + ["acceleration", "f_ext"]:
+        v = np.asarray(getattr(ops, name))
+        view = getattr(gps, name)
+        setattr(gps, name, torch.from_numpy(v.reshape(view.shape).astype(np.float64)))
+    gps._has_fext = bool(np.any(ops.f_ext))
+    gps.element_ids = torch.from_numpy(ops.element_ids.astype(np.int32))
+    gn, on = gsim.mesh.elements.nodes, osim.mesh.elements.nodes
+    for name in NODE_FIELDS:
+        v = np.asarray(getattr(on, name))
+        setattr(gn, name, torch.from_numpy(v.reshape(getattr(gn, name).shape).astype(np.float64)))
+
+
+# ------------------------------------------------------- synthetic cases
+def synthetic_spec(element="quad4", nx=8, ny=6, rows=4, seed=0, scheme="usf",
+                   velocity_update=False, jitter=0.2, traction=True, nodal_force=True,
+                   gravity=True, E=1e6, nu=0.3, dt=1e-5, nsteps=10, per_particle_E=False,
+                   velocity_scale=0.5):
+    """A small seeded case that exercises every term of the step."""
+    from oracle.mpm import gauss_points_2d
+    rng = np.random.Generator(np.random.PCG64(seed))
+    lin = {"type": "Linear", "xvalues": [0.0, 5 * dt, 1.0], "fxvalues": [0.0, 1.0, 1.0]}
+    if element == "quad4":
+        h = 1.0 / max(nx, ny)
+        pts = gauss_points_2d(nx, ny, h, h, rows=rows, jitter=jitter, seed=seed,
+                              dtype=np.float64)
+        n = len(pts)
+        spec = dict(element="quad4", nelements=[nx, ny], el_len=[h, h], particles=pts.tolist())
+        bottom = list(range(nx + 1))
+        left = [j * (nx + 1) for j in range(ny + 1)]
+        top = [rows * (nx + 1) + i for i in range(nx + 1)]
+        spec["constraints"] = [{"nodes": bottom, "dir": 1, "velocity": 0.0},
+                               {"nodes": left, "dir": 0, "velocity": 0.0}]
+        spec["nodal_forces"] = ([{"nodes": top, "dir": 1, "force": -10.0, "function": lin}]
+                                if nodal_force else [])
+        spec["gravity"] = [0.0, -9.81] if gravity else [0.0, 0.0]
+        vel = velocity_scale * rng.standard_normal((n, 2))
+    else:
+        h = 1.0 / nx
+        g = (1.0 - 1.0 / math.sqrt(3.0)) / 2.0
+        base = np.arange(nx)
+        pts = np.stack([base + g, base + 1 - g], axis=1).reshape(-1, 1)
+        pts = pts + rng.uniform(-jitter * 0.5, jitter * 0.5, pts.shape) * g
+        pts = np.clip(pts * h, 0, np.nextafter(np.float32(nx * h), 0))
+        n = len(pts)
+        spec = dict(element="linear1d", nelements=nx, el_len=h, particles=pts.tolist())
+        spec["constraints"] = [{"nodes": [0], "dir": 0, "velocity": 0.0}]
+        spec["nodal_forces"] = ([{"nodes": [nx], "dir": 0, "force": 5.0, "function": lin}]
+                                if nodal_force else [])
+        spec["gravity"] = [-9.81] if gravity else [0.0]
+        vel = velocity_scale * rng.standard_normal((n, 1))
+    spec["init_velocity"] = vel.tolist()
+    spec["material"] = {"type": "LinearElastic", "density": 1000.0, "youngs_modulus": E,
+                        "poisson_ratio": nu}
+    tp = sorted(rng.choice(n, size=min(3, n), replace=False).tolist())
+    spec["tractions"] = ([{"pset": [0], "pids": tp, "dir": 0, "traction": 50.0,
+                           "function": lin}] if traction else [])
+    spec.update(scheme=scheme, dt=dt, nsteps=nsteps, velocity_update=velocity_update)
+    if per_particle_E:
+        spec["youngs"] = (E * np.exp(0.3 * rng.standard_normal(n))).tolist()
+    return spec
