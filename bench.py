@@ -22,6 +22,14 @@ HBM_PEAK_GBS = 8000.0
 # LinearElastic): the persistent particle fields the kernel must read and
 # write; grid arrays, sort keys and recomputable temporaries excluded.
 KERNEL_BYTES = {
+    # fused node-tile kernels (default path)
+    # P2G mass/momentum + nodal velocity: read x(2), v(2), m
+    "tile_kin": 4 * (2 + 2 + 1),
+    # USF strain/volume/stress update + force P2G + nodal update: read x(2), m,
+    # eps(3), sigma(4), V, rho, eps_v; write eps(3), sigma(4), V, rho, eps_v
+    "tile_force": 4 * (2 + 1 + 3 + 4 + 3) + 4 * (3 + 4 + 3),
+    # G2P: read x(2), v(2); write x(2), v(2)
+    "tile_g2p": 4 * 4 + 4 * 4,
     # strain/volume/stress update: read eps(3), sigma(4), V, rho, eps_v,
     # xi(2, read back from the locate kernel), write eps(3), sigma(4), V, rho, eps_v
     "stress": 4 * (3 + 4 + 3 + 2) + 4 * (3 + 4 + 3),
@@ -78,7 +86,8 @@ def main():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--n", type=int, default=1024)
-    ap.add_argument("--kernel", default="stress", help="kernel class timed for the roofline")
+    ap.add_argument("--kernel", default="tile_force", help="kernel class timed for the roofline")
+    ap.add_argument("--tile", type=int, default=0, help="node rows per tile (0: default)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -94,6 +103,8 @@ def main():
     ps = sim.mesh.particles[0]
     plan = plan_for(sim.mesh.elements, ps)
     N = len(ps)
+    if args.tile:
+        plan.set_tile(args.tile)
     print(f"# workload built: {N} particles, {args.n}x{args.n} cells", file=sys.stderr, flush=True)
     plan.run("usf", False, gravity, sim.dt, 0, args.warmup, keep_temporaries=False)
     torch.cuda.synchronize()

@@ -223,9 +223,14 @@ class Plan:
         call("dmpm_phase_particle_kinematics", self.h, self.pb, self.eid, self.nb, float(dt),
              1 if velocity_update else 0, self.s)
 
-    def run(self, scheme, velocity_update, gravity, dt, step0, nsteps, keep_temporaries=True):
+    def set_tile(self, node_rows):
+        call("dmpm_plan_set_tile", self.h, int(node_rows))
+
+    def run(self, scheme, velocity_update, gravity, dt, step0, nsteps, keep_temporaries=True,
+            per_phase=False):
         self.refresh_tables()
-        flags = (1 if keep_temporaries else 0) | (2 if self.particles._has_fext else 0)
+        flags = ((1 if keep_temporaries else 0) | (2 if self.particles._has_fext else 0)
+                 | (4 if per_phase else 0))
         code = _lib.DMPM_USF if scheme == "usf" else _lib.DMPM_USL
         call("dmpm_run", self.h, self.pb, self.eid, self.nb, code,
              1 if velocity_update else 0, self._gravity(gravity), float(dt), int(step0),

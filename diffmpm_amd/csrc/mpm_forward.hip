@@ -40,19 +40,7 @@ int hip_check(hipError_t e, const char* what) {
 int launch_check(const char* what) { return hip_check(hipGetLastError(), what); }
 }  // namespace dmpm
 
-#define DMPM_TRY(x)              \
-  do {                           \
-    int rc__ = (x);              \
-    if (rc__ != DMPM_OK) return rc__; \
-  } while (0)
 
-struct ProfScope {
-  dmpm_plan* P;
-  int cls;
-  hipStream_t s;
-  ProfScope(dmpm_plan* p, int c, hipStream_t st) : P(p), cls(c), s(st) { prof_begin(P, cls, s); }
-  ~ProfScope() { prof_end(P, cls, s); }
-};
 
 static inline unsigned nblocks(int64_t n, int b = kBlock) {
   return (unsigned)((n + b - 1) / b);
@@ -226,6 +214,7 @@ __global__ void k_sort_fix_big(const int* __restrict__ cs, int* __restrict__ per
 
 __global__ void k_sort_done(int* flags) {
   if (threadIdx.x == 0 && blockIdx.x == 0) {
+    if (flags[0]) flags[1] = 0;  // a new permutation: no longer known to be the identity
     flags[0] = 0;
     flags[3] = 0;
   }
@@ -826,7 +815,8 @@ void plan_free(dmpm_plan* P) {
   void* ptrs[] = {P->key, P->perm, P->counts, P->fill, P->cs, P->bsum, P->flags, P->big,
                   P->con_node, P->con_dir, P->con_val, P->nf_id, P->nf_dir, P->nf_off,
                   P->nf_fn, P->nf_val, P->tr_id, P->tr_dir, P->tr_off, P->tr_fn, P->tr_val,
-                  P->fn_kind, P->fn_off, P->fn_cnt, P->fn_x, P->fn_f};
+                  P->fn_kind, P->fn_off, P->fn_cnt, P->fn_x, P->fn_f, P->cidx, P->cmask,
+                  P->cval, P->nfd};
   for (void* p : ptrs) free_ptr(p);
   P->release_extra();
   delete P;
@@ -1013,6 +1003,23 @@ int dmpm_plan_set_constraints(dmpm_plan* P, int32_t n, const int32_t* node_ids,
   DMPM_TRY(dev_upload(&P->con_dir, dr));
   DMPM_TRY(dev_upload(&P->con_val, vl));
   P->ncon = (int)nd.size();
+  // dense per-node records for the tile kernels
+  std::vector<int> cidx(P->nnodes, -1), cmask;
+  std::vector<double> cval;
+  for (size_t k = 0; k < nd.size(); ++k) {
+    int& r = cidx[nd[k]];
+    if (r < 0) {
+      r = (int)cmask.size();
+      cmask.push_back(0);
+      cval.push_back(0.0);
+      cval.push_back(0.0);
+    }
+    cmask[r] |= 1 << dr[k];
+    cval[2 * r + dr[k]] = vl[k];
+  }
+  DMPM_TRY(dev_upload(&P->cidx, cidx));
+  DMPM_TRY(dev_upload(&P->cmask, cmask));
+  DMPM_TRY(dev_upload(&P->cval, cval));
   return DMPM_OK;
 }
 
@@ -1039,6 +1046,9 @@ int dmpm_plan_set_nodal_forces(dmpm_plan* P, int32_t n, const int32_t* node_ids,
   DMPM_TRY(dev_upload(&P->nf_fn, g.fn));
   DMPM_TRY(dev_upload(&P->nf_val, g.val));
   P->nnf = (int)g.id.size();
+  std::vector<int> nfd(2 * (size_t)P->nnodes, -1);
+  for (size_t q = 0; q < g.id.size(); ++q) nfd[(size_t)g.dir[q] * P->nnodes + g.id[q]] = (int)q;
+  DMPM_TRY(dev_upload(&P->nfd, nfd));
   return DMPM_OK;
 }
 
@@ -1066,6 +1076,12 @@ int dmpm_plan_set_tractions(dmpm_plan* P, int32_t n, const int32_t* pids, const 
   DMPM_TRY(dev_upload(&P->tr_val, g.val));
   P->ntr = (int)g.id.size();
   P->has_tractions = n > 0;
+  return DMPM_OK;
+}
+
+int dmpm_plan_set_tile(dmpm_plan* P, int32_t node_rows) {
+  if (!P || node_rows < 0 || node_rows > 4096) return fail(DMPM_EINVAL, "invalid tile size");
+  P->tile_ty = node_rows;
   return DMPM_OK;
 }
 
@@ -1482,6 +1498,8 @@ template <typename T, int DIM> struct RunSteps {
                  hipStream_t s) {
     const int temps = flags & 1;
     const int pfext = (flags >> 1) & 1;
+    if (!(flags & 4) && tile_eligible(P, pfext))
+      return tile_run(P, pbuf, eid, nbuf, scheme, vu, grav, dt, step0, nsteps, temps, s);
     for (int64_t i = step0; i < step0 + nsteps; ++i) {
       DMPM_TRY((PhaseKin<T, DIM>::run(P, pbuf, eid, nbuf, s)));
       if (scheme == DMPM_USF) DMPM_TRY((PhaseStress<T, DIM>::run(P, pbuf, eid, nbuf, dt, temps, s)));

@@ -38,6 +38,13 @@ struct dmpm_plan {
   bool has_tractions = false;
   int *tr_id = nullptr, *tr_dir = nullptr, *tr_off = nullptr, *tr_fn = nullptr;
   double* tr_val = nullptr;
+  // dense per-node views of the constraint and nodal-force tables for the
+  // tile kernels: cidx[node] -> record (cmask bit d: dof d fixed to cval[2r+d]),
+  // nfd[d*nnodes + node] -> nodal-force group or -1
+  int* cidx = nullptr;
+  int* cmask = nullptr;
+  double* cval = nullptr;
+  int* nfd = nullptr;
   // time functions
   int nfn = 0;
   int *fn_kind = nullptr, *fn_off = nullptr, *fn_cnt = nullptr;
@@ -49,6 +56,7 @@ struct dmpm_plan {
   double prof_ms = 0;
   int prof_pending[4] = {0, 0, 0, 0};
   std::vector<hipEvent_t> prof_ev;
+  int tile_ty = 0;  // node rows per tile of the fused 2D kernels (0: default)
   // workspace of the fused tile kernels and the adjoint (allocated lazily)
   void* extra[16] = {nullptr};
   size_t extra_bytes[16] = {0};
@@ -63,12 +71,27 @@ struct dmpm_plan {
   }
 };
 
+#define DMPM_TRY(x)                   \
+  do {                                \
+    int rc__ = (x);                   \
+    if (rc__ != DMPM_OK) return rc__; \
+  } while (0)
+#define DMPM_TRY_H(x) DMPM_TRY(::dmpm::hip_check((x), #x))
+
 namespace dmpm {
 int fail(int code, const std::string& msg);
 int hip_check(hipError_t e, const char* what);
 int launch_check(const char* what);
 int enqueue_sort(dmpm_plan* P, hipStream_t s);
 int ensure_sorted(dmpm_plan* P, const int32_t* eid, hipStream_t s);
+// Returns extra workspace slot `slot` with at least `bytes` bytes (allocated
+// on first use, kept for the plan's lifetime).
+void* plan_workspace(dmpm_plan* P, int slot, size_t bytes);
+// Node-tile step (mpm_tile.hip): 2D, no particle tractions or particle f_ext.
+int tile_run(dmpm_plan* P, void* pbuf, int32_t* eid, void* nbuf, int scheme, int vu,
+             const double* grav, double dt, int64_t step0, int64_t nsteps, int temps,
+             hipStream_t s);
+bool tile_eligible(const dmpm_plan* P, int pfext);
 // Kernel classes for dmpm_timing_enable.
 enum { KC_LOCATE = 0, KC_SORT = 1, KC_P2G_KIN = 2, KC_STRESS = 3, KC_P2G_FORCE = 4,
        KC_GRID = 5, KC_G2P = 6, KC_SMALL = 7, KC_TILE_KIN = 8, KC_TILE_FORCE = 9,
@@ -98,4 +121,12 @@ inline void prof_end(dmpm_plan* P, int cls, hipStream_t s) {
   P->prof_pending[slot] = 1;
   P->prof_n++;
 }
+struct ProfScope {
+  dmpm_plan* P;
+  int cls;
+  hipStream_t s;
+  ProfScope(dmpm_plan* p, int c, hipStream_t st) : P(p), cls(c), s(st) { prof_begin(P, cls, s); }
+  ~ProfScope() { prof_end(P, cls, s); }
+};
 }  // namespace dmpm
+using ProfScopeT = dmpm::ProfScope;

@@ -131,6 +131,9 @@ int dmpm_plan_set_nodal_forces(dmpm_plan* plan, int32_t n, const int32_t* node_i
 int dmpm_plan_set_tractions(dmpm_plan* plan, int32_t n, const int32_t* pids,
                             const int32_t* dirs, const int32_t* fn,
                             const double* tractions);
+/* Node rows per tile of the fused 2D kernels (0 = default 16); a tuning knob
+ * with no effect on results.  No reference counterpart. */
+int dmpm_plan_set_tile(dmpm_plan* plan, int32_t node_rows);
 /* Declares that the element ids changed outside the library (forces a re-sort). */
 int dmpm_mark_unsorted(dmpm_plan* plan, void* stream);
 
@@ -205,7 +208,10 @@ int dmpm_phase_particle_kinematics(dmpm_plan* plan, void* pbuf, const int32_t* e
                                    void* stream);
 /* nsteps explicit steps starting at step index step0 (solver.py:179-213),
  * USF or USL.  flags bit 0: also materialise the per-step temporaries
- * (strain_rate, dstrain, dvolumetric_strain, reference_loc, momentum). */
+ * (strain_rate, dstrain, dvolumetric_strain, reference_loc, momentum);
+ * bit 1: particle f_ext is non-zero; bit 2: use the per-phase kernels even
+ * where the fused node-tile kernels apply (2D, no particle tractions, zero
+ * particle f_ext).  Both paths give bit-identical results. */
 int dmpm_run(dmpm_plan* plan, void* pbuf, int32_t* eid, void* nbuf, int32_t scheme,
              int32_t velocity_update, const double* gravity, double dt, int64_t step0,
              int64_t nsteps, int32_t flags, void* stream);

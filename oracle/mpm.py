@@ -46,6 +46,34 @@ def scatter_add(arr, idx, vals):
     np.add.at(arr, i[ok], v[ok])
 
 
+def lu_inverse(J):
+    """Batched inverse of 1x1 / 2x2 matrices by LU with partial pivoting (as
+    ``jnp.linalg.inv`` computes it); singular matrices give inf/NaN entries
+    instead of raising, like JAX."""
+    J = np.asarray(J)
+    out = np.empty_like(J)
+    with np.errstate(divide="ignore", invalid="ignore", over="ignore"):
+        if J.shape[-1] == 1:
+            out[..., 0, 0] = 1 / J[..., 0, 0]
+            return out
+        a, b, c, d = J[..., 0, 0], J[..., 0, 1], J[..., 1, 0], J[..., 1, 1]
+        swap = np.abs(c) > np.abs(a)
+        p, q = np.where(swap, c, a), np.where(swap, d, b)   # pivot row
+        r, s = np.where(swap, a, c), np.where(swap, b, d)   # other row
+        lf = r / p
+        u22 = s - lf * q
+        # columns of the inverse: solve for e0 and e1 (rows permuted if swap)
+        for col in range(2):
+            e_p = np.where(swap, 1.0 if col == 1 else 0.0, 1.0 if col == 0 else 0.0)
+            e_r = np.where(swap, 1.0 if col == 0 else 0.0, 1.0 if col == 1 else 0.0)
+            y1 = e_r - lf * e_p
+            x1 = y1 / u22
+            x0 = (e_p - q * x1) / p
+            out[..., 0, col] = x0
+            out[..., 1, col] = x1
+    return out
+
+
 def snap(x):
     """``where(|x| < 1e-12, 0, x)`` (element.py:185-189)."""
     return np.where(np.abs(x) < SNAP, np.zeros_like(x), x)
@@ -271,7 +299,7 @@ class Elements:
         J = dN/dxi^T @ X_e;  dN/dx = dN/dxi @ inv(J)^T (LU inverse)."""
         g = self.shapefn_natural_grad(xi)
         J = np.einsum("nka,nkb->nab", g, coords)
-        Jinv = np.linalg.inv(J)
+        Jinv = lu_inverse(J)
         return np.einsum("nka,nca->nkc", g, Jinv)
 
     # -- element search (A1) -------------------------------------------------
@@ -559,7 +587,8 @@ class Particles:
             nu = self.material.properties["poisson_ratio"]
             c = LinearElastic({"density": 1.0, "youngs_modulus": 1.0,
                                "poisson_ratio": nu}).de64.astype(self.dtype)
-            dsig = np.einsum("kl,nl->nk", c, self.dstrain) * E[:, None]
+            # coefficients scaled per particle, then De_p @ dstrain
+            dsig = np.einsum("nkl,nl->nk", c[None, :, :] * E[:, None, None], self.dstrain)
             self.stress = self.stress + dsig
             return
         self.stress = self.stress + self.material.compute_stress(self.dstrain)

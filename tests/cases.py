@@ -184,7 +184,7 @@ def load_oracle_state_into_gpu(osim, gsim):
 def synthetic_spec(element="quad4", nx=8, ny=6, rows=4, seed=0, scheme="usf",
                    velocity_update=False, jitter=0.2, traction=True, nodal_force=True,
                    gravity=True, E=1e6, nu=0.3, dt=1e-5, nsteps=10, per_particle_E=False,
-                   velocity_scale=0.5):
+                   velocity_scale=0.5, drift=None):
     """A small seeded case that exercises every term of the step."""
     from oracle.mpm import gauss_points_2d
     rng = np.random.Generator(np.random.PCG64(seed))
@@ -218,6 +218,8 @@ def synthetic_spec(element="quad4", nx=8, ny=6, rows=4, seed=0, scheme="usf",
                                 if nodal_force else [])
         spec["gravity"] = [-9.81] if gravity else [0.0]
         vel = velocity_scale * rng.standard_normal((n, 1))
+    if drift is not None:
+        vel = vel + np.asarray(drift, dtype=np.float64)[: vel.shape[1]]
     spec["init_velocity"] = vel.tolist()
     spec["material"] = {"type": "LinearElastic", "density": 1000.0, "youngs_modulus": E,
                         "poisson_ratio": nu}
