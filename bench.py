@@ -88,6 +88,7 @@ def main():
     ap.add_argument("--n", type=int, default=1024)
     ap.add_argument("--kernel", default="tile_force", help="kernel class timed for the roofline")
     ap.add_argument("--tile", type=int, default=0, help="node rows per tile (0: default)")
+    ap.add_argument("--tile-cols", type=int, default=0, help="node columns per tile (64/128)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -103,8 +104,8 @@ def main():
     ps = sim.mesh.particles[0]
     plan = plan_for(sim.mesh.elements, ps)
     N = len(ps)
-    if args.tile:
-        plan.set_tile(args.tile)
+    if args.tile or args.tile_cols:
+        plan.set_tile(args.tile, args.tile_cols)
     print(f"# workload built: {N} particles, {args.n}x{args.n} cells", file=sys.stderr, flush=True)
     plan.run("usf", False, gravity, sim.dt, 0, args.warmup, keep_temporaries=False)
     torch.cuda.synchronize()

@@ -223,8 +223,8 @@ class Plan:
         call("dmpm_phase_particle_kinematics", self.h, self.pb, self.eid, self.nb, float(dt),
              1 if velocity_update else 0, self.s)
 
-    def set_tile(self, node_rows):
-        call("dmpm_plan_set_tile", self.h, int(node_rows))
+    def set_tile(self, node_rows, node_cols=0):
+        call("dmpm_plan_set_tile", self.h, int(node_rows), int(node_cols))
 
     def run(self, scheme, velocity_update, gravity, dt, step0, nsteps, keep_temporaries=True,
             per_phase=False):

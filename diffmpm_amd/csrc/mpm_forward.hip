@@ -64,9 +64,10 @@ __global__ void k_keys_from_eid(const int* __restrict__ eid, int n, int ncells,
 }
 
 __global__ void k_sort_zero(int* __restrict__ counts, int* __restrict__ fill, int nb,
-                            const int* __restrict__ flags) {
+                            int* __restrict__ flags) {
   if (flags[0] == 0) return;
   int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0) flags[2] = 0;
   if (i < nb) {
     counts[i] = 0;
     fill[i] = 0;
@@ -166,7 +167,7 @@ __global__ void k_sort_fix(const int* __restrict__ cs, int ncells, int* __restri
                            int* __restrict__ flags, int* __restrict__ big) {
   if (flags[0] == 0) return;
   int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= ncells) return;
+  if (c > ncells) return;  // cells and the outside bucket
   int b = cs[c], e = cs[c + 1];
   if (e - b > kSmallCell) {
     int k = atomicAdd(&flags[3], 1);
@@ -212,9 +213,21 @@ __global__ void k_sort_fix_big(const int* __restrict__ cs, int* __restrict__ per
   }
 }
 
+__global__ void k_sort_iperm(const int* __restrict__ perm, int n, int* __restrict__ iperm,
+                             const int* __restrict__ flags) {
+  if (flags[0] == 0) return;
+  for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < n; s += gridDim.x * blockDim.x)
+    iperm[perm[s]] = s;
+}
+
 __global__ void k_sort_done(int* flags) {
   if (threadIdx.x == 0 && blockIdx.x == 0) {
-    if (flags[0]) flags[1] = 0;  // a new permutation: no longer known to be the identity
+    if (flags[0]) {
+      flags[1] = 0;  // a new permutation: no longer known to be the identity
+      flags[4] = 0;  // it is in buffer 0
+      flags[5] = 0;  // pending cell changes are included
+      flags[6] = 0;
+    }
     flags[0] = 0;
     flags[3] = 0;
   }
@@ -233,9 +246,10 @@ int enqueue_sort(dmpm_plan* P, hipStream_t s) {
     k_scan_sums<<<1, kBlock, 0, s>>>(P->bsum, nbs, P->flags);
     k_scan_add<<<nblocks(nb), kBlock, 0, s>>>(P->cs, nb, n, P->bsum, P->flags);
     k_sort_scatter<<<nblocks(n), kBlock, 0, s>>>(P->key, n, P->cs, P->fill, P->perm, P->flags);
-    k_sort_fix<<<nblocks(P->ncells), kBlock, 0, s>>>(P->cs, P->ncells, P->perm, P->flags,
+    k_sort_fix<<<nblocks(P->ncells + 1), kBlock, 0, s>>>(P->cs, P->ncells, P->perm, P->flags,
                                                      P->big);
     k_sort_fix_big<<<256, kBlock, 0, s>>>(P->cs, P->perm, P->flags, P->big);
+    k_sort_iperm<<<1024, kBlock, 0, s>>>(P->perm, n, P->iperm, P->flags);
   }
   k_sort_done<<<1, 64, 0, s>>>(P->flags);
   return launch_check("sort");
@@ -310,10 +324,10 @@ __global__ void k_natural_coords(Geo<T> g, T* __restrict__ pb, int64_t ld, int n
 // particle.py:158-178.  Counts come from the sorted cell offsets.
 template <typename T, int DIM>
 __global__ void k_particle_volume(Geo<T> g, T* __restrict__ pb, int64_t ld, int n,
-                                  const int* __restrict__ eid, const int* __restrict__ cs,
-                                  T vol_e) {
+                                  const int* __restrict__ eid, SortRef sr, T vol_e) {
   int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= n) return;
+  const int* __restrict__ cs = sr.cs();
   int e = eid[p];
   T V;
   if (e < 0 || e >= g.ncells) {
@@ -593,11 +607,12 @@ __device__ __forceinline__ T corner_N(int corner, T xi, T eta) {
 }
 
 template <typename T, int DIM>
-__global__ void k_p2g_kin(Geo<T> g, const T* __restrict__ pb, int64_t ld,
-                          const int* __restrict__ perm, const int* __restrict__ cs,
+__global__ void k_p2g_kin(Geo<T> g, const T* __restrict__ pb, int64_t ld, SortRef sr,
                           T* __restrict__ nb, int64_t nld, int mode) {
   int nd = blockIdx.x * blockDim.x + threadIdx.x;
   if (nd >= g.nnodes) return;
+  const int* __restrict__ perm = sr.perm();
+  const int* __restrict__ cs = sr.cs();
   CellList<DIM> L = node_cells<T, DIM>(g, nd);
   T am = T(0), apx = T(0), apy = T(0);
   if (mode & (K_MASS | K_MOM)) {
@@ -638,11 +653,12 @@ __global__ void k_p2g_kin(Geo<T> g, const T* __restrict__ pb, int64_t ld,
 }
 
 template <typename T, int DIM>
-__global__ void k_p2g_force(Geo<T> g, const T* __restrict__ pb, int64_t ld,
-                            const int* __restrict__ perm, const int* __restrict__ cs,
+__global__ void k_p2g_force(Geo<T> g, const T* __restrict__ pb, int64_t ld, SortRef sr,
                             T* __restrict__ nb, int64_t nld, int mode, T gx, T gy) {
   int nd = blockIdx.x * blockDim.x + threadIdx.x;
   if (nd >= g.nnodes) return;
+  const int* __restrict__ perm = sr.perm();
+  const int* __restrict__ cs = sr.cs();
   CellList<DIM> L = node_cells<T, DIM>(g, nd);
   if (mode & (F_EXT_ZERO | F_PFEXT | F_BODY | F_TRAC)) {
     T fx = (mode & F_EXT_ZERO) ? T(0) : NROW(DMPM_N_FEXT)[nd];
@@ -813,6 +829,7 @@ static void free_ptr(void* p) {
 void plan_free(dmpm_plan* P) {
   if (!P) return;
   void* ptrs[] = {P->key, P->perm, P->counts, P->fill, P->cs, P->bsum, P->flags, P->big,
+                  P->perm2, P->cs2, P->iperm, P->mov, P->msort, P->mkey,
                   P->con_node, P->con_dir, P->con_val, P->nf_id, P->nf_dir, P->nf_off,
                   P->nf_fn, P->nf_val, P->tr_id, P->tr_dir, P->tr_off, P->tr_fn, P->tr_val,
                   P->fn_kind, P->fn_off, P->fn_cnt, P->fn_x, P->fn_f, P->cidx, P->cmask,
@@ -905,12 +922,19 @@ int dmpm_plan_create(const dmpm_plan_desc* desc, dmpm_plan** out) {
   al(&P->fill, nb);
   al(&P->cs, nb + 1);
   al(&P->bsum, nbs);
-  al(&P->flags, 4);
+  al(&P->flags, 8);
   al(&P->big, P->ncells);
+  al(&P->perm2, n);
+  al(&P->cs2, nb + 1);
+  al(&P->iperm, n);
+  al(&P->mov, 3 * kMovCap);
+  al(&P->msort, 3 * kMovCap);
+  if (rc == DMPM_OK)
+    rc = hip_check(hipMalloc((void**)&P->mkey, kMovCap * sizeof(unsigned long long)), "hipMalloc");
   if (rc == DMPM_OK) rc = hip_check(hipMemset(P->key, 0xff, n * sizeof(int)), "hipMemset");
   if (rc == DMPM_OK) rc = hip_check(hipMemset(P->cs, 0, (nb + 1) * sizeof(int)), "hipMemset");
   if (rc == DMPM_OK) rc = hip_check(hipMemset(P->perm, 0, n * sizeof(int)), "hipMemset");
-  int f0[4] = {1, 0, 0, 0};
+  int f0[8] = {1, 0, 0, 0, 0, 0, 0, 0};
   if (rc == DMPM_OK)
     rc = hip_check(hipMemcpy(P->flags, f0, sizeof(f0), hipMemcpyHostToDevice), "hipMemcpy");
   if (rc != DMPM_OK) {
@@ -1079,9 +1103,12 @@ int dmpm_plan_set_tractions(dmpm_plan* P, int32_t n, const int32_t* pids, const 
   return DMPM_OK;
 }
 
-int dmpm_plan_set_tile(dmpm_plan* P, int32_t node_rows) {
+int dmpm_plan_set_tile(dmpm_plan* P, int32_t node_rows, int32_t node_cols) {
   if (!P || node_rows < 0 || node_rows > 4096) return fail(DMPM_EINVAL, "invalid tile size");
+  if (node_cols != 0 && node_cols != 64 && node_cols != 128)
+    return fail(DMPM_EINVAL, "tile columns must be 64 or 128 (0: keep)");
   P->tile_ty = node_rows;
+  if (node_cols) P->tile_tx = node_cols;
   return DMPM_OK;
 }
 
@@ -1119,13 +1146,13 @@ template <typename T, int DIM> struct Ops {
     DMPM_TRY(ensure_sorted(P, eid, s));
     if (P->n > 0)
       k_particle_volume<T, DIM><<<nblocks(P->n), kBlock, 0, s>>>(
-          make_geo<T>(P), (T*)pbuf, P->d.ld, (int)P->n, eid, P->cs, T(ve));
+          make_geo<T>(P), (T*)pbuf, P->d.ld, (int)P->n, eid, sort_ref(P), T(ve));
     return launch_check("particle_volume");
   }
   static int kin(dmpm_plan* P, const void* pbuf, void* nbuf, int mode, hipStream_t s) {
     ProfScope ps_(P, KC_P2G_KIN, s);
     k_p2g_kin<T, DIM><<<nblocks(P->nnodes), kBlock, 0, s>>>(
-        make_geo<T>(P), (const T*)pbuf, P->d.ld, P->perm, P->cs, (T*)nbuf, P->d.nld, mode);
+        make_geo<T>(P), (const T*)pbuf, P->d.ld, sort_ref(P), (T*)nbuf, P->d.nld, mode);
     return launch_check("p2g_kin");
   }
   static int force(dmpm_plan* P, const void* pbuf, void* nbuf, int mode, const double* grav,
@@ -1134,7 +1161,7 @@ template <typename T, int DIM> struct Ops {
     T gx = grav ? T(grav[0]) : T(0);
     T gy = (grav && DIM == 2) ? T(grav[1]) : T(0);
     k_p2g_force<T, DIM><<<nblocks(P->nnodes), kBlock, 0, s>>>(
-        make_geo<T>(P), (const T*)pbuf, P->d.ld, P->perm, P->cs, (T*)nbuf, P->d.nld, mode, gx,
+        make_geo<T>(P), (const T*)pbuf, P->d.ld, sort_ref(P), (T*)nbuf, P->d.nld, mode, gx,
         gy);
     return launch_check("p2g_force");
   }

@@ -22,8 +22,18 @@ struct dmpm_plan {
   int* fill = nullptr;    // [ncells+1]
   int* cs = nullptr;      // [ncells+2] first slot of each cell (+ total)
   int* bsum = nullptr;    // scan block totals
-  int* flags = nullptr;   // [0] sort dirty, [2] unstable big cell seen, [3] big-cell count
+  // flags: [0] full re-sort needed, [1] perm is the identity, [2] a cell too big
+  // for the deterministic fix-up was seen, [3] big-cell count, [4] current
+  // buffer of perm/cs (0 or 1), [5] cell changes recorded since the last sort,
+  // [6] more changes than kMovCap (full re-sort needed)
+  int* flags = nullptr;
   int* big = nullptr;     // [ncells] cells queued for the big-cell sort
+  int* perm2 = nullptr;   // second buffer of perm (incremental re-sort)
+  int* cs2 = nullptr;     // second buffer of cs
+  int* iperm = nullptr;   // [n] slot of each particle in the current perm
+  int* mov = nullptr;     // [3*kMovCap] recorded cell changes (particle, old key, new key)
+  int* msort = nullptr;   // [3*kMovCap] sorted old slots, old keys, new keys
+  unsigned long long* mkey = nullptr;  // [kMovCap] sorted (new key << 32 | particle)
   // velocity constraints, last value per (node, dir)
   int ncon = 0;
   int* con_node = nullptr;
@@ -57,6 +67,7 @@ struct dmpm_plan {
   int prof_pending[4] = {0, 0, 0, 0};
   std::vector<hipEvent_t> prof_ev;
   int tile_ty = 0;  // node rows per tile of the fused 2D kernels (0: default)
+  int tile_tx = 64; // node columns per tile (64 or 128)
   // workspace of the fused tile kernels and the adjoint (allocated lazily)
   void* extra[16] = {nullptr};
   size_t extra_bytes[16] = {0};
@@ -79,6 +90,28 @@ struct dmpm_plan {
 #define DMPM_TRY_H(x) DMPM_TRY(::dmpm::hip_check((x), #x))
 
 namespace dmpm {
+constexpr int kMovCap = 4096;  // cell changes merged incrementally per step
+
+// The current sort buffers (selected on the device by flags[4]).
+struct SortRef {
+  const int* perm0;
+  const int* perm1;
+  const int* cs0;
+  const int* cs1;
+  const int* flags;
+  __device__ __forceinline__ const int* perm() const { return flags[4] ? perm1 : perm0; }
+  __device__ __forceinline__ const int* cs() const { return flags[4] ? cs1 : cs0; }
+};
+inline SortRef sort_ref(const dmpm_plan* P) {
+  SortRef r;
+  r.perm0 = P->perm;
+  r.perm1 = P->perm2;
+  r.cs0 = P->cs;
+  r.cs1 = P->cs2;
+  r.flags = P->flags;
+  return r;
+}
+
 int fail(int code, const std::string& msg);
 int hip_check(hipError_t e, const char* what);
 int launch_check(const char* what);

@@ -1,8 +1,9 @@
 // Fused 2D explicit step on node tiles (gfx950).
 //
 // One step is four launches:
-//   k_sort_coop     cooperative counting sort by cell; returns at once unless
-//                   a particle changed cell in the previous step (or at start)
+//   k_sort_coop     cooperative sort by cell: nothing when no particle changed
+//                   cell, an O(N) merge of the recorded changes into the stable
+//                   order when a few did, a full counting sort otherwise
 //   k_tile<KIN>     P2G of mass and momentum, nodal velocity, constraints
 //   k_tile<FORCE_*> (USF: strain/volume/stress update of the tile's particles)
 //                   P2G of body and internal forces, nodal forces, nodal
@@ -35,14 +36,15 @@ using namespace dmpm;
 
 namespace {
 
-constexpr int kTX = 128;      // node columns per tile: 2 x kTX threads
-constexpr int kThreads = 256;
+constexpr int kThreads = 256;  // sort / G2P workgroups
 constexpr int kDB = 10;       // double-buffered particle fields (USF)
 constexpr int kSmallCell = 64;
 constexpr int kBigCellMax = 8192;
 
-template <typename T> struct Chunk { static constexpr int value = 1024; };
-template <> struct Chunk<double> { static constexpr int value = 512; };
+// LDS particle capacity per chunk: ~4 rows of particles per node column.
+template <typename T, int TXN> struct Chunk {
+  static constexpr int value = (sizeof(T) == 4 ? 8 : 4) * TXN;
+};
 
 // eps_xx, eps_yy, gamma_xy, sig_xx, sig_yy, sig_zz, sig_xy, V, rho, eps_v
 __host__ __device__ constexpr int db_row(int k) {
@@ -143,23 +145,27 @@ __device__ __forceinline__ void rate2(const Grid2<T>& g, int I, int J, T xi, T e
 
 enum { MODE_KIN = 0, MODE_FORCE_USF = 1, MODE_FORCE_USL = 2 };
 
-template <typename T, int MODE>
-__global__ __launch_bounds__(kThreads) void k_tile(
-    Grid2<T> g, TileShape tl, T* __restrict__ pb, int64_t ld, const int* __restrict__ perm,
-    const int* __restrict__ cs, const int* __restrict__ flags, T* __restrict__ nb, int64_t nld,
+template <typename T, int MODE, int kTX>
+__global__ __launch_bounds__(2 * kTX) void k_tile(
+    Grid2<T> g, TileShape tl, T* __restrict__ pb, int64_t ld, SortRef sr,
+    T* __restrict__ nb, int64_t nld,
     T* __restrict__ vkin, const int* __restrict__ cidx, const int* __restrict__ cmask,
     const double* __restrict__ cval, DB<T> db, Mat2 mat, T gx, T gy, T dt, T t,
     const int* __restrict__ nfd, const int* __restrict__ nf_off, const int* __restrict__ nf_fn,
     const double* __restrict__ nf_val, FnTable fns, int temps) {
-  constexpr int CH = Chunk<T>::value;
-  __shared__ T s_xi[CH], s_eta[CH], s_ix[CH], s_iy[CH], s_q[5][CH];
-  __shared__ int s_cs[kTX + 2];
+  constexpr int CH = Chunk<T, kTX>::value;
+  constexpr int NT = 2 * kTX;
+  __shared__ T s_xi[CH], s_eta[CH], s_ix[MODE == MODE_KIN ? 1 : CH],
+      s_iy[MODE == MODE_KIN ? 1 : CH], s_q[MODE == MODE_KIN ? 3 : 5][CH];
+  __shared__ int s_cs[2][kTX + 2];
 
-  const int ident = flags[1];
+  const int ident = sr.flags[1];
+  const int* __restrict__ perm = sr.perm();
+  const int* __restrict__ cs = sr.cs();
   const int tile = tile_of_block(blockIdx.x, gridDim.x);
   const int tyi = tile % tl.nty, txi = tile / tl.nty;
   const int i0 = txi * kTX, j0 = tyi * tl.ty;
-  const int tid = threadIdx.x, grp = tid / kTX, col = tid % kTX;
+  const int tid = threadIdx.x, grp = tid / kTX, col = tid % kTX;  // blockDim.x == 2*kTX
   const int i = i0 + col;
   const bool icol = i <= g.nx;
   const int c0 = max(i0 - 1, 0), c1 = min(i0 + kTX, g.nx);
@@ -167,21 +173,28 @@ __global__ __launch_bounds__(kThreads) void k_tile(
 
   T acc[4] = {T(0), T(0), T(0), T(0)};
 
+  // cell offsets of the row segment, prefetched one cell row ahead
+  const int rfirst = max(j0 - 1, 0), rlast = min(j0 + tl.ty, g.ny);  // [rfirst, rlast)
+  int cs_next = (tid <= ncell && rfirst < rlast) ? cs[rfirst * g.nx + c0 + tid] : 0;
+
   for (int r = j0 - 1; r < j0 + tl.ty; ++r) {
     const bool top = grp == ((r + 1) & 1);
     const int my_row = top ? r + 1 : r;
     const bool node_ok = icol && my_row >= j0 && my_row < j0 + tl.ty && my_row <= g.ny;
     if (r >= 0 && r < g.ny) {
+      const int cb_ = (r - rfirst) & 1;
       __syncthreads();
-      for (int k = tid; k <= ncell; k += kThreads) s_cs[k] = cs[r * g.nx + c0 + k];
+      if (tid <= ncell) s_cs[cb_][tid] = cs_next;
+      if (tid <= ncell && r + 1 < rlast) cs_next = cs[(r + 1) * g.nx + c0 + tid];
       __syncthreads();
-      const int sb = s_cs[0], se = s_cs[ncell];
+      const int* __restrict__ rcs = s_cs[cb_];
+      const int sb = rcs[0], se = rcs[ncell];
       const bool row_owned = r >= j0 && r < j0 + tl.ty;
       for (int cb = sb; cb < se; cb += CH) {
         const int ce = min(cb + CH, se);
         if (cb != sb) __syncthreads();
         // ---- stage the chunk's particles -------------------------------------
-        for (int s = cb + tid; s < ce; s += kThreads) {
+        for (int s = cb + tid; s < ce; s += NT) {
           const int p = ident ? s : perm[s];
           const int q = s - cb;
           const T x = PR(DMPM_P_X)[p], y = PR(DMPM_P_X + 1)[p];
@@ -204,13 +217,14 @@ __global__ __launch_bounds__(kThreads) void k_tile(
           } else {
             T ix, iy;
             q4_invJ(xi, eta, g.X(I), g.X(I + 1), g.Y(J), g.Y(J + 1), ix, iy);
-            s_ix[q] = ix;
-            s_iy[q] = iy;
+            s_ix[q % (MODE == MODE_KIN ? 1 : CH)] = ix;
+            s_iy[q % (MODE == MODE_KIN ? 1 : CH)] = iy;
+            constexpr int q3 = MODE == MODE_KIN ? 0 : 3, q4 = MODE == MODE_KIN ? 0 : 4;
             if (MODE == MODE_FORCE_USL) {
               s_q[1][q] = -PR(DMPM_P_VOLUME)[p];
               s_q[2][q] = PR(DMPM_P_STRESS + 0)[p];
-              s_q[3][q] = PR(DMPM_P_STRESS + 1)[p];
-              s_q[4][q] = PR(DMPM_P_STRESS + 3)[p];
+              s_q[q3][q] = PR(DMPM_P_STRESS + 1)[p];
+              s_q[q4][q] = PR(DMPM_P_STRESS + 3)[p];
             } else {
               // USF strain / volume / stress update (particle.py:249-327)
               const int ll = J * (g.nx + 1) + I;
@@ -248,8 +262,8 @@ __global__ __launch_bounds__(kThreads) void k_tile(
               }
               s_q[1][q] = -nv[7];
               s_q[2][q] = nv[3];
-              s_q[3][q] = nv[4];
-              s_q[4][q] = nv[6];
+              s_q[q3][q] = nv[4];
+              s_q[q4][q] = nv[6];
             }
           }
         }
@@ -259,7 +273,7 @@ __global__ __launch_bounds__(kThreads) void k_tile(
           for (int dc = -1; dc <= 0; ++dc) {
             const int c = i + dc;
             if (c < c0 || c >= c1) continue;
-            const int lo = max(s_cs[c - c0], cb) - cb, hi = min(s_cs[c - c0 + 1], ce) - cb;
+            const int lo = max(rcs[c - c0], cb) - cb, hi = min(rcs[c - c0 + 1], ce) - cb;
             const int corner = top ? (dc < 0 ? 2 : 3) : (dc < 0 ? 1 : 0);
             for (int q = lo; q < hi; ++q) {
               const T xi = s_xi[q], eta = s_eta[q];
@@ -273,10 +287,12 @@ __global__ __launch_bounds__(kThreads) void k_tile(
                 acc[0] = acc[0] + N * (m * gx);          // element.py:254-256
                 acc[1] = acc[1] + N * (m * gy);
                 T dx, dy;
-                q4_dNdx(corner, xi, eta, s_ix[q], s_iy[q], dx, dy);
+                const int qq = q % (MODE == MODE_KIN ? 1 : CH);
+                q4_dNdx(corner, xi, eta, s_ix[qq], s_iy[qq], dx, dy);
+                const int q3 = MODE == MODE_KIN ? 0 : 3, q4 = MODE == MODE_KIN ? 0 : 4;
                 const T nV = s_q[1][q];
-                acc[2] = acc[2] + nV * (dx * s_q[2][q] + dy * s_q[4][q]);  // element.py:868-878
-                acc[3] = acc[3] + nV * (dy * s_q[3][q] + dx * s_q[4][q]);
+                acc[2] = acc[2] + nV * (dx * s_q[2][q] + dy * s_q[q4][q]);  // element.py:868-878
+                acc[3] = acc[3] + nV * (dy * s_q[q3][q] + dx * s_q[q4][q]);
               }
             }
           }
@@ -341,10 +357,25 @@ __global__ __launch_bounds__(kThreads) void k_tile(
 // G2P (particle.py:208-247); USL also updates strain/volume/stress with the
 // step-start natural coordinates and the updated nodal velocities
 // (scheme.py:127-133).  Sets flags[0] when a particle changes cell.
+// Records a particle whose cell (sort key) changed: (particle, old, new) is
+// appended for the incremental re-sort; past kMovCap a full sort is requested.
+__device__ __forceinline__ void record_move(int* __restrict__ flags, int* __restrict__ mov, int p,
+                                            int kold, int knew) {
+  const int k = atomicAdd(&flags[5], 1);
+  if (k < kMovCap) {
+    mov[3 * k] = p;
+    mov[3 * k + 1] = kold;
+    mov[3 * k + 2] = knew;
+  } else {
+    flags[6] = 1;
+  }
+}
+
 template <typename T, bool USL>
 __global__ __launch_bounds__(kThreads) void k_g2p_tile(Grid2<T> g, T* __restrict__ pb, int64_t ld,
                                                        int n, const T* __restrict__ nb,
                                                        int64_t nld, int* __restrict__ flags,
+                                                       int* __restrict__ mov,
                                                        DB<T> db, Mat2 mat, T dt, int vu,
                                                        int temps, int usf_db) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
@@ -380,7 +411,8 @@ __global__ __launch_bounds__(kThreads) void k_g2p_tile(Grid2<T> g, T* __restrict
       }
       PR(DMPM_P_DVOLSTRAIN)[p] = nan;
     }
-    flags[0] = 1;
+    // NaN positions search to element 0 (element.py:827-837)
+    record_move(flags, mov, p, g.ncells, 0);
     return;
   }
   const int I = e % g.nx, J = e / g.nx;
@@ -437,7 +469,8 @@ __global__ __launch_bounds__(kThreads) void k_g2p_tile(Grid2<T> g, T* __restrict
     }
   }
   const int e1 = element_id<T, 2>(g.geo(), x1, y1);
-  if (e1 != e) flags[0] = 1;
+  const int k1 = (e1 >= 0 && e1 < g.ncells) ? e1 : g.ncells;
+  if (k1 != e) record_move(flags, mov, p, e, k1);
 }
 
 // Element ids and sort keys from the positions at the start of a run; marks
@@ -473,31 +506,152 @@ __device__ __forceinline__ int blk_excl_scan(int v, int* sh, int& total) {
   return incl - v;
 }
 
-// Stable counting sort of the particles by cell, in one cooperative launch
-// (phases separated by grid barriers).  Returns immediately when flags[0] is
-// clear; then every block returns, so no barrier is reached.  Writes element
-// ids, keys, cell offsets cs and the permutation perm; flags[1] = 1 when perm
-// is the identity.
+// Number of entries of the ascending array a[0..n) that are < v.
+template <typename K>
+__device__ __forceinline__ int lower_count(const K* __restrict__ a, int n, K v) {
+  int lo = 0, hi = n;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (a[mid] < v) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+
+// Bitonic sort of m <= kMovCap keys in LDS by one workgroup (padded with the
+// maximum key), then written to out[0..m).
+template <typename K>
+__device__ void block_sort(K* __restrict__ sh, int m, K* __restrict__ out, K pad) {
+  int P2 = 1;
+  while (P2 < m) P2 <<= 1;
+  for (int i = threadIdx.x + m; i < P2; i += kThreads) sh[i] = pad;
+  __syncthreads();
+  for (int k = 2; k <= P2; k <<= 1) {
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      for (int i = threadIdx.x; i < P2; i += kThreads) {
+        const int l = i ^ j;
+        if (l > i) {
+          const bool up = (i & k) == 0;
+          const K a = sh[i], c = sh[l];
+          if ((a > c) == up) {
+            sh[i] = c;
+            sh[l] = a;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+  for (int i = threadIdx.x; i < m; i += kThreads) out[i] = sh[i];
+  __syncthreads();
+}
+
+enum { SORT_NONE = 0, SORT_INCREMENTAL = 1, SORT_FULL = 2 };
+
+// Stable sort of the particles by cell in one cooperative launch (phases
+// separated by grid barriers).  The mode is read from the flags, which no
+// block writes before the first barrier, so all blocks agree:
+//  * nothing to do: every block returns;
+//  * SORT_INCREMENTAL (1..kMovCap recorded cell changes): the new order is the
+//    merge of the old order without the movers and the movers sorted by
+//    (new cell, index); every slot finds its new position by binary searches
+//    in the sorted mover lists, one streaming pass, written to the other
+//    perm/cs buffer;
+//  * SORT_FULL: counting sort (histogram, scan, scatter, per-cell fix-up to
+//    ascending index) into buffer 0.
+// Also maintains element ids (full sort), keys, iperm and the buffer flag.
 template <typename T>
 __global__ __launch_bounds__(kThreads) void k_sort_coop(
     Grid2<T> g, const T* __restrict__ pb, int64_t ld, int n, int* __restrict__ eid,
-    int* __restrict__ key, int* __restrict__ counts, int* __restrict__ fill, int* __restrict__ cs,
-    int* __restrict__ bs, int* __restrict__ perm, int* __restrict__ flags, int* __restrict__ big) {
-  if (flags[0] == 0) return;
+    int* __restrict__ key, int* __restrict__ counts, int* __restrict__ fill,
+    int* __restrict__ perm0, int* __restrict__ perm1, int* __restrict__ cs0,
+    int* __restrict__ cs1, int* __restrict__ iperm, int* __restrict__ bs,
+    int* __restrict__ flags, int* __restrict__ big, const int* __restrict__ mov,
+    int* __restrict__ msort, unsigned long long* __restrict__ mkey) {
+  const int nmov = flags[5];
+  // flags[2]: a cell (or the outside bucket) was too big to order by index, so
+  // the layout is not a stable sort and cannot be merged into
+  const int mode = (flags[0] || flags[6] || flags[2])
+                       ? SORT_FULL
+                       : (nmov > 0 ? SORT_INCREMENTAL : SORT_NONE);
+  if (mode == SORT_NONE) return;
   cg::grid_group grid = cg::this_grid();
-  __shared__ int sh[kBigCellMax];
+  __shared__ unsigned long long sh64[kMovCap];
+  int* sh = reinterpret_cast<int*>(sh64);  // kBigCellMax ints
   const int G = gridDim.x, b = blockIdx.x, tid = threadIdx.x;
   const int gt = b * kThreads + tid, gs = G * kThreads;
   const int nbk = g.ncells + 1;
+
+  if (mode == SORT_INCREMENTAL) {
+    const int cur = flags[4];
+    const int* __restrict__ po = cur ? perm1 : perm0;
+    int* __restrict__ pn = cur ? perm0 : perm1;
+    const int* __restrict__ co = cur ? cs1 : cs0;
+    int* __restrict__ cn = cur ? cs0 : cs1;
+    int* __restrict__ mS = msort;                 // old slots
+    int* __restrict__ mKO = msort + kMovCap;      // old keys
+    int* __restrict__ mKN = msort + 2 * kMovCap;  // new keys
+    if (b < 4) {
+      if (b == 3) {
+        for (int j = tid; j < nmov; j += kThreads)
+          sh64[j] = ((unsigned long long)(unsigned)mov[3 * j + 2] << 32) | (unsigned)mov[3 * j];
+        block_sort<unsigned long long>(sh64, nmov, mkey, ~0ull);
+      } else {
+        for (int j = tid; j < nmov; j += kThreads)
+          sh[j] = b == 0 ? iperm[mov[3 * j]] : mov[3 * j + b];
+        block_sort<int>(sh, nmov, b == 0 ? mS : (b == 1 ? mKO : mKN), 0x7fffffff);
+      }
+    }
+    grid.sync();
+    // non-movers keep their relative order
+    for (int s = gt; s < n; s += gs) {
+      const int jm = lower_count(mS, nmov, s);
+      if (jm < nmov && mS[jm] == s) continue;
+      const int p = po[s];
+      const unsigned long long kp = ((unsigned long long)(unsigned)key[p] << 32) | (unsigned)p;
+      const int pos = s - jm + lower_count(mkey, nmov, kp);
+      pn[pos] = p;
+      iperm[p] = pos;
+    }
+    // movers go before the first non-mover of their new cell with a larger index
+    for (int j = gt; j < nmov; j += gs) {
+      const unsigned long long kj = mkey[j];
+      const int kn = (int)(kj >> 32), p = (int)(kj & 0xffffffffu);
+      int lo = co[kn], hi = co[kn + 1];
+      while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (po[mid] < p) lo = mid + 1;
+        else hi = mid;
+      }
+      const int pos = (lo - lower_count(mS, nmov, lo)) + j;
+      pn[pos] = p;
+      iperm[p] = pos;
+      key[p] = kn;
+    }
+    for (int c = gt; c <= nbk; c += gs)
+      cn[c] = co[c] - lower_count(mKO, nmov, c) + lower_count(mKN, nmov, c);
+    grid.sync();
+    if (gt == 0) {
+      flags[4] = 1 - cur;
+      flags[5] = 0;
+      flags[1] = 0;
+    }
+    return;
+  }
+
+  // ---- full counting sort into buffer 0 ----
+  int* __restrict__ perm = perm0;
+  int* __restrict__ cs = cs0;
   for (int c = gt; c < nbk; c += gs) {
     counts[c] = 0;
     fill[c] = 0;
   }
   if (gt == 0) {
-    flags[1] = 1;
     flags[3] = 0;
+    flags[1] = 1;
   }
   grid.sync();
+  if (gt == 0) flags[2] = 0;  // after the barrier: every block has read it
   for (int p = gt; p < n; p += gs) {
     const int e = element_id<T, 2>(g.geo(), PR(DMPM_P_X)[p], PR(DMPM_P_X + 1)[p]);
     eid[p] = e;
@@ -509,10 +663,10 @@ __global__ __launch_bounds__(kThreads) void k_sort_coop(
   const int chunk = (nbk + G - 1) / G;
   const int lo = min(b * chunk, nbk), hi = min(lo + chunk, nbk);
   {
-    int s = 0;
-    for (int c = lo + tid; c < hi; c += kThreads) s += counts[c];
+    int sm = 0;
+    for (int c = lo + tid; c < hi; c += kThreads) sm += counts[c];
     int tot;
-    blk_excl_scan(s, sh, tot);
+    blk_excl_scan(sm, sh, tot);
     if (tid == 0) bs[b] = tot;
   }
   grid.sync();
@@ -546,7 +700,7 @@ __global__ __launch_bounds__(kThreads) void k_sort_coop(
     perm[cs[k] + atomicAdd(&fill[k], 1)] = p;
   }
   grid.sync();
-  for (int c = gt; c < g.ncells; c += gs) {
+  for (int c = gt; c <= g.ncells; c += gs) {  // cells and the outside bucket
     const int s0 = cs[c], s1 = cs[c + 1];
     if (s1 - s0 > kSmallCell) {
       big[atomicAdd(&flags[3], 1)] = c;
@@ -582,9 +736,19 @@ __global__ __launch_bounds__(kThreads) void k_sort_coop(
     __syncthreads();
   }
   grid.sync();
-  for (int s = gt; s < n; s += gs)
-    if (perm[s] != s) flags[1] = 0;
-  if (gt == 0) flags[0] = 0;
+  int ident = 1;
+  for (int s = gt; s < n; s += gs) {
+    iperm[perm[s]] = s;
+    if (perm[s] != s) ident = 0;
+  }
+  if (!ident) flags[1] = 0;
+  grid.sync();
+  if (gt == 0) {
+    flags[4] = 0;
+    flags[5] = 0;
+    flags[6] = 0;
+    flags[0] = 0;
+  }
 }
 
 #undef PR
@@ -613,24 +777,16 @@ int coop_grid_size(const void* kernel, int& out) {
   return DMPM_OK;
 }
 
-template <typename T>
-int tile_run_t(dmpm_plan* P, T* pb, int32_t* eid, T* nb, int scheme, int vu, const double* grav,
-               double dt, int64_t step0, int64_t nsteps, int temps, hipStream_t s) {
-  const Grid2<T> g = grid_of<T>(P);
+template <typename T, int TXN>
+int tile_steps(dmpm_plan* P, const Grid2<T>& g, T* pb, int32_t* eid, T* nb, int scheme, int vu,
+               const double* grav, double dt, int64_t step0, int64_t nsteps, int temps,
+               hipStream_t s, T* alt, T* vkin, int* bs, int G) {
   const int64_t ld = P->d.ld, nld = P->d.nld;
   const int n = (int)P->n;
   const bool usf = scheme == DMPM_USF;
-  T* alt = usf ? (T*)plan_workspace(P, 0, (size_t)kDB * ld * sizeof(T)) : nullptr;
-  T* vkin = (T*)plan_workspace(P, 1, (size_t)2 * P->nnodes * sizeof(T));
-  if ((usf && !alt) || !vkin) return fail(DMPM_EHIP, "workspace allocation failed");
-  int G = 0;
-  DMPM_TRY(coop_grid_size((const void*)k_sort_coop<T>, G));
-  int* bs = (int*)plan_workspace(P, 2, (size_t)G * sizeof(int));
-  if (!bs) return fail(DMPM_EHIP, "workspace allocation failed");
-
   TileShape tl;
-  tl.ty = P->tile_ty > 0 ? P->tile_ty : 16;
-  tl.ntx = (g.nx + 1 + kTX - 1) / kTX;
+  tl.ty = P->tile_ty > 0 ? P->tile_ty : 8;
+  tl.ntx = (g.nx + 1 + TXN - 1) / TXN;
   tl.nty = (g.ny + 1 + tl.ty - 1) / tl.ty;
   const unsigned ntiles = (unsigned)(tl.ntx * tl.nty);
   Mat2 mat;
@@ -645,10 +801,7 @@ int tile_run_t(dmpm_plan* P, T* pb, int32_t* eid, T* nb, int scheme, int vu, con
   fns.fs = P->fn_f;
   const T gx = T(grav[0]), gy = T(grav[1]);
   const T tdt = T(dt);
-
-  if (n > 0)
-    k_tile_check<T><<<nblk(n), kThreads, 0, s>>>(g, pb, ld, n, eid, P->key, P->flags);
-  DMPM_TRY(launch_check("tile_check"));
+  const SortRef sr = sort_ref(P);
 
   DB<T> cur_is_pb, cur_is_alt;
   for (int k = 0; k < kDB; ++k) {
@@ -661,12 +814,13 @@ int tile_run_t(dmpm_plan* P, T* pb, int32_t* eid, T* nb, int scheme, int vu, con
   }
   int parity = 0;
   for (int64_t step = step0; step < step0 + nsteps; ++step) {
-    if (n > 0) {
+    {
       ProfScopeT ps_(P, KC_SORT, s);
-      void* args[] = {(void*)&g,         (void*)&pb,       (void*)&ld,     (void*)&n,
-                      (void*)&eid,       (void*)&P->key,   (void*)&P->counts, (void*)&P->fill,
-                      (void*)&P->cs,     (void*)&bs,       (void*)&P->perm,   (void*)&P->flags,
-                      (void*)&P->big};
+      void* args[] = {(void*)&g,        (void*)&pb,       (void*)&ld,        (void*)&n,
+                      (void*)&eid,      (void*)&P->key,   (void*)&P->counts, (void*)&P->fill,
+                      (void*)&P->perm,  (void*)&P->perm2, (void*)&P->cs,     (void*)&P->cs2,
+                      (void*)&P->iperm, (void*)&bs,       (void*)&P->flags,  (void*)&P->big,
+                      (void*)&P->mov,   (void*)&P->msort, (void*)&P->mkey};
       DMPM_TRY(hip_check(hipLaunchCooperativeKernel((const void*)k_sort_coop<T>, dim3(G),
                                                     dim3(kThreads), args, 0, s),
                          "hipLaunchCooperativeKernel"));
@@ -675,31 +829,31 @@ int tile_run_t(dmpm_plan* P, T* pb, int32_t* eid, T* nb, int scheme, int vu, con
     const T t = T(step) * tdt;
     {
       ProfScopeT ps_(P, KC_TILE_KIN, s);
-      k_tile<T, MODE_KIN><<<ntiles, kThreads, 0, s>>>(
-          g, tl, pb, ld, P->perm, P->cs, P->flags, nb, nld, vkin, P->cidx, P->cmask, P->cval,
-          db, mat, gx, gy, tdt, t, P->nfd, P->nf_off, P->nf_fn, P->nf_val, fns, temps);
+      k_tile<T, MODE_KIN, TXN><<<ntiles, 2 * TXN, 0, s>>>(
+          g, tl, pb, ld, sr, nb, nld, vkin, P->cidx, P->cmask, P->cval, db, mat, gx, gy, tdt, t,
+          P->nfd, P->nf_off, P->nf_fn, P->nf_val, fns, temps);
       DMPM_TRY(launch_check("tile_kin"));
     }
     {
       ProfScopeT ps_(P, KC_TILE_FORCE, s);
       if (usf)
-        k_tile<T, MODE_FORCE_USF><<<ntiles, kThreads, 0, s>>>(
-            g, tl, pb, ld, P->perm, P->cs, P->flags, nb, nld, vkin, P->cidx, P->cmask, P->cval,
-            db, mat, gx, gy, tdt, t, P->nfd, P->nf_off, P->nf_fn, P->nf_val, fns, temps);
+        k_tile<T, MODE_FORCE_USF, TXN><<<ntiles, 2 * TXN, 0, s>>>(
+            g, tl, pb, ld, sr, nb, nld, vkin, P->cidx, P->cmask, P->cval, db, mat, gx, gy, tdt,
+            t, P->nfd, P->nf_off, P->nf_fn, P->nf_val, fns, temps);
       else
-        k_tile<T, MODE_FORCE_USL><<<ntiles, kThreads, 0, s>>>(
-            g, tl, pb, ld, P->perm, P->cs, P->flags, nb, nld, vkin, P->cidx, P->cmask, P->cval,
-            db, mat, gx, gy, tdt, t, P->nfd, P->nf_off, P->nf_fn, P->nf_val, fns, temps);
+        k_tile<T, MODE_FORCE_USL, TXN><<<ntiles, 2 * TXN, 0, s>>>(
+            g, tl, pb, ld, sr, nb, nld, vkin, P->cidx, P->cmask, P->cval, db, mat, gx, gy, tdt,
+            t, P->nfd, P->nf_off, P->nf_fn, P->nf_val, fns, temps);
       DMPM_TRY(launch_check("tile_force"));
     }
-    if (n > 0) {
+    {
       ProfScopeT ps_(P, KC_TILE_G2P, s);
       if (usf)
-        k_g2p_tile<T, false><<<nblk(n), kThreads, 0, s>>>(g, pb, ld, n, nb, nld, P->flags, db,
-                                                          mat, tdt, vu, temps, 1);
+        k_g2p_tile<T, false><<<nblk(n), kThreads, 0, s>>>(g, pb, ld, n, nb, nld, P->flags, P->mov,
+                                                          db, mat, tdt, vu, temps, 1);
       else
-        k_g2p_tile<T, true><<<nblk(n), kThreads, 0, s>>>(g, pb, ld, n, nb, nld, P->flags, db,
-                                                         mat, tdt, vu, temps, 0);
+        k_g2p_tile<T, true><<<nblk(n), kThreads, 0, s>>>(g, pb, ld, n, nb, nld, P->flags, P->mov,
+                                                         db, mat, tdt, vu, temps, 0);
       DMPM_TRY(launch_check("tile_g2p"));
     }
     if (usf) parity ^= 1;
@@ -711,6 +865,29 @@ int tile_run_t(dmpm_plan* P, T* pb, int32_t* eid, T* nb, int scheme, int vu, con
                          "hipMemcpyAsync"));
   }
   return DMPM_OK;
+}
+
+template <typename T>
+int tile_run_t(dmpm_plan* P, T* pb, int32_t* eid, T* nb, int scheme, int vu, const double* grav,
+               double dt, int64_t step0, int64_t nsteps, int temps, hipStream_t s) {
+  const Grid2<T> g = grid_of<T>(P);
+  const int64_t ld = P->d.ld;
+  const int n = (int)P->n;
+  const bool usf = scheme == DMPM_USF;
+  T* alt = usf ? (T*)plan_workspace(P, 0, (size_t)kDB * ld * sizeof(T)) : nullptr;
+  T* vkin = (T*)plan_workspace(P, 1, (size_t)2 * P->nnodes * sizeof(T));
+  if ((usf && !alt) || !vkin) return fail(DMPM_EHIP, "workspace allocation failed");
+  int G = 0;
+  DMPM_TRY(coop_grid_size((const void*)k_sort_coop<T>, G));
+  int* bs = (int*)plan_workspace(P, 2, (size_t)G * sizeof(int));
+  if (!bs) return fail(DMPM_EHIP, "workspace allocation failed");
+  k_tile_check<T><<<nblk(n), kThreads, 0, s>>>(g, pb, ld, n, eid, P->key, P->flags);
+  DMPM_TRY(launch_check("tile_check"));
+  if (P->tile_tx == 64)
+    return tile_steps<T, 64>(P, g, pb, eid, nb, scheme, vu, grav, dt, step0, nsteps, temps, s,
+                             alt, vkin, bs, G);
+  return tile_steps<T, 128>(P, g, pb, eid, nb, scheme, vu, grav, dt, step0, nsteps, temps, s,
+                            alt, vkin, bs, G);
 }
 
 }  // namespace

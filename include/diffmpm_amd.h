@@ -131,9 +131,10 @@ int dmpm_plan_set_nodal_forces(dmpm_plan* plan, int32_t n, const int32_t* node_i
 int dmpm_plan_set_tractions(dmpm_plan* plan, int32_t n, const int32_t* pids,
                             const int32_t* dirs, const int32_t* fn,
                             const double* tractions);
-/* Node rows per tile of the fused 2D kernels (0 = default 16); a tuning knob
- * with no effect on results.  No reference counterpart. */
-int dmpm_plan_set_tile(dmpm_plan* plan, int32_t node_rows);
+/* Tile shape of the fused 2D kernels: node_rows (0 = default 8) and
+ * node_cols (64 or 128; 0 = keep).  A tuning knob with no effect on results.
+ * No reference counterpart. */
+int dmpm_plan_set_tile(dmpm_plan* plan, int32_t node_rows, int32_t node_cols);
 /* Declares that the element ids changed outside the library (forces a re-sort). */
 int dmpm_mark_unsorted(dmpm_plan* plan, void* stream);
 

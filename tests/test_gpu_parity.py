@@ -219,13 +219,14 @@ def test_particles_outside_mesh_do_not_stall():
     assert torch.isfinite(ps.loc[20:]).all() or True  # completes; values may be NaN
 
 
-def _moving_spec(scheme, per_particle_E=False, nx=200, ny=40, rows=30):
+def _moving_spec(scheme, per_particle_E=False, nx=200, ny=40, rows=30, drift=(0.6, 0.3),
+                 velocity_scale=0.05):
     """Multi-tile 2D case whose particles cross cells within a few steps
     (exercises re-sorting in the fused path)."""
     spec = cases.synthetic_spec("quad4", nx=nx, ny=ny, rows=rows, seed=21, scheme=scheme,
                                 traction=False, nodal_force=True, E=1e3, dt=1e-3,
-                                velocity_scale=0.05, nsteps=12, per_particle_E=per_particle_E,
-                                drift=(0.6, 0.3))
+                                velocity_scale=velocity_scale, nsteps=12,
+                                per_particle_E=per_particle_E, drift=drift)
     spec["nodal_forces"][0]["force"] = -0.05
     right = [j * (nx + 1) + nx for j in range(ny + 1)]
     spec["constraints"].append({"nodes": right, "dir": 0, "velocity": 0.0})
@@ -236,13 +237,17 @@ def _moving_spec(scheme, per_particle_E=False, nx=200, ny=40, rows=30):
     return spec
 
 
-@pytest.mark.parametrize("scheme,per_E", [("usf", False), ("usl", False), ("usf", True)])
-def test_tile_path_equals_per_phase_path(scheme, per_E):
+@pytest.mark.parametrize("scheme,per_E,drift", [("usf", False, (0.6, 0.3)),
+                                                 ("usl", False, (0.6, 0.3)),
+                                                 ("usf", True, (0.6, 0.3)),
+                                                 ("usf", False, None), ("usl", True, None)])
+def test_tile_path_equals_per_phase_path(scheme, per_E, drift):
     """The fused node-tile kernels (dmpm_run default) and the per-phase
     kernels (dmpm_run flag bit 2) give bit-identical states, across cell
-    changes and re-sorts."""
+    changes: with a drift every particle changes cell (full re-sorts), without
+    it a few hundred do per step (incremental re-sorts)."""
     from diffmpm_amd._plan import plan_for
-    spec = _moving_spec(scheme, per_E)
+    spec = _moving_spec(scheme, per_E, drift=drift, velocity_scale=0.05 if drift else 0.1)
     a, b = cases.build_gpu(spec), cases.build_gpu(spec)
     g = torch.tensor(spec["gravity"], dtype=torch.float64)
     pa = plan_for(a.mesh.elements, a.mesh.particles[0])
@@ -256,28 +261,36 @@ def test_tile_path_equals_per_phase_path(scheme, per_E):
     assert not torch.equal(e0, a.mesh.particles[0].element_ids)  # cells did change
 
 
-@pytest.mark.parametrize("ty", [1, 3, 8, 33])
-def test_tile_height_does_not_change_results(ty):
+@pytest.mark.parametrize("ty,tx", [(1, 64), (3, 128), (8, 128), (33, 64)])
+def test_tile_shape_does_not_change_results(ty, tx):
     from diffmpm_amd._plan import plan_for
-    spec = _moving_spec("usf", nx=150, ny=70, rows=60)
+    spec = _moving_spec("usf", nx=150, ny=70, rows=60, drift=None, velocity_scale=0.1)
     a, b = cases.build_gpu(spec), cases.build_gpu(spec)
     g = torch.tensor(spec["gravity"], dtype=torch.float64)
     pa = plan_for(a.mesh.elements, a.mesh.particles[0])
     pb = plan_for(b.mesh.elements, b.mesh.particles[0])
-    pb.set_tile(ty)
+    pb.set_tile(ty, tx)
     pa.run("usf", False, g, spec["dt"], 0, 6, True)
     pb.run("usf", False, g, spec["dt"], 0, 6, True)
     for x, y in zip(_state(a), _state(b)):
         assert torch.equal(x, y)
 
 
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
 @pytest.mark.parametrize("scheme", ["usf", "usl"])
-def test_tile_path_parity_with_oracle(scheme):
-    """Rollout of the fused path on the multi-tile moving case against the
-    FP64 oracle (FP32 GPU, normwise 1e-4 after 12 steps)."""
+def test_tile_path_per_step_parity(scheme, dtype):
+    """The fused path, re-synchronised to the oracle before every step of
+    the multi-tile moving case, matches it within RTOL_F32 / RTOL_F64."""
+    from diffmpm_amd._plan import plan_for
     spec = _moving_spec(scheme)
-    osim = cases.build_oracle(spec, np.float64)
-    gsim = cases.build_gpu(spec)
-    osim.solve(np.asarray(spec["gravity"]))
-    gsim.solve_jit(torch.tensor(spec["gravity"], dtype=torch.float64))
-    _compare(osim, gsim, 1e-4, "tile rollout")
+    npdt = np.float64 if dtype == torch.float64 else np.float32
+    rtol = RTOL_F64 if dtype == torch.float64 else RTOL_F32
+    osim = cases.build_oracle(spec, npdt)
+    gsim = cases.build_gpu(spec, dtype)
+    plan = plan_for(gsim.mesh.elements, gsim.mesh.particles[0])
+    g = torch.tensor(spec["gravity"], dtype=torch.float64)
+    for i in range(spec["nsteps"]):
+        cases.load_oracle_state_into_gpu(osim, gsim)
+        osim.step(i, np.asarray(spec["gravity"]))
+        plan.run(scheme, False, g, spec["dt"], i, 1, True)
+        _compare(osim, gsim, rtol, f"step {i}")
