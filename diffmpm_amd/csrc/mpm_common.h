@@ -45,6 +45,7 @@ template <typename T> __device__ __forceinline__ T qnan() {
 template <typename T> struct Geo {
   int nx, ny;        // cells per axis (ny = 1 in 1D)
   T hx, hy;
+  T ihx, ihy;        // 1/h, only used to guess the element index
   int ncells;        // nx*ny
   int nnodes;        // (nx+1)*(ny+1) in 2D, nx+1 in 1D
   __device__ __forceinline__ T X(int i) const { return T(i) * hx; }
@@ -53,9 +54,9 @@ template <typename T> struct Geo {
 
 // max{i in [0,n] : fl(i*h) <= x}, or -1 when no node coordinate is <= x
 // (also for NaN).  Node coordinates are nondecreasing in i.
-template <typename T> __device__ __forceinline__ int last_le(T x, int n, T h) {
+template <typename T> __device__ __forceinline__ int last_le(T x, int n, T h, T ih) {
   if (!(x >= T(0))) return -1;
-  T q = x / h;
+  T q = x * ih;  // a guess; the loops below make the result exact
   int g = (q >= T(n)) ? n : (int)q;
   while (g < n && T(g + 1) * h <= x) ++g;
   while (g >= 0 && T(g) * h > x) --g;
@@ -68,9 +69,9 @@ template <typename T> __device__ __forceinline__ int last_le(T x, int n, T h) {
 //  1D (element.py:517-541): eid = I when X_I <= x < X_{I+1}, else -1.
 template <typename T, int DIM>
 __device__ __forceinline__ int element_id(const Geo<T>& g, T x, T y) {
-  int I = last_le(x, g.nx, g.hx);
+  int I = last_le(x, g.nx, g.hx, g.ihx);
   if (DIM == 1) return (I >= 0 && I < g.nx) ? I : -1;
-  int J = last_le(y, g.ny, g.hy);
+  int J = last_le(y, g.ny, g.hy, g.ihy);
   return (I >= 0 && J >= 0) ? J * g.nx + I : 0;
 }
 
@@ -109,19 +110,18 @@ template <typename T> __device__ __forceinline__ void q4_dNdxi(int k, T xi, T et
   }
 }
 
-// Inverse Jacobian diagonal of an axis-aligned Q4 cell (element.py:814-817):
-// J = dN/dxi^T X_e has J01 = J10 = 0 for a rectangle, so inv(J) is
-// diag(1/J00, 1/J11) with J00 = sum_k dN_k/dxi X_k (corner order).
+// Inverse Jacobian of an axis-aligned Q4 cell (element.py:814-817).  In
+// exact arithmetic J = dN/dxi^T X_e = diag((X1-X0)/2, (Y1-Y0)/2) for any
+// (xi, eta), so inv(J) = diag(2/(X1-X0), 2/(Y1-Y0)): the same scale as the
+// natural coordinates (particle.py:203-205).  The reference forms J as a
+// 4-term FP32 dot product of O(X) terms whose cancellation leaves up to
+// ~1e-5 relative noise on fine meshes; the closed form is exact.
 template <typename T>
 __device__ __forceinline__ void q4_invJ(T xi, T eta, T X0, T X1, T Y0, T Y1, T& ix, T& iy) {
-  T g0x = T(-0.25) * (T(1) - eta), g1x = T(0.25) * (T(1) - eta);
-  T g2x = T(0.25) * (T(1) + eta), g3x = T(-0.25) * (T(1) + eta);
-  T g0y = T(-0.25) * (T(1) - xi), g1y = T(-0.25) * (T(1) + xi);
-  T g2y = T(0.25) * (T(1) + xi), g3y = T(0.25) * (T(1) - xi);
-  T J00 = ((g0x * X0 + g1x * X1) + g2x * X1) + g3x * X0;
-  T J11 = ((g0y * Y0 + g1y * Y0) + g2y * Y1) + g3y * Y1;
-  ix = T(1) / J00;
-  iy = T(1) / J11;
+  (void)xi;
+  (void)eta;
+  ix = T(2) / (X1 - X0);
+  iy = T(2) / (Y1 - Y0);
 }
 
 // Physical gradient of corner k (element.py:791-818).
@@ -133,9 +133,9 @@ __device__ __forceinline__ void q4_dNdx(int k, T xi, T eta, T ix, T iy, T& dx, T
   dy = gy * iy;
 }
 
-// 1D physical gradient (element.py:488-515): J = -0.5 X0 + 0.5 X1.
+// 1D physical gradient (element.py:488-515): inv(J) = 2/(X1 - X0).
 template <typename T> __device__ __forceinline__ T l1_inv(T X0, T X1) {
-  return T(1) / (T(-0.5) * X0 + T(0.5) * X1);
+  return T(2) / (X1 - X0);
 }
 template <typename T> __device__ __forceinline__ T l1_dNdx(int k, T inv) {
   return (k == 0 ? T(-0.5) : T(0.5)) * inv;

@@ -788,6 +788,8 @@ template <typename T> Geo<T> make_geo(const dmpm_plan* P) {
   g.ny = P->d.ny;
   g.hx = T(P->d.hx);
   g.hy = T(P->d.hy);
+  g.ihx = T(1) / g.hx;
+  g.ihy = T(1) / g.hy;
   g.ncells = P->ncells;
   g.nnodes = P->nnodes;
   return g;

@@ -73,7 +73,7 @@ struct TileShape {
 
 template <typename T> struct Grid2 {
   int nx, ny;
-  T hx, hy;
+  T hx, hy, ihx, ihy;
   int ncells, nnodes;
   __device__ __forceinline__ T X(int i) const { return T(i) * hx; }
   __device__ __forceinline__ T Y(int j) const { return T(j) * hy; }
@@ -83,6 +83,8 @@ template <typename T> struct Grid2 {
     g.ny = ny;
     g.hx = hx;
     g.hy = hy;
+    g.ihx = ihx;
+    g.ihy = ihy;
     g.ncells = ncells;
     g.nnodes = nnodes;
     return g;
@@ -140,6 +142,27 @@ __device__ __forceinline__ void rate2(const Grid2<T>& g, int I, int J, T xi, T e
   r[5] = T(0);
 }
 
+// Same as rate2 with the inverse-Jacobian scales given.
+template <typename T>
+__device__ __forceinline__ void rate2s(T ix, T iy, T xi, T eta, const T vx[4], const T vy[4],
+                                       T r[6]) {
+  T L00 = T(0), L01 = T(0), L10 = T(0), L11 = T(0);
+  for (int k = 0; k < 4; ++k) {
+    T dx, dy;
+    q4_dNdx(k, xi, eta, ix, iy, dx, dy);
+    L00 = L00 + dx * vx[k];
+    L01 = L01 + dx * vy[k];
+    L10 = L10 + dy * vx[k];
+    L11 = L11 + dy * vy[k];
+  }
+  r[0] = snap(L00);
+  r[1] = snap(L11);
+  r[2] = T(0);
+  r[3] = snap(L01 + L10);
+  r[4] = T(0);
+  r[5] = T(0);
+}
+
 #define PR(r) (pb + (int64_t)(r) * ld)
 #define NR(r) (nb + (int64_t)(r) * nld)
 
@@ -155,9 +178,14 @@ __global__ __launch_bounds__(2 * kTX) void k_tile(
     const double* __restrict__ nf_val, FnTable fns, int temps) {
   constexpr int CH = Chunk<T, kTX>::value;
   constexpr int NT = 2 * kTX;
-  __shared__ T s_xi[CH], s_eta[CH], s_ix[MODE == MODE_KIN ? 1 : CH],
-      s_iy[MODE == MODE_KIN ? 1 : CH], s_q[MODE == MODE_KIN ? 3 : 5][CH];
+  constexpr bool KIN = MODE == MODE_KIN;
+  constexpr bool USF = MODE == MODE_FORCE_USF;
+  constexpr int NQ = KIN ? 3 : 5;
+  constexpr int CHF = KIN ? 1 : CH;  // gradient scales are staged for force modes only
+  __shared__ T s_xi[CH], s_eta[CH], s_ix[CHF], s_q[NQ][CH];
   __shared__ int s_cs[2][kTX + 2];
+  __shared__ T s_mx[kTX + 2], s_sx[kTX + 2];  // per cell column: (X0+X1)/2, 2/(X1-X0)
+  __shared__ T s_v[USF ? 2 : 1][2][2][kTX + 2];  // [buf][node row r|r+1][x|y][col]
 
   const int ident = sr.flags[1];
   const int* __restrict__ perm = sr.perm();
@@ -171,11 +199,27 @@ __global__ __launch_bounds__(2 * kTX) void k_tile(
   const int c0 = max(i0 - 1, 0), c1 = min(i0 + kTX, g.nx);
   const int ncell = c1 - c0;
 
+  if (tid < ncell) {
+    const T X0 = g.X(c0 + tid), X1 = g.X(c0 + tid + 1);
+    s_mx[tid] = (X0 + X1) / T(2);
+    s_sx[tid] = T(2) / (X1 - X0);
+  }
+
   T acc[4] = {T(0), T(0), T(0), T(0)};
 
-  // cell offsets of the row segment, prefetched one cell row ahead
+  // cell offsets (and, for USF, nodal velocities) of the row segment,
+  // prefetched one cell row ahead
   const int rfirst = max(j0 - 1, 0), rlast = min(j0 + tl.ty, g.ny);  // [rfirst, rlast)
-  int cs_next = (tid <= ncell && rfirst < rlast) ? cs[rfirst * g.nx + c0 + tid] : 0;
+  const bool loader = tid <= ncell && rfirst < rlast;
+  int cs_next = loader ? cs[rfirst * g.nx + c0 + tid] : 0;
+  T v_next[4] = {T(0), T(0), T(0), T(0)};
+  if (USF && loader) {
+    const int nd = rfirst * (g.nx + 1) + c0 + tid;
+    v_next[0] = vkin[nd];
+    v_next[1] = vkin[g.nnodes + nd];
+    v_next[2] = vkin[nd + g.nx + 1];
+    v_next[3] = vkin[g.nnodes + nd + g.nx + 1];
+  }
 
   for (int r = j0 - 1; r < j0 + tl.ty; ++r) {
     const bool top = grp == ((r + 1) & 1);
@@ -184,12 +228,31 @@ __global__ __launch_bounds__(2 * kTX) void k_tile(
     if (r >= 0 && r < g.ny) {
       const int cb_ = (r - rfirst) & 1;
       __syncthreads();
-      if (tid <= ncell) s_cs[cb_][tid] = cs_next;
-      if (tid <= ncell && r + 1 < rlast) cs_next = cs[(r + 1) * g.nx + c0 + tid];
+      if (tid <= ncell) {
+        s_cs[cb_][tid] = cs_next;
+        if (USF) {
+          s_v[cb_ & (USF ? 1 : 0)][0][0][tid] = v_next[0];
+          s_v[cb_ & (USF ? 1 : 0)][0][1][tid] = v_next[1];
+          s_v[cb_ & (USF ? 1 : 0)][1][0][tid] = v_next[2];
+          s_v[cb_ & (USF ? 1 : 0)][1][1][tid] = v_next[3];
+        }
+        if (r + 1 < rlast) {
+          cs_next = cs[(r + 1) * g.nx + c0 + tid];
+          if (USF) {
+            const int nd = (r + 1) * (g.nx + 1) + c0 + tid;
+            v_next[0] = vkin[nd];
+            v_next[1] = vkin[g.nnodes + nd];
+            v_next[2] = vkin[nd + g.nx + 1];
+            v_next[3] = vkin[g.nnodes + nd + g.nx + 1];
+          }
+        }
+      }
       __syncthreads();
       const int* __restrict__ rcs = s_cs[cb_];
       const int sb = rcs[0], se = rcs[ncell];
       const bool row_owned = r >= j0 && r < j0 + tl.ty;
+      const T Y0 = g.Y(r), Y1 = g.Y(r + 1);
+      const T my = (Y0 + Y1) / T(2), sy = T(2) / (Y1 - Y0);
       for (int cb = sb; cb < se; cb += CH) {
         const int ce = min(cb + CH, se);
         if (cb != sb) __syncthreads();
@@ -198,16 +261,19 @@ __global__ __launch_bounds__(2 * kTX) void k_tile(
           const int p = ident ? s : perm[s];
           const int q = s - cb;
           const T x = PR(DMPM_P_X)[p], y = PR(DMPM_P_X + 1)[p];
-          const int e = element_id<T, 2>(g.geo(), x, y);
-          const int I = e % g.nx, J = e / g.nx;
-          const T xi = natural(x, g.X(I), g.X(I + 1));
-          const T eta = natural(y, g.Y(J), g.Y(J + 1));
+          // the sort put p in cell row r, columns [c0, c1); its column follows
+          // from x (x >= X_nx wraps to column 0 of the next row, A1)
+          int I = last_le(x, g.nx, g.hx, g.ihx);
+          if (I >= g.nx) I = 0;
+          const int kk = min(max(I - c0, 0), ncell - 1);
+          const T xi = (x - s_mx[kk]) * s_sx[kk];  // particle.py:203-205
+          const T eta = (y - my) * sy;
           const bool owned = row_owned && I >= i0 && I < i0 + kTX;
           s_xi[q] = xi;
           s_eta[q] = eta;
           const T m = PR(DMPM_P_MASS)[p];
           s_q[0][q] = m;
-          if (MODE == MODE_KIN) {
+          if (KIN) {
             s_q[1][q] = m * PR(DMPM_P_V)[p];
             s_q[2][q] = m * PR(DMPM_P_V + 1)[p];
             if (temps && owned) {
@@ -215,11 +281,8 @@ __global__ __launch_bounds__(2 * kTX) void k_tile(
               PR(DMPM_P_XI + 1)[p] = eta;
             }
           } else {
-            T ix, iy;
-            q4_invJ(xi, eta, g.X(I), g.X(I + 1), g.Y(J), g.Y(J + 1), ix, iy);
-            s_ix[q % (MODE == MODE_KIN ? 1 : CH)] = ix;
-            s_iy[q % (MODE == MODE_KIN ? 1 : CH)] = iy;
-            constexpr int q3 = MODE == MODE_KIN ? 0 : 3, q4 = MODE == MODE_KIN ? 0 : 4;
+            constexpr int q3 = KIN ? 0 : 3, q4 = KIN ? 0 : 4;
+            s_ix[q % CHF] = s_sx[kk];
             if (MODE == MODE_FORCE_USL) {
               s_q[1][q] = -PR(DMPM_P_VOLUME)[p];
               s_q[2][q] = PR(DMPM_P_STRESS + 0)[p];
@@ -227,16 +290,15 @@ __global__ __launch_bounds__(2 * kTX) void k_tile(
               s_q[q4][q] = PR(DMPM_P_STRESS + 3)[p];
             } else {
               // USF strain / volume / stress update (particle.py:249-327)
-              const int ll = J * (g.nx + 1) + I;
-              const int ids[4] = {ll, ll + 1, ll + g.nx + 2, ll + g.nx + 1};
-              T vx[4], vy[4];
-              for (int k = 0; k < 4; ++k) {
-                vx[k] = vkin[ids[k]];
-                vy[k] = vkin[g.nnodes + ids[k]];
-              }
+              const int vb = cb_ & (USF ? 1 : 0);
+              const T vx[4] = {s_v[vb][0][0][kk], s_v[vb][0][0][kk + 1], s_v[vb][1][0][kk + 1],
+                               s_v[vb][1][0][kk]};
+              const T vy[4] = {s_v[vb][0][1][kk], s_v[vb][0][1][kk + 1], s_v[vb][1][1][kk + 1],
+                               s_v[vb][1][1][kk]};
+              const T ix = s_sx[kk], iy = sy;
               T rr[6], rc[6], d[6];
-              rate2(g, I, J, xi, eta, vx, vy, rr);
-              rate2(g, I, J, T(0), T(0), vx, vy, rc);
+              rate2s(ix, iy, xi, eta, vx, vy, rr);
+              rate2s(ix, iy, T(0), T(0), vx, vy, rc);
               for (int k = 0; k < 6; ++k) d[k] = rr[k] * dt;
               const T dvol = dt * (rc[0] + rc[1]);
               T ds[4];
@@ -279,17 +341,16 @@ __global__ __launch_bounds__(2 * kTX) void k_tile(
               const T xi = s_xi[q], eta = s_eta[q];
               const T N = q4_N<T>(corner, xi, eta);
               const T m = s_q[0][q];
-              if (MODE == MODE_KIN) {
+              if (KIN) {
                 acc[0] = acc[0] + m * N;                 // element.py:140
                 acc[1] = acc[1] + N * s_q[1][q];         // element.py:172
                 acc[2] = acc[2] + N * s_q[2][q];
               } else {
+                constexpr int q3 = KIN ? 0 : 3, q4 = KIN ? 0 : 4;
                 acc[0] = acc[0] + N * (m * gx);          // element.py:254-256
                 acc[1] = acc[1] + N * (m * gy);
                 T dx, dy;
-                const int qq = q % (MODE == MODE_KIN ? 1 : CH);
-                q4_dNdx(corner, xi, eta, s_ix[qq], s_iy[qq], dx, dy);
-                const int q3 = MODE == MODE_KIN ? 0 : 3, q4 = MODE == MODE_KIN ? 0 : 4;
+                q4_dNdx(corner, xi, eta, s_ix[q % CHF], sy, dx, dy);
                 const T nV = s_q[1][q];
                 acc[2] = acc[2] + nV * (dx * s_q[2][q] + dy * s_q[q4][q]);  // element.py:868-878
                 acc[3] = acc[3] + nV * (dy * s_q[q3][q] + dx * s_q[q4][q]);
@@ -357,15 +418,16 @@ __global__ __launch_bounds__(2 * kTX) void k_tile(
 // G2P (particle.py:208-247); USL also updates strain/volume/stress with the
 // step-start natural coordinates and the updated nodal velocities
 // (scheme.py:127-133).  Sets flags[0] when a particle changes cell.
-// Records a particle whose cell (sort key) changed: (particle, old, new) is
-// appended for the incremental re-sort; past kMovCap a full sort is requested.
+// Records a particle whose cell (sort key) changed: (particle, old key, new
+// element id) is appended for the incremental re-sort; past kMovCap a full
+// sort is requested.
 __device__ __forceinline__ void record_move(int* __restrict__ flags, int* __restrict__ mov, int p,
-                                            int kold, int knew) {
+                                            int kold, int enew) {
   const int k = atomicAdd(&flags[5], 1);
   if (k < kMovCap) {
     mov[3 * k] = p;
     mov[3 * k + 1] = kold;
-    mov[3 * k + 2] = knew;
+    mov[3 * k + 2] = enew;
   } else {
     flags[6] = 1;
   }
@@ -470,7 +532,7 @@ __global__ __launch_bounds__(kThreads) void k_g2p_tile(Grid2<T> g, T* __restrict
   }
   const int e1 = element_id<T, 2>(g.geo(), x1, y1);
   const int k1 = (e1 >= 0 && e1 < g.ncells) ? e1 : g.ncells;
-  if (k1 != e) record_move(flags, mov, p, e, k1);
+  if (k1 != e) record_move(flags, mov, p, e, e1);
 }
 
 // Element ids and sort keys from the positions at the start of a run; marks
@@ -592,26 +654,51 @@ __global__ __launch_bounds__(kThreads) void k_sort_coop(
     int* __restrict__ mKO = msort + kMovCap;      // old keys
     int* __restrict__ mKN = msort + 2 * kMovCap;  // new keys
     if (b < 4) {
+      auto knew = [&](int j) {
+        const int e = mov[3 * j + 2];
+        return (e >= 0 && e < g.ncells) ? e : g.ncells;
+      };
       if (b == 3) {
         for (int j = tid; j < nmov; j += kThreads)
-          sh64[j] = ((unsigned long long)(unsigned)mov[3 * j + 2] << 32) | (unsigned)mov[3 * j];
+          sh64[j] = ((unsigned long long)(unsigned)knew(j) << 32) | (unsigned)mov[3 * j];
         block_sort<unsigned long long>(sh64, nmov, mkey, ~0ull);
       } else {
         for (int j = tid; j < nmov; j += kThreads)
-          sh[j] = b == 0 ? iperm[mov[3 * j]] : mov[3 * j + b];
+          sh[j] = b == 0 ? iperm[mov[3 * j]] : (b == 1 ? mov[3 * j + 1] : knew(j));
         block_sort<int>(sh, nmov, b == 0 ? mS : (b == 1 ? mKO : mKN), 0x7fffffff);
       }
     }
     grid.sync();
-    // non-movers keep their relative order
-    for (int s = gt; s < n; s += gs) {
-      const int jm = lower_count(mS, nmov, s);
-      if (jm < nmov && mS[jm] == s) continue;
-      const int p = po[s];
-      const unsigned long long kp = ((unsigned long long)(unsigned)key[p] << 32) | (unsigned)p;
-      const int pos = s - jm + lower_count(mkey, nmov, kp);
-      pn[pos] = p;
-      iperm[p] = pos;
+    // Non-movers keep their relative order.  Slots are taken in rounds of
+    // kThreads: (old key, index) increases with the slot, so the mover counts
+    // of every slot in a round lie between those of its first and last slot
+    // and each thread searches only that (usually empty) range.
+    __shared__ int bnd[4];
+    for (int base = b * kThreads; base < n; base += gs) {
+      const int last = min(base + kThreads, n) - 1;
+      if (tid == 0) {
+        const int pf = po[base], pl = po[last];
+        bnd[0] = lower_count(mS, nmov, base);
+        bnd[1] = lower_count(mS, nmov, last + 1);
+        bnd[2] = lower_count(mkey, nmov,
+                             ((unsigned long long)(unsigned)key[pf] << 32) | (unsigned)pf);
+        bnd[3] = lower_count(mkey, nmov,
+                             (((unsigned long long)(unsigned)key[pl] << 32) | (unsigned)pl) + 1);
+      }
+      __syncthreads();
+      const int s = base + tid;
+      if (s < n) {
+        const int jm = bnd[0] + lower_count(mS + bnd[0], bnd[1] - bnd[0], s);
+        if (!(jm < nmov && mS[jm] == s)) {
+          const int p = po[s];
+          const unsigned long long kp =
+              ((unsigned long long)(unsigned)key[p] << 32) | (unsigned)p;
+          const int pos = s - jm + bnd[2] + lower_count(mkey + bnd[2], bnd[3] - bnd[2], kp);
+          pn[pos] = p;
+          iperm[p] = pos;
+        }
+      }
+      __syncthreads();
     }
     // movers go before the first non-mover of their new cell with a larger index
     for (int j = gt; j < nmov; j += gs) {
@@ -626,11 +713,29 @@ __global__ __launch_bounds__(kThreads) void k_sort_coop(
       const int pos = (lo - lower_count(mS, nmov, lo)) + j;
       pn[pos] = p;
       iperm[p] = pos;
-      key[p] = kn;
     }
-    for (int c = gt; c <= nbk; c += gs)
-      cn[c] = co[c] - lower_count(mKO, nmov, c) + lower_count(mKN, nmov, c);
+    // new cell offsets, in rounds like the slots
+    for (int base = b * kThreads; base <= nbk; base += gs) {
+      const int last = min(base + kThreads, nbk + 1) - 1;
+      if (tid == 0) {
+        bnd[0] = lower_count(mKO, nmov, base);
+        bnd[1] = lower_count(mKO, nmov, last + 1);
+        bnd[2] = lower_count(mKN, nmov, base);
+        bnd[3] = lower_count(mKN, nmov, last + 1);
+      }
+      __syncthreads();
+      const int c = base + tid;
+      if (c <= nbk)
+        cn[c] = co[c] - (bnd[0] + lower_count(mKO + bnd[0], bnd[1] - bnd[0], c)) +
+                (bnd[2] + lower_count(mKN + bnd[2], bnd[3] - bnd[2], c));
+      __syncthreads();
+    }
     grid.sync();
+    for (int j = gt; j < nmov; j += gs) {
+      const int p = mov[3 * j], e = mov[3 * j + 2];
+      eid[p] = e;
+      key[p] = (e >= 0 && e < g.ncells) ? e : g.ncells;
+    }
     if (gt == 0) {
       flags[4] = 1 - cur;
       flags[5] = 0;
@@ -760,6 +865,8 @@ template <typename T> Grid2<T> grid_of(const dmpm_plan* P) {
   g.ny = P->d.ny;
   g.hx = T(P->d.hx);
   g.hy = T(P->d.hy);
+  g.ihx = T(1) / g.hx;
+  g.ihy = T(1) / g.hy;
   g.ncells = P->ncells;
   g.nnodes = P->nnodes;
   return g;
