@@ -89,6 +89,8 @@ def main():
     ap.add_argument("--kernel", default="tile_force", help="kernel class timed for the roofline")
     ap.add_argument("--tile", type=int, default=0, help="node rows per tile (0: default)")
     ap.add_argument("--tile-cols", type=int, default=0, help="node columns per tile (64/128)")
+    ap.add_argument("--usf-fused", action="store_true",
+                    help="update stresses inside the force tile kernel")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -106,6 +108,9 @@ def main():
     N = len(ps)
     if args.tile or args.tile_cols:
         plan.set_tile(args.tile, args.tile_cols)
+    if args.usf_fused:
+        from diffmpm_amd import _lib
+        plan.set_option(_lib.DMPM_OPT_USF_SPLIT, 0)
     print(f"# workload built: {N} particles, {args.n}x{args.n} cells", file=sys.stderr, flush=True)
     plan.run("usf", False, gravity, sim.dt, 0, args.warmup, keep_temporaries=False)
     torch.cuda.synchronize()

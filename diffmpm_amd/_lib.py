@@ -20,6 +20,7 @@ DMPM_LINEAR1D, DMPM_QUAD4 = 1, 2
 DMPM_LINEAR_ELASTIC, DMPM_SIMPLE_MATERIAL = 0, 1
 DMPM_USF, DMPM_USL = 0, 1
 DMPM_FN_UNIT, DMPM_FN_LINEAR = 0, 1
+DMPM_OPT_USF_SPLIT = 1
 
 # Particle block rows (include/diffmpm_amd.h)
 P_X, P_V, P_MASS, P_VOLUME, P_DENSITY, P_SIZE = 0, 2, 4, 5, 6, 7
@@ -60,6 +61,7 @@ SIGNATURES = {
     "dmpm_plan_set_tractions": [_P, c_int32, _I32P, _I32P, _I32P, _DP],
     "dmpm_mark_unsorted": [_P, _P],
     "dmpm_plan_set_tile": [_P, c_int32, c_int32],
+    "dmpm_plan_set_option": [_P, c_int32, c_int32],
     "dmpm_set_particle_element_ids": [_P, _P, _P, _P],
     "dmpm_update_natural_coords": [_P, _P, _P, _P],
     "dmpm_element_volume": [c_int32, c_int32, c_double, c_double, _DP],

@@ -223,6 +223,9 @@ class Plan:
         call("dmpm_phase_particle_kinematics", self.h, self.pb, self.eid, self.nb, float(dt),
              1 if velocity_update else 0, self.s)
 
+    def set_option(self, option, value):
+        call("dmpm_plan_set_option", self.h, int(option), int(value))
+
     def set_tile(self, node_rows, node_cols=0):
         call("dmpm_plan_set_tile", self.h, int(node_rows), int(node_cols))
 

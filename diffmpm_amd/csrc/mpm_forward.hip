@@ -1114,6 +1114,17 @@ int dmpm_plan_set_tile(dmpm_plan* P, int32_t node_rows, int32_t node_cols) {
   return DMPM_OK;
 }
 
+int dmpm_plan_set_option(dmpm_plan* P, int32_t option, int32_t value) {
+  if (!P) return fail(DMPM_EINVAL, "null plan");
+  switch (option) {
+    case DMPM_OPT_USF_SPLIT:
+      P->usf_split = value ? 1 : 0;
+      return DMPM_OK;
+    default:
+      return fail(DMPM_EINVAL, "unknown option");
+  }
+}
+
 int dmpm_mark_unsorted(dmpm_plan* P, void* stream) {
   if (!P) return fail(DMPM_EINVAL, "null plan");
   static const int one = 1;

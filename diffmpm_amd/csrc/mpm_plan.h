@@ -68,6 +68,7 @@ struct dmpm_plan {
   std::vector<hipEvent_t> prof_ev;
   int tile_ty = 0;  // node rows per tile of the fused 2D kernels (0: default)
   int tile_tx = 64; // node columns per tile (64 or 128)
+  int usf_split = 1; // USF: streamed stress kernel + force tile (1) or fused (0)
   // workspace of the fused tile kernels and the adjoint (allocated lazily)
   void* extra[16] = {nullptr};
   size_t extra_bytes[16] = {0};

@@ -261,10 +261,11 @@ __global__ __launch_bounds__(2 * kTX) void k_tile(
           const int p = ident ? s : perm[s];
           const int q = s - cb;
           const T x = PR(DMPM_P_X)[p], y = PR(DMPM_P_X + 1)[p];
-          // the sort put p in cell row r, columns [c0, c1); its column follows
-          // from x (x >= X_nx wraps to column 0 of the next row, A1)
-          int I = last_le(x, g.nx, g.hx, g.ihx);
-          if (I >= g.nx) I = 0;
+          // the sort put p in cell row r, columns [c0, c1); its column is that
+          // of its element id (A1: x >= X_nx wraps to column 0 of the next
+          // row, points below or left of the mesh belong to element 0)
+          const int e = element_id<T, 2>(g.geo(), x, y);
+          const int I = e % g.nx;
           const int kk = min(max(I - c0, 0), ncell - 1);
           const T xi = (x - s_mx[kk]) * s_sx[kk];  // particle.py:203-205
           const T eta = (y - my) * sy;
@@ -535,6 +536,70 @@ __global__ __launch_bounds__(kThreads) void k_g2p_tile(Grid2<T> g, T* __restrict
   if (k1 != e) record_move(flags, mov, p, e, e1);
 }
 
+// USF strain / volume / stress update (particle.py:249-327) streamed over the
+// particles in storage order, in place, from the kinematic nodal velocities;
+// used before k_tile<FORCE_USL> when USF runs split (DMPM_OPT_USF_SPLIT).
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_stress_tile(Grid2<T> g, T* __restrict__ pb,
+                                                          int64_t ld, int n,
+                                                          const T* __restrict__ vkin, Mat2 mat,
+                                                          T dt, int temps) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const T x = PR(DMPM_P_X)[p], y = PR(DMPM_P_X + 1)[p];
+  const int e = element_id<T, 2>(g.geo(), x, y);
+  if (!(e >= 0 && e < g.ncells)) {  // outside the mesh: the state becomes NaN
+    const T nan = qnan<T>();
+    for (int k = 0; k < 6; ++k) {
+      PR(DMPM_P_STRAIN + k)[p] = nan;
+      PR(DMPM_P_STRESS + k)[p] = nan;
+    }
+    PR(DMPM_P_VOLUME)[p] = nan;
+    PR(DMPM_P_DENSITY)[p] = nan;
+    PR(DMPM_P_VOLSTRAIN_C)[p] = nan;
+    if (temps) {
+      for (int k = 0; k < 6; ++k) {
+        PR(DMPM_P_STRAIN_RATE + k)[p] = nan;
+        PR(DMPM_P_DSTRAIN + k)[p] = nan;
+      }
+      PR(DMPM_P_DVOLSTRAIN)[p] = nan;
+    }
+    return;
+  }
+  const int I = e % g.nx, J = e / g.nx;
+  const T xi = natural(x, g.X(I), g.X(I + 1));
+  const T eta = natural(y, g.Y(J), g.Y(J + 1));
+  const int ll = J * (g.nx + 1) + I;
+  const int ids[4] = {ll, ll + 1, ll + g.nx + 2, ll + g.nx + 1};
+  T vx[4], vy[4];
+  for (int k = 0; k < 4; ++k) {
+    vx[k] = vkin[ids[k]];
+    vy[k] = vkin[g.nnodes + ids[k]];
+  }
+  T rr[6], rc[6], d[6];
+  rate2(g, I, J, xi, eta, vx, vy, rr);
+  rate2(g, I, J, T(0), T(0), vx, vy, rc);
+  for (int k = 0; k < 6; ++k) d[k] = rr[k] * dt;
+  const T dvol = dt * (rc[0] + rc[1]);
+  T ds[4];
+  stress_inc(mat, mat.per_E ? PR(DMPM_P_YOUNGS)[p] : T(0), d, ds);
+  const T f = T(1) + dvol;
+  PR(DMPM_P_STRAIN + 0)[p] = PR(DMPM_P_STRAIN + 0)[p] + d[0];
+  PR(DMPM_P_STRAIN + 1)[p] = PR(DMPM_P_STRAIN + 1)[p] + d[1];
+  PR(DMPM_P_STRAIN + 3)[p] = PR(DMPM_P_STRAIN + 3)[p] + d[3];
+  for (int k = 0; k < 4; ++k) PR(DMPM_P_STRESS + k)[p] = PR(DMPM_P_STRESS + k)[p] + ds[k];
+  PR(DMPM_P_VOLUME)[p] = PR(DMPM_P_VOLUME)[p] * f;
+  PR(DMPM_P_DENSITY)[p] = PR(DMPM_P_DENSITY)[p] / f;
+  PR(DMPM_P_VOLSTRAIN_C)[p] = PR(DMPM_P_VOLSTRAIN_C)[p] + dvol;
+  if (temps) {
+    for (int k = 0; k < 6; ++k) {
+      PR(DMPM_P_STRAIN_RATE + k)[p] = rr[k];
+      PR(DMPM_P_DSTRAIN + k)[p] = d[k];
+    }
+    PR(DMPM_P_DVOLSTRAIN)[p] = dvol;
+  }
+}
+
 // Element ids and sort keys from the positions at the start of a run; marks
 // the sort dirty when a key changed.
 template <typename T>
@@ -639,6 +704,7 @@ __global__ __launch_bounds__(kThreads) void k_sort_coop(
   if (mode == SORT_NONE) return;
   cg::grid_group grid = cg::this_grid();
   __shared__ unsigned long long sh64[kMovCap];
+  __shared__ int shS[kMovCap];
   int* sh = reinterpret_cast<int*>(sh64);  // kBigCellMax ints
   const int G = gridDim.x, b = blockIdx.x, tid = threadIdx.x;
   const int gt = b * kThreads + tid, gs = G * kThreads;
@@ -669,40 +735,26 @@ __global__ __launch_bounds__(kThreads) void k_sort_coop(
       }
     }
     grid.sync();
-    // Non-movers keep their relative order.  Slots are taken in rounds of
-    // kThreads: (old key, index) increases with the slot, so the mover counts
-    // of every slot in a round lie between those of its first and last slot
-    // and each thread searches only that (usually empty) range.
-    __shared__ int bnd[4];
-    for (int base = b * kThreads; base < n; base += gs) {
-      const int last = min(base + kThreads, n) - 1;
-      if (tid == 0) {
-        const int pf = po[base], pl = po[last];
-        bnd[0] = lower_count(mS, nmov, base);
-        bnd[1] = lower_count(mS, nmov, last + 1);
-        bnd[2] = lower_count(mkey, nmov,
-                             ((unsigned long long)(unsigned)key[pf] << 32) | (unsigned)pf);
-        bnd[3] = lower_count(mkey, nmov,
-                             (((unsigned long long)(unsigned)key[pl] << 32) | (unsigned)pl) + 1);
-      }
-      __syncthreads();
-      const int s = base + tid;
-      if (s < n) {
-        const int jm = bnd[0] + lower_count(mS + bnd[0], bnd[1] - bnd[0], s);
-        if (!(jm < nmov && mS[jm] == s)) {
-          const int p = po[s];
-          const unsigned long long kp =
-              ((unsigned long long)(unsigned)key[p] << 32) | (unsigned)p;
-          const int pos = s - jm + bnd[2] + lower_count(mkey + bnd[2], bnd[3] - bnd[2], kp);
-          pn[pos] = p;
-          iperm[p] = pos;
-        }
-      }
-      __syncthreads();
+    // Non-movers keep their relative order: slot s moves to s minus the
+    // movers before it plus the movers whose (new cell, index) sorts before
+    // its (cell, index).  The sorted lists are searched in LDS.
+    for (int j = tid; j < nmov; j += kThreads) {
+      shS[j] = mS[j];
+      sh64[j] = mkey[j];
+    }
+    __syncthreads();
+    for (int s = gt; s < n; s += gs) {
+      const int jm = lower_count(shS, nmov, s);
+      if (jm < nmov && shS[jm] == s) continue;
+      const int p = po[s];
+      const unsigned long long kp = ((unsigned long long)(unsigned)key[p] << 32) | (unsigned)p;
+      const int pos = s - jm + lower_count(sh64, nmov, kp);
+      pn[pos] = p;
+      iperm[p] = pos;
     }
     // movers go before the first non-mover of their new cell with a larger index
     for (int j = gt; j < nmov; j += gs) {
-      const unsigned long long kj = mkey[j];
+      const unsigned long long kj = sh64[j];
       const int kn = (int)(kj >> 32), p = (int)(kj & 0xffffffffu);
       int lo = co[kn], hi = co[kn + 1];
       while (lo < hi) {
@@ -710,26 +762,19 @@ __global__ __launch_bounds__(kThreads) void k_sort_coop(
         if (po[mid] < p) lo = mid + 1;
         else hi = mid;
       }
-      const int pos = (lo - lower_count(mS, nmov, lo)) + j;
+      const int pos = (lo - lower_count(shS, nmov, lo)) + j;
       pn[pos] = p;
       iperm[p] = pos;
     }
-    // new cell offsets, in rounds like the slots
-    for (int base = b * kThreads; base <= nbk; base += gs) {
-      const int last = min(base + kThreads, nbk + 1) - 1;
-      if (tid == 0) {
-        bnd[0] = lower_count(mKO, nmov, base);
-        bnd[1] = lower_count(mKO, nmov, last + 1);
-        bnd[2] = lower_count(mKN, nmov, base);
-        bnd[3] = lower_count(mKN, nmov, last + 1);
-      }
-      __syncthreads();
-      const int c = base + tid;
-      if (c <= nbk)
-        cn[c] = co[c] - (bnd[0] + lower_count(mKO + bnd[0], bnd[1] - bnd[0], c)) +
-                (bnd[2] + lower_count(mKN + bnd[2], bnd[3] - bnd[2], c));
-      __syncthreads();
+    // new cell offsets
+    __syncthreads();
+    for (int j = tid; j < nmov; j += kThreads) {
+      shS[j] = mKO[j];
+      sh[j] = mKN[j];
     }
+    __syncthreads();
+    for (int c = gt; c <= nbk; c += gs)
+      cn[c] = co[c] - lower_count(shS, nmov, c) + lower_count(sh, nmov, c);
     grid.sync();
     for (int j = gt; j < nmov; j += gs) {
       const int p = mov[3 * j], e = mov[3 * j + 2];
@@ -891,6 +936,7 @@ int tile_steps(dmpm_plan* P, const Grid2<T>& g, T* pb, int32_t* eid, T* nb, int 
   const int64_t ld = P->d.ld, nld = P->d.nld;
   const int n = (int)P->n;
   const bool usf = scheme == DMPM_USF;
+  const bool split = usf && P->usf_split;
   TileShape tl;
   tl.ty = P->tile_ty > 0 ? P->tile_ty : 8;
   tl.ntx = (g.nx + 1 + TXN - 1) / TXN;
@@ -941,9 +987,14 @@ int tile_steps(dmpm_plan* P, const Grid2<T>& g, T* pb, int32_t* eid, T* nb, int 
           P->nfd, P->nf_off, P->nf_fn, P->nf_val, fns, temps);
       DMPM_TRY(launch_check("tile_kin"));
     }
+    if (split) {
+      ProfScopeT ps_(P, KC_STRESS, s);
+      k_stress_tile<T><<<nblk(n), kThreads, 0, s>>>(g, pb, ld, n, vkin, mat, tdt, temps);
+      DMPM_TRY(launch_check("tile_stress"));
+    }
     {
       ProfScopeT ps_(P, KC_TILE_FORCE, s);
-      if (usf)
+      if (usf && !split)
         k_tile<T, MODE_FORCE_USF, TXN><<<ntiles, 2 * TXN, 0, s>>>(
             g, tl, pb, ld, sr, nb, nld, vkin, P->cidx, P->cmask, P->cval, db, mat, gx, gy, tdt,
             t, P->nfd, P->nf_off, P->nf_fn, P->nf_val, fns, temps);
@@ -957,13 +1008,13 @@ int tile_steps(dmpm_plan* P, const Grid2<T>& g, T* pb, int32_t* eid, T* nb, int 
       ProfScopeT ps_(P, KC_TILE_G2P, s);
       if (usf)
         k_g2p_tile<T, false><<<nblk(n), kThreads, 0, s>>>(g, pb, ld, n, nb, nld, P->flags, P->mov,
-                                                          db, mat, tdt, vu, temps, 1);
+                                                          db, mat, tdt, vu, temps, split ? 0 : 1);
       else
         k_g2p_tile<T, true><<<nblk(n), kThreads, 0, s>>>(g, pb, ld, n, nb, nld, P->flags, P->mov,
                                                          db, mat, tdt, vu, temps, 0);
       DMPM_TRY(launch_check("tile_g2p"));
     }
-    if (usf) parity ^= 1;
+    if (usf && !split) parity ^= 1;
   }
   if (parity) {  // the latest double-buffered fields are in the workspace
     for (int k = 0; k < kDB; ++k)
@@ -981,9 +1032,10 @@ int tile_run_t(dmpm_plan* P, T* pb, int32_t* eid, T* nb, int scheme, int vu, con
   const int64_t ld = P->d.ld;
   const int n = (int)P->n;
   const bool usf = scheme == DMPM_USF;
-  T* alt = usf ? (T*)plan_workspace(P, 0, (size_t)kDB * ld * sizeof(T)) : nullptr;
+  const bool dbuf = usf && !P->usf_split;
+  T* alt = dbuf ? (T*)plan_workspace(P, 0, (size_t)kDB * ld * sizeof(T)) : nullptr;
   T* vkin = (T*)plan_workspace(P, 1, (size_t)2 * P->nnodes * sizeof(T));
-  if ((usf && !alt) || !vkin) return fail(DMPM_EHIP, "workspace allocation failed");
+  if ((dbuf && !alt) || !vkin) return fail(DMPM_EHIP, "workspace allocation failed");
   int G = 0;
   DMPM_TRY(coop_grid_size((const void*)k_sort_coop<T>, G));
   int* bs = (int*)plan_workspace(P, 2, (size_t)G * sizeof(int));

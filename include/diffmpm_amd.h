@@ -135,6 +135,11 @@ int dmpm_plan_set_tractions(dmpm_plan* plan, int32_t n, const int32_t* pids,
  * node_cols (64 or 128; 0 = keep).  A tuning knob with no effect on results.
  * No reference counterpart. */
 int dmpm_plan_set_tile(dmpm_plan* plan, int32_t node_rows, int32_t node_cols);
+/* Execution options (no effect on results; no reference counterpart):
+ * DMPM_OPT_USF_SPLIT = 1: the fused USF step updates stresses in a streamed
+ * particle kernel before the force tile kernel (1, default) or inside it (0). */
+enum { DMPM_OPT_USF_SPLIT = 1 };
+int dmpm_plan_set_option(dmpm_plan* plan, int32_t option, int32_t value);
 /* Declares that the element ids changed outside the library (forces a re-sort). */
 int dmpm_mark_unsorted(dmpm_plan* plan, void* stream);
 

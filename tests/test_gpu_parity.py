@@ -237,11 +237,12 @@ def _moving_spec(scheme, per_particle_E=False, nx=200, ny=40, rows=30, drift=(0.
     return spec
 
 
+@pytest.mark.parametrize("fused", [False, True])
 @pytest.mark.parametrize("scheme,per_E,drift", [("usf", False, (0.6, 0.3)),
                                                  ("usl", False, (0.6, 0.3)),
                                                  ("usf", True, (0.6, 0.3)),
                                                  ("usf", False, None), ("usl", True, None)])
-def test_tile_path_equals_per_phase_path(scheme, per_E, drift):
+def test_tile_path_equals_per_phase_path(scheme, per_E, drift, fused):
     """The fused node-tile kernels (dmpm_run default) and the per-phase
     kernels (dmpm_run flag bit 2) give bit-identical states, across cell
     changes: with a drift every particle changes cell (full re-sorts), without
@@ -252,6 +253,9 @@ def test_tile_path_equals_per_phase_path(scheme, per_E, drift):
     g = torch.tensor(spec["gravity"], dtype=torch.float64)
     pa = plan_for(a.mesh.elements, a.mesh.particles[0])
     pb = plan_for(b.mesh.elements, b.mesh.particles[0])
+    if fused:
+        import diffmpm_amd._lib as L
+        pa.set_option(L.DMPM_OPT_USF_SPLIT, 0)
     e0 = a.mesh.particles[0].element_ids.clone()
     for k in range(0, spec["nsteps"], 3):
         pa.run(scheme, False, g, spec["dt"], k, 3, True, per_phase=False)
@@ -294,3 +298,26 @@ def test_tile_path_per_step_parity(scheme, dtype):
         osim.step(i, np.asarray(spec["gravity"]))
         plan.run(scheme, False, g, spec["dt"], i, 1, True)
         _compare(osim, gsim, rtol, f"step {i}")
+
+
+@pytest.mark.parametrize("scheme", ["usf", "usl"])
+def test_tile_path_particles_below_and_left_of_mesh(scheme):
+    """Points below or left of the mesh belong to element 0 (A1 quirk,
+    element.py:827-837); the fused path treats them like the per-phase path."""
+    from diffmpm_amd._plan import plan_for
+    spec = cases.synthetic_spec("quad4", nx=150, ny=20, rows=10, seed=4, scheme=scheme,
+                                traction=False, E=1e3, dt=1e-3, velocity_scale=0.05, nsteps=4)
+    pts = np.asarray(spec["particles"])
+    pts[:40, 1] = -0.002      # below the mesh, spread over x
+    pts[40:60, 0] = -0.003    # left of the mesh
+    spec["particles"] = pts.tolist()
+    spec["constraints"] = []
+    a, b = cases.build_gpu(spec), cases.build_gpu(spec)
+    g = torch.tensor(spec["gravity"], dtype=torch.float64)
+    pa = plan_for(a.mesh.elements, a.mesh.particles[0])
+    pb = plan_for(b.mesh.elements, b.mesh.particles[0])
+    assert (a.mesh.particles[0].element_ids[:60] == 0).all()
+    pa.run(scheme, False, g, spec["dt"], 0, 4, True, per_phase=False)
+    pb.run(scheme, False, g, spec["dt"], 0, 4, True, per_phase=True)
+    for x, y in zip(_state(a), _state(b)):
+        assert torch.equal(x, y)
