@@ -92,6 +92,9 @@ def main():
     ap.add_argument("--usf-fused", action="store_true",
                     help="update stresses inside the force tile kernel")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="after the timed region, copy 512 MiB with a plain device copy "
+                         "(reference kernel for HBM counter scaling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path "
@@ -146,6 +149,12 @@ def main():
                      "traffic": None, "avg_launch_us": kavg * 1e3,
                      "bytes_per_particle": kbytes, "launches": klaunch},
     }
+    if args.copy_calibration:
+        a = torch.empty(128 * 1024 * 1024, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
     if not args.no_cpu_baseline:
         print("# cpu baseline", file=sys.stderr, flush=True)
         line["cpu_baseline"] = cpu_baseline()

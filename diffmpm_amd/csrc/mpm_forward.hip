@@ -831,7 +831,8 @@ static void free_ptr(void* p) {
 void plan_free(dmpm_plan* P) {
   if (!P) return;
   void* ptrs[] = {P->key, P->perm, P->counts, P->fill, P->cs, P->bsum, P->flags, P->big,
-                  P->perm2, P->cs2, P->iperm, P->mov, P->msort, P->mkey,
+                  P->perm2, P->cs2, P->iperm, P->mov, P->dL, P->dA, P->headL, P->headA,
+                  P->mflag,
                   P->con_node, P->con_dir, P->con_val, P->nf_id, P->nf_dir, P->nf_off,
                   P->nf_fn, P->nf_val, P->tr_id, P->tr_dir, P->tr_off, P->tr_fn, P->tr_val,
                   P->fn_kind, P->fn_off, P->fn_cnt, P->fn_x, P->fn_f, P->cidx, P->cmask,
@@ -929,10 +930,18 @@ int dmpm_plan_create(const dmpm_plan_desc* desc, dmpm_plan** out) {
   al(&P->perm2, n);
   al(&P->cs2, nb + 1);
   al(&P->iperm, n);
-  al(&P->mov, 3 * kMovCap);
-  al(&P->msort, 3 * kMovCap);
-  if (rc == DMPM_OK)
-    rc = hip_check(hipMalloc((void**)&P->mkey, kMovCap * sizeof(unsigned long long)), "hipMalloc");
+  P->movcap = (int)std::max<int64_t>(4096, P->n / 4 + 1);
+  al(&P->mov, (int64_t)kMovRec * P->movcap);
+  al(&P->dL, nb);
+  al(&P->dA, nb);
+  al(&P->headL, nb);
+  al(&P->headA, nb);
+  if (rc == DMPM_OK) rc = hip_check(hipMalloc((void**)&P->mflag, n), "hipMalloc");
+  if (rc == DMPM_OK) rc = hip_check(hipMemset(P->dL, 0, nb * sizeof(int)), "hipMemset");
+  if (rc == DMPM_OK) rc = hip_check(hipMemset(P->dA, 0, nb * sizeof(int)), "hipMemset");
+  if (rc == DMPM_OK) rc = hip_check(hipMemset(P->headL, 0xff, nb * sizeof(int)), "hipMemset");
+  if (rc == DMPM_OK) rc = hip_check(hipMemset(P->headA, 0xff, nb * sizeof(int)), "hipMemset");
+  if (rc == DMPM_OK) rc = hip_check(hipMemset(P->mflag, 0, n), "hipMemset");
   if (rc == DMPM_OK) rc = hip_check(hipMemset(P->key, 0xff, n * sizeof(int)), "hipMemset");
   if (rc == DMPM_OK) rc = hip_check(hipMemset(P->cs, 0, (nb + 1) * sizeof(int)), "hipMemset");
   if (rc == DMPM_OK) rc = hip_check(hipMemset(P->perm, 0, n * sizeof(int)), "hipMemset");

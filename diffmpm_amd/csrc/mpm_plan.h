@@ -25,15 +25,21 @@ struct dmpm_plan {
   // flags: [0] full re-sort needed, [1] perm is the identity, [2] a cell too big
   // for the deterministic fix-up was seen, [3] big-cell count, [4] current
   // buffer of perm/cs (0 or 1), [5] cell changes recorded since the last sort,
-  // [6] more changes than kMovCap (full re-sort needed)
+  // [6] more changes than movcap (full re-sort needed)
   int* flags = nullptr;
   int* big = nullptr;     // [ncells] cells queued for the big-cell sort
   int* perm2 = nullptr;   // second buffer of perm (incremental re-sort)
   int* cs2 = nullptr;     // second buffer of cs
   int* iperm = nullptr;   // [n] slot of each particle in the current perm
-  int* mov = nullptr;     // [3*kMovCap] recorded cell changes (particle, old key, new key)
-  int* msort = nullptr;   // [3*kMovCap] sorted old slots, old keys, new keys
-  unsigned long long* mkey = nullptr;  // [kMovCap] sorted (new key << 32 | particle)
+  // recorded cell changes, kMovRec ints each: particle, old key, new element id,
+  // old slot, next leaver of the old cell, next arrival of the new cell
+  int movcap = 0;
+  int* mov = nullptr;
+  int* dL = nullptr;      // [ncells+1] leavers per cell (zero between sorts)
+  int* dA = nullptr;      // [ncells+1] arrivals per cell (zero between sorts)
+  int* headL = nullptr;   // [ncells+1] leaver list heads (-1 between sorts)
+  int* headA = nullptr;   // [ncells+1] arrival list heads (-1 between sorts)
+  unsigned char* mflag = nullptr;  // [n] 1 at the old slot of a recorded mover during a sort
   // velocity constraints, last value per (node, dir)
   int ncon = 0;
   int* con_node = nullptr;
@@ -91,7 +97,7 @@ struct dmpm_plan {
 #define DMPM_TRY_H(x) DMPM_TRY(::dmpm::hip_check((x), #x))
 
 namespace dmpm {
-constexpr int kMovCap = 4096;  // cell changes merged incrementally per step
+constexpr int kMovRec = 6;  // ints per recorded cell change
 
 // The current sort buffers (selected on the device by flags[4]).
 struct SortRef {

@@ -420,15 +420,15 @@ __global__ __launch_bounds__(2 * kTX) void k_tile(
 // step-start natural coordinates and the updated nodal velocities
 // (scheme.py:127-133).  Sets flags[0] when a particle changes cell.
 // Records a particle whose cell (sort key) changed: (particle, old key, new
-// element id) is appended for the incremental re-sort; past kMovCap a full
-// sort is requested.
-__device__ __forceinline__ void record_move(int* __restrict__ flags, int* __restrict__ mov, int p,
-                                            int kold, int enew) {
+// element id) is appended for the incremental re-sort; past movcap records a
+// full sort is requested.
+__device__ __forceinline__ void record_move(int* __restrict__ flags, int* __restrict__ mov,
+                                            int movcap, int p, int kold, int enew) {
   const int k = atomicAdd(&flags[5], 1);
-  if (k < kMovCap) {
-    mov[3 * k] = p;
-    mov[3 * k + 1] = kold;
-    mov[3 * k + 2] = enew;
+  if (k < movcap) {
+    mov[kMovRec * k] = p;
+    mov[kMovRec * k + 1] = kold;
+    mov[kMovRec * k + 2] = enew;
   } else {
     flags[6] = 1;
   }
@@ -438,7 +438,7 @@ template <typename T, bool USL>
 __global__ __launch_bounds__(kThreads) void k_g2p_tile(Grid2<T> g, T* __restrict__ pb, int64_t ld,
                                                        int n, const T* __restrict__ nb,
                                                        int64_t nld, int* __restrict__ flags,
-                                                       int* __restrict__ mov,
+                                                       int* __restrict__ mov, int movcap,
                                                        DB<T> db, Mat2 mat, T dt, int vu,
                                                        int temps, int usf_db) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
@@ -475,7 +475,7 @@ __global__ __launch_bounds__(kThreads) void k_g2p_tile(Grid2<T> g, T* __restrict
       PR(DMPM_P_DVOLSTRAIN)[p] = nan;
     }
     // NaN positions search to element 0 (element.py:827-837)
-    record_move(flags, mov, p, g.ncells, 0);
+    record_move(flags, mov, movcap, p, g.ncells, 0);
     return;
   }
   const int I = e % g.nx, J = e / g.nx;
@@ -533,7 +533,7 @@ __global__ __launch_bounds__(kThreads) void k_g2p_tile(Grid2<T> g, T* __restrict
   }
   const int e1 = element_id<T, 2>(g.geo(), x1, y1);
   const int k1 = (e1 >= 0 && e1 < g.ncells) ? e1 : g.ncells;
-  if (k1 != e) record_move(flags, mov, p, e, e1);
+  if (k1 != e) record_move(flags, mov, movcap, p, e, e1);
 }
 
 // USF strain / volume / stress update (particle.py:249-327) streamed over the
@@ -645,70 +645,59 @@ __device__ __forceinline__ int lower_count(const K* __restrict__ a, int n, K v) 
   return lo;
 }
 
-// Bitonic sort of m <= kMovCap keys in LDS by one workgroup (padded with the
-// maximum key), then written to out[0..m).
-template <typename K>
-__device__ void block_sort(K* __restrict__ sh, int m, K* __restrict__ out, K pad) {
-  int P2 = 1;
-  while (P2 < m) P2 <<= 1;
-  for (int i = threadIdx.x + m; i < P2; i += kThreads) sh[i] = pad;
-  __syncthreads();
-  for (int k = 2; k <= P2; k <<= 1) {
-    for (int j = k >> 1; j > 0; j >>= 1) {
-      for (int i = threadIdx.x; i < P2; i += kThreads) {
-        const int l = i ^ j;
-        if (l > i) {
-          const bool up = (i & k) == 0;
-          const K a = sh[i], c = sh[l];
-          if ((a > c) == up) {
-            sh[i] = c;
-            sh[l] = a;
-          }
-        }
-      }
-      __syncthreads();
-    }
-  }
-  for (int i = threadIdx.x; i < m; i += kThreads) out[i] = sh[i];
-  __syncthreads();
-}
-
 enum { SORT_NONE = 0, SORT_INCREMENTAL = 1, SORT_FULL = 2 };
+
+// Block-level exclusive scan of f(c) over [lo, hi) written as out[c] = base[c]
+// + carry + prefix, for the chunked grid scans of k_sort_coop.
+template <typename F>
+__device__ __forceinline__ void chunk_scan(int lo, int hi, int carry, int* sh, F f,
+                                           const int* __restrict__ base, int* __restrict__ out) {
+  for (int b0 = lo; b0 < hi; b0 += kThreads) {
+    const int c = b0 + threadIdx.x;
+    const int v = c < hi ? f(c) : 0;
+    int tot;
+    const int ex = blk_excl_scan(v, sh, tot);
+    if (c < hi) out[c] = (base ? base[c] : 0) + carry + ex;
+    carry += tot;
+  }
+}
 
 // Stable sort of the particles by cell in one cooperative launch (phases
 // separated by grid barriers).  The mode is read from the flags, which no
 // block writes before the first barrier, so all blocks agree:
 //  * nothing to do: every block returns;
-//  * SORT_INCREMENTAL (1..kMovCap recorded cell changes): the new order is the
-//    merge of the old order without the movers and the movers sorted by
-//    (new cell, index); every slot finds its new position by binary searches
-//    in the sorted mover lists, one streaming pass, written to the other
-//    perm/cs buffer;
+//  * SORT_INCREMENTAL (1..movcap recorded cell changes): cell counts change
+//    by the recorded leavers/arrivals; one scan gives the new cell offsets;
+//    every particle of an untouched cell moves by its cell's offset change,
+//    every particle of a touched cell also by the leavers before it and the
+//    arrivals with a smaller index (short per-cell lists), every mover goes
+//    before the first particle of its new cell with a larger index.  One
+//    streaming pass; the result is the stable order, in the other buffer;
 //  * SORT_FULL: counting sort (histogram, scan, scatter, per-cell fix-up to
 //    ascending index) into buffer 0.
-// Also maintains element ids (full sort), keys, iperm and the buffer flag.
+// Also maintains element ids, keys, iperm and the buffer flag.
 template <typename T>
 __global__ __launch_bounds__(kThreads) void k_sort_coop(
     Grid2<T> g, const T* __restrict__ pb, int64_t ld, int n, int* __restrict__ eid,
     int* __restrict__ key, int* __restrict__ counts, int* __restrict__ fill,
     int* __restrict__ perm0, int* __restrict__ perm1, int* __restrict__ cs0,
     int* __restrict__ cs1, int* __restrict__ iperm, int* __restrict__ bs,
-    int* __restrict__ flags, int* __restrict__ big, const int* __restrict__ mov,
-    int* __restrict__ msort, unsigned long long* __restrict__ mkey) {
+    int* __restrict__ flags, int* __restrict__ big, int* __restrict__ mov, int movcap,
+    int* __restrict__ dL, int* __restrict__ dA, int* __restrict__ headL,
+    int* __restrict__ headA, unsigned char* __restrict__ mflag) {
   const int nmov = flags[5];
   // flags[2]: a cell (or the outside bucket) was too big to order by index, so
-  // the layout is not a stable sort and cannot be merged into
-  const int mode = (flags[0] || flags[6] || flags[2])
+  // the layout is not a stable sort and cannot be updated incrementally
+  const int mode = (flags[0] || flags[6] || flags[2] || nmov > movcap)
                        ? SORT_FULL
                        : (nmov > 0 ? SORT_INCREMENTAL : SORT_NONE);
   if (mode == SORT_NONE) return;
   cg::grid_group grid = cg::this_grid();
-  __shared__ unsigned long long sh64[kMovCap];
-  __shared__ int shS[kMovCap];
-  int* sh = reinterpret_cast<int*>(sh64);  // kBigCellMax ints
+  __shared__ int sh[kBigCellMax];
   const int G = gridDim.x, b = blockIdx.x, tid = threadIdx.x;
   const int gt = b * kThreads + tid, gs = G * kThreads;
   const int nbk = g.ncells + 1;
+  auto key_of = [&](int e) { return (e >= 0 && e < g.ncells) ? e : g.ncells; };
 
   if (mode == SORT_INCREMENTAL) {
     const int cur = flags[4];
@@ -716,70 +705,89 @@ __global__ __launch_bounds__(kThreads) void k_sort_coop(
     int* __restrict__ pn = cur ? perm0 : perm1;
     const int* __restrict__ co = cur ? cs1 : cs0;
     int* __restrict__ cn = cur ? cs0 : cs1;
-    int* __restrict__ mS = msort;                 // old slots
-    int* __restrict__ mKO = msort + kMovCap;      // old keys
-    int* __restrict__ mKN = msort + 2 * kMovCap;  // new keys
-    if (b < 4) {
-      auto knew = [&](int j) {
-        const int e = mov[3 * j + 2];
-        return (e >= 0 && e < g.ncells) ? e : g.ncells;
-      };
-      if (b == 3) {
-        for (int j = tid; j < nmov; j += kThreads)
-          sh64[j] = ((unsigned long long)(unsigned)knew(j) << 32) | (unsigned)mov[3 * j];
-        block_sort<unsigned long long>(sh64, nmov, mkey, ~0ull);
-      } else {
-        for (int j = tid; j < nmov; j += kThreads)
-          sh[j] = b == 0 ? iperm[mov[3 * j]] : (b == 1 ? mov[3 * j + 1] : knew(j));
-        block_sort<int>(sh, nmov, b == 0 ? mS : (b == 1 ? mKO : mKN), 0x7fffffff);
+    int* __restrict__ M = mov;
+    for (int m = gt; m < nmov; m += gs) {
+      const int p = M[kMovRec * m], ko = M[kMovRec * m + 1];
+      const int kn = key_of(M[kMovRec * m + 2]);
+      const int so = iperm[p];
+      M[kMovRec * m + 3] = so;
+      mflag[so] = 1;
+      M[kMovRec * m + 4] = atomicExch(&headL[ko], m);
+      M[kMovRec * m + 5] = atomicExch(&headA[kn], m);
+      atomicAdd(&dL[ko], 1);
+      atomicAdd(&dA[kn], 1);
+    }
+    grid.sync();
+    // new offsets: cn = co + exclusive scan of (arrivals - leavers)
+    const int chunk = (nbk + G - 1) / G;
+    const int lo = min(b * chunk, nbk), hi = min(lo + chunk, nbk);
+    {
+      int sm = 0;
+      for (int c = lo + tid; c < hi; c += kThreads) sm += dA[c] - dL[c];
+      int tot;
+      blk_excl_scan(sm, sh, tot);
+      if (tid == 0) bs[b] = tot;
+    }
+    grid.sync();
+    if (b == 0) {
+      int carry = 0;
+      for (int base = 0; base < G; base += kThreads) {
+        const int k = base + tid;
+        const int v = k < G ? bs[k] : 0;
+        int tot;
+        const int ex = blk_excl_scan(v, sh, tot);
+        if (k < G) bs[k] = ex + carry;
+        carry += tot;
       }
     }
     grid.sync();
-    // Non-movers keep their relative order: slot s moves to s minus the
-    // movers before it plus the movers whose (new cell, index) sorts before
-    // its (cell, index).  The sorted lists are searched in LDS.
-    for (int j = tid; j < nmov; j += kThreads) {
-      shS[j] = mS[j];
-      sh64[j] = mkey[j];
-    }
-    __syncthreads();
+    chunk_scan(lo, hi, bs[b], sh, [&](int c) { return dA[c] - dL[c]; }, co, cn);
+    if (gt == 0) cn[nbk] = n;
+    grid.sync();
     for (int s = gt; s < n; s += gs) {
-      const int jm = lower_count(shS, nmov, s);
-      if (jm < nmov && shS[jm] == s) continue;
+      if (mflag[s]) continue;
       const int p = po[s];
-      const unsigned long long kp = ((unsigned long long)(unsigned)key[p] << 32) | (unsigned)p;
-      const int pos = s - jm + lower_count(sh64, nmov, kp);
-      pn[pos] = p;
-      iperm[p] = pos;
-    }
-    // movers go before the first non-mover of their new cell with a larger index
-    for (int j = gt; j < nmov; j += gs) {
-      const unsigned long long kj = sh64[j];
-      const int kn = (int)(kj >> 32), p = (int)(kj & 0xffffffffu);
-      int lo = co[kn], hi = co[kn + 1];
-      while (lo < hi) {
-        const int mid = (lo + hi) >> 1;
-        if (po[mid] < p) lo = mid + 1;
-        else hi = mid;
+      const int c = key[p];
+      int pos;
+      if (dL[c] == 0 && dA[c] == 0) {
+        pos = s + (cn[c] - co[c]);
+      } else {
+        int before = 0, arr = 0;
+        for (int m = headL[c]; m >= 0; m = M[kMovRec * m + 4]) before += M[kMovRec * m + 3] < s;
+        for (int m = headA[c]; m >= 0; m = M[kMovRec * m + 5]) arr += M[kMovRec * m] < p;
+        pos = cn[c] + (s - co[c]) - before + arr;
       }
-      const int pos = (lo - lower_count(shS, nmov, lo)) + j;
       pn[pos] = p;
       iperm[p] = pos;
     }
-    // new cell offsets
-    __syncthreads();
-    for (int j = tid; j < nmov; j += kThreads) {
-      shS[j] = mKO[j];
-      sh[j] = mKN[j];
+    for (int m = gt; m < nmov; m += gs) {
+      const int p = M[kMovRec * m];
+      const int kn = key_of(M[kMovRec * m + 2]);
+      int l = co[kn], h = co[kn + 1];
+      const int first = l;
+      while (l < h) {
+        const int mid = (l + h) >> 1;
+        if (po[mid] < p) l = mid + 1;
+        else h = mid;
+      }
+      int lv = 0, arr = 0;
+      for (int q = headL[kn]; q >= 0; q = M[kMovRec * q + 4]) lv += M[kMovRec * q] < p;
+      for (int q = headA[kn]; q >= 0; q = M[kMovRec * q + 5]) arr += M[kMovRec * q] < p;
+      const int pos = cn[kn] + (l - first) - lv + arr;
+      pn[pos] = p;
+      iperm[p] = pos;
     }
-    __syncthreads();
-    for (int c = gt; c <= nbk; c += gs)
-      cn[c] = co[c] - lower_count(shS, nmov, c) + lower_count(sh, nmov, c);
     grid.sync();
-    for (int j = gt; j < nmov; j += gs) {
-      const int p = mov[3 * j], e = mov[3 * j + 2];
+    for (int m = gt; m < nmov; m += gs) {
+      const int p = M[kMovRec * m], ko = M[kMovRec * m + 1], e = M[kMovRec * m + 2];
+      const int kn = key_of(e);
       eid[p] = e;
-      key[p] = (e >= 0 && e < g.ncells) ? e : g.ncells;
+      key[p] = kn;
+      mflag[M[kMovRec * m + 3]] = 0;
+      dL[ko] = 0;
+      dA[kn] = 0;
+      headL[ko] = -1;
+      headA[kn] = -1;
     }
     if (gt == 0) {
       flags[4] = 1 - cur;
@@ -973,7 +981,8 @@ int tile_steps(dmpm_plan* P, const Grid2<T>& g, T* pb, int32_t* eid, T* nb, int 
                       (void*)&eid,      (void*)&P->key,   (void*)&P->counts, (void*)&P->fill,
                       (void*)&P->perm,  (void*)&P->perm2, (void*)&P->cs,     (void*)&P->cs2,
                       (void*)&P->iperm, (void*)&bs,       (void*)&P->flags,  (void*)&P->big,
-                      (void*)&P->mov,   (void*)&P->msort, (void*)&P->mkey};
+                      (void*)&P->mov,   (void*)&P->movcap, (void*)&P->dL,    (void*)&P->dA,
+                      (void*)&P->headL, (void*)&P->headA, (void*)&P->mflag};
       DMPM_TRY(hip_check(hipLaunchCooperativeKernel((const void*)k_sort_coop<T>, dim3(G),
                                                     dim3(kThreads), args, 0, s),
                          "hipLaunchCooperativeKernel"));
@@ -1008,10 +1017,11 @@ int tile_steps(dmpm_plan* P, const Grid2<T>& g, T* pb, int32_t* eid, T* nb, int 
       ProfScopeT ps_(P, KC_TILE_G2P, s);
       if (usf)
         k_g2p_tile<T, false><<<nblk(n), kThreads, 0, s>>>(g, pb, ld, n, nb, nld, P->flags, P->mov,
-                                                          db, mat, tdt, vu, temps, split ? 0 : 1);
+                                                          P->movcap, db, mat, tdt, vu, temps,
+                                                          split ? 0 : 1);
       else
         k_g2p_tile<T, true><<<nblk(n), kThreads, 0, s>>>(g, pb, ld, n, nb, nld, P->flags, P->mov,
-                                                         db, mat, tdt, vu, temps, 0);
+                                                         P->movcap, db, mat, tdt, vu, temps, 0);
       DMPM_TRY(launch_check("tile_g2p"));
     }
     if (usf && !split) parity ^= 1;
