@@ -22,27 +22,28 @@ HBM_PEAK_GBS = 8000.0
 # LinearElastic): the persistent particle fields the kernel must read and
 # write; grid arrays, sort keys and recomputable temporaries excluded.
 KERNEL_BYTES = {
-    # fused node-tile kernels (default path)
-    # P2G mass/momentum + nodal velocity: read x(2), v(2), m
+    # sort by cell: sort keys and permutation only (no persistent particle field)
+    "sort": 0,
+    # P2G mass/momentum + nodal velocity + constraints: read x(2), v(2), m
     "tile_kin": 4 * (2 + 2 + 1),
-    # USF strain/volume/stress update + force P2G + nodal update: read x(2), m,
-    # eps(3), sigma(4), V, rho, eps_v; write eps(3), sigma(4), V, rho, eps_v
-    "tile_force": 4 * (2 + 1 + 3 + 4 + 3) + 4 * (3 + 4 + 3),
+    # USF strain/volume/stress update from the kinematic nodal velocities:
+    # read x(2), eps(3), sigma(4), V, rho, eps_v; write eps(3), sigma(4), V, rho, eps_v
+    "stress": 4 * (2 + 3 + 4 + 3) + 4 * (3 + 4 + 3),
+    # P2G of body and internal forces, nodal forces, nodal update, constraints:
+    # read x(2), m, V, sigma(xx, yy, xy)
+    "tile_force": 4 * (2 + 1 + 1 + 3),
     # G2P: read x(2), v(2); write x(2), v(2)
     "tile_g2p": 4 * 4 + 4 * 4,
-    # strain/volume/stress update: read eps(3), sigma(4), V, rho, eps_v,
-    # xi(2, read back from the locate kernel), write eps(3), sigma(4), V, rho, eps_v
-    "stress": 4 * (3 + 4 + 3 + 2) + 4 * (3 + 4 + 3),
-    # P2G of mass and momentum: read xi(2), m, v(2)
-    "p2g_kin": 4 * (2 + 1 + 2),
-    # P2G of forces: read xi(2), m, V, sigma(3)
-    "p2g_force": 4 * (2 + 1 + 1 + 3),
-    # G2P: read xi(2), x(2), v(2); write x(2), v(2)
-    "g2p": 4 * (2 + 2 + 2) + 4 * (2 + 2),
-    # locate: read x(2); write xi(2)
-    "locate": 4 * 2 + 4 * 2,
 }
+# --usf-fused: the force kernel also does the stress update (the stress class
+# is then not launched): read x(2), m, eps(3), sigma(4), V, rho, eps_v; write
+# eps(3), sigma(4), V, rho, eps_v
+TILE_FORCE_FUSED_BYTES = 4 * (2 + 1 + 3 + 4 + 3) + 4 * (3 + 4 + 3)
 STEP_BYTES = 116  # per particle-step, whole step (FP32 2D LinearElastic)
+# HBM bytes per particle per launch of the default path at 1024x1024, from
+# rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE passes of this benchmark
+# (profiles/step_q4_1024_usf_hbm_counters.md); None until measured.
+TRAFFIC_BYTES = {}
 
 
 def cpu_baseline(rows=32, steps=8):
@@ -86,7 +87,8 @@ def main():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--n", type=int, default=1024)
-    ap.add_argument("--kernel", default="tile_force", help="kernel class timed for the roofline")
+    ap.add_argument("--kernel", default="auto",
+                    help="kernel class of the roofline (auto: the one with the most time)")
     ap.add_argument("--tile", type=int, default=0, help="node rows per tile (0: default)")
     ap.add_argument("--tile-cols", type=int, default=0, help="node columns per tile (64/128)")
     ap.add_argument("--usf-fused", action="store_true",
@@ -111,14 +113,14 @@ def main():
     N = len(ps)
     if args.tile or args.tile_cols:
         plan.set_tile(args.tile, args.tile_cols)
+    from diffmpm_amd import _lib
     if args.usf_fused:
-        from diffmpm_amd import _lib
         plan.set_option(_lib.DMPM_OPT_USF_SPLIT, 0)
     print(f"# workload built: {N} particles, {args.n}x{args.n} cells", file=sys.stderr, flush=True)
     plan.run("usf", False, gravity, sim.dt, 0, args.warmup, keep_temporaries=False)
     torch.cuda.synchronize()
     print("# warm-up done", file=sys.stderr, flush=True)
-    timing_enable(plan, args.kernel, 64 * args.steps)
+    timing_enable(plan, "all")
     stream = torch.cuda.current_stream()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     torch.cuda.synchronize()
@@ -127,13 +129,27 @@ def main():
     e1.record(stream)
     torch.cuda.synchronize()
     ms = e0.elapsed_time(e1)
-    kms, klaunch = timing_read(plan)
-    timing_enable(plan, None, 0)
+    kbytes_tab = dict(KERNEL_BYTES)
+    if args.usf_fused:
+        kbytes_tab["tile_force"] = TILE_FORCE_FUSED_BYTES
+    kernels = {}
+    for kc in kbytes_tab:
+        kms, kn = timing_read(plan, kc)
+        if kn:
+            kernels[kc] = {"avg_us": kms / kn * 1e3, "launches": kn,
+                           "bytes_per_particle": kbytes_tab[kc],
+                           "GBs": kbytes_tab[kc] * N / (kms / kn / 1e3) / 1e9}
+    timing_enable(plan, None)
     assert torch.isfinite(ps.loc).all() and torch.isfinite(ps.stress).all()
     value = N * args.steps / (ms / 1e3)
-    kbytes = KERNEL_BYTES.get(args.kernel)
-    kavg = kms / max(klaunch, 1)
-    achieved = (kbytes * N / (kavg / 1e3) / 1e9) if kbytes else None
+    if args.kernel == "auto":
+        cand = {k: v for k, v in kernels.items() if v["bytes_per_particle"]}
+        kname = max(cand, key=lambda k: cand[k]["avg_us"] * cand[k]["launches"])
+    else:
+        kname = args.kernel
+    kd = kernels[kname]
+    achieved = kd["GBs"]
+    traffic = TRAFFIC_BYTES.get(kname)
     line = {
         "metric": "particle-steps/s fwd at 4.2M ptcl FP32",
         "value": value, "unit": "particle-steps/s", "n_gpus": 1, "steps": args.steps,
@@ -143,11 +159,12 @@ def main():
                                f"{N} particles, USF, LinearElastic, gravity, dt=1e-5",
                    "particles": N, "scheme": "usf"},
         "step_effective_GBs": STEP_BYTES * N * args.steps / (ms / 1e3) / 1e9,
-        "roofline": {"bound": "hbm", "kernel": args.kernel, "achieved": achieved,
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
-                     "traffic": None, "avg_launch_us": kavg * 1e3,
-                     "bytes_per_particle": kbytes, "launches": klaunch},
+        "roofline": {"bound": "hbm", "kernel": kname, "achieved": achieved,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": (traffic * N) if (traffic and args.n == 1024) else None,
+                     "avg_launch_us": kd["avg_us"],
+                     "bytes_per_particle": kd["bytes_per_particle"], "launches": kd["launches"]},
+        "kernels": kernels,
     }
     if args.copy_calibration:
         a = torch.empty(128 * 1024 * 1024, dtype=torch.float32, device="cuda")

@@ -87,6 +87,7 @@ SIGNATURES = {
     "dmpm_phase_particle_kinematics": [_P, _P, _P, _P, c_double, c_int32, _P],
     "dmpm_timing_enable": [_P, c_int32, c_int32],
     "dmpm_timing_read": [_P, _DP, POINTER(c_int64)],
+    "dmpm_timing_read_class": [_P, c_int32, _DP, POINTER(c_int64)],
     "dmpm_run": [_P, _P, _P, _P, c_int32, c_int32, _DP, c_double, c_int64, c_int64, c_int32, _P],
 }
 

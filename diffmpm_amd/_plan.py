@@ -249,16 +249,21 @@ def plan_for(elements, particles):
     return p
 
 
-def timing_enable(plan, kernel_class, max_launches):
+def timing_enable(plan, kernel_class, max_launches=0):
     """Records HIP events around launches of one kernel class (see
-    ``_lib.KC``); ``kernel_class=None`` disables."""
-    kc = -1 if kernel_class is None else _lib.KC[kernel_class]
+    ``_lib.KC``; ``"all"`` for every class); ``kernel_class=None`` disables."""
+    kc = -1 if kernel_class is None else (-2 if kernel_class == "all"
+                                          else _lib.KC[kernel_class])
     call("dmpm_timing_enable", plan.h, kc, int(max_launches))
 
 
-def timing_read(plan):
-    """(total milliseconds, launches) of the recorded kernel class."""
+def timing_read(plan, kernel_class=None):
+    """(total milliseconds, launches) of the enabled kernel class, or of
+    ``kernel_class`` when given."""
     from ctypes import c_double, c_int64
     ms, n = c_double(), c_int64()
-    call("dmpm_timing_read", plan.h, byref(ms), byref(n))
+    if kernel_class is None:
+        call("dmpm_timing_read", plan.h, byref(ms), byref(n))
+    else:
+        call("dmpm_timing_read_class", plan.h, _lib.KC[kernel_class], byref(ms), byref(n))
     return ms.value, n.value

@@ -164,10 +164,13 @@ constexpr int kSmallCell = 64;
 constexpr int kBigCellMax = 8192;  // LDS capacity of the rank sort
 
 __global__ void k_sort_fix(const int* __restrict__ cs, int ncells, int* __restrict__ perm,
-                           int* __restrict__ flags, int* __restrict__ big) {
+                           int* __restrict__ flags, int* __restrict__ big,
+                           int* __restrict__ counts, int* __restrict__ fill) {
   if (flags[0] == 0) return;
   int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c > ncells) return;  // cells and the outside bucket
+  counts[c] = 0;  // both stay zero between sorts
+  fill[c] = 0;
   int b = cs[c], e = cs[c + 1];
   if (e - b > kSmallCell) {
     int k = atomicAdd(&flags[3], 1);
@@ -213,20 +216,12 @@ __global__ void k_sort_fix_big(const int* __restrict__ cs, int* __restrict__ per
   }
 }
 
-__global__ void k_sort_iperm(const int* __restrict__ perm, int n, int* __restrict__ iperm,
-                             const int* __restrict__ flags) {
-  if (flags[0] == 0) return;
-  for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < n; s += gridDim.x * blockDim.x)
-    iperm[perm[s]] = s;
-}
-
 __global__ void k_sort_done(int* flags) {
   if (threadIdx.x == 0 && blockIdx.x == 0) {
     if (flags[0]) {
-      flags[1] = 0;  // a new permutation: no longer known to be the identity
-      flags[4] = 0;  // it is in buffer 0
-      flags[5] = 0;  // pending cell changes are included
-      flags[6] = 0;
+      flags[1] = 0;   // a new permutation: no longer known to be the identity
+      flags[4] = 0;   // it is in buffer 0
+      flags[10] = 1;  // the fused path's recorded cell changes no longer apply
     }
     flags[0] = 0;
     flags[3] = 0;
@@ -247,9 +242,8 @@ int enqueue_sort(dmpm_plan* P, hipStream_t s) {
     k_scan_add<<<nblocks(nb), kBlock, 0, s>>>(P->cs, nb, n, P->bsum, P->flags);
     k_sort_scatter<<<nblocks(n), kBlock, 0, s>>>(P->key, n, P->cs, P->fill, P->perm, P->flags);
     k_sort_fix<<<nblocks(P->ncells + 1), kBlock, 0, s>>>(P->cs, P->ncells, P->perm, P->flags,
-                                                     P->big);
+                                                     P->big, P->counts, P->fill);
     k_sort_fix_big<<<256, kBlock, 0, s>>>(P->cs, P->perm, P->flags, P->big);
-    k_sort_iperm<<<1024, kBlock, 0, s>>>(P->perm, n, P->iperm, P->flags);
   }
   k_sort_done<<<1, 64, 0, s>>>(P->flags);
   return launch_check("sort");
@@ -831,8 +825,8 @@ static void free_ptr(void* p) {
 void plan_free(dmpm_plan* P) {
   if (!P) return;
   void* ptrs[] = {P->key, P->perm, P->counts, P->fill, P->cs, P->bsum, P->flags, P->big,
-                  P->perm2, P->cs2, P->iperm, P->mov, P->dL, P->dA, P->headL, P->headA,
-                  P->mflag,
+                  P->perm2, P->cs2, P->mov, P->mrec, P->dL, P->dA, P->headL, P->headA,
+                  P->chunkD, P->chunkC,
                   P->con_node, P->con_dir, P->con_val, P->nf_id, P->nf_dir, P->nf_off,
                   P->nf_fn, P->nf_val, P->tr_id, P->tr_dir, P->tr_off, P->tr_fn, P->tr_val,
                   P->fn_kind, P->fn_off, P->fn_cnt, P->fn_x, P->fn_f, P->cidx, P->cmask,
@@ -925,27 +919,40 @@ int dmpm_plan_create(const dmpm_plan_desc* desc, dmpm_plan** out) {
   al(&P->fill, nb);
   al(&P->cs, nb + 1);
   al(&P->bsum, nbs);
-  al(&P->flags, 8);
+  al(&P->flags, kFlagWords);
   al(&P->big, P->ncells);
   al(&P->perm2, n);
   al(&P->cs2, nb + 1);
-  al(&P->iperm, n);
-  P->movcap = (int)std::max<int64_t>(4096, P->n / 4 + 1);
+  P->movcap = (int)std::max<int64_t>(1, P->n);
   al(&P->mov, (int64_t)kMovRec * P->movcap);
+  al(&P->mrec, n);
   al(&P->dL, nb);
   al(&P->dA, nb);
   al(&P->headL, nb);
   al(&P->headA, nb);
-  if (rc == DMPM_OK) rc = hip_check(hipMalloc((void**)&P->mflag, n), "hipMalloc");
-  if (rc == DMPM_OK) rc = hip_check(hipMemset(P->dL, 0, nb * sizeof(int)), "hipMemset");
-  if (rc == DMPM_OK) rc = hip_check(hipMemset(P->dA, 0, nb * sizeof(int)), "hipMemset");
-  if (rc == DMPM_OK) rc = hip_check(hipMemset(P->headL, 0xff, nb * sizeof(int)), "hipMemset");
-  if (rc == DMPM_OK) rc = hip_check(hipMemset(P->headA, 0xff, nb * sizeof(int)), "hipMemset");
-  if (rc == DMPM_OK) rc = hip_check(hipMemset(P->mflag, 0, n), "hipMemset");
+  P->chunk_shift = kMinChunkShift;
+  while (((int64_t)nb + (1 << P->chunk_shift) - 1) >> P->chunk_shift > kMaxChunks &&
+         P->chunk_shift < 20)
+    ++P->chunk_shift;
+  P->nchunks = (int)((nb + (1 << P->chunk_shift) - 1) >> P->chunk_shift);
+  al(&P->chunkD, P->nchunks);
+  al(&P->chunkC, P->nchunks);
+  auto zero = [&](void* p, int64_t bytes, int v) {
+    if (rc == DMPM_OK && p) rc = hip_check(hipMemset(p, v, bytes), "hipMemset");
+  };
+  zero(P->counts, nb * sizeof(int), 0);
+  zero(P->fill, nb * sizeof(int), 0);
+  zero(P->mrec, n * sizeof(int), 0);
+  zero(P->dL, nb * sizeof(int), 0);
+  zero(P->dA, nb * sizeof(int), 0);
+  zero(P->headL, nb * sizeof(int), 0xff);
+  zero(P->headA, nb * sizeof(int), 0xff);
+  zero(P->chunkD, P->nchunks * sizeof(int), 0);
+  zero(P->chunkC, P->nchunks * sizeof(int), 0);
   if (rc == DMPM_OK) rc = hip_check(hipMemset(P->key, 0xff, n * sizeof(int)), "hipMemset");
   if (rc == DMPM_OK) rc = hip_check(hipMemset(P->cs, 0, (nb + 1) * sizeof(int)), "hipMemset");
   if (rc == DMPM_OK) rc = hip_check(hipMemset(P->perm, 0, n * sizeof(int)), "hipMemset");
-  int f0[8] = {1, 0, 0, 0, 0, 0, 0, 0};
+  int f0[kFlagWords] = {1};
   if (rc == DMPM_OK)
     rc = hip_check(hipMemcpy(P->flags, f0, sizeof(f0), hipMemcpyHostToDevice), "hipMemcpy");
   if (rc != DMPM_OK) {
@@ -956,9 +963,19 @@ int dmpm_plan_create(const dmpm_plan_desc* desc, dmpm_plan** out) {
   return DMPM_OK;
 }
 
+static void prof_off(dmpm_plan* P) {
+  for (auto& c : P->prof) {
+    for (int k = 0; k < kProfRing; ++k) prof_drain_slot(c, k);
+    if (c.on)
+      for (auto& e : c.ev) (void)hipEventDestroy(e);
+    c = dmpm_plan::ProfClass();
+  }
+  P->prof_single = -1;
+}
+
 int dmpm_plan_destroy(dmpm_plan* plan) {
   if (plan) {
-    for (auto e : plan->prof_ev) (void)hipEventDestroy(e);
+    prof_off(plan);
   }
   plan_free(plan);
   return DMPM_OK;
@@ -967,25 +984,43 @@ int dmpm_plan_destroy(dmpm_plan* plan) {
 int dmpm_timing_enable(dmpm_plan* P, int32_t kernel_class, int32_t max_launches) {
   (void)max_launches;
   if (!P) return fail(DMPM_EINVAL, "null plan");
-  for (int k = 0; k < kProfRing; ++k)
-    if (!P->prof_ev.empty()) prof_drain_slot(P, k);
-  for (auto e : P->prof_ev) (void)hipEventDestroy(e);
-  P->prof_ev.clear();
-  P->prof_n = 0;
-  P->prof_ms = 0;
-  P->prof_class = kernel_class;
-  if (kernel_class < 0) return DMPM_OK;
-  P->prof_ev.resize(2 * kProfRing);
-  for (auto& e : P->prof_ev) DMPM_TRY(hip_check(hipEventCreate(&e), "hipEventCreate"));
+  if (kernel_class >= kNumKC || kernel_class < DMPM_KC_ALL)
+    return fail(DMPM_EINVAL, "unknown kernel class");
+  prof_off(P);
+  if (kernel_class == -1) return DMPM_OK;
+  P->prof_single = kernel_class;
+  for (int k = 0; k < kNumKC; ++k) {
+    if (kernel_class != DMPM_KC_ALL && k != kernel_class) continue;
+    dmpm_plan::ProfClass& c = P->prof[k];
+    for (auto& e : c.ev) DMPM_TRY(hip_check(hipEventCreate(&e), "hipEventCreate"));
+    c.on = true;
+  }
+  return DMPM_OK;
+}
+
+int dmpm_timing_read_class(dmpm_plan* P, int32_t kernel_class, double* total_ms,
+                           int64_t* launches) {
+  if (!P || !total_ms || !launches) return fail(DMPM_EINVAL, "null argument");
+  if (kernel_class < 0 || kernel_class >= kNumKC) return fail(DMPM_EINVAL, "unknown kernel class");
+  dmpm_plan::ProfClass& c = P->prof[kernel_class];
+  for (int k = 0; k < kProfRing; ++k) prof_drain_slot(c, k);
+  *total_ms = c.ms;
+  *launches = c.n;
   return DMPM_OK;
 }
 
 int dmpm_timing_read(dmpm_plan* P, double* total_ms, int64_t* launches) {
   if (!P || !total_ms || !launches) return fail(DMPM_EINVAL, "null argument");
-  if (!P->prof_ev.empty())
-    for (int k = 0; k < kProfRing; ++k) prof_drain_slot(P, k);
-  *total_ms = P->prof_ms;
-  *launches = P->prof_n;
+  *total_ms = 0;
+  *launches = 0;
+  for (int k = 0; k < kNumKC; ++k) {
+    if (P->prof_single != DMPM_KC_ALL && k != P->prof_single) continue;
+    double ms = 0;
+    int64_t n = 0;
+    DMPM_TRY(dmpm_timing_read_class(P, k, &ms, &n));
+    *total_ms += ms;
+    *launches += n;
+  }
   return DMPM_OK;
 }
 

@@ -25,21 +25,27 @@ struct dmpm_plan {
   // flags: [0] full re-sort needed, [1] perm is the identity, [2] a cell too big
   // for the deterministic fix-up was seen, [3] big-cell count, [4] current
   // buffer of perm/cs (0 or 1), [5] cell changes recorded since the last sort,
-  // [6] more changes than movcap (full re-sort needed)
+  // [6] a recorded change overflowed its list bound (full re-sort needed),
+  // [7] big cell left unsorted by the running full sort, [8] finished
+  // workgroups of the last full-sort stage, [10] the per-phase sort rebuilt
+  // buffer 0 since the last fused-path sort
   int* flags = nullptr;
   int* big = nullptr;     // [ncells] cells queued for the big-cell sort
   int* perm2 = nullptr;   // second buffer of perm (incremental re-sort)
   int* cs2 = nullptr;     // second buffer of cs
-  int* iperm = nullptr;   // [n] slot of each particle in the current perm
-  // recorded cell changes, kMovRec ints each: particle, old key, new element id,
-  // old slot, next leaver of the old cell, next arrival of the new cell
+  // recorded cell changes, kMovRec ints each: particle, old key, new element
+  // id, next leaver of the old cell, next arrival of the new cell
   int movcap = 0;
   int* mov = nullptr;
+  int* mrec = nullptr;    // [n] 1 + record index of a recorded particle, else 0
   int* dL = nullptr;      // [ncells+1] leavers per cell (zero between sorts)
   int* dA = nullptr;      // [ncells+1] arrivals per cell (zero between sorts)
   int* headL = nullptr;   // [ncells+1] leaver list heads (-1 between sorts)
   int* headA = nullptr;   // [ncells+1] arrival list heads (-1 between sorts)
-  unsigned char* mflag = nullptr;  // [n] 1 at the old slot of a recorded mover during a sort
+  int nchunks = 0;        // cell chunks of 1 << chunk_shift cells (scan granularity)
+  int chunk_shift = 10;
+  int* chunkD = nullptr;  // [nchunks] net arrivals per chunk (zero between sorts)
+  int* chunkC = nullptr;  // [nchunks] particles per chunk in a full sort (zero between)
   // velocity constraints, last value per (node, dir)
   int ncon = 0;
   int* con_node = nullptr;
@@ -66,12 +72,17 @@ struct dmpm_plan {
   int *fn_kind = nullptr, *fn_off = nullptr, *fn_cnt = nullptr;
   double *fn_x = nullptr, *fn_f = nullptr;
   // per-kernel-class timing (dmpm_timing_enable): event pairs recorded around
-  // every launch of class prof_class on its stream
-  int prof_class = -1;
-  int prof_n = 0;
-  double prof_ms = 0;
-  int prof_pending[4] = {0, 0, 0, 0};
-  std::vector<hipEvent_t> prof_ev;
+  // launches of the enabled kernel classes on their streams, a ring of event
+  // pairs per class
+  struct ProfClass {
+    bool on = false;
+    int n = 0;
+    double ms = 0;
+    int pending[4] = {0, 0, 0, 0};
+    hipEvent_t ev[8] = {};
+  };
+  ProfClass prof[16];
+  int prof_single = -1;  // the class dmpm_timing_read reports (-2: all enabled)
   int tile_ty = 0;  // node rows per tile of the fused 2D kernels (0: default)
   int tile_tx = 64; // node columns per tile (64 or 128)
   int usf_split = 1; // USF: streamed stress kernel + force tile (1) or fused (0)
@@ -97,7 +108,11 @@ struct dmpm_plan {
 #define DMPM_TRY_H(x) DMPM_TRY(::dmpm::hip_check((x), #x))
 
 namespace dmpm {
-constexpr int kMovRec = 6;  // ints per recorded cell change
+constexpr int kMovRec = 5;     // ints per recorded cell change
+constexpr int kListMax = 64;   // recorded leavers/arrivals per cell handled incrementally
+constexpr int kMinChunkShift = 10;     // at least 1024 cells (4 per thread) per scan chunk
+constexpr int kMaxChunks = 8192;       // LDS histogram bound of the full sort
+constexpr int kFlagWords = 16;
 
 // The current sort buffers (selected on the device by flags[4]).
 struct SortRef {
@@ -136,30 +151,33 @@ bool tile_eligible(const dmpm_plan* P, int pfext);
 enum { KC_LOCATE = 0, KC_SORT = 1, KC_P2G_KIN = 2, KC_STRESS = 3, KC_P2G_FORCE = 4,
        KC_GRID = 5, KC_G2P = 6, KC_SMALL = 7, KC_TILE_KIN = 8, KC_TILE_FORCE = 9,
        KC_TILE_G2P = 10, KC_ADJ = 11, KC_BATCH = 12 };
-// A ring of kProfRing event pairs: before a pair is reused its elapsed time is
-// read (host waits for a launch kProfRing launches back), so at most kProfRing
-// timed launches are outstanding.
+constexpr int kNumKC = 13;
+// A ring of kProfRing event pairs per class: before a pair is reused its
+// elapsed time is read (host waits for the launch of that class kProfRing
+// launches back), so at most kProfRing timed launches per class are
+// outstanding.
 constexpr int kProfRing = 4;
-inline void prof_drain_slot(dmpm_plan* P, int slot) {
-  if (!P->prof_pending[slot]) return;
-  (void)hipEventSynchronize(P->prof_ev[2 * slot + 1]);
+inline void prof_drain_slot(dmpm_plan::ProfClass& c, int slot) {
+  if (!c.pending[slot]) return;
+  (void)hipEventSynchronize(c.ev[2 * slot + 1]);
   float ms = 0;
-  if (hipEventElapsedTime(&ms, P->prof_ev[2 * slot], P->prof_ev[2 * slot + 1]) == hipSuccess)
-    P->prof_ms += ms;
-  P->prof_pending[slot] = 0;
+  if (hipEventElapsedTime(&ms, c.ev[2 * slot], c.ev[2 * slot + 1]) == hipSuccess) c.ms += ms;
+  c.pending[slot] = 0;
 }
 inline void prof_begin(dmpm_plan* P, int cls, hipStream_t s) {
-  if (P->prof_class != cls || P->prof_ev.empty()) return;
-  int slot = P->prof_n % kProfRing;
-  prof_drain_slot(P, slot);
-  (void)hipEventRecord(P->prof_ev[2 * slot], s);
+  dmpm_plan::ProfClass& c = P->prof[cls];
+  if (!c.on) return;
+  const int slot = c.n % kProfRing;
+  prof_drain_slot(c, slot);
+  (void)hipEventRecord(c.ev[2 * slot], s);
 }
 inline void prof_end(dmpm_plan* P, int cls, hipStream_t s) {
-  if (P->prof_class != cls || P->prof_ev.empty()) return;
-  int slot = P->prof_n % kProfRing;
-  (void)hipEventRecord(P->prof_ev[2 * slot + 1], s);
-  P->prof_pending[slot] = 1;
-  P->prof_n++;
+  dmpm_plan::ProfClass& c = P->prof[cls];
+  if (!c.on) return;
+  const int slot = c.n % kProfRing;
+  (void)hipEventRecord(c.ev[2 * slot + 1], s);
+  c.pending[slot] = 1;
+  c.n++;
 }
 struct ProfScope {
   dmpm_plan* P;

@@ -1,9 +1,9 @@
 // Fused 2D explicit step on node tiles (gfx950).
 //
-// One step is four launches:
-//   k_sort_coop     cooperative sort by cell: nothing when no particle changed
-//                   cell, an O(N) merge of the recorded changes into the stable
-//                   order when a few did, a full counting sort otherwise
+// One step is a re-sort and three (split USF: four) step launches:
+//   k_sort_s1..s4   sort by cell: nothing when no particle changed cell, an
+//                   O(N) update of the stable order from the recorded cell
+//                   changes, a full counting sort when the layout is unknown
 //   k_tile<KIN>     P2G of mass and momentum, nodal velocity, constraints
 //   k_tile<FORCE_*> (USF: strain/volume/stress update of the tile's particles)
 //                   P2G of body and internal forces, nodal forces, nodal
@@ -24,14 +24,12 @@
 // fields are double-buffered between the caller's block and a plan workspace,
 // and the kinematic nodal velocity lives in its own workspace.
 #include <hip/hip_runtime.h>
-#include <hip/hip_cooperative_groups.h>
 
 #include <algorithm>
 
 #include "mpm_common.h"
 #include "mpm_plan.h"
 
-namespace cg = cooperative_groups;
 using namespace dmpm;
 
 namespace {
@@ -416,29 +414,73 @@ __global__ __launch_bounds__(2 * kTX) void k_tile(
   }
 }
 
-// G2P (particle.py:208-247); USL also updates strain/volume/stress with the
-// step-start natural coordinates and the updated nodal velocities
-// (scheme.py:127-133).  Sets flags[0] when a particle changes cell.
-// Records a particle whose cell (sort key) changed: (particle, old key, new
-// element id) is appended for the incremental re-sort; past movcap records a
-// full sort is requested.
-__device__ __forceinline__ void record_move(int* __restrict__ flags, int* __restrict__ mov,
-                                            int movcap, int p, int kold, int enew) {
-  const int k = atomicAdd(&flags[5], 1);
-  if (k < movcap) {
-    mov[kMovRec * k] = p;
-    mov[kMovRec * k + 1] = kold;
-    mov[kMovRec * k + 2] = enew;
-  } else {
-    flags[6] = 1;
-  }
+// Recorded cell changes: the lists and counters the incremental re-sort
+// reads (all zero / -1 between sorts).
+struct MoveLog {
+  int* flags;
+  int* mov;
+  int* mrec;
+  int* dL;
+  int* dA;
+  int* headL;
+  int* headA;
+  int* chunkD;
+  int movcap;
+  int chunk_shift;  // cells per scan chunk = 1 << chunk_shift
+};
+
+MoveLog move_log(dmpm_plan* P) {
+  MoveLog L;
+  L.flags = P->flags;
+  L.mov = P->mov;
+  L.mrec = P->mrec;
+  L.dL = P->dL;
+  L.dA = P->dA;
+  L.headL = P->headL;
+  L.headA = P->headA;
+  L.chunkD = P->chunkD;
+  L.movcap = P->movcap;
+  L.chunk_shift = P->chunk_shift;
+  return L;
 }
 
+__device__ __forceinline__ int key_of(int e, int ncells) {
+  return (e >= 0 && e < ncells) ? e : ncells;
+}
+
+// Records that particle p moves from cell (key) ko to element id enew: the
+// record joins the leaver list of ko and the arrival list of key(enew), and
+// the per-cell and per-chunk counters.  Past movcap records, or a list longer
+// than kListMax, a full re-sort is requested instead.
+__device__ __forceinline__ void record_move(const MoveLog& L, int ncells, int p, int ko, int enew) {
+  const int m = atomicAdd(&L.flags[5], 1);
+  if (m >= L.movcap) {
+    L.flags[6] = 1;
+    return;
+  }
+  const int kn = key_of(enew, ncells);
+  int* r = L.mov + (int64_t)kMovRec * m;
+  r[0] = p;
+  r[1] = ko;
+  r[2] = enew;
+  r[3] = atomicExch(&L.headL[ko], m);
+  r[4] = atomicExch(&L.headA[kn], m);
+  L.mrec[p] = m + 1;
+  const int cl = atomicAdd(&L.dL[ko], 1);
+  const int ca = atomicAdd(&L.dA[kn], 1);
+  atomicSub(&L.chunkD[ko >> L.chunk_shift], 1);
+  atomicAdd(&L.chunkD[kn >> L.chunk_shift], 1);
+  if (cl >= kListMax || ca >= kListMax) L.flags[6] = 1;
+}
+
+// G2P (particle.py:208-247); USL also updates strain/volume/stress with the
+// step-start natural coordinates and the updated nodal velocities
+// (scheme.py:127-133).  Records the particles that change cell for the next
+// re-sort.
 template <typename T, bool USL>
 __global__ __launch_bounds__(kThreads) void k_g2p_tile(Grid2<T> g, T* __restrict__ pb, int64_t ld,
                                                        int n, const T* __restrict__ nb,
-                                                       int64_t nld, int* __restrict__ flags,
-                                                       int* __restrict__ mov, int movcap,
+                                                       int64_t nld, MoveLog mlog,
                                                        DB<T> db, Mat2 mat, T dt, int vu,
                                                        int temps, int usf_db) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
@@ -475,7 +517,7 @@ __global__ __launch_bounds__(kThreads) void k_g2p_tile(Grid2<T> g, T* __restrict
       PR(DMPM_P_DVOLSTRAIN)[p] = nan;
     }
     // NaN positions search to element 0 (element.py:827-837)
-    record_move(flags, mov, movcap, p, g.ncells, 0);
+    record_move(mlog, g.ncells, p, g.ncells, 0);
     return;
   }
   const int I = e % g.nx, J = e / g.nx;
@@ -533,7 +575,7 @@ __global__ __launch_bounds__(kThreads) void k_g2p_tile(Grid2<T> g, T* __restrict
   }
   const int e1 = element_id<T, 2>(g.geo(), x1, y1);
   const int k1 = (e1 >= 0 && e1 < g.ncells) ? e1 : g.ncells;
-  if (k1 != e) record_move(flags, mov, movcap, p, e, e1);
+  if (k1 != e) record_move(mlog, g.ncells, p, e, e1);
 }
 
 // USF strain / volume / stress update (particle.py:249-327) streamed over the
@@ -600,23 +642,73 @@ __global__ __launch_bounds__(kThreads) void k_stress_tile(Grid2<T> g, T* __restr
   }
 }
 
-// Element ids and sort keys from the positions at the start of a run; marks
-// the sort dirty when a key changed.
+// Element ids from the positions at the start of a run.  A particle whose cell
+// differs from its sort key (or, for a particle with a pending recorded cell
+// change, from the recorded new cell) was moved outside the fused step: the
+// next re-sort is then a full one.
 template <typename T>
 __global__ void k_tile_check(Grid2<T> g, const T* __restrict__ pb, int64_t ld, int n,
-                             int* __restrict__ eid, int* __restrict__ key, int* __restrict__ flags) {
+                             int* __restrict__ eid, const int* __restrict__ key,
+                             int* __restrict__ flags, const int* __restrict__ mrec,
+                             int* __restrict__ mov) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= n) return;
   const int e = element_id<T, 2>(g.geo(), PR(DMPM_P_X)[p], PR(DMPM_P_X + 1)[p]);
   eid[p] = e;
-  const int k = (e >= 0 && e < g.ncells) ? e : g.ncells;
-  if (key[p] != k) {
-    key[p] = k;
+  const int k = key_of(e, g.ncells);
+  const int m = mrec[p] - 1;
+  if (m >= 0) {
+    int* r = mov + (int64_t)kMovRec * m;
+    if (key_of(r[2], g.ncells) == k) r[2] = e;
+    else flags[0] = 1;
+  } else if (key[p] != k) {
     flags[0] = 1;
   }
 }
 
-// ---- cooperative counting sort ------------------------------------------------
+// ---- re-sort by cell ----------------------------------------------------------
+//
+// Four ordinary launches per step (a cooperative launch and its grid barriers
+// cost more than the whole incremental update).  Every workgroup of every
+// stage derives the same mode from the flags, which only the last workgroup
+// of the final stage of a mode rewrites ("last block" completion count):
+//
+//   mode     stage 1           stage 2             stage 3            stage 4
+//   NONE     -                 -                   -                  -
+//   INC      new cell offsets  relayout (stable)   keys, list reset   -
+//   FULL     keys, histogram   offsets, big cells  scatter            order fix-up
+//
+// INC (the recorded cell changes, linked per old and new cell by G2P): the
+// new offset of a cell is its old one plus the net arrivals of all cells
+// before it; a particle of an untouched cell keeps its rank, one of a touched
+// cell loses the leavers before it and gains the arrivals with a smaller
+// index, an arriving particle goes before the first particle of its new cell
+// with a larger index.  The result equals the stable counting sort of the new
+// keys.  FULL is a counting sort with a per-cell fix-up to ascending index.
+enum { SORT_NONE = 0, SORT_INCREMENTAL = 1, SORT_FULL = 2 };
+
+struct SortArgs {
+  int* eid;
+  int* key;
+  int* counts;
+  int* fill;
+  int* perm0;
+  int* perm1;
+  int* cs0;
+  int* cs1;
+  int* big;
+  int* chunkC;
+  MoveLog L;
+  int n, ncells, nchunks;
+};
+
+__device__ __forceinline__ int sort_mode(const SortArgs& a) {
+  const int* f = a.L.flags;
+  const int nmov = f[5];
+  if (f[0] || f[2] || f[6] || f[10] || nmov > a.L.movcap) return SORT_FULL;
+  return nmov > 0 ? SORT_INCREMENTAL : SORT_NONE;
+}
+
 __device__ __forceinline__ int blk_excl_scan(int v, int* sh, int& total) {
   const int t = threadIdx.x;
   sh[t] = v;
@@ -633,237 +725,240 @@ __device__ __forceinline__ int blk_excl_scan(int v, int* sh, int& total) {
   return incl - v;
 }
 
-// Number of entries of the ascending array a[0..n) that are < v.
-template <typename K>
-__device__ __forceinline__ int lower_count(const K* __restrict__ a, int n, K v) {
-  int lo = 0, hi = n;
-  while (lo < hi) {
-    const int mid = (lo + hi) >> 1;
-    if (a[mid] < v) lo = mid + 1;
-    else hi = mid;
+// out[c] = base(c) + (sum of chunk totals before this chunk) + exclusive
+// prefix of val over the chunk, for the cells of chunk blockIdx.x (a run of
+// chunk/256 consecutive cells per thread); the last chunk also writes
+// out[nbk] = total.
+template <typename V, typename B>
+__device__ __forceinline__ void chunk_scan(int nbk, int nchunks, int shift,
+                                           const int* __restrict__ tot, V val, B base,
+                                           int* __restrict__ out, int total, int* sh) {
+  const int b = blockIdx.x, tid = threadIdx.x;
+  int pre = 0;
+  for (int k = tid; k < b; k += kThreads) pre += tot[k];
+  int red;
+  blk_excl_scan(pre, sh, red);
+  const int per = (1 << shift) / kThreads;
+  const int c0 = (b << shift) + tid * per, c1 = min(c0 + per, nbk);
+  int sum = 0;
+  for (int c = c0; c < c1; ++c) sum += val(c);
+  int bt;
+  int ex = red + blk_excl_scan(sum, sh, bt);
+  for (int c = c0; c < c1; ++c) {
+    out[c] = base(c) + ex;
+    ex += val(c);
   }
-  return lo;
+  if (b == nchunks - 1 && tid == 0) out[nbk] = total;
 }
 
-enum { SORT_NONE = 0, SORT_INCREMENTAL = 1, SORT_FULL = 2 };
-
-// Block-level exclusive scan of f(c) over [lo, hi) written as out[c] = base[c]
-// + carry + prefix, for the chunked grid scans of k_sort_coop.
-template <typename F>
-__device__ __forceinline__ void chunk_scan(int lo, int hi, int carry, int* sh, F f,
-                                           const int* __restrict__ base, int* __restrict__ out) {
-  for (int b0 = lo; b0 < hi; b0 += kThreads) {
-    const int c = b0 + threadIdx.x;
-    const int v = c < hi ? f(c) : 0;
-    int tot;
-    const int ex = blk_excl_scan(v, sh, tot);
-    if (c < hi) out[c] = (base ? base[c] : 0) + carry + ex;
-    carry += tot;
+// Counts finished workgroups; true in the last one (after every other
+// workgroup's writes are visible), which also resets the counter.
+__device__ __forceinline__ bool last_block(int* counter) {
+  __shared__ int s_last;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __threadfence();
+    s_last = atomicAdd(counter, 1) == (int)gridDim.x - 1;
   }
+  __syncthreads();
+  if (s_last) {
+    __threadfence();
+    if (threadIdx.x == 0) *counter = 0;
+  }
+  return s_last;
 }
 
-// Stable sort of the particles by cell in one cooperative launch (phases
-// separated by grid barriers).  The mode is read from the flags, which no
-// block writes before the first barrier, so all blocks agree:
-//  * nothing to do: every block returns;
-//  * SORT_INCREMENTAL (1..movcap recorded cell changes): cell counts change
-//    by the recorded leavers/arrivals; one scan gives the new cell offsets;
-//    every particle of an untouched cell moves by its cell's offset change,
-//    every particle of a touched cell also by the leavers before it and the
-//    arrivals with a smaller index (short per-cell lists), every mover goes
-//    before the first particle of its new cell with a larger index.  One
-//    streaming pass; the result is the stable order, in the other buffer;
-//  * SORT_FULL: counting sort (histogram, scan, scatter, per-cell fix-up to
-//    ascending index) into buffer 0.
-// Also maintains element ids, keys, iperm and the buffer flag.
+// Stage 1.  INC: new cell offsets (one workgroup per chunk).  FULL: element
+// ids and keys from the positions, cell histogram, chunk totals (LDS
+// histogram per workgroup), release of the pending change lists.
 template <typename T>
-__global__ __launch_bounds__(kThreads) void k_sort_coop(
-    Grid2<T> g, const T* __restrict__ pb, int64_t ld, int n, int* __restrict__ eid,
-    int* __restrict__ key, int* __restrict__ counts, int* __restrict__ fill,
-    int* __restrict__ perm0, int* __restrict__ perm1, int* __restrict__ cs0,
-    int* __restrict__ cs1, int* __restrict__ iperm, int* __restrict__ bs,
-    int* __restrict__ flags, int* __restrict__ big, int* __restrict__ mov, int movcap,
-    int* __restrict__ dL, int* __restrict__ dA, int* __restrict__ headL,
-    int* __restrict__ headA, unsigned char* __restrict__ mflag) {
-  const int nmov = flags[5];
-  // flags[2]: a cell (or the outside bucket) was too big to order by index, so
-  // the layout is not a stable sort and cannot be updated incrementally
-  const int mode = (flags[0] || flags[6] || flags[2] || nmov > movcap)
-                       ? SORT_FULL
-                       : (nmov > 0 ? SORT_INCREMENTAL : SORT_NONE);
+__global__ __launch_bounds__(kThreads) void k_sort_s1(Grid2<T> g, const T* __restrict__ pb,
+                                                      int64_t ld, SortArgs a) {
+  const int mode = sort_mode(a);
   if (mode == SORT_NONE) return;
-  cg::grid_group grid = cg::this_grid();
-  __shared__ int sh[kBigCellMax];
-  const int G = gridDim.x, b = blockIdx.x, tid = threadIdx.x;
-  const int gt = b * kThreads + tid, gs = G * kThreads;
-  const int nbk = g.ncells + 1;
-  auto key_of = [&](int e) { return (e >= 0 && e < g.ncells) ? e : g.ncells; };
-
+  __shared__ int sh[kMaxChunks];
+  const int nbk = a.ncells + 1;
   if (mode == SORT_INCREMENTAL) {
-    const int cur = flags[4];
-    const int* __restrict__ po = cur ? perm1 : perm0;
-    int* __restrict__ pn = cur ? perm0 : perm1;
-    const int* __restrict__ co = cur ? cs1 : cs0;
-    int* __restrict__ cn = cur ? cs0 : cs1;
-    int* __restrict__ M = mov;
-    for (int m = gt; m < nmov; m += gs) {
-      const int p = M[kMovRec * m], ko = M[kMovRec * m + 1];
-      const int kn = key_of(M[kMovRec * m + 2]);
-      const int so = iperm[p];
-      M[kMovRec * m + 3] = so;
-      mflag[so] = 1;
-      M[kMovRec * m + 4] = atomicExch(&headL[ko], m);
-      M[kMovRec * m + 5] = atomicExch(&headA[kn], m);
-      atomicAdd(&dL[ko], 1);
-      atomicAdd(&dA[kn], 1);
+    if ((int)blockIdx.x >= a.nchunks) return;
+    const int cur = a.L.flags[4];
+    const int* __restrict__ co = cur ? a.cs1 : a.cs0;
+    int* __restrict__ cn = cur ? a.cs0 : a.cs1;
+    const int* __restrict__ dL = a.L.dL;
+    const int* __restrict__ dA = a.L.dA;
+    chunk_scan(
+        nbk, a.nchunks, a.L.chunk_shift, a.L.chunkD, [&](int c) { return dA[c] - dL[c]; },
+        [&](int c) { return co[c]; }, cn, a.n, sh);
+    return;
+  }
+  const int tid = threadIdx.x;
+  const int gt = blockIdx.x * kThreads + tid, gs = gridDim.x * kThreads;
+  for (int k = tid; k < a.nchunks; k += kThreads) sh[k] = 0;
+  __syncthreads();
+  for (int p = gt; p < a.n; p += gs) {
+    const int e = element_id<T, 2>(g.geo(), PR(DMPM_P_X)[p], PR(DMPM_P_X + 1)[p]);
+    a.eid[p] = e;
+    const int k = key_of(e, a.ncells);
+    a.key[p] = k;
+    atomicAdd(&a.counts[k], 1);
+    atomicAdd(&sh[k >> a.L.chunk_shift], 1);
+  }
+  __syncthreads();
+  for (int k = tid; k < a.nchunks; k += kThreads)
+    if (sh[k]) atomicAdd(&a.chunkC[k], sh[k]);
+  const int nrec = min(a.L.flags[5], a.L.movcap);
+  for (int m = gt; m < nrec; m += gs) {
+    const int* r = a.L.mov + (int64_t)kMovRec * m;
+    const int ko = r[1], kn = key_of(r[2], a.ncells);
+    a.L.mrec[r[0]] = 0;
+    a.L.dL[ko] = 0;
+    a.L.dA[kn] = 0;
+    a.L.headL[ko] = -1;
+    a.L.headA[kn] = -1;
+  }
+  for (int k = gt; k < a.nchunks; k += gs) a.L.chunkD[k] = 0;
+  if (gt == 0) a.L.flags[1] = 1;
+}
+
+// Stage 2.  INC: relayout of every slot into the other buffer, placement of
+// the recorded movers, their new element ids.  FULL: cell offsets (one
+// workgroup per chunk) and the queue of cells too big for one thread.
+__global__ __launch_bounds__(kThreads) void k_sort_s2(SortArgs a) {
+  const int mode = sort_mode(a);
+  if (mode == SORT_NONE) return;
+  __shared__ int sh[kThreads];
+  const int nbk = a.ncells + 1;
+  if (mode == SORT_FULL) {
+    if ((int)blockIdx.x >= a.nchunks) return;
+    const int* __restrict__ counts = a.counts;
+    chunk_scan(
+        nbk, a.nchunks, a.L.chunk_shift, a.chunkC, [&](int c) { return counts[c]; },
+        [](int) { return 0; }, a.cs0, a.n, sh);
+    const int c0 = blockIdx.x << a.L.chunk_shift;
+    for (int c = c0 + threadIdx.x; c < min(c0 + (1 << a.L.chunk_shift), nbk); c += kThreads)
+      if (counts[c] > kSmallCell) a.big[atomicAdd(&a.L.flags[3], 1)] = c;
+    return;
+  }
+  const int cur = a.L.flags[4];
+  const int* __restrict__ po = cur ? a.perm1 : a.perm0;
+  int* __restrict__ pn = cur ? a.perm0 : a.perm1;
+  const int* __restrict__ co = cur ? a.cs1 : a.cs0;
+  const int* __restrict__ cn = cur ? a.cs0 : a.cs1;
+  const int* __restrict__ M = a.L.mov;
+  const int* __restrict__ dL = a.L.dL;
+  const int* __restrict__ dA = a.L.dA;
+  const int gt = blockIdx.x * kThreads + threadIdx.x, gs = gridDim.x * kThreads;
+  for (int s = gt; s < a.n; s += gs) {
+    const int p = po[s];
+    const int c = a.key[p];
+    int pos;
+    if ((dL[c] | dA[c]) == 0) {
+      pos = s + (cn[c] - co[c]);
+    } else {
+      if (a.L.mrec[p]) continue;  // placed by the mover loop below
+      int before = 0, arr = 0;
+      for (int m = a.L.headL[c]; m >= 0; m = M[kMovRec * m + 3])
+        before += M[kMovRec * m] < p;  // old segment ascends in particle index
+      for (int m = a.L.headA[c]; m >= 0; m = M[kMovRec * m + 4]) arr += M[kMovRec * m] < p;
+      pos = cn[c] + (s - co[c]) - before + arr;
     }
-    grid.sync();
-    // new offsets: cn = co + exclusive scan of (arrivals - leavers)
-    const int chunk = (nbk + G - 1) / G;
-    const int lo = min(b * chunk, nbk), hi = min(lo + chunk, nbk);
-    {
-      int sm = 0;
-      for (int c = lo + tid; c < hi; c += kThreads) sm += dA[c] - dL[c];
-      int tot;
-      blk_excl_scan(sm, sh, tot);
-      if (tid == 0) bs[b] = tot;
+    pn[pos] = p;
+  }
+  const int nmov = a.L.flags[5];
+  for (int m = gt; m < nmov; m += gs) {
+    const int p = M[kMovRec * m], e = M[kMovRec * m + 2];
+    const int kn = key_of(e, a.ncells);
+    int l = co[kn], h = co[kn + 1];
+    const int first = l;
+    while (l < h) {
+      const int mid = (l + h) >> 1;
+      if (po[mid] < p) l = mid + 1;
+      else h = mid;
     }
-    grid.sync();
-    if (b == 0) {
-      int carry = 0;
-      for (int base = 0; base < G; base += kThreads) {
-        const int k = base + tid;
-        const int v = k < G ? bs[k] : 0;
-        int tot;
-        const int ex = blk_excl_scan(v, sh, tot);
-        if (k < G) bs[k] = ex + carry;
-        carry += tot;
-      }
-    }
-    grid.sync();
-    chunk_scan(lo, hi, bs[b], sh, [&](int c) { return dA[c] - dL[c]; }, co, cn);
-    if (gt == 0) cn[nbk] = n;
-    grid.sync();
-    for (int s = gt; s < n; s += gs) {
-      if (mflag[s]) continue;
-      const int p = po[s];
-      const int c = key[p];
-      int pos;
-      if (dL[c] == 0 && dA[c] == 0) {
-        pos = s + (cn[c] - co[c]);
-      } else {
-        int before = 0, arr = 0;
-        for (int m = headL[c]; m >= 0; m = M[kMovRec * m + 4]) before += M[kMovRec * m + 3] < s;
-        for (int m = headA[c]; m >= 0; m = M[kMovRec * m + 5]) arr += M[kMovRec * m] < p;
-        pos = cn[c] + (s - co[c]) - before + arr;
-      }
-      pn[pos] = p;
-      iperm[p] = pos;
-    }
-    for (int m = gt; m < nmov; m += gs) {
-      const int p = M[kMovRec * m];
-      const int kn = key_of(M[kMovRec * m + 2]);
-      int l = co[kn], h = co[kn + 1];
-      const int first = l;
-      while (l < h) {
-        const int mid = (l + h) >> 1;
-        if (po[mid] < p) l = mid + 1;
-        else h = mid;
-      }
-      int lv = 0, arr = 0;
-      for (int q = headL[kn]; q >= 0; q = M[kMovRec * q + 4]) lv += M[kMovRec * q] < p;
-      for (int q = headA[kn]; q >= 0; q = M[kMovRec * q + 5]) arr += M[kMovRec * q] < p;
-      const int pos = cn[kn] + (l - first) - lv + arr;
-      pn[pos] = p;
-      iperm[p] = pos;
-    }
-    grid.sync();
-    for (int m = gt; m < nmov; m += gs) {
-      const int p = M[kMovRec * m], ko = M[kMovRec * m + 1], e = M[kMovRec * m + 2];
-      const int kn = key_of(e);
-      eid[p] = e;
-      key[p] = kn;
-      mflag[M[kMovRec * m + 3]] = 0;
-      dL[ko] = 0;
-      dA[kn] = 0;
-      headL[ko] = -1;
-      headA[kn] = -1;
-    }
-    if (gt == 0) {
-      flags[4] = 1 - cur;
-      flags[5] = 0;
-      flags[1] = 0;
+    int lv = 0, arr = 0;
+    for (int q = a.L.headL[kn]; q >= 0; q = M[kMovRec * q + 3]) lv += M[kMovRec * q] < p;
+    for (int q = a.L.headA[kn]; q >= 0; q = M[kMovRec * q + 4]) arr += M[kMovRec * q] < p;
+    pn[cn[kn] + (l - first) - lv + arr] = p;
+    a.eid[p] = e;
+  }
+}
+
+// Stage 3.  INC: new keys of the movers, release of the change lists.
+// FULL: scatter into buffer 0.
+__global__ __launch_bounds__(kThreads) void k_sort_s3(SortArgs a) {
+  const int mode = sort_mode(a);
+  if (mode == SORT_NONE) return;
+  const int gt = blockIdx.x * kThreads + threadIdx.x, gs = gridDim.x * kThreads;
+  if (mode == SORT_FULL) {
+    for (int p = gt; p < a.n; p += gs) {
+      const int k = a.key[p];
+      a.perm0[a.cs0[k] + atomicAdd(&a.fill[k], 1)] = p;
     }
     return;
   }
+  const int nmov = a.L.flags[5];
+  for (int m = gt; m < nmov; m += gs) {
+    const int* r = a.L.mov + (int64_t)kMovRec * m;
+    const int p = r[0], ko = r[1], kn = key_of(r[2], a.ncells);
+    a.key[p] = kn;
+    a.L.mrec[p] = 0;
+    a.L.dL[ko] = 0;
+    a.L.dA[kn] = 0;
+    a.L.headL[ko] = -1;
+    a.L.headA[kn] = -1;
+    a.L.chunkD[ko >> a.L.chunk_shift] = 0;
+    a.L.chunkD[kn >> a.L.chunk_shift] = 0;
+  }
+}
 
-  // ---- full counting sort into buffer 0 ----
-  int* __restrict__ perm = perm0;
-  int* __restrict__ cs = cs0;
-  for (int c = gt; c < nbk; c += gs) {
-    counts[c] = 0;
-    fill[c] = 0;
-  }
-  if (gt == 0) {
-    flags[3] = 0;
-    flags[1] = 1;
-  }
-  grid.sync();
-  if (gt == 0) flags[2] = 0;  // after the barrier: every block has read it
-  for (int p = gt; p < n; p += gs) {
-    const int e = element_id<T, 2>(g.geo(), PR(DMPM_P_X)[p], PR(DMPM_P_X + 1)[p]);
-    eid[p] = e;
-    const int k = (e >= 0 && e < g.ncells) ? e : g.ncells;
-    key[p] = k;
-    atomicAdd(&counts[k], 1);
-  }
-  grid.sync();
-  const int chunk = (nbk + G - 1) / G;
-  const int lo = min(b * chunk, nbk), hi = min(lo + chunk, nbk);
-  {
-    int sm = 0;
-    for (int c = lo + tid; c < hi; c += kThreads) sm += counts[c];
-    int tot;
-    blk_excl_scan(sm, sh, tot);
-    if (tid == 0) bs[b] = tot;
-  }
-  grid.sync();
-  if (b == 0) {
-    int carry = 0;
-    for (int base = 0; base < G; base += kThreads) {
-      const int k = base + tid;
-      const int v = k < G ? bs[k] : 0;
-      int tot;
-      const int ex = blk_excl_scan(v, sh, tot);
-      if (k < G) bs[k] = ex + carry;
-      carry += tot;
+// Stage 4.  INC: the first workgroup switches buffers and clears the change
+// count (any other workgroup then reads NONE, and INC leaves it no work).
+// FULL: ascending particle index within each cell — cells of up to
+// kSmallCell particles by one thread (insertion sort), bigger ones by one
+// workgroup (rank sort in LDS up to kBigCellMax; bigger cells keep the
+// scatter order and mark the layout unsorted); restores the zero counters;
+// the last workgroup publishes the new state.
+__global__ __launch_bounds__(kThreads) void k_sort_s4(SortArgs a) {
+  const int mode = sort_mode(a);
+  if (mode == SORT_NONE) return;
+  if (mode == SORT_INCREMENTAL) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      int* f = a.L.flags;
+      f[4] = 1 - f[4];
+      f[5] = 0;
+      f[1] = 0;
     }
+    return;
   }
-  grid.sync();
-  {
-    int carry = bs[b];
-    for (int base = lo; base < hi; base += kThreads) {
-      const int c = base + tid;
-      const int v = c < hi ? counts[c] : 0;
-      int tot;
-      const int ex = blk_excl_scan(v, sh, tot);
-      if (c < hi) cs[c] = carry + ex;
-      carry += tot;
-    }
-    if (gt == 0) cs[nbk] = n;
-  }
-  grid.sync();
-  for (int p = gt; p < n; p += gs) {
-    const int k = key[p];
-    perm[cs[k] + atomicAdd(&fill[k], 1)] = p;
-  }
-  grid.sync();
-  for (int c = gt; c <= g.ncells; c += gs) {  // cells and the outside bucket
-    const int s0 = cs[c], s1 = cs[c + 1];
-    if (s1 - s0 > kSmallCell) {
-      big[atomicAdd(&flags[3], 1)] = c;
+  __shared__ int sh[kBigCellMax];
+  int* __restrict__ perm = a.perm0;
+  const int* __restrict__ cs = a.cs0;
+  const int nbig = a.L.flags[3];
+  for (int q = blockIdx.x; q < nbig; q += gridDim.x) {
+    const int c = a.big[q];
+    const int s0 = cs[c], m = cs[c + 1] - s0;
+    if (m > kBigCellMax) {
+      if (threadIdx.x == 0) a.L.flags[7] = 1;
+      if (threadIdx.x == 0) a.L.flags[1] = 0;
       continue;
     }
+    for (int i = threadIdx.x; i < m; i += kThreads) sh[i] = perm[s0 + i];
+    __syncthreads();
+    for (int i = threadIdx.x; i < m; i += kThreads) {
+      const int v = sh[i];
+      int rk = 0;
+      for (int j = 0; j < m; ++j) rk += sh[j] < v;
+      perm[s0 + rk] = v;
+      if (v != s0 + rk) a.L.flags[1] = 0;
+    }
+    __syncthreads();
+  }
+  const int gt = blockIdx.x * kThreads + threadIdx.x, gs = gridDim.x * kThreads;
+  for (int c = gt; c <= a.ncells; c += gs) {  // cells and the outside bucket
+    a.counts[c] = 0;
+    a.fill[c] = 0;
+    const int s0 = cs[c], s1 = cs[c + 1];
+    if (s1 - s0 > kSmallCell) continue;
+    bool ident = true;
     for (int i = s0 + 1; i < s1; ++i) {
       const int v = perm[i];
       int j = i - 1;
@@ -873,39 +968,22 @@ __global__ __launch_bounds__(kThreads) void k_sort_coop(
       }
       perm[j + 1] = v;
     }
+    for (int i = s0; i < s1; ++i) ident = ident && perm[i] == i;
+    if (!ident) a.L.flags[1] = 0;
   }
-  grid.sync();
-  const int nbig = flags[3];
-  for (int q = b; q < nbig; q += G) {
-    const int c = big[q];
-    const int s0 = cs[c], m = cs[c + 1] - s0;
-    if (m > kBigCellMax) {
-      if (tid == 0) flags[2] = 1;
-      continue;
+  if (last_block(&a.L.flags[8])) {
+    for (int k = threadIdx.x; k < a.nchunks; k += kThreads) a.chunkC[k] = 0;
+    if (threadIdx.x == 0) {
+      int* f = a.L.flags;
+      f[2] = f[7];
+      f[7] = 0;
+      f[3] = 0;
+      f[4] = 0;
+      f[5] = 0;
+      f[6] = 0;
+      f[10] = 0;
+      f[0] = 0;
     }
-    for (int i = tid; i < m; i += kThreads) sh[i] = perm[s0 + i];
-    __syncthreads();
-    for (int i = tid; i < m; i += kThreads) {
-      const int v = sh[i];
-      int rk = 0;
-      for (int j = 0; j < m; ++j) rk += sh[j] < v;
-      perm[s0 + rk] = v;
-    }
-    __syncthreads();
-  }
-  grid.sync();
-  int ident = 1;
-  for (int s = gt; s < n; s += gs) {
-    iperm[perm[s]] = s;
-    if (perm[s] != s) ident = 0;
-  }
-  if (!ident) flags[1] = 0;
-  grid.sync();
-  if (gt == 0) {
-    flags[4] = 0;
-    flags[5] = 0;
-    flags[6] = 0;
-    flags[0] = 0;
   }
 }
 
@@ -927,20 +1005,37 @@ template <typename T> Grid2<T> grid_of(const dmpm_plan* P) {
 
 inline unsigned nblk(int64_t n) { return (unsigned)((n + kThreads - 1) / kThreads); }
 
-int coop_grid_size(const void* kernel, int& out) {
-  int dev = 0, ncu = 0, per = 0;
-  DMPM_TRY_H(hipGetDevice(&dev));
-  DMPM_TRY_H(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-  DMPM_TRY_H(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, kThreads, 0));
-  if (per < 1) return fail(DMPM_EHIP, "cooperative sort kernel cannot be resident");
-  out = std::min(ncu * std::min(per, 4), 4096);
-  return DMPM_OK;
+constexpr int kSortGrid = 2048;  // workgroups of the grid-stride re-sort stages
+
+template <typename T>
+int enqueue_resort(dmpm_plan* P, const Grid2<T>& g, const T* pb, int32_t* eid, hipStream_t s) {
+  SortArgs a;
+  a.eid = eid;
+  a.key = P->key;
+  a.counts = P->counts;
+  a.fill = P->fill;
+  a.perm0 = P->perm;
+  a.perm1 = P->perm2;
+  a.cs0 = P->cs;
+  a.cs1 = P->cs2;
+  a.big = P->big;
+  a.chunkC = P->chunkC;
+  a.L = move_log(P);
+  a.n = (int)P->n;
+  a.ncells = P->ncells;
+  a.nchunks = P->nchunks;
+  const int g12 = std::max(P->nchunks, kSortGrid);
+  k_sort_s1<T><<<g12, kThreads, 0, s>>>(g, pb, P->d.ld, a);
+  k_sort_s2<<<g12, kThreads, 0, s>>>(a);
+  k_sort_s3<<<kSortGrid, kThreads, 0, s>>>(a);
+  k_sort_s4<<<kSortGrid / 4, kThreads, 0, s>>>(a);
+  return launch_check("resort");
 }
 
 template <typename T, int TXN>
 int tile_steps(dmpm_plan* P, const Grid2<T>& g, T* pb, int32_t* eid, T* nb, int scheme, int vu,
                const double* grav, double dt, int64_t step0, int64_t nsteps, int temps,
-               hipStream_t s, T* alt, T* vkin, int* bs, int G) {
+               hipStream_t s, T* alt, T* vkin) {
   const int64_t ld = P->d.ld, nld = P->d.nld;
   const int n = (int)P->n;
   const bool usf = scheme == DMPM_USF;
@@ -977,15 +1072,7 @@ int tile_steps(dmpm_plan* P, const Grid2<T>& g, T* pb, int32_t* eid, T* nb, int 
   for (int64_t step = step0; step < step0 + nsteps; ++step) {
     {
       ProfScopeT ps_(P, KC_SORT, s);
-      void* args[] = {(void*)&g,        (void*)&pb,       (void*)&ld,        (void*)&n,
-                      (void*)&eid,      (void*)&P->key,   (void*)&P->counts, (void*)&P->fill,
-                      (void*)&P->perm,  (void*)&P->perm2, (void*)&P->cs,     (void*)&P->cs2,
-                      (void*)&P->iperm, (void*)&bs,       (void*)&P->flags,  (void*)&P->big,
-                      (void*)&P->mov,   (void*)&P->movcap, (void*)&P->dL,    (void*)&P->dA,
-                      (void*)&P->headL, (void*)&P->headA, (void*)&P->mflag};
-      DMPM_TRY(hip_check(hipLaunchCooperativeKernel((const void*)k_sort_coop<T>, dim3(G),
-                                                    dim3(kThreads), args, 0, s),
-                         "hipLaunchCooperativeKernel"));
+      DMPM_TRY(enqueue_resort<T>(P, g, pb, eid, s));
     }
     const DB<T>& db = parity ? cur_is_alt : cur_is_pb;
     const T t = T(step) * tdt;
@@ -1016,12 +1103,11 @@ int tile_steps(dmpm_plan* P, const Grid2<T>& g, T* pb, int32_t* eid, T* nb, int 
     {
       ProfScopeT ps_(P, KC_TILE_G2P, s);
       if (usf)
-        k_g2p_tile<T, false><<<nblk(n), kThreads, 0, s>>>(g, pb, ld, n, nb, nld, P->flags, P->mov,
-                                                          P->movcap, db, mat, tdt, vu, temps,
-                                                          split ? 0 : 1);
+        k_g2p_tile<T, false><<<nblk(n), kThreads, 0, s>>>(g, pb, ld, n, nb, nld, move_log(P), db,
+                                                          mat, tdt, vu, temps, split ? 0 : 1);
       else
-        k_g2p_tile<T, true><<<nblk(n), kThreads, 0, s>>>(g, pb, ld, n, nb, nld, P->flags, P->mov,
-                                                         P->movcap, db, mat, tdt, vu, temps, 0);
+        k_g2p_tile<T, true><<<nblk(n), kThreads, 0, s>>>(g, pb, ld, n, nb, nld, move_log(P), db,
+                                                         mat, tdt, vu, temps, 0);
       DMPM_TRY(launch_check("tile_g2p"));
     }
     if (usf && !split) parity ^= 1;
@@ -1046,17 +1132,14 @@ int tile_run_t(dmpm_plan* P, T* pb, int32_t* eid, T* nb, int scheme, int vu, con
   T* alt = dbuf ? (T*)plan_workspace(P, 0, (size_t)kDB * ld * sizeof(T)) : nullptr;
   T* vkin = (T*)plan_workspace(P, 1, (size_t)2 * P->nnodes * sizeof(T));
   if ((dbuf && !alt) || !vkin) return fail(DMPM_EHIP, "workspace allocation failed");
-  int G = 0;
-  DMPM_TRY(coop_grid_size((const void*)k_sort_coop<T>, G));
-  int* bs = (int*)plan_workspace(P, 2, (size_t)G * sizeof(int));
-  if (!bs) return fail(DMPM_EHIP, "workspace allocation failed");
-  k_tile_check<T><<<nblk(n), kThreads, 0, s>>>(g, pb, ld, n, eid, P->key, P->flags);
+  k_tile_check<T><<<nblk(n), kThreads, 0, s>>>(g, pb, ld, n, eid, P->key, P->flags, P->mrec,
+                                               P->mov);
   DMPM_TRY(launch_check("tile_check"));
   if (P->tile_tx == 64)
     return tile_steps<T, 64>(P, g, pb, eid, nb, scheme, vu, grav, dt, step0, nsteps, temps, s,
-                             alt, vkin, bs, G);
+                             alt, vkin);
   return tile_steps<T, 128>(P, g, pb, eid, nb, scheme, vu, grav, dt, step0, nsteps, temps, s,
-                            alt, vkin, bs, G);
+                            alt, vkin);
 }
 
 }  // namespace
@@ -1064,7 +1147,7 @@ int tile_run_t(dmpm_plan* P, T* pb, int32_t* eid, T* nb, int scheme, int vu, con
 namespace dmpm {
 
 bool tile_eligible(const dmpm_plan* P, int pfext) {
-  return P->dim == 2 && !P->has_tractions && !pfext && P->n > 0;
+  return P->dim == 2 && !P->has_tractions && !pfext && P->n > 0 && P->nchunks <= kMaxChunks;
 }
 
 int tile_run(dmpm_plan* P, void* pbuf, int32_t* eid, void* nbuf, int scheme, int vu,

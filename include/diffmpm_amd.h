@@ -228,15 +228,20 @@ enum {
   DMPM_KC_LOCATE = 0, DMPM_KC_SORT = 1, DMPM_KC_P2G_KIN = 2, DMPM_KC_STRESS = 3,
   DMPM_KC_P2G_FORCE = 4, DMPM_KC_GRID = 5, DMPM_KC_G2P = 6, DMPM_KC_SMALL = 7,
   DMPM_KC_TILE_KIN = 8, DMPM_KC_TILE_FORCE = 9, DMPM_KC_TILE_G2P = 10,
-  DMPM_KC_ADJ = 11, DMPM_KC_BATCH = 12
+  DMPM_KC_ADJ = 11, DMPM_KC_BATCH = 12, DMPM_KC_ALL = -2
 };
-/* Times every launch of one kernel class with HIP event pairs recorded on the
- * launch stream (kernel_class < 0 disables).  A ring of 4 pairs is reused,
- * so the host waits for the timed launch 4 launches back before recording a
- * new pair.  max_launches is ignored (kept for ABI stability). */
+/* Times every launch of one kernel class (DMPM_KC_ALL: of every class) with
+ * HIP event pairs recorded on the launch stream; kernel_class = -1 disables.
+ * A ring of 4 pairs per class is reused, so the host waits for the timed
+ * launch of that class 4 launches back before recording a new pair.
+ * max_launches is ignored (kept for ABI stability). */
 int dmpm_timing_enable(dmpm_plan* plan, int32_t kernel_class, int32_t max_launches);
-/* Waits for the recorded events; returns the summed duration and count. */
+/* Waits for the recorded events; returns the summed duration and count of the
+ * enabled class (of all classes after DMPM_KC_ALL). */
 int dmpm_timing_read(dmpm_plan* plan, double* total_ms, int64_t* launches);
+/* The same for one class. */
+int dmpm_timing_read_class(dmpm_plan* plan, int32_t kernel_class, double* total_ms,
+                           int64_t* launches);
 
 #ifdef __cplusplus
 }
