@@ -24,20 +24,21 @@ HBM_PEAK_GBS = 8000.0
 KERNEL_BYTES = {
     # sort by cell: sort keys and permutation only (no persistent particle field)
     "sort": 0,
-    # P2G mass/momentum + nodal velocity + constraints: read x(2), v(2), m
+    # P2G of mass and momentum (cell + node pass), nodal velocity, constraints:
+    # read x(2), v(2), m
     "tile_kin": 4 * (2 + 2 + 1),
-    # USF strain/volume/stress update from the kinematic nodal velocities:
-    # read x(2), eps(3), sigma(4), V, rho, eps_v; write eps(3), sigma(4), V, rho, eps_v
+    # USF: strain/volume/stress update from the kinematic nodal
+    # velocities: read x(2), eps(3), sigma(4), V, rho, eps_v; write eps(3),
+    # sigma(4), V, rho, eps_v
     "stress": 4 * (2 + 3 + 4 + 3) + 4 * (3 + 4 + 3),
-    # P2G of body and internal forces, nodal forces, nodal update, constraints:
-    # read x(2), m, V, sigma(xx, yy, xy)
+    # P2G of body and internal forces (cell + node pass), nodal
+    # forces, nodal update, constraints: read x(2), m, V, sigma(xx, yy, xy)
     "tile_force": 4 * (2 + 1 + 1 + 3),
     # G2P: read x(2), v(2); write x(2), v(2)
     "tile_g2p": 4 * 4 + 4 * 4,
 }
-# --usf-fused: the force kernel also does the stress update (the stress class
-# is then not launched): read x(2), m, eps(3), sigma(4), V, rho, eps_v; write
-# eps(3), sigma(4), V, rho, eps_v
+# --usf-fused: the force cell pass also does the stress update in place: read
+# x(2), m, eps(3), sigma(4), V, rho, eps_v; write eps(3), sigma(4), V, rho, eps_v
 TILE_FORCE_FUSED_BYTES = 4 * (2 + 1 + 3 + 4 + 3) + 4 * (3 + 4 + 3)
 STEP_BYTES = 116  # per particle-step, whole step (FP32 2D LinearElastic)
 # HBM bytes per particle per launch of the default path at 1024x1024, from
@@ -89,10 +90,10 @@ def main():
     ap.add_argument("--n", type=int, default=1024)
     ap.add_argument("--kernel", default="auto",
                     help="kernel class of the roofline (auto: the one with the most time)")
-    ap.add_argument("--tile", type=int, default=0, help="node rows per tile (0: default)")
-    ap.add_argument("--tile-cols", type=int, default=0, help="node columns per tile (64/128)")
+    ap.add_argument("--cells-per-block", type=int, default=0,
+                    help="cells per workgroup of the P2G cell pass (64/128/256; 0: default)")
     ap.add_argument("--usf-fused", action="store_true",
-                    help="update stresses inside the force tile kernel")
+                    help="update stresses inside the force P2G cell pass")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="after the timed region, copy 512 MiB with a plain device copy "
@@ -111,8 +112,8 @@ def main():
     ps = sim.mesh.particles[0]
     plan = plan_for(sim.mesh.elements, ps)
     N = len(ps)
-    if args.tile or args.tile_cols:
-        plan.set_tile(args.tile, args.tile_cols)
+    if args.cells_per_block:
+        plan.set_block(args.cells_per_block)
     from diffmpm_amd import _lib
     if args.usf_fused:
         plan.set_option(_lib.DMPM_OPT_USF_SPLIT, 0)

@@ -60,7 +60,7 @@ SIGNATURES = {
     "dmpm_plan_set_nodal_forces": [_P, c_int32, _I32P, _I32P, _I32P, _DP],
     "dmpm_plan_set_tractions": [_P, c_int32, _I32P, _I32P, _I32P, _DP],
     "dmpm_mark_unsorted": [_P, _P],
-    "dmpm_plan_set_tile": [_P, c_int32, c_int32],
+    "dmpm_plan_set_block": [_P, c_int32],
     "dmpm_plan_set_option": [_P, c_int32, c_int32],
     "dmpm_set_particle_element_ids": [_P, _P, _P, _P],
     "dmpm_update_natural_coords": [_P, _P, _P, _P],

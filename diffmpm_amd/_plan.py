@@ -226,8 +226,8 @@ class Plan:
     def set_option(self, option, value):
         call("dmpm_plan_set_option", self.h, int(option), int(value))
 
-    def set_tile(self, node_rows, node_cols=0):
-        call("dmpm_plan_set_tile", self.h, int(node_rows), int(node_cols))
+    def set_block(self, cells_per_block):
+        call("dmpm_plan_set_block", self.h, int(cells_per_block))
 
     def run(self, scheme, velocity_update, gravity, dt, step0, nsteps, keep_temporaries=True,
             per_phase=False):

@@ -610,20 +610,24 @@ __global__ void k_p2g_kin(Geo<T> g, const T* __restrict__ pb, int64_t ld, SortRe
   CellList<DIM> L = node_cells<T, DIM>(g, nd);
   T am = T(0), apx = T(0), apy = T(0);
   if (mode & (K_MASS | K_MOM)) {
-    for (int c = 0; c < L.count; ++c) {
+    for (int c = 0; c < L.count; ++c) {  // per-cell partial sums, then the node sum
       int b = cs[L.cell[c]], e = cs[L.cell[c] + 1];
+      T cm = T(0), cpx = T(0), cpy = T(0);
       for (int s = b; s < e; ++s) {
         int p = perm[s];
         T xi = PROW(DMPM_P_XI)[p];
         T eta = DIM == 2 ? PROW(DMPM_P_XI + 1)[p] : T(0);
         T N = corner_N<T, DIM>(L.corner[c], xi, eta);
         T m = PROW(DMPM_P_MASS)[p];
-        if (mode & K_MASS) am = am + m * N;  // element.py:140
+        if (mode & K_MASS) cm = cm + m * N;  // element.py:140
         if (mode & K_MOM) {                   // element.py:172 with pmom = m*v
-          apx = apx + N * (m * PROW(DMPM_P_V)[p]);
-          if (DIM == 2) apy = apy + N * (m * PROW(DMPM_P_V + 1)[p]);
+          cpx = cpx + N * (m * PROW(DMPM_P_V)[p]);
+          if (DIM == 2) cpy = cpy + N * (m * PROW(DMPM_P_V + 1)[p]);
         }
       }
+      am = am + cm;
+      apx = apx + cpx;
+      apy = apy + cpy;
     }
   }
   if (mode & K_MASS) NROW(DMPM_N_MASS)[nd] = am;
@@ -664,6 +668,7 @@ __global__ void k_p2g_force(Geo<T> g, const T* __restrict__ pb, int64_t ld, Sort
       if (!(mode & bit)) continue;
       for (int c = 0; c < L.count; ++c) {
         int b = cs[L.cell[c]], e = cs[L.cell[c] + 1];
+        T cx = T(0), cy = T(0);
         for (int s = b; s < e; ++s) {
           int p = perm[s];
           T xi = PROW(DMPM_P_XI)[p];
@@ -681,9 +686,11 @@ __global__ void k_p2g_force(Geo<T> g, const T* __restrict__ pb, int64_t ld, Sort
             qx = PROW(DMPM_P_TRACTION)[p];
             if (DIM == 2) qy = PROW(DMPM_P_TRACTION + 1)[p];
           }
-          fx = fx + N * qx;
-          if (DIM == 2) fy = fy + N * qy;
+          cx = cx + N * qx;
+          if (DIM == 2) cy = cy + N * qy;
         }
+        fx = fx + cx;
+        if (DIM == 2) fy = fy + cy;
       }
     }
     NROW(DMPM_N_FEXT)[nd] = fx;
@@ -696,6 +703,7 @@ __global__ void k_p2g_force(Geo<T> g, const T* __restrict__ pb, int64_t ld, Sort
       int b = cs[cell], e = cs[cell + 1];
       int I = DIM == 1 ? cell : cell % g.nx;
       int J = DIM == 1 ? 0 : cell / g.nx;
+      T cx = T(0), cy = T(0);
       for (int s = b; s < e; ++s) {
         int p = perm[s];
         T xi = PROW(DMPM_P_XI)[p];
@@ -703,7 +711,7 @@ __global__ void k_p2g_force(Geo<T> g, const T* __restrict__ pb, int64_t ld, Sort
         T sxx = PROW(DMPM_P_STRESS + 0)[p];
         if (DIM == 1) {
           T dN = l1_dNdx<T>(L.corner[c], l1_inv(g.X(I), g.X(I + 1)));
-          fx = fx + (nV * sxx) * dN;
+          cx = cx + (nV * sxx) * dN;
         } else {
           T eta = PROW(DMPM_P_XI + 1)[p];
           T ix, iy, dx, dy;
@@ -711,10 +719,12 @@ __global__ void k_p2g_force(Geo<T> g, const T* __restrict__ pb, int64_t ld, Sort
           q4_dNdx(L.corner[c], xi, eta, ix, iy, dx, dy);
           T syy = PROW(DMPM_P_STRESS + 1)[p];
           T sxy = PROW(DMPM_P_STRESS + 3)[p];
-          fx = fx + nV * (dx * sxx + dy * sxy);
-          fy = fy + nV * (dy * syy + dx * sxy);
+          cx = cx + nV * (dx * sxx + dy * sxy);
+          cy = cy + nV * (dy * syy + dx * sxy);
         }
       }
+      fx = fx + cx;
+      fy = fy + cy;
     }
     NROW(DMPM_N_FINT)[nd] = fx;
     if (DIM == 2) NROW(DMPM_N_FINT + 1)[nd] = fy;
@@ -1149,12 +1159,12 @@ int dmpm_plan_set_tractions(dmpm_plan* P, int32_t n, const int32_t* pids, const 
   return DMPM_OK;
 }
 
-int dmpm_plan_set_tile(dmpm_plan* P, int32_t node_rows, int32_t node_cols) {
-  if (!P || node_rows < 0 || node_rows > 4096) return fail(DMPM_EINVAL, "invalid tile size");
-  if (node_cols != 0 && node_cols != 64 && node_cols != 128)
-    return fail(DMPM_EINVAL, "tile columns must be 64 or 128 (0: keep)");
-  P->tile_ty = node_rows;
-  if (node_cols) P->tile_tx = node_cols;
+int dmpm_plan_set_block(dmpm_plan* P, int32_t cells_per_block) {
+  if (!P) return fail(DMPM_EINVAL, "null plan");
+  if (cells_per_block != 0 && cells_per_block != 64 && cells_per_block != 128 &&
+      cells_per_block != 256)
+    return fail(DMPM_EINVAL, "cells per block must be 64, 128 or 256 (0: default)");
+  P->cell_block = cells_per_block ? cells_per_block : 128;
   return DMPM_OK;
 }
 

@@ -83,9 +83,8 @@ struct dmpm_plan {
   };
   ProfClass prof[16];
   int prof_single = -1;  // the class dmpm_timing_read reports (-2: all enabled)
-  int tile_ty = 0;  // node rows per tile of the fused 2D kernels (0: default)
-  int tile_tx = 64; // node columns per tile (64 or 128)
-  int usf_split = 1; // USF: streamed stress kernel + force tile (1) or fused (0)
+  int cell_block = 128;  // cells per workgroup of the P2G cell pass (64/128/256)
+  int usf_split = 1;     // USF: stress update streamed (1) or in the force cell pass (0)
   // workspace of the fused tile kernels and the adjoint (allocated lazily)
   void* extra[16] = {nullptr};
   size_t extra_bytes[16] = {0};

@@ -1,28 +1,24 @@
-// Fused 2D explicit step on node tiles (gfx950).
+// Fused 2D explicit step (gfx950): re-sort by cell, P2G as a cell pass and a
+// node pass, G2P.
 //
-// One step is a re-sort and three (split USF: four) step launches:
-//   k_sort_s1..s4   sort by cell: nothing when no particle changed cell, an
-//                   O(N) update of the stable order from the recorded cell
-//                   changes, a full counting sort when the layout is unknown
-//   k_tile<KIN>     P2G of mass and momentum, nodal velocity, constraints
-//   k_tile<FORCE_*> (USF: strain/volume/stress update of the tile's particles)
-//                   P2G of body and internal forces, nodal forces, nodal
-//                   acceleration/velocity update, constraints
-//   k_g2p_*         G2P velocity/position (USL: + strain/volume/stress update),
-//                   flags cell changes for the next sort
+// One step is:
+//   k_sort_s1..s4          sort by cell: nothing when no particle changed
+//                          cell, an O(N) update of the stable order from the
+//                          recorded cell changes, a full counting sort when
+//                          the layout is unknown
+//   k_cell_kin/k_node_kin  P2G of mass and momentum, nodal velocity,
+//                          constraints
+//   k_stress_tile          USF strain/volume/stress update, streamed (or, as
+//                          an option, inside k_cell_force)
+//   k_cell_force/k_node_force  P2G of body and internal forces, nodal forces,
+//                          nodal acceleration/velocity update, constraints
+//   k_g2p_tile             G2P velocity/position (USL: + strain/volume/stress
+//                          update), records cell changes for the next sort
 //
-// A workgroup owns a tile of kTX x ty nodes and streams the particles of the
-// cells around it (cell rows j0-1 .. j0+ty-1, columns i0-1 .. i0+kTX-1) row by
-// row through LDS.  Node (i, j) sums its contributions in the same order as
-// the per-phase kernels — cells (i-1,j-1), (i,j-1), (i-1,j), (i,j), ascending
-// particle index within a cell — with the same per-term arithmetic, so both
-// paths produce bit-identical nodal sums, without atomics.
-//
-// Particles in the one-cell halo are read by two tiles.  In USF the force
-// kernel updates the stress of the particles it owns; so that a neighbour
-// still reads the step-start values of shared halo particles, the ten updated
-// fields are double-buffered between the caller's block and a plan workspace,
-// and the kinematic nodal velocity lives in its own workspace.
+// Nodal sums are formed without atomics and in a fixed order (per-cell sums
+// in ascending particle index, then the four cells of a node in the order
+// (i-1,j-1), (i,j-1), (i-1,j), (i,j)); the per-phase kernels use the same
+// order and the same per-term arithmetic, so both paths are bit-identical.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -35,38 +31,13 @@ using namespace dmpm;
 namespace {
 
 constexpr int kThreads = 256;  // sort / G2P workgroups
-constexpr int kDB = 10;       // double-buffered particle fields (USF)
 constexpr int kSmallCell = 64;
 constexpr int kBigCellMax = 8192;
-
-// LDS particle capacity per chunk: ~4 rows of particles per node column.
-template <typename T, int TXN> struct Chunk {
-  static constexpr int value = (sizeof(T) == 4 ? 8 : 4) * TXN;
-};
-
-// eps_xx, eps_yy, gamma_xy, sig_xx, sig_yy, sig_zz, sig_xy, V, rho, eps_v
-__host__ __device__ constexpr int db_row(int k) {
-  return k < 2 ? DMPM_P_STRAIN + k
-         : k == 2 ? DMPM_P_STRAIN + 3
-         : k < 7 ? DMPM_P_STRESS + (k - 3)
-         : k == 7 ? DMPM_P_VOLUME
-         : k == 8 ? DMPM_P_DENSITY
-                  : DMPM_P_VOLSTRAIN_C;
-}
-
-template <typename T> struct DB {
-  T* cur[kDB];
-  T* nxt[kDB];
-};
 
 struct Mat2 {
   int kind;
   int per_E;
   double de[3];
-};
-
-struct TileShape {
-  int ntx, nty, ty;
 };
 
 template <typename T> struct Grid2 {
@@ -88,14 +59,6 @@ template <typename T> struct Grid2 {
     return g;
   }
 };
-
-// Workgroup -> tile such that consecutive tiles (which share halo cell rows)
-// land on the same XCD (workgroups are dealt to the 8 XCDs round-robin).
-__device__ __forceinline__ int tile_of_block(int b, int nb) {
-  const int x = b % 8, k = b / 8, q = nb / 8, rem = nb % 8;
-  const int base = x < rem ? x * (q + 1) : rem * (q + 1) + (x - rem) * q;
-  return base + k;
-}
 
 template <typename T>
 __device__ __forceinline__ void stress_inc(const Mat2& m, T E_p, const T d[6], T ds[4]) {
@@ -164,253 +127,346 @@ __device__ __forceinline__ void rate2s(T ix, T iy, T xi, T eta, const T vx[4], c
 #define PR(r) (pb + (int64_t)(r) * ld)
 #define NR(r) (nb + (int64_t)(r) * nld)
 
-enum { MODE_KIN = 0, MODE_FORCE_USF = 1, MODE_FORCE_USL = 2 };
+// ---- P2G: cell pass + node pass ------------------------------------------------
+//
+// Cell pass: a workgroup of kThreads threads owns CB consecutive cells and the
+// particles the sort put in them (slots cs[c0] .. cs[c0+CB], ~4 per cell).
+// The slots are loaded in rounds of kSlots (one round at <= kSlots/CB
+// particles per cell), kSlots/kThreads independent coalesced slot runs per
+// thread, all issued before any is used.  Each particle's inputs (natural
+// coordinates and the quantities it deposits) go to LDS; then thread t < CB
+// walks the slots of cell c0+t in order (ascending particle index), forms the
+// terms of the four corner nodes and adds them to running per-corner sums,
+// written to a plan workspace ("partials", one row per corner and quantity).
+// In USF the slot threads also update strain, volume and stress in place
+// from the kinematic nodal velocities of the cell, staged in LDS.
+//
+// Node pass: node (i, j) adds the partials of cells (i-1,j-1), (i,j-1),
+// (i-1,j), (i,j) in that order to a zero start and runs the nodal epilogue.
+// The per-phase kernels accumulate the same terms in the same order, so both
+// paths give bit-identical nodal sums without atomics.
+enum { NQ_KIN = 3, NQ_FORCE = 4 };  // quantities per corner
+constexpr int kSlots = 1024;        // slots per load round of a cell-pass workgroup
+constexpr int kPer = kSlots / kThreads;
 
-template <typename T, int MODE, int kTX>
-__global__ __launch_bounds__(2 * kTX) void k_tile(
-    Grid2<T> g, TileShape tl, T* __restrict__ pb, int64_t ld, SortRef sr,
-    T* __restrict__ nb, int64_t nld,
-    T* __restrict__ vkin, const int* __restrict__ cidx, const int* __restrict__ cmask,
-    const double* __restrict__ cval, DB<T> db, Mat2 mat, T gx, T gy, T dt, T t,
-    const int* __restrict__ nfd, const int* __restrict__ nf_off, const int* __restrict__ nf_fn,
-    const double* __restrict__ nf_val, FnTable fns, int temps) {
-  constexpr int CH = Chunk<T, kTX>::value;
-  constexpr int NT = 2 * kTX;
-  constexpr bool KIN = MODE == MODE_KIN;
-  constexpr bool USF = MODE == MODE_FORCE_USF;
-  constexpr int NQ = KIN ? 3 : 5;
-  constexpr int CHF = KIN ? 1 : CH;  // gradient scales are staged for force modes only
-  __shared__ T s_xi[CH], s_eta[CH], s_ix[CHF], s_q[NQ][CH];
-  __shared__ int s_cs[2][kTX + 2];
-  __shared__ T s_mx[kTX + 2], s_sx[kTX + 2];  // per cell column: (X0+X1)/2, 2/(X1-X0)
-  __shared__ T s_v[USF ? 2 : 1][2][2][kTX + 2];  // [buf][node row r|r+1][x|y][col]
+// Nodes of cell (I, J) in corner order ll, lr, ur, ul (element.py:692-723).
+__device__ __forceinline__ void cell_nodes(int nx, int I, int J, int ids[4]) {
+  const int ll = J * (nx + 1) + I;
+  ids[0] = ll;
+  ids[1] = ll + 1;
+  ids[2] = ll + nx + 2;
+  ids[3] = ll + nx + 1;
+}
 
-  const int ident = sr.flags[1];
+// KIN cell pass: terms m*N_k and N_k*(m*v) of every corner k (element.py:140,
+// 172); with temps also the natural coordinates (particle.py:180-206).
+template <typename T, int CB>
+__global__ __launch_bounds__(kThreads) void k_cell_kin(Grid2<T> g, T* __restrict__ pb,
+                                                       int64_t ld, SortRef sr,
+                                                       T* __restrict__ part, int temps) {
+  constexpr int NIN = 5;  // xi, eta, m, m*vx, m*vy
+  __shared__ T st[NIN][kSlots];
   const int* __restrict__ perm = sr.perm();
   const int* __restrict__ cs = sr.cs();
-  const int tile = tile_of_block(blockIdx.x, gridDim.x);
-  const int tyi = tile % tl.nty, txi = tile / tl.nty;
-  const int i0 = txi * kTX, j0 = tyi * tl.ty;
-  const int tid = threadIdx.x, grp = tid / kTX, col = tid % kTX;  // blockDim.x == 2*kTX
-  const int i = i0 + col;
-  const bool icol = i <= g.nx;
-  const int c0 = max(i0 - 1, 0), c1 = min(i0 + kTX, g.nx);
-  const int ncell = c1 - c0;
-
-  if (tid < ncell) {
-    const T X0 = g.X(c0 + tid), X1 = g.X(c0 + tid + 1);
-    s_mx[tid] = (X0 + X1) / T(2);
-    s_sx[tid] = T(2) / (X1 - X0);
-  }
-
-  T acc[4] = {T(0), T(0), T(0), T(0)};
-
-  // cell offsets (and, for USF, nodal velocities) of the row segment,
-  // prefetched one cell row ahead
-  const int rfirst = max(j0 - 1, 0), rlast = min(j0 + tl.ty, g.ny);  // [rfirst, rlast)
-  const bool loader = tid <= ncell && rfirst < rlast;
-  int cs_next = loader ? cs[rfirst * g.nx + c0 + tid] : 0;
-  T v_next[4] = {T(0), T(0), T(0), T(0)};
-  if (USF && loader) {
-    const int nd = rfirst * (g.nx + 1) + c0 + tid;
-    v_next[0] = vkin[nd];
-    v_next[1] = vkin[g.nnodes + nd];
-    v_next[2] = vkin[nd + g.nx + 1];
-    v_next[3] = vkin[g.nnodes + nd + g.nx + 1];
-  }
-
-  for (int r = j0 - 1; r < j0 + tl.ty; ++r) {
-    const bool top = grp == ((r + 1) & 1);
-    const int my_row = top ? r + 1 : r;
-    const bool node_ok = icol && my_row >= j0 && my_row < j0 + tl.ty && my_row <= g.ny;
-    if (r >= 0 && r < g.ny) {
-      const int cb_ = (r - rfirst) & 1;
-      __syncthreads();
-      if (tid <= ncell) {
-        s_cs[cb_][tid] = cs_next;
-        if (USF) {
-          s_v[cb_ & (USF ? 1 : 0)][0][0][tid] = v_next[0];
-          s_v[cb_ & (USF ? 1 : 0)][0][1][tid] = v_next[1];
-          s_v[cb_ & (USF ? 1 : 0)][1][0][tid] = v_next[2];
-          s_v[cb_ & (USF ? 1 : 0)][1][1][tid] = v_next[3];
-        }
-        if (r + 1 < rlast) {
-          cs_next = cs[(r + 1) * g.nx + c0 + tid];
-          if (USF) {
-            const int nd = (r + 1) * (g.nx + 1) + c0 + tid;
-            v_next[0] = vkin[nd];
-            v_next[1] = vkin[g.nnodes + nd];
-            v_next[2] = vkin[nd + g.nx + 1];
-            v_next[3] = vkin[g.nnodes + nd + g.nx + 1];
-          }
-        }
+  const int ident = sr.flags[1];
+  const int c0 = blockIdx.x * CB, t = threadIdx.x;
+  const int cend = min(c0 + CB, g.ncells);
+  const int mycell = c0 + t;
+  const bool celler = t < CB && mycell < cend;
+  const int ms0 = celler ? cs[mycell] : 0, ms1 = celler ? cs[mycell + 1] : 0;
+  const int s0 = cs[c0], s1 = cs[cend];
+  T acc[4 * NQ_KIN];
+#pragma unroll
+  for (int v = 0; v < 4 * NQ_KIN; ++v) acc[v] = T(0);
+  for (int k0 = s0; k0 < s1; k0 += kSlots) {
+    const int kend = min(k0 + kSlots, s1);
+    int pp[kPer];
+#pragma unroll
+    for (int j = 0; j < kPer; ++j) {
+      const int s = k0 + t + j * kThreads;
+      pp[j] = s < kend ? (ident ? s : perm[s]) : -1;
+    }
+    T x[kPer], y[kPer], m[kPer], vx[kPer], vy[kPer];
+#pragma unroll
+    for (int j = 0; j < kPer; ++j) {
+      if (pp[j] < 0) continue;
+      const int p = pp[j];
+      x[j] = PR(DMPM_P_X)[p];
+      y[j] = PR(DMPM_P_X + 1)[p];
+      m[j] = PR(DMPM_P_MASS)[p];
+      vx[j] = PR(DMPM_P_V)[p];
+      vy[j] = PR(DMPM_P_V + 1)[p];
+    }
+#pragma unroll
+    for (int j = 0; j < kPer; ++j) {
+      if (pp[j] < 0) continue;
+      const int q = t + j * kThreads;
+      const int e = element_id<T, 2>(g.geo(), x[j], y[j]);
+      const int I = e % g.nx, J = e / g.nx;
+      const T xi = natural(x[j], g.X(I), g.X(I + 1));
+      const T eta = natural(y[j], g.Y(J), g.Y(J + 1));
+      st[0][q] = xi;
+      st[1][q] = eta;
+      st[2][q] = m[j];
+      st[3][q] = m[j] * vx[j];
+      st[4][q] = m[j] * vy[j];
+      if (temps) {
+        PR(DMPM_P_XI)[pp[j]] = xi;
+        PR(DMPM_P_XI + 1)[pp[j]] = eta;
       }
-      __syncthreads();
-      const int* __restrict__ rcs = s_cs[cb_];
-      const int sb = rcs[0], se = rcs[ncell];
-      const bool row_owned = r >= j0 && r < j0 + tl.ty;
-      const T Y0 = g.Y(r), Y1 = g.Y(r + 1);
-      const T my = (Y0 + Y1) / T(2), sy = T(2) / (Y1 - Y0);
-      for (int cb = sb; cb < se; cb += CH) {
-        const int ce = min(cb + CH, se);
-        if (cb != sb) __syncthreads();
-        // ---- stage the chunk's particles -------------------------------------
-        for (int s = cb + tid; s < ce; s += NT) {
-          const int p = ident ? s : perm[s];
-          const int q = s - cb;
-          const T x = PR(DMPM_P_X)[p], y = PR(DMPM_P_X + 1)[p];
-          // the sort put p in cell row r, columns [c0, c1); its column is that
-          // of its element id (A1: x >= X_nx wraps to column 0 of the next
-          // row, points below or left of the mesh belong to element 0)
-          const int e = element_id<T, 2>(g.geo(), x, y);
-          const int I = e % g.nx;
-          const int kk = min(max(I - c0, 0), ncell - 1);
-          const T xi = (x - s_mx[kk]) * s_sx[kk];  // particle.py:203-205
-          const T eta = (y - my) * sy;
-          const bool owned = row_owned && I >= i0 && I < i0 + kTX;
-          s_xi[q] = xi;
-          s_eta[q] = eta;
-          const T m = PR(DMPM_P_MASS)[p];
-          s_q[0][q] = m;
-          if (KIN) {
-            s_q[1][q] = m * PR(DMPM_P_V)[p];
-            s_q[2][q] = m * PR(DMPM_P_V + 1)[p];
-            if (temps && owned) {
-              PR(DMPM_P_XI)[p] = xi;
-              PR(DMPM_P_XI + 1)[p] = eta;
-            }
-          } else {
-            constexpr int q3 = KIN ? 0 : 3, q4 = KIN ? 0 : 4;
-            s_ix[q % CHF] = s_sx[kk];
-            if (MODE == MODE_FORCE_USL) {
-              s_q[1][q] = -PR(DMPM_P_VOLUME)[p];
-              s_q[2][q] = PR(DMPM_P_STRESS + 0)[p];
-              s_q[q3][q] = PR(DMPM_P_STRESS + 1)[p];
-              s_q[q4][q] = PR(DMPM_P_STRESS + 3)[p];
-            } else {
-              // USF strain / volume / stress update (particle.py:249-327)
-              const int vb = cb_ & (USF ? 1 : 0);
-              const T vx[4] = {s_v[vb][0][0][kk], s_v[vb][0][0][kk + 1], s_v[vb][1][0][kk + 1],
-                               s_v[vb][1][0][kk]};
-              const T vy[4] = {s_v[vb][0][1][kk], s_v[vb][0][1][kk + 1], s_v[vb][1][1][kk + 1],
-                               s_v[vb][1][1][kk]};
-              const T ix = s_sx[kk], iy = sy;
-              T rr[6], rc[6], d[6];
-              rate2s(ix, iy, xi, eta, vx, vy, rr);
-              rate2s(ix, iy, T(0), T(0), vx, vy, rc);
-              for (int k = 0; k < 6; ++k) d[k] = rr[k] * dt;
-              const T dvol = dt * (rc[0] + rc[1]);
-              T ds[4];
-              stress_inc(mat, mat.per_E ? PR(DMPM_P_YOUNGS)[p] : T(0), d, ds);
-              const T f = T(1) + dvol;
-              T nv[kDB];
-              nv[0] = db.cur[0][p] + d[0];
-              nv[1] = db.cur[1][p] + d[1];
-              nv[2] = db.cur[2][p] + d[3];
-              for (int k = 0; k < 4; ++k) nv[3 + k] = db.cur[3 + k][p] + ds[k];
-              nv[7] = db.cur[7][p] * f;
-              nv[8] = db.cur[8][p] / f;
-              nv[9] = db.cur[9][p] + dvol;
-              if (owned) {
-                for (int k = 0; k < kDB; ++k) db.nxt[k][p] = nv[k];
-                if (temps) {
-                  for (int k = 0; k < 6; ++k) {
-                    PR(DMPM_P_STRAIN_RATE + k)[p] = rr[k];
-                    PR(DMPM_P_DSTRAIN + k)[p] = d[k];
-                  }
-                  PR(DMPM_P_DVOLSTRAIN)[p] = dvol;
-                }
-              }
-              s_q[1][q] = -nv[7];
-              s_q[2][q] = nv[3];
-              s_q[q3][q] = nv[4];
-              s_q[q4][q] = nv[6];
-            }
-          }
-        }
-        __syncthreads();
-        // ---- node threads accumulate cells i-1 then i of this cell row ------------
-        if (node_ok) {
-          for (int dc = -1; dc <= 0; ++dc) {
-            const int c = i + dc;
-            if (c < c0 || c >= c1) continue;
-            const int lo = max(rcs[c - c0], cb) - cb, hi = min(rcs[c - c0 + 1], ce) - cb;
-            const int corner = top ? (dc < 0 ? 2 : 3) : (dc < 0 ? 1 : 0);
-            for (int q = lo; q < hi; ++q) {
-              const T xi = s_xi[q], eta = s_eta[q];
-              const T N = q4_N<T>(corner, xi, eta);
-              const T m = s_q[0][q];
-              if (KIN) {
-                acc[0] = acc[0] + m * N;                 // element.py:140
-                acc[1] = acc[1] + N * s_q[1][q];         // element.py:172
-                acc[2] = acc[2] + N * s_q[2][q];
-              } else {
-                constexpr int q3 = KIN ? 0 : 3, q4 = KIN ? 0 : 4;
-                acc[0] = acc[0] + N * (m * gx);          // element.py:254-256
-                acc[1] = acc[1] + N * (m * gy);
-                T dx, dy;
-                q4_dNdx(corner, xi, eta, s_ix[q % CHF], sy, dx, dy);
-                const T nV = s_q[1][q];
-                acc[2] = acc[2] + nV * (dx * s_q[2][q] + dy * s_q[q4][q]);  // element.py:868-878
-                acc[3] = acc[3] + nV * (dy * s_q[q3][q] + dx * s_q[q4][q]);
-              }
-            }
-          }
+    }
+    __syncthreads();
+    if (celler) {
+      const int lo = max(ms0, k0) - k0, hi = min(ms1, kend) - k0;
+      for (int q = lo; q < hi; ++q) {
+        T N[4];
+        q4_N4(st[0][q], st[1][q], N);
+        const T mq = st[2][q], mvx = st[3][q], mvy = st[4][q];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          acc[NQ_KIN * k + 0] = acc[NQ_KIN * k + 0] + mq * N[k];
+          acc[NQ_KIN * k + 1] = acc[NQ_KIN * k + 1] + N[k] * mvx;
+          acc[NQ_KIN * k + 2] = acc[NQ_KIN * k + 2] + N[k] * mvy;
         }
       }
     }
-    // ---- node row r is complete: epilogue ---------------------------------------
-    if (!top) {
-      if (node_ok) {
-        const int nd = my_row * (g.nx + 1) + i;
-        const int cr = cidx ? cidx[nd] : -1;
-        const int mask = cr >= 0 ? cmask[cr] : 0;
-        if (MODE == MODE_KIN) {  // element.py:122-202, constraint.py:29-44
-          const T m = acc[0];
-          T p[2] = {snap(acc[1]), snap(acc[2])};
-          NR(DMPM_N_MASS)[nd] = m;
-          for (int d = 0; d < 2; ++d) {
-            T v = (m == T(0)) ? NR(DMPM_N_V + d)[nd] : p[d] / m;
-            v = snap(v);
-            if (mask & (1 << d)) {
-              const T c = T(cval[2 * cr + d]);
-              v = c;
-              p[d] = m * c;
-            }
-            NR(DMPM_N_P + d)[nd] = p[d];
-            vkin[(int64_t)d * g.nnodes + nd] = v;
+    __syncthreads();
+  }
+  if (celler)
+#pragma unroll
+    for (int v = 0; v < 4 * NQ_KIN; ++v) part[(int64_t)v * g.ncells + mycell] = acc[v];
+}
+
+// FORCE cell pass: terms N_k*(m*g) (element.py:254-256) and
+// -V*(dN_k/dx . sigma) (element.py:868-878) of every corner k.  USF first
+// updates strain, volume and stress in place (particle.py:249-327).
+template <typename T, int CB, bool USF>
+__global__ __launch_bounds__(kThreads) void k_cell_force(Grid2<T> g, T* __restrict__ pb,
+                                                         int64_t ld, SortRef sr,
+                                                         T* __restrict__ part,
+                                                         const T* __restrict__ vkin, Mat2 mat,
+                                                         T gx, T gy, T dt, int temps) {
+  constexpr int NIN = 7;  // xi, eta, m, -V, sxx, syy, sxy
+  __shared__ T st[NIN][kSlots];
+  __shared__ T sv[USF ? 8 : 1][USF ? CB : 1];
+  const int* __restrict__ perm = sr.perm();
+  const int* __restrict__ cs = sr.cs();
+  const int ident = sr.flags[1];
+  const int c0 = blockIdx.x * CB, t = threadIdx.x;
+  const int cend = min(c0 + CB, g.ncells);
+  const int mycell = c0 + t;
+  const bool celler = t < CB && mycell < cend;
+  const int ms0 = celler ? cs[mycell] : 0, ms1 = celler ? cs[mycell + 1] : 0;
+  const int s0 = cs[c0], s1 = cs[cend];
+  if (USF && celler) {
+    int ids[4];
+    cell_nodes(g.nx, mycell % g.nx, mycell / g.nx, ids);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      sv[k][t] = vkin[ids[k]];
+      sv[4 + k][t] = vkin[g.nnodes + ids[k]];
+    }
+  }
+  T acc[4 * NQ_FORCE];
+#pragma unroll
+  for (int v = 0; v < 4 * NQ_FORCE; ++v) acc[v] = T(0);
+  for (int k0 = s0; k0 < s1; k0 += kSlots) {
+    const int kend = min(k0 + kSlots, s1);
+    int pp[kPer];
+#pragma unroll
+    for (int j = 0; j < kPer; ++j) {
+      const int s = k0 + t + j * kThreads;
+      pp[j] = s < kend ? (ident ? s : perm[s]) : -1;
+    }
+    if (USF && k0 == s0) __syncthreads();  // nodal velocities staged
+#pragma unroll
+    for (int j = 0; j < kPer; ++j) {
+      if (pp[j] < 0) continue;
+      const int p = pp[j];
+      const int q = t + j * kThreads;
+      const T x = PR(DMPM_P_X)[p], y = PR(DMPM_P_X + 1)[p];
+      const int e = element_id<T, 2>(g.geo(), x, y);
+      const int I = e % g.nx, J = e / g.nx;
+      const T X0 = g.X(I), X1 = g.X(I + 1), Y0 = g.Y(J), Y1 = g.Y(J + 1);
+      const T xi = natural(x, X0, X1);
+      const T eta = natural(y, Y0, Y1);
+      T V, sxx, syy, sxy;
+      if (USF) {
+        T ix, iy;
+        q4_invJ(xi, eta, X0, X1, Y0, Y1, ix, iy);
+        const int lc = min(max(e - c0, 0), CB - 1);
+        const T vxn[4] = {sv[0][lc], sv[1][lc], sv[2][lc], sv[3][lc]};
+        const T vyn[4] = {sv[4][lc], sv[5][lc], sv[6][lc], sv[7][lc]};
+        T rr[6], rc[6], d[6];
+        rate2s(ix, iy, xi, eta, vxn, vyn, rr);
+        rate2s(ix, iy, T(0), T(0), vxn, vyn, rc);
+        for (int k = 0; k < 6; ++k) d[k] = rr[k] * dt;
+        const T dvol = dt * (rc[0] + rc[1]);
+        T ds[4];
+        stress_inc(mat, mat.per_E ? PR(DMPM_P_YOUNGS)[p] : T(0), d, ds);
+        const T f = T(1) + dvol;
+        PR(DMPM_P_STRAIN + 0)[p] = PR(DMPM_P_STRAIN + 0)[p] + d[0];
+        PR(DMPM_P_STRAIN + 1)[p] = PR(DMPM_P_STRAIN + 1)[p] + d[1];
+        PR(DMPM_P_STRAIN + 3)[p] = PR(DMPM_P_STRAIN + 3)[p] + d[3];
+        sxx = PR(DMPM_P_STRESS + 0)[p] + ds[0];
+        syy = PR(DMPM_P_STRESS + 1)[p] + ds[1];
+        PR(DMPM_P_STRESS + 2)[p] = PR(DMPM_P_STRESS + 2)[p] + ds[2];
+        sxy = PR(DMPM_P_STRESS + 3)[p] + ds[3];
+        PR(DMPM_P_STRESS + 0)[p] = sxx;
+        PR(DMPM_P_STRESS + 1)[p] = syy;
+        PR(DMPM_P_STRESS + 3)[p] = sxy;
+        V = PR(DMPM_P_VOLUME)[p] * f;
+        PR(DMPM_P_VOLUME)[p] = V;
+        PR(DMPM_P_DENSITY)[p] = PR(DMPM_P_DENSITY)[p] / f;
+        PR(DMPM_P_VOLSTRAIN_C)[p] = PR(DMPM_P_VOLSTRAIN_C)[p] + dvol;
+        if (temps) {
+          for (int k = 0; k < 6; ++k) {
+            PR(DMPM_P_STRAIN_RATE + k)[p] = rr[k];
+            PR(DMPM_P_DSTRAIN + k)[p] = d[k];
           }
-        } else {  // element.py:204-329
-          const T m = NR(DMPM_N_MASS)[nd];
-          for (int d = 0; d < 2; ++d) {
-            T fe = acc[d];
-            const int qg = nfd ? nfd[(int64_t)d * g.nnodes + nd] : -1;
-            if (qg >= 0)
-              for (int k = nf_off[qg]; k < nf_off[qg + 1]; ++k)
-                fe = fe + fn_value<T>(fns, nf_fn[k], t) * T(nf_val[k]);
-            const T fi = acc[2 + d];
-            NR(DMPM_N_FEXT + d)[nd] = fe;
-            NR(DMPM_N_FINT + d)[nd] = fi;
-            const T F = (fi + fe) + NR(DMPM_N_FDAMP + d)[nd];
-            T a = nan_to_num(F / m);
-            T v = vkin[(int64_t)d * g.nnodes + nd] + a * dt;
-            T pm = m * v;
-            v = snap(v);
-            a = snap(a);
-            if (mask & (1 << d)) {
-              const T c = T(cval[2 * cr + d]);
-              v = snap(c);
-              pm = m * c;
-              a = T(0);
-            }
-            NR(DMPM_N_V + d)[nd] = v;
-            NR(DMPM_N_A + d)[nd] = a;
-            NR(DMPM_N_P + d)[nd] = pm;
-          }
+          PR(DMPM_P_DVOLSTRAIN)[p] = dvol;
+        }
+      } else {
+        V = PR(DMPM_P_VOLUME)[p];
+        sxx = PR(DMPM_P_STRESS + 0)[p];
+        syy = PR(DMPM_P_STRESS + 1)[p];
+        sxy = PR(DMPM_P_STRESS + 3)[p];
+      }
+      st[0][q] = xi;
+      st[1][q] = eta;
+      st[2][q] = PR(DMPM_P_MASS)[p];
+      st[3][q] = -V;
+      st[4][q] = sxx;
+      st[5][q] = syy;
+      st[6][q] = sxy;
+    }
+    __syncthreads();
+    if (celler) {
+      const int I = mycell % g.nx, J = mycell / g.nx;
+      const T X0 = g.X(I), X1 = g.X(I + 1), Y0 = g.Y(J), Y1 = g.Y(J + 1);
+      const int lo = max(ms0, k0) - k0, hi = min(ms1, kend) - k0;
+      for (int q = lo; q < hi; ++q) {
+        const T xi = st[0][q], eta = st[1][q];
+        const T bx = st[2][q] * gx, by = st[2][q] * gy;
+        const T nV = st[3][q], sxx = st[4][q], syy = st[5][q], sxy = st[6][q];
+        T ix, iy;
+        q4_invJ(xi, eta, X0, X1, Y0, Y1, ix, iy);
+        T N[4];
+        q4_N4(xi, eta, N);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          T dx, dy;
+          q4_dNdx(k, xi, eta, ix, iy, dx, dy);
+          acc[NQ_FORCE * k + 0] = acc[NQ_FORCE * k + 0] + N[k] * bx;
+          acc[NQ_FORCE * k + 1] = acc[NQ_FORCE * k + 1] + N[k] * by;
+          acc[NQ_FORCE * k + 2] = acc[NQ_FORCE * k + 2] + nV * (dx * sxx + dy * sxy);
+          acc[NQ_FORCE * k + 3] = acc[NQ_FORCE * k + 3] + nV * (dy * syy + dx * sxy);
         }
       }
-      acc[0] = acc[1] = acc[2] = acc[3] = T(0);
     }
+    __syncthreads();
+  }
+  if (celler)
+#pragma unroll
+    for (int v = 0; v < 4 * NQ_FORCE; ++v) part[(int64_t)v * g.ncells + mycell] = acc[v];
+}
+
+// Sum over the (up to four) cells of node (i, j), canonical order, of the
+// partials of quantity q: corner 2 of cell (i-1,j-1), 3 of (i,j-1), 1 of
+// (i-1,j), 0 of (i,j).
+template <typename T, int NQ>
+__device__ __forceinline__ void node_gather(const Grid2<T>& g, const T* __restrict__ part, int i,
+                                            int j, T acc[NQ]) {
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) acc[q] = T(0);
+  const int cells[4] = {(j - 1) * g.nx + i - 1, (j - 1) * g.nx + i, j * g.nx + i - 1,
+                        j * g.nx + i};
+  const bool ok[4] = {j > 0 && i > 0, j > 0 && i < g.nx, j < g.ny && i > 0, j < g.ny && i < g.nx};
+  const int corner[4] = {2, 3, 1, 0};
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    if (!ok[c]) continue;
+#pragma unroll
+    for (int q = 0; q < NQ; ++q)
+      acc[q] = acc[q] + part[(int64_t)(NQ * corner[c] + q) * g.ncells + cells[c]];
+  }
+}
+
+// KIN node pass: nodal mass, momentum (snapped), velocity (element.py:122-202)
+// and velocity constraints (constraint.py:29-44); the kinematic velocity also
+// goes to vkin for the force phase.
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_node_kin(Grid2<T> g, const T* __restrict__ part,
+                                                       T* __restrict__ nb, int64_t nld,
+                                                       T* __restrict__ vkin,
+                                                       const int* __restrict__ cidx,
+                                                       const int* __restrict__ cmask,
+                                                       const double* __restrict__ cval) {
+  const int nd = blockIdx.x * blockDim.x + threadIdx.x;
+  if (nd >= g.nnodes) return;
+  const int i = nd % (g.nx + 1), j = nd / (g.nx + 1);
+  T acc[NQ_KIN];
+  node_gather<T, NQ_KIN>(g, part, i, j, acc);
+  const int cr = cidx ? cidx[nd] : -1;
+  const int mask = cr >= 0 ? cmask[cr] : 0;
+  const T m = acc[0];
+  T p[2] = {snap(acc[1]), snap(acc[2])};
+  NR(DMPM_N_MASS)[nd] = m;
+  for (int d = 0; d < 2; ++d) {
+    T v = (m == T(0)) ? NR(DMPM_N_V + d)[nd] : p[d] / m;
+    v = snap(v);
+    if (mask & (1 << d)) {
+      const T c = T(cval[2 * cr + d]);
+      v = c;
+      p[d] = m * c;
+    }
+    NR(DMPM_N_P + d)[nd] = p[d];
+    vkin[(int64_t)d * g.nnodes + nd] = v;
+  }
+}
+
+// FORCE node pass: external force (body + concentrated nodal forces at t,
+// element.py:204-284), internal force, acceleration/velocity/momentum update
+// (element.py:305-329) and constraints.
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_node_force(
+    Grid2<T> g, const T* __restrict__ part, T* __restrict__ nb, int64_t nld,
+    const T* __restrict__ vkin, const int* __restrict__ cidx, const int* __restrict__ cmask,
+    const double* __restrict__ cval, T dt, T t, const int* __restrict__ nfd,
+    const int* __restrict__ nf_off, const int* __restrict__ nf_fn,
+    const double* __restrict__ nf_val, FnTable fns) {
+  const int nd = blockIdx.x * blockDim.x + threadIdx.x;
+  if (nd >= g.nnodes) return;
+  const int i = nd % (g.nx + 1), j = nd / (g.nx + 1);
+  T acc[NQ_FORCE];
+  node_gather<T, NQ_FORCE>(g, part, i, j, acc);
+  const int cr = cidx ? cidx[nd] : -1;
+  const int mask = cr >= 0 ? cmask[cr] : 0;
+  const T m = NR(DMPM_N_MASS)[nd];
+  for (int d = 0; d < 2; ++d) {
+    T fe = acc[d];
+    const int qg = nfd ? nfd[(int64_t)d * g.nnodes + nd] : -1;
+    if (qg >= 0)
+      for (int k = nf_off[qg]; k < nf_off[qg + 1]; ++k)
+        fe = fe + fn_value<T>(fns, nf_fn[k], t) * T(nf_val[k]);
+    const T fi = acc[2 + d];
+    NR(DMPM_N_FEXT + d)[nd] = fe;
+    NR(DMPM_N_FINT + d)[nd] = fi;
+    const T F = (fi + fe) + NR(DMPM_N_FDAMP + d)[nd];
+    T a = nan_to_num(F / m);
+    T v = vkin[(int64_t)d * g.nnodes + nd] + a * dt;
+    T pm = m * v;
+    v = snap(v);
+    a = snap(a);
+    if (mask & (1 << d)) {
+      const T c = T(cval[2 * cr + d]);
+      v = snap(c);
+      pm = m * c;
+      a = T(0);
+    }
+    NR(DMPM_N_V + d)[nd] = v;
+    NR(DMPM_N_A + d)[nd] = a;
+    NR(DMPM_N_P + d)[nd] = pm;
   }
 }
 
@@ -480,9 +536,8 @@ __device__ __forceinline__ void record_move(const MoveLog& L, int ncells, int p,
 template <typename T, bool USL>
 __global__ __launch_bounds__(kThreads) void k_g2p_tile(Grid2<T> g, T* __restrict__ pb, int64_t ld,
                                                        int n, const T* __restrict__ nb,
-                                                       int64_t nld, MoveLog mlog,
-                                                       DB<T> db, Mat2 mat, T dt, int vu,
-                                                       int temps, int usf_db) {
+                                                       int64_t nld, MoveLog mlog, Mat2 mat,
+                                                       T dt, int vu, int temps) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= n) return;
   const T x = PR(DMPM_P_X)[p], y = PR(DMPM_P_X + 1)[p];
@@ -496,15 +551,13 @@ __global__ __launch_bounds__(kThreads) void k_g2p_tile(Grid2<T> g, T* __restrict
     PR(DMPM_P_X + 1)[p] = nan;
     PR(DMPM_P_V)[p] = nan;
     PR(DMPM_P_V + 1)[p] = nan;
-    if (usf_db || USL)
-      for (int k = 0; k < kDB; ++k) (USL ? PR(db_row(k)) : db.nxt[k])[p] = nan;
-    if (usf_db || USL) {
-      PR(DMPM_P_STRAIN + 2)[p] = nan;
-      PR(DMPM_P_STRAIN + 4)[p] = nan;
-      PR(DMPM_P_STRAIN + 5)[p] = nan;
-      PR(DMPM_P_STRESS + 4)[p] = nan;
-      PR(DMPM_P_STRESS + 5)[p] = nan;
+    for (int k = 0; k < 6; ++k) {
+      PR(DMPM_P_STRAIN + k)[p] = nan;
+      PR(DMPM_P_STRESS + k)[p] = nan;
     }
+    PR(DMPM_P_VOLUME)[p] = nan;
+    PR(DMPM_P_DENSITY)[p] = nan;
+    PR(DMPM_P_VOLSTRAIN_C)[p] = nan;
     if (temps) {
       PR(DMPM_P_XI)[p] = nan;
       PR(DMPM_P_XI + 1)[p] = nan;
@@ -1032,19 +1085,16 @@ int enqueue_resort(dmpm_plan* P, const Grid2<T>& g, const T* pb, int32_t* eid, h
   return launch_check("resort");
 }
 
-template <typename T, int TXN>
+template <typename T, int CB>
 int tile_steps(dmpm_plan* P, const Grid2<T>& g, T* pb, int32_t* eid, T* nb, int scheme, int vu,
                const double* grav, double dt, int64_t step0, int64_t nsteps, int temps,
-               hipStream_t s, T* alt, T* vkin) {
+               hipStream_t s, T* part, T* vkin) {
   const int64_t ld = P->d.ld, nld = P->d.nld;
   const int n = (int)P->n;
   const bool usf = scheme == DMPM_USF;
   const bool split = usf && P->usf_split;
-  TileShape tl;
-  tl.ty = P->tile_ty > 0 ? P->tile_ty : 8;
-  tl.ntx = (g.nx + 1 + TXN - 1) / TXN;
-  tl.nty = (g.ny + 1 + tl.ty - 1) / tl.ty;
-  const unsigned ntiles = (unsigned)(tl.ntx * tl.nty);
+  const unsigned ncb = (unsigned)((g.ncells + CB - 1) / CB);
+  const unsigned nnb = nblk(g.nnodes);
   Mat2 mat;
   mat.kind = P->d.material;
   mat.per_E = P->d.per_particle_E;
@@ -1058,65 +1108,47 @@ int tile_steps(dmpm_plan* P, const Grid2<T>& g, T* pb, int32_t* eid, T* nb, int 
   const T gx = T(grav[0]), gy = T(grav[1]);
   const T tdt = T(dt);
   const SortRef sr = sort_ref(P);
-
-  DB<T> cur_is_pb, cur_is_alt;
-  for (int k = 0; k < kDB; ++k) {
-    T* a = pb + (int64_t)db_row(k) * ld;
-    T* w = alt ? alt + (int64_t)k * ld : a;
-    cur_is_pb.cur[k] = a;
-    cur_is_pb.nxt[k] = w;
-    cur_is_alt.cur[k] = w;
-    cur_is_alt.nxt[k] = a;
-  }
-  int parity = 0;
   for (int64_t step = step0; step < step0 + nsteps; ++step) {
     {
       ProfScopeT ps_(P, KC_SORT, s);
       DMPM_TRY(enqueue_resort<T>(P, g, pb, eid, s));
     }
-    const DB<T>& db = parity ? cur_is_alt : cur_is_pb;
     const T t = T(step) * tdt;
     {
       ProfScopeT ps_(P, KC_TILE_KIN, s);
-      k_tile<T, MODE_KIN, TXN><<<ntiles, 2 * TXN, 0, s>>>(
-          g, tl, pb, ld, sr, nb, nld, vkin, P->cidx, P->cmask, P->cval, db, mat, gx, gy, tdt, t,
-          P->nfd, P->nf_off, P->nf_fn, P->nf_val, fns, temps);
-      DMPM_TRY(launch_check("tile_kin"));
+      k_cell_kin<T, CB><<<ncb, kThreads, 0, s>>>(g, pb, ld, sr, part, temps);
+      k_node_kin<T><<<nnb, kThreads, 0, s>>>(g, part, nb, nld, vkin, P->cidx, P->cmask,
+                                             P->cval);
+      DMPM_TRY(launch_check("p2g_kin"));
     }
     if (split) {
       ProfScopeT ps_(P, KC_STRESS, s);
       k_stress_tile<T><<<nblk(n), kThreads, 0, s>>>(g, pb, ld, n, vkin, mat, tdt, temps);
-      DMPM_TRY(launch_check("tile_stress"));
+      DMPM_TRY(launch_check("stress"));
     }
     {
       ProfScopeT ps_(P, KC_TILE_FORCE, s);
       if (usf && !split)
-        k_tile<T, MODE_FORCE_USF, TXN><<<ntiles, 2 * TXN, 0, s>>>(
-            g, tl, pb, ld, sr, nb, nld, vkin, P->cidx, P->cmask, P->cval, db, mat, gx, gy, tdt,
-            t, P->nfd, P->nf_off, P->nf_fn, P->nf_val, fns, temps);
+        k_cell_force<T, CB, true><<<ncb, kThreads, 0, s>>>(g, pb, ld, sr, part, vkin, mat, gx,
+                                                           gy, tdt, temps);
       else
-        k_tile<T, MODE_FORCE_USL, TXN><<<ntiles, 2 * TXN, 0, s>>>(
-            g, tl, pb, ld, sr, nb, nld, vkin, P->cidx, P->cmask, P->cval, db, mat, gx, gy, tdt,
-            t, P->nfd, P->nf_off, P->nf_fn, P->nf_val, fns, temps);
-      DMPM_TRY(launch_check("tile_force"));
+        k_cell_force<T, CB, false><<<ncb, kThreads, 0, s>>>(g, pb, ld, sr, part, vkin, mat, gx,
+                                                            gy, tdt, temps);
+      k_node_force<T><<<nnb, kThreads, 0, s>>>(g, part, nb, nld, vkin, P->cidx, P->cmask,
+                                               P->cval, tdt, t, P->nfd, P->nf_off, P->nf_fn,
+                                               P->nf_val, fns);
+      DMPM_TRY(launch_check("p2g_force"));
     }
     {
       ProfScopeT ps_(P, KC_TILE_G2P, s);
       if (usf)
-        k_g2p_tile<T, false><<<nblk(n), kThreads, 0, s>>>(g, pb, ld, n, nb, nld, move_log(P), db,
-                                                          mat, tdt, vu, temps, split ? 0 : 1);
+        k_g2p_tile<T, false><<<nblk(n), kThreads, 0, s>>>(g, pb, ld, n, nb, nld, move_log(P), mat,
+                                                          tdt, vu, temps);
       else
-        k_g2p_tile<T, true><<<nblk(n), kThreads, 0, s>>>(g, pb, ld, n, nb, nld, move_log(P), db,
-                                                         mat, tdt, vu, temps, 0);
-      DMPM_TRY(launch_check("tile_g2p"));
+        k_g2p_tile<T, true><<<nblk(n), kThreads, 0, s>>>(g, pb, ld, n, nb, nld, move_log(P), mat,
+                                                         tdt, vu, temps);
+      DMPM_TRY(launch_check("g2p"));
     }
-    if (usf && !split) parity ^= 1;
-  }
-  if (parity) {  // the latest double-buffered fields are in the workspace
-    for (int k = 0; k < kDB; ++k)
-      DMPM_TRY(hip_check(hipMemcpyAsync(pb + (int64_t)db_row(k) * ld, alt + (int64_t)k * ld,
-                                        (size_t)n * sizeof(T), hipMemcpyDeviceToDevice, s),
-                         "hipMemcpyAsync"));
   }
   return DMPM_OK;
 }
@@ -1127,19 +1159,23 @@ int tile_run_t(dmpm_plan* P, T* pb, int32_t* eid, T* nb, int scheme, int vu, con
   const Grid2<T> g = grid_of<T>(P);
   const int64_t ld = P->d.ld;
   const int n = (int)P->n;
-  const bool usf = scheme == DMPM_USF;
-  const bool dbuf = usf && !P->usf_split;
-  T* alt = dbuf ? (T*)plan_workspace(P, 0, (size_t)kDB * ld * sizeof(T)) : nullptr;
+  T* part = (T*)plan_workspace(P, 0, (size_t)4 * NQ_FORCE * g.ncells * sizeof(T));
   T* vkin = (T*)plan_workspace(P, 1, (size_t)2 * P->nnodes * sizeof(T));
-  if ((dbuf && !alt) || !vkin) return fail(DMPM_EHIP, "workspace allocation failed");
+  if (!part || !vkin) return fail(DMPM_EHIP, "workspace allocation failed");
   k_tile_check<T><<<nblk(n), kThreads, 0, s>>>(g, pb, ld, n, eid, P->key, P->flags, P->mrec,
                                                P->mov);
   DMPM_TRY(launch_check("tile_check"));
-  if (P->tile_tx == 64)
-    return tile_steps<T, 64>(P, g, pb, eid, nb, scheme, vu, grav, dt, step0, nsteps, temps, s,
-                             alt, vkin);
-  return tile_steps<T, 128>(P, g, pb, eid, nb, scheme, vu, grav, dt, step0, nsteps, temps, s,
-                            alt, vkin);
+  switch (P->cell_block) {
+    case 64:
+      return tile_steps<T, 64>(P, g, pb, eid, nb, scheme, vu, grav, dt, step0, nsteps, temps, s,
+                               part, vkin);
+    default:
+      return tile_steps<T, 128>(P, g, pb, eid, nb, scheme, vu, grav, dt, step0, nsteps, temps, s,
+                                part, vkin);
+    case 256:
+      return tile_steps<T, 256>(P, g, pb, eid, nb, scheme, vu, grav, dt, step0, nsteps, temps, s,
+                                part, vkin);
+  }
 }
 
 }  // namespace

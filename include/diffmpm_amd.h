@@ -131,13 +131,14 @@ int dmpm_plan_set_nodal_forces(dmpm_plan* plan, int32_t n, const int32_t* node_i
 int dmpm_plan_set_tractions(dmpm_plan* plan, int32_t n, const int32_t* pids,
                             const int32_t* dirs, const int32_t* fn,
                             const double* tractions);
-/* Tile shape of the fused 2D kernels: node_rows (0 = default 8) and
- * node_cols (64 or 128; 0 = keep).  A tuning knob with no effect on results.
- * No reference counterpart. */
-int dmpm_plan_set_tile(dmpm_plan* plan, int32_t node_rows, int32_t node_cols);
+/* Cells per workgroup of the fused 2D step's P2G cell pass (64, 128 or 256;
+ * 0 = default 128).  A tuning knob with no effect on results.  No reference
+ * counterpart. */
+int dmpm_plan_set_block(dmpm_plan* plan, int32_t cells_per_block);
 /* Execution options (no effect on results; no reference counterpart):
  * DMPM_OPT_USF_SPLIT = 1: the fused USF step updates stresses in a streamed
- * particle kernel before the force tile kernel (1, default) or inside it (0). */
+ * particle kernel before the force P2G cell pass (1, default) or inside that
+ * pass (0). */
 enum { DMPM_OPT_USF_SPLIT = 1 };
 int dmpm_plan_set_option(dmpm_plan* plan, int32_t option, int32_t value);
 /* Declares that the element ids changed outside the library (forces a re-sort). */

@@ -237,25 +237,25 @@ def _moving_spec(scheme, per_particle_E=False, nx=200, ny=40, rows=30, drift=(0.
     return spec
 
 
-@pytest.mark.parametrize("fused", [False, True])
+@pytest.mark.parametrize("split", [False, True])
 @pytest.mark.parametrize("scheme,per_E,drift", [("usf", False, (0.6, 0.3)),
                                                  ("usl", False, (0.6, 0.3)),
                                                  ("usf", True, (0.6, 0.3)),
                                                  ("usf", False, None), ("usl", True, None)])
-def test_tile_path_equals_per_phase_path(scheme, per_E, drift, fused):
-    """The fused node-tile kernels (dmpm_run default) and the per-phase
-    kernels (dmpm_run flag bit 2) give bit-identical states, across cell
-    changes: with a drift every particle changes cell (full re-sorts), without
-    it a few hundred do per step (incremental re-sorts)."""
+def test_tile_path_equals_per_phase_path(scheme, per_E, drift, split):
+    """The fused step (dmpm_run default: re-sort, P2G cell and node passes,
+    G2P) and the per-phase kernels (dmpm_run flag bit 2) give bit-identical
+    states, across cell changes: with a drift every particle changes cell in
+    every step, without it a few hundred do (incremental re-sorts either
+    way)."""
     from diffmpm_amd._plan import plan_for
     spec = _moving_spec(scheme, per_E, drift=drift, velocity_scale=0.05 if drift else 0.1)
     a, b = cases.build_gpu(spec), cases.build_gpu(spec)
     g = torch.tensor(spec["gravity"], dtype=torch.float64)
     pa = plan_for(a.mesh.elements, a.mesh.particles[0])
     pb = plan_for(b.mesh.elements, b.mesh.particles[0])
-    if fused:
-        import diffmpm_amd._lib as L
-        pa.set_option(L.DMPM_OPT_USF_SPLIT, 0)
+    import diffmpm_amd._lib as L
+    pa.set_option(L.DMPM_OPT_USF_SPLIT, 1 if split else 0)
     e0 = a.mesh.particles[0].element_ids.clone()
     for k in range(0, spec["nsteps"], 3):
         pa.run(scheme, False, g, spec["dt"], k, 3, True, per_phase=False)
@@ -265,15 +265,15 @@ def test_tile_path_equals_per_phase_path(scheme, per_E, drift, fused):
     assert not torch.equal(e0, a.mesh.particles[0].element_ids)  # cells did change
 
 
-@pytest.mark.parametrize("ty,tx", [(1, 64), (3, 128), (8, 128), (33, 64)])
-def test_tile_shape_does_not_change_results(ty, tx):
+@pytest.mark.parametrize("cells_per_block", [64, 256])
+def test_block_size_does_not_change_results(cells_per_block):
     from diffmpm_amd._plan import plan_for
     spec = _moving_spec("usf", nx=150, ny=70, rows=60, drift=None, velocity_scale=0.1)
     a, b = cases.build_gpu(spec), cases.build_gpu(spec)
     g = torch.tensor(spec["gravity"], dtype=torch.float64)
     pa = plan_for(a.mesh.elements, a.mesh.particles[0])
     pb = plan_for(b.mesh.elements, b.mesh.particles[0])
-    pb.set_tile(ty, tx)
+    pb.set_block(cells_per_block)
     pa.run("usf", False, g, spec["dt"], 0, 6, True)
     pb.run("usf", False, g, spec["dt"], 0, 6, True)
     for x, y in zip(_state(a), _state(b)):
