@@ -1,0 +1,157 @@
+"""Differentiable rollouts of the fused 2D explicit step (torch.autograd).
+
+The reference differentiates its rollouts with ``jax.grad`` through
+``MPMExplicit.solve_jit`` (diffmpm/solver.py:179-222).  Here each step is a
+``torch.autograd.Function`` whose backward calls the hand-written adjoint
+kernels (``dmpm_step_adjoint``), which recompute the forward step from the
+step's saved input state.  ``rollout(..., checkpoint_every=K)`` keeps only
+every K-th state during the forward pass and recomputes the states inside a
+segment when the backward pass reaches it.
+
+Differentiable state, one ``(8, N)`` tensor: rows x, y, vx, vy, sigma_xx,
+sigma_yy, sigma_xy, volume (``STATE_ROWS``).  Optional parameter: log of
+the Young's modulus, per particle ``(N,)`` or one scalar for the whole set.
+Mass, mesh, constraints and nodal forces are constants; strain, density,
+volumetric strain and sigma_zz do not feed back into the 2D dynamics and get
+no gradient.  The covered configurations are those of the fused path: 2D
+Quadrilateral4Node, one particle set, LinearElastic or SimpleMaterial (log E
+only for LinearElastic), no particle tractions.
+
+Example::
+
+    d = Differentiable(sim)
+    log_e = torch.full((N,), math.log(1e6), device="cuda", requires_grad=True)
+    final = rollout(d, 100, gravity, log_e=log_e, checkpoint_every=10)
+    loss = ((final[0:2] - target) ** 2).sum()
+    loss.backward()          # log_e.grad
+"""
+import math
+
+import torch
+import torch.utils.checkpoint
+
+from . import _lib
+from ._lib import call, ptr
+from ._plan import Plan, plan_for
+
+STATE_ROWS = (_lib.P_X, _lib.P_X + 1, _lib.P_V, _lib.P_V + 1, _lib.P_STRESS + 0,
+              _lib.P_STRESS + 1, _lib.P_STRESS + 3, _lib.P_VOLUME)
+C_ROWS = 8
+
+
+class Differentiable:
+    """Scratch buffers and plans of the differentiable step of ``sim`` (an
+    ``MPMExplicit`` with one particle set on a Quadrilateral4Node mesh)."""
+
+    def __init__(self, sim, per_particle_E=True):
+        mesh = sim.mesh
+        if len(mesh.particles) != 1:
+            raise ValueError("the differentiable step covers one particle set")
+        ps, el = mesh.particles[0], mesh.elements
+        if el.ndim != 2:
+            raise ValueError("the differentiable step covers 2D Quadrilateral4Node meshes")
+        if per_particle_E and not ps._has_youngs:
+            props = ps.material.properties
+            if "youngs_modulus" not in props:
+                raise ValueError("log E gradients need a LinearElastic material")
+            ps.youngs_modulus = torch.full((len(ps),), float(props["youngs_modulus"]),
+                                           dtype=ps.dtype, device=ps.device)
+            ps._plans.clear()
+        self.sim, self.ps, self.el = sim, ps, el
+        self.per_particle_E = per_particle_E
+        self.n = len(ps)
+        self.fplan = plan_for(el, ps)
+        self.aplan = Plan(el, ps)  # own sort state for the recomputed steps
+        self.pin = torch.empty_like(ps._buf)
+        self.pwork = torch.empty_like(ps._buf)
+        self.nwork = torch.empty_like(el.nodes._buf)
+        self.eid = torch.empty_like(ps._eid)
+        self.scheme = sim.scheme
+        self.code = _lib.DMPM_USF if sim.scheme == "usf" else _lib.DMPM_USL
+        self.vu = 1 if sim.velocity_update else 0
+        self.dt = float(sim.dt)
+
+    # -- state <-> particle block ---------------------------------------------
+    def get_state(self):
+        """The current dynamic state as a new (8, N) tensor."""
+        return self.ps._buf[list(STATE_ROWS), : self.n].clone()
+
+    def _write(self, buf, state, log_e):
+        buf[list(STATE_ROWS), : self.n] = state
+        if log_e is not None:
+            buf[_lib.P_YOUNGS, : self.n] = torch.exp(log_e.detach()).to(buf.dtype).expand(self.n)
+
+    def set_state(self, state, log_e=None):
+        self._write(self.ps._buf, state.detach(), log_e)
+
+    # -- one step ---------------------------------------------------------------
+    def forward_step(self, state, log_e, gravity, i):
+        self.set_state(state, log_e)
+        self.fplan.run(self.scheme, self.sim.velocity_update, gravity, self.dt, i, 1,
+                       keep_temporaries=False)
+        return self.get_state()
+
+    def adjoint_step(self, state, log_e, gravity, i, gout):
+        """Cotangents (state, log E per particle) of step i's input from the
+        cotangent of its output."""
+        self.aplan.refresh_tables()
+        self.pin.copy_(self.ps._buf)
+        self._write(self.pin, state.detach(), log_e)
+        self.nwork.copy_(self.el.nodes._buf)
+        cot = torch.zeros((C_ROWS, self.ps._ld), dtype=self.ps.dtype, device=self.ps.device)
+        cot[:, : self.n] = gout
+        lebar = torch.zeros(self.ps._ld, dtype=self.ps.dtype, device=self.ps.device)
+        call("dmpm_step_adjoint", self.aplan.h, ptr(self.pin), ptr(self.pwork), ptr(self.eid),
+             ptr(self.nwork), ptr(cot), ptr(lebar), self.code, self.vu,
+             self.aplan._gravity(gravity), self.dt, int(i), self.aplan.s)
+        return cot[:, : self.n], lebar[: self.n]
+
+
+class _StepFn(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, state, log_e, d, gravity, i):
+        out = d.forward_step(state, log_e, gravity, i)
+        ctx.d, ctx.gravity, ctx.i = d, gravity, i
+        ctx.has_log_e = log_e is not None
+        ctx.save_for_backward(state, log_e if log_e is not None else state.new_empty(0))
+        return out
+
+    @staticmethod
+    def backward(ctx, gout):
+        state, log_e = ctx.saved_tensors
+        d = ctx.d
+        gs, ge = d.adjoint_step(state, log_e if ctx.has_log_e else None, ctx.gravity, ctx.i,
+                                gout.contiguous())
+        g_log_e = None
+        if ctx.has_log_e and ctx.needs_input_grad[1]:
+            g_log_e = ge if log_e.dim() > 0 else ge.sum()
+            g_log_e = g_log_e.to(log_e.dtype).reshape(log_e.shape)
+        return gs, g_log_e, None, None, None
+
+
+def step(d, state, gravity, i, log_e=None):
+    """One differentiable step i (time i*dt) of ``d`` (a ``Differentiable``)."""
+    return _StepFn.apply(state, log_e, d, gravity, int(i))
+
+
+def rollout(d, nsteps, gravity, state=None, log_e=None, start_step=0, checkpoint_every=0):
+    """Differentiable rollout of ``nsteps`` steps from ``state`` (default: the
+    current state of the simulation).  Returns the final (8, N) state.  With
+    ``checkpoint_every=K`` only segment-start states are kept for backward.
+    After a backward pass the simulation's own particle block holds a
+    recomputed intermediate state; use the returned tensors."""
+    if state is None:
+        state = d.get_state()
+
+    def seg(st, le, i0, k):
+        for i in range(i0, i0 + k):
+            st = _StepFn.apply(st, le, d, gravity, i)
+        return st
+
+    if checkpoint_every and checkpoint_every < nsteps:
+        for i0 in range(start_step, start_step + nsteps, checkpoint_every):
+            k = min(checkpoint_every, start_step + nsteps - i0)
+            state = torch.utils.checkpoint.checkpoint(seg, state, log_e, i0, k,
+                                                      use_reentrant=False)
+        return state
+    return seg(state, log_e, start_step, nsteps)

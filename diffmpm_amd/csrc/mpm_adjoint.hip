@@ -1,0 +1,535 @@
+// Reverse-mode (adjoint) of one fused 2D explicit step (gfx950).
+//
+// The reference differentiates its rollout with jax.grad; there is no
+// hand-written adjoint in it.  This is the adjoint of the step as this
+// library computes it (mpm_tile.hip), for the dynamic particle state
+//   S = (x, y, vx, vy, sigma_xx, sigma_yy, sigma_xy, V)
+// and the per-particle log Young's modulus, with the particle mass, the mesh,
+// the constraints and the nodal forces held fixed.  Strain, density,
+// volumetric strain and sigma_zz do not feed back into the dynamics of a 2D
+// LinearElastic step; their cotangents are not propagated.
+//
+// dmpm_step_adjoint recomputes the forward step from its input state (on a
+// scratch block, keeping the nodal arrays and the kinematic nodal velocity),
+// then runs, for cotangents Sbar' of the step output:
+//   k_adj_g2p     G2P adjoint, particle-local part: cotangents of the nodal
+//                 velocity/acceleration as per-particle weights, the
+//                 natural-coordinate cotangent of G2P; USL: stress adjoint
+//   cell + node 1 scatter of those weights to the nodes (deterministic cell
+//                 pass), grid-update adjoint (a = F/m, v' = v + a dt, snaps,
+//                 constraints); USL: kinematics adjoint (v = P/m)
+//   k_adj_force   internal/body force adjoint (gather of Fbar); USF: stress
+//                 adjoint (sigma' = sigma + D deps, V' = V (1 + dvol))
+//   cell + node 2 (USF) scatter of the strain-rate cotangents to the kinematic
+//                 nodal velocities, kinematics adjoint
+//   k_adj_kin     P2G of mass/momentum adjoint (gather of Pbar, mbar), chain
+//                 to x through the natural coordinates
+// Element ids are piecewise constant in x: no cotangent flows through them.
+// snap(x) = where(|x| < 1e-12, 0, x) passes a cotangent where it kept x;
+// nan_to_num passes it where the value was finite.  Particles outside the
+// mesh get zero cotangents.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "mpm_common.h"
+#include "mpm_plan.h"
+#include "mpm_step.h"
+
+using namespace dmpm;
+
+namespace {
+
+#define PR(r) (pb + (int64_t)(r) * ld)
+#define PW(r) (pw + (int64_t)(r) * ld)
+#define NR(r) (nb + (int64_t)(r) * nld)
+#define CR(r) (cot + (int64_t)(r) * ld)
+#define WR(r) (wk + (int64_t)(r) * ld)
+#define AR(r) (na + (int64_t)(r) * g.nnodes)
+
+// rows of the per-particle weight workspace: N-weighted nodal-velocity and
+// acceleration cotangents, strain-rate cotangents (L00, L11, L01, L10 at the
+// particle, L00, L11 at the centroid), natural-coordinate cotangent
+enum { W_NVX = 0, W_NVY, W_NAX, W_NAY, W_L00, W_L11, W_L01, W_L10, W_C00, W_C11, W_XI, W_ETA,
+       W_ROWS };
+// rows of the nodal adjoint workspace
+enum { A_FX = 0, A_FY, A_M, A_VKX, A_VKY, A_PX, A_PY, A_ROWS };
+
+// Per-particle geometry at the step-start position.
+template <typename T> struct PGeo {
+  bool in;
+  int I, J;
+  T xi, eta, ix, iy;
+};
+template <typename T>
+__device__ __forceinline__ PGeo<T> pgeo(const Grid2<T>& g, T x, T y) {
+  PGeo<T> q;
+  const int e = element_id<T, 2>(g.geo(), x, y);
+  q.in = e >= 0 && e < g.ncells;
+  q.I = q.in ? e % g.nx : 0;
+  q.J = q.in ? e / g.nx : 0;
+  const T X0 = g.X(q.I), X1 = g.X(q.I + 1), Y0 = g.Y(q.J), Y1 = g.Y(q.J + 1);
+  q.xi = natural(x, X0, X1);
+  q.eta = natural(y, Y0, Y1);
+  q4_invJ(q.xi, q.eta, X0, X1, Y0, Y1, q.ix, q.iy);
+  return q;
+}
+
+// d^2 N_k / (dxi deta) of the bilinear shape functions (element.py:748-755).
+__device__ __forceinline__ float mixed2(int k) { return (k & 1) ? -0.25f : 0.25f; }
+
+// Material tangent D (symmetric) applied to a 2D [xx, yy, -, xy] vector.
+template <typename T>
+__device__ __forceinline__ void d_apply(const Mat2& m, T E_p, const T in[4], T out[4]) {
+  if (m.kind == DMPM_SIMPLE_MATERIAL) {
+    const T E = T(m.de[0]);
+    for (int k = 0; k < 4; ++k) out[k] = in[k] * E;
+    return;
+  }
+  T a1 = T(m.de[0]), a2 = T(m.de[1]), G = T(m.de[2]);
+  if (m.per_E) {
+    a1 = a1 * E_p;
+    a2 = a2 * E_p;
+    G = G * E_p;
+  }
+  out[0] = a1 * in[0] + a2 * in[1];
+  out[1] = a2 * in[0] + a1 * in[1];
+  out[2] = T(0);
+  out[3] = G * in[3];
+}
+
+// Adjoint of the strain/volume/stress update (particle.py:249-327) at one
+// particle with nodal velocities vx, vy: given the cotangents sb (sigma'
+// xx, yy, xy) and Vb (V'), adds sigma' . dsigma to lebar (dL/dlogE: D is
+// linear in E), returns the strain-rate cotangents Lb[6] (L00, L11, L01, L10
+// at the particle, L00, L11 at the centroid), the natural-coordinate
+// cotangent xb[2] and the cotangent of the step-start volume.
+template <typename T>
+__device__ __forceinline__ void stress_adjoint(const Mat2& mat, T E_p, T dt, T V_in,
+                                               const PGeo<T>& q, const T vx[4], const T vy[4],
+                                               const T sb[3], T Vb, T* lebar, T Lb[6], T xb[2],
+                                               T& Vb_in) {
+  T rr[6], rc[6], d[6];
+  rate2s(q.ix, q.iy, q.xi, q.eta, vx, vy, rr);
+  rate2s(q.ix, q.iy, T(0), T(0), vx, vy, rc);
+  for (int k = 0; k < 6; ++k) d[k] = rr[k] * dt;
+  const T dvol = dt * (rc[0] + rc[1]);
+  T ds[4];
+  stress_inc(mat, E_p, d, ds);
+  *lebar = *lebar + (sb[0] * ds[0] + sb[1] * ds[1] + sb[2] * ds[3]);
+  const T sg[4] = {sb[0], sb[1], T(0), sb[2]};
+  T de[4];
+  d_apply(mat, E_p, sg, de);
+  Lb[0] = rr[0] != T(0) ? dt * de[0] : T(0);
+  Lb[1] = rr[1] != T(0) ? dt * de[1] : T(0);
+  Lb[2] = rr[3] != T(0) ? dt * de[3] : T(0);
+  Lb[3] = Lb[2];
+  const T dvb = Vb * V_in;
+  Lb[4] = rc[0] != T(0) ? dt * dvb : T(0);
+  Lb[5] = rc[1] != T(0) ? dt * dvb : T(0);
+  Vb_in = Vb * (T(1) + dvol);
+  // L00 = sum dx_k vx_k, L01 = sum dx_k vy_k depend on eta; L11, L10 on xi
+  T sxi = T(0), seta = T(0);
+  for (int k = 0; k < 4; ++k) {
+    const T s = T(mixed2(k));
+    sxi = sxi + s * (Lb[1] * vy[k] + Lb[3] * vx[k]);
+    seta = seta + s * (Lb[0] * vx[k] + Lb[2] * vy[k]);
+  }
+  xb[0] = sxi * q.iy;
+  xb[1] = seta * q.ix;
+}
+
+// G2P adjoint (particle.py:208-247), particle-local part; USL also the
+// adjoint of the stress update made with the updated nodal velocities
+// (scheme.py:127-133).
+template <typename T, bool USL>
+__global__ __launch_bounds__(kThreads) void k_adj_g2p(Grid2<T> g, const T* __restrict__ pb,
+                                                      int64_t ld, int n, const T* __restrict__ nb,
+                                                      int64_t nld, T* __restrict__ cot,
+                                                      T* __restrict__ wk, T* __restrict__ lebar,
+                                                      Mat2 mat, T dt, int vu) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const PGeo<T> q = pgeo(g, PR(DMPM_P_X)[p], PR(DMPM_P_X + 1)[p]);
+  if (!q.in) {
+    for (int r = 0; r < DMPM_C_ROWS; ++r) CR(r)[p] = T(0);
+    for (int r = 0; r < W_ROWS; ++r) WR(r)[p] = T(0);
+    return;
+  }
+  int ids[4];
+  cell_nodes(g.nx, q.I, q.J, ids);
+  const T xbx = CR(DMPM_C_X)[p], xby = CR(DMPM_C_X + 1)[p];
+  const T vbx = CR(DMPM_C_V)[p], vby = CR(DMPM_C_V + 1)[p];
+  T vx[4], vy[4];
+  T sxi = T(0), seta = T(0);
+  for (int k = 0; k < 4; ++k) {
+    vx[k] = NR(DMPM_N_V)[ids[k]];
+    vy[k] = NR(DMPM_N_V + 1)[ids[k]];
+    T s = dt * (vx[k] * xbx + vy[k] * xby);
+    if (vu) s = s + (vx[k] * vbx + vy[k] * vby);
+    else s = s + dt * (NR(DMPM_N_A)[ids[k]] * vbx + NR(DMPM_N_A + 1)[ids[k]] * vby);
+    T gx, gy;
+    q4_dNdxi(k, q.xi, q.eta, gx, gy);
+    sxi = sxi + gx * s;
+    seta = seta + gy * s;
+  }
+  WR(W_NVX)[p] = vu ? dt * xbx + vbx : dt * xbx;
+  WR(W_NVY)[p] = vu ? dt * xby + vby : dt * xby;
+  WR(W_NAX)[p] = vu ? T(0) : dt * vbx;
+  WR(W_NAY)[p] = vu ? T(0) : dt * vby;
+  if (vu) {
+    CR(DMPM_C_V)[p] = T(0);
+    CR(DMPM_C_V + 1)[p] = T(0);
+  }
+  T Lb[6] = {T(0), T(0), T(0), T(0), T(0), T(0)};
+  if (USL) {
+    const T sb[3] = {CR(DMPM_C_SXX)[p], CR(DMPM_C_SYY)[p], CR(DMPM_C_SXY)[p]};
+    T xb[2], Vb_in, unused = T(0);
+    stress_adjoint(mat, mat.per_E ? PR(DMPM_P_YOUNGS)[p] : T(0), dt, PR(DMPM_P_VOLUME)[p], q, vx,
+                   vy, sb, CR(DMPM_C_VOL)[p], lebar ? lebar + p : &unused, Lb, xb, Vb_in);
+    CR(DMPM_C_VOL)[p] = Vb_in;
+    sxi = sxi + xb[0];
+    seta = seta + xb[1];
+  }
+  for (int k = 0; k < 6; ++k) WR(W_L00 + k)[p] = Lb[k];
+  WR(W_XI)[p] = sxi;
+  WR(W_ETA)[p] = seta;
+}
+
+// Force P2G adjoint (element.py:236-268, 842-901): gathers the nodal force
+// cotangent; USF then the adjoint of the stress update made with the
+// kinematic nodal velocities.
+template <typename T, bool USF>
+__global__ __launch_bounds__(kThreads) void k_adj_force(
+    Grid2<T> g, const T* __restrict__ pb, const T* __restrict__ pw, int64_t ld, int n,
+    const T* __restrict__ na, const T* __restrict__ vkin, T* __restrict__ cot,
+    T* __restrict__ wk, T* __restrict__ lebar, Mat2 mat, T Gx, T Gy, T dt) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const PGeo<T> q = pgeo(g, PR(DMPM_P_X)[p], PR(DMPM_P_X + 1)[p]);
+  if (!q.in) return;
+  int ids[4];
+  cell_nodes(g.nx, q.I, q.J, ids);
+  // the stress and volume the force P2G used: after the USF update, or the
+  // step-start ones (USL)
+  const T* __restrict__ src = USF ? pw : pb;
+  const T V = src[(int64_t)DMPM_P_VOLUME * ld + p];
+  const T sxx = src[(int64_t)(DMPM_P_STRESS + 0) * ld + p];
+  const T syy = src[(int64_t)(DMPM_P_STRESS + 1) * ld + p];
+  const T sxy = src[(int64_t)(DMPM_P_STRESS + 3) * ld + p];
+  const T m = PR(DMPM_P_MASS)[p];
+  T Vb = T(0), sbxx = T(0), sbyy = T(0), sbxy = T(0), sxi = T(0), seta = T(0);
+  for (int k = 0; k < 4; ++k) {
+    const T fx = AR(A_FX)[ids[k]], fy = AR(A_FY)[ids[k]];
+    T gx, gy, dx, dy;
+    q4_dNdxi(k, q.xi, q.eta, gx, gy);
+    dx = gx * q.ix;
+    dy = gy * q.iy;
+    const T body = m * (Gx * fx + Gy * fy);
+    sxi = sxi + gx * body;
+    seta = seta + gy * body;
+    Vb = Vb - (fx * (dx * sxx + dy * sxy) + fy * (dy * syy + dx * sxy));
+    sbxx = sbxx + fx * dx;
+    sbyy = sbyy + fy * dy;
+    sbxy = sbxy + (fx * dy + fy * dx);
+    const T s = T(mixed2(k));
+    sxi = sxi - V * (s * q.iy) * (fx * sxy + fy * syy);
+    seta = seta - V * (s * q.ix) * (fx * sxx + fy * sxy);
+  }
+  const T sb[3] = {CR(DMPM_C_SXX)[p] - V * sbxx, CR(DMPM_C_SYY)[p] - V * sbyy,
+                   CR(DMPM_C_SXY)[p] - V * sbxy};
+  const T Vbt = CR(DMPM_C_VOL)[p] + Vb;
+  CR(DMPM_C_SXX)[p] = sb[0];
+  CR(DMPM_C_SYY)[p] = sb[1];
+  CR(DMPM_C_SXY)[p] = sb[2];
+  if (USF) {
+    T vx[4], vy[4];
+    for (int k = 0; k < 4; ++k) {
+      vx[k] = vkin[ids[k]];
+      vy[k] = vkin[g.nnodes + ids[k]];
+    }
+    T Lb[6], xb[2], Vb_in;
+    T dummy = T(0);
+    stress_adjoint(mat, mat.per_E ? PR(DMPM_P_YOUNGS)[p] : T(0), dt, PR(DMPM_P_VOLUME)[p], q, vx,
+                   vy, sb, Vbt, lebar ? lebar + p : &dummy, Lb, xb, Vb_in);
+    CR(DMPM_C_VOL)[p] = Vb_in;
+    for (int k = 0; k < 6; ++k) WR(W_L00 + k)[p] = Lb[k];
+    for (int k = 0; k < 4; ++k) WR(W_NVX + k)[p] = T(0);
+    sxi = sxi + xb[0];
+    seta = seta + xb[1];
+  } else {
+    CR(DMPM_C_VOL)[p] = Vbt;
+  }
+  WR(W_XI)[p] = WR(W_XI)[p] + sxi;
+  WR(W_ETA)[p] = WR(W_ETA)[p] + seta;
+}
+
+// P2G of mass and momentum adjoint (element.py:122-189): gathers the nodal
+// momentum and mass cotangents; the natural-coordinate cotangent goes to x.
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_adj_kin(Grid2<T> g, const T* __restrict__ pb,
+                                                      int64_t ld, int n,
+                                                      const T* __restrict__ na,
+                                                      T* __restrict__ cot, T* __restrict__ wk) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const PGeo<T> q = pgeo(g, PR(DMPM_P_X)[p], PR(DMPM_P_X + 1)[p]);
+  if (!q.in) return;
+  int ids[4];
+  cell_nodes(g.nx, q.I, q.J, ids);
+  const T m = PR(DMPM_P_MASS)[p];
+  const T vx = PR(DMPM_P_V)[p], vy = PR(DMPM_P_V + 1)[p];
+  T N[4];
+  q4_N4(q.xi, q.eta, N);
+  T sxi = WR(W_XI)[p], seta = WR(W_ETA)[p];
+  T vbx = T(0), vby = T(0);
+  for (int k = 0; k < 4; ++k) {
+    const T px = AR(A_PX)[ids[k]], py = AR(A_PY)[ids[k]], mb = AR(A_M)[ids[k]];
+    vbx = vbx + N[k] * px;
+    vby = vby + N[k] * py;
+    T gx, gy;
+    q4_dNdxi(k, q.xi, q.eta, gx, gy);
+    const T s = m * (vx * px + vy * py) + m * mb;
+    sxi = sxi + gx * s;
+    seta = seta + gy * s;
+  }
+  CR(DMPM_C_V)[p] = CR(DMPM_C_V)[p] + m * vbx;
+  CR(DMPM_C_V + 1)[p] = CR(DMPM_C_V + 1)[p] + m * vby;
+  CR(DMPM_C_X)[p] = CR(DMPM_C_X)[p] + sxi * q.ix;
+  CR(DMPM_C_X + 1)[p] = CR(DMPM_C_X + 1)[p] + seta * q.iy;
+}
+
+// Cell pass of a cotangent scatter to the nodes: per corner k of a
+// particle's cell, 4 quantities
+//   q0 = N_k w_nvx + dNk/dx L00b + dNk/dy L10b + dNk/dx(0) C00b
+//   q1 = N_k w_nvy + dNk/dy L11b + dNk/dx L01b + dNk/dy(0) C11b
+//   q2 = N_k w_nax,  q3 = N_k w_nay
+// summed per cell in slot order into partials[16][ncells] (node_gather then
+// adds the four cells of a node in the canonical order).
+constexpr int kSlotsA = 512;
+template <typename T, int CB>
+__global__ __launch_bounds__(kThreads) void k_adj_cell(Grid2<T> g, const T* __restrict__ pb,
+                                                       int64_t ld, SortRef sr,
+                                                       const T* __restrict__ wk,
+                                                       T* __restrict__ part) {
+  constexpr int NIN = 12;  // xi, eta, 10 weights
+  __shared__ T st[NIN][kSlotsA];
+  const int* __restrict__ perm = sr.perm();
+  const int* __restrict__ cs = sr.cs();
+  const int ident = sr.flags[1];
+  const int c0 = blockIdx.x * CB, t = threadIdx.x;
+  const int cend = min(c0 + CB, g.ncells);
+  const int mycell = c0 + t;
+  const bool celler = t < CB && mycell < cend;
+  const int ms0 = celler ? cs[mycell] : 0, ms1 = celler ? cs[mycell + 1] : 0;
+  const int s0 = cs[c0], s1 = cs[cend];
+  T acc[16];
+  for (int v = 0; v < 16; ++v) acc[v] = T(0);
+  for (int k0 = s0; k0 < s1; k0 += kSlotsA) {
+    const int kend = min(k0 + kSlotsA, s1);
+    for (int s = k0 + t; s < kend; s += kThreads) {
+      const int p = ident ? s : perm[s];
+      const int qq = s - k0;
+      const PGeo<T> q = pgeo(g, PR(DMPM_P_X)[p], PR(DMPM_P_X + 1)[p]);
+      st[0][qq] = q.xi;
+      st[1][qq] = q.eta;
+      for (int r = 0; r < 10; ++r) st[2 + r][qq] = WR(r)[p];
+    }
+    __syncthreads();
+    if (celler) {
+      const int I = mycell % g.nx, J = mycell / g.nx;
+      const T X0 = g.X(I), X1 = g.X(I + 1), Y0 = g.Y(J), Y1 = g.Y(J + 1);
+      const int lo = max(ms0, k0) - k0, hi = min(ms1, kend) - k0;
+      for (int qq = lo; qq < hi; ++qq) {
+        const T xi = st[0][qq], eta = st[1][qq];
+        T ix, iy;
+        q4_invJ(xi, eta, X0, X1, Y0, Y1, ix, iy);
+        T N[4];
+        q4_N4(xi, eta, N);
+#define WQ(r) st[2 + (r)][qq]
+        for (int k = 0; k < 4; ++k) {
+          T dx, dy, cx, cy;
+          q4_dNdx(k, xi, eta, ix, iy, dx, dy);
+          q4_dNdx(k, T(0), T(0), ix, iy, cx, cy);
+          const T q0 = ((N[k] * WQ(W_NVX) + dx * WQ(W_L00)) + dy * WQ(W_L10)) + cx * WQ(W_C00);
+          const T q1 = ((N[k] * WQ(W_NVY) + dy * WQ(W_L11)) + dx * WQ(W_L01)) + cy * WQ(W_C11);
+          acc[4 * k + 0] = acc[4 * k + 0] + q0;
+          acc[4 * k + 1] = acc[4 * k + 1] + q1;
+          acc[4 * k + 2] = acc[4 * k + 2] + N[k] * WQ(W_NAX);
+          acc[4 * k + 3] = acc[4 * k + 3] + N[k] * WQ(W_NAY);
+        }
+#undef WQ
+      }
+    }
+    __syncthreads();
+  }
+  if (celler)
+    for (int v = 0; v < 16; ++v) part[(int64_t)v * g.ncells + mycell] = acc[v];
+}
+
+// Kinematics adjoint at one node and direction (element.py:191-202 with
+// constraint.py:29-44): v = snap(snap(P)/m) (or the constraint value).
+template <typename T>
+__device__ __forceinline__ void kin_adjoint(T m, T v, bool constrained, T vb, T& Pb, T& mb) {
+  Pb = T(0);
+  if (constrained || m == T(0) || v == T(0)) return;
+  Pb = vb / m;
+  mb = mb - vb * v / m;
+}
+
+// Node pass 1: nodal-velocity/acceleration cotangents of G2P, adjoint of the
+// nodal update (element.py:305-329): F = f_int + f_ext + f_damp,
+// a0 = nan_to_num(F/m), v = snap(v_kin + a0 dt), a = snap(a0), constraints.
+// USL: the kinematic velocity's cotangent is then complete and the
+// kinematics adjoint follows.
+template <typename T, bool USL>
+__global__ __launch_bounds__(kThreads) void k_adj_node1(
+    Grid2<T> g, const T* __restrict__ part, const T* __restrict__ nb, int64_t nld,
+    const T* __restrict__ vkin, T* __restrict__ na, const int* __restrict__ cidx,
+    const int* __restrict__ cmask, T dt) {
+  const int nd = blockIdx.x * blockDim.x + threadIdx.x;
+  if (nd >= g.nnodes) return;
+  const int i = nd % (g.nx + 1), j = nd / (g.nx + 1);
+  T acc[4];
+  node_gather<T, 4>(g, part, i, j, acc);
+  const int cr = cidx ? cidx[nd] : -1;
+  const int mask = cr >= 0 ? cmask[cr] : 0;
+  const T m = NR(DMPM_N_MASS)[nd];
+  T mb = T(0);
+  for (int d = 0; d < 2; ++d) {
+    T Fb = T(0), vkb = T(0);
+    if (!(mask & (1 << d))) {
+      const T F = (NR(DMPM_N_FINT + d)[nd] + NR(DMPM_N_FEXT + d)[nd]) + NR(DMPM_N_FDAMP + d)[nd];
+      const T r = F / m;
+      const T v1b = NR(DMPM_N_V + d)[nd] != T(0) ? acc[d] : T(0);
+      const T a0b = (NR(DMPM_N_A + d)[nd] != T(0) ? acc[2 + d] : T(0)) + dt * v1b;
+      if (isfinite(r)) {
+        Fb = a0b / m;
+        mb = mb - a0b * r / m;
+      }
+      vkb = v1b;
+    }
+    AR(A_FX + d)[nd] = Fb;
+    if (USL) {
+      T Pb;
+      kin_adjoint(m, vkin[(int64_t)d * g.nnodes + nd], (mask >> d) & 1, vkb, Pb, mb);
+      AR(A_PX + d)[nd] = Pb;
+    } else {
+      AR(A_VKX + d)[nd] = vkb;
+    }
+  }
+  AR(A_M)[nd] = mb;
+}
+
+// Node pass 2 (USF): the kinematic velocity's cotangent from the nodal update
+// plus the strain-rate scatter; kinematics adjoint.
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_adj_node2(Grid2<T> g, const T* __restrict__ part,
+                                                        const T* __restrict__ nb, int64_t nld,
+                                                        const T* __restrict__ vkin,
+                                                        T* __restrict__ na,
+                                                        const int* __restrict__ cidx,
+                                                        const int* __restrict__ cmask) {
+  const int nd = blockIdx.x * blockDim.x + threadIdx.x;
+  if (nd >= g.nnodes) return;
+  const int i = nd % (g.nx + 1), j = nd / (g.nx + 1);
+  T acc[4];
+  node_gather<T, 4>(g, part, i, j, acc);
+  const int cr = cidx ? cidx[nd] : -1;
+  const int mask = cr >= 0 ? cmask[cr] : 0;
+  const T m = NR(DMPM_N_MASS)[nd];
+  T mb = AR(A_M)[nd];
+  for (int d = 0; d < 2; ++d) {
+    const T vb = AR(A_VKX + d)[nd] + acc[d];
+    T Pb;
+    kin_adjoint(m, vkin[(int64_t)d * g.nnodes + nd], (mask >> d) & 1, vb, Pb, mb);
+    AR(A_PX + d)[nd] = Pb;
+  }
+  AR(A_M)[nd] = mb;
+}
+
+template <typename T, int CB>
+int adjoint_t(dmpm_plan* P, const T* pb, T* pw, int32_t* eid, T* nb, T* cot, T* lebar,
+              int scheme, int vu, const double* grav, double dt, int64_t step, hipStream_t s) {
+  const Grid2<T> g = grid_of<T>(P);
+  const int64_t ld = P->d.ld, nld = P->d.nld;
+  const int n = (int)P->n;
+  const bool usf = scheme == DMPM_USF;
+  DMPM_TRY(hip_check(hipMemcpyAsync(pw, pb, (size_t)DMPM_P_ROWS * ld * sizeof(T),
+                                    hipMemcpyDeviceToDevice, s),
+                     "hipMemcpyAsync"));
+  DMPM_TRY(tile_run(P, pw, eid, nb, scheme, vu, grav, dt, step, 1, 0, s));
+  ProfScopeT ps_(P, KC_ADJ, s);
+  T* part = (T*)plan_workspace(P, 0, (size_t)16 * g.ncells * sizeof(T));
+  const T* vkin = (const T*)plan_workspace(P, 1, (size_t)2 * P->nnodes * sizeof(T));
+  T* wk = (T*)plan_workspace(P, 3, (size_t)W_ROWS * ld * sizeof(T));
+  T* na = (T*)plan_workspace(P, 4, (size_t)A_ROWS * P->nnodes * sizeof(T));
+  if (!part || !vkin || !wk || !na) return fail(DMPM_EHIP, "workspace allocation failed");
+  Mat2 mat;
+  mat.kind = P->d.material;
+  mat.per_E = P->d.per_particle_E;
+  for (int k = 0; k < 3; ++k) mat.de[k] = P->d.de[k];
+  const T tdt = T(dt), Gx = T(grav[0]), Gy = T(grav[1]);
+  const SortRef sr = sort_ref(P);
+  const unsigned np = nblk(n), nn = nblk(P->nnodes);
+  const unsigned ncb = (unsigned)((g.ncells + CB - 1) / CB);
+  if (usf)
+    k_adj_g2p<T, false><<<np, kThreads, 0, s>>>(g, pb, ld, n, nb, nld, cot, wk, lebar, mat, tdt,
+                                                 vu);
+  else
+    k_adj_g2p<T, true><<<np, kThreads, 0, s>>>(g, pb, ld, n, nb, nld, cot, wk, lebar, mat, tdt,
+                                                vu);
+  k_adj_cell<T, CB><<<ncb, kThreads, 0, s>>>(g, pb, ld, sr, wk, part);
+  if (usf)
+    k_adj_node1<T, false><<<nn, kThreads, 0, s>>>(g, part, nb, nld, vkin, na, P->cidx, P->cmask,
+                                                   tdt);
+  else
+    k_adj_node1<T, true><<<nn, kThreads, 0, s>>>(g, part, nb, nld, vkin, na, P->cidx, P->cmask,
+                                                  tdt);
+  if (usf)
+    k_adj_force<T, true><<<np, kThreads, 0, s>>>(g, pb, pw, ld, n, na, vkin, cot, wk, lebar, mat,
+                                                  Gx, Gy, tdt);
+  else
+    k_adj_force<T, false><<<np, kThreads, 0, s>>>(g, pb, pw, ld, n, na, vkin, cot, wk, lebar,
+                                                   mat, Gx, Gy, tdt);
+  if (usf) {
+    k_adj_cell<T, CB><<<ncb, kThreads, 0, s>>>(g, pb, ld, sr, wk, part);
+    k_adj_node2<T><<<nn, kThreads, 0, s>>>(g, part, nb, nld, vkin, na, P->cidx, P->cmask);
+  }
+  k_adj_kin<T><<<np, kThreads, 0, s>>>(g, pb, ld, n, na, cot, wk);
+  return launch_check("step_adjoint");
+}
+
+#undef PR
+#undef PW
+#undef NR
+#undef CR
+#undef WR
+#undef AR
+
+}  // namespace
+
+extern "C" int dmpm_step_adjoint(dmpm_plan* P, const void* pin, void* pwork, int32_t* eid_work,
+                                 void* nwork, void* cot, void* lebar, int32_t scheme,
+                                 int32_t velocity_update, const double* gravity, double dt,
+                                 int64_t step, void* stream) {
+  if (!P) return fail(DMPM_EINVAL, "null plan");
+  if (!pin || !pwork || !eid_work || !nwork || !cot || !gravity)
+    return fail(DMPM_EINVAL, "null buffer");
+  if (scheme != DMPM_USF && scheme != DMPM_USL)
+    return fail(DMPM_EINVAL, "scheme must be DMPM_USF or DMPM_USL");
+  if (!tile_eligible(P, 0))
+    return fail(DMPM_EINVAL,
+                "the adjoint step covers 2D Quadrilateral4Node meshes without particle "
+                "tractions or particle external forces");
+  if (P->d.material != DMPM_LINEAR_ELASTIC && P->d.material != DMPM_SIMPLE_MATERIAL)
+    return fail(DMPM_EINVAL, "unknown material");
+  hipStream_t s = (hipStream_t)stream;
+  if (P->d.dtype == DMPM_F32)
+    return adjoint_t<float, 128>(P, (const float*)pin, (float*)pwork, eid_work, (float*)nwork,
+                                 (float*)cot, (float*)lebar, scheme, velocity_update, gravity, dt,
+                                 step, s);
+  return adjoint_t<double, 128>(P, (const double*)pin, (double*)pwork, eid_work, (double*)nwork,
+                                (double*)cot, (double*)lebar, scheme, velocity_update, gravity,
+                                dt, step, s);
+}

@@ -223,6 +223,36 @@ int dmpm_run(dmpm_plan* plan, void* pbuf, int32_t* eid, void* nbuf, int32_t sche
              int32_t velocity_update, const double* gravity, double dt, int64_t step0,
              int64_t nsteps, int32_t flags, void* stream);
 
+/* ---- adjoint of the fused 2D step ------------------------------------------
+ * The reference differentiates a rollout with jax.grad (no hand-written
+ * adjoint; diffmpm/solver.py:179-213).  This is the reverse-mode
+ * step of dmpm_run's fused 2D path for the dynamic particle state, with the
+ * cotangent rows below (block [DMPM_C_ROWS][ld], same element type as the
+ * particle block).  Mass, mesh, constraints and nodal forces are held fixed;
+ * strain, density, volumetric strain and sigma_zz do not feed back into the
+ * 2D dynamics and carry no cotangent. */
+enum {
+  DMPM_C_X = 0,    /* 2 rows: x, y */
+  DMPM_C_V = 2,    /* 2 rows: vx, vy */
+  DMPM_C_SXX = 4,
+  DMPM_C_SYY = 5,
+  DMPM_C_SXY = 6,
+  DMPM_C_VOL = 7,
+  DMPM_C_ROWS = 8
+};
+/* pin: particle block at the start of step `step` (read only); pwork, nwork,
+ * eid_work: scratch particle block, node block and element ids (the forward
+ * step is recomputed there); cot: in = cotangent of the state after the
+ * step, out = cotangent of the state before it; lebar (may be NULL): [ld],
+ * += dL/dlog(E_p) (per-particle E) or its per-particle terms (scalar E: sum
+ * them).  Same restrictions as the fused path (2D Q4, no particle tractions
+ * or particle f_ext).  Uses the plan's sort state: use a separate plan from
+ * the forward rollout's to keep that one's incremental re-sort. */
+int dmpm_step_adjoint(dmpm_plan* plan, const void* pin, void* pwork, int32_t* eid_work,
+                      void* nwork, void* cot, void* lebar, int32_t scheme,
+                      int32_t velocity_update, const double* gravity, double dt, int64_t step,
+                      void* stream);
+
 /* ---- measurement (no reference counterpart) ------------------------------- */
 /* Kernel classes for dmpm_timing_enable. */
 enum {

@@ -1,0 +1,188 @@
+"""Adjoint of the fused 2D step (dmpm_step_adjoint via diffmpm_amd.autodiff).
+
+The reference has no gradient test (it relies on jax.grad), so the adjoint is
+pinned by central finite differences of the FP64 forward rollout: per-particle
+log E on 16 seeded particles (relative step 1e-3, as in the inverse-problem
+configuration), and directional derivatives along seeded random directions of
+the whole initial state and of all log E.  A rollout is short and slow enough
+that no particle changes cell under the perturbations (checked), so the loss
+is smooth there.
+"""
+import math
+
+import numpy as np
+import pytest
+import torch
+
+import cases
+
+pytestmark = pytest.mark.gpu
+
+NSTEPS = 4
+FD_H_LOGE = 1e-3
+RTOL_FD = 2e-5
+
+
+def _sim(scheme, vu, dtype=torch.float64, seed=3):
+    spec = cases.synthetic_spec("quad4", nx=10, ny=8, rows=5, seed=seed, scheme=scheme,
+                                velocity_update=vu, traction=False, nodal_force=True, E=1e6,
+                                dt=1e-5, nsteps=NSTEPS, per_particle_E=True, velocity_scale=0.5)
+    return cases.build_gpu(spec, dtype, keep_temporaries=False), spec
+
+
+def _weights(state0, seed=5):
+    """Seeded loss weights, each state row scaled to O(1) contributions."""
+    g = torch.Generator(device="cpu").manual_seed(seed)
+    w = torch.randn(state0.shape, generator=g, dtype=torch.float64).to(state0.device)
+    scale = state0.abs().amax(dim=1, keepdim=True).clamp_min(1.0)
+    return (w / scale).to(state0.dtype)
+
+
+def _setup(scheme, vu, dtype=torch.float64):
+    from diffmpm_amd.autodiff import Differentiable
+    sim, spec = _sim(scheme, vu, dtype)
+    d = Differentiable(sim)
+    g = torch.tensor(spec["gravity"], dtype=torch.float64)
+    state0 = d.get_state()
+    # give the stress and volume rows non-trivial values so that every term
+    # of the step carries a gradient
+    gen = torch.Generator(device="cpu").manual_seed(11)
+    noise = torch.randn(state0.shape, generator=gen, dtype=torch.float64).to(state0.device)
+    state0[4:7] = (2e3 * noise[4:7]).to(dtype)
+    state0[7] = state0[7] * (1 + 0.05 * noise[7]).to(dtype)
+    log_e = torch.log(d.ps.youngs_modulus.detach().clone())
+    return d, g, state0, log_e
+
+
+def _loss(d, g, state0, log_e, w):
+    from diffmpm_amd.autodiff import rollout
+    with torch.no_grad():
+        out = rollout(d, NSTEPS, g, state=state0, log_e=log_e)
+    return float((w * out).sum())
+
+
+def _cells(d, state):
+    h = d.el._hx
+    return torch.floor(state[0:2].double() / h).long()
+
+
+def _grad(d, g, state0, log_e, w, checkpoint_every=0):
+    from diffmpm_amd.autodiff import rollout
+    s = state0.clone().requires_grad_(True)
+    le = log_e.clone().requires_grad_(True)
+    out = rollout(d, NSTEPS, g, state=s, log_e=le, checkpoint_every=checkpoint_every)
+    (w * out).sum().backward()
+    return s.grad.detach(), le.grad.detach(), out.detach()
+
+
+@pytest.mark.parametrize("scheme,vu", [("usf", False), ("usl", False), ("usf", True),
+                                       ("usl", True)])
+def test_log_e_gradient_matches_central_differences(scheme, vu):
+    d, g, state0, log_e = _setup(scheme, vu)
+    w = _weights(state0)
+    gs, ge, out = _grad(d, g, state0, log_e, w)
+    assert torch.equal(_cells(d, state0), _cells(d, out)), "a particle changed cell"
+    rng = np.random.Generator(np.random.PCG64(0))
+    pids = sorted(rng.choice(d.n, size=16, replace=False).tolist())
+    fd = []
+    for p in pids:
+        lp, lm = log_e.clone(), log_e.clone()
+        lp[p] += FD_H_LOGE
+        lm[p] -= FD_H_LOGE
+        fd.append((_loss(d, g, state0, lp, w) - _loss(d, g, state0, lm, w)) / (2 * FD_H_LOGE))
+    fd = torch.tensor(fd, dtype=torch.float64)
+    ad = ge[pids].double().cpu()
+    scale = fd.abs().max()
+    assert scale > 0
+    assert torch.allclose(ad, fd, rtol=RTOL_FD, atol=RTOL_FD * scale), (ad, fd)
+
+
+@pytest.mark.parametrize("scheme,vu", [("usf", False), ("usl", False), ("usf", True),
+                                       ("usl", True)])
+def test_directional_derivatives_match_central_differences(scheme, vu):
+    d, g, state0, log_e = _setup(scheme, vu)
+    w = _weights(state0)
+    gs, ge, _ = _grad(d, g, state0, log_e, w)
+    gen = torch.Generator(device="cpu").manual_seed(7)
+    # direction in the state: each row scaled to its magnitude; positions by a
+    # small absolute amount so that no particle leaves its cell
+    u = torch.randn(state0.shape, generator=gen, dtype=torch.float64).to(state0.device)
+    row = state0.abs().amax(dim=1, keepdim=True).clamp_min(1.0)
+    u = u * row
+    u[0:2] = u[0:2] / row[0:2] * 1e-3 * d.el._hx
+    h = 1e-4
+    lp = _loss(d, g, state0 + h * u, log_e, w)
+    lm = _loss(d, g, state0 - h * u, log_e, w)
+    fd = (lp - lm) / (2 * h)
+    ad = float((gs.double() * u).sum())
+    assert abs(fd) > 1e-6
+    assert math.isclose(ad, fd, rel_tol=RTOL_FD, abs_tol=1e-12), (ad, fd)
+    ue = torch.randn(log_e.shape, generator=gen, dtype=torch.float64).to(log_e.device)
+    lp = _loss(d, g, state0, log_e + FD_H_LOGE * ue, w)
+    lm = _loss(d, g, state0, log_e - FD_H_LOGE * ue, w)
+    fd = (lp - lm) / (2 * FD_H_LOGE)
+    ad = float((ge.double() * ue).sum())
+    assert abs(fd) > 1e-6
+    assert math.isclose(ad, fd, rel_tol=RTOL_FD, abs_tol=1e-12), (ad, fd)
+
+
+@pytest.mark.parametrize("scheme,vu", [("usf", False), ("usl", True)])
+def test_every_state_component_matches_central_differences(scheme, vu):
+    """d loss / d state[r, p] for every row r on 4 particles, one central
+    difference each (steps relative to the row's magnitude; positions by a
+    small absolute amount)."""
+    d, g, state0, log_e = _setup(scheme, vu)
+    w = _weights(state0)
+    gs, _, _ = _grad(d, g, state0, log_e, w)
+    rng = np.random.Generator(np.random.PCG64(1))
+    pids = sorted(rng.choice(d.n, size=4, replace=False).tolist())
+    row = state0.abs().amax(dim=1)
+    for r in range(8):
+        h = 1e-6 * d.el._hx if r < 2 else 1e-5 * float(row[r].clamp_min(1.0))
+        for p in pids:
+            sp, sm = state0.clone(), state0.clone()
+            sp[r, p] += h
+            sm[r, p] -= h
+            fd = (_loss(d, g, sp, log_e, w) - _loss(d, g, sm, log_e, w)) / (2 * h)
+            ad = float(gs[r, p])
+            assert math.isclose(ad, fd, rel_tol=1e-4, abs_tol=1e-7 * max(1.0, abs(fd))), \
+                (r, p, ad, fd)
+
+
+@pytest.mark.parametrize("scheme", ["usf", "usl"])
+def test_checkpointed_rollout_gives_identical_gradients(scheme):
+    d, g, state0, log_e = _setup(scheme, False)
+    w = _weights(state0)
+    a = _grad(d, g, state0, log_e, w, checkpoint_every=0)
+    b = _grad(d, g, state0, log_e, w, checkpoint_every=3)
+    c = _grad(d, g, state0, log_e, w, checkpoint_every=0)
+    for x, y, z in zip(a, b, c):
+        assert torch.equal(x, y)
+        assert torch.equal(x, z)
+
+
+@pytest.mark.parametrize("scheme", ["usf", "usl"])
+def test_fp32_adjoint_matches_fp64(scheme):
+    d64, g, s64, le64 = _setup(scheme, False, torch.float64)
+    d32, _, s32, le32 = _setup(scheme, False, torch.float32)
+    w = _weights(s64)
+    gs64, ge64, _ = _grad(d64, g, s64, le64, w)
+    gs32, ge32, _ = _grad(d32, g, s32, le32, w.float())
+    for a, b in ((gs32, gs64), (ge32, ge64)):
+        err = (a.double() - b).norm() / b.norm()
+        assert err < 1e-3, float(err)
+
+
+def test_scalar_log_e_gradient_is_the_sum_of_the_per_particle_terms():
+    d, g, state0, log_e = _setup("usl", False)
+    w = _weights(state0)
+    _, ge, _ = _grad(d, g, state0, log_e, w)
+    from diffmpm_amd.autodiff import rollout
+    le0 = torch.tensor(math.log(1e6), dtype=torch.float64, device=state0.device,
+                       requires_grad=True)
+    out = rollout(d, NSTEPS, g, state=state0.clone(), log_e=le0)
+    (w * out).sum().backward()
+    lp = _loss(d, g, state0, le0.detach() + FD_H_LOGE, w)
+    lm = _loss(d, g, state0, le0.detach() - FD_H_LOGE, w)
+    fd = (lp - lm) / (2 * FD_H_LOGE)
+    assert math.isclose(float(le0.grad), fd, rel_tol=RTOL_FD), (float(le0.grad), fd)
