@@ -41,6 +41,9 @@ KERNEL_BYTES = {
 # x(2), m, eps(3), sigma(4), V, rho, eps_v; write eps(3), sigma(4), V, rho, eps_v
 TILE_FORCE_FUSED_BYTES = 4 * (2 + 1 + 3 + 4 + 3) + 4 * (3 + 4 + 3)
 STEP_BYTES = 116  # per particle-step, whole step (FP32 2D LinearElastic)
+# adjoint step: read primal x(2), v(2), m, V, sigma(xx, yy, xy); read and write
+# the cotangents of x(2), v(2), V, sigma(xx, yy, xy); write dL/dlogE_p
+ADJOINT_BYTES = 4 * 9 + 4 * 8 * 2 + 4
 # HBM bytes per particle per launch of the default path at 1024x1024, from
 # rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE passes of this benchmark
 # (profiles/step_q4_1024_usf_hbm_counters.md); None until measured.
@@ -82,6 +85,48 @@ def cpu_baseline(rows=32, steps=8):
                       f"{steps} steps after 1 warm-up ({el_s:.1f} s)"}
 
 
+def fwd_adjoint(sim, gravity, K):
+    """Throughput of a differentiable rollout: K forward steps recorded for
+    autograd, then the backward pass (per step: recomputation of the forward
+    step and the adjoint kernels), per-particle log E as the parameter."""
+    import math
+    import torch
+    from diffmpm_amd.autodiff import Differentiable, rollout
+    from diffmpm_amd._plan import timing_enable, timing_read
+    d = Differentiable(sim)
+    N = d.n
+    log_e = torch.full((N,), math.log(1e6), dtype=d.ps.dtype, device=d.ps.device,
+                       requires_grad=True)
+    state0 = d.get_state().requires_grad_(True)
+    out = rollout(d, 1, gravity, state=state0, log_e=log_e)  # warm-up
+    out[0:2].sum().backward()
+    torch.cuda.synchronize()
+    timing_enable(d.aplan, "adj")
+    stream = torch.cuda.current_stream()
+    e0, e1, e2 = (torch.cuda.Event(enable_timing=True) for _ in range(3))
+    torch.cuda.synchronize()
+    e0.record(stream)
+    out = rollout(d, K, gravity, state=state0, log_e=log_e)
+    e1.record(stream)
+    out[0:2].sum().backward()
+    e2.record(stream)
+    torch.cuda.synchronize()
+    fwd_ms, bwd_ms = e0.elapsed_time(e1), e1.elapsed_time(e2)
+    adj_ms, adj_n = timing_read(d.aplan)
+    timing_enable(d.aplan, None)
+    assert torch.isfinite(log_e.grad).all() and torch.isfinite(state0.grad).all()
+    ms = (fwd_ms + bwd_ms) / K
+    adj_us = adj_ms / max(adj_n, 1) * 1e3
+    return {"metric": "particle-steps/s fwd+adjoint at 4.2M ptcl FP32",
+            "value": N / (ms / 1e3), "unit": "particle-steps/s", "steps": K,
+            "ms_per_step": ms, "forward_ms_per_step": fwd_ms / K,
+            "backward_ms_per_step": bwd_ms / K,
+            "adjoint_kernels_us_per_step": adj_us,
+            "adjoint_kernels_GBs": ADJOINT_BYTES * N / (adj_us / 1e6) / 1e9,
+            "note": "backward = forward recomputation from the saved step input + adjoint "
+                    "kernels; per-particle log E; loss = sum of final positions"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -94,6 +139,8 @@ def main():
                     help="cells per workgroup of the P2G cell pass (64/128/256; 0: default)")
     ap.add_argument("--usf-fused", action="store_true",
                     help="update stresses inside the force P2G cell pass")
+    ap.add_argument("--adjoint-steps", type=int, default=20,
+                    help="steps of the timed forward+adjoint rollout (0: skip)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="after the timed region, copy 512 MiB with a plain device copy "
@@ -167,6 +214,8 @@ def main():
                      "bytes_per_particle": kd["bytes_per_particle"], "launches": kd["launches"]},
         "kernels": kernels,
     }
+    if args.adjoint_steps:
+        line["fwd_adjoint"] = fwd_adjoint(sim, gravity, args.adjoint_steps)
     if args.copy_calibration:
         a = torch.empty(128 * 1024 * 1024, dtype=torch.float32, device="cuda")
         b = torch.empty_like(a)

@@ -62,9 +62,11 @@ class Differentiable:
         self.n = len(ps)
         self.fplan = plan_for(el, ps)
         self.aplan = Plan(el, ps)  # own sort state for the recomputed steps
-        self.pin = torch.empty_like(ps._buf)
-        self.pwork = torch.empty_like(ps._buf)
-        self.nwork = torch.empty_like(el.nodes._buf)
+        # step-input block (mass, E and the other constant rows set once),
+        # recomputation scratch blocks
+        self.pin = ps._buf.clone()
+        self.pwork = ps._buf.clone()
+        self.nwork = el.nodes._buf.clone()
         self.eid = torch.empty_like(ps._eid)
         self.scheme = sim.scheme
         self.code = _lib.DMPM_USF if sim.scheme == "usf" else _lib.DMPM_USL
@@ -95,10 +97,8 @@ class Differentiable:
         """Cotangents (state, log E per particle) of step i's input from the
         cotangent of its output."""
         self.aplan.refresh_tables()
-        self.pin.copy_(self.ps._buf)
         self._write(self.pin, state.detach(), log_e)
-        self.nwork.copy_(self.el.nodes._buf)
-        cot = torch.zeros((C_ROWS, self.ps._ld), dtype=self.ps.dtype, device=self.ps.device)
+        cot = torch.empty((C_ROWS, self.ps._ld), dtype=self.ps.dtype, device=self.ps.device)
         cot[:, : self.n] = gout
         lebar = torch.zeros(self.ps._ld, dtype=self.ps.dtype, device=self.ps.device)
         call("dmpm_step_adjoint", self.aplan.h, ptr(self.pin), ptr(self.pwork), ptr(self.eid),

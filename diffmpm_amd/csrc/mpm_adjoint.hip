@@ -455,9 +455,15 @@ int adjoint_t(dmpm_plan* P, const T* pb, T* pw, int32_t* eid, T* nb, T* cot, T* 
   const int64_t ld = P->d.ld, nld = P->d.nld;
   const int n = (int)P->n;
   const bool usf = scheme == DMPM_USF;
-  DMPM_TRY(hip_check(hipMemcpyAsync(pw, pb, (size_t)DMPM_P_ROWS * ld * sizeof(T),
-                                    hipMemcpyDeviceToDevice, s),
-                     "hipMemcpyAsync"));
+  // the rows the forward step reads: x, v, mass, volume, density, stress, E
+  const int spans[3][2] = {{DMPM_P_X, DMPM_P_DENSITY + 1},
+                           {DMPM_P_STRESS, DMPM_P_STRESS + 4},
+                           {DMPM_P_YOUNGS, DMPM_P_YOUNGS + 1}};
+  for (const auto& sp : spans)
+    DMPM_TRY(hip_check(hipMemcpyAsync(pw + (int64_t)sp[0] * ld, pb + (int64_t)sp[0] * ld,
+                                      (size_t)(sp[1] - sp[0]) * ld * sizeof(T),
+                                      hipMemcpyDeviceToDevice, s),
+                       "hipMemcpyAsync"));
   DMPM_TRY(tile_run(P, pw, eid, nb, scheme, vu, grav, dt, step, 1, 0, s));
   ProfScopeT ps_(P, KC_ADJ, s);
   T* part = (T*)plan_workspace(P, 0, (size_t)16 * g.ncells * sizeof(T));

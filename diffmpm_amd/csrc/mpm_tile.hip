@@ -576,26 +576,32 @@ __global__ __launch_bounds__(kThreads) void k_stress_tile(Grid2<T> g, T* __restr
 }
 
 // Element ids from the positions at the start of a run.  A particle whose cell
-// differs from its sort key (or, for a particle with a pending recorded cell
-// change, from the recorded new cell) was moved outside the fused step: the
-// next re-sort is then a full one.
+// differs from its sort key was moved outside the fused step (positions set
+// by the caller, a recomputed step): the change is recorded like one from
+// G2P, for the incremental re-sort.  A particle with a pending recorded
+// change whose recorded cell no longer holds, or one never sorted, makes the
+// next re-sort a full one.
 template <typename T>
 __global__ void k_tile_check(Grid2<T> g, const T* __restrict__ pb, int64_t ld, int n,
-                             int* __restrict__ eid, const int* __restrict__ key,
-                             int* __restrict__ flags, const int* __restrict__ mrec,
-                             int* __restrict__ mov) {
+                             int* __restrict__ eid, const int* __restrict__ key, MoveLog L) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= n) return;
   const int e = element_id<T, 2>(g.geo(), PR(DMPM_P_X)[p], PR(DMPM_P_X + 1)[p]);
-  eid[p] = e;
   const int k = key_of(e, g.ncells);
-  const int m = mrec[p] - 1;
+  const int m = L.mrec[p] - 1;
   if (m >= 0) {
-    int* r = mov + (int64_t)kMovRec * m;
+    int* r = L.mov + (int64_t)kMovRec * m;
     if (key_of(r[2], g.ncells) == k) r[2] = e;
-    else flags[0] = 1;
-  } else if (key[p] != k) {
-    flags[0] = 1;
+    else L.flags[0] = 1;
+    return;
+  }
+  const int ko = key[p];
+  if (ko == k) {
+    eid[p] = e;
+  } else if (ko < 0 || ko > g.ncells) {
+    L.flags[0] = 1;
+  } else {
+    record_move(L, g.ncells, p, ko, e);
   }
 }
 
@@ -1028,8 +1034,14 @@ int tile_run_t(dmpm_plan* P, T* pb, int32_t* eid, T* nb, int scheme, int vu, con
   T* part = (T*)plan_workspace(P, 0, (size_t)4 * NQ_FORCE * g.ncells * sizeof(T));
   T* vkin = (T*)plan_workspace(P, 1, (size_t)2 * P->nnodes * sizeof(T));
   if (!part || !vkin) return fail(DMPM_EHIP, "workspace allocation failed");
-  k_tile_check<T><<<nblk(n), kThreads, 0, s>>>(g, pb, ld, n, eid, P->key, P->flags, P->mrec,
-                                               P->mov);
+  // apply the cell changes still pending from the last step of the previous
+  // run, then record the particles the caller moved since (both incremental
+  // unless the layout is unknown)
+  {
+    ProfScopeT ps_(P, KC_SORT, s);
+    DMPM_TRY(enqueue_resort<T>(P, g, pb, eid, s));
+  }
+  k_tile_check<T><<<nblk(n), kThreads, 0, s>>>(g, pb, ld, n, eid, P->key, move_log(P));
   DMPM_TRY(launch_check("tile_check"));
   switch (P->cell_block) {
     case 64:

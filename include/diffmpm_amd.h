@@ -240,9 +240,11 @@ enum {
   DMPM_C_VOL = 7,
   DMPM_C_ROWS = 8
 };
-/* pin: particle block at the start of step `step` (read only); pwork, nwork,
- * eid_work: scratch particle block, node block and element ids (the forward
- * step is recomputed there); cot: in = cotangent of the state after the
+/* pin: particle block at the start of step `step` (read only; the rows
+ * loc, velocity, mass, volume, density, stress and (per-particle E)
+ * youngs_modulus are used); pwork, nwork, eid_work: scratch particle block,
+ * node block and element ids (the forward step is recomputed there; pwork's
+ * other rows must hold finite values); cot: in = cotangent of the state after the
  * step, out = cotangent of the state before it; lebar (may be NULL): [ld],
  * += dL/dlog(E_p) (per-particle E) or its per-particle terms (scalar E: sum
  * them).  Same restrictions as the fused path (2D Q4, no particle tractions
