@@ -54,6 +54,7 @@ _DP = POINTER(c_double)
 SIGNATURES = {
     "dmpm_abi_version": [],
     "dmpm_plan_create": [POINTER(PlanDesc), POINTER(c_void_p)],
+    "dmpm_batch_plan_create": [_P, c_int32, c_int64, c_int64, _P],
     "dmpm_plan_destroy": [_P],
     "dmpm_plan_set_functions": [_P, c_int32, _I32P, _I32P, _I32P, _DP, _DP],
     "dmpm_plan_set_constraints": [_P, c_int32, _I32P, _I32P, _DP],

@@ -41,7 +41,21 @@ C_ROWS = 8
 
 class Differentiable:
     """Scratch buffers and plans of the differentiable step of ``sim`` (an
-    ``MPMExplicit`` with one particle set on a Quadrilateral4Node mesh)."""
+    ``MPMExplicit`` with one particle set on a Quadrilateral4Node mesh) or of
+    an ``ensemble.Ensemble`` (``Differentiable.of_ensemble``; per-simulation
+    parameters map to particles through ``Ensemble.sim_index()``)."""
+
+    @classmethod
+    def of_ensemble(cls, ens):
+        from .ensemble import BatchPlan
+        d = cls.__new__(cls)
+        d.sim, d.ps, d.el = None, None, None
+        d.per_particle_E = all(s.mesh.particles[0]._has_youngs for s in ens.sims)
+        d.n = ens.n
+        d._pbuf, d._nbuf, d._eid_f = ens.pbuf, ens.nbuf, ens.eid
+        d.fplan, d.aplan = ens.plan, BatchPlan(ens)
+        d._finish(ens.scheme, ens.velocity_update, ens.dt)
+        return d
 
     def __init__(self, sim, per_particle_E=True):
         mesh = sim.mesh
@@ -60,23 +74,28 @@ class Differentiable:
         self.sim, self.ps, self.el = sim, ps, el
         self.per_particle_E = per_particle_E
         self.n = len(ps)
+        self._pbuf, self._nbuf, self._eid_f = ps._buf, el.nodes._buf, ps._eid
         self.fplan = plan_for(el, ps)
         self.aplan = Plan(el, ps)  # own sort state for the recomputed steps
+        self._finish(sim.scheme, sim.velocity_update, sim.dt)
+
+    def _finish(self, scheme, velocity_update, dt):
         # step-input block (mass, E and the other constant rows set once),
         # recomputation scratch blocks
-        self.pin = ps._buf.clone()
-        self.pwork = ps._buf.clone()
-        self.nwork = el.nodes._buf.clone()
-        self.eid = torch.empty_like(ps._eid)
-        self.scheme = sim.scheme
-        self.code = _lib.DMPM_USF if sim.scheme == "usf" else _lib.DMPM_USL
-        self.vu = 1 if sim.velocity_update else 0
-        self.dt = float(sim.dt)
+        self.pin = self._pbuf.clone()
+        self.pwork = self._pbuf.clone()
+        self.nwork = self._nbuf.clone()
+        self.eid = torch.empty_like(self._eid_f)
+        self.scheme = scheme
+        self.velocity_update = velocity_update
+        self.code = _lib.DMPM_USF if scheme == "usf" else _lib.DMPM_USL
+        self.vu = 1 if velocity_update else 0
+        self.dt = float(dt)
 
     # -- state <-> particle block ---------------------------------------------
     def get_state(self):
         """The current dynamic state as a new (8, N) tensor."""
-        return self.ps._buf[list(STATE_ROWS), : self.n].clone()
+        return self._pbuf[list(STATE_ROWS), : self.n].clone()
 
     def _write(self, buf, state, log_e):
         buf[list(STATE_ROWS), : self.n] = state
@@ -84,12 +103,12 @@ class Differentiable:
             buf[_lib.P_YOUNGS, : self.n] = torch.exp(log_e.detach()).to(buf.dtype).expand(self.n)
 
     def set_state(self, state, log_e=None):
-        self._write(self.ps._buf, state.detach(), log_e)
+        self._write(self._pbuf, state.detach(), log_e)
 
     # -- one step ---------------------------------------------------------------
     def forward_step(self, state, log_e, gravity, i):
         self.set_state(state, log_e)
-        self.fplan.run(self.scheme, self.sim.velocity_update, gravity, self.dt, i, 1,
+        self.fplan.run(self.scheme, self.velocity_update, gravity, self.dt, i, 1,
                        keep_temporaries=False)
         return self.get_state()
 
@@ -98,9 +117,10 @@ class Differentiable:
         cotangent of its output."""
         self.aplan.refresh_tables()
         self._write(self.pin, state.detach(), log_e)
-        cot = torch.empty((C_ROWS, self.ps._ld), dtype=self.ps.dtype, device=self.ps.device)
+        ld = self._pbuf.shape[1]
+        cot = torch.empty((C_ROWS, ld), dtype=self._pbuf.dtype, device=self._pbuf.device)
         cot[:, : self.n] = gout
-        lebar = torch.zeros(self.ps._ld, dtype=self.ps.dtype, device=self.ps.device)
+        lebar = torch.zeros(ld, dtype=self._pbuf.dtype, device=self._pbuf.device)
         call("dmpm_step_adjoint", self.aplan.h, ptr(self.pin), ptr(self.pwork), ptr(self.eid),
              ptr(self.nwork), ptr(cot), ptr(lebar), self.code, self.vu,
              self.aplan._gravity(gravity), self.dt, int(i), self.aplan.s)

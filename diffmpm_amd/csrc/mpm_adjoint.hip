@@ -45,7 +45,7 @@ namespace {
 #define NR(r) (nb + (int64_t)(r) * nld)
 #define CR(r) (cot + (int64_t)(r) * ld)
 #define WR(r) (wk + (int64_t)(r) * ld)
-#define AR(r) (na + (int64_t)(r) * g.nnodes)
+#define AR(r) (na + (int64_t)(r) * M.nnodes)
 
 // rows of the per-particle weight workspace: N-weighted nodal-velocity and
 // acceleration cotangents, strain-rate cotangents (L00, L11, L01, L10 at the
@@ -55,15 +55,19 @@ enum { W_NVX = 0, W_NVY, W_NAX, W_NAY, W_L00, W_L11, W_L01, W_L10, W_C00, W_C11,
 // rows of the nodal adjoint workspace
 enum { A_FX = 0, A_FY, A_M, A_VKX, A_VKY, A_PX, A_PY, A_ROWS };
 
-// Per-particle geometry at the step-start position.
+// Per-particle geometry at the step-start position: simulation, cell of its
+// own mesh, natural coordinates, inverse-Jacobian scales, corner nodes.
 template <typename T> struct PGeo {
   bool in;
-  int I, J;
+  int s, I, J;
   T xi, eta, ix, iy;
+  int ids[4];
 };
 template <typename T>
-__device__ __forceinline__ PGeo<T> pgeo(const Grid2<T>& g, T x, T y) {
+__device__ __forceinline__ PGeo<T> pgeo(const Mesh2<T>& M, int p, T x, T y) {
   PGeo<T> q;
+  q.s = M.sim_p(p);
+  const Grid2<T> g = M.grid(q.s);
   const int e = element_id<T, 2>(g.geo(), x, y);
   q.in = e >= 0 && e < g.ncells;
   q.I = q.in ? e % g.nx : 0;
@@ -72,6 +76,7 @@ __device__ __forceinline__ PGeo<T> pgeo(const Grid2<T>& g, T x, T y) {
   q.xi = natural(x, X0, X1);
   q.eta = natural(y, Y0, Y1);
   q4_invJ(q.xi, q.eta, X0, X1, Y0, Y1, q.ix, q.iy);
+  cell_nodes(g.nx, q.I, q.J, q.ids, M.noff(q.s));
   return q;
 }
 
@@ -143,21 +148,20 @@ __device__ __forceinline__ void stress_adjoint(const Mat2& mat, T E_p, T dt, T V
 // adjoint of the stress update made with the updated nodal velocities
 // (scheme.py:127-133).
 template <typename T, bool USL>
-__global__ __launch_bounds__(kThreads) void k_adj_g2p(Grid2<T> g, const T* __restrict__ pb,
+__global__ __launch_bounds__(kThreads) void k_adj_g2p(Mesh2<T> M, const T* __restrict__ pb,
                                                       int64_t ld, int n, const T* __restrict__ nb,
                                                       int64_t nld, T* __restrict__ cot,
                                                       T* __restrict__ wk, T* __restrict__ lebar,
                                                       Mat2 mat, T dt, int vu) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= n) return;
-  const PGeo<T> q = pgeo(g, PR(DMPM_P_X)[p], PR(DMPM_P_X + 1)[p]);
+  const PGeo<T> q = pgeo(M, p, PR(DMPM_P_X)[p], PR(DMPM_P_X + 1)[p]);
   if (!q.in) {
     for (int r = 0; r < DMPM_C_ROWS; ++r) CR(r)[p] = T(0);
     for (int r = 0; r < W_ROWS; ++r) WR(r)[p] = T(0);
     return;
   }
-  int ids[4];
-  cell_nodes(g.nx, q.I, q.J, ids);
+  const int* ids = q.ids;
   const T xbx = CR(DMPM_C_X)[p], xby = CR(DMPM_C_X + 1)[p];
   const T vbx = CR(DMPM_C_V)[p], vby = CR(DMPM_C_V + 1)[p];
   T vx[4], vy[4];
@@ -185,8 +189,9 @@ __global__ __launch_bounds__(kThreads) void k_adj_g2p(Grid2<T> g, const T* __res
   if (USL) {
     const T sb[3] = {CR(DMPM_C_SXX)[p], CR(DMPM_C_SYY)[p], CR(DMPM_C_SXY)[p]};
     T xb[2], Vb_in, unused = T(0);
-    stress_adjoint(mat, mat.per_E ? PR(DMPM_P_YOUNGS)[p] : T(0), dt, PR(DMPM_P_VOLUME)[p], q, vx,
-                   vy, sb, CR(DMPM_C_VOL)[p], lebar ? lebar + p : &unused, Lb, xb, Vb_in);
+    stress_adjoint(M.mat(mat, q.s), mat.per_E ? PR(DMPM_P_YOUNGS)[p] : T(0), dt,
+                   PR(DMPM_P_VOLUME)[p], q, vx, vy, sb, CR(DMPM_C_VOL)[p],
+                   lebar ? lebar + p : &unused, Lb, xb, Vb_in);
     CR(DMPM_C_VOL)[p] = Vb_in;
     sxi = sxi + xb[0];
     seta = seta + xb[1];
@@ -201,15 +206,14 @@ __global__ __launch_bounds__(kThreads) void k_adj_g2p(Grid2<T> g, const T* __res
 // kinematic nodal velocities.
 template <typename T, bool USF>
 __global__ __launch_bounds__(kThreads) void k_adj_force(
-    Grid2<T> g, const T* __restrict__ pb, const T* __restrict__ pw, int64_t ld, int n,
+    Mesh2<T> M, const T* __restrict__ pb, const T* __restrict__ pw, int64_t ld, int n,
     const T* __restrict__ na, const T* __restrict__ vkin, T* __restrict__ cot,
     T* __restrict__ wk, T* __restrict__ lebar, Mat2 mat, T Gx, T Gy, T dt) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= n) return;
-  const PGeo<T> q = pgeo(g, PR(DMPM_P_X)[p], PR(DMPM_P_X + 1)[p]);
+  const PGeo<T> q = pgeo(M, p, PR(DMPM_P_X)[p], PR(DMPM_P_X + 1)[p]);
   if (!q.in) return;
-  int ids[4];
-  cell_nodes(g.nx, q.I, q.J, ids);
+  const int* ids = q.ids;
   // the stress and volume the force P2G used: after the USF update, or the
   // step-start ones (USL)
   const T* __restrict__ src = USF ? pw : pb;
@@ -246,12 +250,13 @@ __global__ __launch_bounds__(kThreads) void k_adj_force(
     T vx[4], vy[4];
     for (int k = 0; k < 4; ++k) {
       vx[k] = vkin[ids[k]];
-      vy[k] = vkin[g.nnodes + ids[k]];
+      vy[k] = vkin[M.nnodes + ids[k]];
     }
     T Lb[6], xb[2], Vb_in;
     T dummy = T(0);
-    stress_adjoint(mat, mat.per_E ? PR(DMPM_P_YOUNGS)[p] : T(0), dt, PR(DMPM_P_VOLUME)[p], q, vx,
-                   vy, sb, Vbt, lebar ? lebar + p : &dummy, Lb, xb, Vb_in);
+    stress_adjoint(M.mat(mat, q.s), mat.per_E ? PR(DMPM_P_YOUNGS)[p] : T(0), dt,
+                   PR(DMPM_P_VOLUME)[p], q, vx, vy, sb, Vbt, lebar ? lebar + p : &dummy, Lb, xb,
+                   Vb_in);
     CR(DMPM_C_VOL)[p] = Vb_in;
     for (int k = 0; k < 6; ++k) WR(W_L00 + k)[p] = Lb[k];
     for (int k = 0; k < 4; ++k) WR(W_NVX + k)[p] = T(0);
@@ -267,16 +272,15 @@ __global__ __launch_bounds__(kThreads) void k_adj_force(
 // P2G of mass and momentum adjoint (element.py:122-189): gathers the nodal
 // momentum and mass cotangents; the natural-coordinate cotangent goes to x.
 template <typename T>
-__global__ __launch_bounds__(kThreads) void k_adj_kin(Grid2<T> g, const T* __restrict__ pb,
+__global__ __launch_bounds__(kThreads) void k_adj_kin(Mesh2<T> M, const T* __restrict__ pb,
                                                       int64_t ld, int n,
                                                       const T* __restrict__ na,
                                                       T* __restrict__ cot, T* __restrict__ wk) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= n) return;
-  const PGeo<T> q = pgeo(g, PR(DMPM_P_X)[p], PR(DMPM_P_X + 1)[p]);
+  const PGeo<T> q = pgeo(M, p, PR(DMPM_P_X)[p], PR(DMPM_P_X + 1)[p]);
   if (!q.in) return;
-  int ids[4];
-  cell_nodes(g.nx, q.I, q.J, ids);
+  const int* ids = q.ids;
   const T m = PR(DMPM_P_MASS)[p];
   const T vx = PR(DMPM_P_V)[p], vy = PR(DMPM_P_V + 1)[p];
   T N[4];
@@ -308,7 +312,7 @@ __global__ __launch_bounds__(kThreads) void k_adj_kin(Grid2<T> g, const T* __res
 // adds the four cells of a node in the canonical order).
 constexpr int kSlotsA = 512;
 template <typename T, int CB>
-__global__ __launch_bounds__(kThreads) void k_adj_cell(Grid2<T> g, const T* __restrict__ pb,
+__global__ __launch_bounds__(kThreads) void k_adj_cell(Mesh2<T> M, const T* __restrict__ pb,
                                                        int64_t ld, SortRef sr,
                                                        const T* __restrict__ wk,
                                                        T* __restrict__ part) {
@@ -318,7 +322,7 @@ __global__ __launch_bounds__(kThreads) void k_adj_cell(Grid2<T> g, const T* __re
   const int* __restrict__ cs = sr.cs();
   const int ident = sr.flags[1];
   const int c0 = blockIdx.x * CB, t = threadIdx.x;
-  const int cend = min(c0 + CB, g.ncells);
+  const int cend = min(c0 + CB, M.ncells);
   const int mycell = c0 + t;
   const bool celler = t < CB && mycell < cend;
   const int ms0 = celler ? cs[mycell] : 0, ms1 = celler ? cs[mycell + 1] : 0;
@@ -330,14 +334,17 @@ __global__ __launch_bounds__(kThreads) void k_adj_cell(Grid2<T> g, const T* __re
     for (int s = k0 + t; s < kend; s += kThreads) {
       const int p = ident ? s : perm[s];
       const int qq = s - k0;
-      const PGeo<T> q = pgeo(g, PR(DMPM_P_X)[p], PR(DMPM_P_X + 1)[p]);
+      const PGeo<T> q = pgeo(M, p, PR(DMPM_P_X)[p], PR(DMPM_P_X + 1)[p]);
       st[0][qq] = q.xi;
       st[1][qq] = q.eta;
       for (int r = 0; r < 10; ++r) st[2 + r][qq] = WR(r)[p];
     }
     __syncthreads();
     if (celler) {
-      const int I = mycell % g.nx, J = mycell / g.nx;
+      const int cs_ = M.sim_c(mycell);
+      const Grid2<T> g = M.grid(cs_);
+      const int lc = mycell - M.coff(cs_);
+      const int I = lc % g.nx, J = lc / g.nx;
       const T X0 = g.X(I), X1 = g.X(I + 1), Y0 = g.Y(J), Y1 = g.Y(J + 1);
       const int lo = max(ms0, k0) - k0, hi = min(ms1, kend) - k0;
       for (int qq = lo; qq < hi; ++qq) {
@@ -364,7 +371,7 @@ __global__ __launch_bounds__(kThreads) void k_adj_cell(Grid2<T> g, const T* __re
     __syncthreads();
   }
   if (celler)
-    for (int v = 0; v < 16; ++v) part[(int64_t)v * g.ncells + mycell] = acc[v];
+    for (int v = 0; v < 16; ++v) part[(int64_t)v * M.ncells + mycell] = acc[v];
 }
 
 // Kinematics adjoint at one node and direction (element.py:191-202 with
@@ -384,14 +391,17 @@ __device__ __forceinline__ void kin_adjoint(T m, T v, bool constrained, T vb, T&
 // kinematics adjoint follows.
 template <typename T, bool USL>
 __global__ __launch_bounds__(kThreads) void k_adj_node1(
-    Grid2<T> g, const T* __restrict__ part, const T* __restrict__ nb, int64_t nld,
+    Mesh2<T> M, const T* __restrict__ part, const T* __restrict__ nb, int64_t nld,
     const T* __restrict__ vkin, T* __restrict__ na, const int* __restrict__ cidx,
     const int* __restrict__ cmask, T dt) {
   const int nd = blockIdx.x * blockDim.x + threadIdx.x;
-  if (nd >= g.nnodes) return;
-  const int i = nd % (g.nx + 1), j = nd / (g.nx + 1);
+  if (nd >= M.nnodes) return;
+  const int sm = M.sim_n(nd);
+  const Grid2<T> g = M.grid(sm);
+  const int nl = nd - M.noff(sm);
+  const int i = nl % (g.nx + 1), j = nl / (g.nx + 1);
   T acc[4];
-  node_gather<T, 4>(g, part, i, j, acc);
+  node_gather<T, 4>(g, M.coff(sm), M.ncells, part, i, j, acc);
   const int cr = cidx ? cidx[nd] : -1;
   const int mask = cr >= 0 ? cmask[cr] : 0;
   const T m = NR(DMPM_N_MASS)[nd];
@@ -412,7 +422,7 @@ __global__ __launch_bounds__(kThreads) void k_adj_node1(
     AR(A_FX + d)[nd] = Fb;
     if (USL) {
       T Pb;
-      kin_adjoint(m, vkin[(int64_t)d * g.nnodes + nd], (mask >> d) & 1, vkb, Pb, mb);
+      kin_adjoint(m, vkin[(int64_t)d * M.nnodes + nd], (mask >> d) & 1, vkb, Pb, mb);
       AR(A_PX + d)[nd] = Pb;
     } else {
       AR(A_VKX + d)[nd] = vkb;
@@ -424,17 +434,20 @@ __global__ __launch_bounds__(kThreads) void k_adj_node1(
 // Node pass 2 (USF): the kinematic velocity's cotangent from the nodal update
 // plus the strain-rate scatter; kinematics adjoint.
 template <typename T>
-__global__ __launch_bounds__(kThreads) void k_adj_node2(Grid2<T> g, const T* __restrict__ part,
+__global__ __launch_bounds__(kThreads) void k_adj_node2(Mesh2<T> M, const T* __restrict__ part,
                                                         const T* __restrict__ nb, int64_t nld,
                                                         const T* __restrict__ vkin,
                                                         T* __restrict__ na,
                                                         const int* __restrict__ cidx,
                                                         const int* __restrict__ cmask) {
   const int nd = blockIdx.x * blockDim.x + threadIdx.x;
-  if (nd >= g.nnodes) return;
-  const int i = nd % (g.nx + 1), j = nd / (g.nx + 1);
+  if (nd >= M.nnodes) return;
+  const int sm = M.sim_n(nd);
+  const Grid2<T> g = M.grid(sm);
+  const int nl = nd - M.noff(sm);
+  const int i = nl % (g.nx + 1), j = nl / (g.nx + 1);
   T acc[4];
-  node_gather<T, 4>(g, part, i, j, acc);
+  node_gather<T, 4>(g, M.coff(sm), M.ncells, part, i, j, acc);
   const int cr = cidx ? cidx[nd] : -1;
   const int mask = cr >= 0 ? cmask[cr] : 0;
   const T m = NR(DMPM_N_MASS)[nd];
@@ -442,7 +455,7 @@ __global__ __launch_bounds__(kThreads) void k_adj_node2(Grid2<T> g, const T* __r
   for (int d = 0; d < 2; ++d) {
     const T vb = AR(A_VKX + d)[nd] + acc[d];
     T Pb;
-    kin_adjoint(m, vkin[(int64_t)d * g.nnodes + nd], (mask >> d) & 1, vb, Pb, mb);
+    kin_adjoint(m, vkin[(int64_t)d * M.nnodes + nd], (mask >> d) & 1, vb, Pb, mb);
     AR(A_PX + d)[nd] = Pb;
   }
   AR(A_M)[nd] = mb;
@@ -451,7 +464,7 @@ __global__ __launch_bounds__(kThreads) void k_adj_node2(Grid2<T> g, const T* __r
 template <typename T, int CB>
 int adjoint_t(dmpm_plan* P, const T* pb, T* pw, int32_t* eid, T* nb, T* cot, T* lebar,
               int scheme, int vu, const double* grav, double dt, int64_t step, hipStream_t s) {
-  const Grid2<T> g = grid_of<T>(P);
+  const Mesh2<T> M = mesh_of<T>(P);
   const int64_t ld = P->d.ld, nld = P->d.nld;
   const int n = (int)P->n;
   const bool usf = scheme == DMPM_USF;
@@ -466,7 +479,7 @@ int adjoint_t(dmpm_plan* P, const T* pb, T* pw, int32_t* eid, T* nb, T* cot, T* 
                        "hipMemcpyAsync"));
   DMPM_TRY(tile_run(P, pw, eid, nb, scheme, vu, grav, dt, step, 1, 0, s));
   ProfScopeT ps_(P, KC_ADJ, s);
-  T* part = (T*)plan_workspace(P, 0, (size_t)16 * g.ncells * sizeof(T));
+  T* part = (T*)plan_workspace(P, 0, (size_t)16 * M.ncells * sizeof(T));
   const T* vkin = (const T*)plan_workspace(P, 1, (size_t)2 * P->nnodes * sizeof(T));
   T* wk = (T*)plan_workspace(P, 3, (size_t)W_ROWS * ld * sizeof(T));
   T* na = (T*)plan_workspace(P, 4, (size_t)A_ROWS * P->nnodes * sizeof(T));
@@ -478,31 +491,31 @@ int adjoint_t(dmpm_plan* P, const T* pb, T* pw, int32_t* eid, T* nb, T* cot, T* 
   const T tdt = T(dt), Gx = T(grav[0]), Gy = T(grav[1]);
   const SortRef sr = sort_ref(P);
   const unsigned np = nblk(n), nn = nblk(P->nnodes);
-  const unsigned ncb = (unsigned)((g.ncells + CB - 1) / CB);
+  const unsigned ncb = (unsigned)((M.ncells + CB - 1) / CB);
   if (usf)
-    k_adj_g2p<T, false><<<np, kThreads, 0, s>>>(g, pb, ld, n, nb, nld, cot, wk, lebar, mat, tdt,
+    k_adj_g2p<T, false><<<np, kThreads, 0, s>>>(M, pb, ld, n, nb, nld, cot, wk, lebar, mat, tdt,
                                                  vu);
   else
-    k_adj_g2p<T, true><<<np, kThreads, 0, s>>>(g, pb, ld, n, nb, nld, cot, wk, lebar, mat, tdt,
+    k_adj_g2p<T, true><<<np, kThreads, 0, s>>>(M, pb, ld, n, nb, nld, cot, wk, lebar, mat, tdt,
                                                 vu);
-  k_adj_cell<T, CB><<<ncb, kThreads, 0, s>>>(g, pb, ld, sr, wk, part);
+  k_adj_cell<T, CB><<<ncb, kThreads, 0, s>>>(M, pb, ld, sr, wk, part);
   if (usf)
-    k_adj_node1<T, false><<<nn, kThreads, 0, s>>>(g, part, nb, nld, vkin, na, P->cidx, P->cmask,
+    k_adj_node1<T, false><<<nn, kThreads, 0, s>>>(M, part, nb, nld, vkin, na, P->cidx, P->cmask,
                                                    tdt);
   else
-    k_adj_node1<T, true><<<nn, kThreads, 0, s>>>(g, part, nb, nld, vkin, na, P->cidx, P->cmask,
+    k_adj_node1<T, true><<<nn, kThreads, 0, s>>>(M, part, nb, nld, vkin, na, P->cidx, P->cmask,
                                                   tdt);
   if (usf)
-    k_adj_force<T, true><<<np, kThreads, 0, s>>>(g, pb, pw, ld, n, na, vkin, cot, wk, lebar, mat,
+    k_adj_force<T, true><<<np, kThreads, 0, s>>>(M, pb, pw, ld, n, na, vkin, cot, wk, lebar, mat,
                                                   Gx, Gy, tdt);
   else
-    k_adj_force<T, false><<<np, kThreads, 0, s>>>(g, pb, pw, ld, n, na, vkin, cot, wk, lebar,
+    k_adj_force<T, false><<<np, kThreads, 0, s>>>(M, pb, pw, ld, n, na, vkin, cot, wk, lebar,
                                                    mat, Gx, Gy, tdt);
   if (usf) {
-    k_adj_cell<T, CB><<<ncb, kThreads, 0, s>>>(g, pb, ld, sr, wk, part);
-    k_adj_node2<T><<<nn, kThreads, 0, s>>>(g, part, nb, nld, vkin, na, P->cidx, P->cmask);
+    k_adj_cell<T, CB><<<ncb, kThreads, 0, s>>>(M, pb, ld, sr, wk, part);
+    k_adj_node2<T><<<nn, kThreads, 0, s>>>(M, part, nb, nld, vkin, na, P->cidx, P->cmask);
   }
-  k_adj_kin<T><<<np, kThreads, 0, s>>>(g, pb, ld, n, na, cot, wk);
+  k_adj_kin<T><<<np, kThreads, 0, s>>>(M, pb, ld, n, na, cot, wk);
   return launch_check("step_adjoint");
 }
 

@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <type_traits>
 #include <cstdio>
 #include <cstring>
 #include <map>
@@ -836,7 +837,7 @@ void plan_free(dmpm_plan* P) {
   if (!P) return;
   void* ptrs[] = {P->key, P->perm, P->counts, P->fill, P->cs, P->bsum, P->flags, P->big,
                   P->perm2, P->cs2, P->mov, P->mrec, P->dL, P->dA, P->headL, P->headA,
-                  P->chunkD, P->chunkC,
+                  P->chunkD, P->chunkC, P->sims,
                   P->con_node, P->con_dir, P->con_val, P->nf_id, P->nf_dir, P->nf_off,
                   P->nf_fn, P->nf_val, P->tr_id, P->tr_dir, P->tr_off, P->tr_fn, P->tr_val,
                   P->fn_kind, P->fn_off, P->fn_cnt, P->fn_x, P->fn_f, P->cidx, P->cmask,
@@ -889,32 +890,9 @@ const char* dmpm_last_error(void) { return g_last_error.c_str(); }
 
 int dmpm_abi_version(void) { return 100; }
 
-int dmpm_plan_create(const dmpm_plan_desc* desc, dmpm_plan** out) {
-  if (!desc || !out) return fail(DMPM_EINVAL, "null argument");
-  const dmpm_plan_desc& d = *desc;
-  if (d.element != DMPM_LINEAR1D && d.element != DMPM_QUAD4)
-    return fail(DMPM_EINVAL, "element must be DMPM_LINEAR1D or DMPM_QUAD4");
-  if (d.dtype != DMPM_F32 && d.dtype != DMPM_F64)
-    return fail(DMPM_EINVAL, "dtype must be DMPM_F32 or DMPM_F64");
-  if (d.nx < 1 || d.ny < 1 || (d.element == DMPM_LINEAR1D && d.ny != 1))
-    return fail(DMPM_EINVAL, "invalid mesh size");
-  if (!(d.hx > 0) || (d.element == DMPM_QUAD4 && !(d.hy > 0)))
-    return fail(DMPM_EINVAL, "element lengths must be positive");
-  if ((int64_t)d.nx * d.ny >= (1ll << 30)) return fail(DMPM_EINVAL, "mesh too large");
-  if (d.nparticles < 0 || d.nparticles >= (1ll << 31) || d.ld < d.nparticles)
-    return fail(DMPM_EINVAL, "invalid particle count or row stride");
-  if (d.material != DMPM_LINEAR_ELASTIC && d.material != DMPM_SIMPLE_MATERIAL)
-    return fail(DMPM_EINVAL, "unknown material");
-  dmpm_plan* P = new dmpm_plan();
-  P->d = d;
-  P->n = d.nparticles;
-  P->dim = d.element == DMPM_LINEAR1D ? 1 : 2;
-  P->ncells = d.nx * d.ny;
-  P->nnodes = P->dim == 1 ? d.nx + 1 : (d.nx + 1) * (d.ny + 1);
-  if (d.nld < P->nnodes) {
-    delete P;
-    return fail(DMPM_EINVAL, "node row stride smaller than the node count");
-  }
+// Allocates and initialises the sort state and the simulation table of a
+// plan whose sizes are set; frees it on failure.
+static int plan_init(dmpm_plan* P, dmpm_plan** out) {
   hipGetDevice(&P->device);
   int64_t n = std::max<int64_t>(P->n, 1);
   int nb = P->ncells + 1;
@@ -962,6 +940,13 @@ int dmpm_plan_create(const dmpm_plan_desc* desc, dmpm_plan** out) {
   if (rc == DMPM_OK) rc = hip_check(hipMemset(P->key, 0xff, n * sizeof(int)), "hipMemset");
   if (rc == DMPM_OK) rc = hip_check(hipMemset(P->cs, 0, (nb + 1) * sizeof(int)), "hipMemset");
   if (rc == DMPM_OK) rc = hip_check(hipMemset(P->perm, 0, n * sizeof(int)), "hipMemset");
+  P->nsims = (int)P->sims_h.size();
+  if (rc == DMPM_OK)
+    rc = hip_check(hipMalloc((void**)&P->sims, P->sims_h.size() * sizeof(SimRec)), "hipMalloc");
+  if (rc == DMPM_OK)
+    rc = hip_check(hipMemcpy(P->sims, P->sims_h.data(), P->sims_h.size() * sizeof(SimRec),
+                             hipMemcpyHostToDevice),
+                   "hipMemcpy");
   int f0[kFlagWords] = {1};
   if (rc == DMPM_OK)
     rc = hip_check(hipMemcpy(P->flags, f0, sizeof(f0), hipMemcpyHostToDevice), "hipMemcpy");
@@ -972,6 +957,92 @@ int dmpm_plan_create(const dmpm_plan_desc* desc, dmpm_plan** out) {
   *out = P;
   return DMPM_OK;
 }
+
+int dmpm_plan_create(const dmpm_plan_desc* desc, dmpm_plan** out) {
+  if (!desc || !out) return fail(DMPM_EINVAL, "null argument");
+  const dmpm_plan_desc& d = *desc;
+  if (d.element != DMPM_LINEAR1D && d.element != DMPM_QUAD4)
+    return fail(DMPM_EINVAL, "element must be DMPM_LINEAR1D or DMPM_QUAD4");
+  if (d.dtype != DMPM_F32 && d.dtype != DMPM_F64)
+    return fail(DMPM_EINVAL, "dtype must be DMPM_F32 or DMPM_F64");
+  if (d.nx < 1 || d.ny < 1 || (d.element == DMPM_LINEAR1D && d.ny != 1))
+    return fail(DMPM_EINVAL, "invalid mesh size");
+  if (!(d.hx > 0) || (d.element == DMPM_QUAD4 && !(d.hy > 0)))
+    return fail(DMPM_EINVAL, "element lengths must be positive");
+  if ((int64_t)d.nx * d.ny >= (1ll << 30)) return fail(DMPM_EINVAL, "mesh too large");
+  if (d.nparticles < 0 || d.nparticles >= (1ll << 31) || d.ld < d.nparticles)
+    return fail(DMPM_EINVAL, "invalid particle count or row stride");
+  if (d.material != DMPM_LINEAR_ELASTIC && d.material != DMPM_SIMPLE_MATERIAL)
+    return fail(DMPM_EINVAL, "unknown material");
+  dmpm_plan* P = new dmpm_plan();
+  P->d = d;
+  P->n = d.nparticles;
+  P->dim = d.element == DMPM_LINEAR1D ? 1 : 2;
+  P->ncells = d.nx * d.ny;
+  P->nnodes = P->dim == 1 ? d.nx + 1 : (d.nx + 1) * (d.ny + 1);
+  if (d.nld < P->nnodes) {
+    delete P;
+    return fail(DMPM_EINVAL, "node row stride smaller than the node count");
+  }
+  SimRec r = {};
+  r.nx = d.nx;
+  r.ny = d.ny;
+  r.n = (int)d.nparticles;
+  r.hx = d.hx;
+  r.hy = d.hy;
+  for (int k = 0; k < 3; ++k) r.de[k] = d.de[k];
+  P->sims_h.assign(1, r);
+  return plan_init(P, out);
+}
+
+int dmpm_batch_plan_create(const dmpm_plan_desc* descs, int32_t nsims, int64_t ld, int64_t nld,
+                           dmpm_plan** out) {
+  if (!descs || !out || nsims < 1) return fail(DMPM_EINVAL, "null argument or empty batch");
+  const dmpm_plan_desc& d0 = descs[0];
+  std::vector<SimRec> recs(nsims);
+  int64_t np = 0, nc = 0, nn = 0;
+  for (int s = 0; s < nsims; ++s) {
+    const dmpm_plan_desc& d = descs[s];
+    if (d.element != DMPM_QUAD4) return fail(DMPM_EINVAL, "batch plans hold DMPM_QUAD4 meshes");
+    if (d.dtype != d0.dtype || d.material != d0.material ||
+        d.per_particle_E != d0.per_particle_E)
+      return fail(DMPM_EINVAL, "the simulations of a batch share dtype and material kind");
+    if (d.dtype != DMPM_F32 && d.dtype != DMPM_F64)
+      return fail(DMPM_EINVAL, "dtype must be DMPM_F32 or DMPM_F64");
+    if (d.material != DMPM_LINEAR_ELASTIC && d.material != DMPM_SIMPLE_MATERIAL)
+      return fail(DMPM_EINVAL, "unknown material");
+    if (d.nx < 1 || d.ny < 1 || !(d.hx > 0) || !(d.hy > 0) || d.nparticles < 0)
+      return fail(DMPM_EINVAL, "invalid simulation in batch");
+    SimRec& r = recs[s];
+    r.nx = d.nx;
+    r.ny = d.ny;
+    r.n = (int)d.nparticles;
+    r.hx = d.hx;
+    r.hy = d.hy;
+    for (int k = 0; k < 3; ++k) r.de[k] = d.de[k];
+    r.part_off = (int)np;
+    r.cell_off = (int)nc;
+    r.node_off = (int)nn;
+    np += d.nparticles;
+    nc += (int64_t)d.nx * d.ny;
+    nn += (int64_t)(d.nx + 1) * (d.ny + 1);
+  }
+  if (np >= (1ll << 31) || nc >= (1ll << 30) || nn >= (1ll << 30))
+    return fail(DMPM_EINVAL, "batch too large");
+  if (ld < np || nld < nn) return fail(DMPM_EINVAL, "row strides smaller than the batch");
+  dmpm_plan* P = new dmpm_plan();
+  P->d = d0;
+  P->d.nparticles = np;
+  P->d.ld = ld;
+  P->d.nld = nld;
+  P->n = np;
+  P->dim = 2;
+  P->ncells = (int)nc;
+  P->nnodes = (int)nn;
+  P->sims_h = recs;
+  return plan_init(P, out);
+}
+
 
 static void prof_off(dmpm_plan* P) {
   for (auto& c : P->prof) {
@@ -1282,8 +1353,15 @@ template <typename T, int DIM> struct Ops {
   static T step_time(int64_t step, double dt) { return T(step) * T(dt); }
 };
 
+// Entry points whose body declares kBatch = true also accept batch plans.
+template <class F, class = void> struct BatchOK : std::false_type {};
+template <class F>
+struct BatchOK<F, std::void_t<decltype(F::kBatch)>> : std::integral_constant<bool, F::kBatch> {};
+
 template <template <typename, int> class F, typename... A>
 int dispatch(dmpm_plan* P, A... a) {
+  if (P->nsims > 1 && !BatchOK<F<float, 2>>::value)
+    return fail(DMPM_EINVAL, "batch plans run the fused 2D step (dmpm_run) and its adjoint only");
   if (P->d.dtype == DMPM_F32)
     return P->dim == 1 ? F<float, 1>::run(P, a...) : F<float, 2>::run(P, a...);
   return P->dim == 1 ? F<double, 1>::run(P, a...) : F<double, 2>::run(P, a...);
@@ -1587,6 +1665,7 @@ void* plan_workspace(dmpm_plan* P, int slot, size_t bytes) {
 }
 
 template <typename T, int DIM> struct RunSteps {
+  static constexpr bool kBatch = true;
   static int run(dmpm_plan* P, void* pbuf, int32_t* eid, void* nbuf, int scheme, int vu,
                  const double* grav, double dt, int64_t step0, int64_t nsteps, int flags,
                  hipStream_t s) {
@@ -1594,6 +1673,10 @@ template <typename T, int DIM> struct RunSteps {
     const int pfext = (flags >> 1) & 1;
     if (!(flags & 4) && tile_eligible(P, pfext))
       return tile_run(P, pbuf, eid, nbuf, scheme, vu, grav, dt, step0, nsteps, temps, s);
+    if (P->nsims > 1)
+      return fail(DMPM_EINVAL,
+                  "batch plans run the fused 2D step only (no per-phase path, particle "
+                  "tractions or particle external forces)");
     for (int64_t i = step0; i < step0 + nsteps; ++i) {
       DMPM_TRY((PhaseKin<T, DIM>::run(P, pbuf, eid, nbuf, s)));
       if (scheme == DMPM_USF) DMPM_TRY((PhaseStress<T, DIM>::run(P, pbuf, eid, nbuf, dt, temps, s)));

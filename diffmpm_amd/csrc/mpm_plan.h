@@ -8,12 +8,26 @@
 
 #include "../../include/diffmpm_amd.h"
 
+// One simulation of a plan (device table; 64 bytes).
+struct SimRec {
+  int nx, ny;                         // cells per axis
+  int cell_off, node_off, part_off;   // first global cell, node, particle
+  int n;                              // particles
+  int pad0, pad1;
+  double hx, hy;
+  double de[3];                       // material coefficients
+  double pad2;
+};
+
 struct dmpm_plan {
   dmpm_plan_desc d{};
   int64_t n = 0;        // particles
   int dim = 2;
-  int ncells = 0;
+  int ncells = 0;   // all simulations (a batch plan holds several)
   int nnodes = 0;
+  int nsims = 1;
+  SimRec* sims = nullptr;          // [nsims] device table
+  std::vector<SimRec> sims_h;      // host copy
   int device = 0;
   // counting-sort workspace
   int* key = nullptr;     // [n] cell of each particle, ncells = outside the mesh
@@ -34,7 +48,8 @@ struct dmpm_plan {
   int* perm2 = nullptr;   // second buffer of perm (incremental re-sort)
   int* cs2 = nullptr;     // second buffer of cs
   // recorded cell changes, kMovRec ints each: particle, old key, new element
-  // id, next leaver of the old cell, next arrival of the new cell
+  // id (of the particle's own mesh), next leaver of the old cell, next arrival
+  // of the new cell, new key
   int movcap = 0;
   int* mov = nullptr;
   int* mrec = nullptr;    // [n] 1 + record index of a recorded particle, else 0
@@ -107,7 +122,7 @@ struct dmpm_plan {
 #define DMPM_TRY_H(x) DMPM_TRY(::dmpm::hip_check((x), #x))
 
 namespace dmpm {
-constexpr int kMovRec = 5;     // ints per recorded cell change
+constexpr int kMovRec = 6;     // ints per recorded cell change
 constexpr int kListMax = 64;   // recorded leavers/arrivals per cell handled incrementally
 constexpr int kMinChunkShift = 10;     // at least 1024 cells (4 per thread) per scan chunk
 constexpr int kMaxChunks = 8192;       // LDS histogram bound of the full sort

@@ -101,34 +101,106 @@ __device__ __forceinline__ void rate2s(T ix, T iy, T xi, T eta, const T vx[4], c
   r[5] = T(0);
 }
 
-template <typename T> Grid2<T> grid_of(const dmpm_plan* P) {
-  Grid2<T> g;
-  g.nx = P->d.nx;
-  g.ny = P->d.ny;
-  g.hx = T(P->d.hx);
-  g.hy = T(P->d.hy);
+// The simulations of a plan as the kernels see them.  A plan holds one
+// simulation or, for a batch plan, several with their own meshes, materials,
+// particles and constraints, laid end to end in global index spaces: the
+// particles of simulation s are [part_off, part_off + n), its cells
+// [cell_off, cell_off + nx*ny) and its nodes [node_off, node_off +
+// (nx+1)(ny+1)).  Particles outside their own mesh all share the key
+// ncells (the total).  One launch of each kernel serves every simulation.
+template <typename T> struct Mesh2 {
+  const SimRec* rec;  // device table, nsims entries
+  int nsims;
+  int ncells, nnodes, n;  // totals
+  Grid2<T> g0;            // geometry of simulation 0 (the only one when nsims == 1)
+
+  // index of the simulation owning global index v in the space with offsets
+  // rec[s].*off (nondecreasing)
+  template <typename F> __device__ __forceinline__ int find(int v, F off) const {
+    if (nsims == 1) return 0;
+    int lo = 0, hi = nsims - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (off(rec[mid]) <= v) lo = mid;
+      else hi = mid - 1;
+    }
+    return lo;
+  }
+  __device__ __forceinline__ int sim_p(int p) const {
+    return find(p, [](const SimRec& r) { return r.part_off; });
+  }
+  __device__ __forceinline__ int sim_c(int c) const {
+    return find(c, [](const SimRec& r) { return r.cell_off; });
+  }
+  __device__ __forceinline__ int sim_n(int nd) const {
+    return find(nd, [](const SimRec& r) { return r.node_off; });
+  }
+  __device__ __forceinline__ int coff(int s) const { return nsims == 1 ? 0 : rec[s].cell_off; }
+  __device__ __forceinline__ int noff(int s) const { return nsims == 1 ? 0 : rec[s].node_off; }
+  __device__ __forceinline__ Grid2<T> grid(int s) const {
+    if (nsims == 1) return g0;
+    const SimRec& r = rec[s];
+    Grid2<T> g;
+    g.nx = r.nx;
+    g.ny = r.ny;
+    g.hx = T(r.hx);
+    g.hy = T(r.hy);
+    g.ihx = T(1) / g.hx;
+    g.ihy = T(1) / g.hy;
+    g.ncells = r.nx * r.ny;
+    g.nnodes = (r.nx + 1) * (r.ny + 1);
+    return g;
+  }
+  // sort key of local element id e of simulation s
+  __device__ __forceinline__ int key(int s, const Grid2<T>& g, int e) const {
+    return (e >= 0 && e < g.ncells) ? coff(s) + e : ncells;
+  }
+  // material coefficients of simulation s
+  __device__ __forceinline__ Mat2 mat(const Mat2& m, int s) const {
+    if (nsims == 1) return m;
+    Mat2 r = m;
+    for (int k = 0; k < 3; ++k) r.de[k] = rec[s].de[k];
+    return r;
+  }
+};
+
+template <typename T> Mesh2<T> mesh_of(const dmpm_plan* P) {
+  Mesh2<T> M;
+  M.rec = P->sims;
+  M.nsims = P->nsims;
+  M.ncells = P->ncells;
+  M.nnodes = P->nnodes;
+  M.n = (int)P->n;
+  const SimRec& r = P->sims_h[0];
+  Grid2<T>& g = M.g0;
+  g.nx = r.nx;
+  g.ny = r.ny;
+  g.hx = T(r.hx);
+  g.hy = T(r.hy);
   g.ihx = T(1) / g.hx;
   g.ihy = T(1) / g.hy;
-  g.ncells = P->ncells;
-  g.nnodes = P->nnodes;
-  return g;
+  g.ncells = r.nx * r.ny;
+  g.nnodes = (r.nx + 1) * (r.ny + 1);
+  return M;
 }
 
-// Nodes of cell (I, J) in corner order ll, lr, ur, ul (element.py:692-723).
-__device__ __forceinline__ void cell_nodes(int nx, int I, int J, int ids[4]) {
-  const int ll = J * (nx + 1) + I;
+// Nodes of cell (I, J) in corner order ll, lr, ur, ul (element.py:692-723),
+// offset by the simulation's first node.
+__device__ __forceinline__ void cell_nodes(int nx, int I, int J, int ids[4], int noff = 0) {
+  const int ll = noff + J * (nx + 1) + I;
   ids[0] = ll;
   ids[1] = ll + 1;
   ids[2] = ll + nx + 2;
   ids[3] = ll + nx + 1;
 }
 
-// Sum over the (up to four) cells of node (i, j), canonical order, of the
-// partials of quantity q: corner 2 of cell (i-1,j-1), 3 of (i,j-1), 1 of
+// Sum over the (up to four) cells of node (i, j) of a mesh g whose cells
+// start at global index coff, canonical order, of the partials of quantity q
+// (rows of stride `stride`): corner 2 of cell (i-1,j-1), 3 of (i,j-1), 1 of
 // (i-1,j), 0 of (i,j).
 template <typename T, int NQ>
-__device__ __forceinline__ void node_gather(const Grid2<T>& g, const T* __restrict__ part, int i,
-                                            int j, T acc[NQ]) {
+__device__ __forceinline__ void node_gather(const Grid2<T>& g, int coff, int stride,
+                                            const T* __restrict__ part, int i, int j, T acc[NQ]) {
 #pragma unroll
   for (int q = 0; q < NQ; ++q) acc[q] = T(0);
   const int cells[4] = {(j - 1) * g.nx + i - 1, (j - 1) * g.nx + i, j * g.nx + i - 1,
@@ -140,7 +212,7 @@ __device__ __forceinline__ void node_gather(const Grid2<T>& g, const T* __restri
     if (!ok[c]) continue;
 #pragma unroll
     for (int q = 0; q < NQ; ++q)
-      acc[q] = acc[q] + part[(int64_t)(NQ * corner[c] + q) * g.ncells + cells[c]];
+      acc[q] = acc[q] + part[(int64_t)(NQ * corner[c] + q) * stride + coff + cells[c]];
   }
 }
 

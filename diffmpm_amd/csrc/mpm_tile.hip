@@ -62,7 +62,7 @@ constexpr int kPer = kSlots / kThreads;
 // KIN cell pass: terms m*N_k and N_k*(m*v) of every corner k (element.py:140,
 // 172); with temps also the natural coordinates (particle.py:180-206).
 template <typename T, int CB>
-__global__ __launch_bounds__(kThreads) void k_cell_kin(Grid2<T> g, T* __restrict__ pb,
+__global__ __launch_bounds__(kThreads) void k_cell_kin(Mesh2<T> M, T* __restrict__ pb,
                                                        int64_t ld, SortRef sr,
                                                        T* __restrict__ part, int temps) {
   constexpr int NIN = 5;  // xi, eta, m, m*vx, m*vy
@@ -71,7 +71,7 @@ __global__ __launch_bounds__(kThreads) void k_cell_kin(Grid2<T> g, T* __restrict
   const int* __restrict__ cs = sr.cs();
   const int ident = sr.flags[1];
   const int c0 = blockIdx.x * CB, t = threadIdx.x;
-  const int cend = min(c0 + CB, g.ncells);
+  const int cend = min(c0 + CB, M.ncells);
   const int mycell = c0 + t;
   const bool celler = t < CB && mycell < cend;
   const int ms0 = celler ? cs[mycell] : 0, ms1 = celler ? cs[mycell + 1] : 0;
@@ -102,6 +102,7 @@ __global__ __launch_bounds__(kThreads) void k_cell_kin(Grid2<T> g, T* __restrict
     for (int j = 0; j < kPer; ++j) {
       if (pp[j] < 0) continue;
       const int q = t + j * kThreads;
+      const Grid2<T> g = M.grid(M.sim_p(pp[j]));
       const int e = element_id<T, 2>(g.geo(), x[j], y[j]);
       const int I = e % g.nx, J = e / g.nx;
       const T xi = natural(x[j], g.X(I), g.X(I + 1));
@@ -135,14 +136,14 @@ __global__ __launch_bounds__(kThreads) void k_cell_kin(Grid2<T> g, T* __restrict
   }
   if (celler)
 #pragma unroll
-    for (int v = 0; v < 4 * NQ_KIN; ++v) part[(int64_t)v * g.ncells + mycell] = acc[v];
+    for (int v = 0; v < 4 * NQ_KIN; ++v) part[(int64_t)v * M.ncells + mycell] = acc[v];
 }
 
 // FORCE cell pass: terms N_k*(m*g) (element.py:254-256) and
 // -V*(dN_k/dx . sigma) (element.py:868-878) of every corner k.  USF first
 // updates strain, volume and stress in place (particle.py:249-327).
 template <typename T, int CB, bool USF>
-__global__ __launch_bounds__(kThreads) void k_cell_force(Grid2<T> g, T* __restrict__ pb,
+__global__ __launch_bounds__(kThreads) void k_cell_force(Mesh2<T> M, T* __restrict__ pb,
                                                          int64_t ld, SortRef sr,
                                                          T* __restrict__ part,
                                                          const T* __restrict__ vkin, Mat2 mat,
@@ -154,18 +155,21 @@ __global__ __launch_bounds__(kThreads) void k_cell_force(Grid2<T> g, T* __restri
   const int* __restrict__ cs = sr.cs();
   const int ident = sr.flags[1];
   const int c0 = blockIdx.x * CB, t = threadIdx.x;
-  const int cend = min(c0 + CB, g.ncells);
+  const int cend = min(c0 + CB, M.ncells);
   const int mycell = c0 + t;
   const bool celler = t < CB && mycell < cend;
   const int ms0 = celler ? cs[mycell] : 0, ms1 = celler ? cs[mycell + 1] : 0;
   const int s0 = cs[c0], s1 = cs[cend];
+  const int csim = celler ? M.sim_c(mycell) : 0;
+  const Grid2<T> gc = M.grid(csim);
+  const int lcell = mycell - M.coff(csim);
   if (USF && celler) {
     int ids[4];
-    cell_nodes(g.nx, mycell % g.nx, mycell / g.nx, ids);
+    cell_nodes(gc.nx, lcell % gc.nx, lcell / gc.nx, ids, M.noff(csim));
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
       sv[k][t] = vkin[ids[k]];
-      sv[4 + k][t] = vkin[g.nnodes + ids[k]];
+      sv[4 + k][t] = vkin[M.nnodes + ids[k]];
     }
   }
   T acc[4 * NQ_FORCE];
@@ -186,6 +190,8 @@ __global__ __launch_bounds__(kThreads) void k_cell_force(Grid2<T> g, T* __restri
       const int p = pp[j];
       const int q = t + j * kThreads;
       const T x = PR(DMPM_P_X)[p], y = PR(DMPM_P_X + 1)[p];
+      const int ps = M.sim_p(p);
+      const Grid2<T> g = M.grid(ps);
       const int e = element_id<T, 2>(g.geo(), x, y);
       const int I = e % g.nx, J = e / g.nx;
       const T X0 = g.X(I), X1 = g.X(I + 1), Y0 = g.Y(J), Y1 = g.Y(J + 1);
@@ -195,7 +201,7 @@ __global__ __launch_bounds__(kThreads) void k_cell_force(Grid2<T> g, T* __restri
       if (USF) {
         T ix, iy;
         q4_invJ(xi, eta, X0, X1, Y0, Y1, ix, iy);
-        const int lc = min(max(e - c0, 0), CB - 1);
+        const int lc = min(max(M.coff(ps) + e - c0, 0), CB - 1);
         const T vxn[4] = {sv[0][lc], sv[1][lc], sv[2][lc], sv[3][lc]};
         const T vyn[4] = {sv[4][lc], sv[5][lc], sv[6][lc], sv[7][lc]};
         T rr[6], rc[6], d[6];
@@ -204,7 +210,7 @@ __global__ __launch_bounds__(kThreads) void k_cell_force(Grid2<T> g, T* __restri
         for (int k = 0; k < 6; ++k) d[k] = rr[k] * dt;
         const T dvol = dt * (rc[0] + rc[1]);
         T ds[4];
-        stress_inc(mat, mat.per_E ? PR(DMPM_P_YOUNGS)[p] : T(0), d, ds);
+        stress_inc(M.mat(mat, ps), mat.per_E ? PR(DMPM_P_YOUNGS)[p] : T(0), d, ds);
         const T f = T(1) + dvol;
         PR(DMPM_P_STRAIN + 0)[p] = PR(DMPM_P_STRAIN + 0)[p] + d[0];
         PR(DMPM_P_STRAIN + 1)[p] = PR(DMPM_P_STRAIN + 1)[p] + d[1];
@@ -243,8 +249,8 @@ __global__ __launch_bounds__(kThreads) void k_cell_force(Grid2<T> g, T* __restri
     }
     __syncthreads();
     if (celler) {
-      const int I = mycell % g.nx, J = mycell / g.nx;
-      const T X0 = g.X(I), X1 = g.X(I + 1), Y0 = g.Y(J), Y1 = g.Y(J + 1);
+      const int I = lcell % gc.nx, J = lcell / gc.nx;
+      const T X0 = gc.X(I), X1 = gc.X(I + 1), Y0 = gc.Y(J), Y1 = gc.Y(J + 1);
       const int lo = max(ms0, k0) - k0, hi = min(ms1, kend) - k0;
       for (int q = lo; q < hi; ++q) {
         const T xi = st[0][q], eta = st[1][q];
@@ -269,24 +275,27 @@ __global__ __launch_bounds__(kThreads) void k_cell_force(Grid2<T> g, T* __restri
   }
   if (celler)
 #pragma unroll
-    for (int v = 0; v < 4 * NQ_FORCE; ++v) part[(int64_t)v * g.ncells + mycell] = acc[v];
+    for (int v = 0; v < 4 * NQ_FORCE; ++v) part[(int64_t)v * M.ncells + mycell] = acc[v];
 }
 
 // KIN node pass: nodal mass, momentum (snapped), velocity (element.py:122-202)
 // and velocity constraints (constraint.py:29-44); the kinematic velocity also
 // goes to vkin for the force phase.
 template <typename T>
-__global__ __launch_bounds__(kThreads) void k_node_kin(Grid2<T> g, const T* __restrict__ part,
+__global__ __launch_bounds__(kThreads) void k_node_kin(Mesh2<T> M, const T* __restrict__ part,
                                                        T* __restrict__ nb, int64_t nld,
                                                        T* __restrict__ vkin,
                                                        const int* __restrict__ cidx,
                                                        const int* __restrict__ cmask,
                                                        const double* __restrict__ cval) {
   const int nd = blockIdx.x * blockDim.x + threadIdx.x;
-  if (nd >= g.nnodes) return;
-  const int i = nd % (g.nx + 1), j = nd / (g.nx + 1);
+  if (nd >= M.nnodes) return;
+  const int sm = M.sim_n(nd);
+  const Grid2<T> g = M.grid(sm);
+  const int nl = nd - M.noff(sm);
+  const int i = nl % (g.nx + 1), j = nl / (g.nx + 1);
   T acc[NQ_KIN];
-  node_gather<T, NQ_KIN>(g, part, i, j, acc);
+  node_gather<T, NQ_KIN>(g, M.coff(sm), M.ncells, part, i, j, acc);
   const int cr = cidx ? cidx[nd] : -1;
   const int mask = cr >= 0 ? cmask[cr] : 0;
   const T m = acc[0];
@@ -301,7 +310,7 @@ __global__ __launch_bounds__(kThreads) void k_node_kin(Grid2<T> g, const T* __re
       p[d] = m * c;
     }
     NR(DMPM_N_P + d)[nd] = p[d];
-    vkin[(int64_t)d * g.nnodes + nd] = v;
+    vkin[(int64_t)d * M.nnodes + nd] = v;
   }
 }
 
@@ -310,22 +319,25 @@ __global__ __launch_bounds__(kThreads) void k_node_kin(Grid2<T> g, const T* __re
 // (element.py:305-329) and constraints.
 template <typename T>
 __global__ __launch_bounds__(kThreads) void k_node_force(
-    Grid2<T> g, const T* __restrict__ part, T* __restrict__ nb, int64_t nld,
+    Mesh2<T> M, const T* __restrict__ part, T* __restrict__ nb, int64_t nld,
     const T* __restrict__ vkin, const int* __restrict__ cidx, const int* __restrict__ cmask,
     const double* __restrict__ cval, T dt, T t, const int* __restrict__ nfd,
     const int* __restrict__ nf_off, const int* __restrict__ nf_fn,
     const double* __restrict__ nf_val, FnTable fns) {
   const int nd = blockIdx.x * blockDim.x + threadIdx.x;
-  if (nd >= g.nnodes) return;
-  const int i = nd % (g.nx + 1), j = nd / (g.nx + 1);
+  if (nd >= M.nnodes) return;
+  const int sm = M.sim_n(nd);
+  const Grid2<T> g = M.grid(sm);
+  const int nl = nd - M.noff(sm);
+  const int i = nl % (g.nx + 1), j = nl / (g.nx + 1);
   T acc[NQ_FORCE];
-  node_gather<T, NQ_FORCE>(g, part, i, j, acc);
+  node_gather<T, NQ_FORCE>(g, M.coff(sm), M.ncells, part, i, j, acc);
   const int cr = cidx ? cidx[nd] : -1;
   const int mask = cr >= 0 ? cmask[cr] : 0;
   const T m = NR(DMPM_N_MASS)[nd];
   for (int d = 0; d < 2; ++d) {
     T fe = acc[d];
-    const int qg = nfd ? nfd[(int64_t)d * g.nnodes + nd] : -1;
+    const int qg = nfd ? nfd[(int64_t)d * M.nnodes + nd] : -1;
     if (qg >= 0)
       for (int k = nf_off[qg]; k < nf_off[qg + 1]; ++k)
         fe = fe + fn_value<T>(fns, nf_fn[k], t) * T(nf_val[k]);
@@ -334,7 +346,7 @@ __global__ __launch_bounds__(kThreads) void k_node_force(
     NR(DMPM_N_FINT + d)[nd] = fi;
     const T F = (fi + fe) + NR(DMPM_N_FDAMP + d)[nd];
     T a = nan_to_num(F / m);
-    T v = vkin[(int64_t)d * g.nnodes + nd] + a * dt;
+    T v = vkin[(int64_t)d * M.nnodes + nd] + a * dt;
     T pm = m * v;
     v = snap(v);
     a = snap(a);
@@ -380,27 +392,24 @@ MoveLog move_log(dmpm_plan* P) {
   return L;
 }
 
-__device__ __forceinline__ int key_of(int e, int ncells) {
-  return (e >= 0 && e < ncells) ? e : ncells;
-}
-
-// Records that particle p moves from cell (key) ko to element id enew: the
-// record joins the leaver list of ko and the arrival list of key(enew), and
-// the per-cell and per-chunk counters.  Past movcap records, or a list longer
-// than kListMax, a full re-sort is requested instead.
-__device__ __forceinline__ void record_move(const MoveLog& L, int ncells, int p, int ko, int enew) {
+// Records that particle p moves from cell (key) ko to cell kn, element id
+// enew of its own mesh: the record joins the leaver list of ko and the
+// arrival list of kn, and the per-cell and per-chunk counters.  Past movcap
+// records, or a list longer than kListMax, a full re-sort is requested
+// instead.
+__device__ __forceinline__ void record_move(const MoveLog& L, int p, int ko, int enew, int kn) {
   const int m = atomicAdd(&L.flags[5], 1);
   if (m >= L.movcap) {
     L.flags[6] = 1;
     return;
   }
-  const int kn = key_of(enew, ncells);
   int* r = L.mov + (int64_t)kMovRec * m;
   r[0] = p;
   r[1] = ko;
   r[2] = enew;
   r[3] = atomicExch(&L.headL[ko], m);
   r[4] = atomicExch(&L.headA[kn], m);
+  r[5] = kn;
   L.mrec[p] = m + 1;
   const int cl = atomicAdd(&L.dL[ko], 1);
   const int ca = atomicAdd(&L.dA[kn], 1);
@@ -414,12 +423,14 @@ __device__ __forceinline__ void record_move(const MoveLog& L, int ncells, int p,
 // (scheme.py:127-133).  Records the particles that change cell for the next
 // re-sort.
 template <typename T, bool USL>
-__global__ __launch_bounds__(kThreads) void k_g2p_tile(Grid2<T> g, T* __restrict__ pb, int64_t ld,
+__global__ __launch_bounds__(kThreads) void k_g2p_tile(Mesh2<T> M, T* __restrict__ pb, int64_t ld,
                                                        int n, const T* __restrict__ nb,
                                                        int64_t nld, MoveLog mlog, Mat2 mat,
                                                        T dt, int vu, int temps) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= n) return;
+  const int ps = M.sim_p(p);
+  const Grid2<T> g = M.grid(ps);
   const T x = PR(DMPM_P_X)[p], y = PR(DMPM_P_X + 1)[p];
   const int e = element_id<T, 2>(g.geo(), x, y);
   const bool inside = e >= 0 && e < g.ncells;
@@ -450,14 +461,14 @@ __global__ __launch_bounds__(kThreads) void k_g2p_tile(Grid2<T> g, T* __restrict
       PR(DMPM_P_DVOLSTRAIN)[p] = nan;
     }
     // NaN positions search to element 0 (element.py:827-837)
-    record_move(mlog, g.ncells, p, g.ncells, 0);
+    record_move(mlog, p, M.ncells, 0, M.key(ps, g, 0));
     return;
   }
   const int I = e % g.nx, J = e / g.nx;
   const T xi = natural(x, g.X(I), g.X(I + 1));
   const T eta = natural(y, g.Y(J), g.Y(J + 1));
-  const int ll = J * (g.nx + 1) + I;
-  const int ids[4] = {ll, ll + 1, ll + g.nx + 2, ll + g.nx + 1};
+  int ids[4];
+  cell_nodes(g.nx, I, J, ids, M.noff(ps));
   T N[4];
   q4_N4(xi, eta, N);
   T vx[4], vy[4];
@@ -489,7 +500,7 @@ __global__ __launch_bounds__(kThreads) void k_g2p_tile(Grid2<T> g, T* __restrict
     for (int k = 0; k < 6; ++k) d[k] = rr[k] * dt;
     const T dvol = dt * (rc[0] + rc[1]);
     T ds[4];
-    stress_inc(mat, mat.per_E ? PR(DMPM_P_YOUNGS)[p] : T(0), d, ds);
+    stress_inc(M.mat(mat, ps), mat.per_E ? PR(DMPM_P_YOUNGS)[p] : T(0), d, ds);
     const T f = T(1) + dvol;
     PR(DMPM_P_STRAIN + 0)[p] = PR(DMPM_P_STRAIN + 0)[p] + d[0];
     PR(DMPM_P_STRAIN + 1)[p] = PR(DMPM_P_STRAIN + 1)[p] + d[1];
@@ -507,20 +518,23 @@ __global__ __launch_bounds__(kThreads) void k_g2p_tile(Grid2<T> g, T* __restrict
     }
   }
   const int e1 = element_id<T, 2>(g.geo(), x1, y1);
-  const int k1 = (e1 >= 0 && e1 < g.ncells) ? e1 : g.ncells;
-  if (k1 != e) record_move(mlog, g.ncells, p, e, e1);
+  const int k1 = M.key(ps, g, e1);
+  const int k0 = M.key(ps, g, e);
+  if (k1 != k0) record_move(mlog, p, k0, e1, k1);
 }
 
 // USF strain / volume / stress update (particle.py:249-327) streamed over the
 // particles in storage order, in place, from the kinematic nodal velocities;
 // used before k_tile<FORCE_USL> when USF runs split (DMPM_OPT_USF_SPLIT).
 template <typename T>
-__global__ __launch_bounds__(kThreads) void k_stress_tile(Grid2<T> g, T* __restrict__ pb,
+__global__ __launch_bounds__(kThreads) void k_stress_tile(Mesh2<T> M, T* __restrict__ pb,
                                                           int64_t ld, int n,
                                                           const T* __restrict__ vkin, Mat2 mat,
                                                           T dt, int temps) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= n) return;
+  const int ps = M.sim_p(p);
+  const Grid2<T> g = M.grid(ps);
   const T x = PR(DMPM_P_X)[p], y = PR(DMPM_P_X + 1)[p];
   const int e = element_id<T, 2>(g.geo(), x, y);
   if (!(e >= 0 && e < g.ncells)) {  // outside the mesh: the state becomes NaN
@@ -544,12 +558,12 @@ __global__ __launch_bounds__(kThreads) void k_stress_tile(Grid2<T> g, T* __restr
   const int I = e % g.nx, J = e / g.nx;
   const T xi = natural(x, g.X(I), g.X(I + 1));
   const T eta = natural(y, g.Y(J), g.Y(J + 1));
-  const int ll = J * (g.nx + 1) + I;
-  const int ids[4] = {ll, ll + 1, ll + g.nx + 2, ll + g.nx + 1};
+  int ids[4];
+  cell_nodes(g.nx, I, J, ids, M.noff(ps));
   T vx[4], vy[4];
   for (int k = 0; k < 4; ++k) {
     vx[k] = vkin[ids[k]];
-    vy[k] = vkin[g.nnodes + ids[k]];
+    vy[k] = vkin[M.nnodes + ids[k]];
   }
   T rr[6], rc[6], d[6];
   rate2(g, I, J, xi, eta, vx, vy, rr);
@@ -557,7 +571,7 @@ __global__ __launch_bounds__(kThreads) void k_stress_tile(Grid2<T> g, T* __restr
   for (int k = 0; k < 6; ++k) d[k] = rr[k] * dt;
   const T dvol = dt * (rc[0] + rc[1]);
   T ds[4];
-  stress_inc(mat, mat.per_E ? PR(DMPM_P_YOUNGS)[p] : T(0), d, ds);
+  stress_inc(M.mat(mat, ps), mat.per_E ? PR(DMPM_P_YOUNGS)[p] : T(0), d, ds);
   const T f = T(1) + dvol;
   PR(DMPM_P_STRAIN + 0)[p] = PR(DMPM_P_STRAIN + 0)[p] + d[0];
   PR(DMPM_P_STRAIN + 1)[p] = PR(DMPM_P_STRAIN + 1)[p] + d[1];
@@ -582,26 +596,28 @@ __global__ __launch_bounds__(kThreads) void k_stress_tile(Grid2<T> g, T* __restr
 // change whose recorded cell no longer holds, or one never sorted, makes the
 // next re-sort a full one.
 template <typename T>
-__global__ void k_tile_check(Grid2<T> g, const T* __restrict__ pb, int64_t ld, int n,
+__global__ void k_tile_check(Mesh2<T> M, const T* __restrict__ pb, int64_t ld, int n,
                              int* __restrict__ eid, const int* __restrict__ key, MoveLog L) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= n) return;
+  const int ps = M.sim_p(p);
+  const Grid2<T> g = M.grid(ps);
   const int e = element_id<T, 2>(g.geo(), PR(DMPM_P_X)[p], PR(DMPM_P_X + 1)[p]);
-  const int k = key_of(e, g.ncells);
+  const int k = M.key(ps, g, e);
   const int m = L.mrec[p] - 1;
   if (m >= 0) {
     int* r = L.mov + (int64_t)kMovRec * m;
-    if (key_of(r[2], g.ncells) == k) r[2] = e;
+    if (r[5] == k) r[2] = e;
     else L.flags[0] = 1;
     return;
   }
   const int ko = key[p];
   if (ko == k) {
     eid[p] = e;
-  } else if (ko < 0 || ko > g.ncells) {
+  } else if (ko < 0 || ko > M.ncells) {
     L.flags[0] = 1;
   } else {
-    record_move(L, g.ncells, p, ko, e);
+    record_move(L, p, ko, e, k);
   }
 }
 
@@ -711,7 +727,7 @@ __device__ __forceinline__ bool last_block(int* counter) {
 // ids and keys from the positions, cell histogram, chunk totals (LDS
 // histogram per workgroup), release of the pending change lists.
 template <typename T>
-__global__ __launch_bounds__(kThreads) void k_sort_s1(Grid2<T> g, const T* __restrict__ pb,
+__global__ __launch_bounds__(kThreads) void k_sort_s1(Mesh2<T> Mg, const T* __restrict__ pb,
                                                       int64_t ld, SortArgs a) {
   const int mode = sort_mode(a);
   if (mode == SORT_NONE) return;
@@ -734,9 +750,11 @@ __global__ __launch_bounds__(kThreads) void k_sort_s1(Grid2<T> g, const T* __res
   for (int k = tid; k < a.nchunks; k += kThreads) sh[k] = 0;
   __syncthreads();
   for (int p = gt; p < a.n; p += gs) {
+    const int ps = Mg.sim_p(p);
+    const Grid2<T> g = Mg.grid(ps);
     const int e = element_id<T, 2>(g.geo(), PR(DMPM_P_X)[p], PR(DMPM_P_X + 1)[p]);
     a.eid[p] = e;
-    const int k = key_of(e, a.ncells);
+    const int k = Mg.key(ps, g, e);
     a.key[p] = k;
     atomicAdd(&a.counts[k], 1);
     atomicAdd(&sh[k >> a.L.chunk_shift], 1);
@@ -747,7 +765,7 @@ __global__ __launch_bounds__(kThreads) void k_sort_s1(Grid2<T> g, const T* __res
   const int nrec = min(a.L.flags[5], a.L.movcap);
   for (int m = gt; m < nrec; m += gs) {
     const int* r = a.L.mov + (int64_t)kMovRec * m;
-    const int ko = r[1], kn = key_of(r[2], a.ncells);
+    const int ko = r[1], kn = r[5];
     a.L.mrec[r[0]] = 0;
     a.L.dL[ko] = 0;
     a.L.dA[kn] = 0;
@@ -805,7 +823,7 @@ __global__ __launch_bounds__(kThreads) void k_sort_s2(SortArgs a) {
   const int nmov = a.L.flags[5];
   for (int m = gt; m < nmov; m += gs) {
     const int p = M[kMovRec * m], e = M[kMovRec * m + 2];
-    const int kn = key_of(e, a.ncells);
+    const int kn = M[kMovRec * m + 5];
     int l = co[kn], h = co[kn + 1];
     const int first = l;
     while (l < h) {
@@ -837,7 +855,7 @@ __global__ __launch_bounds__(kThreads) void k_sort_s3(SortArgs a) {
   const int nmov = a.L.flags[5];
   for (int m = gt; m < nmov; m += gs) {
     const int* r = a.L.mov + (int64_t)kMovRec * m;
-    const int p = r[0], ko = r[1], kn = key_of(r[2], a.ncells);
+    const int p = r[0], ko = r[1], kn = r[5];
     a.key[p] = kn;
     a.L.mrec[p] = 0;
     a.L.dL[ko] = 0;
@@ -933,7 +951,7 @@ __global__ __launch_bounds__(kThreads) void k_sort_s4(SortArgs a) {
 constexpr int kSortGrid = 2048;  // workgroups of the grid-stride re-sort stages
 
 template <typename T>
-int enqueue_resort(dmpm_plan* P, const Grid2<T>& g, const T* pb, int32_t* eid, hipStream_t s) {
+int enqueue_resort(dmpm_plan* P, const Mesh2<T>& M, const T* pb, int32_t* eid, hipStream_t s) {
   SortArgs a;
   a.eid = eid;
   a.key = P->key;
@@ -950,7 +968,7 @@ int enqueue_resort(dmpm_plan* P, const Grid2<T>& g, const T* pb, int32_t* eid, h
   a.ncells = P->ncells;
   a.nchunks = P->nchunks;
   const int g12 = std::max(P->nchunks, kSortGrid);
-  k_sort_s1<T><<<g12, kThreads, 0, s>>>(g, pb, P->d.ld, a);
+  k_sort_s1<T><<<g12, kThreads, 0, s>>>(M, pb, P->d.ld, a);
   k_sort_s2<<<g12, kThreads, 0, s>>>(a);
   k_sort_s3<<<kSortGrid, kThreads, 0, s>>>(a);
   k_sort_s4<<<kSortGrid / 4, kThreads, 0, s>>>(a);
@@ -958,15 +976,15 @@ int enqueue_resort(dmpm_plan* P, const Grid2<T>& g, const T* pb, int32_t* eid, h
 }
 
 template <typename T, int CB>
-int tile_steps(dmpm_plan* P, const Grid2<T>& g, T* pb, int32_t* eid, T* nb, int scheme, int vu,
+int tile_steps(dmpm_plan* P, const Mesh2<T>& M, T* pb, int32_t* eid, T* nb, int scheme, int vu,
                const double* grav, double dt, int64_t step0, int64_t nsteps, int temps,
                hipStream_t s, T* part, T* vkin) {
   const int64_t ld = P->d.ld, nld = P->d.nld;
   const int n = (int)P->n;
   const bool usf = scheme == DMPM_USF;
   const bool split = usf && P->usf_split;
-  const unsigned ncb = (unsigned)((g.ncells + CB - 1) / CB);
-  const unsigned nnb = nblk(g.nnodes);
+  const unsigned ncb = (unsigned)((M.ncells + CB - 1) / CB);
+  const unsigned nnb = nblk(M.nnodes);
   Mat2 mat;
   mat.kind = P->d.material;
   mat.per_E = P->d.per_particle_E;
@@ -983,30 +1001,30 @@ int tile_steps(dmpm_plan* P, const Grid2<T>& g, T* pb, int32_t* eid, T* nb, int 
   for (int64_t step = step0; step < step0 + nsteps; ++step) {
     {
       ProfScopeT ps_(P, KC_SORT, s);
-      DMPM_TRY(enqueue_resort<T>(P, g, pb, eid, s));
+      DMPM_TRY(enqueue_resort<T>(P, M, pb, eid, s));
     }
     const T t = T(step) * tdt;
     {
       ProfScopeT ps_(P, KC_TILE_KIN, s);
-      k_cell_kin<T, CB><<<ncb, kThreads, 0, s>>>(g, pb, ld, sr, part, temps);
-      k_node_kin<T><<<nnb, kThreads, 0, s>>>(g, part, nb, nld, vkin, P->cidx, P->cmask,
+      k_cell_kin<T, CB><<<ncb, kThreads, 0, s>>>(M, pb, ld, sr, part, temps);
+      k_node_kin<T><<<nnb, kThreads, 0, s>>>(M, part, nb, nld, vkin, P->cidx, P->cmask,
                                              P->cval);
       DMPM_TRY(launch_check("p2g_kin"));
     }
     if (split) {
       ProfScopeT ps_(P, KC_STRESS, s);
-      k_stress_tile<T><<<nblk(n), kThreads, 0, s>>>(g, pb, ld, n, vkin, mat, tdt, temps);
+      k_stress_tile<T><<<nblk(n), kThreads, 0, s>>>(M, pb, ld, n, vkin, mat, tdt, temps);
       DMPM_TRY(launch_check("stress"));
     }
     {
       ProfScopeT ps_(P, KC_TILE_FORCE, s);
       if (usf && !split)
-        k_cell_force<T, CB, true><<<ncb, kThreads, 0, s>>>(g, pb, ld, sr, part, vkin, mat, gx,
+        k_cell_force<T, CB, true><<<ncb, kThreads, 0, s>>>(M, pb, ld, sr, part, vkin, mat, gx,
                                                            gy, tdt, temps);
       else
-        k_cell_force<T, CB, false><<<ncb, kThreads, 0, s>>>(g, pb, ld, sr, part, vkin, mat, gx,
+        k_cell_force<T, CB, false><<<ncb, kThreads, 0, s>>>(M, pb, ld, sr, part, vkin, mat, gx,
                                                             gy, tdt, temps);
-      k_node_force<T><<<nnb, kThreads, 0, s>>>(g, part, nb, nld, vkin, P->cidx, P->cmask,
+      k_node_force<T><<<nnb, kThreads, 0, s>>>(M, part, nb, nld, vkin, P->cidx, P->cmask,
                                                P->cval, tdt, t, P->nfd, P->nf_off, P->nf_fn,
                                                P->nf_val, fns);
       DMPM_TRY(launch_check("p2g_force"));
@@ -1014,10 +1032,10 @@ int tile_steps(dmpm_plan* P, const Grid2<T>& g, T* pb, int32_t* eid, T* nb, int 
     {
       ProfScopeT ps_(P, KC_TILE_G2P, s);
       if (usf)
-        k_g2p_tile<T, false><<<nblk(n), kThreads, 0, s>>>(g, pb, ld, n, nb, nld, move_log(P), mat,
+        k_g2p_tile<T, false><<<nblk(n), kThreads, 0, s>>>(M, pb, ld, n, nb, nld, move_log(P), mat,
                                                           tdt, vu, temps);
       else
-        k_g2p_tile<T, true><<<nblk(n), kThreads, 0, s>>>(g, pb, ld, n, nb, nld, move_log(P), mat,
+        k_g2p_tile<T, true><<<nblk(n), kThreads, 0, s>>>(M, pb, ld, n, nb, nld, move_log(P), mat,
                                                          tdt, vu, temps);
       DMPM_TRY(launch_check("g2p"));
     }
@@ -1028,10 +1046,10 @@ int tile_steps(dmpm_plan* P, const Grid2<T>& g, T* pb, int32_t* eid, T* nb, int 
 template <typename T>
 int tile_run_t(dmpm_plan* P, T* pb, int32_t* eid, T* nb, int scheme, int vu, const double* grav,
                double dt, int64_t step0, int64_t nsteps, int temps, hipStream_t s) {
-  const Grid2<T> g = grid_of<T>(P);
+  const Mesh2<T> M = mesh_of<T>(P);
   const int64_t ld = P->d.ld;
   const int n = (int)P->n;
-  T* part = (T*)plan_workspace(P, 0, (size_t)4 * NQ_FORCE * g.ncells * sizeof(T));
+  T* part = (T*)plan_workspace(P, 0, (size_t)4 * NQ_FORCE * M.ncells * sizeof(T));
   T* vkin = (T*)plan_workspace(P, 1, (size_t)2 * P->nnodes * sizeof(T));
   if (!part || !vkin) return fail(DMPM_EHIP, "workspace allocation failed");
   // apply the cell changes still pending from the last step of the previous
@@ -1039,19 +1057,19 @@ int tile_run_t(dmpm_plan* P, T* pb, int32_t* eid, T* nb, int scheme, int vu, con
   // unless the layout is unknown)
   {
     ProfScopeT ps_(P, KC_SORT, s);
-    DMPM_TRY(enqueue_resort<T>(P, g, pb, eid, s));
+    DMPM_TRY(enqueue_resort<T>(P, M, pb, eid, s));
   }
-  k_tile_check<T><<<nblk(n), kThreads, 0, s>>>(g, pb, ld, n, eid, P->key, move_log(P));
+  k_tile_check<T><<<nblk(n), kThreads, 0, s>>>(M, pb, ld, n, eid, P->key, move_log(P));
   DMPM_TRY(launch_check("tile_check"));
   switch (P->cell_block) {
     case 64:
-      return tile_steps<T, 64>(P, g, pb, eid, nb, scheme, vu, grav, dt, step0, nsteps, temps, s,
+      return tile_steps<T, 64>(P, M, pb, eid, nb, scheme, vu, grav, dt, step0, nsteps, temps, s,
                                part, vkin);
     default:
-      return tile_steps<T, 128>(P, g, pb, eid, nb, scheme, vu, grav, dt, step0, nsteps, temps, s,
+      return tile_steps<T, 128>(P, M, pb, eid, nb, scheme, vu, grav, dt, step0, nsteps, temps, s,
                                 part, vkin);
     case 256:
-      return tile_steps<T, 256>(P, g, pb, eid, nb, scheme, vu, grav, dt, step0, nsteps, temps, s,
+      return tile_steps<T, 256>(P, M, pb, eid, nb, scheme, vu, grav, dt, step0, nsteps, temps, s,
                                 part, vkin);
   }
 }

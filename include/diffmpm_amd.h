@@ -107,6 +107,20 @@ typedef struct dmpm_plan_desc {
 /* Allocates the sort workspace for one particle set on one mesh.  No
  * reference counterpart (the reference has no persistent workspace). */
 int dmpm_plan_create(const dmpm_plan_desc* desc, dmpm_plan** out);
+/* A batch plan: nsims independent 2D Quadrilateral4Node simulations (own
+ * mesh, particles, material coefficients de, constraints) stepped together,
+ * one launch of each kernel per step for all of them.  Their particles are
+ * laid end to end in one particle block (simulation s owns the columns
+ * [sum of the earlier nparticles, + its nparticles)), their nodes end to end
+ * in one node block, likewise; ld and nld are those blocks' row strides.
+ * Node ids in the constraint and nodal-force tables are global (offset by the
+ * earlier simulations' node counts); element ids stay per simulation.  All
+ * simulations share dtype, material kind, per_particle_E, scheme, time step
+ * and gravity.  Batch plans run dmpm_run's fused 2D path and
+ * dmpm_step_adjoint; the per-method entry points reject them.  No reference
+ * counterpart (the reference runs one simulation per MPMExplicit). */
+int dmpm_batch_plan_create(const dmpm_plan_desc* descs, int32_t nsims, int64_t ld, int64_t nld,
+                           dmpm_plan** out);
 int dmpm_plan_destroy(dmpm_plan* plan);
 const char* dmpm_last_error(void);
 /* ABI version (major*100 + minor) and the row counts above, for bindings. */
