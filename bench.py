@@ -85,6 +85,45 @@ def cpu_baseline(rows=32, steps=8):
                       f"{steps} steps after 1 warm-up ({el_s:.1f} s)"}
 
 
+def ensemble_bench(K):
+    """64 uneven USL members (32..256 cells per side, 30-100 % of the rows
+    filled, particles at the Gauss points): K steps with one batch plan,
+    against the same K steps of every member with its own plan, one member
+    after the other."""
+    import torch
+    from diffmpm_amd import workloads
+    from diffmpm_amd._plan import plan_for
+    from diffmpm_amd.ensemble import Ensemble
+    sims, g, specs = workloads.ensemble(64, seed=0)
+    ntot = sum(len(s.mesh.particles[0]) for s in sims)
+    plans = [plan_for(s.mesh.elements, s.mesh.particles[0]) for s in sims]
+    for p, s in zip(plans, sims):  # warm-up (workspaces, first full sort)
+        p.run("usl", False, g, s.dt, 0, 2, keep_temporaries=False)
+    ens = Ensemble(sims)
+    ens.run(2, g, start_step=2)
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    e0, e1, e2 = (torch.cuda.Event(enable_timing=True) for _ in range(3))
+    e0.record(stream)
+    ens.run(K, g, start_step=4)
+    e1.record(stream)
+    for p, s in zip(plans, sims):
+        p.run("usl", False, g, s.dt, 4, K, keep_temporaries=False)
+    e2.record(stream)
+    torch.cuda.synchronize()
+    tb, ts = e0.elapsed_time(e1), e1.elapsed_time(e2)
+    assert torch.isfinite(ens.pbuf[0:2, :ens.n]).all()
+    return {"workload": "64 uneven Q4 members, n = 32..256 cells per side (log-uniform), "
+                        "30-100 % rows filled at the Gauss points, E 1e5..1e7, nu 0.2..0.4, "
+                        "USL, dt=1e-5, seed 0",
+            "particles": ntot, "steps": K,
+            "batched": {"value": ntot * K / (tb / 1e3), "unit": "particle-steps/s",
+                        "ms_per_step": tb / K},
+            "sequential": {"value": ntot * K / (ts / 1e3), "unit": "particle-steps/s",
+                           "ms_per_step": ts / K},
+            "speedup": ts / tb}
+
+
 def fwd_adjoint(sim, gravity, K):
     """Throughput of a differentiable rollout: K forward steps recorded for
     autograd, then the backward pass (per step: recomputation of the forward
@@ -98,8 +137,12 @@ def fwd_adjoint(sim, gravity, K):
     log_e = torch.full((N,), math.log(1e6), dtype=d.ps.dtype, device=d.ps.device,
                        requires_grad=True)
     state0 = d.get_state().requires_grad_(True)
-    out = rollout(d, 1, gravity, state=state0, log_e=log_e)  # warm-up
+    # warm-up of the same length (fills the allocator's cache for the saved
+    # states and cotangents)
+    out = rollout(d, K, gravity, state=state0, log_e=log_e)
     out[0:2].sum().backward()
+    log_e.grad = None
+    state0.grad = None
     torch.cuda.synchronize()
     timing_enable(d.aplan, "adj")
     stream = torch.cuda.current_stream()
@@ -139,6 +182,8 @@ def main():
                     help="cells per workgroup of the P2G cell pass (64/128/256; 0: default)")
     ap.add_argument("--usf-fused", action="store_true",
                     help="update stresses inside the force P2G cell pass")
+    ap.add_argument("--ensemble-steps", type=int, default=50,
+                    help="steps of the timed 64-member uneven ensemble (0: skip)")
     ap.add_argument("--adjoint-steps", type=int, default=20,
                     help="steps of the timed forward+adjoint rollout (0: skip)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -216,6 +261,10 @@ def main():
     }
     if args.adjoint_steps:
         line["fwd_adjoint"] = fwd_adjoint(sim, gravity, args.adjoint_steps)
+    if args.ensemble_steps:
+        del sim, ps, plan
+        torch.cuda.empty_cache()
+        line["ensemble"] = ensemble_bench(args.ensemble_steps)
     if args.copy_calibration:
         a = torch.empty(128 * 1024 * 1024, dtype=torch.float32, device="cuda")
         b = torch.empty_like(a)

@@ -3,7 +3,8 @@
 ``block_2d`` builds the 2D uniform-grid cases: Quadrilateral4Node mesh of
 n x n cells on [0,1]^2, 4 particles per cell at the 2x2 Gauss points of the
 lower ``rows`` cell rows (optionally jittered uniformly by +-jitter*h per
-coordinate, then clamped into [0, 1) so no particle leaves the mesh),
+coordinate, then clamped into [0, X_n) with X_n = fl32(n fl32(1/n)) the last
+node coordinate, so that no particle starts outside the mesh),
 LinearElastic, gravity (0, -9.81), the bottom node row fixed in x and y and
 the left and right node columns fixed in x (rollers).  Without the rollers
 the side particles leave the mesh within a few steps under the lateral
@@ -28,9 +29,41 @@ def gauss_points(nx, rows, h, jitter=0.0, seed=0):
         rng = np.random.Generator(np.random.PCG64(seed))
         pts += rng.uniform(-jitter, jitter, size=pts.shape)
     pts *= h
-    hi = np.nextafter(np.float32(1.0), np.float32(0.0))
+    # stay below the last node coordinate fl32(n * fl32(h)) (1 for n = 1024):
+    # points at or beyond it belong to the next cell row (element.py:820-840)
+    last = np.float32(nx) * np.float32(h)
+    hi = float(np.nextafter(last, np.float32(0.0)))
     np.clip(pts, 0.0, hi, out=pts)
     return pts
+
+
+def ensemble_specs(nsims=64, seed=0):
+    """Member parameters of the uneven ensemble: n = round(exp(U(ln 32,
+    ln 256))) cells per side, rows = ceil(U(0.3, 1.0) n) particle rows,
+    E = exp(U(ln 1e5, ln 1e7)), nu = U(0.2, 0.4); one seeded generator."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    out = []
+    for _ in range(nsims):
+        n = int(round(math.exp(rng.uniform(math.log(32), math.log(256)))))
+        rows = int(math.ceil(rng.uniform(0.3, 1.0) * n))
+        E = float(math.exp(rng.uniform(math.log(1e5), math.log(1e7))))
+        nu = float(rng.uniform(0.2, 0.4))
+        out.append(dict(n=n, rows=min(rows, n), E=E, nu=nu))
+    return out
+
+
+def ensemble(nsims=64, seed=0, scheme="usl", dtype=torch.float32, device="cuda"):
+    """The uneven ensemble: ``nsims`` block_2d members (particles at the 2x2
+    Gauss points of the cells of the lower ``rows`` rows, no jitter, bottom
+    fixed, side rollers, gravity, rho = 1000, dt = 1e-5, USL).  Returns (list
+    of MPMExplicit, gravity, member parameters)."""
+    specs = ensemble_specs(nsims, seed)
+    sims = []
+    for k, m in enumerate(specs):
+        sim, g = block_2d(m["n"], rows=m["rows"], jitter=0.0, seed=seed + 1 + k, E=m["E"],
+                          nu=m["nu"], scheme=scheme, dtype=dtype, device=device)
+        sims.append(sim)
+    return sims, g, specs
 
 
 def block_2d(n=1024, rows=None, jitter=0.25, seed=0, E=1e6, nu=0.3, density=1000.0,
