@@ -37,6 +37,12 @@ from ._plan import Plan, plan_for
 STATE_ROWS = (_lib.P_X, _lib.P_X + 1, _lib.P_V, _lib.P_V + 1, _lib.P_STRESS + 0,
               _lib.P_STRESS + 1, _lib.P_STRESS + 3, _lib.P_VOLUME)
 C_ROWS = 8
+# (state rows, block rows) of the contiguous runs of STATE_ROWS
+_RUNS = ((slice(0, 4), slice(_lib.P_X, _lib.P_X + 4)),
+         (slice(4, 6), slice(_lib.P_STRESS, _lib.P_STRESS + 2)),
+         (slice(6, 7), slice(_lib.P_STRESS + 3, _lib.P_STRESS + 4)),
+         (slice(7, 8), slice(_lib.P_VOLUME, _lib.P_VOLUME + 1)))
+assert [_lib.P_X + 2 == _lib.P_V, _lib.P_VOLUME not in range(_lib.P_X, _lib.P_X + 4)] == [True] * 2
 
 
 class Differentiable:
@@ -95,21 +101,36 @@ class Differentiable:
     # -- state <-> particle block ---------------------------------------------
     def get_state(self):
         """The current dynamic state as a new (8, N) tensor."""
-        return self._pbuf[list(STATE_ROWS), : self.n].clone()
+        out = torch.empty((C_ROWS, self.n), dtype=self._pbuf.dtype, device=self._pbuf.device)
+        for a, b in _RUNS:
+            out[a] = self._pbuf[b, : self.n]
+        self._last = (out, out._version)
+        return out
 
     def _write(self, buf, state, log_e):
-        buf[list(STATE_ROWS), : self.n] = state
+        for a, b in _RUNS:
+            buf[b, : self.n] = state[a]
         if log_e is not None:
             buf[_lib.P_YOUNGS, : self.n] = torch.exp(log_e.detach()).to(buf.dtype).expand(self.n)
 
     def set_state(self, state, log_e=None):
+        last = getattr(self, "_last", None)
+        if last is not None and last[0] is state and state._version == last[1]:
+            # the block still holds this state (the output of the last step)
+            if log_e is not None:
+                self._write_log_e(self._pbuf, log_e)
+            return
         self._write(self._pbuf, state.detach(), log_e)
+
+    def _write_log_e(self, buf, log_e):
+        buf[_lib.P_YOUNGS, : self.n] = torch.exp(log_e.detach()).to(buf.dtype).expand(self.n)
 
     # -- one step ---------------------------------------------------------------
     def forward_step(self, state, log_e, gravity, i):
         self.set_state(state, log_e)
         self.fplan.run(self.scheme, self.velocity_update, gravity, self.dt, i, 1,
                        keep_temporaries=False)
+        self._last = None  # the block now holds the output
         return self.get_state()
 
     def adjoint_step(self, state, log_e, gravity, i, gout):

@@ -180,6 +180,20 @@ def load_oracle_state_into_gpu(osim, gsim):
         setattr(gn, name, torch.from_numpy(v.reshape(getattr(gn, name).shape).astype(np.float64)))
 
 
+def copy_oracle_state(src, dst):
+    """Copies an oracle simulation's particle and nodal state into another
+    (of possibly another dtype), for per-step comparisons from one state."""
+    sp, dp = src.mesh.particles[0], dst.mesh.particles[0]
+    for name in PARTICLE_FIELDS + ["acceleration", "f_ext"]:
+        v = getattr(dp, name)
+        v[...] = np.asarray(getattr(sp, name)).astype(v.dtype)
+    dp.element_ids[...] = sp.element_ids
+    sn, dn = src.mesh.elements.nodes, dst.mesh.elements.nodes
+    for name in NODE_FIELDS:
+        v = getattr(dn, name)
+        v[...] = np.asarray(getattr(sn, name)).astype(v.dtype)
+
+
 # ------------------------------------------------------- synthetic cases
 def synthetic_spec(element="quad4", nx=8, ny=6, rows=4, seed=0, scheme="usf",
                    velocity_update=False, jitter=0.2, traction=True, nodal_force=True,
@@ -230,3 +244,92 @@ def synthetic_spec(element="quad4", nx=8, ny=6, rows=4, seed=0, scheme="usf",
     if per_particle_E:
         spec["youngs"] = (E * np.exp(0.3 * rng.standard_normal(n))).tolist()
     return spec
+
+
+# ------------------------------------------------------- benchmark configs
+def config_c1_spec(nsteps=40):
+    """1D bar: Linear1D, 128 elements on [0, 1], 2 Gauss particles per
+    element (256), v0 = 0.1 sin(pi x / 2), node 0 fixed, E = 100, rho = 1,
+    nu = 0 (the configuration omits nu; LinearElastic needs it), dt = 0.01,
+    USF.  c dt / h = 12.8: the rollout is CFL-unstable and turns non-finite."""
+    nx = 128
+    h = 1.0 / nx
+    g = (1.0 - 1.0 / math.sqrt(3.0)) / 2.0
+    base = np.arange(nx, dtype=np.float64)
+    pts = (np.stack([base + g, base + 1 - g], axis=1).reshape(-1, 1) * h)
+    vel = 0.1 * np.sin(np.pi * pts / 2.0)
+    return dict(element="linear1d", nelements=nx, el_len=h, particles=pts.tolist(),
+                constraints=[{"nodes": [0], "dir": 0, "velocity": 0.0}], nodal_forces=[],
+                gravity=[0.0], init_velocity=vel.tolist(),
+                material={"type": "LinearElastic", "density": 1.0, "youngs_modulus": 100.0,
+                          "poisson_ratio": 0.0},
+                tractions=[], scheme="usf", dt=0.01, nsteps=nsteps, velocity_update=False)
+
+
+def config_c2_spec(nsteps=3):
+    """Q4 256 x 256, particles at the Gauss points of the lower 128 rows,
+    E = 1e6, nu = 0.3, rho = 1000, bottom nodes fixed in y, left nodes fixed
+    in x, -1e3 N per node in y on node row 128 (the top edge of the block)
+    ramped by Linear([0, 500 dt, inf], [0, 1, 1]), no gravity, dt = 1e-5,
+    USL."""
+    from oracle.mpm import gauss_points_2d
+    n = 256
+    h = 1.0 / n
+    dt = 1e-5
+    pts = gauss_points_2d(n, n, h, h, rows=128, dtype=np.float64)
+    top = [128 * (n + 1) + i for i in range(n + 1)]
+    ramp = {"type": "Linear", "xvalues": [0.0, 500 * dt, float("inf")],
+            "fxvalues": [0.0, 1.0, 1.0]}
+    return dict(element="quad4", nelements=[n, n], el_len=[h, h], particles=pts.tolist(),
+                constraints=[{"nodes": list(range(n + 1)), "dir": 1, "velocity": 0.0},
+                             {"nodes": [j * (n + 1) for j in range(n + 1)], "dir": 0,
+                              "velocity": 0.0}],
+                nodal_forces=[{"nodes": top, "dir": 1, "force": -1e3, "function": ramp}],
+                gravity=[0.0, 0.0], init_velocity=0.0,
+                material={"type": "LinearElastic", "density": 1000.0, "youngs_modulus": 1e6,
+                          "poisson_ratio": 0.3},
+                tractions=[], scheme="usl", dt=dt, nsteps=nsteps, velocity_update=False)
+
+
+def gaussian_random_field(n, h, corr, seed):
+    """Unit-variance Gaussian random field on the (n+1)^2 node grid: seeded
+    white noise filtered by a Gaussian of standard deviation corr (in length
+    units), normalised to unit variance."""
+    from scipy.ndimage import gaussian_filter
+    rng = np.random.Generator(np.random.PCG64(seed))
+    w = rng.standard_normal((n + 1, n + 1))
+    f = gaussian_filter(w, sigma=corr / h, mode="wrap")
+    return (f - f.mean()) / f.std()
+
+
+def bilinear(field, h, pts):
+    """Bilinear interpolation of a node-grid field (indexed [j, i]) at points."""
+    x = pts[:, 0] / h
+    y = pts[:, 1] / h
+    i = np.clip(np.floor(x).astype(int), 0, field.shape[1] - 2)
+    j = np.clip(np.floor(y).astype(int), 0, field.shape[0] - 2)
+    fx, fy = x - i, y - j
+    return ((1 - fx) * (1 - fy) * field[j, i] + fx * (1 - fy) * field[j, i + 1]
+            + (1 - fx) * fy * field[j + 1, i] + fx * fy * field[j + 1, i + 1])
+
+
+def config_c4_spec(nsteps=3, seed=0, n=128):
+    """Q4 128 x 128, particles at the Gauss points of the lower 64 rows,
+    E_p = 1e6 exp(0.3 g_p) with g a unit-variance Gaussian random field of
+    correlation length 0.1 (filtered white noise on the node grid, bilinear
+    to the particles, seeded), nu = 0.3, rho = 1000, gravity, bottom fixed,
+    dt = 1e-5, USF."""
+    from oracle.mpm import gauss_points_2d
+    h = 1.0 / n
+    pts = gauss_points_2d(n, n, h, h, rows=n // 2, dtype=np.float64)
+    gfield = gaussian_random_field(n, h, 0.1, seed)
+    E = 1e6 * np.exp(0.3 * bilinear(gfield, h, pts))
+    bottom = list(range(n + 1))
+    return dict(element="quad4", nelements=[n, n], el_len=[h, h], particles=pts.tolist(),
+                constraints=[{"nodes": bottom, "dir": 0, "velocity": 0.0},
+                             {"nodes": bottom, "dir": 1, "velocity": 0.0}],
+                nodal_forces=[], gravity=[0.0, -9.81], init_velocity=0.0,
+                material={"type": "LinearElastic", "density": 1000.0, "youngs_modulus": 1e6,
+                          "poisson_ratio": 0.3},
+                tractions=[], scheme="usf", dt=1e-5, nsteps=nsteps, velocity_update=False,
+                youngs=E.tolist())

@@ -1,0 +1,184 @@
+"""The benchmark configurations C1, C2, C4 through the HIP path against the
+oracle (re-synchronised per step, FP32 1e-5 / FP64 1e-12 normwise), and the
+C4 inverse problem's gradient (FP64 central differences on 16 particles,
+relative step 1e-3) and optimisation."""
+import math
+
+import numpy as np
+import pytest
+import torch
+
+import cases
+from test_gpu_parity import RTOL_F32, RTOL_F64, _compare
+
+pytestmark = pytest.mark.gpu
+
+
+def _per_step(spec, dtype, fused):
+    from diffmpm_amd._plan import plan_for
+    npdt = np.float64 if dtype == torch.float64 else np.float32
+    rtol = RTOL_F64 if dtype == torch.float64 else RTOL_F32
+    osim = cases.build_oracle(spec, npdt)
+    gsim = cases.build_gpu(spec, dtype)
+    g = torch.tensor(spec["gravity"], dtype=torch.float64)
+    compared = 0
+    for i in range(spec["nsteps"]):
+        cases.load_oracle_state_into_gpu(osim, gsim)
+        osim.step(i, np.asarray(spec["gravity"]))
+        if not np.all(np.isfinite(osim.mesh.particles[0].loc)):
+            break  # past the blow-up: nothing left to compare
+        if fused:
+            plan = plan_for(gsim.mesh.elements, gsim.mesh.particles[0])
+            plan.run(spec["scheme"], spec["velocity_update"], g, spec["dt"], i, 1, True)
+        else:
+            gsim._step(i, g)
+        _compare(osim, gsim, rtol, f"step {i}")
+        compared += 1
+    return compared
+
+
+def _inside_bar(osim):
+    """All particles finite and on the bar after the step.  A particle that
+    leaves the bar gets NaN state here (documented difference: the
+    reference's clamped gathers give finite garbage), so the comparison ends
+    there."""
+    x = np.asarray(osim.mesh.particles[0].loc, np.float64)
+    return bool(np.all(np.isfinite(x)) and np.all(x >= 0.0) and np.all(x < 1.0))
+
+
+def test_c1_bar_per_step_fp32_within_the_fp32_noise_floor():
+    """The 1D bar is ill-conditioned (interior internal forces cancel; the FP32
+    oracle alone differs from the FP64 oracle by ~6e-5 in f_int and ~6e-4 in
+    the accelerations after one step), so no FP32 implementation meets 1e-5
+    there.  Each step starts all three from the same state; the GPU's FP32
+    error to the FP32 oracle must stay within 4x the FP32 oracle's own error
+    to an FP64 oracle (or 1e-5), field by field, until the bar blows up."""
+    spec = cases.config_c1_spec(nsteps=40)
+    o32 = cases.build_oracle(spec, np.float32)
+    o64 = cases.build_oracle(spec, np.float64)
+    gsim = cases.build_gpu(spec, torch.float32)
+    g = torch.tensor(spec["gravity"], dtype=torch.float64)
+    steps = 0
+    for i in range(spec["nsteps"]):
+        cases.load_oracle_state_into_gpu(o32, gsim)
+        cases.copy_oracle_state(o32, o64)
+        o32.step(i, np.asarray(spec["gravity"]))
+        o64.step(i, np.asarray(spec["gravity"]))
+        if not _inside_bar(o32):
+            break
+        gsim._step(i, g)
+        ops, gps = o32.mesh.particles[0], gsim.mesh.particles[0]
+        for name in ("loc", "velocity", "stress", "strain", "volume"):
+            ref = cases.oracle_particle_field(ops, name)
+            floor = cases.normwise(ref, cases.oracle_particle_field(o64.mesh.particles[0], name))
+            err = cases.normwise(cases.gpu_particle_field(gps, name), ref)
+            assert err <= max(1e-5, 4 * floor), (i, name, err, floor)
+        for name in cases.NODE_FIELDS:
+            ref = cases.oracle_node_field(o32.mesh.elements.nodes, name)
+            floor = cases.normwise(ref, cases.oracle_node_field(o64.mesh.elements.nodes, name))
+            err = cases.normwise(cases.gpu_node_field(gsim.mesh.elements.nodes, name), ref)
+            assert err <= max(1e-5, 4 * floor), (i, name, err, floor)
+        steps += 1
+    # the free end (x = 0.998, v = 0.1, dt = 0.01) leaves the bar at step 1
+    assert steps >= 1
+
+
+def test_c1_bar_per_step_fp64():
+    """FP64: the one-step amplification of rounding seen in FP32 (~1e4) puts
+    the FP64 noise floor near 1e-12; the bound is 1e-10."""
+    spec = cases.config_c1_spec(nsteps=40)
+    osim = cases.build_oracle(spec, np.float64)
+    gsim = cases.build_gpu(spec, torch.float64)
+    g = torch.tensor(spec["gravity"], dtype=torch.float64)
+    steps = 0
+    for i in range(spec["nsteps"]):
+        cases.load_oracle_state_into_gpu(osim, gsim)
+        osim.step(i, np.asarray(spec["gravity"]))
+        if not _inside_bar(osim):
+            break
+        gsim._step(i, g)
+        _compare(osim, gsim, 1e-10, f"step {i}")
+        steps += 1
+    assert steps >= 1
+
+
+@pytest.mark.parametrize("fused", [False, True])
+def test_c2_usl_top_force_per_step(fused):
+    assert _per_step(cases.config_c2_spec(nsteps=3), torch.float32, fused) == 3
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+def test_c4_per_particle_e_per_step(dtype):
+    assert _per_step(cases.config_c4_spec(nsteps=3), dtype, fused=True) == 3
+
+
+def _c4_sim(dtype, n):
+    spec = cases.config_c4_spec(seed=0, n=n)
+    return cases.build_gpu(spec, dtype, keep_temporaries=False), spec
+
+
+def test_c4_log_e_gradient_matches_central_differences():
+    """FP64, a 6-step rollout with checkpoints every 2 steps, loss = a seeded
+    weighted sum of the final state (each row scaled to O(1)): d loss / d
+    log E_p on 16 particles against central differences (step 1e-3)."""
+    from diffmpm_amd.autodiff import Differentiable, rollout
+    sim, spec = _c4_sim(torch.float64, 32)
+    d = Differentiable(sim)
+    g = torch.tensor(spec["gravity"], dtype=torch.float64)
+    le0 = torch.log(d.ps.youngs_modulus.detach().clone())
+    state0 = d.get_state()
+    K, CK = 6, 2
+    with torch.no_grad():
+        ref = rollout(d, K, g, state=state0, log_e=le0)
+    gen = torch.Generator(device="cpu").manual_seed(3)
+    w = torch.randn(ref.shape, generator=gen, dtype=torch.float64).to(ref.device)
+    w = w / ref.abs().amax(dim=1, keepdim=True).clamp_min(1e-30)
+
+    def loss_of(le):
+        return (w * rollout(d, K, g, state=state0, log_e=le, checkpoint_every=CK)).sum()
+
+    le = le0.clone().requires_grad_(True)
+    loss_of(le).backward()
+    ad = le.grad.detach()
+    rng = np.random.Generator(np.random.PCG64(0))
+    pids = sorted(rng.choice(d.n, size=16, replace=False).tolist())
+    h = 1e-3
+    fd = []
+    with torch.no_grad():
+        for p in pids:
+            lp, lm = le0.clone(), le0.clone()
+            lp[p] += h
+            lm[p] -= h
+            fd.append((float(loss_of(lp)) - float(loss_of(lm))) / (2 * h))
+    fd = torch.tensor(fd, dtype=torch.float64)
+    got = ad[pids].cpu()
+    assert fd.abs().max() > 1e-8
+    assert torch.allclose(got, fd, rtol=1e-4, atol=1e-4 * fd.abs().max()), (got, fd)
+
+
+def test_c4_inverse_problem_reduces_the_loss():
+    """Adam (lr 1e-2) on log E from E = 1e6 against the stresses of the true
+    field after a short rollout: a few iterations lower the loss."""
+    from diffmpm_amd.autodiff import Differentiable, rollout
+    sim, spec = _c4_sim(torch.float32, 64)
+    d = Differentiable(sim)
+    g = torch.tensor(spec["gravity"], dtype=torch.float64)
+    state0 = d.get_state()
+    K = 20
+    with torch.no_grad():
+        target = rollout(d, K, g, state=state0,
+                         log_e=torch.log(d.ps.youngs_modulus.detach().clone()))[4:7].clone()
+    scale = target.abs().max()
+    le = torch.full((d.n,), math.log(1e6), dtype=torch.float32, device="cuda",
+                    requires_grad=True)
+    opt = torch.optim.Adam([le], lr=1e-2)
+    losses = []
+    for _ in range(6):
+        opt.zero_grad()
+        out = rollout(d, K, g, state=state0, log_e=le, checkpoint_every=5)
+        loss = (((out[4:7] - target) / scale) ** 2).sum()
+        loss.backward()
+        opt.step()
+        losses.append(float(loss))
+    assert all(math.isfinite(x) for x in losses)
+    assert losses[-1] < 0.9 * losses[0], losses
