@@ -44,10 +44,12 @@ STEP_BYTES = 116  # per particle-step, whole step (FP32 2D LinearElastic)
 # adjoint step: read primal x(2), v(2), m, V, sigma(xx, yy, xy); read and write
 # the cotangents of x(2), v(2), V, sigma(xx, yy, xy); write dL/dlogE_p
 ADJOINT_BYTES = 4 * 9 + 4 * 8 * 2 + 4
-# HBM bytes per particle per launch of the default path at 1024x1024, from
-# rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE passes of this benchmark
-# (profiles/step_q4_1024_usf_hbm_counters.md); None until measured.
-TRAFFIC_BYTES = {}
+# HBM bytes per particle per launch of each kernel class of the default path at
+# 1024x1024, from rocprofv3 --pmc passes of this benchmark: reads = TCC_EA0_RDREQ
+# x 128 B, writes = WRITE_SIZE (calibrated on a 512 MiB device copy; see
+# profiles/step_q4_1024_usf_hbm_counters.md)
+TRAFFIC_BYTES = {"sort": 23.6, "tile_kin": 57.7, "stress": 91.0, "tile_force": 86.4,
+                 "tile_g2p": 38.1}
 
 
 def cpu_baseline(rows=32, steps=8):
@@ -254,7 +256,8 @@ def main():
         "step_effective_GBs": STEP_BYTES * N * args.steps / (ms / 1e3) / 1e9,
         "roofline": {"bound": "hbm", "kernel": kname, "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": (traffic * N) if (traffic and args.n == 1024) else None,
+                     "traffic": (traffic * N) if (traffic and args.n == 1024
+                                                   and not args.usf_fused) else None,
                      "avg_launch_us": kd["avg_us"],
                      "bytes_per_particle": kd["bytes_per_particle"], "launches": kd["launches"]},
         "kernels": kernels,
