@@ -146,7 +146,8 @@ def fwd_adjoint(sim, gravity, K):
     log_e.grad = None
     state0.grad = None
     torch.cuda.synchronize()
-    timing_enable(d.aplan, "adj")
+    aplan = d.fplan if d.save_forward else d.aplan  # the plan the adjoint runs on
+    timing_enable(aplan, "adj")
     stream = torch.cuda.current_stream()
     e0, e1, e2 = (torch.cuda.Event(enable_timing=True) for _ in range(3))
     torch.cuda.synchronize()
@@ -157,8 +158,8 @@ def fwd_adjoint(sim, gravity, K):
     e2.record(stream)
     torch.cuda.synchronize()
     fwd_ms, bwd_ms = e0.elapsed_time(e1), e1.elapsed_time(e2)
-    adj_ms, adj_n = timing_read(d.aplan)
-    timing_enable(d.aplan, None)
+    adj_ms, adj_n = timing_read(aplan)
+    timing_enable(aplan, None)
     assert torch.isfinite(log_e.grad).all() and torch.isfinite(state0.grad).all()
     ms = (fwd_ms + bwd_ms) / K
     adj_us = adj_ms / max(adj_n, 1) * 1e3
@@ -168,8 +169,9 @@ def fwd_adjoint(sim, gravity, K):
             "backward_ms_per_step": bwd_ms / K,
             "adjoint_kernels_us_per_step": adj_us,
             "adjoint_kernels_GBs": ADJOINT_BYTES * N / (adj_us / 1e6) / 1e9,
-            "note": "backward = forward recomputation from the saved step input + adjoint "
-                    "kernels; per-particle log E; loss = sum of final positions"}
+            "note": "forward steps keep the node block, kinematic nodal velocity and sort "
+                    "order their adjoint needs; backward = adjoint kernels; per-particle log "
+                    "E; loss = sum of final positions"}
 
 
 def main():

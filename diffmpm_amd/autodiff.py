@@ -60,8 +60,14 @@ class Differentiable:
         d.n = ens.n
         d._pbuf, d._nbuf, d._eid_f = ens.pbuf, ens.nbuf, ens.eid
         d.fplan, d.aplan = ens.plan, BatchPlan(ens)
+        d._ens = ens
         d._finish(ens.scheme, ens.velocity_update, ens.dt)
         return d
+
+    # keep each step's forward quantities for its adjoint (about 20 + 15 *
+    # nodes/particles values per particle of extra memory per step) instead of
+    # recomputing the step in the backward pass
+    save_forward = True
 
     def __init__(self, sim, per_particle_E=True):
         mesh = sim.mesh
@@ -126,12 +132,43 @@ class Differentiable:
         buf[_lib.P_YOUNGS, : self.n] = torch.exp(log_e.detach()).to(buf.dtype).expand(self.n)
 
     # -- one step ---------------------------------------------------------------
-    def forward_step(self, state, log_e, gravity, i):
+    def forward_step(self, state, log_e, gravity, i, save=False):
+        """One forward step; with save=True also returns what its adjoint
+        needs (node block, kinematic nodal velocity, sort by cell)."""
         self.set_state(state, log_e)
         self.fplan.run(self.scheme, self.velocity_update, gravity, self.dt, i, 1,
                        keep_temporaries=False)
         self._last = None  # the block now holds the output
-        return self.get_state()
+        out = self.get_state()
+        if not save:
+            return out, None
+        nsaved = self._nbuf.clone()
+        vkin = torch.empty((2, self._nbuf.shape[1]), dtype=self._nbuf.dtype,
+                           device=self._nbuf.device)
+        perm = torch.empty(max(self.n, 1), dtype=torch.int32, device=self._nbuf.device)
+        cs = torch.empty(self._ncells() + 2, dtype=torch.int32, device=self._nbuf.device)
+        call("dmpm_plan_export_step", self.fplan.h, ptr(vkin), ptr(perm), ptr(cs), self.fplan.s)
+        return out, (nsaved, vkin, perm, cs)
+
+    def _ncells(self):
+        if self.el is not None:
+            return self.el._nx * self.el._ny
+        return sum(s.mesh.elements._nx * s.mesh.elements._ny for s in self._ens.sims)
+
+    def adjoint_step_saved(self, state, log_e, out, saved, gravity, gout):
+        """Like adjoint_step, from the quantities forward_step(save=True)
+        returned (no recomputation of the step)."""
+        nsaved, vkin, perm, cs = saved
+        self._write(self.pin, state.detach(), log_e)
+        self._write(self.pwork, out.detach(), None)
+        ld = self._pbuf.shape[1]
+        cot = torch.empty((C_ROWS, ld), dtype=self._pbuf.dtype, device=self._pbuf.device)
+        cot[:, : self.n] = gout
+        lebar = torch.zeros(ld, dtype=self._pbuf.dtype, device=self._pbuf.device)
+        call("dmpm_step_adjoint_saved", self.fplan.h, ptr(self.pin), ptr(self.pwork),
+             ptr(nsaved), ptr(vkin), ptr(perm), ptr(cs), ptr(cot), ptr(lebar), self.code, self.vu,
+             self.fplan._gravity(gravity), self.dt, self.fplan.s)
+        return cot[:, : self.n], lebar[: self.n]
 
     def adjoint_step(self, state, log_e, gravity, i, gout):
         """Cotangents (state, log E per particle) of step i's input from the
@@ -151,18 +188,25 @@ class Differentiable:
 class _StepFn(torch.autograd.Function):
     @staticmethod
     def forward(ctx, state, log_e, d, gravity, i):
-        out = d.forward_step(state, log_e, gravity, i)
+        save = d.save_forward
+        out, saved = d.forward_step(state, log_e, gravity, i, save=save)
         ctx.d, ctx.gravity, ctx.i = d, gravity, i
         ctx.has_log_e = log_e is not None
-        ctx.save_for_backward(state, log_e if log_e is not None else state.new_empty(0))
+        ctx.saved_fwd = save
+        extra = (out,) + saved if save else ()
+        ctx.save_for_backward(state, log_e if log_e is not None else state.new_empty(0), *extra)
         return out
 
     @staticmethod
     def backward(ctx, gout):
-        state, log_e = ctx.saved_tensors
+        state, log_e, *extra = ctx.saved_tensors
         d = ctx.d
-        gs, ge = d.adjoint_step(state, log_e if ctx.has_log_e else None, ctx.gravity, ctx.i,
-                                gout.contiguous())
+        le = log_e if ctx.has_log_e else None
+        if ctx.saved_fwd:
+            gs, ge = d.adjoint_step_saved(state, le, extra[0], tuple(extra[1:]), ctx.gravity,
+                                          gout.contiguous())
+        else:
+            gs, ge = d.adjoint_step(state, le, ctx.gravity, ctx.i, gout.contiguous())
         g_log_e = None
         if ctx.has_log_e and ctx.needs_input_grad[1]:
             g_log_e = ge if log_e.dim() > 0 else ge.sum()

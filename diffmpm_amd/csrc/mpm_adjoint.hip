@@ -462,6 +462,11 @@ __global__ __launch_bounds__(kThreads) void k_adj_node2(Mesh2<T> M, const T* __r
 }
 
 template <typename T, int CB>
+int adjoint_kernels(dmpm_plan* P, const Mesh2<T>& M, const T* pb, const T* pw, const T* nb,
+                    const T* vkin, SortRef sr, T* cot, T* lebar, int scheme, int vu,
+                    const double* grav, double dt, hipStream_t s);
+
+template <typename T, int CB>
 int adjoint_t(dmpm_plan* P, const T* pb, T* pw, int32_t* eid, T* nb, T* cot, T* lebar,
               int scheme, int vu, const double* grav, double dt, int64_t step, hipStream_t s) {
   const Mesh2<T> M = mesh_of<T>(P);
@@ -478,9 +483,25 @@ int adjoint_t(dmpm_plan* P, const T* pb, T* pw, int32_t* eid, T* nb, T* cot, T* 
                                       hipMemcpyDeviceToDevice, s),
                        "hipMemcpyAsync"));
   DMPM_TRY(tile_run(P, pw, eid, nb, scheme, vu, grav, dt, step, 1, 0, s));
+  const T* vkin = (const T*)plan_workspace(P, 1, (size_t)2 * P->nnodes * sizeof(T));
+  if (!vkin) return fail(DMPM_EHIP, "workspace allocation failed");
+  return adjoint_kernels<T, CB>(P, M, pb, pw, nb, vkin, sort_ref(P), cot, lebar, scheme, vu,
+                                grav, dt, s);
+}
+
+// The adjoint kernels of one step from its forward quantities: pb the step's
+// input block, pw a block whose stress and volume rows hold the values after
+// the (USF) stress update, nb the node block after the step, vkin the
+// kinematic nodal velocity, sr the step's sort by cell.
+template <typename T, int CB>
+int adjoint_kernels(dmpm_plan* P, const Mesh2<T>& M, const T* pb, const T* pw, const T* nb,
+                    const T* vkin, SortRef sr, T* cot, T* lebar, int scheme, int vu,
+                    const double* grav, double dt, hipStream_t s) {
+  const int64_t ld = P->d.ld, nld = P->d.nld;
+  const int n = (int)P->n;
+  const bool usf = scheme == DMPM_USF;
   ProfScopeT ps_(P, KC_ADJ, s);
   T* part = (T*)plan_workspace(P, 0, (size_t)16 * M.ncells * sizeof(T));
-  const T* vkin = (const T*)plan_workspace(P, 1, (size_t)2 * P->nnodes * sizeof(T));
   T* wk = (T*)plan_workspace(P, 3, (size_t)W_ROWS * ld * sizeof(T));
   T* na = (T*)plan_workspace(P, 4, (size_t)A_ROWS * P->nnodes * sizeof(T));
   if (!part || !vkin || !wk || !na) return fail(DMPM_EHIP, "workspace allocation failed");
@@ -489,7 +510,6 @@ int adjoint_t(dmpm_plan* P, const T* pb, T* pw, int32_t* eid, T* nb, T* cot, T* 
   mat.per_E = P->d.per_particle_E;
   for (int k = 0; k < 3; ++k) mat.de[k] = P->d.de[k];
   const T tdt = T(dt), Gx = T(grav[0]), Gy = T(grav[1]);
-  const SortRef sr = sort_ref(P);
   const unsigned np = nblk(n), nn = nblk(P->nnodes);
   const unsigned ncb = (unsigned)((M.ncells + CB - 1) / CB);
   if (usf)
@@ -519,6 +539,19 @@ int adjoint_t(dmpm_plan* P, const T* pb, T* pw, int32_t* eid, T* nb, T* cot, T* 
   return launch_check("step_adjoint");
 }
 
+// Copies the sort order of the current step (buffer selected by flags[4]).
+__global__ void k_export_sort(const int* __restrict__ perm0, const int* __restrict__ perm1,
+                              const int* __restrict__ cs0, const int* __restrict__ cs1,
+                              const int* __restrict__ flags, int n, int ncs,
+                              int* __restrict__ perm_out, int* __restrict__ cs_out) {
+  const int cur = flags[4];
+  const int* __restrict__ perm = cur ? perm1 : perm0;
+  const int* __restrict__ cs = cur ? cs1 : cs0;
+  const int gs = gridDim.x * blockDim.x;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gs) perm_out[i] = perm[i];
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < ncs; i += gs) cs_out[i] = cs[i];
+}
+
 #undef PR
 #undef PW
 #undef NR
@@ -527,6 +560,54 @@ int adjoint_t(dmpm_plan* P, const T* pb, T* pw, int32_t* eid, T* nb, T* cot, T* 
 #undef AR
 
 }  // namespace
+
+extern "C" int dmpm_plan_export_step(dmpm_plan* P, void* vkin_out, int32_t* perm_out,
+                                     int32_t* cs_out, void* stream) {
+  if (!P) return fail(DMPM_EINVAL, "null plan");
+  if (!vkin_out || !perm_out || !cs_out) return fail(DMPM_EINVAL, "null buffer");
+  if (!tile_eligible(P, 0)) return fail(DMPM_EINVAL, "no fused 2D step on this plan");
+  hipStream_t s = (hipStream_t)stream;
+  const size_t esz = P->d.dtype == DMPM_F32 ? sizeof(float) : sizeof(double);
+  void* vkin = plan_workspace(P, 1, 2 * (size_t)P->nnodes * esz);
+  if (!vkin) return fail(DMPM_EHIP, "workspace allocation failed");
+  DMPM_TRY(hip_check(hipMemcpyAsync(vkin_out, vkin, 2 * (size_t)P->nnodes * esz,
+                                    hipMemcpyDeviceToDevice, s),
+                     "hipMemcpyAsync"));
+  k_export_sort<<<1024, kThreads, 0, s>>>(P->perm, P->perm2, P->cs, P->cs2, P->flags, (int)P->n,
+                                          P->ncells + 2, perm_out, cs_out);
+  return launch_check("export_step");
+}
+
+extern "C" int dmpm_step_adjoint_saved(dmpm_plan* P, const void* pin, const void* pout,
+                                       const void* nsaved, const void* vkin_saved,
+                                       const int32_t* perm_saved, const int32_t* cs_saved,
+                                       void* cot, void* lebar, int32_t scheme,
+                                       int32_t velocity_update, const double* gravity,
+                                       double dt, void* stream) {
+  if (!P) return fail(DMPM_EINVAL, "null plan");
+  if (!pin || !pout || !nsaved || !vkin_saved || !perm_saved || !cs_saved || !cot || !gravity)
+    return fail(DMPM_EINVAL, "null buffer");
+  if (scheme != DMPM_USF && scheme != DMPM_USL)
+    return fail(DMPM_EINVAL, "scheme must be DMPM_USF or DMPM_USL");
+  if (!tile_eligible(P, 0))
+    return fail(DMPM_EINVAL,
+                "the adjoint step covers 2D Quadrilateral4Node meshes without particle "
+                "tractions or particle external forces");
+  SortRef sr;
+  sr.perm0 = sr.perm1 = perm_saved;
+  sr.cs0 = sr.cs1 = cs_saved;
+  sr.flags = P->zflags;  // not the identity, buffer 0
+  hipStream_t s = (hipStream_t)stream;
+  if (P->d.dtype == DMPM_F32)
+    return adjoint_kernels<float, 128>(P, mesh_of<float>(P), (const float*)pin,
+                                       (const float*)pout, (const float*)nsaved,
+                                       (const float*)vkin_saved, sr, (float*)cot, (float*)lebar,
+                                       scheme, velocity_update, gravity, dt, s);
+  return adjoint_kernels<double, 128>(P, mesh_of<double>(P), (const double*)pin,
+                                      (const double*)pout, (const double*)nsaved,
+                                      (const double*)vkin_saved, sr, (double*)cot,
+                                      (double*)lebar, scheme, velocity_update, gravity, dt, s);
+}
 
 extern "C" int dmpm_step_adjoint(dmpm_plan* P, const void* pin, void* pwork, int32_t* eid_work,
                                  void* nwork, void* cot, void* lebar, int32_t scheme,

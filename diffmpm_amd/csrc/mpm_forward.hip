@@ -835,7 +835,8 @@ static void free_ptr(void* p) {
 
 void plan_free(dmpm_plan* P) {
   if (!P) return;
-  void* ptrs[] = {P->key, P->perm, P->counts, P->fill, P->cs, P->bsum, P->flags, P->big,
+  void* ptrs[] = {P->key, P->perm, P->counts, P->fill, P->cs, P->bsum, P->flags, P->zflags,
+                  P->big,
                   P->perm2, P->cs2, P->mov, P->mrec, P->dL, P->dA, P->headL, P->headA,
                   P->chunkD, P->chunkC, P->sims,
                   P->con_node, P->con_dir, P->con_val, P->nf_id, P->nf_dir, P->nf_off,
@@ -908,6 +909,7 @@ static int plan_init(dmpm_plan* P, dmpm_plan** out) {
   al(&P->cs, nb + 1);
   al(&P->bsum, nbs);
   al(&P->flags, kFlagWords);
+  al(&P->zflags, kFlagWords);
   al(&P->big, P->ncells);
   al(&P->perm2, n);
   al(&P->cs2, nb + 1);
@@ -928,6 +930,7 @@ static int plan_init(dmpm_plan* P, dmpm_plan** out) {
   auto zero = [&](void* p, int64_t bytes, int v) {
     if (rc == DMPM_OK && p) rc = hip_check(hipMemset(p, v, bytes), "hipMemset");
   };
+  zero(P->zflags, kFlagWords * sizeof(int), 0);
   zero(P->counts, nb * sizeof(int), 0);
   zero(P->fill, nb * sizeof(int), 0);
   zero(P->mrec, n * sizeof(int), 0);

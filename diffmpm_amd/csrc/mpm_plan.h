@@ -44,6 +44,7 @@ struct dmpm_plan {
   // workgroups of the last full-sort stage, [10] the per-phase sort rebuilt
   // buffer 0 since the last fused-path sort
   int* flags = nullptr;
+  int* zflags = nullptr;  // [kFlagWords] zeros (flags of a caller-held sort order)
   int* big = nullptr;     // [ncells] cells queued for the big-cell sort
   int* perm2 = nullptr;   // second buffer of perm (incremental re-sort)
   int* cs2 = nullptr;     // second buffer of cs

@@ -268,6 +268,21 @@ int dmpm_step_adjoint(dmpm_plan* plan, const void* pin, void* pwork, int32_t* ei
                       void* nwork, void* cot, void* lebar, int32_t scheme,
                       int32_t velocity_update, const double* gravity, double dt, int64_t step,
                       void* stream);
+/* After a one-step dmpm_run of the fused path: copies the step's kinematic
+ * nodal velocity (vkin_out [2][nnodes], particle dtype) and its sort by cell
+ * (perm_out [nparticles], cs_out [ncells+2]) so that the adjoint of that step
+ * can run later without recomputing it. */
+int dmpm_plan_export_step(dmpm_plan* plan, void* vkin_out, int32_t* perm_out, int32_t* cs_out,
+                          void* stream);
+/* dmpm_step_adjoint from saved forward quantities instead of a recomputation:
+ * pin the step's input block; pout a particle block whose stress and volume
+ * rows hold the step's output values; nsaved the node block after the step;
+ * vkin_saved, perm_saved, cs_saved from dmpm_plan_export_step. */
+int dmpm_step_adjoint_saved(dmpm_plan* plan, const void* pin, const void* pout,
+                            const void* nsaved, const void* vkin_saved,
+                            const int32_t* perm_saved, const int32_t* cs_saved, void* cot,
+                            void* lebar, int32_t scheme, int32_t velocity_update,
+                            const double* gravity, double dt, void* stream);
 
 /* ---- measurement (no reference counterpart) ------------------------------- */
 /* Kernel classes for dmpm_timing_enable. */

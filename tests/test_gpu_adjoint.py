@@ -186,3 +186,17 @@ def test_scalar_log_e_gradient_is_the_sum_of_the_per_particle_terms():
     lm = _loss(d, g, state0, le0.detach() - FD_H_LOGE, w)
     fd = (lp - lm) / (2 * FD_H_LOGE)
     assert math.isclose(float(le0.grad), fd, rel_tol=RTOL_FD), (float(le0.grad), fd)
+
+
+@pytest.mark.parametrize("scheme", ["usf", "usl"])
+def test_saved_forward_and_recomputation_give_identical_gradients(scheme):
+    """The backward pass from the forward step's saved quantities (default)
+    and from a recomputation of the step give bit-identical gradients."""
+    d, g, state0, log_e = _setup(scheme, False)
+    w = _weights(state0)
+    d.save_forward = True
+    a = _grad(d, g, state0, log_e, w)
+    d.save_forward = False
+    b = _grad(d, g, state0, log_e, w)
+    for x, y in zip(a, b):
+        assert torch.equal(x, y)
