@@ -2,8 +2,9 @@
 
 This package is the oracle that the HIP path of ``diffmpm_amd`` is checked
 against.  Only ``tests/``, ``__graft_entry__.smoke()`` and the
-``cpu_baseline`` leg of ``bench.py`` may import it, and only as the checker
-(or, for ``cpu_baseline``, as the timed CPU column).  The product package
+``cpu_baseline`` leg of ``bench.py`` may import it, and only as the source of
+the values the HIP path is compared with (or, for ``cpu_baseline``, as the
+timed CPU column).  The product package
 ``diffmpm_amd`` never imports it and has no CPU fallback.
 
 ``oracle.mpm`` restates, in NumPy, the arithmetic of the reference's
