@@ -321,3 +321,57 @@ def test_tile_path_particles_below_and_left_of_mesh(scheme):
     pb.run(scheme, False, g, spec["dt"], 0, 4, True, per_phase=True)
     for x, y in zip(_state(a), _state(b)):
         assert torch.equal(x, y)
+
+
+@pytest.mark.parametrize("scheme", ["usf", "usl"])
+def test_tile_path_many_arrivals_into_one_cell(scheme):
+    """A sheet of 200 particles just left of a cell edge, moving right fast
+    enough to cross it in one step: more arrivals into one cell than the
+    incremental re-sort takes (full re-sort), then incremental steps; the
+    fused path equals the per-phase path bit for bit."""
+    from diffmpm_amd._plan import plan_for
+    h, dt = 0.1, 1e-3
+    rng = np.random.Generator(np.random.PCG64(8))
+    sheet = np.stack([np.full(200, 0.4 - 0.0005), 0.31 + 0.08 * rng.random(200)], 1)
+    bg = cases.synthetic_spec("quad4", nx=8, ny=8, rows=6, seed=4)["particles"]
+    pts = np.concatenate([np.asarray(bg) * 0.8 + 0.1, sheet])
+    vel = np.zeros_like(pts)
+    vel[-200:, 0] = 0.02 * h / dt
+    spec = dict(element="quad4", nelements=[10, 10], el_len=[h, h], particles=pts.tolist(),
+                init_velocity=vel.tolist(),
+                material={"type": "LinearElastic", "density": 1000.0, "youngs_modulus": 1e2,
+                          "poisson_ratio": 0.3},
+                constraints=[{"nodes": list(range(11)), "dir": 1, "velocity": 0.0}],
+                nodal_forces=[], tractions=[], gravity=[0.0, -9.81], scheme=scheme, dt=dt,
+                nsteps=4, velocity_update=True)
+    a, b = cases.build_gpu(spec), cases.build_gpu(spec)
+    g = torch.tensor(spec["gravity"], dtype=torch.float64)
+    pa = plan_for(a.mesh.elements, a.mesh.particles[0])
+    pb = plan_for(b.mesh.elements, b.mesh.particles[0])
+    e0 = a.mesh.particles[0].element_ids.clone()
+    pa.run(scheme, True, g, dt, 0, 1, True, per_phase=False)
+    pb.run(scheme, True, g, dt, 0, 1, True, per_phase=True)
+    for k in range(1, 4):
+        pa.run(scheme, True, g, dt, k, 1, True, per_phase=False)
+        pb.run(scheme, True, g, dt, k, 1, True, per_phase=True)
+        for x, y in zip(_state(a), _state(b)):
+            assert torch.equal(x, y), k
+    moved = int((e0 != a.mesh.particles[0].element_ids).sum())
+    assert moved > 64  # the sheet crossed into the next cell
+
+
+@pytest.mark.parametrize("scheme", ["usf", "usl"])
+def test_tile_path_crowded_cell_equals_per_phase_path(scheme):
+    """3000 particles in one cell (rank-sorted by one workgroup in the full
+    re-sort): the fused path equals the per-phase path bit for bit."""
+    from diffmpm_amd._plan import plan_for
+    spec = _crowded_spec(3000)
+    spec["scheme"] = scheme
+    a, b = cases.build_gpu(spec), cases.build_gpu(spec)
+    g = torch.tensor(spec["gravity"], dtype=torch.float64)
+    pa = plan_for(a.mesh.elements, a.mesh.particles[0])
+    pb = plan_for(b.mesh.elements, b.mesh.particles[0])
+    pa.run(scheme, False, g, spec["dt"], 0, 3, True, per_phase=False)
+    pb.run(scheme, False, g, spec["dt"], 0, 3, True, per_phase=True)
+    for x, y in zip(_state(a), _state(b)):
+        assert torch.equal(x, y)
