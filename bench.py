@@ -36,6 +36,9 @@ KERNEL_BYTES = {
     "tile_force": 4 * (2 + 1 + 1 + 3),
     # G2P: read x(2), v(2); write x(2), v(2)
     "tile_g2p": 4 * 4 + 4 * 4,
+    # the one-launch step (default): every persistent field of the step once,
+    # read x(2), v(2), m, V, rho, sigma(4), eps(3), eps_v; write all but m
+    "step": 4 * (2 + 2 + 1 + 1 + 1 + 4 + 3 + 1) + 4 * (2 + 2 + 1 + 1 + 4 + 3 + 1),
 }
 # --usf-fused: the force cell pass also does the stress update in place: read
 # x(2), m, eps(3), sigma(4), V, rho, eps_v; write eps(3), sigma(4), V, rho, eps_v
@@ -185,7 +188,9 @@ def main():
     ap.add_argument("--cells-per-block", type=int, default=0,
                     help="cells per workgroup of the P2G cell pass (64/128/256; 0: default)")
     ap.add_argument("--usf-fused", action="store_true",
-                    help="update stresses inside the force P2G cell pass")
+                    help="phase launches: update stresses inside the force P2G cell pass")
+    ap.add_argument("--phase-launches", action="store_true",
+                    help="one launch per phase instead of the one-launch step")
     ap.add_argument("--ensemble-steps", type=int, default=50,
                     help="steps of the timed 64-member uneven ensemble (0: skip)")
     ap.add_argument("--adjoint-steps", type=int, default=20,
@@ -213,6 +218,8 @@ def main():
     from diffmpm_amd import _lib
     if args.usf_fused:
         plan.set_option(_lib.DMPM_OPT_USF_SPLIT, 0)
+    if args.phase_launches or args.usf_fused:
+        plan.set_option(_lib.DMPM_OPT_STEP_KERNEL, 0)
     print(f"# workload built: {N} particles, {args.n}x{args.n} cells", file=sys.stderr, flush=True)
     plan.run("usf", False, gravity, sim.dt, 0, args.warmup, keep_temporaries=False)
     torch.cuda.synchronize()

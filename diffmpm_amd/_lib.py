@@ -21,6 +21,7 @@ DMPM_LINEAR_ELASTIC, DMPM_SIMPLE_MATERIAL = 0, 1
 DMPM_USF, DMPM_USL = 0, 1
 DMPM_FN_UNIT, DMPM_FN_LINEAR = 0, 1
 DMPM_OPT_USF_SPLIT = 1
+DMPM_OPT_STEP_KERNEL = 2
 
 # Particle block rows (include/diffmpm_amd.h)
 P_X, P_V, P_MASS, P_VOLUME, P_DENSITY, P_SIZE = 0, 2, 4, 5, 6, 7
@@ -31,7 +32,7 @@ N_LOC, N_V, N_A, N_MASS, N_P, N_FINT, N_FEXT, N_FDAMP, N_ROWS = 0, 2, 4, 6, 7, 9
 
 # Kernel classes (dmpm_timing_enable)
 KC = dict(locate=0, sort=1, p2g_kin=2, stress=3, p2g_force=4, grid=5, g2p=6, small=7,
-          tile_kin=8, tile_force=9, tile_g2p=10, adj=11, batch=12)
+          tile_kin=8, tile_force=9, tile_g2p=10, adj=11, batch=12, step=13)
 
 ROW_ALIGN = 64  # row stride multiple (256 B in fp32) so every row starts aligned
 

@@ -1248,6 +1248,9 @@ int dmpm_plan_set_option(dmpm_plan* P, int32_t option, int32_t value) {
     case DMPM_OPT_USF_SPLIT:
       P->usf_split = value ? 1 : 0;
       return DMPM_OK;
+    case DMPM_OPT_STEP_KERNEL:
+      P->step_kernel = value ? 1 : 0;
+      return DMPM_OK;
     default:
       return fail(DMPM_EINVAL, "unknown option");
   }

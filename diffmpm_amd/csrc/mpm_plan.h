@@ -101,6 +101,7 @@ struct dmpm_plan {
   int prof_single = -1;  // the class dmpm_timing_read reports (-2: all enabled)
   int cell_block = 128;  // cells per workgroup of the P2G cell pass (64/128/256)
   int usf_split = 1;     // USF: stress update streamed (1) or in the force cell pass (0)
+  int step_kernel = 1;   // single-simulation fused step: one launch (1) or one per phase (0)
   // workspace of the fused tile kernels and the adjoint (allocated lazily)
   void* extra[16] = {nullptr};
   size_t extra_bytes[16] = {0};
@@ -165,8 +166,8 @@ bool tile_eligible(const dmpm_plan* P, int pfext);
 // Kernel classes for dmpm_timing_enable.
 enum { KC_LOCATE = 0, KC_SORT = 1, KC_P2G_KIN = 2, KC_STRESS = 3, KC_P2G_FORCE = 4,
        KC_GRID = 5, KC_G2P = 6, KC_SMALL = 7, KC_TILE_KIN = 8, KC_TILE_FORCE = 9,
-       KC_TILE_G2P = 10, KC_ADJ = 11, KC_BATCH = 12 };
-constexpr int kNumKC = 13;
+       KC_TILE_G2P = 10, KC_ADJ = 11, KC_BATCH = 12, KC_STEP = 13 };
+constexpr int kNumKC = 14;
 // A ring of kProfRing event pairs per class: before a pair is reused its
 // elapsed time is read (host waits for the launch of that class kProfRing
 // launches back), so at most kProfRing timed launches per class are

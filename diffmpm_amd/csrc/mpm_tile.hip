@@ -589,6 +589,1155 @@ __global__ __launch_bounds__(kThreads) void k_stress_tile(Mesh2<T> M, T* __restr
   }
 }
 
+// ---- single-launch step ----------------------------------------------------------
+//
+// One workgroup per tile of kTW x kTH cells runs a whole step for the
+// particles of its cells, recomputing what it needs from rings of halo cells
+// (R_k: the tile grown by k cells on each side, clipped to the mesh):
+//   K  P2G of mass and momentum over R_H (H = 2 in USF, 1 in USL): kinematic
+//      nodal velocities at the nodes of R_{H-1};
+//   S  (USF) strain, volume and stress update of the particles of R_1 (only
+//      the tile's own results are written);
+//   F  P2G of body and internal force over R_1: nodal acceleration and
+//      velocity at the nodes of the tile;
+//   G  G2P of the tile's particles (USL: then their stress update), with the
+//      cell changes recorded for the next re-sort.
+// Cell sums run over each cell's slots in order and node sums over the four
+// cells in the fixed order of node_gather, with the same expressions as the
+// multi-kernel path, so both give bit-identical results.  Workgroups never
+// wait for each other: the step reads the particle state from `src` and
+// writes it to `dst` (the run alternates between the caller's block and a
+// shadow block), and the previous nodal velocity that a massless node keeps
+// is read from a copy (`vold`) that the step does not write.  Particles
+// outside the mesh (slots from cs[ncells]) are handled by kTailBlocks extra
+// workgroups.
+//
+// Fast path (FP32, at most kPPT slots of R_H per thread): every slot's
+// particle data is loaded once, at the start, into registers; the phases
+// exchange per-particle values through LDS only.  Otherwise (crowded tiles,
+// FP64) the phases stage the slots in rounds and reload what they need.
+constexpr int kTW = 16, kTH = 16;  // cells per tile
+constexpr int kFT = 512;           // threads per workgroup
+constexpr int kPPT = 4;            // fast path: slots of R_H per thread
+constexpr int kKCAP = kFT * kPPT;  // fast path: slots of R_H
+constexpr int kFCAP = 1408;        // fast path: slots of R_1
+constexpr int kRound = 1024;       // staged slots per round of the other path
+constexpr int kTailBlocks = 8;
+constexpr int kSegRows = kTH + 4;  // rows of R_2
+static_assert(kRound % kFT == 0, "round size");
+// particle rows the step updates (alternating blocks) and the strain/stress
+// rows it leaves alone (only set to NaN outside the mesh, in the caller's block)
+constexpr int kDynRows = 14, kFixRows = 5;
+__constant__ const int kDyn[kDynRows] = {
+    DMPM_P_X,          DMPM_P_X + 1,      DMPM_P_V,          DMPM_P_V + 1,
+    DMPM_P_VOLUME,     DMPM_P_DENSITY,    DMPM_P_VOLSTRAIN_C, DMPM_P_STRAIN + 0,
+    DMPM_P_STRAIN + 1, DMPM_P_STRAIN + 3, DMPM_P_STRESS + 0, DMPM_P_STRESS + 1,
+    DMPM_P_STRESS + 2, DMPM_P_STRESS + 3};
+__constant__ const int kFix[kFixRows] = {DMPM_P_STRAIN + 2, DMPM_P_STRAIN + 4, DMPM_P_STRAIN + 5,
+                                         DMPM_P_STRESS + 4, DMPM_P_STRESS + 5};
+
+// Element i of row r of a particle or node block, through buffer
+// instructions: one resource per block (base, size in bytes), the row offset
+// a scalar, the element offset a 32-bit vector register (the host checks that
+// every block is below 4 GiB).  Accesses past a block's size read 0 and write
+// nothing.  Dynamic particle rows are read from src and written to dst.
+template <typename T>
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t brsrc(const T* base, uint32_t nbytes) {
+  return __builtin_amdgcn_make_buffer_rsrc((void*)base, (short)0, (int)nbytes, 0x00020000);
+}
+template <typename T>
+__device__ __forceinline__ T bld(const T* base, uint32_t nbytes, uint32_t row_elems,
+                                 uint32_t i) {
+  const __amdgpu_buffer_rsrc_t rs = brsrc(base, nbytes);
+  if constexpr (sizeof(T) == 4)
+    return __builtin_bit_cast(
+        T, __builtin_amdgcn_raw_buffer_load_b32(rs, i * 4u, row_elems * 4u, 0));
+  else
+    return __builtin_bit_cast(
+        T, __builtin_amdgcn_raw_buffer_load_b64(rs, i * 8u, row_elems * 8u, 0));
+}
+template <typename T>
+__device__ __forceinline__ void bst(T* base, uint32_t nbytes, uint32_t row_elems, uint32_t i,
+                                    T v) {
+  const __amdgpu_buffer_rsrc_t rs = brsrc(base, nbytes);
+  if constexpr (sizeof(T) == 4)
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), rs, i * 4u,
+                                          row_elems * 4u, 0);
+  else
+    __builtin_amdgcn_raw_buffer_store_b64(
+        __builtin_bit_cast(__attribute__((ext_vector_type(2))) uint32_t, v), rs, i * 8u,
+        row_elems * 8u, 0);
+}
+#define SR(r, i) bld(a.src, a.pbytes, (uint32_t)(r) * a.ld32, (uint32_t)(i))
+#define PB(r, i) bld<T>(a.pb, a.pbytes, (uint32_t)(r) * a.ld32, (uint32_t)(i))
+#define NB(r, i) bld<T>(a.nb, a.nbytes, (uint32_t)(r) * a.nld32, (uint32_t)(i))
+#define DR_W(r, i, v) bst(a.dst, a.pbytes, (uint32_t)(r) * a.ld32, (uint32_t)(i), (T)(v))
+#define PB_W(r, i, v) bst(a.pb, a.pbytes, (uint32_t)(r) * a.ld32, (uint32_t)(i), (T)(v))
+#define NB_W(r, i, v) bst(a.nb, a.nbytes, (uint32_t)(r) * a.nld32, (uint32_t)(i), (T)(v))
+
+template <typename T> struct StepArgs {
+  Grid2<T> g;
+  const T* src;
+  T* dst;
+  T* pb;  // static rows (mass, E_p), rows the step leaves alone, temporaries
+  uint32_t ld32;
+  T* nb;
+  uint32_t nld32;
+  uint32_t pbytes;  // bytes of a particle block (the shadow block is as large)
+  uint32_t nbytes;  // bytes of the node block
+  const T* vold;  // [2][nnodes] nodal velocity at the start of the step
+  T* vnew;        // [2][nnodes] nodal velocity after the step
+  T* vkin;        // [2][nnodes] kinematic nodal velocity (for the adjoint's export)
+  SortRef sr;
+  MoveLog L;
+  const int* cidx;
+  const int* cmask;
+  const double* cval;
+  const int* nfd;
+  const int* nf_off;
+  const int* nf_fn;
+  const double* nf_val;
+  FnTable fns;
+  Mat2 mat;
+  T gx, gy, dt, t;
+  int vu, temps, tiles_x, ntiles, n;
+  int* ovf;  // [2] overflow counts by step parity, then [2][ntiles] tile lists
+  int par;   // step parity
+  int nrb;   // workgroups of k_step_rounds that process tiles
+};
+
+// n / d for 0 <= n < 1024 and 1 <= d <= 64 without a division:
+// floor(n * ceil(2^16 / d) / 2^16) (exact there: the rounding error of the
+// multiplier, n * (ceil(2^16/d) d - 2^16) / (d 2^16), stays below 1/d).
+struct SmallDiv {
+  int d;
+  uint32_t m;
+};
+__device__ __forceinline__ SmallDiv small_div(int d) {
+  return SmallDiv{d, (65536u + (uint32_t)d - 1u) / (uint32_t)d};
+}
+__device__ __forceinline__ int sdiv(const SmallDiv& s, int n) {
+  return (int)(__umul24((uint32_t)n, s.m) >> 16);
+}
+
+struct Rgn {
+  int i0, i1, j0, j1;  // cells [i0, i1) x [j0, j1)
+};
+__device__ __forceinline__ Rgn grow(const Rgn& r, int k, int nx, int ny) {
+  return Rgn{max(r.i0 - k, 0), min(r.i1 + k, nx), max(r.j0 - k, 0), min(r.j1 + k, ny)};
+}
+__device__ __forceinline__ bool in_rgn(const Rgn& r, int I, int J) {
+  return I >= r.i0 && I < r.i1 && J >= r.j0 && J < r.j1;
+}
+
+// Cell (I, J) of element_id<T, 2>(x, y) without the integer division: the
+// element search's quirks map to element 0 (below/left of the mesh, NaN) and
+// to the first cell of the next row (x at or beyond the last node).
+template <typename T>
+__device__ __forceinline__ void cell_of(const Grid2<T>& g, T x, T y, int& I, int& J) {
+  I = last_le(x, g.nx, g.hx, g.ihx);
+  J = last_le(y, g.ny, g.hy, g.ihy);
+  if (I < 0 || J < 0) {
+    I = 0;
+    J = 0;
+  } else if (I == g.nx) {
+    I = 0;
+    J = J + 1;
+  }
+}
+
+// Row of virtual index v of a region whose rows hold P[r+1] - P[r] slots.
+__device__ __forceinline__ int seg_row(const int* P, int h, int v) {
+  int lo = 0, hi = h - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (P[mid] <= v) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+// Tile of workgroup b: the tiles (row-major) are split into 8 contiguous
+// ranges, one per XCD (workgroups are dispatched to the XCDs round-robin), so
+// neighbouring tiles share an L2.
+__device__ __forceinline__ int tile_of(int b, int ntiles) {
+  const int q = ntiles >> 3, r = ntiles & 7;
+  const int x = b & 7, k = b >> 3;
+  return x * q + min(x, r) + k;
+}
+
+// KIN cell sum over staged slots [lo, hi) (k_cell_kin's expressions).
+template <typename T>
+__device__ __forceinline__ void kin_cell(const T* st, int stride, int lo, int hi,
+                                         T acc[4 * NQ_KIN]) {
+  for (int q = lo; q < hi; ++q) {
+    T N[4];
+    q4_N4(st[q], st[stride + q], N);
+    const T mq = st[2 * stride + q], mvx = st[3 * stride + q], mvy = st[4 * stride + q];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      acc[NQ_KIN * k + 0] = acc[NQ_KIN * k + 0] + mq * N[k];
+      acc[NQ_KIN * k + 1] = acc[NQ_KIN * k + 1] + N[k] * mvx;
+      acc[NQ_KIN * k + 2] = acc[NQ_KIN * k + 2] + N[k] * mvy;
+    }
+  }
+}
+
+// FORCE cell sum over staged slots [lo, hi) of cell (X0..X1, Y0..Y1)
+// (k_cell_force's expressions).
+template <typename T>
+__device__ __forceinline__ void force_cell(const T* st, int stride, int lo, int hi, T X0, T X1,
+                                           T Y0, T Y1, T gx, T gy, T acc[4 * NQ_FORCE]) {
+  // q4_invJ depends on the cell only
+  T ix, iy;
+  q4_invJ(T(0), T(0), X0, X1, Y0, Y1, ix, iy);
+  for (int q = lo; q < hi; ++q) {
+    const T xi = st[q], eta = st[stride + q];
+    const T bx = st[2 * stride + q] * gx, by = st[2 * stride + q] * gy;
+    const T nV = st[3 * stride + q], sxx = st[4 * stride + q], syy = st[5 * stride + q],
+            sxy = st[6 * stride + q];
+    T N[4];
+    q4_N4(xi, eta, N);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      T dx, dy;
+      q4_dNdx(k, xi, eta, ix, iy, dx, dy);
+      acc[NQ_FORCE * k + 0] = acc[NQ_FORCE * k + 0] + N[k] * bx;
+      acc[NQ_FORCE * k + 1] = acc[NQ_FORCE * k + 1] + N[k] * by;
+      acc[NQ_FORCE * k + 2] = acc[NQ_FORCE * k + 2] + nV * (dx * sxx + dy * sxy);
+      acc[NQ_FORCE * k + 3] = acc[NQ_FORCE * k + 3] + nV * (dy * syy + dx * sxy);
+    }
+  }
+}
+
+// Node sums (node_gather's order) for the nodes of region RN from the cell
+// partials of region RC (which holds every cell of those nodes), through LDS
+// buffer sl (4 * NQ * cells of RC).  Called by every thread; ends with a
+// barrier.
+template <typename T, int NQ, int C, int NPT>
+__device__ __forceinline__ void node_sums(T* sl, const T (&acc)[C][4 * NQ], const Rgn& RC,
+                                          const Rgn& RN, int nx, int ny, T (&nacc)[NPT][NQ]) {
+  const int t = threadIdx.x;
+  const int wc = RC.i1 - RC.i0, nc = wc * (RC.j1 - RC.j0);
+  const int wn = RN.i1 - RN.i0 + 1, nn = wn * (RN.j1 - RN.j0 + 1);
+  const int dI[4] = {-1, 0, -1, 0}, dJ[4] = {-1, -1, 0, 0}, corner[4] = {2, 3, 1, 0};
+  const SmallDiv dwn = small_div(wn);
+#pragma unroll
+  for (int u = 0; u < C; ++u) {
+    const int lc = t + u * kFT;
+    if (lc < nc)
+#pragma unroll
+      for (int v = 0; v < 4 * NQ; ++v) sl[v * nc + lc] = acc[u][v];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int w = 0; w < NPT; ++w) {
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) nacc[w][q] = T(0);
+    const int ln = t + w * kFT;
+    if (ln >= nn) continue;
+    const int jr = sdiv(dwn, ln);
+    const int i = RN.i0 + (ln - jr * wn), j = RN.j0 + jr;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      const int ci = i + dI[c], cj = j + dJ[c];
+      if (ci < 0 || ci >= nx || cj < 0 || cj >= ny) continue;
+      const int lc = (ci - RC.i0) + (cj - RC.j0) * wc;
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) nacc[w][q] = nacc[w][q] + sl[(NQ * corner[c] + q) * nc + lc];
+    }
+  }
+  __syncthreads();
+}
+
+// Node-table values of this thread's nodes of region RN (w-th node: index
+// t + w * kFT), loaded ahead of the epilogues: constraint row and mask,
+// damping force.
+template <int NPT> struct NodePre {
+  int cr[NPT], mask[NPT];
+};
+template <typename T, int NPT>
+__device__ __forceinline__ void node_pre(const StepArgs<T>& a, const Rgn& RN, NodePre<NPT>& q,
+                                         T (*fd)[2]) {
+  const int t = threadIdx.x, nx = a.g.nx;
+  const int wn = RN.i1 - RN.i0 + 1, nn = wn * (RN.j1 - RN.j0 + 1);
+  const SmallDiv dwn = small_div(wn);
+#pragma unroll
+  for (int w = 0; w < NPT; ++w) {
+    const int ln = min(t + w * kFT, nn - 1);
+    const int jr = sdiv(dwn, ln);
+    const int nd = (RN.j0 + jr) * (nx + 1) + RN.i0 + (ln - jr * wn);
+    q.cr[w] = a.cidx ? a.cidx[nd] : -1;
+    q.mask[w] = 0;
+    if (fd) {
+      fd[w][0] = NB(DMPM_N_FDAMP, nd);
+      fd[w][1] = NB(DMPM_N_FDAMP + 1, nd);
+    }
+  }
+}
+template <typename T, int NPT>
+__device__ __forceinline__ void node_pre_mask(const StepArgs<T>& a, NodePre<NPT>& q) {
+#pragma unroll
+  for (int w = 0; w < NPT; ++w) q.mask[w] = q.cr[w] >= 0 ? a.cmask[q.cr[w]] : 0;
+}
+
+// Kinematic nodal epilogue (k_node_kin) of the nodes of RN: nodal mass and
+// velocity into LDS, the velocity of the tile's own nodes to a.vkin.
+template <typename T, int NPT>
+__device__ __forceinline__ void kin_epilogue(const StepArgs<T>& a, const T (&nacc)[NPT][NQ_KIN],
+                                             const NodePre<NPT>& pre, const Rgn& RN,
+                                             const Rgn& R0, T* vk, T* nm, int nkn) {
+  const int t = threadIdx.x, nx = a.g.nx, ny = a.g.ny, nnodes = a.g.nnodes;
+  const int wn = RN.i1 - RN.i0 + 1, nn = wn * (RN.j1 - RN.j0 + 1);
+  const SmallDiv dwn = small_div(wn);
+#pragma unroll
+  for (int w = 0; w < NPT; ++w) {
+    const int ln = t + w * kFT;
+    if (ln >= nn) continue;
+    const int jr = sdiv(dwn, ln);
+    const int i = RN.i0 + (ln - jr * wn), j = RN.j0 + jr;
+    const int nd = j * (nx + 1) + i;
+    const int cr = pre.cr[w], mask = pre.mask[w];
+    const T m = nacc[w][0];
+    const T p[2] = {snap(nacc[w][1]), snap(nacc[w][2])};
+    const bool own = i >= R0.i0 && j >= R0.j0 && (i < R0.i1 || i == nx) && (j < R0.j1 || j == ny);
+    nm[ln] = m;
+#pragma unroll
+    for (int d = 0; d < 2; ++d) {
+      T v = (m == T(0)) ? a.vold[(int64_t)d * nnodes + nd] : p[d] / m;
+      v = snap(v);
+      if (mask & (1 << d)) v = T(a.cval[2 * cr + d]);
+      vk[d * nkn + ln] = v;
+      if (own) a.vkin[(int64_t)d * nnodes + nd] = v;
+    }
+  }
+}
+
+// Force nodal epilogue (k_node_force) of the tile's nodes: acceleration and
+// velocity into LDS; the tile's own nodes are written to the node block.
+template <typename T, int NPT>
+__device__ __forceinline__ void force_epilogue(const StepArgs<T>& a,
+                                               const T (&nacc)[NPT][NQ_FORCE],
+                                               const NodePre<NPT>& pre, const T (*fd)[2],
+                                               const Rgn& R0, const Rgn& RN, const T* vk,
+                                               const T* nm, int nkn, T* nv, T* na, int ntn) {
+  const int t = threadIdx.x, nx = a.g.nx, ny = a.g.ny, nnodes = a.g.nnodes;
+  const int wt = R0.i1 - R0.i0 + 1, nt = wt * (R0.j1 - R0.j0 + 1);
+  const int wkn = RN.i1 - RN.i0 + 1;
+  const SmallDiv dwt = small_div(wt);
+#pragma unroll
+  for (int w = 0; w < NPT; ++w) {
+    const int ln = t + w * kFT;
+    if (ln >= nt) continue;
+    const int jr = sdiv(dwt, ln);
+    const int i = R0.i0 + (ln - jr * wt), j = R0.j0 + jr;
+    const int nd = j * (nx + 1) + i;
+    const int lk = (i - RN.i0) + (j - RN.j0) * wkn;
+    const bool own = (i < R0.i1 || i == nx) && (j < R0.j1 || j == ny);
+    const int cr = pre.cr[w], mask = pre.mask[w];
+    const T m = nm[lk];
+    if (own) NB_W(DMPM_N_MASS, nd, m);
+#pragma unroll
+    for (int d = 0; d < 2; ++d) {
+      T fe = nacc[w][d];
+      const int qg = a.nfd ? a.nfd[(int64_t)d * nnodes + nd] : -1;
+      if (qg >= 0)
+        for (int k = a.nf_off[qg]; k < a.nf_off[qg + 1]; ++k)
+          fe = fe + fn_value<T>(a.fns, a.nf_fn[k], a.t) * T(a.nf_val[k]);
+      const T fi = nacc[w][2 + d];
+      const T F = (fi + fe) + fd[w][d];
+      T ac = nan_to_num(F / m);
+      T v = vk[d * nkn + lk] + ac * a.dt;
+      T pm = m * v;
+      v = snap(v);
+      ac = snap(ac);
+      if (mask & (1 << d)) {
+        const T c = T(a.cval[2 * cr + d]);
+        v = snap(c);
+        pm = m * c;
+        ac = T(0);
+      }
+      nv[d * ntn + ln] = v;
+      na[d * ntn + ln] = ac;
+      if (own) {
+        NB_W(DMPM_N_FEXT + d, nd, fe);
+        NB_W(DMPM_N_FINT + d, nd, fi);
+        NB_W(DMPM_N_V + d, nd, v);
+        NB_W(DMPM_N_A + d, nd, ac);
+        NB_W(DMPM_N_P + d, nd, pm);
+        a.vnew[(int64_t)d * nnodes + nd] = v;
+      }
+    }
+  }
+}
+
+// Virtual slot range [v0, v1) of each of this thread's cells of region R
+// (table q of the slot segments).
+template <int C>
+__device__ __forceinline__ void cell_ranges(const Rgn& R, int nx, const int* cs, const int* A,
+                                            const int* P, int (&v0)[C], int (&v1)[C]) {
+  const int w = R.i1 - R.i0, nc = w * (R.j1 - R.j0);
+  const SmallDiv dw = small_div(w);
+#pragma unroll
+  for (int u = 0; u < C; ++u) {
+    const int lc = threadIdx.x + u * kFT;
+    v0[u] = v1[u] = 0;
+    if (lc < nc) {
+      const int r = sdiv(dw, lc);
+      const int k = (R.j0 + r) * nx + R.i0 + (lc - r * w);
+      const int s0 = cs[k];
+      v0[u] = P[r] + (s0 - A[r]);
+      v1[u] = v0[u] + (cs[k + 1] - s0);
+    }
+  }
+}
+
+// The strain/volume/stress update of one particle (particle.py:249-327, as in
+// k_stress_tile) from the four nodal velocities of its cell.
+template <typename T> struct StressUpd {
+  T rr[6], d[6], ds[4], dvol, f;
+};
+template <typename T>
+__device__ __forceinline__ void stress_update(const Mat2& mat, T E_p, T ix, T iy, T xi, T eta,
+                                              const T vx[4], const T vy[4], T dt,
+                                              StressUpd<T>& u) {
+  T rc[6];
+  rate2s(ix, iy, xi, eta, vx, vy, u.rr);
+  rate2s(ix, iy, T(0), T(0), vx, vy, rc);
+  for (int k = 0; k < 6; ++k) u.d[k] = u.rr[k] * dt;
+  u.dvol = dt * (rc[0] + rc[1]);
+  stress_inc(mat, E_p, u.d, u.ds);
+  u.f = T(1) + u.dvol;
+}
+
+// The step for the tiles whose slots fit the fast path (FP32); a tile that
+// does not fit appends itself to the overflow list of this step for
+// k_step_rounds.
+template <typename T, bool USF>
+__global__ __launch_bounds__(kFT) __attribute__((amdgpu_waves_per_eu(4))) void k_step_fast(
+    StepArgs<T> a) {
+  constexpr int H = USF ? 2 : 1;
+  constexpr int CK = ((kTW + 2 * H) * (kTH + 2 * H) + kFT - 1) / kFT;  // cells/thread, K
+  constexpr int CF = ((kTW + 2) * (kTH + 2) + kFT - 1) / kFT;          // cells/thread, F
+  constexpr int NKN = (kTW + 2 * H - 1) * (kTH + 2 * H - 1);            // nodes of R_{H-1}
+  constexpr int NTN = (kTW + 1) * (kTH + 1);                           // nodes of the tile
+  constexpr int NK = (NKN + kFT - 1) / kFT, NF = (NTN + kFT - 1) / kFT;
+  constexpr int NCOL = kTW + 2 * H, NROW = kTH + 2 * H;
+  static_assert(CK * kFT <= 1024 && CF * kFT <= 1024 && NK * kFT <= 1024 &&
+                    NF * kFT <= 1024 && NCOL + 1 <= 64,
+                "small_div range");
+  __shared__ T st[5 * kKCAP];  // staged particle inputs; node-pass slices alias it
+  static_assert(7 * kFCAP <= 5 * kKCAP, "F staging");
+  __shared__ T s1f[5 * kFCAP];  // sxx, syy, sxy, V, E_p of the R_1 slots
+  __shared__ T vk[2 * NKN];
+  __shared__ T nm[NKN];
+  __shared__ T nv[2 * NTN];
+  __shared__ T na[2 * NTN];
+  __shared__ T colC[NCOL], colS[NCOL], rowC[NROW], rowS[NROW];
+  __shared__ int segA[3][kSegRows], segP[3][kSegRows + 1];
+  const Grid2<T> g = a.g;
+  const int nx = g.nx, ny = g.ny;
+  const int t = threadIdx.x;
+  const int* __restrict__ cs = a.sr.cs();
+  const int* __restrict__ perm = a.sr.perm();
+  const int ident = a.sr.flags[1];
+
+  const int tile = tile_of(blockIdx.x, a.ntiles);
+  const int ti = tile % a.tiles_x, tj = tile / a.tiles_x;
+  const Rgn R0{ti * kTW, min(ti * kTW + kTW, nx), tj * kTH, min(tj * kTH + kTH, ny)};
+  const Rgn R1 = grow(R0, 1, nx, ny);
+  const Rgn RH = grow(R0, H, nx, ny);
+  const Rgn RN = USF ? R1 : R0;  // cells whose nodes get kinematic velocities
+  const int h0 = R0.j1 - R0.j0, h1 = R1.j1 - R1.j0, hH = RH.j1 - RH.j0;
+
+  // slot segments of the rows of R0 (table 0), R1 (1) and RH (2): one wave
+  // per table, lane r = row r, prefix by shuffles
+  {
+    const int q = t >> 6, r = t & 63;
+    if (q < 3) {
+      const Rgn R = q == 0 ? R0 : (q == 1 ? R1 : RH);
+      int len = 0;
+      if (r < R.j1 - R.j0) {
+        const int k = (R.j0 + r) * nx + R.i0;
+        const int s0 = cs[k];
+        segA[q][r] = s0;
+        len = cs[k + (R.i1 - R.i0)] - s0;
+      }
+#pragma unroll
+      for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_up(len, o, 64);
+        if (r >= o) len += y;
+      }
+      if (r < kSegRows) segP[q][r + 1] = len;
+      if (r == 0) segP[q][0] = 0;
+    } else if (q == 3) {
+      // cell centres and scales 2/(X1 - X0) of the columns and rows of RH
+      // (natural coordinates and inverse Jacobians, particle.py:180-206)
+      if (r < RH.i1 - RH.i0) {
+        const T X0 = g.X(RH.i0 + r), X1 = g.X(RH.i0 + r + 1);
+        colC[r] = (X0 + X1) / T(2);
+        colS[r] = T(2) / (X1 - X0);
+      }
+      if (r < hH) {
+        const T Y0 = g.Y(RH.j0 + r), Y1 = g.Y(RH.j0 + r + 1);
+        rowC[r] = (Y0 + Y1) / T(2);
+        rowS[r] = T(2) / (Y1 - Y0);
+      }
+    }
+  }
+  __syncthreads();
+  const int SH = segP[2][hH], S1 = segP[1][h1];
+  const int wK = RH.i1 - RH.i0;
+  const int wKN = RN.i1 - RN.i0 + 1, wTN = R0.i1 - R0.i0 + 1;
+
+  if (SH > kKCAP || S1 > kFCAP) {
+    if (t == 0) {
+      const int i = atomicAdd(&a.ovf[a.par], 1);
+      a.ovf[2 + a.par * a.ntiles + i] = tile;
+    }
+    return;
+  }
+  constexpr int PPT = kPPT;
+  // node tables of the epilogues, loaded ahead
+  NodePre<NK> kpre;
+  NodePre<NF> fpre;
+  T fd[NF][2];
+  node_pre<T, NK>(a, RN, kpre, nullptr);
+  node_pre<T, NF>(a, R0, fpre, fd);
+  // slot of each virtual index t + j*kFT of the R_H list (lanes past the
+  // end take the last slot and are flagged off): binary searches over the
+  // rows, in lockstep
+  int pp[PPT], v1i[PPT], Ii[PPT], Ji[PPT], lo[PPT], hi[PPT];
+  unsigned own = 0, live = 0;
+#pragma unroll
+  for (int j = 0; j < PPT; ++j) {
+    lo[j] = 0;
+    hi[j] = hH - 1;
+  }
+#pragma unroll
+  for (int it = 0; it < 5; ++it)  // 2^5 >= kSegRows rows
+#pragma unroll
+    for (int j = 0; j < PPT; ++j) {
+      const int v = min(t + j * kFT, SH - 1);
+      const int mid = (lo[j] + hi[j] + 1) >> 1;
+      if (segP[2][mid] <= v) lo[j] = mid;
+      else hi[j] = mid - 1;
+    }
+  static_assert(kSegRows <= 32, "search depth");
+#pragma unroll
+  for (int j = 0; j < PPT; ++j) {
+    const int v = min(t + j * kFT, SH - 1);
+    const int r = lo[j];
+    // (clamped so that an empty tile's lanes still read a valid particle)
+    const int s = min(max(segA[2][r] + (v - segP[2][r]), 0), a.n - 1);
+    if (t + j * kFT < SH) live |= 1u << j;
+    pp[j] = s;
+    const int J = RH.j0 + r;
+    v1i[j] = -1;
+    const int r1 = J - R1.j0;
+    if (r1 >= 0 && r1 < h1) {
+      const int a1 = segA[1][r1];
+      if (s >= a1 && s < a1 + (segP[1][r1 + 1] - segP[1][r1])) v1i[j] = segP[1][r1] + (s - a1);
+    }
+    const int r0 = J - R0.j0;
+    if (r0 >= 0 && r0 < h0) {
+      const int a0 = segA[0][r0];
+      if (s >= a0 && s < a0 + (segP[0][r0 + 1] - segP[0][r0])) own |= 1u << j;
+    }
+    if (!(live & (1u << j))) v1i[j] = -1;
+  }
+  own &= live;
+  if (!ident)
+#pragma unroll
+    for (int j = 0; j < PPT; ++j) pp[j] = perm[pp[j]];
+  // all particle loads of the step, issued together (lanes without a
+  // field read particle 0's)
+  T x[PPT], y[PPT], vx[PPT], vy[PPT], m[PPT];
+  T szz[PPT], ex[PPT], ey[PPT], gxy[PPT], rho[PPT], evc[PPT];
+  {
+    T sxx[PPT], syy[PPT], sxy[PPT], V[PPT], E[PPT];
+#pragma unroll
+    for (int j = 0; j < PPT; ++j) {
+      const int p = pp[j];
+      const int p1 = v1i[j] >= 0 ? p : 0;
+      const int po = (own & (1u << j)) ? p : 0;
+      x[j] = SR(DMPM_P_X, p);
+      y[j] = SR(DMPM_P_X + 1, p);
+      vx[j] = SR(DMPM_P_V, p);
+      vy[j] = SR(DMPM_P_V + 1, p);
+      m[j] = PB(DMPM_P_MASS, p);
+      sxx[j] = SR(DMPM_P_STRESS + 0, p1);
+      syy[j] = SR(DMPM_P_STRESS + 1, p1);
+      sxy[j] = SR(DMPM_P_STRESS + 3, p1);
+      V[j] = SR(DMPM_P_VOLUME, p1);
+      szz[j] = SR(DMPM_P_STRESS + 2, po);
+      ex[j] = SR(DMPM_P_STRAIN + 0, po);
+      ey[j] = SR(DMPM_P_STRAIN + 1, po);
+      gxy[j] = SR(DMPM_P_STRAIN + 3, po);
+      rho[j] = SR(DMPM_P_DENSITY, po);
+      evc[j] = SR(DMPM_P_VOLSTRAIN_C, po);
+    }
+    if (a.mat.per_E)
+#pragma unroll
+      for (int j = 0; j < PPT; ++j) E[j] = PB(DMPM_P_YOUNGS, v1i[j] >= 0 ? pp[j] : 0);
+    else
+#pragma unroll
+      for (int j = 0; j < PPT; ++j) E[j] = T(0);
+    // the R_1 fields wait in LDS until the S phase
+#pragma unroll
+    for (int j = 0; j < PPT; ++j) {
+      if (v1i[j] < 0) continue;
+      const int q = v1i[j];
+      s1f[q] = sxx[j];
+      s1f[kFCAP + q] = syy[j];
+      s1f[2 * kFCAP + q] = sxy[j];
+      s1f[3 * kFCAP + q] = V[j];
+      s1f[4 * kFCAP + q] = E[j];
+    }
+  }
+  int kv0[CK], kv1[CK];
+  cell_ranges<CK>(RH, nx, cs, segA[2], segP[2], kv0, kv1);
+  node_pre_mask<T, NK>(a, kpre);
+  node_pre_mask<T, NF>(a, fpre);
+  // ---- K ----
+  T xi[PPT], eta[PPT];
+#pragma unroll
+  for (int j = 0; j < PPT; ++j) {
+    const int v = t + j * kFT;
+    int I, J;
+    cell_of(g, x[j], y[j], I, J);
+    Ii[j] = I;
+    Ji[j] = J;
+    const int ci = min(max(I - RH.i0, 0), NCOL - 1), rj = min(max(J - RH.j0, 0), NROW - 1);
+    xi[j] = (x[j] - colC[ci]) * colS[ci];
+    eta[j] = (y[j] - rowC[rj]) * rowS[rj];
+    st[v] = xi[j];
+    st[kKCAP + v] = eta[j];
+    st[2 * kKCAP + v] = m[j];
+    st[3 * kKCAP + v] = m[j] * vx[j];
+    st[4 * kKCAP + v] = m[j] * vy[j];
+    if (a.temps && (own & (1u << j))) {
+      PB_W(DMPM_P_XI, pp[j], xi[j]);
+      PB_W(DMPM_P_XI + 1, pp[j], eta[j]);
+    }
+  }
+  __syncthreads();
+  {
+    T acc[CK][4 * NQ_KIN];
+#pragma unroll
+    for (int u = 0; u < CK; ++u) {
+#pragma unroll
+      for (int v = 0; v < 4 * NQ_KIN; ++v) acc[u][v] = T(0);
+      kin_cell(st, kKCAP, kv0[u], kv1[u], acc[u]);
+    }
+    __syncthreads();
+    T nacc[NK][NQ_KIN];
+    node_sums<T, NQ_KIN, CK, NK>(st, acc, RH, RN, nx, ny, nacc);
+    kin_epilogue<T, NK>(a, nacc, kpre, RN, R0, vk, nm, NKN);
+  }
+  int fv0[CF], fv1[CF];
+  cell_ranges<CF>(R1, nx, cs, segA[1], segP[1], fv0, fv1);
+  __syncthreads();
+  // ---- S: stresses (USF); F inputs staged by R_1 slot ----
+#pragma unroll
+  for (int j = 0; j < PPT; ++j) {
+    if (v1i[j] < 0) continue;
+    const int q = v1i[j];
+    const T sxx = s1f[q], syy = s1f[kFCAP + q], sxy = s1f[2 * kFCAP + q],
+            V = s1f[3 * kFCAP + q], E = s1f[4 * kFCAP + q];
+    T Vn = V, sx = sxx, sy = syy, sq = sxy;
+    if (USF) {
+      const int I = Ii[j], J = Ji[j];
+      const int Ic = min(max(I, RN.i0), RN.i1 - 1), Jc = min(max(J, RN.j0), RN.j1 - 1);
+      const int ll = (Ic - RN.i0) + (Jc - RN.j0) * wKN;
+      const int ids[4] = {ll, ll + 1, ll + wKN + 1, ll + wKN};
+      T nvx[4], nvy[4];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        nvx[k] = vk[ids[k]];
+        nvy[k] = vk[NKN + ids[k]];
+      }
+      const int ci = min(max(I - RH.i0, 0), NCOL - 1), rj = min(max(J - RH.j0, 0), NROW - 1);
+      StressUpd<T> u;
+      stress_update(a.mat, E, colS[ci], rowS[rj], xi[j], eta[j], nvx, nvy, a.dt, u);
+      sx = sxx + u.ds[0];
+      sy = syy + u.ds[1];
+      sq = sxy + u.ds[3];
+      Vn = V * u.f;
+      if (own & (1u << j)) {
+        const int p = pp[j];
+        DR_W(DMPM_P_STRAIN + 0, p, ex[j] + u.d[0]);
+        DR_W(DMPM_P_STRAIN + 1, p, ey[j] + u.d[1]);
+        DR_W(DMPM_P_STRAIN + 3, p, gxy[j] + u.d[3]);
+        DR_W(DMPM_P_STRESS + 0, p, sx);
+        DR_W(DMPM_P_STRESS + 1, p, sy);
+        DR_W(DMPM_P_STRESS + 2, p, szz[j] + u.ds[2]);
+        DR_W(DMPM_P_STRESS + 3, p, sq);
+        DR_W(DMPM_P_VOLUME, p, Vn);
+        DR_W(DMPM_P_DENSITY, p, rho[j] / u.f);
+        DR_W(DMPM_P_VOLSTRAIN_C, p, evc[j] + u.dvol);
+        if (a.temps) {
+          for (int k = 0; k < 6; ++k) {
+            PB_W(DMPM_P_STRAIN_RATE + k, p, u.rr[k]);
+            PB_W(DMPM_P_DSTRAIN + k, p, u.d[k]);
+          }
+          PB_W(DMPM_P_DVOLSTRAIN, p, u.dvol);
+        }
+      }
+    }
+    st[q] = xi[j];
+    st[kFCAP + q] = eta[j];
+    st[2 * kFCAP + q] = m[j];
+    st[3 * kFCAP + q] = -Vn;
+    st[4 * kFCAP + q] = sx;
+    st[5 * kFCAP + q] = sy;
+    st[6 * kFCAP + q] = sq;
+  }
+  __syncthreads();
+  // ---- F ----
+  {
+    T acc[CF][4 * NQ_FORCE];
+    const int wF = R1.i1 - R1.i0;
+    const SmallDiv dwF = small_div(wF);
+#pragma unroll
+    for (int u = 0; u < CF; ++u) {
+#pragma unroll
+      for (int v = 0; v < 4 * NQ_FORCE; ++v) acc[u][v] = T(0);
+      const int lc = t + u * kFT;
+      if (lc < wF * h1) {
+        const int jr = sdiv(dwF, lc);
+        const int I = R1.i0 + (lc - jr * wF), J = R1.j0 + jr;
+        force_cell(st, kFCAP, fv0[u], fv1[u], g.X(I), g.X(I + 1), g.Y(J), g.Y(J + 1), a.gx,
+                   a.gy, acc[u]);
+      }
+    }
+    __syncthreads();
+    T nacc[NF][NQ_FORCE];
+    node_sums<T, NQ_FORCE, CF, NF>(st, acc, R1, R0, nx, ny, nacc);
+    force_epilogue<T, NF>(a, nacc, fpre, fd, R0, RN, vk, nm, NKN, nv, na, NTN);
+  }
+  __syncthreads();
+  // ---- G ----
+#pragma unroll
+  for (int j = 0; j < PPT; ++j) {
+    if (!(own & (1u << j))) continue;
+    const int p = pp[j];
+    const int I = Ii[j], J = Ji[j];
+    const int Ic = min(max(I, R0.i0), R0.i1 - 1), Jc = min(max(J, R0.j0), R0.j1 - 1);
+    const int ll = (Ic - R0.i0) + (Jc - R0.j0) * wTN;
+    const int ids[4] = {ll, ll + 1, ll + wTN + 1, ll + wTN};
+    T N[4];
+    q4_N4(xi[j], eta[j], N);
+    T gvx[4], gvy[4];
+    T nvx = T(0), nvy = T(0), nax = T(0), nay = T(0);
+    for (int k = 0; k < 4; ++k) {
+      gvx[k] = nv[ids[k]];
+      gvy[k] = nv[NTN + ids[k]];
+      nvx = nvx + N[k] * gvx[k];
+      nvy = nvy + N[k] * gvy[k];
+      nax = nax + N[k] * na[ids[k]];
+      nay = nay + N[k] * na[NTN + ids[k]];
+    }
+    const T pvx = a.vu ? nvx : vx[j] + nax * a.dt;
+    const T pvy = a.vu ? nvy : vy[j] + nay * a.dt;
+    const T x1 = x[j] + nvx * a.dt, y1 = y[j] + nvy * a.dt;
+    DR_W(DMPM_P_V, p, pvx);
+    DR_W(DMPM_P_V + 1, p, pvy);
+    DR_W(DMPM_P_X, p, x1);
+    DR_W(DMPM_P_X + 1, p, y1);
+    if (a.temps) {
+      PB_W(DMPM_P_MOMENTUM, p, m[j] * pvx);
+      PB_W(DMPM_P_MOMENTUM + 1, p, m[j] * pvy);
+    }
+    if (!USF) {
+      const int ci = min(max(I - RH.i0, 0), NCOL - 1), rj = min(max(J - RH.j0, 0), NROW - 1);
+      const int q = v1i[j];
+      StressUpd<T> u;
+      stress_update(a.mat, s1f[4 * kFCAP + q], colS[ci], rowS[rj], xi[j], eta[j], gvx, gvy,
+                    a.dt, u);
+      DR_W(DMPM_P_STRAIN + 0, p, ex[j] + u.d[0]);
+      DR_W(DMPM_P_STRAIN + 1, p, ey[j] + u.d[1]);
+      DR_W(DMPM_P_STRAIN + 3, p, gxy[j] + u.d[3]);
+      DR_W(DMPM_P_STRESS + 0, p, s1f[q] + u.ds[0]);
+      DR_W(DMPM_P_STRESS + 1, p, s1f[kFCAP + q] + u.ds[1]);
+      DR_W(DMPM_P_STRESS + 2, p, szz[j] + u.ds[2]);
+      DR_W(DMPM_P_STRESS + 3, p, s1f[2 * kFCAP + q] + u.ds[3]);
+      DR_W(DMPM_P_VOLUME, p, s1f[3 * kFCAP + q] * u.f);
+      DR_W(DMPM_P_DENSITY, p, rho[j] / u.f);
+      DR_W(DMPM_P_VOLSTRAIN_C, p, evc[j] + u.dvol);
+      if (a.temps) {
+        for (int k = 0; k < 6; ++k) {
+          PB_W(DMPM_P_STRAIN_RATE + k, p, u.rr[k]);
+          PB_W(DMPM_P_DSTRAIN + k, p, u.d[k]);
+        }
+        PB_W(DMPM_P_DVOLSTRAIN, p, u.dvol);
+      }
+    }
+    const int e0 = J * nx + I;
+    const int e1 = element_id<T, 2>(g.geo(), x1, y1);
+    const int k1 = (e1 >= 0 && e1 < g.ncells) ? e1 : g.ncells;
+    if (k1 != e0) record_move(a.L, p, e0, e1, k1);
+  }
+}
+
+// The step for the tiles of the overflow list of this step (all tiles when
+// ovf is null: FP64), slots staged in rounds; workgroups from nrb on handle
+// the particles outside the mesh.
+template <typename T, bool USF>
+__global__ __launch_bounds__(kFT) void k_step_rounds(StepArgs<T> a) {
+  constexpr int H = USF ? 2 : 1;
+  constexpr int CK = ((kTW + 2 * H) * (kTH + 2 * H) + kFT - 1) / kFT;  // cells/thread, K
+  constexpr int CF = ((kTW + 2) * (kTH + 2) + kFT - 1) / kFT;          // cells/thread, F
+  constexpr int NKN = (kTW + 2 * H - 1) * (kTH + 2 * H - 1);            // nodes of R_{H-1}
+  constexpr int NTN = (kTW + 1) * (kTH + 1);                           // nodes of the tile
+  constexpr int NK = (NKN + kFT - 1) / kFT, NF = (NTN + kFT - 1) / kFT;
+  constexpr int NCOL = kTW + 2 * H, NROW = kTH + 2 * H;
+  static_assert(CK * kFT <= 1024 && CF * kFT <= 1024 && NK * kFT <= 1024 &&
+                    NF * kFT <= 1024 && NCOL + 1 <= 64,
+                "small_div range");
+  __shared__ T st[7 * kRound];  // staged particle inputs; node-pass slices alias it
+  __shared__ T vk[2 * NKN];
+  __shared__ T nm[NKN];
+  __shared__ T nv[2 * NTN];
+  __shared__ T na[2 * NTN];
+  __shared__ T colC[NCOL], colS[NCOL], rowC[NROW], rowS[NROW];
+  __shared__ int segA[3][kSegRows], segP[3][kSegRows + 1];
+  const Grid2<T> g = a.g;
+  const int nx = g.nx, ny = g.ny;
+  const int t = threadIdx.x;
+  const int* __restrict__ cs = a.sr.cs();
+  const int* __restrict__ perm = a.sr.perm();
+  const int ident = a.sr.flags[1];
+
+  if ((int)blockIdx.x >= a.nrb) {
+    // particles outside the mesh: the state becomes NaN (as in k_g2p_tile)
+    const T nan = qnan<T>();
+    const int gs = kTailBlocks * kFT;
+    for (int s = cs[g.ncells] + ((int)blockIdx.x - a.nrb) * kFT + t; s < a.n; s += gs) {
+      const int p = ident ? s : perm[s];
+      for (int k = 0; k < kDynRows; ++k) DR_W(kDyn[k], p, nan);
+      for (int k = 0; k < kFixRows; ++k) PB_W(kFix[k], p, nan);
+      if (a.temps) {
+        PB_W(DMPM_P_XI, p, nan);
+        PB_W(DMPM_P_XI + 1, p, nan);
+        PB_W(DMPM_P_MOMENTUM, p, nan);
+        PB_W(DMPM_P_MOMENTUM + 1, p, nan);
+        for (int k = 0; k < 6; ++k) {
+          PB_W(DMPM_P_STRAIN_RATE + k, p, nan);
+          PB_W(DMPM_P_DSTRAIN + k, p, nan);
+        }
+        PB_W(DMPM_P_DVOLSTRAIN, p, nan);
+      }
+      // NaN positions search to element 0 (element.py:827-837)
+      record_move(a.L, p, g.ncells, 0, 0);
+    }
+    return;
+  }
+
+  if (a.ovf && blockIdx.x == 0 && t == 0) a.ovf[1 - a.par] = 0;  // the next step's count
+  const int cnt = a.ovf ? a.ovf[a.par] : a.ntiles;
+  for (int qt = blockIdx.x; qt < cnt; qt += a.nrb) {
+    const int tile = a.ovf ? a.ovf[2 + a.par * a.ntiles + qt] : qt;
+    __syncthreads();  // the previous tile's LDS
+    const int ti = tile % a.tiles_x, tj = tile / a.tiles_x;
+    const Rgn R0{ti * kTW, min(ti * kTW + kTW, nx), tj * kTH, min(tj * kTH + kTH, ny)};
+    const Rgn R1 = grow(R0, 1, nx, ny);
+    const Rgn RH = grow(R0, H, nx, ny);
+    const Rgn RN = USF ? R1 : R0;  // cells whose nodes get kinematic velocities
+    const int h0 = R0.j1 - R0.j0, h1 = R1.j1 - R1.j0, hH = RH.j1 - RH.j0;
+
+    // slot segments of the rows of R0 (table 0), R1 (1) and RH (2): one wave
+    // per table, lane r = row r, prefix by shuffles
+    {
+      const int q = t >> 6, r = t & 63;
+      if (q < 3) {
+        const Rgn R = q == 0 ? R0 : (q == 1 ? R1 : RH);
+        int len = 0;
+        if (r < R.j1 - R.j0) {
+          const int k = (R.j0 + r) * nx + R.i0;
+          const int s0 = cs[k];
+          segA[q][r] = s0;
+          len = cs[k + (R.i1 - R.i0)] - s0;
+        }
+  #pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+          const int y = __shfl_up(len, o, 64);
+          if (r >= o) len += y;
+        }
+        if (r < kSegRows) segP[q][r + 1] = len;
+        if (r == 0) segP[q][0] = 0;
+      } else if (q == 3) {
+        // cell centres and scales 2/(X1 - X0) of the columns and rows of RH
+        // (natural coordinates and inverse Jacobians, particle.py:180-206)
+        if (r < RH.i1 - RH.i0) {
+          const T X0 = g.X(RH.i0 + r), X1 = g.X(RH.i0 + r + 1);
+          colC[r] = (X0 + X1) / T(2);
+          colS[r] = T(2) / (X1 - X0);
+        }
+        if (r < hH) {
+          const T Y0 = g.Y(RH.j0 + r), Y1 = g.Y(RH.j0 + r + 1);
+          rowC[r] = (Y0 + Y1) / T(2);
+          rowS[r] = T(2) / (Y1 - Y0);
+        }
+      }
+    }
+    __syncthreads();
+    const int SH = segP[2][hH], S1 = segP[1][h1];
+    const int wK = RH.i1 - RH.i0;
+    const int wKN = RN.i1 - RN.i0 + 1, wTN = R0.i1 - R0.i0 + 1;
+
+    // ================= rounds: stage slots, reload per phase =====================
+    constexpr int RPT = kRound / kFT;  // slots per thread per round
+    // ---- K ----
+    {
+      int kv0[CK], kv1[CK];
+      cell_ranges<CK>(RH, nx, cs, segA[2], segP[2], kv0, kv1);
+      T acc[CK][4 * NQ_KIN];
+  #pragma unroll
+      for (int u = 0; u < CK; ++u)
+  #pragma unroll
+        for (int v = 0; v < 4 * NQ_KIN; ++v) acc[u][v] = T(0);
+      for (int v0 = 0; v0 < SH; v0 += kRound) {
+        const int vend = min(v0 + kRound, SH);
+        int pp[RPT];
+  #pragma unroll
+        for (int j = 0; j < RPT; ++j) {
+          const int v = v0 + t + j * kFT;
+          int p = -1;
+          if (v < vend) {
+            const int r = seg_row(segP[2], hH, v);
+            const int s = segA[2][r] + (v - segP[2][r]);
+            p = ident ? s : perm[s];
+          }
+          pp[j] = p;
+        }
+        T x[RPT], y[RPT], m[RPT], vx[RPT], vy[RPT];
+  #pragma unroll
+        for (int j = 0; j < RPT; ++j) {
+          if (pp[j] < 0) continue;
+          const int p = pp[j];
+          x[j] = SR(DMPM_P_X, p);
+          y[j] = SR(DMPM_P_X + 1, p);
+          m[j] = PB(DMPM_P_MASS, p);
+          vx[j] = SR(DMPM_P_V, p);
+          vy[j] = SR(DMPM_P_V + 1, p);
+        }
+  #pragma unroll
+        for (int j = 0; j < RPT; ++j) {
+          if (pp[j] < 0) continue;
+          const int q = t + j * kFT;
+          const int e = element_id<T, 2>(g.geo(), x[j], y[j]);
+          const int I = e % nx, J = e / nx;
+          const T xi = natural(x[j], g.X(I), g.X(I + 1));
+          const T eta = natural(y[j], g.Y(J), g.Y(J + 1));
+          st[q] = xi;
+          st[kRound + q] = eta;
+          st[2 * kRound + q] = m[j];
+          st[3 * kRound + q] = m[j] * vx[j];
+          st[4 * kRound + q] = m[j] * vy[j];
+          if (a.temps && in_rgn(R0, I, J)) {
+            PB_W(DMPM_P_XI, pp[j], xi);
+            PB_W(DMPM_P_XI + 1, pp[j], eta);
+          }
+        }
+        __syncthreads();
+  #pragma unroll
+        for (int u = 0; u < CK; ++u)
+          kin_cell(st, kRound, max(kv0[u], v0) - v0, min(kv1[u], vend) - v0, acc[u]);
+        __syncthreads();
+      }
+      NodePre<NK> kpre;
+      node_pre<T, NK>(a, RN, kpre, nullptr);
+      node_pre_mask<T, NK>(a, kpre);
+      T nacc[NK][NQ_KIN];
+      node_sums<T, NQ_KIN, CK, NK>(st, acc, RH, RN, nx, ny, nacc);
+      kin_epilogue<T, NK>(a, nacc, kpre, RN, R0, vk, nm, NKN);
+      __syncthreads();
+    }
+    // ---- S + F ----
+    {
+      int fv0[CF], fv1[CF];
+      cell_ranges<CF>(R1, nx, cs, segA[1], segP[1], fv0, fv1);
+      T acc[CF][4 * NQ_FORCE];
+  #pragma unroll
+      for (int u = 0; u < CF; ++u)
+  #pragma unroll
+        for (int v = 0; v < 4 * NQ_FORCE; ++v) acc[u][v] = T(0);
+      const int wF = R1.i1 - R1.i0;
+    const SmallDiv dwF = small_div(wF);
+      for (int v0 = 0; v0 < S1; v0 += kRound) {
+        const int vend = min(v0 + kRound, S1);
+        int pp[RPT];
+  #pragma unroll
+        for (int j = 0; j < RPT; ++j) {
+          const int v = v0 + t + j * kFT;
+          int p = -1;
+          if (v < vend) {
+            const int r = seg_row(segP[1], h1, v);
+            const int s = segA[1][r] + (v - segP[1][r]);
+            p = ident ? s : perm[s];
+          }
+          pp[j] = p;
+        }
+  #pragma unroll
+        for (int j = 0; j < RPT; ++j) {
+          if (pp[j] < 0) continue;
+          const int p = pp[j];
+          const int q = t + j * kFT;
+          const T x = SR(DMPM_P_X, p), y = SR(DMPM_P_X + 1, p);
+          const int e = element_id<T, 2>(g.geo(), x, y);
+          const int I = e % nx, J = e / nx;
+          const T X0 = g.X(I), X1 = g.X(I + 1), Y0 = g.Y(J), Y1 = g.Y(J + 1);
+          const T xi = natural(x, X0, X1);
+          const T eta = natural(y, Y0, Y1);
+          T V, sxx, syy, sxy;
+          if (USF) {
+            const int Ic = min(max(I, RN.i0), RN.i1 - 1), Jc = min(max(J, RN.j0), RN.j1 - 1);
+            const int ll = (Ic - RN.i0) + (Jc - RN.j0) * wKN;
+            const int ids[4] = {ll, ll + 1, ll + wKN + 1, ll + wKN};
+            T nvx[4], nvy[4];
+  #pragma unroll
+            for (int k = 0; k < 4; ++k) {
+              nvx[k] = vk[ids[k]];
+              nvy[k] = vk[NKN + ids[k]];
+            }
+            T ix, iy;
+            q4_invJ(xi, eta, X0, X1, Y0, Y1, ix, iy);
+            StressUpd<T> u;
+            stress_update(a.mat, a.mat.per_E ? PB(DMPM_P_YOUNGS, p) : T(0), ix, iy, xi, eta, nvx,
+                          nvy, a.dt, u);
+            sxx = SR(DMPM_P_STRESS + 0, p) + u.ds[0];
+            syy = SR(DMPM_P_STRESS + 1, p) + u.ds[1];
+            sxy = SR(DMPM_P_STRESS + 3, p) + u.ds[3];
+            V = SR(DMPM_P_VOLUME, p) * u.f;
+            if (in_rgn(R0, I, J)) {
+              DR_W(DMPM_P_STRAIN + 0, p, SR(DMPM_P_STRAIN + 0, p) + u.d[0]);
+              DR_W(DMPM_P_STRAIN + 1, p, SR(DMPM_P_STRAIN + 1, p) + u.d[1]);
+              DR_W(DMPM_P_STRAIN + 3, p, SR(DMPM_P_STRAIN + 3, p) + u.d[3]);
+              DR_W(DMPM_P_STRESS + 0, p, sxx);
+              DR_W(DMPM_P_STRESS + 1, p, syy);
+              DR_W(DMPM_P_STRESS + 2, p, SR(DMPM_P_STRESS + 2, p) + u.ds[2]);
+              DR_W(DMPM_P_STRESS + 3, p, sxy);
+              DR_W(DMPM_P_VOLUME, p, V);
+              DR_W(DMPM_P_DENSITY, p, SR(DMPM_P_DENSITY, p) / u.f);
+              DR_W(DMPM_P_VOLSTRAIN_C, p, SR(DMPM_P_VOLSTRAIN_C, p) + u.dvol);
+              if (a.temps) {
+                for (int k = 0; k < 6; ++k) {
+                  PB_W(DMPM_P_STRAIN_RATE + k, p, u.rr[k]);
+                  PB_W(DMPM_P_DSTRAIN + k, p, u.d[k]);
+                }
+                PB_W(DMPM_P_DVOLSTRAIN, p, u.dvol);
+              }
+            }
+          } else {
+            V = SR(DMPM_P_VOLUME, p);
+            sxx = SR(DMPM_P_STRESS + 0, p);
+            syy = SR(DMPM_P_STRESS + 1, p);
+            sxy = SR(DMPM_P_STRESS + 3, p);
+          }
+          st[q] = xi;
+          st[kRound + q] = eta;
+          st[2 * kRound + q] = PB(DMPM_P_MASS, p);
+          st[3 * kRound + q] = -V;
+          st[4 * kRound + q] = sxx;
+          st[5 * kRound + q] = syy;
+          st[6 * kRound + q] = sxy;
+        }
+        __syncthreads();
+  #pragma unroll
+        for (int u = 0; u < CF; ++u) {
+          const int lc = t + u * kFT;
+          if (lc >= wF * h1) continue;
+          const int jr = sdiv(dwF, lc);
+        const int I = R1.i0 + (lc - jr * wF), J = R1.j0 + jr;
+          force_cell(st, kRound, max(fv0[u], v0) - v0, min(fv1[u], vend) - v0, g.X(I), g.X(I + 1),
+                     g.Y(J), g.Y(J + 1), a.gx, a.gy, acc[u]);
+        }
+        __syncthreads();
+      }
+      NodePre<NF> fpre;
+      T fd[NF][2];
+      node_pre<T, NF>(a, R0, fpre, fd);
+      node_pre_mask<T, NF>(a, fpre);
+      T nacc[NF][NQ_FORCE];
+      node_sums<T, NQ_FORCE, CF, NF>(st, acc, R1, R0, nx, ny, nacc);
+      force_epilogue<T, NF>(a, nacc, fpre, fd, R0, RN, vk, nm, NKN, nv, na, NTN);
+      __syncthreads();
+    }
+    // ---- G ----
+    const int S0 = segP[0][h0];
+    for (int v = t; v < S0; v += kFT) {
+      const int r = seg_row(segP[0], h0, v);
+      const int s = segA[0][r] + (v - segP[0][r]);
+      const int p = ident ? s : perm[s];
+      const T x = SR(DMPM_P_X, p), y = SR(DMPM_P_X + 1, p);
+      const int e = element_id<T, 2>(g.geo(), x, y);
+      const int I = e % nx, J = e / nx;
+      const T X0 = g.X(I), X1 = g.X(I + 1), Y0 = g.Y(J), Y1 = g.Y(J + 1);
+      const T xi = natural(x, X0, X1);
+      const T eta = natural(y, Y0, Y1);
+      const int Ic = min(max(I, R0.i0), R0.i1 - 1), Jc = min(max(J, R0.j0), R0.j1 - 1);
+      const int ll = (Ic - R0.i0) + (Jc - R0.j0) * wTN;
+      const int ids[4] = {ll, ll + 1, ll + wTN + 1, ll + wTN};
+      T N[4];
+      q4_N4(xi, eta, N);
+      T gvx[4], gvy[4];
+      T nvx = T(0), nvy = T(0), nax = T(0), nay = T(0);
+      for (int k = 0; k < 4; ++k) {
+        gvx[k] = nv[ids[k]];
+        gvy[k] = nv[NTN + ids[k]];
+        nvx = nvx + N[k] * gvx[k];
+        nvy = nvy + N[k] * gvy[k];
+        nax = nax + N[k] * na[ids[k]];
+        nay = nay + N[k] * na[NTN + ids[k]];
+      }
+      const T pvx = a.vu ? nvx : SR(DMPM_P_V, p) + nax * a.dt;
+      const T pvy = a.vu ? nvy : SR(DMPM_P_V + 1, p) + nay * a.dt;
+      const T x1 = x + nvx * a.dt, y1 = y + nvy * a.dt;
+      DR_W(DMPM_P_V, p, pvx);
+      DR_W(DMPM_P_V + 1, p, pvy);
+      DR_W(DMPM_P_X, p, x1);
+      DR_W(DMPM_P_X + 1, p, y1);
+      if (a.temps) {
+        const T m = PB(DMPM_P_MASS, p);
+        PB_W(DMPM_P_MOMENTUM, p, m * pvx);
+        PB_W(DMPM_P_MOMENTUM + 1, p, m * pvy);
+      }
+      if (!USF) {
+        T ix, iy;
+        q4_invJ(xi, eta, X0, X1, Y0, Y1, ix, iy);
+        StressUpd<T> u;
+        stress_update(a.mat, a.mat.per_E ? PB(DMPM_P_YOUNGS, p) : T(0), ix, iy, xi, eta, gvx, gvy,
+                      a.dt, u);
+        DR_W(DMPM_P_STRAIN + 0, p, SR(DMPM_P_STRAIN + 0, p) + u.d[0]);
+        DR_W(DMPM_P_STRAIN + 1, p, SR(DMPM_P_STRAIN + 1, p) + u.d[1]);
+        DR_W(DMPM_P_STRAIN + 3, p, SR(DMPM_P_STRAIN + 3, p) + u.d[3]);
+        for (int k = 0; k < 4; ++k) DR_W(DMPM_P_STRESS + k, p, SR(DMPM_P_STRESS + k, p) + u.ds[k]);
+        DR_W(DMPM_P_VOLUME, p, SR(DMPM_P_VOLUME, p) * u.f);
+        DR_W(DMPM_P_DENSITY, p, SR(DMPM_P_DENSITY, p) / u.f);
+        DR_W(DMPM_P_VOLSTRAIN_C, p, SR(DMPM_P_VOLSTRAIN_C, p) + u.dvol);
+        if (a.temps) {
+          for (int k = 0; k < 6; ++k) {
+            PB_W(DMPM_P_STRAIN_RATE + k, p, u.rr[k]);
+            PB_W(DMPM_P_DSTRAIN + k, p, u.d[k]);
+          }
+          PB_W(DMPM_P_DVOLSTRAIN, p, u.dvol);
+        }
+      }
+      const int e1 = element_id<T, 2>(g.geo(), x1, y1);
+      const int k1 = (e1 >= 0 && e1 < g.ncells) ? e1 : g.ncells;
+      if (k1 != e) record_move(a.L, p, e, e1, k1);
+    }
+  }
+}
+
+#undef SR
+#undef PB
+#undef NB
+#undef DR_W
+#undef PB_W
+#undef NB_W
+
 // Element ids from the positions at the start of a run.  A particle whose cell
 // differs from its sort key was moved outside the fused step (positions set
 // by the caller, a recomputed step): the change is recorded like one from
@@ -1043,6 +2192,113 @@ int tile_steps(dmpm_plan* P, const Mesh2<T>& M, T* pb, int32_t* eid, T* nb, int 
   return DMPM_OK;
 }
 
+__global__ void k_zero_ints(int* p, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
+// Whether the one-launch step can run this plan: one simulation, and the
+// particle and node blocks below 4 GiB (the kernels' 32-bit buffer offsets).
+bool step_eligible(const dmpm_plan* P) {
+  const uint64_t esz = P->d.dtype == DMPM_F32 ? 4 : 8;
+  return P->step_kernel && P->nsims == 1 &&
+         (uint64_t)DMPM_P_ROWS * (uint64_t)P->d.ld * esz < (1ull << 32) &&
+         (uint64_t)DMPM_N_ROWS * (uint64_t)P->d.nld * esz < (1ull << 32);
+}
+
+// The one-launch step for single-simulation plans: per step, the re-sort,
+// k_step_fast over every tile (FP32) and k_step_rounds over the tiles that
+// overflowed the fast path and the particles outside the mesh.  The
+// particle state alternates between the caller's block and a shadow block
+// (dynamic rows only); after an odd number of steps those rows are copied
+// back.
+template <typename T>
+int step_run(dmpm_plan* P, const Mesh2<T>& M, T* pb, int32_t* eid, T* nb, int scheme, int vu,
+             const double* grav, double dt, int64_t step0, int64_t nsteps, int temps,
+             hipStream_t s, T* vkin) {
+  constexpr bool kFast = sizeof(T) == 4;
+  const int64_t ld = P->d.ld, nld = P->d.nld;
+  const int nn = P->nnodes;
+  StepArgs<T> a;
+  a.tiles_x = (M.g0.nx + kTW - 1) / kTW;
+  a.ntiles = a.tiles_x * ((M.g0.ny + kTH - 1) / kTH);
+  T* shadow = (T*)plan_workspace(P, 5, (size_t)DMPM_P_ROWS * ld * sizeof(T));
+  T* vbuf = (T*)plan_workspace(P, 6, (size_t)4 * nn * sizeof(T));
+  int* ovf = kFast ? (int*)plan_workspace(P, 7, (size_t)(2 + 2 * a.ntiles) * sizeof(int)) : nullptr;
+  if (!shadow || !vbuf || (kFast && !ovf)) return fail(DMPM_EHIP, "workspace allocation failed");
+  if (kFast) k_zero_ints<<<1, 64, 0, s>>>(ovf, 2);
+  DMPM_TRY(hip_check(hipMemcpy2DAsync(vbuf, nn * sizeof(T), nb + DMPM_N_V * nld, nld * sizeof(T),
+                                      nn * sizeof(T), 2, hipMemcpyDeviceToDevice, s),
+                     "hipMemcpy2DAsync"));
+  a.g = M.g0;
+  a.pb = pb;
+  a.ld32 = (uint32_t)ld;
+  a.nb = nb;
+  a.nld32 = (uint32_t)nld;
+  a.pbytes = (uint32_t)((uint64_t)DMPM_P_ROWS * ld * sizeof(T));
+  a.nbytes = (uint32_t)((uint64_t)DMPM_N_ROWS * nld * sizeof(T));
+  a.vkin = vkin;
+  a.sr = sort_ref(P);
+  a.L = move_log(P);
+  a.cidx = P->cidx;
+  a.cmask = P->cmask;
+  a.cval = P->cval;
+  a.nfd = P->nfd;
+  a.nf_off = P->nf_off;
+  a.nf_fn = P->nf_fn;
+  a.nf_val = P->nf_val;
+  a.fns.kind = P->fn_kind;
+  a.fns.off = P->fn_off;
+  a.fns.cnt = P->fn_cnt;
+  a.fns.xs = P->fn_x;
+  a.fns.fs = P->fn_f;
+  a.mat.kind = P->d.material;
+  a.mat.per_E = P->d.per_particle_E;
+  for (int k = 0; k < 3; ++k) a.mat.de[k] = P->d.de[k];
+  a.gx = T(grav[0]);
+  a.gy = T(grav[1]);
+  a.dt = T(dt);
+  a.vu = vu;
+  a.temps = temps;
+  a.n = (int)P->n;
+  a.ovf = ovf;
+  // FP32: overflow tiles are rare, a few workgroups loop over them
+  a.nrb = kFast ? std::min(a.ntiles, 64) : a.ntiles;
+  for (int64_t k = 0; k < nsteps; ++k) {
+    T* src = (k & 1) ? shadow : pb;
+    {
+      ProfScopeT ps_(P, KC_SORT, s);
+      DMPM_TRY(enqueue_resort<T>(P, M, src, eid, s));
+    }
+    a.src = src;
+    a.dst = (k & 1) ? pb : shadow;
+    a.vold = vbuf + (k & 1) * 2 * (int64_t)nn;
+    a.vnew = vbuf + ((k + 1) & 1) * 2 * (int64_t)nn;
+    a.t = T(step0 + k) * a.dt;
+    a.par = (int)(k & 1);
+    ProfScopeT ps_(P, KC_STEP, s);
+    if (scheme == DMPM_USF) {
+      if constexpr (kFast) k_step_fast<T, true><<<a.ntiles, kFT, 0, s>>>(a);
+      k_step_rounds<T, true><<<a.nrb + kTailBlocks, kFT, 0, s>>>(a);
+    } else {
+      if constexpr (kFast) k_step_fast<T, false><<<a.ntiles, kFT, 0, s>>>(a);
+      k_step_rounds<T, false><<<a.nrb + kTailBlocks, kFT, 0, s>>>(a);
+    }
+    DMPM_TRY(launch_check("step"));
+  }
+  if (nsteps & 1) {
+    // dynamic rows back into the caller's block, in runs of adjacent rows
+    const int runs[6][2] = {{DMPM_P_X, 4},          {DMPM_P_VOLUME, 2},     {DMPM_P_STRAIN, 2},
+                            {DMPM_P_STRAIN + 3, 1}, {DMPM_P_STRESS, 4}, {DMPM_P_VOLSTRAIN_C, 1}};
+    for (const auto& r : runs)
+      DMPM_TRY(hip_check(hipMemcpy2DAsync(pb + r[0] * ld, ld * sizeof(T), shadow + r[0] * ld,
+                                          ld * sizeof(T), P->n * sizeof(T), r[1],
+                                          hipMemcpyDeviceToDevice, s),
+                         "hipMemcpy2DAsync"));
+  }
+  return DMPM_OK;
+}
+
 template <typename T>
 int tile_run_t(dmpm_plan* P, T* pb, int32_t* eid, T* nb, int scheme, int vu, const double* grav,
                double dt, int64_t step0, int64_t nsteps, int temps, hipStream_t s) {
@@ -1061,6 +2317,8 @@ int tile_run_t(dmpm_plan* P, T* pb, int32_t* eid, T* nb, int scheme, int vu, con
   }
   k_tile_check<T><<<nblk(n), kThreads, 0, s>>>(M, pb, ld, n, eid, P->key, move_log(P));
   DMPM_TRY(launch_check("tile_check"));
+  if (step_eligible(P))
+    return step_run<T>(P, M, pb, eid, nb, scheme, vu, grav, dt, step0, nsteps, temps, s, vkin);
   switch (P->cell_block) {
     case 64:
       return tile_steps<T, 64>(P, M, pb, eid, nb, scheme, vu, grav, dt, step0, nsteps, temps, s,

@@ -152,8 +152,12 @@ int dmpm_plan_set_block(dmpm_plan* plan, int32_t cells_per_block);
 /* Execution options (no effect on results; no reference counterpart):
  * DMPM_OPT_USF_SPLIT = 1: the fused USF step updates stresses in a streamed
  * particle kernel before the force P2G cell pass (1, default) or inside that
- * pass (0). */
-enum { DMPM_OPT_USF_SPLIT = 1 };
+ * pass (0).
+ * DMPM_OPT_STEP_KERNEL = 2: a single-simulation plan runs each fused 2D step
+ * as one kernel launch after the re-sort (1, default; workgroups own tiles of
+ * cells and recompute a halo) or as one launch per phase (0; batch plans
+ * always do). */
+enum { DMPM_OPT_USF_SPLIT = 1, DMPM_OPT_STEP_KERNEL = 2 };
 int dmpm_plan_set_option(dmpm_plan* plan, int32_t option, int32_t value);
 /* Declares that the element ids changed outside the library (forces a re-sort). */
 int dmpm_mark_unsorted(dmpm_plan* plan, void* stream);
@@ -290,7 +294,7 @@ enum {
   DMPM_KC_LOCATE = 0, DMPM_KC_SORT = 1, DMPM_KC_P2G_KIN = 2, DMPM_KC_STRESS = 3,
   DMPM_KC_P2G_FORCE = 4, DMPM_KC_GRID = 5, DMPM_KC_G2P = 6, DMPM_KC_SMALL = 7,
   DMPM_KC_TILE_KIN = 8, DMPM_KC_TILE_FORCE = 9, DMPM_KC_TILE_G2P = 10,
-  DMPM_KC_ADJ = 11, DMPM_KC_BATCH = 12, DMPM_KC_ALL = -2
+  DMPM_KC_ADJ = 11, DMPM_KC_BATCH = 12, DMPM_KC_STEP = 13, DMPM_KC_ALL = -2
 };
 /* Times every launch of one kernel class (DMPM_KC_ALL: of every class) with
  * HIP event pairs recorded on the launch stream; kernel_class = -1 disables.

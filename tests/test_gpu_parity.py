@@ -375,3 +375,70 @@ def test_tile_path_crowded_cell_equals_per_phase_path(scheme):
     pb.run(scheme, False, g, spec["dt"], 0, 3, True, per_phase=True)
     for x, y in zip(_state(a), _state(b)):
         assert torch.equal(x, y)
+
+
+def _run_pair(spec, scheme, nsteps, chunk, vu=False, per_phase_b=False):
+    """Runs spec on two fresh simulations: a with the one-launch step, b with
+    one launch per phase (or the per-phase kernels); compares after every
+    chunk of steps."""
+    from diffmpm_amd import _lib as L
+    from diffmpm_amd._plan import plan_for
+    a, b = cases.build_gpu(spec), cases.build_gpu(spec)
+    g = torch.tensor(spec["gravity"], dtype=torch.float64)
+    pa = plan_for(a.mesh.elements, a.mesh.particles[0])
+    pb = plan_for(b.mesh.elements, b.mesh.particles[0])
+    pa.set_option(L.DMPM_OPT_STEP_KERNEL, 1)
+    pb.set_option(L.DMPM_OPT_STEP_KERNEL, 0)
+    for k in range(0, nsteps, chunk):
+        pa.run(scheme, vu, g, spec["dt"], k, chunk, True)
+        pb.run(scheme, vu, g, spec["dt"], k, chunk, True, per_phase=per_phase_b)
+        for x, y in zip(_state(a), _state(b)):
+            assert torch.equal(x, y), k
+    return a
+
+
+@pytest.mark.parametrize("chunk", [1, 3, 4])
+@pytest.mark.parametrize("scheme,per_E,drift", [("usf", False, (0.6, 0.3)),
+                                                 ("usl", False, (0.6, 0.3)),
+                                                 ("usf", True, None), ("usl", True, (0.6, 0.3))])
+def test_one_launch_step_equals_phase_launches(scheme, per_E, drift, chunk):
+    """The one-launch step (tiles of cells with halos, alternating
+    particle blocks) gives bit-identical states to one launch per phase, for
+    odd and even step counts per run (the state ends in the caller's block
+    either way), across cell changes, on a mesh of partial tiles (cases that
+    stay finite in the oracle too)."""
+    spec = _moving_spec(scheme, per_E, drift=drift, velocity_scale=0.05 if drift else 0.1)
+    a = _run_pair(spec, scheme, 12, chunk)
+    assert torch.isfinite(a.mesh.particles[0].loc).all()
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+@pytest.mark.parametrize("scheme", ["usf", "usl"])
+def test_one_launch_step_large_mesh(scheme, dtype):
+    """Several full tiles in each direction (96x70 cells), nodal force and
+    velocity update: one-launch step equals the per-phase kernels."""
+    spec = _moving_spec(scheme, nx=96, ny=70, rows=60, drift=None, velocity_scale=0.1)
+    spec["velocity_update"] = True
+    from diffmpm_amd import _lib as L
+    from diffmpm_amd._plan import plan_for
+    a, b = cases.build_gpu(spec, dtype), cases.build_gpu(spec, dtype)
+    g = torch.tensor(spec["gravity"], dtype=torch.float64)
+    pa = plan_for(a.mesh.elements, a.mesh.particles[0])
+    pb = plan_for(b.mesh.elements, b.mesh.particles[0])
+    pa.set_option(L.DMPM_OPT_STEP_KERNEL, 1)
+    pa.run(scheme, True, g, spec["dt"], 0, 5, True)
+    pb.run(scheme, True, g, spec["dt"], 0, 5, True, per_phase=True)
+    for x, y in zip(_state(a), _state(b)):
+        assert torch.equal(x, y)
+
+
+@pytest.mark.parametrize("scheme", ["usf", "usl"])
+def test_one_launch_step_crowded_and_outside(scheme):
+    """A 3000-particle cell (several staging rounds in one tile) and
+    particles outside the mesh (NaN state, handled by the tail workgroups)."""
+    spec = _crowded_spec(3000)
+    spec["scheme"] = scheme
+    pts = np.asarray(spec["particles"])
+    pts[:3] = [[1.2, 0.3], [np.nan, 0.4], [0.3, 1.5]]
+    spec["particles"] = pts.tolist()
+    _run_pair(spec, scheme, 3, 3)
