@@ -675,6 +675,19 @@ __device__ __forceinline__ void bst(T* base, uint32_t nbytes, uint32_t row_elems
 #define PB_W(r, i, v) bst(a.pb, a.pbytes, (uint32_t)(r) * a.ld32, (uint32_t)(i), (T)(v))
 #define NB_W(r, i, v) bst(a.nb, a.nbytes, (uint32_t)(r) * a.nld32, (uint32_t)(i), (T)(v))
 
+// Step parameters the kernels read only in rare branches (cell changes,
+// constrained nodes, concentrated nodal forces), kept in device memory so
+// that they take no registers elsewhere.
+struct StepAux {
+  MoveLog L;
+  const double* cval;
+  const int* nfd;
+  const int* nf_off;
+  const int* nf_fn;
+  const double* nf_val;
+  FnTable fns;
+};
+
 template <typename T> struct StepArgs {
   Grid2<T> g;
   const T* src;
@@ -689,15 +702,10 @@ template <typename T> struct StepArgs {
   T* vnew;        // [2][nnodes] nodal velocity after the step
   T* vkin;        // [2][nnodes] kinematic nodal velocity (for the adjoint's export)
   SortRef sr;
-  MoveLog L;
   const int* cidx;
   const int* cmask;
-  const double* cval;
-  const int* nfd;
-  const int* nf_off;
-  const int* nf_fn;
-  const double* nf_val;
-  FnTable fns;
+  const StepAux* aux;  // tables used in rare branches
+  int has_nf;          // concentrated nodal forces present
   Mat2 mat;
   T gx, gy, dt, t;
   int vu, temps, tiles_x, ntiles, n;
@@ -906,7 +914,7 @@ __device__ __forceinline__ void kin_epilogue(const StepArgs<T>& a, const T (&nac
     for (int d = 0; d < 2; ++d) {
       T v = (m == T(0)) ? a.vold[(int64_t)d * nnodes + nd] : p[d] / m;
       v = snap(v);
-      if (mask & (1 << d)) v = T(a.cval[2 * cr + d]);
+      if (mask & (1 << d)) v = T(a.aux->cval[2 * cr + d]);
       vk[d * nkn + ln] = v;
       if (own) a.vkin[(int64_t)d * nnodes + nd] = v;
     }
@@ -940,10 +948,13 @@ __device__ __forceinline__ void force_epilogue(const StepArgs<T>& a,
 #pragma unroll
     for (int d = 0; d < 2; ++d) {
       T fe = nacc[w][d];
-      const int qg = a.nfd ? a.nfd[(int64_t)d * nnodes + nd] : -1;
-      if (qg >= 0)
-        for (int k = a.nf_off[qg]; k < a.nf_off[qg + 1]; ++k)
-          fe = fe + fn_value<T>(a.fns, a.nf_fn[k], a.t) * T(a.nf_val[k]);
+      if (a.has_nf) {
+        const StepAux& x = *a.aux;
+        const int qg = x.nfd[(int64_t)d * nnodes + nd];
+        if (qg >= 0)
+          for (int k = x.nf_off[qg]; k < x.nf_off[qg + 1]; ++k)
+            fe = fe + fn_value<T>(x.fns, x.nf_fn[k], a.t) * T(x.nf_val[k]);
+      }
       const T fi = nacc[w][2 + d];
       const T F = (fi + fe) + fd[w][d];
       T ac = nan_to_num(F / m);
@@ -952,7 +963,7 @@ __device__ __forceinline__ void force_epilogue(const StepArgs<T>& a,
       v = snap(v);
       ac = snap(ac);
       if (mask & (1 << d)) {
-        const T c = T(a.cval[2 * cr + d]);
+        const T c = T(a.aux->cval[2 * cr + d]);
         v = snap(c);
         pm = m * c;
         ac = T(0);
@@ -1133,6 +1144,7 @@ __global__ __launch_bounds__(kFT) __attribute__((amdgpu_waves_per_eu(4))) void k
     if (t + j * kFT < SH) live |= 1u << j;
     pp[j] = s;
     const int J = RH.j0 + r;
+    Ji[j] = J;
     v1i[j] = -1;
     const int r1 = J - R1.j0;
     if (r1 >= 0 && r1 < h1) {
@@ -1204,10 +1216,13 @@ __global__ __launch_bounds__(kFT) __attribute__((amdgpu_waves_per_eu(4))) void k
 #pragma unroll
   for (int j = 0; j < PPT; ++j) {
     const int v = t + j * kFT;
-    int I, J;
-    cell_of(g, x[j], y[j], I, J);
+    // the row is the slot's (the re-sort keyed the slot by this position's
+    // element); the column as in element_id, with its element-0 and
+    // next-row quirks
+    const int J = Ji[j];
+    int I = last_le(x[j], nx, g.hx, g.ihx);
+    if (I < 0 || !(y[j] >= T(0)) || I == nx) I = 0;
     Ii[j] = I;
-    Ji[j] = J;
     const int ci = min(max(I - RH.i0, 0), NCOL - 1), rj = min(max(J - RH.j0, 0), NROW - 1);
     xi[j] = (x[j] - colC[ci]) * colS[ci];
     eta[j] = (y[j] - rowC[rj]) * rowS[rj];
@@ -1374,9 +1389,11 @@ __global__ __launch_bounds__(kFT) __attribute__((amdgpu_waves_per_eu(4))) void k
       }
     }
     const int e0 = J * nx + I;
+    // still inside cell (I, J): no element search (the same element id)
+    if (x1 >= g.X(I) && x1 < g.X(I + 1) && y1 >= g.Y(J) && y1 < g.Y(J + 1)) continue;
     const int e1 = element_id<T, 2>(g.geo(), x1, y1);
     const int k1 = (e1 >= 0 && e1 < g.ncells) ? e1 : g.ncells;
-    if (k1 != e0) record_move(a.L, p, e0, e1, k1);
+    if (k1 != e0) record_move(a.aux->L, p, e0, e1, k1);
   }
 }
 
@@ -1429,7 +1446,7 @@ __global__ __launch_bounds__(kFT) void k_step_rounds(StepArgs<T> a) {
         PB_W(DMPM_P_DVOLSTRAIN, p, nan);
       }
       // NaN positions search to element 0 (element.py:827-837)
-      record_move(a.L, p, g.ncells, 0, 0);
+      record_move(a.aux->L, p, g.ncells, 0, 0);
     }
     return;
   }
@@ -1726,7 +1743,7 @@ __global__ __launch_bounds__(kFT) void k_step_rounds(StepArgs<T> a) {
       }
       const int e1 = element_id<T, 2>(g.geo(), x1, y1);
       const int k1 = (e1 >= 0 && e1 < g.ncells) ? e1 : g.ncells;
-      if (k1 != e) record_move(a.L, p, e, e1, k1);
+      if (k1 != e) record_move(a.aux->L, p, e, e1, k1);
     }
   }
 }
@@ -2239,19 +2256,26 @@ int step_run(dmpm_plan* P, const Mesh2<T>& M, T* pb, int32_t* eid, T* nb, int sc
   a.nbytes = (uint32_t)((uint64_t)DMPM_N_ROWS * nld * sizeof(T));
   a.vkin = vkin;
   a.sr = sort_ref(P);
-  a.L = move_log(P);
   a.cidx = P->cidx;
   a.cmask = P->cmask;
-  a.cval = P->cval;
-  a.nfd = P->nfd;
-  a.nf_off = P->nf_off;
-  a.nf_fn = P->nf_fn;
-  a.nf_val = P->nf_val;
-  a.fns.kind = P->fn_kind;
-  a.fns.off = P->fn_off;
-  a.fns.cnt = P->fn_cnt;
-  a.fns.xs = P->fn_x;
-  a.fns.fs = P->fn_f;
+  StepAux x;
+  x.L = move_log(P);
+  x.cval = P->cval;
+  x.nfd = P->nfd;
+  x.nf_off = P->nf_off;
+  x.nf_fn = P->nf_fn;
+  x.nf_val = P->nf_val;
+  x.fns.kind = P->fn_kind;
+  x.fns.off = P->fn_off;
+  x.fns.cnt = P->fn_cnt;
+  x.fns.xs = P->fn_x;
+  x.fns.fs = P->fn_f;
+  a.has_nf = P->nfd != nullptr;
+  StepAux* aux = (StepAux*)plan_workspace(P, 8, sizeof(StepAux));
+  if (!aux) return fail(DMPM_EHIP, "workspace allocation failed");
+  DMPM_TRY(hip_check(hipMemcpyAsync(aux, &x, sizeof(StepAux), hipMemcpyHostToDevice, s),
+                     "hipMemcpyAsync"));
+  a.aux = aux;
   a.mat.kind = P->d.material;
   a.mat.per_E = P->d.per_particle_E;
   for (int k = 0; k < 3; ++k) a.mat.de[k] = P->d.de[k];
