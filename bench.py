@@ -47,12 +47,13 @@ STEP_BYTES = 116  # per particle-step, whole step (FP32 2D LinearElastic)
 # adjoint step: read primal x(2), v(2), m, V, sigma(xx, yy, xy); read and write
 # the cotangents of x(2), v(2), V, sigma(xx, yy, xy); write dL/dlogE_p
 ADJOINT_BYTES = 4 * 9 + 4 * 8 * 2 + 4
-# HBM bytes per particle per launch of each kernel class of the default path at
-# 1024x1024, from rocprofv3 --pmc passes of this benchmark: reads = TCC_EA0_RDREQ
+# HBM bytes per particle per launch of each kernel class at 1024x1024 (step: the
+# default one-launch path; the others: --phase-launches), from rocprofv3 --pmc
+# passes of this benchmark: reads = TCC_EA0_RDREQ
 # x 128 B, writes = WRITE_SIZE (calibrated on a 512 MiB device copy; see
 # profiles/step_q4_1024_usf_hbm_counters.md)
-TRAFFIC_BYTES = {"sort": 23.6, "tile_kin": 57.7, "stress": 91.0, "tile_force": 86.4,
-                 "tile_g2p": 38.1}
+TRAFFIC_BYTES = {"step": 161.9, "sort": 23.6, "tile_kin": 57.7, "stress": 91.0,
+                 "tile_force": 86.4, "tile_g2p": 38.1}
 
 
 def cpu_baseline(rows=32, steps=8):
