@@ -1,11 +1,20 @@
-// Fused 2D explicit step (gfx950): re-sort by cell, P2G as a cell pass and a
-// node pass, G2P.
+// Fused 2D explicit step (gfx950): re-sort by cell, then either one launch
+// for the whole step (single-simulation plans, the default) or one launch per
+// phase (P2G as a cell pass and a node pass, stress update, G2P).
 //
 // One step is:
 //   k_sort_s1..s4          sort by cell: nothing when no particle changed
 //                          cell, an O(N) update of the stable order from the
 //                          recorded cell changes, a full counting sort when
 //                          the layout is unknown
+// then, one launch:
+//   k_step_fast            per tile of cells, with a recomputed halo: P2G of
+//                          mass and momentum, nodal velocity, USF stress
+//                          update, P2G of forces, nodal update, G2P (USL:
+//                          stress update), cell changes for the next sort
+//   k_step_rounds          tiles that do not fit k_step_fast, and particles
+//                          outside the mesh
+// or, one launch per phase:
 //   k_cell_kin/k_node_kin  P2G of mass and momentum, nodal velocity,
 //                          constraints
 //   k_stress_tile          USF strain/volume/stress update, streamed (or, as
@@ -17,8 +26,8 @@
 //
 // Nodal sums are formed without atomics and in a fixed order (per-cell sums
 // in ascending particle index, then the four cells of a node in the order
-// (i-1,j-1), (i,j-1), (i-1,j), (i,j)); the per-phase kernels use the same
-// order and the same per-term arithmetic, so both paths are bit-identical.
+// (i-1,j-1), (i,j-1), (i-1,j), (i,j)); every path uses the same order and the
+// same per-term arithmetic, so all of them are bit-identical.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
