@@ -155,15 +155,31 @@ class Differentiable:
             return self.el._nx * self.el._ny
         return sum(s.mesh.elements._nx * s.mesh.elements._ny for s in self._ens.sims)
 
+    def _cot_buffer(self, gout):
+        """An (8, ld) cotangent block holding gout in its first n columns: the
+        block itself when gout is the (8, n) view of one that an earlier
+        adjoint step returned (the kernels update it in place), else a new
+        one."""
+        ld = self._pbuf.shape[1]
+        if (gout.dim() == 2 and gout.shape == (C_ROWS, self.n) and gout.stride() == (ld, 1)
+                and gout.storage_offset() == 0 and gout.dtype == self._pbuf.dtype
+                and gout.untyped_storage().nbytes() >= C_ROWS * ld * gout.element_size()):
+            return torch.as_strided(gout, (C_ROWS, ld), (ld, 1))
+        cot = torch.empty((C_ROWS, ld), dtype=self._pbuf.dtype, device=self._pbuf.device)
+        cot[:, : self.n] = gout
+        return cot
+
     def adjoint_step_saved(self, state, log_e, out, saved, gravity, gout):
         """Like adjoint_step, from the quantities forward_step(save=True)
         returned (no recomputation of the step)."""
         nsaved, vkin, perm, cs = saved
         self._write(self.pin, state.detach(), log_e)
-        self._write(self.pwork, out.detach(), None)
+        if self.code == _lib.DMPM_USF:
+            # the kernels read the stress and volume after the step (USF only)
+            for a, b in _RUNS[1:]:
+                self.pwork[b, : self.n] = out.detach()[a]
         ld = self._pbuf.shape[1]
-        cot = torch.empty((C_ROWS, ld), dtype=self._pbuf.dtype, device=self._pbuf.device)
-        cot[:, : self.n] = gout
+        cot = self._cot_buffer(gout)
         lebar = torch.zeros(ld, dtype=self._pbuf.dtype, device=self._pbuf.device)
         call("dmpm_step_adjoint_saved", self.fplan.h, ptr(self.pin), ptr(self.pwork),
              ptr(nsaved), ptr(vkin), ptr(perm), ptr(cs), ptr(cot), ptr(lebar), self.code, self.vu,
@@ -176,8 +192,7 @@ class Differentiable:
         self.aplan.refresh_tables()
         self._write(self.pin, state.detach(), log_e)
         ld = self._pbuf.shape[1]
-        cot = torch.empty((C_ROWS, ld), dtype=self._pbuf.dtype, device=self._pbuf.device)
-        cot[:, : self.n] = gout
+        cot = self._cot_buffer(gout)
         lebar = torch.zeros(ld, dtype=self._pbuf.dtype, device=self._pbuf.device)
         call("dmpm_step_adjoint", self.aplan.h, ptr(self.pin), ptr(self.pwork), ptr(self.eid),
              ptr(self.nwork), ptr(cot), ptr(lebar), self.code, self.vu,
@@ -204,9 +219,9 @@ class _StepFn(torch.autograd.Function):
         le = log_e if ctx.has_log_e else None
         if ctx.saved_fwd:
             gs, ge = d.adjoint_step_saved(state, le, extra[0], tuple(extra[1:]), ctx.gravity,
-                                          gout.contiguous())
+                                          gout)
         else:
-            gs, ge = d.adjoint_step(state, le, ctx.gravity, ctx.i, gout.contiguous())
+            gs, ge = d.adjoint_step(state, le, ctx.gravity, ctx.i, gout)
         g_log_e = None
         if ctx.has_log_e and ctx.needs_input_grad[1]:
             g_log_e = ge if log_e.dim() > 0 else ge.sum()

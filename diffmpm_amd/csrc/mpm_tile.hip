@@ -2145,8 +2145,10 @@ int enqueue_resort(dmpm_plan* P, const Mesh2<T>& M, const T* pb, int32_t* eid, h
   const int g12 = std::max(P->nchunks, kSortGrid);
   k_sort_s1<T><<<g12, kThreads, 0, s>>>(M, pb, P->d.ld, a);
   k_sort_s2<<<g12, kThreads, 0, s>>>(a);
-  k_sort_s3<<<kSortGrid, kThreads, 0, s>>>(a);
-  k_sort_s4<<<kSortGrid / 4, kThreads, 0, s>>>(a);
+  // stages 3 and 4 have little to do in the incremental mode (the movers, a
+  // flag): small grids, since scheduling idle workgroups costs more
+  k_sort_s3<<<kSortGrid / 4, kThreads, 0, s>>>(a);
+  k_sort_s4<<<kSortGrid / 8, kThreads, 0, s>>>(a);
   return launch_check("resort");
 }
 
@@ -2236,8 +2238,7 @@ bool step_eligible(const dmpm_plan* P) {
 // k_step_fast over every tile (FP32) and k_step_rounds over the tiles that
 // overflowed the fast path and the particles outside the mesh.  The
 // particle state alternates between the caller's block and a shadow block
-// (dynamic rows only); after an odd number of steps those rows are copied
-// back.
+// (dynamic rows only); nsteps is even, so it ends in the caller's block.
 template <typename T>
 int step_run(dmpm_plan* P, const Mesh2<T>& M, T* pb, int32_t* eid, T* nb, int scheme, int vu,
              const double* grav, double dt, int64_t step0, int64_t nsteps, int temps,
@@ -2319,18 +2320,13 @@ int step_run(dmpm_plan* P, const Mesh2<T>& M, T* pb, int32_t* eid, T* nb, int sc
     }
     DMPM_TRY(launch_check("step"));
   }
-  if (nsteps & 1) {
-    // dynamic rows back into the caller's block, in runs of adjacent rows
-    const int runs[6][2] = {{DMPM_P_X, 4},          {DMPM_P_VOLUME, 2},     {DMPM_P_STRAIN, 2},
-                            {DMPM_P_STRAIN + 3, 1}, {DMPM_P_STRESS, 4}, {DMPM_P_VOLSTRAIN_C, 1}};
-    for (const auto& r : runs)
-      DMPM_TRY(hip_check(hipMemcpy2DAsync(pb + r[0] * ld, ld * sizeof(T), shadow + r[0] * ld,
-                                          ld * sizeof(T), P->n * sizeof(T), r[1],
-                                          hipMemcpyDeviceToDevice, s),
-                         "hipMemcpy2DAsync"));
-  }
   return DMPM_OK;
 }
+
+template <typename T>
+int tile_steps_cb(dmpm_plan* P, const Mesh2<T>& M, T* pb, int32_t* eid, T* nb, int scheme,
+                  int vu, const double* grav, double dt, int64_t step0, int64_t nsteps, int temps,
+                  hipStream_t s, T* part, T* vkin);
 
 template <typename T>
 int tile_run_t(dmpm_plan* P, T* pb, int32_t* eid, T* nb, int scheme, int vu, const double* grav,
@@ -2350,8 +2346,29 @@ int tile_run_t(dmpm_plan* P, T* pb, int32_t* eid, T* nb, int scheme, int vu, con
   }
   k_tile_check<T><<<nblk(n), kThreads, 0, s>>>(M, pb, ld, n, eid, P->key, move_log(P));
   DMPM_TRY(launch_check("tile_check"));
-  if (step_eligible(P))
-    return step_run<T>(P, M, pb, eid, nb, scheme, vu, grav, dt, step0, nsteps, temps, s, vkin);
+  if (step_eligible(P)) {
+    // the one-launch step alternates blocks, so it runs an even number of
+    // steps and the state ends in the caller's block; the first step of an
+    // odd-length run uses the (in-place) per-phase launches
+    if (nsteps & 1) {
+      DMPM_TRY(tile_steps_cb<T>(P, M, pb, eid, nb, scheme, vu, grav, dt, step0, 1, temps, s,
+                                part, vkin));
+      ++step0;
+      --nsteps;
+    }
+    return nsteps ? step_run<T>(P, M, pb, eid, nb, scheme, vu, grav, dt, step0, nsteps, temps,
+                                s, vkin)
+                  : DMPM_OK;
+  }
+  return tile_steps_cb<T>(P, M, pb, eid, nb, scheme, vu, grav, dt, step0, nsteps, temps, s, part,
+                          vkin);
+}
+
+// tile_steps for the plan's cells per workgroup of the cell passes.
+template <typename T>
+int tile_steps_cb(dmpm_plan* P, const Mesh2<T>& M, T* pb, int32_t* eid, T* nb, int scheme,
+                  int vu, const double* grav, double dt, int64_t step0, int64_t nsteps, int temps,
+                  hipStream_t s, T* part, T* vkin) {
   switch (P->cell_block) {
     case 64:
       return tile_steps<T, 64>(P, M, pb, eid, nb, scheme, vu, grav, dt, step0, nsteps, temps, s,
