@@ -196,7 +196,8 @@ __global__ __launch_bounds__(kThreads) void k_adj_g2p(Mesh2<T> M, const T* __res
     sxi = sxi + xb[0];
     seta = seta + xb[1];
   }
-  for (int k = 0; k < 6; ++k) WR(W_L00 + k)[p] = Lb[k];
+  if (USL)  // USF: k_adj_force writes them
+    for (int k = 0; k < 6; ++k) WR(W_L00 + k)[p] = Lb[k];
   WR(W_XI)[p] = sxi;
   WR(W_ETA)[p] = seta;
 }
@@ -259,7 +260,6 @@ __global__ __launch_bounds__(kThreads) void k_adj_force(
                    Vb_in);
     CR(DMPM_C_VOL)[p] = Vb_in;
     for (int k = 0; k < 6; ++k) WR(W_L00 + k)[p] = Lb[k];
-    for (int k = 0; k < 4; ++k) WR(W_NVX + k)[p] = T(0);
     sxi = sxi + xb[0];
     seta = seta + xb[1];
   } else {
@@ -304,20 +304,27 @@ __global__ __launch_bounds__(kThreads) void k_adj_kin(Mesh2<T> M, const T* __res
 }
 
 // Cell pass of a cotangent scatter to the nodes: per corner k of a
-// particle's cell, 4 quantities
+// particle's cell, the quantities
 //   q0 = N_k w_nvx + dNk/dx L00b + dNk/dy L10b + dNk/dx(0) C00b
 //   q1 = N_k w_nvy + dNk/dy L11b + dNk/dx L01b + dNk/dy(0) C11b
 //   q2 = N_k w_nax,  q3 = N_k w_nay
-// summed per cell in slot order into partials[16][ncells] (node_gather then
-// adds the four cells of a node in the canonical order).
+// summed per cell in slot order into partials[NQ * 4][ncells] (node_gather
+// then adds the four cells of a node in the canonical order).  MODE 0 (USL)
+// forms all four; the USF step scatters in two passes whose other weights
+// are zero: MODE 1 the N-weighted terms (q0..q3 without the gradient terms),
+// MODE 2 the strain-rate terms (q0, q1 without the N terms; NQ = 2).  The
+// cell thread forms the natural coordinates from the staged positions with
+// its cell's centre and scale (natural's expressions, once per cell).
 constexpr int kSlotsA = 512;
-template <typename T, int CB>
+template <typename T, int CB, int MODE>
 __global__ __launch_bounds__(kThreads) void k_adj_cell(Mesh2<T> M, const T* __restrict__ pb,
                                                        int64_t ld, SortRef sr,
                                                        const T* __restrict__ wk,
                                                        T* __restrict__ part) {
-  constexpr int NIN = 12;  // xi, eta, 10 weights
-  __shared__ T st[NIN][kSlotsA];
+  constexpr int NW = MODE == 0 ? 10 : (MODE == 1 ? 4 : 6);  // staged weight rows
+  constexpr int W0 = MODE == 2 ? W_L00 : W_NVX;              // first staged row
+  constexpr int NQ = MODE == 2 ? 2 : 4;
+  __shared__ T st[2 + NW][kSlotsA];
   const int* __restrict__ perm = sr.perm();
   const int* __restrict__ cs = sr.cs();
   const int ident = sr.flags[1];
@@ -327,39 +334,58 @@ __global__ __launch_bounds__(kThreads) void k_adj_cell(Mesh2<T> M, const T* __re
   const bool celler = t < CB && mycell < cend;
   const int ms0 = celler ? cs[mycell] : 0, ms1 = celler ? cs[mycell + 1] : 0;
   const int s0 = cs[c0], s1 = cs[cend];
-  T acc[16];
-  for (int v = 0; v < 16; ++v) acc[v] = T(0);
+  // the cell's geometry: centre and 2/h per axis (= the inverse-Jacobian
+  // scales), the shape-function gradients at the centroid
+  T Xc = T(0), Yc = T(0), ix = T(0), iy = T(0), cx[4], cy[4];
+  if (celler) {
+    const int cs_ = M.sim_c(mycell);
+    const Grid2<T> g = M.grid(cs_);
+    const int lc = mycell - M.coff(cs_);
+    const int I = lc % g.nx, J = lc / g.nx;
+    const T X0 = g.X(I), X1 = g.X(I + 1), Y0 = g.Y(J), Y1 = g.Y(J + 1);
+    Xc = (X0 + X1) / T(2);
+    Yc = (Y0 + Y1) / T(2);
+    q4_invJ(T(0), T(0), X0, X1, Y0, Y1, ix, iy);
+    for (int k = 0; k < 4; ++k) q4_dNdx(k, T(0), T(0), ix, iy, cx[k], cy[k]);
+  }
+  T acc[4 * NQ];
+  for (int v = 0; v < 4 * NQ; ++v) acc[v] = T(0);
   for (int k0 = s0; k0 < s1; k0 += kSlotsA) {
     const int kend = min(k0 + kSlotsA, s1);
     for (int s = k0 + t; s < kend; s += kThreads) {
       const int p = ident ? s : perm[s];
       const int qq = s - k0;
-      const PGeo<T> q = pgeo(M, p, PR(DMPM_P_X)[p], PR(DMPM_P_X + 1)[p]);
-      st[0][qq] = q.xi;
-      st[1][qq] = q.eta;
-      for (int r = 0; r < 10; ++r) st[2 + r][qq] = WR(r)[p];
+      st[0][qq] = PR(DMPM_P_X)[p];
+      st[1][qq] = PR(DMPM_P_X + 1)[p];
+      for (int r = 0; r < NW; ++r) st[2 + r][qq] = WR(W0 + r)[p];
     }
     __syncthreads();
     if (celler) {
-      const int cs_ = M.sim_c(mycell);
-      const Grid2<T> g = M.grid(cs_);
-      const int lc = mycell - M.coff(cs_);
-      const int I = lc % g.nx, J = lc / g.nx;
-      const T X0 = g.X(I), X1 = g.X(I + 1), Y0 = g.Y(J), Y1 = g.Y(J + 1);
       const int lo = max(ms0, k0) - k0, hi = min(ms1, kend) - k0;
       for (int qq = lo; qq < hi; ++qq) {
-        const T xi = st[0][qq], eta = st[1][qq];
-        T ix, iy;
-        q4_invJ(xi, eta, X0, X1, Y0, Y1, ix, iy);
+        // natural(x, X0, X1) = (x - (X0 + X1)/2) * (2/(X1 - X0))
+        const T xi = (st[0][qq] - Xc) * ix, eta = (st[1][qq] - Yc) * iy;
         T N[4];
         q4_N4(xi, eta, N);
-#define WQ(r) st[2 + (r)][qq]
+#define WQ(r) st[2 + (r) - W0][qq]
+#pragma unroll
         for (int k = 0; k < 4; ++k) {
-          T dx, dy, cx, cy;
+          if (MODE == 1) {
+            acc[4 * k + 0] = acc[4 * k + 0] + N[k] * WQ(W_NVX);
+            acc[4 * k + 1] = acc[4 * k + 1] + N[k] * WQ(W_NVY);
+            acc[4 * k + 2] = acc[4 * k + 2] + N[k] * WQ(W_NAX);
+            acc[4 * k + 3] = acc[4 * k + 3] + N[k] * WQ(W_NAY);
+            continue;
+          }
+          T dx, dy;
           q4_dNdx(k, xi, eta, ix, iy, dx, dy);
-          q4_dNdx(k, T(0), T(0), ix, iy, cx, cy);
-          const T q0 = ((N[k] * WQ(W_NVX) + dx * WQ(W_L00)) + dy * WQ(W_L10)) + cx * WQ(W_C00);
-          const T q1 = ((N[k] * WQ(W_NVY) + dy * WQ(W_L11)) + dx * WQ(W_L01)) + cy * WQ(W_C11);
+          if (MODE == 2) {
+            acc[2 * k + 0] = acc[2 * k + 0] + ((dx * WQ(W_L00) + dy * WQ(W_L10)) + cx[k] * WQ(W_C00));
+            acc[2 * k + 1] = acc[2 * k + 1] + ((dy * WQ(W_L11) + dx * WQ(W_L01)) + cy[k] * WQ(W_C11));
+            continue;
+          }
+          const T q0 = ((N[k] * WQ(W_NVX) + dx * WQ(W_L00)) + dy * WQ(W_L10)) + cx[k] * WQ(W_C00);
+          const T q1 = ((N[k] * WQ(W_NVY) + dy * WQ(W_L11)) + dx * WQ(W_L01)) + cy[k] * WQ(W_C11);
           acc[4 * k + 0] = acc[4 * k + 0] + q0;
           acc[4 * k + 1] = acc[4 * k + 1] + q1;
           acc[4 * k + 2] = acc[4 * k + 2] + N[k] * WQ(W_NAX);
@@ -371,7 +397,7 @@ __global__ __launch_bounds__(kThreads) void k_adj_cell(Mesh2<T> M, const T* __re
     __syncthreads();
   }
   if (celler)
-    for (int v = 0; v < 16; ++v) part[(int64_t)v * M.ncells + mycell] = acc[v];
+    for (int v = 0; v < 4 * NQ; ++v) part[(int64_t)v * M.ncells + mycell] = acc[v];
 }
 
 // Kinematics adjoint at one node and direction (element.py:191-202 with
@@ -446,8 +472,8 @@ __global__ __launch_bounds__(kThreads) void k_adj_node2(Mesh2<T> M, const T* __r
   const Grid2<T> g = M.grid(sm);
   const int nl = nd - M.noff(sm);
   const int i = nl % (g.nx + 1), j = nl / (g.nx + 1);
-  T acc[4];
-  node_gather<T, 4>(g, M.coff(sm), M.ncells, part, i, j, acc);
+  T acc[2];
+  node_gather<T, 2>(g, M.coff(sm), M.ncells, part, i, j, acc);
   const int cr = cidx ? cidx[nd] : -1;
   const int mask = cr >= 0 ? cmask[cr] : 0;
   const T m = NR(DMPM_N_MASS)[nd];
@@ -518,7 +544,10 @@ int adjoint_kernels(dmpm_plan* P, const Mesh2<T>& M, const T* pb, const T* pw, c
   else
     k_adj_g2p<T, true><<<np, kThreads, 0, s>>>(M, pb, ld, n, nb, nld, cot, wk, lebar, mat, tdt,
                                                 vu);
-  k_adj_cell<T, CB><<<ncb, kThreads, 0, s>>>(M, pb, ld, sr, wk, part);
+  if (usf)
+    k_adj_cell<T, CB, 1><<<ncb, kThreads, 0, s>>>(M, pb, ld, sr, wk, part);
+  else
+    k_adj_cell<T, CB, 0><<<ncb, kThreads, 0, s>>>(M, pb, ld, sr, wk, part);
   if (usf)
     k_adj_node1<T, false><<<nn, kThreads, 0, s>>>(M, part, nb, nld, vkin, na, P->cidx, P->cmask,
                                                    tdt);
@@ -532,7 +561,7 @@ int adjoint_kernels(dmpm_plan* P, const Mesh2<T>& M, const T* pb, const T* pw, c
     k_adj_force<T, false><<<np, kThreads, 0, s>>>(M, pb, pw, ld, n, na, vkin, cot, wk, lebar,
                                                    mat, Gx, Gy, tdt);
   if (usf) {
-    k_adj_cell<T, CB><<<ncb, kThreads, 0, s>>>(M, pb, ld, sr, wk, part);
+    k_adj_cell<T, CB, 2><<<ncb, kThreads, 0, s>>>(M, pb, ld, sr, wk, part);
     k_adj_node2<T><<<nn, kThreads, 0, s>>>(M, part, nb, nld, vkin, na, P->cidx, P->cmask);
   }
   k_adj_kin<T><<<np, kThreads, 0, s>>>(M, pb, ld, n, na, cot, wk);
