@@ -147,20 +147,18 @@ __device__ __forceinline__ void stress_adjoint(const Mat2& mat, T E_p, T dt, T V
 // G2P adjoint (particle.py:208-247), particle-local part; USL also the
 // adjoint of the stress update made with the updated nodal velocities
 // (scheme.py:127-133).
+// G2P adjoint (particle.py:208-247) of particle p (inside the mesh, geometry
+// q), particle-local part: cotangent updates, the natural-coordinate
+// cotangent (rows W_XI, W_ETA of wk) and the weights w[W_NVX .. W_C11] the
+// scatter to the nodes needs; USL also the adjoint of the stress update made
+// with the updated nodal velocities (scheme.py:127-133).
 template <typename T, bool USL>
-__global__ __launch_bounds__(kThreads) void k_adj_g2p(Mesh2<T> M, const T* __restrict__ pb,
-                                                      int64_t ld, int n, const T* __restrict__ nb,
-                                                      int64_t nld, T* __restrict__ cot,
-                                                      T* __restrict__ wk, T* __restrict__ lebar,
-                                                      Mat2 mat, T dt, int vu) {
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= n) return;
-  const PGeo<T> q = pgeo(M, p, PR(DMPM_P_X)[p], PR(DMPM_P_X + 1)[p]);
-  if (!q.in) {
-    for (int r = 0; r < DMPM_C_ROWS; ++r) CR(r)[p] = T(0);
-    for (int r = 0; r < W_ROWS; ++r) WR(r)[p] = T(0);
-    return;
-  }
+__device__ __forceinline__ void adj_g2p_particle(const Mesh2<T>& M, const T* __restrict__ pb,
+                                                 int64_t ld, const T* __restrict__ nb,
+                                                 int64_t nld, T* __restrict__ cot,
+                                                 T* __restrict__ wk, T* __restrict__ lebar,
+                                                 const Mat2& mat, T dt, int vu, int p,
+                                                 const PGeo<T>& q, T w[10]) {
   const int* ids = q.ids;
   const T xbx = CR(DMPM_C_X)[p], xby = CR(DMPM_C_X + 1)[p];
   const T vbx = CR(DMPM_C_V)[p], vby = CR(DMPM_C_V + 1)[p];
@@ -177,10 +175,10 @@ __global__ __launch_bounds__(kThreads) void k_adj_g2p(Mesh2<T> M, const T* __res
     sxi = sxi + gx * s;
     seta = seta + gy * s;
   }
-  WR(W_NVX)[p] = vu ? dt * xbx + vbx : dt * xbx;
-  WR(W_NVY)[p] = vu ? dt * xby + vby : dt * xby;
-  WR(W_NAX)[p] = vu ? T(0) : dt * vbx;
-  WR(W_NAY)[p] = vu ? T(0) : dt * vby;
+  w[W_NVX] = vu ? dt * xbx + vbx : dt * xbx;
+  w[W_NVY] = vu ? dt * xby + vby : dt * xby;
+  w[W_NAX] = vu ? T(0) : dt * vbx;
+  w[W_NAY] = vu ? T(0) : dt * vby;
   if (vu) {
     CR(DMPM_C_V)[p] = T(0);
     CR(DMPM_C_V + 1)[p] = T(0);
@@ -196,8 +194,7 @@ __global__ __launch_bounds__(kThreads) void k_adj_g2p(Mesh2<T> M, const T* __res
     sxi = sxi + xb[0];
     seta = seta + xb[1];
   }
-  if (USL)  // USF: k_adj_force writes them
-    for (int k = 0; k < 6; ++k) WR(W_L00 + k)[p] = Lb[k];
+  for (int k = 0; k < 6; ++k) w[W_L00 + k] = Lb[k];
   WR(W_XI)[p] = sxi;
   WR(W_ETA)[p] = seta;
 }
@@ -205,15 +202,16 @@ __global__ __launch_bounds__(kThreads) void k_adj_g2p(Mesh2<T> M, const T* __res
 // Force P2G adjoint (element.py:236-268, 842-901): gathers the nodal force
 // cotangent; USF then the adjoint of the stress update made with the
 // kinematic nodal velocities.
+// Force P2G adjoint (element.py:236-268, 842-901) of particle p (inside the
+// mesh, geometry q): gathers the nodal force cotangent; USF then the adjoint
+// of the stress update made with the kinematic nodal velocities, whose
+// strain-rate cotangents (L00, L11, L01, L10, C00, C11) it returns in w.
 template <typename T, bool USF>
-__global__ __launch_bounds__(kThreads) void k_adj_force(
-    Mesh2<T> M, const T* __restrict__ pb, const T* __restrict__ pw, int64_t ld, int n,
+__device__ __forceinline__ void adj_force_particle(
+    const Mesh2<T>& M, const T* __restrict__ pb, const T* __restrict__ pw, int64_t ld,
     const T* __restrict__ na, const T* __restrict__ vkin, T* __restrict__ cot,
-    T* __restrict__ wk, T* __restrict__ lebar, Mat2 mat, T Gx, T Gy, T dt) {
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= n) return;
-  const PGeo<T> q = pgeo(M, p, PR(DMPM_P_X)[p], PR(DMPM_P_X + 1)[p]);
-  if (!q.in) return;
+    T* __restrict__ wk, T* __restrict__ lebar, const Mat2& mat, T Gx, T Gy, T dt, int p,
+    const PGeo<T>& q, T w[6]) {
   const int* ids = q.ids;
   // the stress and volume the force P2G used: after the USF update, or the
   // step-start ones (USL)
@@ -259,7 +257,7 @@ __global__ __launch_bounds__(kThreads) void k_adj_force(
                    PR(DMPM_P_VOLUME)[p], q, vx, vy, sb, Vbt, lebar ? lebar + p : &dummy, Lb, xb,
                    Vb_in);
     CR(DMPM_C_VOL)[p] = Vb_in;
-    for (int k = 0; k < 6; ++k) WR(W_L00 + k)[p] = Lb[k];
+    for (int k = 0; k < 6; ++k) w[k] = Lb[k];
     sxi = sxi + xb[0];
     seta = seta + xb[1];
   } else {
@@ -267,6 +265,21 @@ __global__ __launch_bounds__(kThreads) void k_adj_force(
   }
   WR(W_XI)[p] = WR(W_XI)[p] + sxi;
   WR(W_ETA)[p] = WR(W_ETA)[p] + seta;
+}
+
+// USL: the force adjoint of every particle in the mesh.
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_adj_force(
+    Mesh2<T> M, const T* __restrict__ pb, int64_t ld, int n, const T* __restrict__ na,
+    const T* __restrict__ vkin, T* __restrict__ cot, T* __restrict__ wk, T* __restrict__ lebar,
+    Mat2 mat, T Gx, T Gy, T dt) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const PGeo<T> q = pgeo(M, p, PR(DMPM_P_X)[p], PR(DMPM_P_X + 1)[p]);
+  if (!q.in) return;
+  T w[6];
+  adj_force_particle<T, false>(M, pb, pb, ld, na, vkin, cot, wk, lebar, mat, Gx, Gy, dt, p, q,
+                               w);
 }
 
 // P2G of mass and momentum adjoint (element.py:122-189): gathers the nodal
@@ -303,49 +316,72 @@ __global__ __launch_bounds__(kThreads) void k_adj_kin(Mesh2<T> M, const T* __res
   CR(DMPM_C_X + 1)[p] = CR(DMPM_C_X + 1)[p] + seta * q.iy;
 }
 
-// Cell pass of a cotangent scatter to the nodes: per corner k of a
-// particle's cell, the quantities
+// Particle pass + cell pass of a cotangent scatter to the nodes.  The slot
+// threads run a particle-local adjoint for the particles the sort put in the
+// workgroup's cells (MODE 0, 1: G2P, adj_g2p_particle; MODE 2: force,
+// adj_force_particle) and stage their natural coordinates and weights in
+// LDS; then per corner k of a particle's cell the quantities
 //   q0 = N_k w_nvx + dNk/dx L00b + dNk/dy L10b + dNk/dx(0) C00b
 //   q1 = N_k w_nvy + dNk/dy L11b + dNk/dx L01b + dNk/dy(0) C11b
 //   q2 = N_k w_nax,  q3 = N_k w_nay
-// summed per cell in slot order into partials[NQ * 4][ncells] (node_gather
-// then adds the four cells of a node in the canonical order).  MODE 0 (USL)
-// forms all four; the USF step scatters in two passes whose other weights
-// are zero: MODE 1 the N-weighted terms (q0..q3 without the gradient terms),
-// MODE 2 the strain-rate terms (q0, q1 without the N terms; NQ = 2).  The
-// cell thread forms the natural coordinates from the staged positions with
-// its cell's centre and scale (natural's expressions, once per cell).
+// are summed per cell in slot order into partials[NQ * 4][ncells]
+// (node_gather then adds the four cells of a node in the canonical order).
+// MODE 0 (USL) forms all four; the USF step scatters in two passes whose
+// other weights are zero: MODE 1 the N-weighted terms, MODE 2 the
+// strain-rate terms (q0, q1 only; NQ = 2).  Workgroups from ncb on (MODE 0,
+// 1) clear the cotangents of the particles outside the mesh.
 constexpr int kSlotsA = 512;
+template <typename T> struct AdjArgs {
+  const T* pb;
+  const T* pw;
+  int64_t ld;
+  const T* nb;
+  int64_t nld;
+  const T* na;
+  const T* vkin;
+  T* cot;
+  T* wk;
+  T* lebar;
+  Mat2 mat;
+  T Gx, Gy, dt;
+  int vu, n, ncb;
+};
 template <typename T, int CB, int MODE>
-__global__ __launch_bounds__(kThreads) void k_adj_cell(Mesh2<T> M, const T* __restrict__ pb,
-                                                       int64_t ld, SortRef sr,
-                                                       const T* __restrict__ wk,
-                                                       T* __restrict__ part) {
-  constexpr int NW = MODE == 0 ? 10 : (MODE == 1 ? 4 : 6);  // staged weight rows
-  constexpr int W0 = MODE == 2 ? W_L00 : W_NVX;              // first staged row
+__global__ __launch_bounds__(kThreads) void k_adj_scatter(Mesh2<T> M, AdjArgs<T> a, SortRef sr,
+                                                          T* __restrict__ part) {
+  constexpr int NW = MODE == 0 ? 10 : (MODE == 1 ? 4 : 6);  // staged weights
+  constexpr int W0 = MODE == 2 ? W_L00 : W_NVX;              // first staged weight
   constexpr int NQ = MODE == 2 ? 2 : 4;
   __shared__ T st[2 + NW][kSlotsA];
   const int* __restrict__ perm = sr.perm();
   const int* __restrict__ cs = sr.cs();
   const int ident = sr.flags[1];
+  const T* __restrict__ pb = a.pb;
+  const int64_t ld = a.ld;
+  T* __restrict__ cot = a.cot;
+  if ((int)blockIdx.x >= a.ncb) {
+    for (int s = cs[M.ncells] + ((int)blockIdx.x - a.ncb) * kThreads + threadIdx.x; s < a.n;
+         s += (gridDim.x - a.ncb) * kThreads) {
+      const int p = ident ? s : perm[s];
+      for (int r = 0; r < DMPM_C_ROWS; ++r) CR(r)[p] = T(0);
+    }
+    return;
+  }
   const int c0 = blockIdx.x * CB, t = threadIdx.x;
   const int cend = min(c0 + CB, M.ncells);
   const int mycell = c0 + t;
   const bool celler = t < CB && mycell < cend;
   const int ms0 = celler ? cs[mycell] : 0, ms1 = celler ? cs[mycell + 1] : 0;
   const int s0 = cs[c0], s1 = cs[cend];
-  // the cell's geometry: centre and 2/h per axis (= the inverse-Jacobian
-  // scales), the shape-function gradients at the centroid
-  T Xc = T(0), Yc = T(0), ix = T(0), iy = T(0), cx[4], cy[4];
+  // the cell's inverse-Jacobian scales and shape-function gradients at the
+  // centroid
+  T ix = T(0), iy = T(0), cx[4], cy[4];
   if (celler) {
     const int cs_ = M.sim_c(mycell);
     const Grid2<T> g = M.grid(cs_);
     const int lc = mycell - M.coff(cs_);
     const int I = lc % g.nx, J = lc / g.nx;
-    const T X0 = g.X(I), X1 = g.X(I + 1), Y0 = g.Y(J), Y1 = g.Y(J + 1);
-    Xc = (X0 + X1) / T(2);
-    Yc = (Y0 + Y1) / T(2);
-    q4_invJ(T(0), T(0), X0, X1, Y0, Y1, ix, iy);
+    q4_invJ(T(0), T(0), g.X(I), g.X(I + 1), g.Y(J), g.Y(J + 1), ix, iy);
     for (int k = 0; k < 4; ++k) q4_dNdx(k, T(0), T(0), ix, iy, cx[k], cy[k]);
   }
   T acc[4 * NQ];
@@ -355,16 +391,26 @@ __global__ __launch_bounds__(kThreads) void k_adj_cell(Mesh2<T> M, const T* __re
     for (int s = k0 + t; s < kend; s += kThreads) {
       const int p = ident ? s : perm[s];
       const int qq = s - k0;
-      st[0][qq] = PR(DMPM_P_X)[p];
-      st[1][qq] = PR(DMPM_P_X + 1)[p];
-      for (int r = 0; r < NW; ++r) st[2 + r][qq] = WR(W0 + r)[p];
+      const PGeo<T> q = pgeo(M, p, PR(DMPM_P_X)[p], PR(DMPM_P_X + 1)[p]);
+      st[0][qq] = q.xi;
+      st[1][qq] = q.eta;
+      if (MODE == 2) {
+        T w[6];
+        adj_force_particle<T, true>(M, pb, a.pw, ld, a.na, a.vkin, cot, a.wk, a.lebar, a.mat,
+                                    a.Gx, a.Gy, a.dt, p, q, w);
+        for (int r = 0; r < 6; ++r) st[2 + r][qq] = w[r];
+      } else {
+        T w[10];
+        adj_g2p_particle<T, MODE == 0>(M, pb, ld, a.nb, a.nld, cot, a.wk, a.lebar, a.mat, a.dt,
+                                       a.vu, p, q, w);
+        for (int r = 0; r < NW; ++r) st[2 + r][qq] = w[W0 + r];
+      }
     }
     __syncthreads();
     if (celler) {
       const int lo = max(ms0, k0) - k0, hi = min(ms1, kend) - k0;
       for (int qq = lo; qq < hi; ++qq) {
-        // natural(x, X0, X1) = (x - (X0 + X1)/2) * (2/(X1 - X0))
-        const T xi = (st[0][qq] - Xc) * ix, eta = (st[1][qq] - Yc) * iy;
+        const T xi = st[0][qq], eta = st[1][qq];
         T N[4];
         q4_N4(xi, eta, N);
 #define WQ(r) st[2 + (r) - W0][qq]
@@ -380,8 +426,10 @@ __global__ __launch_bounds__(kThreads) void k_adj_cell(Mesh2<T> M, const T* __re
           T dx, dy;
           q4_dNdx(k, xi, eta, ix, iy, dx, dy);
           if (MODE == 2) {
-            acc[2 * k + 0] = acc[2 * k + 0] + ((dx * WQ(W_L00) + dy * WQ(W_L10)) + cx[k] * WQ(W_C00));
-            acc[2 * k + 1] = acc[2 * k + 1] + ((dy * WQ(W_L11) + dx * WQ(W_L01)) + cy[k] * WQ(W_C11));
+            acc[2 * k + 0] =
+                acc[2 * k + 0] + ((dx * WQ(W_L00) + dy * WQ(W_L10)) + cx[k] * WQ(W_C00));
+            acc[2 * k + 1] =
+                acc[2 * k + 1] + ((dy * WQ(W_L11) + dx * WQ(W_L01)) + cy[k] * WQ(W_C11));
             continue;
           }
           const T q0 = ((N[k] * WQ(W_NVX) + dx * WQ(W_L00)) + dy * WQ(W_L10)) + cx[k] * WQ(W_C00);
@@ -538,31 +586,41 @@ int adjoint_kernels(dmpm_plan* P, const Mesh2<T>& M, const T* pb, const T* pw, c
   const T tdt = T(dt), Gx = T(grav[0]), Gy = T(grav[1]);
   const unsigned np = nblk(n), nn = nblk(P->nnodes);
   const unsigned ncb = (unsigned)((M.ncells + CB - 1) / CB);
+  AdjArgs<T> aa;
+  aa.pb = pb;
+  aa.pw = pw;
+  aa.ld = ld;
+  aa.nb = nb;
+  aa.nld = nld;
+  aa.na = na;
+  aa.vkin = vkin;
+  aa.cot = cot;
+  aa.wk = wk;
+  aa.lebar = lebar;
+  aa.mat = mat;
+  aa.Gx = Gx;
+  aa.Gy = Gy;
+  aa.dt = tdt;
+  aa.vu = vu;
+  aa.n = n;
+  aa.ncb = (int)ncb;
+  constexpr int kOutsideBlocks = 8;
   if (usf)
-    k_adj_g2p<T, false><<<np, kThreads, 0, s>>>(M, pb, ld, n, nb, nld, cot, wk, lebar, mat, tdt,
-                                                 vu);
+    k_adj_scatter<T, CB, 1><<<ncb + kOutsideBlocks, kThreads, 0, s>>>(M, aa, sr, part);
   else
-    k_adj_g2p<T, true><<<np, kThreads, 0, s>>>(M, pb, ld, n, nb, nld, cot, wk, lebar, mat, tdt,
-                                                vu);
-  if (usf)
-    k_adj_cell<T, CB, 1><<<ncb, kThreads, 0, s>>>(M, pb, ld, sr, wk, part);
-  else
-    k_adj_cell<T, CB, 0><<<ncb, kThreads, 0, s>>>(M, pb, ld, sr, wk, part);
+    k_adj_scatter<T, CB, 0><<<ncb + kOutsideBlocks, kThreads, 0, s>>>(M, aa, sr, part);
   if (usf)
     k_adj_node1<T, false><<<nn, kThreads, 0, s>>>(M, part, nb, nld, vkin, na, P->cidx, P->cmask,
                                                    tdt);
   else
     k_adj_node1<T, true><<<nn, kThreads, 0, s>>>(M, part, nb, nld, vkin, na, P->cidx, P->cmask,
                                                   tdt);
-  if (usf)
-    k_adj_force<T, true><<<np, kThreads, 0, s>>>(M, pb, pw, ld, n, na, vkin, cot, wk, lebar, mat,
-                                                  Gx, Gy, tdt);
-  else
-    k_adj_force<T, false><<<np, kThreads, 0, s>>>(M, pb, pw, ld, n, na, vkin, cot, wk, lebar,
-                                                   mat, Gx, Gy, tdt);
   if (usf) {
-    k_adj_cell<T, CB, 2><<<ncb, kThreads, 0, s>>>(M, pb, ld, sr, wk, part);
+    k_adj_scatter<T, CB, 2><<<ncb, kThreads, 0, s>>>(M, aa, sr, part);
     k_adj_node2<T><<<nn, kThreads, 0, s>>>(M, part, nb, nld, vkin, na, P->cidx, P->cmask);
+  } else {
+    k_adj_force<T><<<np, kThreads, 0, s>>>(M, pb, ld, n, na, vkin, cot, wk, lebar, mat, Gx, Gy,
+                                          tdt);
   }
   k_adj_kin<T><<<np, kThreads, 0, s>>>(M, pb, ld, n, na, cot, wk);
   return launch_check("step_adjoint");
