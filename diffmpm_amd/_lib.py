@@ -91,11 +91,15 @@ SIGNATURES = {
     "dmpm_timing_read": [_P, _DP, POINTER(c_int64)],
     "dmpm_timing_read_class": [_P, c_int32, _DP, POINTER(c_int64)],
     "dmpm_run": [_P, _P, _P, _P, c_int32, c_int32, _DP, c_double, c_int64, c_int64, c_int32, _P],
+    "dmpm_step_between": [_P, _P, _P, _P, _P, c_int32, c_int32, _DP, c_double, c_int64, c_int32,
+                          _P, c_int64, _P],
     "dmpm_step_adjoint": [_P, _P, _P, _P, _P, _P, _P, c_int32, c_int32, _DP, c_double, c_int64,
                           _P],
     "dmpm_plan_export_step": [_P, _P, _P, _P, _P],
     "dmpm_step_adjoint_saved": [_P, _P, _P, _P, _P, _P, _P, _P, _P, c_int32, c_int32, _DP,
                                 c_double, _P],
+    "dmpm_step_adjoint_saved_state": [_P, _P, _P, _P, c_int64, _P, _P, _P, _P, _P, _P, c_int32,
+                                      c_int32, _DP, c_double, _P],
 }
 
 _lib = None

@@ -56,6 +56,7 @@ class Differentiable:
         from .ensemble import BatchPlan
         d = cls.__new__(cls)
         d.sim, d.ps, d.el = None, None, None
+        d.bplan, d._blocks = None, None
         d.per_particle_E = all(s.mesh.particles[0]._has_youngs for s in ens.sims)
         d.n = ens.n
         d._pbuf, d._nbuf, d._eid_f = ens.pbuf, ens.nbuf, ens.eid
@@ -89,6 +90,11 @@ class Differentiable:
         self._pbuf, self._nbuf, self._eid_f = ps._buf, el.nodes._buf, ps._eid
         self.fplan = plan_for(el, ps)
         self.aplan = Plan(el, ps)  # own sort state for the recomputed steps
+        # forward steps alternate between two private blocks (one-launch step
+        # from one to the other, no copy of the state); own sort state
+        self.bplan = Plan(el, ps)
+        self._blocks = None
+        self._eid_b = self._eid_f.clone()
         self._finish(sim.scheme, sim.velocity_update, sim.dt)
 
     def _finish(self, scheme, velocity_update, dt):
@@ -134,12 +140,38 @@ class Differentiable:
     # -- one step ---------------------------------------------------------------
     def forward_step(self, state, log_e, gravity, i, save=False):
         """One forward step; with save=True also returns what its adjoint
-        needs (node block, kinematic nodal velocity, sort by cell)."""
-        self.set_state(state, log_e)
-        self.fplan.run(self.scheme, self.velocity_update, gravity, self.dt, i, 1,
-                       keep_temporaries=False)
-        self._last = None  # the block now holds the output
-        out = self.get_state()
+        needs (node block, kinematic nodal velocity, sort by cell).  A single
+        simulation steps between two private particle blocks (the
+        simulation's own block is not written); an ensemble steps its batch
+        block in place."""
+        if self.bplan is None:
+            self.set_state(state, log_e)
+            self.fplan.run(self.scheme, self.velocity_update, gravity, self.dt, i, 1,
+                           keep_temporaries=False)
+            self._last = None  # the block now holds the output
+            out = self.get_state()
+            plan = self.fplan
+        else:
+            if self._blocks is None:
+                self._blocks = (self._pbuf.clone(), self._pbuf.clone())
+            last = getattr(self, "_last_b", None)
+            chained = last is not None and last[0] is state and state._version == last[1]
+            if chained:
+                src = last[2]  # the block holds this state, as the plan's last step left it
+            else:
+                src = 0
+                self._write(self._blocks[0], state.detach(), None)
+            dst = 1 - src
+            if log_e is not None:
+                self._write_log_e(self._blocks[dst], log_e)
+            plan = self.bplan
+            plan.refresh_tables()
+            out = torch.empty((C_ROWS, self.n), dtype=self._pbuf.dtype,
+                              device=self._pbuf.device)
+            call("dmpm_step_between", plan.h, ptr(self._blocks[src]), ptr(self._blocks[dst]),
+                 ptr(self._eid_b), ptr(self._nbuf), self.code, self.vu, plan._gravity(gravity),
+                 self.dt, int(i), 2 if chained else 0, ptr(out), self.n, plan.s)
+            self._last_b = (out, out._version, dst)
         if not save:
             return out, None
         nsaved = self._nbuf.clone()
@@ -147,7 +179,7 @@ class Differentiable:
                            device=self._nbuf.device)
         perm = torch.empty(max(self.n, 1), dtype=torch.int32, device=self._nbuf.device)
         cs = torch.empty(self._ncells() + 2, dtype=torch.int32, device=self._nbuf.device)
-        call("dmpm_plan_export_step", self.fplan.h, ptr(vkin), ptr(perm), ptr(cs), self.fplan.s)
+        call("dmpm_plan_export_step", plan.h, ptr(vkin), ptr(perm), ptr(cs), plan.s)
         return out, (nsaved, vkin, perm, cs)
 
     def _ncells(self):
@@ -173,14 +205,24 @@ class Differentiable:
         """Like adjoint_step, from the quantities forward_step(save=True)
         returned (no recomputation of the step)."""
         nsaved, vkin, perm, cs = saved
-        self._write(self.pin, state.detach(), log_e)
-        if self.code == _lib.DMPM_USF:
-            # the kernels read the stress and volume after the step (USF only)
-            for a, b in _RUNS[1:]:
-                self.pwork[b, : self.n] = out.detach()[a]
+        st_in, st_out = state.detach(), out.detach()
         ld = self._pbuf.shape[1]
         cot = self._cot_buffer(gout)
         lebar = torch.zeros(ld, dtype=self._pbuf.dtype, device=self._pbuf.device)
+        if st_in.is_contiguous() and st_out.is_contiguous():
+            # the kernels read the state tensors; mass and E from pin
+            if log_e is not None:
+                self._write_log_e(self.pin, log_e)
+            call("dmpm_step_adjoint_saved_state", self.fplan.h, ptr(self.pin), ptr(st_in),
+                 ptr(st_out), self.n, ptr(nsaved), ptr(vkin), ptr(perm), ptr(cs), ptr(cot),
+                 ptr(lebar), self.code, self.vu, self.fplan._gravity(gravity), self.dt,
+                 self.fplan.s)
+            return cot[:, : self.n], lebar[: self.n]
+        self._write(self.pin, st_in, log_e)
+        if self.code == _lib.DMPM_USF:
+            # the kernels read the stress and volume after the step (USF only)
+            for a, b in _RUNS[1:]:
+                self.pwork[b, : self.n] = st_out[a]
         call("dmpm_step_adjoint_saved", self.fplan.h, ptr(self.pin), ptr(self.pwork),
              ptr(nsaved), ptr(vkin), ptr(perm), ptr(cs), ptr(cot), ptr(lebar), self.code, self.vu,
              self.fplan._gravity(gravity), self.dt, self.fplan.s)
@@ -238,8 +280,10 @@ def rollout(d, nsteps, gravity, state=None, log_e=None, start_step=0, checkpoint
     """Differentiable rollout of ``nsteps`` steps from ``state`` (default: the
     current state of the simulation).  Returns the final (8, N) state.  With
     ``checkpoint_every=K`` only segment-start states are kept for backward.
-    After a backward pass the simulation's own particle block holds a
-    recomputed intermediate state; use the returned tensors."""
+    A single simulation's rollout does not write the simulation's own
+    particle block (its steps alternate between two private blocks); after a
+    backward pass an ensemble's batch block holds a recomputed intermediate
+    state.  Use the returned tensors."""
     if state is None:
         state = d.get_state()
 

@@ -55,6 +55,72 @@ enum { W_NVX = 0, W_NVY, W_NAX, W_NAY, W_L00, W_L11, W_L01, W_L10, W_C00, W_C11,
 // rows of the nodal adjoint workspace
 enum { A_FX = 0, A_FY, A_M, A_VKX, A_VKY, A_PX, A_PY, A_ROWS };
 
+// The particle rows the adjoint reads (element p at [p]): the step's input
+// (from a particle block, row r at pb + r*ld, or from a state tensor plus a
+// block of the constant rows) and the stress and volume after the step.
+template <typename T> struct PRows {
+  const T* x;
+  const T* y;
+  const T* vx;
+  const T* vy;
+  const T* m;
+  const T* V;
+  const T* sxx;
+  const T* syy;
+  const T* sxy;
+  const T* E;
+};
+template <typename T> struct POut {
+  const T* V;
+  const T* sxx;
+  const T* syy;
+  const T* sxy;
+};
+template <typename T> PRows<T> rows_of_block(const T* pb, int64_t ld) {
+  PRows<T> r;
+  r.x = pb + (int64_t)DMPM_P_X * ld;
+  r.y = pb + (int64_t)(DMPM_P_X + 1) * ld;
+  r.vx = pb + (int64_t)DMPM_P_V * ld;
+  r.vy = pb + (int64_t)(DMPM_P_V + 1) * ld;
+  r.m = pb + (int64_t)DMPM_P_MASS * ld;
+  r.V = pb + (int64_t)DMPM_P_VOLUME * ld;
+  r.sxx = pb + (int64_t)DMPM_P_STRESS * ld;
+  r.syy = pb + (int64_t)(DMPM_P_STRESS + 1) * ld;
+  r.sxy = pb + (int64_t)(DMPM_P_STRESS + 3) * ld;
+  r.E = pb + (int64_t)DMPM_P_YOUNGS * ld;
+  return r;
+}
+template <typename T> POut<T> out_of_block(const T* pw, int64_t ld) {
+  POut<T> o;
+  o.V = pw + (int64_t)DMPM_P_VOLUME * ld;
+  o.sxx = pw + (int64_t)DMPM_P_STRESS * ld;
+  o.syy = pw + (int64_t)(DMPM_P_STRESS + 1) * ld;
+  o.sxy = pw + (int64_t)(DMPM_P_STRESS + 3) * ld;
+  return o;
+}
+// state tensor rows x, y, vx, vy, sxx, syy, sxy, V (row q at st + q*sld)
+template <typename T> PRows<T> rows_of_state(const T* st, int64_t sld, const T* pconst,
+                                             int64_t ld) {
+  PRows<T> r = rows_of_block(pconst, ld);
+  r.x = st;
+  r.y = st + sld;
+  r.vx = st + 2 * sld;
+  r.vy = st + 3 * sld;
+  r.sxx = st + 4 * sld;
+  r.syy = st + 5 * sld;
+  r.sxy = st + 6 * sld;
+  r.V = st + 7 * sld;
+  return r;
+}
+template <typename T> POut<T> out_of_state(const T* st, int64_t sld) {
+  POut<T> o;
+  o.sxx = st + 4 * sld;
+  o.syy = st + 5 * sld;
+  o.sxy = st + 6 * sld;
+  o.V = st + 7 * sld;
+  return o;
+}
+
 // Per-particle geometry at the step-start position: simulation, cell of its
 // own mesh, natural coordinates, inverse-Jacobian scales, corner nodes.
 template <typename T> struct PGeo {
@@ -153,7 +219,7 @@ __device__ __forceinline__ void stress_adjoint(const Mat2& mat, T E_p, T dt, T V
 // scatter to the nodes needs; USL also the adjoint of the stress update made
 // with the updated nodal velocities (scheme.py:127-133).
 template <typename T, bool USL>
-__device__ __forceinline__ void adj_g2p_particle(const Mesh2<T>& M, const T* __restrict__ pb,
+__device__ __forceinline__ void adj_g2p_particle(const Mesh2<T>& M, const PRows<T>& R,
                                                  int64_t ld, const T* __restrict__ nb,
                                                  int64_t nld, T* __restrict__ cot,
                                                  T* __restrict__ wk, T* __restrict__ lebar,
@@ -187,8 +253,8 @@ __device__ __forceinline__ void adj_g2p_particle(const Mesh2<T>& M, const T* __r
   if (USL) {
     const T sb[3] = {CR(DMPM_C_SXX)[p], CR(DMPM_C_SYY)[p], CR(DMPM_C_SXY)[p]};
     T xb[2], Vb_in, unused = T(0);
-    stress_adjoint(M.mat(mat, q.s), mat.per_E ? PR(DMPM_P_YOUNGS)[p] : T(0), dt,
-                   PR(DMPM_P_VOLUME)[p], q, vx, vy, sb, CR(DMPM_C_VOL)[p],
+    stress_adjoint(M.mat(mat, q.s), mat.per_E ? R.E[p] : T(0), dt,
+                   R.V[p], q, vx, vy, sb, CR(DMPM_C_VOL)[p],
                    lebar ? lebar + p : &unused, Lb, xb, Vb_in);
     CR(DMPM_C_VOL)[p] = Vb_in;
     sxi = sxi + xb[0];
@@ -208,19 +274,18 @@ __device__ __forceinline__ void adj_g2p_particle(const Mesh2<T>& M, const T* __r
 // strain-rate cotangents (L00, L11, L01, L10, C00, C11) it returns in w.
 template <typename T, bool USF>
 __device__ __forceinline__ void adj_force_particle(
-    const Mesh2<T>& M, const T* __restrict__ pb, const T* __restrict__ pw, int64_t ld,
+    const Mesh2<T>& M, const PRows<T>& R, const POut<T>& O, int64_t ld,
     const T* __restrict__ na, const T* __restrict__ vkin, T* __restrict__ cot,
     T* __restrict__ wk, T* __restrict__ lebar, const Mat2& mat, T Gx, T Gy, T dt, int p,
     const PGeo<T>& q, T w[6]) {
   const int* ids = q.ids;
   // the stress and volume the force P2G used: after the USF update, or the
   // step-start ones (USL)
-  const T* __restrict__ src = USF ? pw : pb;
-  const T V = src[(int64_t)DMPM_P_VOLUME * ld + p];
-  const T sxx = src[(int64_t)(DMPM_P_STRESS + 0) * ld + p];
-  const T syy = src[(int64_t)(DMPM_P_STRESS + 1) * ld + p];
-  const T sxy = src[(int64_t)(DMPM_P_STRESS + 3) * ld + p];
-  const T m = PR(DMPM_P_MASS)[p];
+  const T V = USF ? O.V[p] : R.V[p];
+  const T sxx = USF ? O.sxx[p] : R.sxx[p];
+  const T syy = USF ? O.syy[p] : R.syy[p];
+  const T sxy = USF ? O.sxy[p] : R.sxy[p];
+  const T m = R.m[p];
   T Vb = T(0), sbxx = T(0), sbyy = T(0), sbxy = T(0), sxi = T(0), seta = T(0);
   for (int k = 0; k < 4; ++k) {
     const T fx = AR(A_FX)[ids[k]], fy = AR(A_FY)[ids[k]];
@@ -253,8 +318,8 @@ __device__ __forceinline__ void adj_force_particle(
     }
     T Lb[6], xb[2], Vb_in;
     T dummy = T(0);
-    stress_adjoint(M.mat(mat, q.s), mat.per_E ? PR(DMPM_P_YOUNGS)[p] : T(0), dt,
-                   PR(DMPM_P_VOLUME)[p], q, vx, vy, sb, Vbt, lebar ? lebar + p : &dummy, Lb, xb,
+    stress_adjoint(M.mat(mat, q.s), mat.per_E ? R.E[p] : T(0), dt,
+                   R.V[p], q, vx, vy, sb, Vbt, lebar ? lebar + p : &dummy, Lb, xb,
                    Vb_in);
     CR(DMPM_C_VOL)[p] = Vb_in;
     for (int k = 0; k < 6; ++k) w[k] = Lb[k];
@@ -270,32 +335,36 @@ __device__ __forceinline__ void adj_force_particle(
 // USL: the force adjoint of every particle in the mesh.
 template <typename T>
 __global__ __launch_bounds__(kThreads) void k_adj_force(
-    Mesh2<T> M, const T* __restrict__ pb, int64_t ld, int n, const T* __restrict__ na,
+    Mesh2<T> M, PRows<T> R, int64_t ld, int n, const T* __restrict__ na,
     const T* __restrict__ vkin, T* __restrict__ cot, T* __restrict__ wk, T* __restrict__ lebar,
     Mat2 mat, T Gx, T Gy, T dt) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= n) return;
-  const PGeo<T> q = pgeo(M, p, PR(DMPM_P_X)[p], PR(DMPM_P_X + 1)[p]);
+  const PGeo<T> q = pgeo(M, p, R.x[p], R.y[p]);
   if (!q.in) return;
   T w[6];
-  adj_force_particle<T, false>(M, pb, pb, ld, na, vkin, cot, wk, lebar, mat, Gx, Gy, dt, p, q,
-                               w);
+  POut<T> O;
+  O.V = R.V;
+  O.sxx = R.sxx;
+  O.syy = R.syy;
+  O.sxy = R.sxy;
+  adj_force_particle<T, false>(M, R, O, ld, na, vkin, cot, wk, lebar, mat, Gx, Gy, dt, p, q, w);
 }
 
 // P2G of mass and momentum adjoint (element.py:122-189): gathers the nodal
 // momentum and mass cotangents; the natural-coordinate cotangent goes to x.
 template <typename T>
-__global__ __launch_bounds__(kThreads) void k_adj_kin(Mesh2<T> M, const T* __restrict__ pb,
+__global__ __launch_bounds__(kThreads) void k_adj_kin(Mesh2<T> M, PRows<T> R,
                                                       int64_t ld, int n,
                                                       const T* __restrict__ na,
                                                       T* __restrict__ cot, T* __restrict__ wk) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= n) return;
-  const PGeo<T> q = pgeo(M, p, PR(DMPM_P_X)[p], PR(DMPM_P_X + 1)[p]);
+  const PGeo<T> q = pgeo(M, p, R.x[p], R.y[p]);
   if (!q.in) return;
   const int* ids = q.ids;
-  const T m = PR(DMPM_P_MASS)[p];
-  const T vx = PR(DMPM_P_V)[p], vy = PR(DMPM_P_V + 1)[p];
+  const T m = R.m[p];
+  const T vx = R.vx[p], vy = R.vy[p];
   T N[4];
   q4_N4(q.xi, q.eta, N);
   T sxi = WR(W_XI)[p], seta = WR(W_ETA)[p];
@@ -332,8 +401,8 @@ __global__ __launch_bounds__(kThreads) void k_adj_kin(Mesh2<T> M, const T* __res
 // 1) clear the cotangents of the particles outside the mesh.
 constexpr int kSlotsA = 512;
 template <typename T> struct AdjArgs {
-  const T* pb;
-  const T* pw;
+  PRows<T> R;
+  POut<T> O;
   int64_t ld;
   const T* nb;
   int64_t nld;
@@ -356,7 +425,7 @@ __global__ __launch_bounds__(kThreads) void k_adj_scatter(Mesh2<T> M, AdjArgs<T>
   const int* __restrict__ perm = sr.perm();
   const int* __restrict__ cs = sr.cs();
   const int ident = sr.flags[1];
-  const T* __restrict__ pb = a.pb;
+  const PRows<T>& R = a.R;
   const int64_t ld = a.ld;
   T* __restrict__ cot = a.cot;
   if ((int)blockIdx.x >= a.ncb) {
@@ -391,17 +460,17 @@ __global__ __launch_bounds__(kThreads) void k_adj_scatter(Mesh2<T> M, AdjArgs<T>
     for (int s = k0 + t; s < kend; s += kThreads) {
       const int p = ident ? s : perm[s];
       const int qq = s - k0;
-      const PGeo<T> q = pgeo(M, p, PR(DMPM_P_X)[p], PR(DMPM_P_X + 1)[p]);
+      const PGeo<T> q = pgeo(M, p, R.x[p], R.y[p]);
       st[0][qq] = q.xi;
       st[1][qq] = q.eta;
       if (MODE == 2) {
         T w[6];
-        adj_force_particle<T, true>(M, pb, a.pw, ld, a.na, a.vkin, cot, a.wk, a.lebar, a.mat,
+        adj_force_particle<T, true>(M, R, a.O, ld, a.na, a.vkin, cot, a.wk, a.lebar, a.mat,
                                     a.Gx, a.Gy, a.dt, p, q, w);
         for (int r = 0; r < 6; ++r) st[2 + r][qq] = w[r];
       } else {
         T w[10];
-        adj_g2p_particle<T, MODE == 0>(M, pb, ld, a.nb, a.nld, cot, a.wk, a.lebar, a.mat, a.dt,
+        adj_g2p_particle<T, MODE == 0>(M, R, ld, a.nb, a.nld, cot, a.wk, a.lebar, a.mat, a.dt,
                                        a.vu, p, q, w);
         for (int r = 0; r < NW; ++r) st[2 + r][qq] = w[W0 + r];
       }
@@ -536,8 +605,8 @@ __global__ __launch_bounds__(kThreads) void k_adj_node2(Mesh2<T> M, const T* __r
 }
 
 template <typename T, int CB>
-int adjoint_kernels(dmpm_plan* P, const Mesh2<T>& M, const T* pb, const T* pw, const T* nb,
-                    const T* vkin, SortRef sr, T* cot, T* lebar, int scheme, int vu,
+int adjoint_kernels(dmpm_plan* P, const Mesh2<T>& M, const PRows<T>& R, const POut<T>& O,
+                    const T* nb, const T* vkin, SortRef sr, T* cot, T* lebar, int scheme, int vu,
                     const double* grav, double dt, hipStream_t s);
 
 template <typename T, int CB>
@@ -559,8 +628,8 @@ int adjoint_t(dmpm_plan* P, const T* pb, T* pw, int32_t* eid, T* nb, T* cot, T* 
   DMPM_TRY(tile_run(P, pw, eid, nb, scheme, vu, grav, dt, step, 1, 0, s));
   const T* vkin = (const T*)plan_workspace(P, 1, (size_t)2 * P->nnodes * sizeof(T));
   if (!vkin) return fail(DMPM_EHIP, "workspace allocation failed");
-  return adjoint_kernels<T, CB>(P, M, pb, pw, nb, vkin, sort_ref(P), cot, lebar, scheme, vu,
-                                grav, dt, s);
+  return adjoint_kernels<T, CB>(P, M, rows_of_block(pb, ld), out_of_block<T>(pw, ld), nb, vkin,
+                                sort_ref(P), cot, lebar, scheme, vu, grav, dt, s);
 }
 
 // The adjoint kernels of one step from its forward quantities: pb the step's
@@ -568,8 +637,8 @@ int adjoint_t(dmpm_plan* P, const T* pb, T* pw, int32_t* eid, T* nb, T* cot, T* 
 // the (USF) stress update, nb the node block after the step, vkin the
 // kinematic nodal velocity, sr the step's sort by cell.
 template <typename T, int CB>
-int adjoint_kernels(dmpm_plan* P, const Mesh2<T>& M, const T* pb, const T* pw, const T* nb,
-                    const T* vkin, SortRef sr, T* cot, T* lebar, int scheme, int vu,
+int adjoint_kernels(dmpm_plan* P, const Mesh2<T>& M, const PRows<T>& R, const POut<T>& O,
+                    const T* nb, const T* vkin, SortRef sr, T* cot, T* lebar, int scheme, int vu,
                     const double* grav, double dt, hipStream_t s) {
   const int64_t ld = P->d.ld, nld = P->d.nld;
   const int n = (int)P->n;
@@ -587,8 +656,8 @@ int adjoint_kernels(dmpm_plan* P, const Mesh2<T>& M, const T* pb, const T* pw, c
   const unsigned np = nblk(n), nn = nblk(P->nnodes);
   const unsigned ncb = (unsigned)((M.ncells + CB - 1) / CB);
   AdjArgs<T> aa;
-  aa.pb = pb;
-  aa.pw = pw;
+  aa.R = R;
+  aa.O = O;
   aa.ld = ld;
   aa.nb = nb;
   aa.nld = nld;
@@ -619,10 +688,10 @@ int adjoint_kernels(dmpm_plan* P, const Mesh2<T>& M, const T* pb, const T* pw, c
     k_adj_scatter<T, CB, 2><<<ncb, kThreads, 0, s>>>(M, aa, sr, part);
     k_adj_node2<T><<<nn, kThreads, 0, s>>>(M, part, nb, nld, vkin, na, P->cidx, P->cmask);
   } else {
-    k_adj_force<T><<<np, kThreads, 0, s>>>(M, pb, ld, n, na, vkin, cot, wk, lebar, mat, Gx, Gy,
+    k_adj_force<T><<<np, kThreads, 0, s>>>(M, R, ld, n, na, vkin, cot, wk, lebar, mat, Gx, Gy,
                                           tdt);
   }
-  k_adj_kin<T><<<np, kThreads, 0, s>>>(M, pb, ld, n, na, cot, wk);
+  k_adj_kin<T><<<np, kThreads, 0, s>>>(M, R, ld, n, na, cot, wk);
   return launch_check("step_adjoint");
 }
 
@@ -685,15 +754,53 @@ extern "C" int dmpm_step_adjoint_saved(dmpm_plan* P, const void* pin, const void
   sr.cs0 = sr.cs1 = cs_saved;
   sr.flags = P->zflags;  // not the identity, buffer 0
   hipStream_t s = (hipStream_t)stream;
+  const int64_t ld = P->d.ld;
   if (P->d.dtype == DMPM_F32)
-    return adjoint_kernels<float, 128>(P, mesh_of<float>(P), (const float*)pin,
-                                       (const float*)pout, (const float*)nsaved,
-                                       (const float*)vkin_saved, sr, (float*)cot, (float*)lebar,
-                                       scheme, velocity_update, gravity, dt, s);
-  return adjoint_kernels<double, 128>(P, mesh_of<double>(P), (const double*)pin,
-                                      (const double*)pout, (const double*)nsaved,
-                                      (const double*)vkin_saved, sr, (double*)cot,
-                                      (double*)lebar, scheme, velocity_update, gravity, dt, s);
+    return adjoint_kernels<float, 128>(
+        P, mesh_of<float>(P), rows_of_block((const float*)pin, ld),
+        out_of_block((const float*)pout, ld), (const float*)nsaved, (const float*)vkin_saved, sr,
+        (float*)cot, (float*)lebar, scheme, velocity_update, gravity, dt, s);
+  return adjoint_kernels<double, 128>(
+      P, mesh_of<double>(P), rows_of_block((const double*)pin, ld),
+      out_of_block((const double*)pout, ld), (const double*)nsaved, (const double*)vkin_saved,
+      sr, (double*)cot, (double*)lebar, scheme, velocity_update, gravity, dt, s);
+}
+
+extern "C" int dmpm_step_adjoint_saved_state(
+    dmpm_plan* P, const void* pconst, const void* state_in, const void* state_out,
+    int64_t state_ld, const void* nsaved, const void* vkin_saved, const int32_t* perm_saved,
+    const int32_t* cs_saved, void* cot, void* lebar, int32_t scheme, int32_t velocity_update,
+    const double* gravity, double dt, void* stream) {
+  if (!P) return fail(DMPM_EINVAL, "null plan");
+  if (!pconst || !state_in || !state_out || !nsaved || !vkin_saved || !perm_saved ||
+      !cs_saved || !cot || !gravity)
+    return fail(DMPM_EINVAL, "null buffer");
+  if (state_ld < P->n) return fail(DMPM_EINVAL, "state_ld below the particle count");
+  if (scheme != DMPM_USF && scheme != DMPM_USL)
+    return fail(DMPM_EINVAL, "scheme must be DMPM_USF or DMPM_USL");
+  if (!tile_eligible(P, 0))
+    return fail(DMPM_EINVAL,
+                "the adjoint step covers 2D Quadrilateral4Node meshes without particle "
+                "tractions or particle external forces");
+  SortRef sr;
+  sr.perm0 = sr.perm1 = perm_saved;
+  sr.cs0 = sr.cs1 = cs_saved;
+  sr.flags = P->zflags;  // not the identity, buffer 0
+  hipStream_t s = (hipStream_t)stream;
+  const int64_t ld = P->d.ld;
+  if (P->d.dtype == DMPM_F32)
+    return adjoint_kernels<float, 128>(
+        P, mesh_of<float>(P),
+        rows_of_state((const float*)state_in, state_ld, (const float*)pconst, ld),
+        out_of_state((const float*)state_out, state_ld), (const float*)nsaved,
+        (const float*)vkin_saved, sr, (float*)cot, (float*)lebar, scheme, velocity_update,
+        gravity, dt, s);
+  return adjoint_kernels<double, 128>(
+      P, mesh_of<double>(P),
+      rows_of_state((const double*)state_in, state_ld, (const double*)pconst, ld),
+      out_of_state((const double*)state_out, state_ld), (const double*)nsaved,
+      (const double*)vkin_saved, sr, (double*)cot, (double*)lebar, scheme, velocity_update,
+      gravity, dt, s);
 }
 
 extern "C" int dmpm_step_adjoint(dmpm_plan* P, const void* pin, void* pwork, int32_t* eid_work,

@@ -1696,6 +1696,27 @@ template <typename T, int DIM> struct RunSteps {
 
 }  // namespace dmpm
 
+extern "C" int dmpm_step_between(dmpm_plan* P, const void* pbuf_in, void* pbuf_out,
+                                 int32_t* eid, void* nbuf, int32_t scheme,
+                                 int32_t velocity_update, const double* gravity, double dt,
+                                 int64_t step, int32_t flags, void* state_out,
+                                 int64_t state_ld, void* stream) {
+  PCHK(P);
+  if (!pbuf_in || !pbuf_out || !eid || !nbuf) return fail(DMPM_EINVAL, "null buffer");
+  if (state_out && state_ld < P->n) return fail(DMPM_EINVAL, "state_ld below the particle count");
+  if (pbuf_in == pbuf_out) return fail(DMPM_EINVAL, "input and output blocks must differ");
+  if (scheme != DMPM_USF && scheme != DMPM_USL)
+    return fail(DMPM_EINVAL, "scheme must be DMPM_USF or DMPM_USL");
+  if (!gravity) return fail(DMPM_EINVAL, "null gravity");
+  if (!tile_eligible(P, 0))
+    return fail(DMPM_EINVAL,
+                "dmpm_step_between covers the fused 2D step (Quadrilateral4Node, no particle "
+                "tractions or particle external forces)");
+  return tile_between(P, pbuf_in, pbuf_out, eid, nbuf, (int)scheme, (int)velocity_update,
+                      gravity, dt, step, (int)(flags & 1), (int)((flags >> 1) & 1), state_out,
+                      state_ld, ST(stream));
+}
+
 extern "C" int dmpm_run(dmpm_plan* P, void* pbuf, int32_t* eid, void* nbuf, int32_t scheme,
                         int32_t velocity_update, const double* gravity, double dt, int64_t step0,
                         int64_t nsteps, int32_t flags, void* stream) {

@@ -102,6 +102,8 @@ struct dmpm_plan {
   int cell_block = 128;  // cells per workgroup of the P2G cell pass (64/128/256)
   int usf_split = 1;     // USF: stress update streamed (1) or in the force cell pass (0)
   int step_kernel = 1;   // single-simulation fused step: one launch (1) or one per phase (0)
+  unsigned char step_aux[256];  // host copy of the one-launch step's device-side tables
+  int step_aux_bytes = 0;       // bytes of step_aux uploaded (0: none yet)
   // workspace of the fused tile kernels and the adjoint (allocated lazily)
   void* extra[16] = {nullptr};
   size_t extra_bytes[16] = {0};
@@ -163,6 +165,10 @@ int tile_run(dmpm_plan* P, void* pbuf, int32_t* eid, void* nbuf, int scheme, int
              const double* grav, double dt, int64_t step0, int64_t nsteps, int temps,
              hipStream_t s);
 bool tile_eligible(const dmpm_plan* P, int pfext);
+// One fused 2D step from block pin to block pout (mpm_tile.hip).
+int tile_between(dmpm_plan* P, const void* pin, void* pout, int32_t* eid, void* nbuf,
+                 int scheme, int vu, const double* grav, double dt, int64_t step, int temps,
+                 int chained, void* sout, int64_t sld, hipStream_t s);
 // Kernel classes for dmpm_timing_enable.
 enum { KC_LOCATE = 0, KC_SORT = 1, KC_P2G_KIN = 2, KC_STRESS = 3, KC_P2G_FORCE = 4,
        KC_GRID = 5, KC_G2P = 6, KC_SMALL = 7, KC_TILE_KIN = 8, KC_TILE_FORCE = 9,

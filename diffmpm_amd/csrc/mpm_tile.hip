@@ -31,6 +31,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstring>
 
 #include "mpm_common.h"
 #include "mpm_plan.h"
@@ -680,7 +681,7 @@ __device__ __forceinline__ void bst(T* base, uint32_t nbytes, uint32_t row_elems
 #define SR(r, i) bld(a.src, a.pbytes, (uint32_t)(r) * a.ld32, (uint32_t)(i))
 #define PB(r, i) bld<T>(a.pb, a.pbytes, (uint32_t)(r) * a.ld32, (uint32_t)(i))
 #define NB(r, i) bld<T>(a.nb, a.nbytes, (uint32_t)(r) * a.nld32, (uint32_t)(i))
-#define DR_W(r, i, v) bst(a.dst, a.pbytes, (uint32_t)(r) * a.ld32, (uint32_t)(i), (T)(v))
+#define DR_W(r, i, v) dyn_write(a, (r), (uint32_t)(i), (T)(v))
 #define PB_W(r, i, v) bst(a.pb, a.pbytes, (uint32_t)(r) * a.ld32, (uint32_t)(i), (T)(v))
 #define NB_W(r, i, v) bst(a.nb, a.nbytes, (uint32_t)(r) * a.nld32, (uint32_t)(i), (T)(v))
 
@@ -721,7 +722,34 @@ template <typename T> struct StepArgs {
   int* ovf;  // [2] overflow counts by step parity, then [2][ntiles] tile lists
   int par;   // step parity
   int nrb;   // workgroups of k_step_rounds that process tiles
+  T* sout;   // optional copy of the state rows x, y, vx, vy, sxx, syy, sxy, V after the step
+  uint32_t sld32, sbytes;
 };
+
+// Row of the state copy (x, y, vx, vy, sxx, syy, sxy, V) that particle row r
+// goes to, or -1.
+__device__ __forceinline__ int state_row(int r) {
+  switch (r) {
+    case DMPM_P_X: return 0;
+    case DMPM_P_X + 1: return 1;
+    case DMPM_P_V: return 2;
+    case DMPM_P_V + 1: return 3;
+    case DMPM_P_STRESS + 0: return 4;
+    case DMPM_P_STRESS + 1: return 5;
+    case DMPM_P_STRESS + 3: return 6;
+    case DMPM_P_VOLUME: return 7;
+    default: return -1;
+  }
+}
+// A dynamic row's value after the step: to dst, and to the state copy.
+template <typename T>
+__device__ __forceinline__ void dyn_write(const StepArgs<T>& a, int r, uint32_t i, T v) {
+  bst(a.dst, a.pbytes, (uint32_t)r * a.ld32, i, v);
+  if (a.sout) {
+    const int q = state_row(r);
+    if (q >= 0) bst(a.sout, a.sbytes, (uint32_t)q * a.sld32, i, v);
+  }
+}
 
 // n / d for 0 <= n < 1024 and 1 <= d <= 64 without a division:
 // floor(n * ceil(2^16 / d) / 2^16) (exact there: the rounding error of the
@@ -2237,19 +2265,20 @@ bool step_eligible(const dmpm_plan* P) {
 // The one-launch step for single-simulation plans: per step, the re-sort,
 // k_step_fast over every tile (FP32) and k_step_rounds over the tiles that
 // overflowed the fast path and the particles outside the mesh.  The
-// particle state alternates between the caller's block and a shadow block
-// (dynamic rows only); nsteps is even, so it ends in the caller's block.
+// particle state alternates between blocks pa and pb2 (dynamic rows only;
+// pb2 null: the plan's shadow block), starting from pa; pstat holds the
+// static rows (mass, E) and takes the temporaries.
 template <typename T>
-int step_run(dmpm_plan* P, const Mesh2<T>& M, T* pb, int32_t* eid, T* nb, int scheme, int vu,
-             const double* grav, double dt, int64_t step0, int64_t nsteps, int temps,
-             hipStream_t s, T* vkin) {
+int step_run(dmpm_plan* P, const Mesh2<T>& M, T* pa, T* pb2, T* pstat, int32_t* eid, T* nb,
+             int scheme, int vu, const double* grav, double dt, int64_t step0, int64_t nsteps,
+             int temps, hipStream_t s, T* vkin, T* sout = nullptr, int64_t sld = 0) {
   constexpr bool kFast = sizeof(T) == 4;
   const int64_t ld = P->d.ld, nld = P->d.nld;
   const int nn = P->nnodes;
   StepArgs<T> a;
   a.tiles_x = (M.g0.nx + kTW - 1) / kTW;
   a.ntiles = a.tiles_x * ((M.g0.ny + kTH - 1) / kTH);
-  T* shadow = (T*)plan_workspace(P, 5, (size_t)DMPM_P_ROWS * ld * sizeof(T));
+  T* shadow = pb2 ? pb2 : (T*)plan_workspace(P, 5, (size_t)DMPM_P_ROWS * ld * sizeof(T));
   T* vbuf = (T*)plan_workspace(P, 6, (size_t)4 * nn * sizeof(T));
   int* ovf = kFast ? (int*)plan_workspace(P, 7, (size_t)(2 + 2 * a.ntiles) * sizeof(int)) : nullptr;
   if (!shadow || !vbuf || (kFast && !ovf)) return fail(DMPM_EHIP, "workspace allocation failed");
@@ -2258,7 +2287,7 @@ int step_run(dmpm_plan* P, const Mesh2<T>& M, T* pb, int32_t* eid, T* nb, int sc
                                       nn * sizeof(T), 2, hipMemcpyDeviceToDevice, s),
                      "hipMemcpy2DAsync"));
   a.g = M.g0;
-  a.pb = pb;
+  a.pb = pstat;
   a.ld32 = (uint32_t)ld;
   a.nb = nb;
   a.nld32 = (uint32_t)nld;
@@ -2269,6 +2298,7 @@ int step_run(dmpm_plan* P, const Mesh2<T>& M, T* pb, int32_t* eid, T* nb, int sc
   a.cidx = P->cidx;
   a.cmask = P->cmask;
   StepAux x;
+  std::memset(&x, 0, sizeof x);  // padding too: the struct is compared bytewise
   x.L = move_log(P);
   x.cval = P->cval;
   x.nfd = P->nfd;
@@ -2281,10 +2311,18 @@ int step_run(dmpm_plan* P, const Mesh2<T>& M, T* pb, int32_t* eid, T* nb, int sc
   x.fns.xs = P->fn_x;
   x.fns.fs = P->fn_f;
   a.has_nf = P->nfd != nullptr;
+  static_assert(sizeof(StepAux) <= sizeof(P->step_aux), "step_aux size");
   StepAux* aux = (StepAux*)plan_workspace(P, 8, sizeof(StepAux));
   if (!aux) return fail(DMPM_EHIP, "workspace allocation failed");
-  DMPM_TRY(hip_check(hipMemcpyAsync(aux, &x, sizeof(StepAux), hipMemcpyHostToDevice, s),
-                     "hipMemcpyAsync"));
+  // upload only when the tables changed (a copy from host memory orders the
+  // host behind the stream)
+  if (P->step_aux_bytes != (int)sizeof(StepAux) ||
+      std::memcmp(P->step_aux, &x, sizeof(StepAux)) != 0) {
+    std::memcpy(P->step_aux, &x, sizeof(StepAux));
+    P->step_aux_bytes = (int)sizeof(StepAux);
+    DMPM_TRY(hip_check(hipMemcpyAsync(aux, P->step_aux, sizeof(StepAux), hipMemcpyHostToDevice, s),
+                       "hipMemcpyAsync"));
+  }
   a.aux = aux;
   a.mat.kind = P->d.material;
   a.mat.per_E = P->d.per_particle_E;
@@ -2296,16 +2334,19 @@ int step_run(dmpm_plan* P, const Mesh2<T>& M, T* pb, int32_t* eid, T* nb, int sc
   a.temps = temps;
   a.n = (int)P->n;
   a.ovf = ovf;
+  a.sout = sout;
+  a.sld32 = (uint32_t)sld;
+  a.sbytes = (uint32_t)(8 * (uint64_t)sld * sizeof(T));
   // FP32: overflow tiles are rare, a few workgroups loop over them
   a.nrb = kFast ? std::min(a.ntiles, 64) : a.ntiles;
   for (int64_t k = 0; k < nsteps; ++k) {
-    T* src = (k & 1) ? shadow : pb;
+    T* src = (k & 1) ? shadow : pa;
     {
       ProfScopeT ps_(P, KC_SORT, s);
       DMPM_TRY(enqueue_resort<T>(P, M, src, eid, s));
     }
     a.src = src;
-    a.dst = (k & 1) ? pb : shadow;
+    a.dst = (k & 1) ? pa : shadow;
     a.vold = vbuf + (k & 1) * 2 * (int64_t)nn;
     a.vnew = vbuf + ((k + 1) & 1) * 2 * (int64_t)nn;
     a.t = T(step0 + k) * a.dt;
@@ -2356,12 +2397,61 @@ int tile_run_t(dmpm_plan* P, T* pb, int32_t* eid, T* nb, int scheme, int vu, con
       ++step0;
       --nsteps;
     }
-    return nsteps ? step_run<T>(P, M, pb, eid, nb, scheme, vu, grav, dt, step0, nsteps, temps,
-                                s, vkin)
+    return nsteps ? step_run<T>(P, M, pb, nullptr, pb, eid, nb, scheme, vu, grav, dt, step0,
+                                nsteps, temps, s, vkin)
                   : DMPM_OK;
   }
   return tile_steps_cb<T>(P, M, pb, eid, nb, scheme, vu, grav, dt, step0, nsteps, temps, s, part,
                           vkin);
+}
+
+// One step from block pin to block pout (pin unchanged; pout's static rows
+// must hold the particle set's mass and E).  Single-simulation plans run the
+// one-launch step between the two blocks; others copy the dynamic rows and
+// step pout in place.
+template <typename T>
+int tile_between_t(dmpm_plan* P, const T* pin, T* pout, int32_t* eid, T* nb, int scheme, int vu,
+                   const double* grav, double dt, int64_t step, int temps, int chained,
+                   T* sout, int64_t sld, hipStream_t s) {
+  const Mesh2<T> M = mesh_of<T>(P);
+  const int64_t ld = P->d.ld;
+  const int n = (int)P->n;
+  T* part = (T*)plan_workspace(P, 0, (size_t)4 * NQ_FORCE * M.ncells * sizeof(T));
+  T* vkin = (T*)plan_workspace(P, 1, (size_t)2 * P->nnodes * sizeof(T));
+  if (!part || !vkin) return fail(DMPM_EHIP, "workspace allocation failed");
+  if (!chained) {
+    // record the particles moved since the plan's last step (as dmpm_run does)
+    {
+      ProfScopeT ps_(P, KC_SORT, s);
+      DMPM_TRY(enqueue_resort<T>(P, M, pin, eid, s));
+    }
+    k_tile_check<T><<<nblk(n), kThreads, 0, s>>>(M, pin, ld, n, eid, P->key, move_log(P));
+    DMPM_TRY(launch_check("tile_check"));
+  }
+  if (step_eligible(P) && (uint64_t)8 * (uint64_t)sld * sizeof(T) < (1ull << 32))
+    return step_run<T>(P, M, const_cast<T*>(pin), pout, pout, eid, nb, scheme, vu, grav, dt,
+                       step, 1, temps, s, vkin, sout, sld);
+  const int runs[6][2] = {{DMPM_P_X, 4},          {DMPM_P_VOLUME, 2}, {DMPM_P_STRAIN, 2},
+                          {DMPM_P_STRAIN + 3, 1}, {DMPM_P_STRESS, 4}, {DMPM_P_VOLSTRAIN_C, 1}};
+  for (const auto& r : runs)
+    DMPM_TRY(hip_check(hipMemcpy2DAsync(pout + r[0] * ld, ld * sizeof(T), pin + r[0] * ld,
+                                        ld * sizeof(T), (size_t)n * sizeof(T), r[1],
+                                        hipMemcpyDeviceToDevice, s),
+                       "hipMemcpy2DAsync"));
+  DMPM_TRY(tile_steps_cb<T>(P, M, pout, eid, nb, scheme, vu, grav, dt, step, 1, temps, s, part,
+                            vkin));
+  if (sout) {
+    const int srows[4][3] = {{DMPM_P_X, 0, 4},
+                             {DMPM_P_STRESS, 4, 2},
+                             {DMPM_P_STRESS + 3, 6, 1},
+                             {DMPM_P_VOLUME, 7, 1}};
+    for (const auto& r : srows)
+      DMPM_TRY(hip_check(hipMemcpy2DAsync(sout + r[1] * sld, sld * sizeof(T), pout + r[0] * ld,
+                                          ld * sizeof(T), (size_t)n * sizeof(T), r[2],
+                                          hipMemcpyDeviceToDevice, s),
+                         "hipMemcpy2DAsync"));
+  }
+  return DMPM_OK;
 }
 
 // tile_steps for the plan's cells per workgroup of the cell passes.
@@ -2388,6 +2478,16 @@ namespace dmpm {
 
 bool tile_eligible(const dmpm_plan* P, int pfext) {
   return P->dim == 2 && !P->has_tractions && !pfext && P->n > 0 && P->nchunks <= kMaxChunks;
+}
+
+int tile_between(dmpm_plan* P, const void* pin, void* pout, int32_t* eid, void* nbuf,
+                 int scheme, int vu, const double* grav, double dt, int64_t step, int temps,
+                 int chained, void* sout, int64_t sld, hipStream_t s) {
+  if (P->d.dtype == DMPM_F32)
+    return tile_between_t<float>(P, (const float*)pin, (float*)pout, eid, (float*)nbuf, scheme,
+                                 vu, grav, dt, step, temps, chained, (float*)sout, sld, s);
+  return tile_between_t<double>(P, (const double*)pin, (double*)pout, eid, (double*)nbuf, scheme,
+                                vu, grav, dt, step, temps, chained, (double*)sout, sld, s);
 }
 
 int tile_run(dmpm_plan* P, void* pbuf, int32_t* eid, void* nbuf, int scheme, int vu,

@@ -240,6 +240,23 @@ int dmpm_phase_particle_kinematics(dmpm_plan* plan, void* pbuf, const int32_t* e
 int dmpm_run(dmpm_plan* plan, void* pbuf, int32_t* eid, void* nbuf, int32_t scheme,
              int32_t velocity_update, const double* gravity, double dt, int64_t step0,
              int64_t nsteps, int32_t flags, void* stream);
+/* One fused 2D step (index step) from particle block pbuf_in to particle
+ * block pbuf_out: pbuf_in is left unchanged, pbuf_out receives the state
+ * after the step; pbuf_out's constant rows (mass, Young's modulus) must hold
+ * the particle set's values.  eid and the plan's sort state follow the
+ * particles as in dmpm_run.  flags bit 0: materialise the temporaries (into
+ * pbuf_out); bit 1: pbuf_in is the pbuf_out of this plan's previous call,
+ * unchanged since (skips the check for particles moved by the caller).
+ * state_out (or NULL): also receives the rows x, y, vx, vy, sigma_xx,
+ * sigma_yy, sigma_xy, V after the step (row r at state_out + r*state_ld).
+ * Results equal dmpm_run's on a copy of pbuf_in.  A caller that keeps two
+ * blocks and alternates them steps without copying the state (the
+ * reference's solver.py:179-213 step, split for differentiable rollouts); no
+ * reference counterpart. */
+int dmpm_step_between(dmpm_plan* plan, const void* pbuf_in, void* pbuf_out, int32_t* eid,
+                      void* nbuf, int32_t scheme, int32_t velocity_update,
+                      const double* gravity, double dt, int64_t step, int32_t flags,
+                      void* state_out, int64_t state_ld, void* stream);
 
 /* ---- adjoint of the fused 2D step ------------------------------------------
  * The reference differentiates a rollout with jax.grad (no hand-written
@@ -287,6 +304,18 @@ int dmpm_step_adjoint_saved(dmpm_plan* plan, const void* pin, const void* pout,
                             const int32_t* perm_saved, const int32_t* cs_saved, void* cot,
                             void* lebar, int32_t scheme, int32_t velocity_update,
                             const double* gravity, double dt, void* stream);
+/* dmpm_step_adjoint_saved with the step's input and output state given as
+ * (8 x state_ld) state tensors (rows x, y, vx, vy, sigma_xx, sigma_yy,
+ * sigma_xy, V; row r at state + r*state_ld) instead of particle blocks;
+ * pconst: a particle block whose mass and Young's modulus rows hold the
+ * particle set's values.  Same results.  No reference counterpart (the
+ * reference differentiates with jax.grad). */
+int dmpm_step_adjoint_saved_state(dmpm_plan* plan, const void* pconst, const void* state_in,
+                                  const void* state_out, int64_t state_ld, const void* nsaved,
+                                  const void* vkin_saved, const int32_t* perm_saved,
+                                  const int32_t* cs_saved, void* cot, void* lebar,
+                                  int32_t scheme, int32_t velocity_update,
+                                  const double* gravity, double dt, void* stream);
 
 /* ---- measurement (no reference counterpart) ------------------------------- */
 /* Kernel classes for dmpm_timing_enable. */
