@@ -92,7 +92,7 @@ SIGNATURES = {
     "dmpm_timing_read_class": [_P, c_int32, _DP, POINTER(c_int64)],
     "dmpm_run": [_P, _P, _P, _P, c_int32, c_int32, _DP, c_double, c_int64, c_int64, c_int32, _P],
     "dmpm_step_between": [_P, _P, _P, _P, _P, c_int32, c_int32, _DP, c_double, c_int64, c_int32,
-                          _P, c_int64, _P],
+                          _P, _P, c_int64, _P],
     "dmpm_step_adjoint": [_P, _P, _P, _P, _P, _P, _P, c_int32, c_int32, _DP, c_double, c_int64,
                           _P],
     "dmpm_plan_export_step": [_P, _P, _P, _P, _P],

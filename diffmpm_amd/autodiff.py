@@ -154,23 +154,25 @@ class Differentiable:
         else:
             if self._blocks is None:
                 self._blocks = (self._pbuf.clone(), self._pbuf.clone())
+            # the state rows are read from `state` and written to `out`; the
+            # two blocks carry the other rows (strain, density, sigma_zz,
+            # volumetric strain) and the constants
             last = getattr(self, "_last_b", None)
             chained = last is not None and last[0] is state and state._version == last[1]
-            if chained:
-                src = last[2]  # the block holds this state, as the plan's last step left it
-            else:
-                src = 0
-                self._write(self._blocks[0], state.detach(), None)
+            src = last[2] if last is not None else 0
             dst = 1 - src
             if log_e is not None:
                 self._write_log_e(self._blocks[dst], log_e)
             plan = self.bplan
             plan.refresh_tables()
+            sin = state.detach()
+            if not sin.is_contiguous():
+                sin = sin.contiguous()
             out = torch.empty((C_ROWS, self.n), dtype=self._pbuf.dtype,
                               device=self._pbuf.device)
             call("dmpm_step_between", plan.h, ptr(self._blocks[src]), ptr(self._blocks[dst]),
                  ptr(self._eid_b), ptr(self._nbuf), self.code, self.vu, plan._gravity(gravity),
-                 self.dt, int(i), 2 if chained else 0, ptr(out), self.n, plan.s)
+                 self.dt, int(i), 2 if chained else 0, ptr(sin), ptr(out), self.n, plan.s)
             self._last_b = (out, out._version, dst)
         if not save:
             return out, None

@@ -1699,10 +1699,12 @@ template <typename T, int DIM> struct RunSteps {
 extern "C" int dmpm_step_between(dmpm_plan* P, const void* pbuf_in, void* pbuf_out,
                                  int32_t* eid, void* nbuf, int32_t scheme,
                                  int32_t velocity_update, const double* gravity, double dt,
-                                 int64_t step, int32_t flags, void* state_out,
-                                 int64_t state_ld, void* stream) {
+                                 int64_t step, int32_t flags, const void* state_in,
+                                 void* state_out, int64_t state_ld, void* stream) {
   PCHK(P);
   if (!pbuf_in || !pbuf_out || !eid || !nbuf) return fail(DMPM_EINVAL, "null buffer");
+  if (state_in && !state_out) return fail(DMPM_EINVAL, "state_in needs state_out");
+  if (state_in && state_in == state_out) return fail(DMPM_EINVAL, "state_in and state_out must differ");
   if (state_out && state_ld < P->n) return fail(DMPM_EINVAL, "state_ld below the particle count");
   if (pbuf_in == pbuf_out) return fail(DMPM_EINVAL, "input and output blocks must differ");
   if (scheme != DMPM_USF && scheme != DMPM_USL)
@@ -1713,8 +1715,8 @@ extern "C" int dmpm_step_between(dmpm_plan* P, const void* pbuf_in, void* pbuf_o
                 "dmpm_step_between covers the fused 2D step (Quadrilateral4Node, no particle "
                 "tractions or particle external forces)");
   return tile_between(P, pbuf_in, pbuf_out, eid, nbuf, (int)scheme, (int)velocity_update,
-                      gravity, dt, step, (int)(flags & 1), (int)((flags >> 1) & 1), state_out,
-                      state_ld, ST(stream));
+                      gravity, dt, step, (int)(flags & 1), (int)((flags >> 1) & 1), state_in,
+                      state_out, state_ld, ST(stream));
 }
 
 extern "C" int dmpm_run(dmpm_plan* P, void* pbuf, int32_t* eid, void* nbuf, int32_t scheme,

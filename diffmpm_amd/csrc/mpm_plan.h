@@ -168,7 +168,7 @@ bool tile_eligible(const dmpm_plan* P, int pfext);
 // One fused 2D step from block pin to block pout (mpm_tile.hip).
 int tile_between(dmpm_plan* P, const void* pin, void* pout, int32_t* eid, void* nbuf,
                  int scheme, int vu, const double* grav, double dt, int64_t step, int temps,
-                 int chained, void* sout, int64_t sld, hipStream_t s);
+                 int chained, const void* sin, void* sout, int64_t sld, hipStream_t s);
 // Kernel classes for dmpm_timing_enable.
 enum { KC_LOCATE = 0, KC_SORT = 1, KC_P2G_KIN = 2, KC_STRESS = 3, KC_P2G_FORCE = 4,
        KC_GRID = 5, KC_G2P = 6, KC_SMALL = 7, KC_TILE_KIN = 8, KC_TILE_FORCE = 9,

@@ -678,7 +678,7 @@ __device__ __forceinline__ void bst(T* base, uint32_t nbytes, uint32_t row_elems
         __builtin_bit_cast(__attribute__((ext_vector_type(2))) uint32_t, v), rs, i * 8u,
         row_elems * 8u, 0);
 }
-#define SR(r, i) bld(a.src, a.pbytes, (uint32_t)(r) * a.ld32, (uint32_t)(i))
+#define SR(r, i) dyn_read(a, (r), (uint32_t)(i))
 #define PB(r, i) bld<T>(a.pb, a.pbytes, (uint32_t)(r) * a.ld32, (uint32_t)(i))
 #define NB(r, i) bld<T>(a.nb, a.nbytes, (uint32_t)(r) * a.nld32, (uint32_t)(i))
 #define DR_W(r, i, v) dyn_write(a, (r), (uint32_t)(i), (T)(v))
@@ -722,7 +722,9 @@ template <typename T> struct StepArgs {
   int* ovf;  // [2] overflow counts by step parity, then [2][ntiles] tile lists
   int par;   // step parity
   int nrb;   // workgroups of k_step_rounds that process tiles
-  T* sout;   // optional copy of the state rows x, y, vx, vy, sxx, syy, sxy, V after the step
+  T* sout;   // optional: the state rows x, y, vx, vy, sxx, syy, sxy, V after the step
+  const T* sin;  // optional: the state rows before the step (then read from here, and
+                 // written only to sout)
   uint32_t sld32, sbytes;
 };
 
@@ -741,14 +743,23 @@ __device__ __forceinline__ int state_row(int r) {
     default: return -1;
   }
 }
-// A dynamic row's value after the step: to dst, and to the state copy.
+// A dynamic row's value before the step: from the state tensor sin (state
+// rows, when given) or from src.
+template <typename T>
+__device__ __forceinline__ T dyn_read(const StepArgs<T>& a, int r, uint32_t i) {
+  if (a.sin) {
+    const int q = state_row(r);
+    if (q >= 0) return bld(a.sin, a.sbytes, (uint32_t)q * a.sld32, i);
+  }
+  return bld(a.src, a.pbytes, (uint32_t)r * a.ld32, i);
+}
+// A dynamic row's value after the step: to dst, and to the state tensor sout
+// (state rows; only there when sin is given).
 template <typename T>
 __device__ __forceinline__ void dyn_write(const StepArgs<T>& a, int r, uint32_t i, T v) {
-  bst(a.dst, a.pbytes, (uint32_t)r * a.ld32, i, v);
-  if (a.sout) {
-    const int q = state_row(r);
-    if (q >= 0) bst(a.sout, a.sbytes, (uint32_t)q * a.sld32, i, v);
-  }
+  const int q = a.sout ? state_row(r) : -1;
+  if (q >= 0) bst(a.sout, a.sbytes, (uint32_t)q * a.sld32, i, v);
+  if (q < 0 || !a.sin) bst(a.dst, a.pbytes, (uint32_t)r * a.ld32, i, v);
 }
 
 // n / d for 0 <= n < 1024 and 1 <= d <= 64 without a division:
@@ -2154,7 +2165,8 @@ __global__ __launch_bounds__(kThreads) void k_sort_s4(SortArgs a) {
 constexpr int kSortGrid = 2048;  // workgroups of the grid-stride re-sort stages
 
 template <typename T>
-int enqueue_resort(dmpm_plan* P, const Mesh2<T>& M, const T* pb, int32_t* eid, hipStream_t s) {
+int enqueue_resort(dmpm_plan* P, const Mesh2<T>& M, const T* pb, int32_t* eid, hipStream_t s,
+                   int64_t ld = -1) {
   SortArgs a;
   a.eid = eid;
   a.key = P->key;
@@ -2171,7 +2183,8 @@ int enqueue_resort(dmpm_plan* P, const Mesh2<T>& M, const T* pb, int32_t* eid, h
   a.ncells = P->ncells;
   a.nchunks = P->nchunks;
   const int g12 = std::max(P->nchunks, kSortGrid);
-  k_sort_s1<T><<<g12, kThreads, 0, s>>>(M, pb, P->d.ld, a);
+  // (positions: rows 0 and 1 of pb, row stride ld)
+  k_sort_s1<T><<<g12, kThreads, 0, s>>>(M, pb, ld < 0 ? P->d.ld : ld, a);
   k_sort_s2<<<g12, kThreads, 0, s>>>(a);
   // stages 3 and 4 have little to do in the incremental mode (the movers, a
   // flag): small grids, since scheduling idle workgroups costs more
@@ -2271,7 +2284,8 @@ bool step_eligible(const dmpm_plan* P) {
 template <typename T>
 int step_run(dmpm_plan* P, const Mesh2<T>& M, T* pa, T* pb2, T* pstat, int32_t* eid, T* nb,
              int scheme, int vu, const double* grav, double dt, int64_t step0, int64_t nsteps,
-             int temps, hipStream_t s, T* vkin, T* sout = nullptr, int64_t sld = 0) {
+             int temps, hipStream_t s, T* vkin, T* sout = nullptr, int64_t sld = 0,
+             const T* sin = nullptr) {
   constexpr bool kFast = sizeof(T) == 4;
   const int64_t ld = P->d.ld, nld = P->d.nld;
   const int nn = P->nnodes;
@@ -2335,6 +2349,7 @@ int step_run(dmpm_plan* P, const Mesh2<T>& M, T* pa, T* pb2, T* pstat, int32_t* 
   a.n = (int)P->n;
   a.ovf = ovf;
   a.sout = sout;
+  a.sin = sin;
   a.sld32 = (uint32_t)sld;
   a.sbytes = (uint32_t)(8 * (uint64_t)sld * sizeof(T));
   // FP32: overflow tiles are rare, a few workgroups loop over them
@@ -2343,7 +2358,11 @@ int step_run(dmpm_plan* P, const Mesh2<T>& M, T* pa, T* pb2, T* pstat, int32_t* 
     T* src = (k & 1) ? shadow : pa;
     {
       ProfScopeT ps_(P, KC_SORT, s);
-      DMPM_TRY(enqueue_resort<T>(P, M, src, eid, s));
+      // positions: from the state tensor when the state rows live there
+      if (sin && k == 0)
+        DMPM_TRY(enqueue_resort<T>(P, M, sin, eid, s, sld));
+      else
+        DMPM_TRY(enqueue_resort<T>(P, M, src, eid, s));
     }
     a.src = src;
     a.dst = (k & 1) ? pa : shadow;
@@ -2412,7 +2431,7 @@ int tile_run_t(dmpm_plan* P, T* pb, int32_t* eid, T* nb, int scheme, int vu, con
 template <typename T>
 int tile_between_t(dmpm_plan* P, const T* pin, T* pout, int32_t* eid, T* nb, int scheme, int vu,
                    const double* grav, double dt, int64_t step, int temps, int chained,
-                   T* sout, int64_t sld, hipStream_t s) {
+                   const T* sin, T* sout, int64_t sld, hipStream_t s) {
   const Mesh2<T> M = mesh_of<T>(P);
   const int64_t ld = P->d.ld;
   const int n = (int)P->n;
@@ -2421,16 +2440,19 @@ int tile_between_t(dmpm_plan* P, const T* pin, T* pout, int32_t* eid, T* nb, int
   if (!part || !vkin) return fail(DMPM_EHIP, "workspace allocation failed");
   if (!chained) {
     // record the particles moved since the plan's last step (as dmpm_run does)
+    // positions: rows 0, 1 of the state tensor when the state rows live there
+    const T* px = sin ? sin : pin;
+    const int64_t pld = sin ? sld : ld;
     {
       ProfScopeT ps_(P, KC_SORT, s);
-      DMPM_TRY(enqueue_resort<T>(P, M, pin, eid, s));
+      DMPM_TRY(enqueue_resort<T>(P, M, px, eid, s, pld));
     }
-    k_tile_check<T><<<nblk(n), kThreads, 0, s>>>(M, pin, ld, n, eid, P->key, move_log(P));
+    k_tile_check<T><<<nblk(n), kThreads, 0, s>>>(M, px, pld, n, eid, P->key, move_log(P));
     DMPM_TRY(launch_check("tile_check"));
   }
   if (step_eligible(P) && (uint64_t)8 * (uint64_t)sld * sizeof(T) < (1ull << 32))
     return step_run<T>(P, M, const_cast<T*>(pin), pout, pout, eid, nb, scheme, vu, grav, dt,
-                       step, 1, temps, s, vkin, sout, sld);
+                       step, 1, temps, s, vkin, sout, sld, sin);
   const int runs[6][2] = {{DMPM_P_X, 4},          {DMPM_P_VOLUME, 2}, {DMPM_P_STRAIN, 2},
                           {DMPM_P_STRAIN + 3, 1}, {DMPM_P_STRESS, 4}, {DMPM_P_VOLSTRAIN_C, 1}};
   for (const auto& r : runs)
@@ -2438,13 +2460,19 @@ int tile_between_t(dmpm_plan* P, const T* pin, T* pout, int32_t* eid, T* nb, int
                                         ld * sizeof(T), (size_t)n * sizeof(T), r[1],
                                         hipMemcpyDeviceToDevice, s),
                        "hipMemcpy2DAsync"));
+  const int srows[4][3] = {{DMPM_P_X, 0, 4},
+                           {DMPM_P_STRESS, 4, 2},
+                           {DMPM_P_STRESS + 3, 6, 1},
+                           {DMPM_P_VOLUME, 7, 1}};
+  if (sin)
+    for (const auto& r : srows)
+      DMPM_TRY(hip_check(hipMemcpy2DAsync(pout + r[0] * ld, ld * sizeof(T), sin + r[1] * sld,
+                                          sld * sizeof(T), (size_t)n * sizeof(T), r[2],
+                                          hipMemcpyDeviceToDevice, s),
+                         "hipMemcpy2DAsync"));
   DMPM_TRY(tile_steps_cb<T>(P, M, pout, eid, nb, scheme, vu, grav, dt, step, 1, temps, s, part,
                             vkin));
   if (sout) {
-    const int srows[4][3] = {{DMPM_P_X, 0, 4},
-                             {DMPM_P_STRESS, 4, 2},
-                             {DMPM_P_STRESS + 3, 6, 1},
-                             {DMPM_P_VOLUME, 7, 1}};
     for (const auto& r : srows)
       DMPM_TRY(hip_check(hipMemcpy2DAsync(sout + r[1] * sld, sld * sizeof(T), pout + r[0] * ld,
                                           ld * sizeof(T), (size_t)n * sizeof(T), r[2],
@@ -2482,12 +2510,14 @@ bool tile_eligible(const dmpm_plan* P, int pfext) {
 
 int tile_between(dmpm_plan* P, const void* pin, void* pout, int32_t* eid, void* nbuf,
                  int scheme, int vu, const double* grav, double dt, int64_t step, int temps,
-                 int chained, void* sout, int64_t sld, hipStream_t s) {
+                 int chained, const void* sin, void* sout, int64_t sld, hipStream_t s) {
   if (P->d.dtype == DMPM_F32)
     return tile_between_t<float>(P, (const float*)pin, (float*)pout, eid, (float*)nbuf, scheme,
-                                 vu, grav, dt, step, temps, chained, (float*)sout, sld, s);
+                                 vu, grav, dt, step, temps, chained, (const float*)sin,
+                                 (float*)sout, sld, s);
   return tile_between_t<double>(P, (const double*)pin, (double*)pout, eid, (double*)nbuf, scheme,
-                                vu, grav, dt, step, temps, chained, (double*)sout, sld, s);
+                                vu, grav, dt, step, temps, chained, (const double*)sin,
+                                (double*)sout, sld, s);
 }
 
 int tile_run(dmpm_plan* P, void* pbuf, int32_t* eid, void* nbuf, int scheme, int vu,

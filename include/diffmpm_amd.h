@@ -245,18 +245,20 @@ int dmpm_run(dmpm_plan* plan, void* pbuf, int32_t* eid, void* nbuf, int32_t sche
  * after the step; pbuf_out's constant rows (mass, Young's modulus) must hold
  * the particle set's values.  eid and the plan's sort state follow the
  * particles as in dmpm_run.  flags bit 0: materialise the temporaries (into
- * pbuf_out); bit 1: pbuf_in is the pbuf_out of this plan's previous call,
+ * pbuf_out); bit 1: the input is the output of this plan's previous call,
  * unchanged since (skips the check for particles moved by the caller).
- * state_out (or NULL): also receives the rows x, y, vx, vy, sigma_xx,
- * sigma_yy, sigma_xy, V after the step (row r at state_out + r*state_ld).
- * Results equal dmpm_run's on a copy of pbuf_in.  A caller that keeps two
- * blocks and alternates them steps without copying the state (the
- * reference's solver.py:179-213 step, split for differentiable rollouts); no
- * reference counterpart. */
+ * State tensors (8 x state_ld, rows x, y, vx, vy, sigma_xx, sigma_yy,
+ * sigma_xy, V; row r at base + r*state_ld): with state_out only, it also
+ * receives those rows after the step; with state_in and state_out, those
+ * rows are read from state_in instead of pbuf_in and written to state_out
+ * instead of pbuf_out (the blocks carry the other rows).  Results equal
+ * dmpm_run's on a copy of the input.  A caller that alternates two blocks
+ * steps without copying the state (the reference's solver.py:179-213 step,
+ * split for differentiable rollouts); no reference counterpart. */
 int dmpm_step_between(dmpm_plan* plan, const void* pbuf_in, void* pbuf_out, int32_t* eid,
                       void* nbuf, int32_t scheme, int32_t velocity_update,
                       const double* gravity, double dt, int64_t step, int32_t flags,
-                      void* state_out, int64_t state_ld, void* stream);
+                      const void* state_in, void* state_out, int64_t state_ld, void* stream);
 
 /* ---- adjoint of the fused 2D step ------------------------------------------
  * The reference differentiates a rollout with jax.grad (no hand-written

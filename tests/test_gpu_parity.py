@@ -444,12 +444,14 @@ def test_one_launch_step_crowded_and_outside(scheme):
     _run_pair(spec, scheme, 3, 3)
 
 
+@pytest.mark.parametrize("split", [False, True])
 @pytest.mark.parametrize("scheme", ["usf", "usl"])
-def test_step_between_blocks_equals_run(scheme):
+def test_step_between_blocks_equals_run(scheme, split):
     """dmpm_step_between alternating two blocks (chained after the first
     step) gives the states of dmpm_run stepping one block in place, bit for
-    bit, also in its copy of the state rows, and leaves its input block
-    unchanged."""
+    bit, and leaves its input block unchanged: with a copy of the state rows
+    (split=False), and with the state rows read from and written to state
+    tensors instead of the blocks (split=True)."""
     from diffmpm_amd import _lib
     from diffmpm_amd._lib import call, ptr
     from diffmpm_amd._plan import Plan, plan_for
@@ -463,25 +465,27 @@ def test_step_between_blocks_equals_run(scheme):
     eid = psb._eid.clone()
     code = _lib.DMPM_USF if scheme == "usf" else _lib.DMPM_USL
     n = len(psb)
-    rows = [_lib.P_X, _lib.P_X + 1, _lib.P_V, _lib.P_V + 1, _lib.P_VOLUME, _lib.P_DENSITY,
-            _lib.P_STRAIN, _lib.P_STRAIN + 1, _lib.P_STRAIN + 3, _lib.P_STRESS,
-            _lib.P_STRESS + 1, _lib.P_STRESS + 2, _lib.P_STRESS + 3, _lib.P_VOLSTRAIN_C]
+    state_rows = [_lib.P_X, _lib.P_X + 1, _lib.P_V, _lib.P_V + 1, _lib.P_STRESS,
+                  _lib.P_STRESS + 1, _lib.P_STRESS + 3, _lib.P_VOLUME]
+    other_rows = [_lib.P_DENSITY, _lib.P_STRAIN, _lib.P_STRAIN + 1, _lib.P_STRAIN + 3,
+                  _lib.P_STRESS + 2, _lib.P_VOLSTRAIN_C]
+    sin = torch.stack([psb._buf[r, :n] for r in state_rows]).contiguous()
     for k in range(7):
         src, dst = blocks[k % 2], blocks[1 - k % 2]
-        before = src.clone()
+        before, sin_before = src.clone(), sin.clone()
         pa.run(scheme, False, g, spec["dt"], k, 1, False)
         sout = torch.full((8, n), float("nan"), dtype=src.dtype, device=src.device)
-        call("dmpm_step_between", pb.h, ptr(src), ptr(dst), ptr(eid), ptr(b.mesh.elements.nodes._buf),
-             code, 0, pb._gravity(g), float(spec["dt"]), k, 2 if k else 0, ptr(sout), n, pb.s)
-        assert torch.equal(src, before)
+        call("dmpm_step_between", pb.h, ptr(src), ptr(dst), ptr(eid),
+             ptr(b.mesh.elements.nodes._buf), code, 0, pb._gravity(g), float(spec["dt"]), k,
+             2 if k else 0, ptr(sin) if split else None, ptr(sout), n, pb.s)
+        assert torch.equal(src, before) and torch.equal(sin, sin_before)
         ref = a.mesh.particles[0]._buf
-        for r in rows:
+        for r in other_rows + ([] if split else state_rows):
             assert torch.equal(dst[r, :n].view(torch.int32), ref[r, :n].view(torch.int32)), (k, r)
-        state_rows = [_lib.P_X, _lib.P_X + 1, _lib.P_V, _lib.P_V + 1, _lib.P_STRESS,
-                      _lib.P_STRESS + 1, _lib.P_STRESS + 3, _lib.P_VOLUME]
         for q, r in enumerate(state_rows):
             assert torch.equal(sout[q].view(torch.int32), ref[r, :n].view(torch.int32)), (k, q)
         assert torch.equal(eid[:n], a.mesh.particles[0]._eid[:n])
         na, nb = a.mesh.elements.nodes, b.mesh.elements.nodes
         m = na._count
         assert torch.equal(na._buf[:, :m].view(torch.int32), nb._buf[:, :m].view(torch.int32))
+        sin = sout
