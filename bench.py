@@ -132,8 +132,8 @@ def ensemble_bench(K):
 
 def fwd_adjoint(sim, gravity, K):
     """Throughput of a differentiable rollout: K forward steps recorded for
-    autograd, then the backward pass (per step: recomputation of the forward
-    step and the adjoint kernels), per-particle log E as the parameter."""
+    autograd (each keeps what its adjoint needs), then the backward pass (the
+    adjoint kernels per step), per-particle log E as the parameter."""
     import math
     import torch
     from diffmpm_amd.autodiff import Differentiable, rollout
@@ -173,7 +173,8 @@ def fwd_adjoint(sim, gravity, K):
             "backward_ms_per_step": bwd_ms / K,
             "adjoint_kernels_us_per_step": adj_us,
             "adjoint_kernels_GBs": ADJOINT_BYTES * N / (adj_us / 1e6) / 1e9,
-            "note": "forward steps keep the node block, kinematic nodal velocity and sort "
+            "note": "forward steps (one-launch step between two blocks, state rows in the "
+                    "autograd tensors) keep the node block, kinematic nodal velocity and sort "
                     "order their adjoint needs; backward = adjoint kernels; per-particle log "
                     "E; loss = sum of final positions"}
 
