@@ -106,6 +106,7 @@ def ensemble_bench(K):
     for p, s in zip(plans, sims):  # warm-up (workspaces, first full sort)
         p.run("usl", False, g, s.dt, 0, 2, keep_temporaries=False)
     ens = Ensemble(sims)
+    tuned = ens.autotune(g)
     ens.run(2, g, start_step=2)
     torch.cuda.synchronize()
     stream = torch.cuda.current_stream()
@@ -127,7 +128,9 @@ def ensemble_bench(K):
                         "ms_per_step": tb / K},
             "sequential": {"value": ntot * K / (ts / 1e3), "unit": "particle-steps/s",
                            "ms_per_step": ts / K},
-            "speedup": ts / tb}
+            "speedup": ts / tb,
+            "autotune_ms_per_step": {str(k): v for k, v in tuned.items()},
+            "cells_per_workgroup": ens.cells_per_block}
 
 
 def fwd_adjoint(sim, gravity, K):

@@ -116,3 +116,21 @@ def test_batch_plan_rejects_per_method_calls():
     with pytest.raises(ValueError, match="batch plans"):
         _lib.call("dmpm_set_particle_element_ids", ens.plan.h, _lib.ptr(ens.pbuf),
                   _lib.ptr(ens.eid), ens.plan.s)
+
+
+def test_autotune_keeps_state_and_results():
+    """Ensemble.autotune leaves the ensemble's state alone, picks one of the
+    candidates, and the tuned ensemble steps to the same states as an
+    untuned one."""
+    from diffmpm_amd.ensemble import Ensemble
+    g = _grav(_spec(*MEMBERS[0], "usl"))
+    a = Ensemble(_members("usl", torch.float32))
+    b = Ensemble(_members("usl", torch.float32))
+    before = a.pbuf.clone()
+    times = a.autotune(g, steps=2)
+    assert set(times) == {64, 128, 256} and a.cells_per_block in times
+    assert torch.equal(a.pbuf, before)
+    a.run(6, g)
+    b.run(6, g)
+    assert torch.equal(a.pbuf.view(torch.int32), b.pbuf.view(torch.int32))
+    assert torch.equal(a.eid, b.eid)
