@@ -813,13 +813,29 @@ __device__ __forceinline__ int seg_row(const int* P, int h, int v) {
   return lo;
 }
 
-// Tile of workgroup b: the tiles (row-major) are split into 8 contiguous
-// ranges, one per XCD (workgroups are dispatched to the XCDs round-robin), so
-// neighbouring tiles share an L2.
-__device__ __forceinline__ int tile_of(int b, int ntiles) {
+// Tile of workgroup b.  The tiles are ordered in strips of kStrip tile
+// columns (row-major within a strip), and that order is split into 8
+// contiguous ranges, one per XCD (workgroups are dispatched to the XCDs
+// round-robin): the tiles an XCD runs at the same time form a block of about
+// kStrip x kStrip tiles whose halos overlap, and share its L2.
+constexpr int kStrip = 8;
+__device__ __forceinline__ int tile_of(int b, int ntiles, int tiles_x) {
   const int q = ntiles >> 3, r = ntiles & 7;
   const int x = b & 7, k = b >> 3;
-  return x * q + min(x, r) + k;
+  const int g = x * q + min(x, r) + k;  // position in strip order
+  const int tiles_y = ntiles / tiles_x;
+  const int full = tiles_x / kStrip, per = kStrip * tiles_y;
+  int row, col;
+  if (g < full * per) {
+    const int st = g / per, rem = g - st * per;
+    row = rem / kStrip;
+    col = st * kStrip + (rem - row * kStrip);
+  } else {
+    const int w = tiles_x - full * kStrip, rem = g - full * per;
+    row = rem / w;
+    col = full * kStrip + (rem - row * w);
+  }
+  return row * tiles_x + col;
 }
 
 // KIN cell sum over staged slots [lo, hi) (k_cell_kin's expressions).
@@ -1101,7 +1117,7 @@ __global__ __launch_bounds__(kFT) __attribute__((amdgpu_waves_per_eu(4))) void k
   const int* __restrict__ perm = a.sr.perm();
   const int ident = a.sr.flags[1];
 
-  const int tile = tile_of(blockIdx.x, a.ntiles);
+  const int tile = tile_of(blockIdx.x, a.ntiles, a.tiles_x);
   const int ti = tile % a.tiles_x, tj = tile / a.tiles_x;
   const Rgn R0{ti * kTW, min(ti * kTW + kTW, nx), tj * kTH, min(tj * kTH + kTH, ny)};
   const Rgn R1 = grow(R0, 1, nx, ny);
