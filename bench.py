@@ -130,7 +130,8 @@ def ensemble_bench(K):
                            "ms_per_step": ts / K},
             "speedup": ts / tb,
             "autotune_ms_per_step": {str(k): v for k, v in tuned.items()},
-            "cells_per_workgroup": ens.cells_per_block}
+            "batch_step": "one launch" if ens.step_kernel else
+                          "per phase, %d cells per workgroup" % ens.cells_per_block}
 
 
 def fwd_adjoint(sim, gravity, K):

@@ -838,7 +838,7 @@ void plan_free(dmpm_plan* P) {
   void* ptrs[] = {P->key, P->perm, P->counts, P->fill, P->cs, P->bsum, P->flags, P->zflags,
                   P->big,
                   P->perm2, P->cs2, P->mov, P->mrec, P->dL, P->dA, P->headL, P->headA,
-                  P->chunkD, P->chunkC, P->sims,
+                  P->chunkD, P->chunkC, P->sims, P->tile_sim,
                   P->con_node, P->con_dir, P->con_val, P->nf_id, P->nf_dir, P->nf_off,
                   P->nf_fn, P->nf_val, P->tr_id, P->tr_dir, P->tr_off, P->tr_fn, P->tr_val,
                   P->fn_kind, P->fn_off, P->fn_cnt, P->fn_x, P->fn_f, P->cidx, P->cmask,
@@ -944,6 +944,28 @@ static int plan_init(dmpm_plan* P, dmpm_plan** out) {
   if (rc == DMPM_OK) rc = hip_check(hipMemset(P->cs, 0, (nb + 1) * sizeof(int)), "hipMemset");
   if (rc == DMPM_OK) rc = hip_check(hipMemset(P->perm, 0, n * sizeof(int)), "hipMemset");
   P->nsims = (int)P->sims_h.size();
+  // tiles of the one-launch step, numbered simulation after simulation (row
+  // major within a mesh)
+  int64_t ntiles = 0;
+  for (SimRec& r : P->sims_h) {
+    r.tiles_x = (r.nx + kStepTile - 1) / kStepTile;
+    r.tile_off = (int)ntiles;
+    ntiles += (int64_t)r.tiles_x * ((r.ny + kStepTile - 1) / kStepTile);
+  }
+  P->ntiles = (int)ntiles;
+  if (rc == DMPM_OK && P->nsims > 1) {
+    std::vector<int> ts((size_t)ntiles);
+    for (int s = 0; s < P->nsims; ++s) {
+      const SimRec& r = P->sims_h[s];
+      const int64_t end = s + 1 < P->nsims ? P->sims_h[s + 1].tile_off : ntiles;
+      for (int64_t k = r.tile_off; k < end; ++k) ts[(size_t)k] = s;
+    }
+    al(&P->tile_sim, ntiles);
+    if (rc == DMPM_OK)
+      rc = hip_check(hipMemcpy(P->tile_sim, ts.data(), ts.size() * sizeof(int),
+                               hipMemcpyHostToDevice),
+                     "hipMemcpy");
+  }
   if (rc == DMPM_OK)
     rc = hip_check(hipMalloc((void**)&P->sims, P->sims_h.size() * sizeof(SimRec)), "hipMalloc");
   if (rc == DMPM_OK)

@@ -8,12 +8,15 @@
 
 #include "../../include/diffmpm_amd.h"
 
+// Cells per side of the square tiles of the one-launch step.
+constexpr int kStepTile = 16;
+
 // One simulation of a plan (device table; 64 bytes).
 struct SimRec {
   int nx, ny;                         // cells per axis
   int cell_off, node_off, part_off;   // first global cell, node, particle
   int n;                              // particles
-  int pad0, pad1;
+  int tile_off, tiles_x;              // one-launch step: first global tile, tile columns
   double hx, hy;
   double de[3];                       // material coefficients
   double pad2;
@@ -28,6 +31,8 @@ struct dmpm_plan {
   int nsims = 1;
   SimRec* sims = nullptr;          // [nsims] device table
   std::vector<SimRec> sims_h;      // host copy
+  int ntiles = 0;                  // tiles of the one-launch step, all simulations
+  int* tile_sim = nullptr;         // [ntiles] simulation of each tile (batch plans)
   int device = 0;
   // counting-sort workspace
   int* key = nullptr;     // [n] cell of each particle, ncells = outside the mesh

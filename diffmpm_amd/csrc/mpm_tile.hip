@@ -626,7 +626,7 @@ __global__ __launch_bounds__(kThreads) void k_stress_tile(Mesh2<T> M, T* __restr
 // particle data is loaded once, at the start, into registers; the phases
 // exchange per-particle values through LDS only.  Otherwise (crowded tiles,
 // FP64) the phases stage the slots in rounds and reload what they need.
-constexpr int kTW = 16, kTH = 16;  // cells per tile
+constexpr int kTW = kStepTile, kTH = kStepTile;  // cells per tile
 constexpr int kFT = 512;           // threads per workgroup
 constexpr int kPPT = 4;            // fast path: slots of R_H per thread
 constexpr int kKCAP = kFT * kPPT;  // fast path: slots of R_H
@@ -722,6 +722,9 @@ template <typename T> struct StepArgs {
   int* ovf;  // [2] overflow counts by step parity, then [2][ntiles] tile lists
   int par;   // step parity
   int nrb;   // workgroups of k_step_rounds that process tiles
+  const SimRec* sims;   // batch plans: the simulation table (tile_off, tiles_x, ...)
+  const int* tile_sim;  // batch plans: the simulation of each tile
+  int nc_all, nn_all, nsims;  // cells, nodes and simulations of the plan
   T* sout;   // optional: the state rows x, y, vx, vy, sxx, syy, sxy, V after the step
   const T* sin;  // optional: the state rows before the step (then read from here, and
                  // written only to sout)
@@ -813,16 +816,21 @@ __device__ __forceinline__ int seg_row(const int* P, int h, int v) {
   return lo;
 }
 
-// Tile of workgroup b.  The tiles are ordered in strips of kStrip tile
-// columns (row-major within a strip), and that order is split into 8
+// Position of workgroup b in the tile numbering: the tiles are split into 8
 // contiguous ranges, one per XCD (workgroups are dispatched to the XCDs
-// round-robin): the tiles an XCD runs at the same time form a block of about
-// kStrip x kStrip tiles whose halos overlap, and share its L2.
-constexpr int kStrip = 8;
-__device__ __forceinline__ int tile_of(int b, int ntiles, int tiles_x) {
+// round-robin), so that an XCD steps neighbouring tiles.
+__device__ __forceinline__ int xcd_pos(int b, int ntiles) {
   const int q = ntiles >> 3, r = ntiles & 7;
   const int x = b & 7, k = b >> 3;
-  const int g = x * q + min(x, r) + k;  // position in strip order
+  return x * q + min(x, r) + k;
+}
+
+// Row-major tile of position g of a mesh's strip order: strips of kStrip tile
+// columns, row-major within a strip, so that the tiles an XCD runs at the
+// same time form a block of about kStrip x kStrip tiles whose halos overlap
+// and share its L2.
+constexpr int kStrip = 8;
+__device__ __forceinline__ int strip_tile(int g, int ntiles, int tiles_x) {
   const int tiles_y = ntiles / tiles_x;
   const int full = tiles_x / kStrip, per = kStrip * tiles_y;
   int row, col;
@@ -836,6 +844,50 @@ __device__ __forceinline__ int tile_of(int b, int ntiles, int tiles_x) {
     col = full * kStrip + (rem - row * w);
   }
   return row * tiles_x + col;
+}
+
+// The tile a workgroup steps and its simulation: mesh, material, first global
+// cell and node.  tile is the global tile id (the simulation's first tile plus
+// the row-major tile of its mesh); (ti, tj) its column and row in the mesh.
+template <typename T> struct TileCtx {
+  Grid2<T> g;
+  Mat2 mat;
+  int coff, noff, tile, ti, tj;
+};
+// Context of global tile position gid (B: batch plan), read as a position of
+// each mesh's strip order (strip) or as a global tile id.
+template <typename T, bool B>
+__device__ __forceinline__ TileCtx<T> tile_ctx(const StepArgs<T>& a, int gid, bool strip) {
+  TileCtx<T> c;
+  int base = 0, tx = a.tiles_x, nt = a.ntiles;
+  if constexpr (B) {
+    const SimRec& r = a.sims[a.tile_sim[gid]];
+    c.g.nx = r.nx;
+    c.g.ny = r.ny;
+    c.g.hx = T(r.hx);
+    c.g.hy = T(r.hy);
+    c.g.ihx = T(1) / c.g.hx;
+    c.g.ihy = T(1) / c.g.hy;
+    c.g.ncells = r.nx * r.ny;
+    c.g.nnodes = (r.nx + 1) * (r.ny + 1);
+    c.mat = a.mat;
+    for (int k = 0; k < 3; ++k) c.mat.de[k] = r.de[k];
+    c.coff = r.cell_off;
+    c.noff = r.node_off;
+    base = r.tile_off;
+    tx = r.tiles_x;
+    nt = tx * ((r.ny + kTH - 1) / kTH);
+  } else {
+    c.g = a.g;
+    c.mat = a.mat;
+    c.coff = 0;
+    c.noff = 0;
+  }
+  const int lt = strip ? strip_tile(gid - base, nt, tx) : gid - base;
+  c.ti = lt % tx;
+  c.tj = lt / tx;
+  c.tile = base + lt;
+  return c;
 }
 
 // KIN cell sum over staged slots [lo, hi) (k_cell_kin's expressions).
@@ -929,16 +981,16 @@ template <int NPT> struct NodePre {
   int cr[NPT], mask[NPT];
 };
 template <typename T, int NPT>
-__device__ __forceinline__ void node_pre(const StepArgs<T>& a, const Rgn& RN, NodePre<NPT>& q,
-                                         T (*fd)[2]) {
-  const int t = threadIdx.x, nx = a.g.nx;
+__device__ __forceinline__ void node_pre(const StepArgs<T>& a, const TileCtx<T>& c, const Rgn& RN,
+                                         NodePre<NPT>& q, T (*fd)[2]) {
+  const int t = threadIdx.x, nx = c.g.nx;
   const int wn = RN.i1 - RN.i0 + 1, nn = wn * (RN.j1 - RN.j0 + 1);
   const SmallDiv dwn = small_div(wn);
 #pragma unroll
   for (int w = 0; w < NPT; ++w) {
     const int ln = min(t + w * kFT, nn - 1);
     const int jr = sdiv(dwn, ln);
-    const int nd = (RN.j0 + jr) * (nx + 1) + RN.i0 + (ln - jr * wn);
+    const int nd = c.noff + (RN.j0 + jr) * (nx + 1) + RN.i0 + (ln - jr * wn);
     q.cr[w] = a.cidx ? a.cidx[nd] : -1;
     q.mask[w] = 0;
     if (fd) {
@@ -956,10 +1008,11 @@ __device__ __forceinline__ void node_pre_mask(const StepArgs<T>& a, NodePre<NPT>
 // Kinematic nodal epilogue (k_node_kin) of the nodes of RN: nodal mass and
 // velocity into LDS, the velocity of the tile's own nodes to a.vkin.
 template <typename T, int NPT>
-__device__ __forceinline__ void kin_epilogue(const StepArgs<T>& a, const T (&nacc)[NPT][NQ_KIN],
+__device__ __forceinline__ void kin_epilogue(const StepArgs<T>& a, const TileCtx<T>& c,
+                                             const T (&nacc)[NPT][NQ_KIN],
                                              const NodePre<NPT>& pre, const Rgn& RN,
                                              const Rgn& R0, T* vk, T* nm, int nkn) {
-  const int t = threadIdx.x, nx = a.g.nx, ny = a.g.ny, nnodes = a.g.nnodes;
+  const int t = threadIdx.x, nx = c.g.nx, ny = c.g.ny, nnodes = a.nn_all;
   const int wn = RN.i1 - RN.i0 + 1, nn = wn * (RN.j1 - RN.j0 + 1);
   const SmallDiv dwn = small_div(wn);
 #pragma unroll
@@ -968,7 +1021,7 @@ __device__ __forceinline__ void kin_epilogue(const StepArgs<T>& a, const T (&nac
     if (ln >= nn) continue;
     const int jr = sdiv(dwn, ln);
     const int i = RN.i0 + (ln - jr * wn), j = RN.j0 + jr;
-    const int nd = j * (nx + 1) + i;
+    const int nd = c.noff + j * (nx + 1) + i;
     const int cr = pre.cr[w], mask = pre.mask[w];
     const T m = nacc[w][0];
     const T p[2] = {snap(nacc[w][1]), snap(nacc[w][2])};
@@ -988,12 +1041,12 @@ __device__ __forceinline__ void kin_epilogue(const StepArgs<T>& a, const T (&nac
 // Force nodal epilogue (k_node_force) of the tile's nodes: acceleration and
 // velocity into LDS; the tile's own nodes are written to the node block.
 template <typename T, int NPT>
-__device__ __forceinline__ void force_epilogue(const StepArgs<T>& a,
+__device__ __forceinline__ void force_epilogue(const StepArgs<T>& a, const TileCtx<T>& c,
                                                const T (&nacc)[NPT][NQ_FORCE],
                                                const NodePre<NPT>& pre, const T (*fd)[2],
                                                const Rgn& R0, const Rgn& RN, const T* vk,
                                                const T* nm, int nkn, T* nv, T* na, int ntn) {
-  const int t = threadIdx.x, nx = a.g.nx, ny = a.g.ny, nnodes = a.g.nnodes;
+  const int t = threadIdx.x, nx = c.g.nx, ny = c.g.ny, nnodes = a.nn_all;
   const int wt = R0.i1 - R0.i0 + 1, nt = wt * (R0.j1 - R0.j0 + 1);
   const int wkn = RN.i1 - RN.i0 + 1;
   const SmallDiv dwt = small_div(wt);
@@ -1003,7 +1056,7 @@ __device__ __forceinline__ void force_epilogue(const StepArgs<T>& a,
     if (ln >= nt) continue;
     const int jr = sdiv(dwt, ln);
     const int i = R0.i0 + (ln - jr * wt), j = R0.j0 + jr;
-    const int nd = j * (nx + 1) + i;
+    const int nd = c.noff + j * (nx + 1) + i;
     const int lk = (i - RN.i0) + (j - RN.j0) * wkn;
     const bool own = (i < R0.i1 || i == nx) && (j < R0.j1 || j == ny);
     const int cr = pre.cr[w], mask = pre.mask[w];
@@ -1049,8 +1102,9 @@ __device__ __forceinline__ void force_epilogue(const StepArgs<T>& a,
 // Virtual slot range [v0, v1) of each of this thread's cells of region R
 // (table q of the slot segments).
 template <int C>
-__device__ __forceinline__ void cell_ranges(const Rgn& R, int nx, const int* cs, const int* A,
-                                            const int* P, int (&v0)[C], int (&v1)[C]) {
+__device__ __forceinline__ void cell_ranges(const Rgn& R, int nx, int coff, const int* cs,
+                                            const int* A, const int* P, int (&v0)[C],
+                                            int (&v1)[C]) {
   const int w = R.i1 - R.i0, nc = w * (R.j1 - R.j0);
   const SmallDiv dw = small_div(w);
 #pragma unroll
@@ -1059,7 +1113,7 @@ __device__ __forceinline__ void cell_ranges(const Rgn& R, int nx, const int* cs,
     v0[u] = v1[u] = 0;
     if (lc < nc) {
       const int r = sdiv(dw, lc);
-      const int k = (R.j0 + r) * nx + R.i0 + (lc - r * w);
+      const int k = coff + (R.j0 + r) * nx + R.i0 + (lc - r * w);
       const int s0 = cs[k];
       v0[u] = P[r] + (s0 - A[r]);
       v1[u] = v0[u] + (cs[k + 1] - s0);
@@ -1088,7 +1142,7 @@ __device__ __forceinline__ void stress_update(const Mat2& mat, T E_p, T ix, T iy
 // The step for the tiles whose slots fit the fast path (FP32); a tile that
 // does not fit appends itself to the overflow list of this step for
 // k_step_rounds.
-template <typename T, bool USF>
+template <typename T, bool USF, bool B>
 __global__ __launch_bounds__(kFT) __attribute__((amdgpu_waves_per_eu(4))) void k_step_fast(
     StepArgs<T> a) {
   constexpr int H = USF ? 2 : 1;
@@ -1110,15 +1164,15 @@ __global__ __launch_bounds__(kFT) __attribute__((amdgpu_waves_per_eu(4))) void k
   __shared__ T na[2 * NTN];
   __shared__ T colC[NCOL], colS[NCOL], rowC[NROW], rowS[NROW];
   __shared__ int segA[3][kSegRows], segP[3][kSegRows + 1];
-  const Grid2<T> g = a.g;
+  const TileCtx<T> c = tile_ctx<T, B>(a, xcd_pos(blockIdx.x, a.ntiles), true);
+  const Grid2<T> g = c.g;
   const int nx = g.nx, ny = g.ny;
   const int t = threadIdx.x;
   const int* __restrict__ cs = a.sr.cs();
   const int* __restrict__ perm = a.sr.perm();
   const int ident = a.sr.flags[1];
 
-  const int tile = tile_of(blockIdx.x, a.ntiles, a.tiles_x);
-  const int ti = tile % a.tiles_x, tj = tile / a.tiles_x;
+  const int tile = c.tile, ti = c.ti, tj = c.tj;
   const Rgn R0{ti * kTW, min(ti * kTW + kTW, nx), tj * kTH, min(tj * kTH + kTH, ny)};
   const Rgn R1 = grow(R0, 1, nx, ny);
   const Rgn RH = grow(R0, H, nx, ny);
@@ -1133,7 +1187,7 @@ __global__ __launch_bounds__(kFT) __attribute__((amdgpu_waves_per_eu(4))) void k
       const Rgn R = q == 0 ? R0 : (q == 1 ? R1 : RH);
       int len = 0;
       if (r < R.j1 - R.j0) {
-        const int k = (R.j0 + r) * nx + R.i0;
+        const int k = c.coff + (R.j0 + r) * nx + R.i0;
         const int s0 = cs[k];
         segA[q][r] = s0;
         len = cs[k + (R.i1 - R.i0)] - s0;
@@ -1177,8 +1231,8 @@ __global__ __launch_bounds__(kFT) __attribute__((amdgpu_waves_per_eu(4))) void k
   NodePre<NK> kpre;
   NodePre<NF> fpre;
   T fd[NF][2];
-  node_pre<T, NK>(a, RN, kpre, nullptr);
-  node_pre<T, NF>(a, R0, fpre, fd);
+  node_pre<T, NK>(a, c, RN, kpre, nullptr);
+  node_pre<T, NF>(a, c, R0, fpre, fd);
   // slot of each virtual index t + j*kFT of the R_H list (lanes past the
   // end take the last slot and are flagged off): binary searches over the
   // rows, in lockstep
@@ -1272,7 +1326,7 @@ __global__ __launch_bounds__(kFT) __attribute__((amdgpu_waves_per_eu(4))) void k
     }
   }
   int kv0[CK], kv1[CK];
-  cell_ranges<CK>(RH, nx, cs, segA[2], segP[2], kv0, kv1);
+  cell_ranges<CK>(RH, nx, c.coff, cs, segA[2], segP[2], kv0, kv1);
   node_pre_mask<T, NK>(a, kpre);
   node_pre_mask<T, NF>(a, fpre);
   // ---- K ----
@@ -1312,10 +1366,10 @@ __global__ __launch_bounds__(kFT) __attribute__((amdgpu_waves_per_eu(4))) void k
     __syncthreads();
     T nacc[NK][NQ_KIN];
     node_sums<T, NQ_KIN, CK, NK>(st, acc, RH, RN, nx, ny, nacc);
-    kin_epilogue<T, NK>(a, nacc, kpre, RN, R0, vk, nm, NKN);
+    kin_epilogue<T, NK>(a, c, nacc, kpre, RN, R0, vk, nm, NKN);
   }
   int fv0[CF], fv1[CF];
-  cell_ranges<CF>(R1, nx, cs, segA[1], segP[1], fv0, fv1);
+  cell_ranges<CF>(R1, nx, c.coff, cs, segA[1], segP[1], fv0, fv1);
   __syncthreads();
   // ---- S: stresses (USF); F inputs staged by R_1 slot ----
 #pragma unroll
@@ -1338,7 +1392,7 @@ __global__ __launch_bounds__(kFT) __attribute__((amdgpu_waves_per_eu(4))) void k
       }
       const int ci = min(max(I - RH.i0, 0), NCOL - 1), rj = min(max(J - RH.j0, 0), NROW - 1);
       StressUpd<T> u;
-      stress_update(a.mat, E, colS[ci], rowS[rj], xi[j], eta[j], nvx, nvy, a.dt, u);
+      stress_update(c.mat, E, colS[ci], rowS[rj], xi[j], eta[j], nvx, nvy, a.dt, u);
       sx = sxx + u.ds[0];
       sy = syy + u.ds[1];
       sq = sxy + u.ds[3];
@@ -1393,7 +1447,7 @@ __global__ __launch_bounds__(kFT) __attribute__((amdgpu_waves_per_eu(4))) void k
     __syncthreads();
     T nacc[NF][NQ_FORCE];
     node_sums<T, NQ_FORCE, CF, NF>(st, acc, R1, R0, nx, ny, nacc);
-    force_epilogue<T, NF>(a, nacc, fpre, fd, R0, RN, vk, nm, NKN, nv, na, NTN);
+    force_epilogue<T, NF>(a, c, nacc, fpre, fd, R0, RN, vk, nm, NKN, nv, na, NTN);
   }
   __syncthreads();
   // ---- G ----
@@ -1432,7 +1486,7 @@ __global__ __launch_bounds__(kFT) __attribute__((amdgpu_waves_per_eu(4))) void k
       const int ci = min(max(I - RH.i0, 0), NCOL - 1), rj = min(max(J - RH.j0, 0), NROW - 1);
       const int q = v1i[j];
       StressUpd<T> u;
-      stress_update(a.mat, s1f[4 * kFCAP + q], colS[ci], rowS[rj], xi[j], eta[j], gvx, gvy,
+      stress_update(c.mat, s1f[4 * kFCAP + q], colS[ci], rowS[rj], xi[j], eta[j], gvx, gvy,
                     a.dt, u);
       DR_W(DMPM_P_STRAIN + 0, p, ex[j] + u.d[0]);
       DR_W(DMPM_P_STRAIN + 1, p, ey[j] + u.d[1]);
@@ -1452,19 +1506,19 @@ __global__ __launch_bounds__(kFT) __attribute__((amdgpu_waves_per_eu(4))) void k
         PB_W(DMPM_P_DVOLSTRAIN, p, u.dvol);
       }
     }
-    const int e0 = J * nx + I;
+    const int k0 = c.coff + J * nx + I;
     // still inside cell (I, J): no element search (the same element id)
     if (x1 >= g.X(I) && x1 < g.X(I + 1) && y1 >= g.Y(J) && y1 < g.Y(J + 1)) continue;
     const int e1 = element_id<T, 2>(g.geo(), x1, y1);
-    const int k1 = (e1 >= 0 && e1 < g.ncells) ? e1 : g.ncells;
-    if (k1 != e0) record_move(a.aux->L, p, e0, e1, k1);
+    const int k1 = (e1 >= 0 && e1 < g.ncells) ? c.coff + e1 : a.nc_all;
+    if (k1 != k0) record_move(a.aux->L, p, k0, e1, k1);
   }
 }
 
 // The step for the tiles of the overflow list of this step (all tiles when
 // ovf is null: FP64), slots staged in rounds; workgroups from nrb on handle
 // the particles outside the mesh.
-template <typename T, bool USF>
+template <typename T, bool USF, bool B>
 __global__ __launch_bounds__(kFT) void k_step_rounds(StepArgs<T> a) {
   constexpr int H = USF ? 2 : 1;
   constexpr int CK = ((kTW + 2 * H) * (kTH + 2 * H) + kFT - 1) / kFT;  // cells/thread, K
@@ -1483,8 +1537,6 @@ __global__ __launch_bounds__(kFT) void k_step_rounds(StepArgs<T> a) {
   __shared__ T na[2 * NTN];
   __shared__ T colC[NCOL], colS[NCOL], rowC[NROW], rowS[NROW];
   __shared__ int segA[3][kSegRows], segP[3][kSegRows + 1];
-  const Grid2<T> g = a.g;
-  const int nx = g.nx, ny = g.ny;
   const int t = threadIdx.x;
   const int* __restrict__ cs = a.sr.cs();
   const int* __restrict__ perm = a.sr.perm();
@@ -1494,7 +1546,7 @@ __global__ __launch_bounds__(kFT) void k_step_rounds(StepArgs<T> a) {
     // particles outside the mesh: the state becomes NaN (as in k_g2p_tile)
     const T nan = qnan<T>();
     const int gs = kTailBlocks * kFT;
-    for (int s = cs[g.ncells] + ((int)blockIdx.x - a.nrb) * kFT + t; s < a.n; s += gs) {
+    for (int s = cs[a.nc_all] + ((int)blockIdx.x - a.nrb) * kFT + t; s < a.n; s += gs) {
       const int p = ident ? s : perm[s];
       for (int k = 0; k < kDynRows; ++k) DR_W(kDyn[k], p, nan);
       for (int k = 0; k < kFixRows; ++k) PB_W(kFix[k], p, nan);
@@ -1509,8 +1561,19 @@ __global__ __launch_bounds__(kFT) void k_step_rounds(StepArgs<T> a) {
         }
         PB_W(DMPM_P_DVOLSTRAIN, p, nan);
       }
-      // NaN positions search to element 0 (element.py:827-837)
-      record_move(a.aux->L, p, g.ncells, 0, 0);
+      // NaN positions search to element 0 (element.py:827-837) of the
+      // particle's own mesh
+      int c0 = 0;
+      if constexpr (B) {
+        int lo = 0, hi = a.nsims - 1;
+        while (lo < hi) {
+          const int mid = (lo + hi + 1) >> 1;
+          if (a.sims[mid].part_off <= p) lo = mid;
+          else hi = mid - 1;
+        }
+        c0 = a.sims[lo].cell_off;
+      }
+      record_move(a.aux->L, p, a.nc_all, 0, c0);
     }
     return;
   }
@@ -1520,7 +1583,10 @@ __global__ __launch_bounds__(kFT) void k_step_rounds(StepArgs<T> a) {
   for (int qt = blockIdx.x; qt < cnt; qt += a.nrb) {
     const int tile = a.ovf ? a.ovf[2 + a.par * a.ntiles + qt] : qt;
     __syncthreads();  // the previous tile's LDS
-    const int ti = tile % a.tiles_x, tj = tile / a.tiles_x;
+    const TileCtx<T> c = tile_ctx<T, B>(a, tile, false);
+    const Grid2<T> g = c.g;
+    const int nx = g.nx, ny = g.ny;
+    const int ti = c.ti, tj = c.tj;
     const Rgn R0{ti * kTW, min(ti * kTW + kTW, nx), tj * kTH, min(tj * kTH + kTH, ny)};
     const Rgn R1 = grow(R0, 1, nx, ny);
     const Rgn RH = grow(R0, H, nx, ny);
@@ -1535,7 +1601,7 @@ __global__ __launch_bounds__(kFT) void k_step_rounds(StepArgs<T> a) {
         const Rgn R = q == 0 ? R0 : (q == 1 ? R1 : RH);
         int len = 0;
         if (r < R.j1 - R.j0) {
-          const int k = (R.j0 + r) * nx + R.i0;
+          const int k = c.coff + (R.j0 + r) * nx + R.i0;
           const int s0 = cs[k];
           segA[q][r] = s0;
           len = cs[k + (R.i1 - R.i0)] - s0;
@@ -1572,7 +1638,7 @@ __global__ __launch_bounds__(kFT) void k_step_rounds(StepArgs<T> a) {
     // ---- K ----
     {
       int kv0[CK], kv1[CK];
-      cell_ranges<CK>(RH, nx, cs, segA[2], segP[2], kv0, kv1);
+      cell_ranges<CK>(RH, nx, c.coff, cs, segA[2], segP[2], kv0, kv1);
       T acc[CK][4 * NQ_KIN];
   #pragma unroll
       for (int u = 0; u < CK; ++u)
@@ -1628,17 +1694,17 @@ __global__ __launch_bounds__(kFT) void k_step_rounds(StepArgs<T> a) {
         __syncthreads();
       }
       NodePre<NK> kpre;
-      node_pre<T, NK>(a, RN, kpre, nullptr);
+      node_pre<T, NK>(a, c, RN, kpre, nullptr);
       node_pre_mask<T, NK>(a, kpre);
       T nacc[NK][NQ_KIN];
       node_sums<T, NQ_KIN, CK, NK>(st, acc, RH, RN, nx, ny, nacc);
-      kin_epilogue<T, NK>(a, nacc, kpre, RN, R0, vk, nm, NKN);
+      kin_epilogue<T, NK>(a, c, nacc, kpre, RN, R0, vk, nm, NKN);
       __syncthreads();
     }
     // ---- S + F ----
     {
       int fv0[CF], fv1[CF];
-      cell_ranges<CF>(R1, nx, cs, segA[1], segP[1], fv0, fv1);
+      cell_ranges<CF>(R1, nx, c.coff, cs, segA[1], segP[1], fv0, fv1);
       T acc[CF][4 * NQ_FORCE];
   #pragma unroll
       for (int u = 0; u < CF; ++u)
@@ -1685,7 +1751,7 @@ __global__ __launch_bounds__(kFT) void k_step_rounds(StepArgs<T> a) {
             T ix, iy;
             q4_invJ(xi, eta, X0, X1, Y0, Y1, ix, iy);
             StressUpd<T> u;
-            stress_update(a.mat, a.mat.per_E ? PB(DMPM_P_YOUNGS, p) : T(0), ix, iy, xi, eta, nvx,
+            stress_update(c.mat, a.mat.per_E ? PB(DMPM_P_YOUNGS, p) : T(0), ix, iy, xi, eta, nvx,
                           nvy, a.dt, u);
             sxx = SR(DMPM_P_STRESS + 0, p) + u.ds[0];
             syy = SR(DMPM_P_STRESS + 1, p) + u.ds[1];
@@ -1738,11 +1804,11 @@ __global__ __launch_bounds__(kFT) void k_step_rounds(StepArgs<T> a) {
       }
       NodePre<NF> fpre;
       T fd[NF][2];
-      node_pre<T, NF>(a, R0, fpre, fd);
+      node_pre<T, NF>(a, c, R0, fpre, fd);
       node_pre_mask<T, NF>(a, fpre);
       T nacc[NF][NQ_FORCE];
       node_sums<T, NQ_FORCE, CF, NF>(st, acc, R1, R0, nx, ny, nacc);
-      force_epilogue<T, NF>(a, nacc, fpre, fd, R0, RN, vk, nm, NKN, nv, na, NTN);
+      force_epilogue<T, NF>(a, c, nacc, fpre, fd, R0, RN, vk, nm, NKN, nv, na, NTN);
       __syncthreads();
     }
     // ---- G ----
@@ -1788,7 +1854,7 @@ __global__ __launch_bounds__(kFT) void k_step_rounds(StepArgs<T> a) {
         T ix, iy;
         q4_invJ(xi, eta, X0, X1, Y0, Y1, ix, iy);
         StressUpd<T> u;
-        stress_update(a.mat, a.mat.per_E ? PB(DMPM_P_YOUNGS, p) : T(0), ix, iy, xi, eta, gvx, gvy,
+        stress_update(c.mat, a.mat.per_E ? PB(DMPM_P_YOUNGS, p) : T(0), ix, iy, xi, eta, gvx, gvy,
                       a.dt, u);
         DR_W(DMPM_P_STRAIN + 0, p, SR(DMPM_P_STRAIN + 0, p) + u.d[0]);
         DR_W(DMPM_P_STRAIN + 1, p, SR(DMPM_P_STRAIN + 1, p) + u.d[1]);
@@ -1806,8 +1872,8 @@ __global__ __launch_bounds__(kFT) void k_step_rounds(StepArgs<T> a) {
         }
       }
       const int e1 = element_id<T, 2>(g.geo(), x1, y1);
-      const int k1 = (e1 >= 0 && e1 < g.ncells) ? e1 : g.ncells;
-      if (k1 != e) record_move(a.aux->L, p, e, e1, k1);
+      const int k1 = (e1 >= 0 && e1 < g.ncells) ? c.coff + e1 : a.nc_all;
+      if (k1 != c.coff + e) record_move(a.aux->L, p, c.coff + e, e1, k1);
     }
   }
 }
@@ -2282,11 +2348,11 @@ __global__ void k_zero_ints(int* p, int n) {
   if (i < n) p[i] = 0;
 }
 
-// Whether the one-launch step can run this plan: one simulation, and the
-// particle and node blocks below 4 GiB (the kernels' 32-bit buffer offsets).
+// Whether the one-launch step can run this plan: the particle and node blocks
+// below 4 GiB (the kernels' 32-bit buffer offsets).
 bool step_eligible(const dmpm_plan* P) {
   const uint64_t esz = P->d.dtype == DMPM_F32 ? 4 : 8;
-  return P->step_kernel && P->nsims == 1 &&
+  return P->step_kernel &&
          (uint64_t)DMPM_P_ROWS * (uint64_t)P->d.ld * esz < (1ull << 32) &&
          (uint64_t)DMPM_N_ROWS * (uint64_t)P->d.nld * esz < (1ull << 32);
 }
@@ -2297,6 +2363,19 @@ bool step_eligible(const dmpm_plan* P) {
 // particle state alternates between blocks pa and pb2 (dynamic rows only;
 // pb2 null: the plan's shadow block), starting from pa; pstat holds the
 // static rows (mass, E) and takes the temporaries.
+// The kernels of one step (B: batch plan).
+template <typename T, bool B>
+void step_launch(const StepArgs<T>& a, bool usf, hipStream_t s) {
+  constexpr bool kFast = sizeof(T) == 4;
+  if (usf) {
+    if constexpr (kFast) k_step_fast<T, true, B><<<a.ntiles, kFT, 0, s>>>(a);
+    k_step_rounds<T, true, B><<<a.nrb + kTailBlocks, kFT, 0, s>>>(a);
+  } else {
+    if constexpr (kFast) k_step_fast<T, false, B><<<a.ntiles, kFT, 0, s>>>(a);
+    k_step_rounds<T, false, B><<<a.nrb + kTailBlocks, kFT, 0, s>>>(a);
+  }
+}
+
 template <typename T>
 int step_run(dmpm_plan* P, const Mesh2<T>& M, T* pa, T* pb2, T* pstat, int32_t* eid, T* nb,
              int scheme, int vu, const double* grav, double dt, int64_t step0, int64_t nsteps,
@@ -2306,8 +2385,13 @@ int step_run(dmpm_plan* P, const Mesh2<T>& M, T* pa, T* pb2, T* pstat, int32_t* 
   const int64_t ld = P->d.ld, nld = P->d.nld;
   const int nn = P->nnodes;
   StepArgs<T> a;
-  a.tiles_x = (M.g0.nx + kTW - 1) / kTW;
-  a.ntiles = a.tiles_x * ((M.g0.ny + kTH - 1) / kTH);
+  a.tiles_x = P->sims_h[0].tiles_x;
+  a.ntiles = P->ntiles;
+  a.sims = P->sims;
+  a.tile_sim = P->tile_sim;
+  a.nc_all = P->ncells;
+  a.nn_all = nn;
+  a.nsims = P->nsims;
   T* shadow = pb2 ? pb2 : (T*)plan_workspace(P, 5, (size_t)DMPM_P_ROWS * ld * sizeof(T));
   T* vbuf = (T*)plan_workspace(P, 6, (size_t)4 * nn * sizeof(T));
   int* ovf = kFast ? (int*)plan_workspace(P, 7, (size_t)(2 + 2 * a.ntiles) * sizeof(int)) : nullptr;
@@ -2387,13 +2471,10 @@ int step_run(dmpm_plan* P, const Mesh2<T>& M, T* pa, T* pb2, T* pstat, int32_t* 
     a.t = T(step0 + k) * a.dt;
     a.par = (int)(k & 1);
     ProfScopeT ps_(P, KC_STEP, s);
-    if (scheme == DMPM_USF) {
-      if constexpr (kFast) k_step_fast<T, true><<<a.ntiles, kFT, 0, s>>>(a);
-      k_step_rounds<T, true><<<a.nrb + kTailBlocks, kFT, 0, s>>>(a);
-    } else {
-      if constexpr (kFast) k_step_fast<T, false><<<a.ntiles, kFT, 0, s>>>(a);
-      k_step_rounds<T, false><<<a.nrb + kTailBlocks, kFT, 0, s>>>(a);
-    }
+    if (P->nsims > 1)
+      step_launch<T, true>(a, scheme == DMPM_USF, s);
+    else
+      step_launch<T, false>(a, scheme == DMPM_USF, s);
     DMPM_TRY(launch_check("step"));
   }
   return DMPM_OK;

@@ -5,7 +5,8 @@ an ensemble of them costs one launch sequence per simulation and step.
 ``Ensemble`` stacks the particle and node blocks of several 2D
 Quadrilateral4Node simulations (different meshes, sizes, materials,
 constraints) end to end and drives them with one batch plan: each step is
-one launch of each step kernel for all simulations together.
+one launch for all simulations together (every member's tiles of cells in
+one grid, after the re-sort), or one launch per phase (``autotune`` picks).
 
 Each simulation keeps its own particle and node objects; ``gather()`` copies
 their state into the ensemble's blocks (done at construction) and
@@ -184,32 +185,43 @@ class Ensemble:
         self.plan.run(self.scheme, self.velocity_update, gravity, self.dt, start_step, nsteps,
                       keep_temporaries)
 
-    def autotune(self, gravity, steps=3, start_step=0, blocks=(64, 128, 256)):
-        """Chooses the cells per workgroup of the batch plan's cell passes by
-        timing ``steps`` steps with each candidate (after one warm-up step)
-        on scratch copies of the ensemble's state, with a scratch plan; the
-        ensemble's state and sort are untouched, and the choice does not
-        change results.  Returns {cells per workgroup: ms per step} and
-        applies the fastest."""
+    def autotune(self, gravity, steps=4, start_step=0, blocks=(64, 128, 256)):
+        """Chooses how the batch plan steps: the one-launch step (every
+        member's tiles of 16 x 16 cells in one launch per step, key "step")
+        or one launch per phase with ``blocks`` cells per workgroup of the
+        cell passes (keys 64, 128, 256), by timing ``steps`` steps (even, so
+        that the one-launch step runs every one of them) after a warm-up of
+        two, on scratch copies of the ensemble's state with a scratch plan.
+        The ensemble's state and sort are untouched, and no choice changes
+        results.  Applies the fastest (and, for odd-length runs, whose first
+        step runs per phase, the fastest cells per workgroup) and returns
+        {candidate: ms per step}."""
+        if steps < 2 or steps % 2:
+            raise ValueError("steps must be even and at least 2")
         scratch = BatchPlan(self)
         times = {}
         stream = torch.cuda.current_stream(self.device)
-        for cb in blocks:
-            scratch.set_block(cb)
+        for cand in ("step",) + tuple(blocks):
+            scratch.set_option(_lib.DMPM_OPT_STEP_KERNEL, 1 if cand == "step" else 0)
+            if cand != "step":
+                scratch.set_block(cand)
             pbuf, eid, nbuf = self.pbuf.clone(), self.eid.clone(), self.nbuf.clone()
-            scratch.run(self.scheme, self.velocity_update, gravity, self.dt, start_step, 1,
+            scratch.run(self.scheme, self.velocity_update, gravity, self.dt, start_step, 2,
                         pbuf=pbuf, eid=eid, nbuf=nbuf)
             e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
             e0.record(stream)
-            scratch.run(self.scheme, self.velocity_update, gravity, self.dt, start_step + 1,
+            scratch.run(self.scheme, self.velocity_update, gravity, self.dt, start_step + 2,
                         steps, pbuf=pbuf, eid=eid, nbuf=nbuf)
             e1.record(stream)
             torch.cuda.synchronize(self.device)
-            times[cb] = e0.elapsed_time(e1) / steps
+            times[cand] = e0.elapsed_time(e1) / steps
         del scratch
+        best_block = min((c for c in times if c != "step"), key=times.get)
         best = min(times, key=times.get)
-        self.plan.set_block(best)
-        self.cells_per_block = best
+        self.plan.set_option(_lib.DMPM_OPT_STEP_KERNEL, 1 if best == "step" else 0)
+        self.plan.set_block(best_block)
+        self.step_kernel = best == "step"
+        self.cells_per_block = best_block
         return times
 
     def sim_index(self):

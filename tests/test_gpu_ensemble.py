@@ -109,6 +109,73 @@ def test_ensemble_log_e_gradients_equal_members_alone(scheme):
         assert math.isclose(float(le.grad[i]), ga, rel_tol=1e-10, abs_tol=1e-14), (i, ga)
 
 
+def _mixed_specs(scheme):
+    """Members of several tiles per side (one 10 tiles wide: two strips of
+    the tile order, the second narrower), a small one, and a 4 x 4 mesh with
+    3000 particles in one cell (more than one workgroup stages at once: the
+    overflow workgroups) and three particles outside the mesh (the tail
+    workgroups)."""
+    specs = []
+    for nx, ny, rows, seed in ((150, 20, 15, 31), (40, 36, 30, 32), (9, 7, 4, 33)):
+        s = cases.synthetic_spec("quad4", nx=nx, ny=ny, rows=rows, seed=seed, scheme=scheme,
+                                 traction=False, nodal_force=True, E=1e3, dt=1e-3, nsteps=8,
+                                 velocity_scale=0.05, drift=(0.6, 0.3))
+        s["nodal_forces"][0]["force"] = -0.05
+        h = s["el_len"][0]
+        s["particles"] = (np.asarray(s["particles"]) * np.array([(nx - 6) / nx, 1.0])
+                          + 3 * h).tolist()
+        specs.append(s)
+    rng = np.random.Generator(np.random.PCG64(11))
+    h = 0.25
+    crowd = np.stack([rng.uniform(1.05, 1.95, 3000), rng.uniform(1.05, 1.95, 3000)], 1) * h
+    bg = rng.uniform(0.02, 0.98, size=(40, 2))
+    pts = np.concatenate([bg[:20], crowd, bg[20:]])
+    pts[:3] = [[1.2, 0.3], [np.nan, 0.4], [0.3, 1.5]]
+    specs.insert(1, dict(
+        element="quad4", nelements=[4, 4], el_len=[h, h], particles=pts.tolist(),
+        init_velocity=(0.1 * rng.standard_normal(pts.shape)).tolist(),
+        material={"type": "LinearElastic", "density": 1000.0, "youngs_modulus": 1e5,
+                  "poisson_ratio": 0.3},
+        constraints=[{"nodes": list(range(5)), "dir": 1, "velocity": 0.0}],
+        nodal_forces=[], tractions=[], gravity=[0.0, -9.81], scheme=scheme, dt=1e-3,
+        nsteps=8, velocity_update=False))
+    return specs
+
+
+def _bits(t):
+    """Bit pattern of a block (every NaN encoded alike)."""
+    t = t.clone()
+    if t.is_floating_point():
+        t = torch.where(torch.isnan(t), torch.full_like(t, float("nan")), t)
+        t = t.view(torch.int32 if t.dtype == torch.float32 else torch.int64)
+    return t
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+@pytest.mark.parametrize("scheme", ["usf", "usl"])
+def test_batch_one_launch_step_equals_phase_launches(scheme, dtype):
+    """A batch plan's one-launch step (every member's tiles in one grid)
+    gives bit-identical states to its launches per phase, across cell
+    changes, crowded cells and particles outside their mesh."""
+    from diffmpm_amd import _lib
+    from diffmpm_amd.ensemble import Ensemble
+    specs = _mixed_specs(scheme)
+    a = Ensemble([cases.build_gpu(s, dtype, keep_temporaries=False) for s in specs])
+    b = Ensemble([cases.build_gpu(s, dtype, keep_temporaries=False) for s in specs])
+    a.plan.set_option(_lib.DMPM_OPT_STEP_KERNEL, 1)
+    b.plan.set_option(_lib.DMPM_OPT_STEP_KERNEL, 0)
+    g = torch.tensor(specs[0]["gravity"], dtype=torch.float64)
+    e0 = a.eid.clone()
+    for k in (0, 4):
+        a.run(4, g, start_step=k)
+        b.run(4, g, start_step=k)
+        for x, y in ((a.pbuf, b.pbuf), (a.nbuf, b.nbuf), (a.eid, b.eid)):
+            assert torch.equal(_bits(x), _bits(y)), k
+    assert not torch.equal(e0, a.eid)  # cells did change
+    n1 = a.counts[0]
+    assert torch.isnan(a.pbuf[0, n1 + 1])  # the crowded member's particle at a NaN position
+
+
 def test_batch_plan_rejects_per_method_calls():
     from diffmpm_amd import _lib
     from diffmpm_amd.ensemble import Ensemble
@@ -128,7 +195,8 @@ def test_autotune_keeps_state_and_results():
     b = Ensemble(_members("usl", torch.float32))
     before = a.pbuf.clone()
     times = a.autotune(g, steps=2)
-    assert set(times) == {64, 128, 256} and a.cells_per_block in times
+    assert set(times) == {"step", 64, 128, 256} and a.cells_per_block in (64, 128, 256)
+    assert a.step_kernel == (min(times, key=times.get) == "step")
     assert torch.equal(a.pbuf, before)
     a.run(6, g)
     b.run(6, g)
