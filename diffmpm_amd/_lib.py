@@ -22,6 +22,7 @@ DMPM_USF, DMPM_USL = 0, 1
 DMPM_FN_UNIT, DMPM_FN_LINEAR = 0, 1
 DMPM_OPT_USF_SPLIT = 1
 DMPM_OPT_STEP_KERNEL = 2
+DMPM_RUN_CLOCK = 8  # dmpm_run flag: step indices from the plan's device step counter
 
 # Particle block rows (include/diffmpm_amd.h)
 P_X, P_V, P_MASS, P_VOLUME, P_DENSITY, P_SIZE = 0, 2, 4, 5, 6, 7
@@ -91,6 +92,9 @@ SIGNATURES = {
     "dmpm_timing_read": [_P, _DP, POINTER(c_int64)],
     "dmpm_timing_read_class": [_P, c_int32, _DP, POINTER(c_int64)],
     "dmpm_run": [_P, _P, _P, _P, c_int32, c_int32, _DP, c_double, c_int64, c_int64, c_int32, _P],
+    "dmpm_plan_set_clock": [_P, c_int64, _P],
+    "dmpm_plan_get_clock": [_P, POINTER(c_int64), _P],
+    "dmpm_prepare_run": [_P, c_int32, c_int32, _P],
     "dmpm_step_between": [_P, _P, _P, _P, _P, c_int32, c_int32, _DP, c_double, c_int64, c_int32,
                           _P, _P, c_int64, _P],
     "dmpm_step_adjoint": [_P, _P, _P, _P, _P, _P, _P, c_int32, c_int32, _DP, c_double, c_int64,

@@ -625,7 +625,7 @@ int adjoint_t(dmpm_plan* P, const T* pb, T* pw, int32_t* eid, T* nb, T* cot, T* 
                                       (size_t)(sp[1] - sp[0]) * ld * sizeof(T),
                                       hipMemcpyDeviceToDevice, s),
                        "hipMemcpyAsync"));
-  DMPM_TRY(tile_run(P, pw, eid, nb, scheme, vu, grav, dt, step, 1, 0, s));
+  DMPM_TRY(tile_run(P, pw, eid, nb, scheme, vu, grav, dt, step, 1, 0, s, nullptr));
   const T* vkin = (const T*)plan_workspace(P, 1, (size_t)2 * P->nnodes * sizeof(T));
   if (!vkin) return fail(DMPM_EHIP, "workspace allocation failed");
   return adjoint_kernels<T, CB>(P, M, rows_of_block(pb, ld), out_of_block<T>(pw, ld), nb, vkin,

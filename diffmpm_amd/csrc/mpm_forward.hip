@@ -838,7 +838,7 @@ void plan_free(dmpm_plan* P) {
   void* ptrs[] = {P->key, P->perm, P->counts, P->fill, P->cs, P->bsum, P->flags, P->zflags,
                   P->big,
                   P->perm2, P->cs2, P->mov, P->mrec, P->dL, P->dA, P->headL, P->headA,
-                  P->chunkD, P->chunkC, P->sims, P->tile_sim,
+                  P->chunkD, P->chunkC, P->sims, P->tile_sim, P->clock,
                   P->con_node, P->con_dir, P->con_val, P->nf_id, P->nf_dir, P->nf_off,
                   P->nf_fn, P->nf_val, P->tr_id, P->tr_dir, P->tr_off, P->tr_fn, P->tr_val,
                   P->fn_kind, P->fn_off, P->fn_cnt, P->fn_x, P->fn_f, P->cidx, P->cmask,
@@ -953,6 +953,8 @@ static int plan_init(dmpm_plan* P, dmpm_plan** out) {
     ntiles += (int64_t)r.tiles_x * ((r.ny + kStepTile - 1) / kStepTile);
   }
   P->ntiles = (int)ntiles;
+  if (rc == DMPM_OK) rc = hip_check(hipMalloc((void**)&P->clock, sizeof(int64_t)), "hipMalloc");
+  if (rc == DMPM_OK) rc = hip_check(hipMemset(P->clock, 0, sizeof(int64_t)), "hipMemset");
   if (rc == DMPM_OK && P->nsims > 1) {
     std::vector<int> ts((size_t)ntiles);
     for (int s = 0; s < P->nsims; ++s) {
@@ -1700,7 +1702,12 @@ template <typename T, int DIM> struct RunSteps {
     const int temps = flags & 1;
     const int pfext = (flags >> 1) & 1;
     if (!(flags & 4) && tile_eligible(P, pfext))
-      return tile_run(P, pbuf, eid, nbuf, scheme, vu, grav, dt, step0, nsteps, temps, s);
+      return tile_run(P, pbuf, eid, nbuf, scheme, vu, grav, dt, step0, nsteps, temps, s,
+                      (flags & DMPM_RUN_CLOCK) ? P->clock : nullptr);
+    if (flags & DMPM_RUN_CLOCK)
+      return fail(DMPM_EINVAL,
+                  "DMPM_RUN_CLOCK covers the fused 2D step (Quadrilateral4Node, no particle "
+                  "tractions or particle external forces, flag bit 2 clear)");
     if (P->nsims > 1)
       return fail(DMPM_EINVAL,
                   "batch plans run the fused 2D step only (no per-phase path, particle "
@@ -1739,6 +1746,37 @@ extern "C" int dmpm_step_between(dmpm_plan* P, const void* pbuf_in, void* pbuf_o
   return tile_between(P, pbuf_in, pbuf_out, eid, nbuf, (int)scheme, (int)velocity_update,
                       gravity, dt, step, (int)(flags & 1), (int)((flags >> 1) & 1), state_in,
                       state_out, state_ld, ST(stream));
+}
+
+namespace dmpm {
+__global__ void k_clock_set(int64_t* clock, int64_t v) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) *clock = v;
+}
+}  // namespace dmpm
+
+extern "C" int dmpm_plan_set_clock(dmpm_plan* P, int64_t step, void* stream) {
+  PCHK(P);
+  dmpm::k_clock_set<<<1, 64, 0, ST(stream)>>>(P->clock, step);
+  return launch_check("clock_set");
+}
+
+extern "C" int dmpm_plan_get_clock(dmpm_plan* P, int64_t* step, void* stream) {
+  PCHK(P);
+  if (!step) return fail(DMPM_EINVAL, "null output");
+  DMPM_TRY(hip_check(hipMemcpyAsync(step, P->clock, sizeof(int64_t), hipMemcpyDeviceToHost,
+                                    ST(stream)),
+                     "hipMemcpyAsync"));
+  return hip_check(hipStreamSynchronize(ST(stream)), "hipStreamSynchronize");
+}
+
+extern "C" int dmpm_prepare_run(dmpm_plan* P, int32_t scheme, int32_t flags, void* stream) {
+  PCHK(P);
+  if (scheme != DMPM_USF && scheme != DMPM_USL)
+    return fail(DMPM_EINVAL, "scheme must be DMPM_USF or DMPM_USL");
+  // the per-phase kernels use only the plan's own buffers
+  if ((flags & 4) || !tile_eligible(P, (flags >> 1) & 1)) return DMPM_OK;
+  DMPM_TRY(tile_prepare(P, (int)scheme, ST(stream)));
+  return hip_check(hipStreamSynchronize(ST(stream)), "hipStreamSynchronize");
 }
 
 extern "C" int dmpm_run(dmpm_plan* P, void* pbuf, int32_t* eid, void* nbuf, int32_t scheme,

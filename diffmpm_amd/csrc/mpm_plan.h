@@ -32,6 +32,7 @@ struct dmpm_plan {
   SimRec* sims = nullptr;          // [nsims] device table
   std::vector<SimRec> sims_h;      // host copy
   int ntiles = 0;                  // tiles of the one-launch step, all simulations
+  int64_t* clock = nullptr;        // device step counter (dmpm_run's DMPM_RUN_CLOCK)
   int* tile_sim = nullptr;         // [ntiles] simulation of each tile (batch plans)
   int device = 0;
   // counting-sort workspace
@@ -166,9 +167,13 @@ int ensure_sorted(dmpm_plan* P, const int32_t* eid, hipStream_t s);
 // on first use, kept for the plan's lifetime).
 void* plan_workspace(dmpm_plan* P, int slot, size_t bytes);
 // Node-tile step (mpm_tile.hip): 2D, no particle tractions or particle f_ext.
+// clock: the steps' indices are *clock + 0 .. nsteps-1, read on the device,
+// and the run adds nsteps to *clock at its end (null: step0 .. on the host).
 int tile_run(dmpm_plan* P, void* pbuf, int32_t* eid, void* nbuf, int scheme, int vu,
              const double* grav, double dt, int64_t step0, int64_t nsteps, int temps,
-             hipStream_t s);
+             hipStream_t s, int64_t* clock);
+// Allocates the workspaces of tile_run and uploads its tables (no other work).
+int tile_prepare(dmpm_plan* P, int scheme, hipStream_t s);
 bool tile_eligible(const dmpm_plan* P, int pfext);
 // One fused 2D step from block pin to block pout (mpm_tile.hip).
 int tile_between(dmpm_plan* P, const void* pin, void* pout, int32_t* eid, void* nbuf,

@@ -324,6 +324,13 @@ __global__ __launch_bounds__(kThreads) void k_node_kin(Mesh2<T> M, const T* __re
   }
 }
 
+// Time of a step: t, or (clock given) T(*clock + kofs) * dt with the step
+// index read on the device (the same rounding as T(step) * dt on the host).
+template <typename T>
+__device__ __forceinline__ T step_time(const int64_t* clock, int kofs, T t, T dt) {
+  return clock ? T(*clock + kofs) * dt : t;
+}
+
 // FORCE node pass: external force (body + concentrated nodal forces at t,
 // element.py:204-284), internal force, acceleration/velocity/momentum update
 // (element.py:305-329) and constraints.
@@ -333,7 +340,8 @@ __global__ __launch_bounds__(kThreads) void k_node_force(
     const T* __restrict__ vkin, const int* __restrict__ cidx, const int* __restrict__ cmask,
     const double* __restrict__ cval, T dt, T t, const int* __restrict__ nfd,
     const int* __restrict__ nf_off, const int* __restrict__ nf_fn,
-    const double* __restrict__ nf_val, FnTable fns) {
+    const double* __restrict__ nf_val, FnTable fns, const int64_t* __restrict__ clock,
+    int kofs) {
   const int nd = blockIdx.x * blockDim.x + threadIdx.x;
   if (nd >= M.nnodes) return;
   const int sm = M.sim_n(nd);
@@ -348,9 +356,11 @@ __global__ __launch_bounds__(kThreads) void k_node_force(
   for (int d = 0; d < 2; ++d) {
     T fe = acc[d];
     const int qg = nfd ? nfd[(int64_t)d * M.nnodes + nd] : -1;
-    if (qg >= 0)
+    if (qg >= 0) {
+      const T tt = step_time(clock, kofs, t, dt);
       for (int k = nf_off[qg]; k < nf_off[qg + 1]; ++k)
-        fe = fe + fn_value<T>(fns, nf_fn[k], t) * T(nf_val[k]);
+        fe = fe + fn_value<T>(fns, nf_fn[k], tt) * T(nf_val[k]);
+    }
     const T fi = acc[2 + d];
     NR(DMPM_N_FEXT + d)[nd] = fe;
     NR(DMPM_N_FINT + d)[nd] = fi;
@@ -725,6 +735,8 @@ template <typename T> struct StepArgs {
   const SimRec* sims;   // batch plans: the simulation table (tile_off, tiles_x, ...)
   const int* tile_sim;  // batch plans: the simulation of each tile
   int nc_all, nn_all, nsims;  // cells, nodes and simulations of the plan
+  const int64_t* clock;  // optional: device step counter (time = (*clock + kofs) * dt)
+  int kofs;
   T* sout;   // optional: the state rows x, y, vx, vy, sxx, syy, sxy, V after the step
   const T* sin;  // optional: the state rows before the step (then read from here, and
                  // written only to sout)
@@ -1070,7 +1082,8 @@ __device__ __forceinline__ void force_epilogue(const StepArgs<T>& a, const TileC
         const int qg = x.nfd[(int64_t)d * nnodes + nd];
         if (qg >= 0)
           for (int k = x.nf_off[qg]; k < x.nf_off[qg + 1]; ++k)
-            fe = fe + fn_value<T>(x.fns, x.nf_fn[k], a.t) * T(x.nf_val[k]);
+            fe = fe + fn_value<T>(x.fns, x.nf_fn[k], step_time(a.clock, a.kofs, a.t, a.dt)) *
+                          T(x.nf_val[k]);
       }
       const T fi = nacc[w][2 + d];
       const T F = (fi + fe) + fd[w][d];
@@ -2278,7 +2291,7 @@ int enqueue_resort(dmpm_plan* P, const Mesh2<T>& M, const T* pb, int32_t* eid, h
 template <typename T, int CB>
 int tile_steps(dmpm_plan* P, const Mesh2<T>& M, T* pb, int32_t* eid, T* nb, int scheme, int vu,
                const double* grav, double dt, int64_t step0, int64_t nsteps, int temps,
-               hipStream_t s, T* part, T* vkin) {
+               hipStream_t s, T* part, T* vkin, const int64_t* clock, int kbase) {
   const int64_t ld = P->d.ld, nld = P->d.nld;
   const int n = (int)P->n;
   const bool usf = scheme == DMPM_USF;
@@ -2326,7 +2339,8 @@ int tile_steps(dmpm_plan* P, const Mesh2<T>& M, T* pb, int32_t* eid, T* nb, int 
                                                             gy, tdt, temps);
       k_node_force<T><<<nnb, kThreads, 0, s>>>(M, part, nb, nld, vkin, P->cidx, P->cmask,
                                                P->cval, tdt, t, P->nfd, P->nf_off, P->nf_fn,
-                                               P->nf_val, fns);
+                                               P->nf_val, fns, clock,
+                                               kbase + (int)(step - step0));
       DMPM_TRY(launch_check("p2g_force"));
     }
     {
@@ -2348,6 +2362,10 @@ __global__ void k_zero_ints(int* p, int n) {
   if (i < n) p[i] = 0;
 }
 
+__global__ void k_clock_add(int64_t* clock, int64_t n) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) *clock += n;
+}
+
 // Whether the one-launch step can run this plan: the particle and node blocks
 // below 4 GiB (the kernels' 32-bit buffer offsets).
 bool step_eligible(const dmpm_plan* P) {
@@ -2357,12 +2375,6 @@ bool step_eligible(const dmpm_plan* P) {
          (uint64_t)DMPM_N_ROWS * (uint64_t)P->d.nld * esz < (1ull << 32);
 }
 
-// The one-launch step for single-simulation plans: per step, the re-sort,
-// k_step_fast over every tile (FP32) and k_step_rounds over the tiles that
-// overflowed the fast path and the particles outside the mesh.  The
-// particle state alternates between blocks pa and pb2 (dynamic rows only;
-// pb2 null: the plan's shadow block), starting from pa; pstat holds the
-// static rows (mass, E) and takes the temporaries.
 // The kernels of one step (B: batch plan).
 template <typename T, bool B>
 void step_launch(const StepArgs<T>& a, bool usf, hipStream_t s) {
@@ -2376,11 +2388,20 @@ void step_launch(const StepArgs<T>& a, bool usf, hipStream_t s) {
   }
 }
 
+// The one-launch step: per step, the re-sort, k_step_fast over every tile
+// (FP32; batch plans: the tiles of every simulation) and k_step_rounds over
+// the tiles that overflowed the fast path and the particles outside the
+// mesh.  The particle state alternates between blocks pa and pb2 (dynamic
+// rows only; pb2 null: the plan's shadow block), starting from pa; pstat
+// holds the static rows (mass, E) and takes the temporaries.  clock: step
+// k's time is read on the device as (*clock + kbase + k) * dt.  prepare:
+// only allocate the workspaces and upload the tables.
 template <typename T>
 int step_run(dmpm_plan* P, const Mesh2<T>& M, T* pa, T* pb2, T* pstat, int32_t* eid, T* nb,
              int scheme, int vu, const double* grav, double dt, int64_t step0, int64_t nsteps,
              int temps, hipStream_t s, T* vkin, T* sout = nullptr, int64_t sld = 0,
-             const T* sin = nullptr) {
+             const T* sin = nullptr, const int64_t* clock = nullptr, int kbase = 0,
+             bool prepare = false) {
   constexpr bool kFast = sizeof(T) == 4;
   const int64_t ld = P->d.ld, nld = P->d.nld;
   const int nn = P->nnodes;
@@ -2396,10 +2417,13 @@ int step_run(dmpm_plan* P, const Mesh2<T>& M, T* pa, T* pb2, T* pstat, int32_t* 
   T* vbuf = (T*)plan_workspace(P, 6, (size_t)4 * nn * sizeof(T));
   int* ovf = kFast ? (int*)plan_workspace(P, 7, (size_t)(2 + 2 * a.ntiles) * sizeof(int)) : nullptr;
   if (!shadow || !vbuf || (kFast && !ovf)) return fail(DMPM_EHIP, "workspace allocation failed");
-  if (kFast) k_zero_ints<<<1, 64, 0, s>>>(ovf, 2);
-  DMPM_TRY(hip_check(hipMemcpy2DAsync(vbuf, nn * sizeof(T), nb + DMPM_N_V * nld, nld * sizeof(T),
-                                      nn * sizeof(T), 2, hipMemcpyDeviceToDevice, s),
-                     "hipMemcpy2DAsync"));
+  if (!prepare) {
+    if (kFast) k_zero_ints<<<1, 64, 0, s>>>(ovf, 2);
+    DMPM_TRY(hip_check(hipMemcpy2DAsync(vbuf, nn * sizeof(T), nb + DMPM_N_V * nld,
+                                        nld * sizeof(T), nn * sizeof(T), 2,
+                                        hipMemcpyDeviceToDevice, s),
+                       "hipMemcpy2DAsync"));
+  }
   a.g = M.g0;
   a.pb = pstat;
   a.ld32 = (uint32_t)ld;
@@ -2437,7 +2461,9 @@ int step_run(dmpm_plan* P, const Mesh2<T>& M, T* pa, T* pb2, T* pstat, int32_t* 
     DMPM_TRY(hip_check(hipMemcpyAsync(aux, P->step_aux, sizeof(StepAux), hipMemcpyHostToDevice, s),
                        "hipMemcpyAsync"));
   }
+  if (prepare) return DMPM_OK;
   a.aux = aux;
+  a.clock = clock;
   a.mat.kind = P->d.material;
   a.mat.per_E = P->d.per_particle_E;
   for (int k = 0; k < 3; ++k) a.mat.de[k] = P->d.de[k];
@@ -2469,6 +2495,7 @@ int step_run(dmpm_plan* P, const Mesh2<T>& M, T* pa, T* pb2, T* pstat, int32_t* 
     a.vold = vbuf + (k & 1) * 2 * (int64_t)nn;
     a.vnew = vbuf + ((k + 1) & 1) * 2 * (int64_t)nn;
     a.t = T(step0 + k) * a.dt;
+    a.kofs = kbase + (int)k;
     a.par = (int)(k & 1);
     ProfScopeT ps_(P, KC_STEP, s);
     if (P->nsims > 1)
@@ -2483,11 +2510,13 @@ int step_run(dmpm_plan* P, const Mesh2<T>& M, T* pa, T* pb2, T* pstat, int32_t* 
 template <typename T>
 int tile_steps_cb(dmpm_plan* P, const Mesh2<T>& M, T* pb, int32_t* eid, T* nb, int scheme,
                   int vu, const double* grav, double dt, int64_t step0, int64_t nsteps, int temps,
-                  hipStream_t s, T* part, T* vkin);
+                  hipStream_t s, T* part, T* vkin, const int64_t* clock = nullptr,
+                  int kbase = 0);
 
 template <typename T>
 int tile_run_t(dmpm_plan* P, T* pb, int32_t* eid, T* nb, int scheme, int vu, const double* grav,
-               double dt, int64_t step0, int64_t nsteps, int temps, hipStream_t s) {
+               double dt, int64_t step0, int64_t nsteps, int temps, hipStream_t s,
+               int64_t* clock) {
   const Mesh2<T> M = mesh_of<T>(P);
   const int64_t ld = P->d.ld;
   const int n = (int)P->n;
@@ -2507,18 +2536,35 @@ int tile_run_t(dmpm_plan* P, T* pb, int32_t* eid, T* nb, int scheme, int vu, con
     // the one-launch step alternates blocks, so it runs an even number of
     // steps and the state ends in the caller's block; the first step of an
     // odd-length run uses the (in-place) per-phase launches
-    if (nsteps & 1) {
+    const int odd = (int)(nsteps & 1);
+    if (odd)
       DMPM_TRY(tile_steps_cb<T>(P, M, pb, eid, nb, scheme, vu, grav, dt, step0, 1, temps, s,
-                                part, vkin));
-      ++step0;
-      --nsteps;
-    }
-    return nsteps ? step_run<T>(P, M, pb, nullptr, pb, eid, nb, scheme, vu, grav, dt, step0,
-                                nsteps, temps, s, vkin)
-                  : DMPM_OK;
+                                part, vkin, clock, 0));
+    if (nsteps > odd)
+      DMPM_TRY(step_run<T>(P, M, pb, nullptr, pb, eid, nb, scheme, vu, grav, dt, step0 + odd,
+                           nsteps - odd, temps, s, vkin, nullptr, 0, nullptr, clock, odd));
+  } else {
+    DMPM_TRY(tile_steps_cb<T>(P, M, pb, eid, nb, scheme, vu, grav, dt, step0, nsteps, temps, s,
+                              part, vkin, clock, 0));
   }
-  return tile_steps_cb<T>(P, M, pb, eid, nb, scheme, vu, grav, dt, step0, nsteps, temps, s, part,
-                          vkin);
+  if (clock) {
+    k_clock_add<<<1, 64, 0, s>>>(clock, nsteps);
+    DMPM_TRY(launch_check("clock"));
+  }
+  return DMPM_OK;
+}
+
+// Workspaces of tile_run_t (and its table upload), with no other work.
+template <typename T>
+int tile_prepare_t(dmpm_plan* P, int scheme, hipStream_t s) {
+  const Mesh2<T> M = mesh_of<T>(P);
+  T* part = (T*)plan_workspace(P, 0, (size_t)4 * NQ_FORCE * M.ncells * sizeof(T));
+  T* vkin = (T*)plan_workspace(P, 1, (size_t)2 * P->nnodes * sizeof(T));
+  if (!part || !vkin) return fail(DMPM_EHIP, "workspace allocation failed");
+  if (!step_eligible(P)) return DMPM_OK;
+  const double grav[2] = {0, 0};
+  return step_run<T>(P, M, nullptr, nullptr, nullptr, nullptr, nullptr, scheme, 0, grav, 0, 0,
+                     0, 0, s, vkin, nullptr, 0, nullptr, nullptr, 0, true);
 }
 
 // One step from block pin to block pout (pin unchanged; pout's static rows
@@ -2583,17 +2629,17 @@ int tile_between_t(dmpm_plan* P, const T* pin, T* pout, int32_t* eid, T* nb, int
 template <typename T>
 int tile_steps_cb(dmpm_plan* P, const Mesh2<T>& M, T* pb, int32_t* eid, T* nb, int scheme,
                   int vu, const double* grav, double dt, int64_t step0, int64_t nsteps, int temps,
-                  hipStream_t s, T* part, T* vkin) {
+                  hipStream_t s, T* part, T* vkin, const int64_t* clock, int kbase) {
   switch (P->cell_block) {
     case 64:
       return tile_steps<T, 64>(P, M, pb, eid, nb, scheme, vu, grav, dt, step0, nsteps, temps, s,
-                               part, vkin);
+                               part, vkin, clock, kbase);
     default:
       return tile_steps<T, 128>(P, M, pb, eid, nb, scheme, vu, grav, dt, step0, nsteps, temps, s,
-                                part, vkin);
+                                part, vkin, clock, kbase);
     case 256:
       return tile_steps<T, 256>(P, M, pb, eid, nb, scheme, vu, grav, dt, step0, nsteps, temps, s,
-                                part, vkin);
+                                part, vkin, clock, kbase);
   }
 }
 
@@ -2619,12 +2665,17 @@ int tile_between(dmpm_plan* P, const void* pin, void* pout, int32_t* eid, void* 
 
 int tile_run(dmpm_plan* P, void* pbuf, int32_t* eid, void* nbuf, int scheme, int vu,
              const double* grav, double dt, int64_t step0, int64_t nsteps, int temps,
-             hipStream_t s) {
+             hipStream_t s, int64_t* clock) {
   if (P->d.dtype == DMPM_F32)
     return tile_run_t<float>(P, (float*)pbuf, eid, (float*)nbuf, scheme, vu, grav, dt, step0,
-                             nsteps, temps, s);
+                             nsteps, temps, s, clock);
   return tile_run_t<double>(P, (double*)pbuf, eid, (double*)nbuf, scheme, vu, grav, dt, step0,
-                            nsteps, temps, s);
+                            nsteps, temps, s, clock);
+}
+
+int tile_prepare(dmpm_plan* P, int scheme, hipStream_t s) {
+  if (P->d.dtype == DMPM_F32) return tile_prepare_t<float>(P, scheme, s);
+  return tile_prepare_t<double>(P, scheme, s);
 }
 
 }  // namespace dmpm

@@ -1,0 +1,76 @@
+"""Rollouts captured into HIP graphs.
+
+The reference compiles its step loop once (``jax.lax.fori_loop`` /
+``lax.scan`` in diffmpm/solver.py:179-213) and replays it.  ``StepGraph`` is
+the HIP counterpart: ``nsteps`` steps of a simulation (one particle set on a
+2D Quadrilateral4Node mesh, the fused path) or of an ``Ensemble`` are
+captured once into a HIP graph through ``torch.cuda.CUDAGraph``, and every
+``replay()`` launches the whole sequence (per step: the re-sort stages and
+the one-launch step) with one call.  The steps' indices live in the plan's
+device step counter (``DMPM_RUN_CLOCK``), so each replay continues in time
+where the previous one ended (time functions of nodal forces see the right
+t), and results equal the same steps run eagerly, bit for bit.
+
+The graph holds the addresses of the simulation's particle and node blocks:
+it stays valid while they live and the plan's tables (constraints, nodal
+forces, functions) are not changed.  Replays run on the current stream.
+"""
+from ctypes import c_int64, byref
+
+import torch
+
+from . import _lib
+from ._lib import call, ptr
+
+
+class StepGraph:
+    """``nsteps`` steps of ``target`` (``MPMExplicit`` or ``Ensemble``)
+    captured into one HIP graph; the first replay runs steps start_step ..
+    start_step + nsteps - 1, the next one the following nsteps, and so on."""
+
+    def __init__(self, target, gravity, nsteps, start_step=0, keep_temporaries=None):
+        from .ensemble import Ensemble
+        if nsteps < 1:
+            raise ValueError("nsteps must be positive")
+        if isinstance(target, Ensemble):
+            plan = target.plan
+            pbuf, eid, nbuf = target.pbuf, target.eid, target.nbuf
+            temps = bool(keep_temporaries)
+            fext = False
+        else:
+            if len(target.mesh.particles) != 1:
+                raise ValueError("a captured rollout holds one particle set")
+            from ._plan import plan_for
+            ps = target.mesh.particles[0]
+            plan = plan_for(target.mesh.elements, ps)
+            plan.refresh_tables()
+            pbuf, eid, nbuf = ps._buf, ps._eid, target.mesh.elements.nodes._buf
+            temps = target.keep_temporaries if keep_temporaries is None else keep_temporaries
+            fext = ps._has_fext
+        self.target, self.plan, self.nsteps = target, plan, int(nsteps)
+        self._keep = (pbuf, eid, nbuf)  # the graph holds their addresses
+        code = _lib.DMPM_USF if target.scheme == "usf" else _lib.DMPM_USL
+        flags = (1 if temps else 0) | (2 if fext else 0) | _lib.DMPM_RUN_CLOCK
+        grav = plan._gravity(gravity)
+        device = pbuf.device
+        with torch.cuda.device(device):
+            stream = _lib.stream_of(pbuf)
+            call("dmpm_prepare_run", plan.h, code, flags, stream)
+            call("dmpm_plan_set_clock", plan.h, int(start_step), stream)
+            torch.cuda.synchronize(device)
+            self.graph = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(self.graph):
+                call("dmpm_run", plan.h, ptr(pbuf), ptr(eid), ptr(nbuf), code,
+                     1 if target.velocity_update else 0, grav, float(target.dt), 0,
+                     self.nsteps, flags, _lib.stream_of(pbuf))
+
+    def replay(self):
+        """Runs the next ``nsteps`` steps (enqueued on the current stream)."""
+        self.graph.replay()
+
+    @property
+    def step(self):
+        """Index of the next step a replay runs (waits for the stream)."""
+        v = c_int64()
+        call("dmpm_plan_get_clock", self.plan.h, byref(v), _lib.stream_of(self._keep[0]))
+        return int(v.value)

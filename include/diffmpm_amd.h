@@ -236,10 +236,34 @@ int dmpm_phase_particle_kinematics(dmpm_plan* plan, void* pbuf, const int32_t* e
  * (strain_rate, dstrain, dvolumetric_strain, reference_loc, momentum);
  * bit 1: particle f_ext is non-zero; bit 2: use the per-phase kernels even
  * where the fused node-tile kernels apply (2D, no particle tractions, zero
- * particle f_ext).  Both paths give bit-identical results. */
+ * particle f_ext).  Both paths give bit-identical results.  Bit 3
+ * (DMPM_RUN_CLOCK, fused 2D path only): see below. */
 int dmpm_run(dmpm_plan* plan, void* pbuf, int32_t* eid, void* nbuf, int32_t scheme,
              int32_t velocity_update, const double* gravity, double dt, int64_t step0,
              int64_t nsteps, int32_t flags, void* stream);
+
+/* ---- graph capture (no reference counterpart; the reference traces its
+ * step loop with jax.lax, solver.py:179-213) ----------------------------------
+ * dmpm_run on the fused 2D path enqueues only kernels and device-to-device
+ * copies on its stream, with no host synchronisation, so a run can be
+ * captured into a HIP graph (hipStreamBeginCapture ... hipGraphLaunch) once
+ * dmpm_prepare_run has allocated its workspaces.  With DMPM_RUN_CLOCK the
+ * steps' indices (the time t = step * dt of the time functions) are read
+ * from the plan's device step counter when the kernels run, step0 is
+ * ignored, and the run adds nsteps to the counter at its end: each replay
+ * of a captured run continues where the previous one ended.  A graph stays
+ * valid while the plan and its buffers live and its tables (constraints,
+ * functions, nodal forces) are not set again; replays and other work on the
+ * plan must be ordered on one stream. */
+enum { DMPM_RUN_CLOCK = 8 };
+/* Sets the plan's step counter (enqueued on stream). */
+int dmpm_plan_set_clock(dmpm_plan* plan, int64_t step, void* stream);
+/* Reads the plan's step counter (waits for stream). */
+int dmpm_plan_get_clock(dmpm_plan* plan, int64_t* step, void* stream);
+/* Allocates every workspace that dmpm_run with this scheme and flags uses and
+ * uploads its tables, and waits for stream; enqueues no work on the particle
+ * or node state. */
+int dmpm_prepare_run(dmpm_plan* plan, int32_t scheme, int32_t flags, void* stream);
 /* One fused 2D step (index step) from particle block pbuf_in to particle
  * block pbuf_out: pbuf_in is left unchanged, pbuf_out receives the state
  * after the step; pbuf_out's constant rows (mass, Young's modulus) must hold

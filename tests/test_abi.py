@@ -60,6 +60,9 @@ def test_python_constants_match_header():
     assert e["DMPM_QUAD4"] == L.DMPM_QUAD4 and e["DMPM_LINEAR1D"] == L.DMPM_LINEAR1D
     for k, v in L.KC.items():
         assert e["DMPM_KC_" + k.upper()] == v, k
+    assert e["DMPM_OPT_USF_SPLIT"] == L.DMPM_OPT_USF_SPLIT
+    assert e["DMPM_OPT_STEP_KERNEL"] == L.DMPM_OPT_STEP_KERNEL
+    assert e["DMPM_RUN_CLOCK"] == L.DMPM_RUN_CLOCK
 
 
 def test_plan_desc_layout():
