@@ -123,7 +123,8 @@ class Differentiable:
         for a, b in _RUNS:
             buf[b, : self.n] = state[a]
         if log_e is not None:
-            buf[_lib.P_YOUNGS, : self.n] = torch.exp(log_e.detach()).to(buf.dtype).expand(self.n)
+            self._write_log_e(buf, log_e)
+            self.__dict__.setdefault("_e_written", {})[buf.data_ptr()] = (log_e, log_e._version)
 
     def set_state(self, state, log_e=None):
         last = getattr(self, "_last", None)
@@ -136,6 +137,18 @@ class Differentiable:
 
     def _write_log_e(self, buf, log_e):
         buf[_lib.P_YOUNGS, : self.n] = torch.exp(log_e.detach()).to(buf.dtype).expand(self.n)
+
+    def _write_log_e_private(self, buf, log_e):
+        """_write_log_e for a block only this object writes (the private
+        step blocks, the adjoint's block): skipped when the block already
+        holds exp of this very log_e (same tensor, unchanged since)."""
+        cache = self.__dict__.setdefault("_e_written", {})
+        key = buf.data_ptr()
+        hit = cache.get(key)
+        if hit is not None and hit[0] is log_e and hit[1] == log_e._version:
+            return
+        self._write_log_e(buf, log_e)
+        cache[key] = (log_e, log_e._version)
 
     # -- one step ---------------------------------------------------------------
     def forward_step(self, state, log_e, gravity, i, save=False):
@@ -162,7 +175,7 @@ class Differentiable:
             src = last[2] if last is not None else 0
             dst = 1 - src
             if log_e is not None:
-                self._write_log_e(self._blocks[dst], log_e)
+                self._write_log_e_private(self._blocks[dst], log_e)
             plan = self.bplan
             plan.refresh_tables()
             sin = state.detach()
@@ -214,7 +227,7 @@ class Differentiable:
         if st_in.is_contiguous() and st_out.is_contiguous():
             # the kernels read the state tensors; mass and E from pin
             if log_e is not None:
-                self._write_log_e(self.pin, log_e)
+                self._write_log_e_private(self.pin, log_e)
             call("dmpm_step_adjoint_saved_state", self.fplan.h, ptr(self.pin), ptr(st_in),
                  ptr(st_out), self.n, ptr(nsaved), ptr(vkin), ptr(perm), ptr(cs), ptr(cot),
                  ptr(lebar), self.code, self.vu, self.fplan._gravity(gravity), self.dt,

@@ -1591,7 +1591,6 @@ __global__ __launch_bounds__(kFT) void k_step_rounds(StepArgs<T> a) {
     return;
   }
 
-  if (a.ovf && blockIdx.x == 0 && t == 0) a.ovf[1 - a.par] = 0;  // the next step's count
   const int cnt = a.ovf ? a.ovf[a.par] : a.ntiles;
   for (int qt = blockIdx.x; qt < cnt; qt += a.nrb) {
     const int tile = a.ovf ? a.ovf[2 + a.par * a.ntiles + qt] : qt;
@@ -1964,6 +1963,7 @@ struct SortArgs {
   int* chunkC;
   MoveLog L;
   int n, ncells, nchunks;
+  int* ovf;  // optional: the one-launch step's overflow counters, zeroed by stage 4
 };
 
 __device__ __forceinline__ int sort_mode(const SortArgs& a) {
@@ -2184,6 +2184,8 @@ __global__ __launch_bounds__(kThreads) void k_sort_s3(SortArgs a) {
 // scatter order and mark the layout unsorted); restores the zero counters;
 // the last workgroup publishes the new state.
 __global__ __launch_bounds__(kThreads) void k_sort_s4(SortArgs a) {
+  // the overflow counters of the step that follows (read by no stage)
+  if (a.ovf && blockIdx.x == 0 && threadIdx.x < 2) a.ovf[threadIdx.x] = 0;
   const int mode = sort_mode(a);
   if (mode == SORT_NONE) return;
   if (mode == SORT_INCREMENTAL) {
@@ -2261,8 +2263,9 @@ constexpr int kSortGrid = 2048;  // workgroups of the grid-stride re-sort stages
 
 template <typename T>
 int enqueue_resort(dmpm_plan* P, const Mesh2<T>& M, const T* pb, int32_t* eid, hipStream_t s,
-                   int64_t ld = -1) {
+                   int64_t ld = -1, int* ovf = nullptr) {
   SortArgs a;
+  a.ovf = ovf;
   a.eid = eid;
   a.key = P->key;
   a.counts = P->counts;
@@ -2357,9 +2360,13 @@ int tile_steps(dmpm_plan* P, const Mesh2<T>& M, T* pb, int32_t* eid, T* nb, int 
   return DMPM_OK;
 }
 
-__global__ void k_zero_ints(int* p, int n) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) p[i] = 0;
+// Row r (blockIdx.y) of a block: dst[r * dld + i] = src[r * sld + i], i < n.
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_copy_rows(T* __restrict__ dst, int64_t dld,
+                                                        const T* __restrict__ src, int64_t sld,
+                                                        int n) {
+  const int i = blockIdx.x * kThreads + threadIdx.x;
+  if (i < n) dst[blockIdx.y * dld + i] = src[blockIdx.y * sld + i];
 }
 
 __global__ void k_clock_add(int64_t* clock, int64_t n) {
@@ -2418,11 +2425,11 @@ int step_run(dmpm_plan* P, const Mesh2<T>& M, T* pa, T* pb2, T* pstat, int32_t* 
   int* ovf = kFast ? (int*)plan_workspace(P, 7, (size_t)(2 + 2 * a.ntiles) * sizeof(int)) : nullptr;
   if (!shadow || !vbuf || (kFast && !ovf)) return fail(DMPM_EHIP, "workspace allocation failed");
   if (!prepare) {
-    if (kFast) k_zero_ints<<<1, 64, 0, s>>>(ovf, 2);
-    DMPM_TRY(hip_check(hipMemcpy2DAsync(vbuf, nn * sizeof(T), nb + DMPM_N_V * nld,
-                                        nld * sizeof(T), nn * sizeof(T), 2,
-                                        hipMemcpyDeviceToDevice, s),
-                       "hipMemcpy2DAsync"));
+    // the nodal velocity a massless node keeps (the counters of ovf are
+    // zeroed by each step's re-sort)
+    k_copy_rows<T><<<dim3((unsigned)((nn + kThreads - 1) / kThreads), 2), kThreads, 0, s>>>(
+        vbuf, nn, nb + DMPM_N_V * nld, nld, nn);
+    DMPM_TRY(launch_check("copy_rows"));
   }
   a.g = M.g0;
   a.pb = pstat;
@@ -2486,9 +2493,9 @@ int step_run(dmpm_plan* P, const Mesh2<T>& M, T* pa, T* pb2, T* pstat, int32_t* 
       ProfScopeT ps_(P, KC_SORT, s);
       // positions: from the state tensor when the state rows live there
       if (sin && k == 0)
-        DMPM_TRY(enqueue_resort<T>(P, M, sin, eid, s, sld));
+        DMPM_TRY(enqueue_resort<T>(P, M, sin, eid, s, sld, ovf));
       else
-        DMPM_TRY(enqueue_resort<T>(P, M, src, eid, s));
+        DMPM_TRY(enqueue_resort<T>(P, M, src, eid, s, -1, ovf));
     }
     a.src = src;
     a.dst = (k & 1) ? pa : shadow;
