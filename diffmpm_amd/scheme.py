@@ -53,7 +53,10 @@ class _MPMScheme(abc.ABC):
         if p is not None:
             return p.phase_forces(gravity, step, self.dt)
         import numpy as np
-        t = float(np.float32(step) * np.float32(self.dt))
+        import torch
+        # t = step * dt in the working precision (the fused phases do the same)
+        wt = np.float64 if self.mesh.particles[0].dtype == torch.float64 else np.float32
+        t = float(wt(step) * wt(self.dt))
         m = self.mesh
         m.apply_on_elements("compute_external_force")
         m.apply_on_elements("compute_body_force", args=(gravity,))

@@ -70,7 +70,13 @@ def build_oracle(spec, dtype=np.float32):
     trs = [om.ParticleTraction(t["pset"], np.asarray(t["pids"]),
                                oracle_function(t.get("function")), t["dir"], t["traction"])
            for t in spec["tractions"]]
-    mesh = om.Mesh(el, [ps], trs)
+    sets = [ps]
+    for es in spec.get("extra_sets", []):  # further particle sets on the same mesh
+        q = np.asarray(es["particles"], dtype=np.float64).reshape(len(es["particles"]), -1)
+        e = om.Particles(q, oracle_material(es["material"]), dtype=dtype)
+        e.velocity[:] = _velocity(es, len(q), q.shape[1]).astype(dtype)
+        sets.append(e)
+    mesh = om.Mesh(el, sets, trs)
     return om.MPMExplicit(mesh, spec["dt"], spec["scheme"], spec["velocity_update"],
                           spec["nsteps"])
 
@@ -121,7 +127,15 @@ def build_gpu(spec, dtype=None, keep_temporaries=True):
     trs = [d.ParticleTraction(t["pset"], torch.tensor(t["pids"]),
                               gpu_function(t.get("function")), t["dir"], t["traction"])
            for t in spec["tractions"]]
-    mesh = Mesh({"particles": [ps], "elements": el, "particle_surface_traction": trs})
+    sets = [ps]
+    for es in spec.get("extra_sets", []):  # further particle sets on the same mesh
+        q = np.asarray(es["particles"], dtype=np.float64).reshape(len(es["particles"]), 1, ndim)
+        e = d.Particles(torch.from_numpy(q.astype(np_dt)), gpu_material(es["material"]),
+                        torch.zeros(len(q), dtype=torch.int32), device=dev, dtype=dtype)
+        e.velocity = torch.from_numpy(
+            _velocity(es, len(q), ndim).astype(np_dt).reshape(len(q), 1, ndim))
+        sets.append(e)
+    mesh = Mesh({"particles": sets, "elements": el, "particle_surface_traction": trs})
     return d.MPMExplicit(mesh, spec["dt"], spec["scheme"], spec["velocity_update"],
                          spec["nsteps"], keep_temporaries=keep_temporaries)
 
@@ -164,16 +178,17 @@ def normwise(a, b):
 
 
 def load_oracle_state_into_gpu(osim, gsim):
-    """Copies the oracle's full particle state and the persistent nodal
-    velocity into the GPU objects (for per-step, re-synchronised parity)."""
+    """Copies the oracle's full particle state (every particle set) and the
+    persistent nodal velocity into the GPU objects (for per-step,
+    re-synchronised parity)."""
     import torch
-    ops, gps = osim.mesh.particles[0], gsim.mesh.particles[0]
-    for name in PARTICLE_FIELDS + ["acceleration", "f_ext"]:
-        v = np.asarray(getattr(ops, name))
-        view = getattr(gps, name)
-        setattr(gps, name, torch.from_numpy(v.reshape(view.shape).astype(np.float64)))
-    gps._has_fext = bool(np.any(ops.f_ext))
-    gps.element_ids = torch.from_numpy(ops.element_ids.astype(np.int32))
+    for ops, gps in zip(osim.mesh.particles, gsim.mesh.particles):
+        for name in PARTICLE_FIELDS + ["acceleration", "f_ext"]:
+            v = np.asarray(getattr(ops, name))
+            view = getattr(gps, name)
+            setattr(gps, name, torch.from_numpy(v.reshape(view.shape).astype(np.float64)))
+        gps._has_fext = bool(np.any(ops.f_ext))
+        gps.element_ids = torch.from_numpy(ops.element_ids.astype(np.int32))
     gn, on = gsim.mesh.elements.nodes, osim.mesh.elements.nodes
     for name in NODE_FIELDS:
         v = np.asarray(getattr(on, name))

@@ -92,6 +92,69 @@ def test_per_step_parity(variant, dtype):
         _compare(osim, gsim, rtol, f"step {i}")
 
 
+def _two_set_spec(scheme):
+    """Two particle sets of the same size on one 8x6 mesh, both over every
+    cell: E = 1e6 and E = 3e6 (nu = 0.25), own jitter and velocities.  The
+    reference maps each set to the nodes after zeroing the nodal arrays, so
+    the nodal quantities of a step are the last set's (element.py:143, 175,
+    225, 586, 887) and the first set moves with them; the second set covers
+    every node, so no node is massless.  Both sides run the sets in list
+    order (solve_jit stacks the sets' arrays, so they have one size, as in
+    the reference's solver.py:214-222)."""
+    from oracle.mpm import gauss_points_2d
+    spec = cases.synthetic_spec(nx=8, ny=6, rows=6, seed=13, scheme=scheme, traction=False,
+                                nsteps=6)
+    h = spec["el_len"][0]
+    second = gauss_points_2d(8, 6, h, h, rows=6, jitter=0.2, seed=14, dtype=np.float64)
+    rng = np.random.Generator(np.random.PCG64(15))
+    spec["extra_sets"] = [dict(particles=second.reshape(len(second), -1).tolist(),
+                               init_velocity=(0.3 * rng.standard_normal((len(second), 2))).tolist(),
+                               material={"type": "LinearElastic", "density": 1500.0,
+                                         "youngs_modulus": 3e6, "poisson_ratio": 0.25})]
+    return spec
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+@pytest.mark.parametrize("scheme", ["usf", "usl"])
+def test_two_particle_sets_per_step_parity(scheme, dtype):
+    """Every particle field of both sets and every nodal field after each step
+    (state re-synchronised from the oracle before each step)."""
+    spec = _two_set_spec(scheme)
+    npdt = np.float64 if dtype == torch.float64 else np.float32
+    rtol = RTOL_F64 if dtype == torch.float64 else RTOL_F32
+    osim = cases.build_oracle(spec, npdt)
+    gsim = cases.build_gpu(spec, dtype)
+    assert len(gsim.mesh.particles) == 2
+    g = np.asarray(spec["gravity"])
+    for i in range(spec["nsteps"]):
+        cases.load_oracle_state_into_gpu(osim, gsim)
+        osim.step(i, g)
+        gsim._step(i, torch.tensor(spec["gravity"], dtype=torch.float64))
+        for k, (ops, gps) in enumerate(zip(osim.mesh.particles, gsim.mesh.particles)):
+            assert np.array_equal(ops.element_ids, gps.element_ids.cpu().numpy()), (i, k)
+            for name in cases.PARTICLE_FIELDS:
+                err = cases.normwise(cases.gpu_particle_field(gps, name),
+                                     cases.oracle_particle_field(ops, name))
+                assert err <= rtol, (i, k, name, err)
+        for name in cases.NODE_FIELDS:
+            err = cases.normwise(cases.gpu_node_field(gsim.mesh.elements.nodes, name),
+                                 cases.oracle_node_field(osim.mesh.elements.nodes, name))
+            assert err <= rtol, (i, name, err)
+
+
+def test_two_particle_sets_solve_jit_equals_steps():
+    """solve_jit with two sets (the per-method step, one set after the
+    other) equals the same steps through MPMExplicit._step."""
+    spec = _two_set_spec("usf")
+    a, b = cases.build_gpu(spec), cases.build_gpu(spec)
+    g = torch.tensor(spec["gravity"], dtype=torch.float64)
+    a.solve_jit(g)
+    for i in range(spec["nsteps"]):
+        b._step(i, g)
+    for pa, pb in zip(a.mesh.particles, b.mesh.particles):
+        assert torch.equal(pa._buf.view(torch.int32), pb._buf.view(torch.int32))
+
+
 @pytest.mark.parametrize("variant", VARIANTS[:2] + VARIANTS[5:6],
                          ids=lambda v: "-".join(f"{k}={v[k]}" for k in v))
 def test_rollout_drift(variant):
