@@ -92,6 +92,32 @@ def test_per_step_parity(variant, dtype):
         _compare(osim, gsim, rtol, f"step {i}")
 
 
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+@pytest.mark.parametrize("scheme", ["usf", "usl"])
+def test_simple_material_per_step_parity(scheme, dtype):
+    """SimpleMaterial (material.py:119-131: sigma += E * dstrain) through the
+    fused phases, per step against the oracle."""
+    spec = cases.synthetic_spec(nsteps=6, scheme=scheme, seed=17)
+    spec["material"] = {"type": "SimpleMaterial", "E": 1e6, "density": 1000.0}
+    npdt = np.float64 if dtype == torch.float64 else np.float32
+    rtol = RTOL_F64 if dtype == torch.float64 else RTOL_F32
+    osim = cases.build_oracle(spec, npdt)
+    gsim = cases.build_gpu(spec, dtype)
+    g = np.asarray(spec["gravity"])
+    for i in range(spec["nsteps"]):
+        cases.load_oracle_state_into_gpu(osim, gsim)
+        osim.step(i, g)
+        gsim._step(i, torch.tensor(spec["gravity"], dtype=torch.float64))
+        _compare(osim, gsim, rtol, f"step {i}")
+
+
+@pytest.mark.parametrize("scheme", ["usf", "usl"])
+def test_simple_material_one_launch_step_equals_phase_launches(scheme):
+    spec = _moving_spec(scheme, drift=(0.6, 0.3))
+    spec["material"] = {"type": "SimpleMaterial", "E": 1e3, "density": 1000.0}
+    _run_pair(spec, scheme, 8, 4)
+
+
 def _two_set_spec(scheme):
     """Two particle sets of the same size on one 8x6 mesh, both over every
     cell: E = 1e6 and E = 3e6 (nu = 0.25), own jitter and velocities.  The
