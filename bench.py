@@ -4,10 +4,12 @@ Workload: 2D Quadrilateral4Node 1024x1024 cells on [0,1]^2, 4 particles per
 cell at jittered (+-0.25 h, seed 0) Gauss points: 4,194,304 particles,
 LinearElastic E=1e6, nu=0.3, rho=1000, gravity (0,-9.81), bottom nodes fixed,
 dt=1e-5, USF, FP32.  W warm-up steps, then K timed steps bracketed by
-synchronisation; the step inputs are resident in HBM.
+synchronisation (the throughput); the step inputs are resident in HBM.  The
+next K steps are timed again with HIP events around every kernel class for
+the per-kernel durations of the roofline.
 
 Prints one JSON line with particle-steps/s, the roofline of the dominant
-kernel (HIP-event durations measured inside the timed region) and the CPU
+kernel (HIP-event durations measured inside the second timed region) and the CPU
 baseline (the oracle, a NumPy restatement of the reference step, timed on a
 bounded strip of the same workload on this host).
 """
@@ -52,7 +54,7 @@ ADJOINT_BYTES = 4 * 9 + 4 * 8 * 2 + 4
 # passes of this benchmark: reads = TCC_EA0_RDREQ
 # x 128 B, writes = WRITE_SIZE (calibrated on a 512 MiB device copy; see
 # profiles/step_q4_1024_usf_hbm_counters.md)
-TRAFFIC_BYTES = {"step": 161.9, "sort": 23.6, "tile_kin": 57.7, "stress": 91.0,
+TRAFFIC_BYTES = {"step": 152.8, "sort": 22.7, "tile_kin": 57.7, "stress": 91.0,
                  "tile_force": 86.4, "tile_g2p": 38.1}
 
 
@@ -163,17 +165,26 @@ def fwd_adjoint(sim, gravity, K):
     state0.grad = None
     torch.cuda.synchronize()
     aplan = d.fplan if d.save_forward else d.aplan  # the plan the adjoint runs on
-    timing_enable(aplan, "adj")
     stream = torch.cuda.current_stream()
     e0, e1, e2 = (torch.cuda.Event(enable_timing=True) for _ in range(3))
-    torch.cuda.synchronize()
-    e0.record(stream)
-    out = rollout(d, K, gravity, state=state0, log_e=log_e)
-    e1.record(stream)
-    out[0:2].sum().backward()
-    e2.record(stream)
-    torch.cuda.synchronize()
-    fwd_ms, bwd_ms = e0.elapsed_time(e1), e1.elapsed_time(e2)
+
+    def timed():
+        log_e.grad = None
+        state0.grad = None
+        torch.cuda.synchronize()
+        e0.record(stream)
+        out = rollout(d, K, gravity, state=state0, log_e=log_e)
+        e1.record(stream)
+        out[0:2].sum().backward()
+        e2.record(stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1), e1.elapsed_time(e2)
+
+    # throughput: no events between the kernels; then the adjoint kernels'
+    # durations from a second rollout with HIP events around them
+    fwd_ms, bwd_ms = timed()
+    timing_enable(aplan, "adj")
+    timed()
     adj_ms, adj_n = timing_read(aplan)
     timing_enable(aplan, None)
     assert torch.isfinite(log_e.grad).all() and torch.isfinite(state0.grad).all()
@@ -238,15 +249,24 @@ def main():
     plan.run("usf", False, gravity, sim.dt, 0, args.warmup, keep_temporaries=False)
     torch.cuda.synchronize()
     print("# warm-up done", file=sys.stderr, flush=True)
-    timing_enable(plan, "all")
     stream = torch.cuda.current_stream()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
-    e0.record(stream)
-    plan.run("usf", False, gravity, sim.dt, args.warmup, args.steps, keep_temporaries=False)
-    e1.record(stream)
-    torch.cuda.synchronize()
-    ms = e0.elapsed_time(e1)
+
+    def timed(step0):
+        torch.cuda.synchronize()
+        e0.record(stream)
+        plan.run("usf", False, gravity, sim.dt, step0, args.steps, keep_temporaries=False)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1)
+
+    # throughput: K steps with no events between the kernels
+    ms = timed(args.warmup)
+    # per-kernel durations: the next K steps with HIP events recorded around
+    # each kernel class on the launch stream (the events add gaps between
+    # launches, so this region is not the one the throughput comes from)
+    timing_enable(plan, "all")
+    ms_instr = timed(args.warmup + args.steps)
     kbytes_tab = dict(KERNEL_BYTES)
     if args.usf_fused:
         kbytes_tab["tile_force"] = TILE_FORCE_FUSED_BYTES
@@ -277,6 +297,7 @@ def main():
                                f"{N} particles, USF, LinearElastic, gravity, dt=1e-5",
                    "particles": N, "scheme": "usf"},
         "step_effective_GBs": STEP_BYTES * N * args.steps / (ms / 1e3) / 1e9,
+        "ms_per_step_with_kernel_events": ms_instr / args.steps,
         "roofline": {"bound": "hbm", "kernel": kname, "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": (traffic * N) if (traffic and args.n == 1024

@@ -18,7 +18,9 @@ FLAGS = [
     "-O3",
     "-std=c++17",
     "-fPIC",
-    "-ffp-contract=off",
+    # a * b + c within one source expression becomes one fused multiply-add
+    # (llvm.fmuladd), never across statements
+    "-ffp-contract=on",
     # Keep the compiler from emitting packed FP32 (v_pk_*_f32) instructions.
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-result",

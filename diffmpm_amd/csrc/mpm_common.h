@@ -1,9 +1,12 @@
 // Device helpers shared by the diffmpm_amd HIP sources (gfx950).
 //
-// All sources are compiled with -ffp-contract=off so every expression rounds
-// as written: the per-method kernels, the fused step kernels and the adjoint
-// recompute produce bit-identical values for the same inputs, independent of
-// inlining, and the operation order can follow the reference's.
+// All sources are compiled with -ffp-contract=on: a product added within the
+// same source expression (a * b + c) is one fused multiply-add, and nothing is
+// fused across statements.  Rounding is therefore fixed by the source text, so
+// the per-method kernels, the fused step kernels and the adjoint recompute,
+// which write each expression the same way, produce bit-identical values for
+// the same inputs, independent of inlining; the operation order follows the
+// reference's.
 #pragma once
 
 #include <hip/hip_runtime.h>

@@ -974,14 +974,23 @@ __device__ __forceinline__ void node_sums(T* sl, const T (&acc)[C][4 * NQ], cons
     if (ln >= nn) continue;
     const int jr = sdiv(dwn, ln);
     const int i = RN.i0 + (ln - jr * wn), j = RN.j0 + jr;
+    // every corner's partials are read first (cells outside the mesh read
+    // cell 0 of RC and are not added), so the reads overlap
+    T pv[4][NQ];
+    bool in[4];
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
       const int ci = i + dI[c], cj = j + dJ[c];
-      if (ci < 0 || ci >= nx || cj < 0 || cj >= ny) continue;
-      const int lc = (ci - RC.i0) + (cj - RC.j0) * wc;
+      in[c] = ci >= 0 && ci < nx && cj >= 0 && cj < ny;
+      const int lc = in[c] ? (ci - RC.i0) + (cj - RC.j0) * wc : 0;
 #pragma unroll
-      for (int q = 0; q < NQ; ++q) nacc[w][q] = nacc[w][q] + sl[(NQ * corner[c] + q) * nc + lc];
+      for (int q = 0; q < NQ; ++q) pv[c][q] = sl[(NQ * corner[c] + q) * nc + lc];
     }
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+      if (in[c])
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) nacc[w][q] = nacc[w][q] + pv[c][q];
   }
   __syncthreads();
 }

@@ -168,12 +168,20 @@ def oracle_node_field(nodes, name):
 
 
 def normwise(a, b):
+    """||a - b|| / ||b|| (||a - b|| when b is zero), with both vectors scaled by
+    their largest finite magnitude first, so that values near the float range
+    (nan_to_num's +-max of a massless node under a nodal force) do not
+    overflow the sums of squares."""
     a = np.asarray(a, np.float64)
     b = np.asarray(b, np.float64)
-    den = np.linalg.norm(b)
-    num = np.linalg.norm(a - b)
+    fin = np.concatenate([np.abs(a[np.isfinite(a)]), np.abs(b[np.isfinite(b)])])
+    s = float(fin.max()) if fin.size else 0.0
+    if s == 0.0 or not np.isfinite(s):
+        s = 1.0
+    den = np.linalg.norm(b / s)
+    num = np.linalg.norm((a - b) / s)
     if den == 0:
-        return num
+        return num * s
     return num / den
 
 

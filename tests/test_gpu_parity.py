@@ -450,6 +450,42 @@ def test_tile_path_many_arrivals_into_one_cell(scheme):
 
 
 @pytest.mark.parametrize("scheme", ["usf", "usl"])
+def test_incremental_resort_is_the_stable_counting_sort(scheme):
+    """After each fused step of the multi-tile moving case, the sort by cell
+    that the step used (exported: slot -> particle, first slot of each cell)
+    equals the stable counting sort of the oracle's element ids of the
+    step-start positions: cells in order, ascending particle index within a
+    cell, particles outside the mesh last."""
+    from diffmpm_amd._plan import plan_for
+    from diffmpm_amd._lib import call, ptr
+    spec = _moving_spec(scheme)
+    sim = cases.build_gpu(spec)
+    el = cases.build_oracle(spec, np.float32).mesh.elements
+    ps, nodes = sim.mesh.particles[0], sim.mesh.elements.nodes
+    plan = plan_for(sim.mesh.elements, ps)
+    g = torch.tensor(spec["gravity"], dtype=torch.float64)
+    ncells = spec["nelements"][0] * spec["nelements"][1]
+    n, dev = len(ps), ps.loc.device
+    keys = []
+    for k in range(spec["nsteps"]):
+        loc = ps.loc.detach().cpu().numpy().astype(np.float32).reshape(n, 2)
+        e = np.asarray(el.element_ids(loc))
+        key = np.where((e >= 0) & (e < ncells), e, ncells)
+        keys.append(key)
+        plan.run(scheme, False, g, spec["dt"], k, 1, True)
+        vkin = torch.empty((2, nodes._buf.shape[1]), dtype=nodes._buf.dtype, device=dev)
+        perm = torch.empty(n, dtype=torch.int32, device=dev)
+        cs = torch.empty(ncells + 2, dtype=torch.int32, device=dev)
+        call("dmpm_plan_export_step", plan.h, ptr(vkin), ptr(perm), ptr(cs), plan.s)
+        torch.cuda.synchronize()
+        want_cs = np.concatenate([[0], np.cumsum(np.bincount(key, minlength=ncells + 1))])
+        assert np.array_equal(perm.cpu().numpy(), np.argsort(key, kind="stable")), k
+        assert np.array_equal(cs.cpu().numpy()[:ncells + 2], want_cs), k
+    moved = sum(int((a != b).sum()) for a, b in zip(keys, keys[1:]))
+    assert moved > 50  # incremental re-sorts with cell changes
+
+
+@pytest.mark.parametrize("scheme", ["usf", "usl"])
 def test_tile_path_crowded_cell_equals_per_phase_path(scheme):
     """3000 particles in one cell (rank-sorted by one workgroup in the full
     re-sort): the fused path equals the per-phase path bit for bit."""
