@@ -56,7 +56,7 @@ class Differentiable:
         from .ensemble import BatchPlan
         d = cls.__new__(cls)
         d.sim, d.ps, d.el = None, None, None
-        d.bplan, d._blocks = None, None
+        d.bplan, d._blocks, d._nodes_last = None, None, None
         d.per_particle_E = all(s.mesh.particles[0]._has_youngs for s in ens.sims)
         d.n = ens.n
         d._pbuf, d._nbuf, d._eid_f = ens.pbuf, ens.nbuf, ens.eid
@@ -94,6 +94,7 @@ class Differentiable:
         # from one to the other, no copy of the state); own sort state
         self.bplan = Plan(el, ps)
         self._blocks = None
+        self._nodes_last = None  # node block of the last private-block step
         self._eid_b = self._eid_f.clone()
         self._finish(sim.scheme, sim.velocity_update, sim.dt)
 
@@ -183,19 +184,39 @@ class Differentiable:
                 sin = sin.contiguous()
             out = torch.empty((C_ROWS, self.n), dtype=self._pbuf.dtype,
                               device=self._pbuf.device)
+            nb = self._nbuf
+            if save:
+                # a fresh node block per saved step, kept for its adjoint: the
+                # step writes mass, velocity, acceleration, momentum and the
+                # forces of every node; the rows it reads (node coordinates,
+                # the velocity a massless node keeps, damping force) come from
+                # the previous step's block
+                prev = self._nodes_last if chained and self._nodes_last is not None else self._nbuf
+                nb = torch.empty_like(self._nbuf)
+                nb[_lib.N_LOC:_lib.N_A] = prev[_lib.N_LOC:_lib.N_A]
+                nb[_lib.N_FDAMP:_lib.N_ROWS] = prev[_lib.N_FDAMP:_lib.N_ROWS]
             call("dmpm_step_between", plan.h, ptr(self._blocks[src]), ptr(self._blocks[dst]),
-                 ptr(self._eid_b), ptr(self._nbuf), self.code, self.vu, plan._gravity(gravity),
+                 ptr(self._eid_b), ptr(nb), self.code, self.vu, plan._gravity(gravity),
                  self.dt, int(i), 2 if chained else 0, ptr(sin), ptr(out), self.n, plan.s)
             self._last_b = (out, out._version, dst)
+            self._nodes_last = nb
+            if save:
+                vkin, perm, cs = self._export(plan)
+                return out, (nb, vkin, perm, cs)
         if not save:
             return out, None
         nsaved = self._nbuf.clone()
+        vkin, perm, cs = self._export(plan)
+        return out, (nsaved, vkin, perm, cs)
+
+    def _export(self, plan):
+        """The kinematic nodal velocity and the sort by cell of the plan's last step."""
         vkin = torch.empty((2, self._nbuf.shape[1]), dtype=self._nbuf.dtype,
                            device=self._nbuf.device)
         perm = torch.empty(max(self.n, 1), dtype=torch.int32, device=self._nbuf.device)
         cs = torch.empty(self._ncells() + 2, dtype=torch.int32, device=self._nbuf.device)
         call("dmpm_plan_export_step", plan.h, ptr(vkin), ptr(perm), ptr(cs), plan.s)
-        return out, (nsaved, vkin, perm, cs)
+        return vkin, perm, cs
 
     def _ncells(self):
         if self.el is not None:
