@@ -58,9 +58,20 @@ TRAFFIC_BYTES = {"step": 152.8, "sort": 22.7, "tile_kin": 57.7, "stress": 91.0,
                  "tile_force": 86.4, "tile_g2p": 38.1}
 
 
-def cpu_baseline(rows=32, steps=8):
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(rows=32, steps=24):
     """Oracle (NumPy restatement of the reference step, one core) on a
-    1024 x `rows`-cell strip of the same workload."""
+    1024 x `rows`-cell strip of the same workload (about 10 s of CPU work)."""
     import numpy as np
     sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
     from oracle import mpm as om
@@ -87,7 +98,8 @@ def cpu_baseline(rows=32, steps=8):
     if lim is not None:
         lim.unregister() if hasattr(lim, "unregister") else None
     return {"value": len(pts) * steps / el_s, "unit": "particle-steps/s", "cores": 1,
-            "kind": "port",
+            "kind": "port", "cpu_model": _cpu_model(),
+            "host_cores_available": len(os.sched_getaffinity(0)),
             "sample": f"oracle (NumPy FP32 restatement of the reference step, 1 thread) on a "
                       f"1024x{rows}-cell strip ({len(pts)} particles) of the same workload, "
                       f"{steps} steps after 1 warm-up ({el_s:.1f} s)"}
