@@ -43,6 +43,7 @@ namespace {
 
 constexpr int kSmallCell = 64;
 constexpr int kBigCellMax = 8192;
+constexpr int kResortBatch = 8;  // slots per thread whose loads the re-sort issues together
 
 #define PR(r) (pb + (int64_t)(r) * ld)
 #define NR(r) (nb + (int64_t)(r) * nld)
@@ -1983,43 +1984,78 @@ __device__ __forceinline__ int sort_mode(const SortArgs& a) {
 }
 
 __device__ __forceinline__ int blk_excl_scan(int v, int* sh, int& total) {
-  const int t = threadIdx.x;
-  sh[t] = v;
-  __syncthreads();
-  for (int off = 1; off < kThreads; off <<= 1) {
-    const int a = (t >= off) ? sh[t - off] : 0;
-    __syncthreads();
-    sh[t] += a;
-    __syncthreads();
+  // inclusive scan within each wave by shuffles, then the wave totals
+  static_assert(kThreads % 64 == 0 && kThreads / 64 <= 64, "waves per workgroup");
+  constexpr int NW = kThreads / 64;
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  int x = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int y = __shfl_up(x, o, 64);
+    if (lane >= o) x += y;
   }
-  const int incl = sh[t];
-  total = sh[kThreads - 1];
+  if (lane == 63) sh[w] = x;
   __syncthreads();
-  return incl - v;
+  int before = 0, all = 0;
+#pragma unroll
+  for (int i = 0; i < NW; ++i) {
+    const int u = sh[i];
+    before += i < w ? u : 0;
+    all += u;
+  }
+  __syncthreads();  // sh is reused by the caller
+  total = all;
+  return before + x - v;
 }
 
 // out[c] = base(c) + (sum of chunk totals before this chunk) + exclusive
 // prefix of val over the chunk, for the cells of chunk blockIdx.x (a run of
 // chunk/256 consecutive cells per thread); the last chunk also writes
-// out[nbk] = total.
+// out[nbk] = total.  The loads of a thread's cells, and of the chunk totals,
+// are issued together.
 template <typename V, typename B>
 __device__ __forceinline__ void chunk_scan(int nbk, int nchunks, int shift,
                                            const int* __restrict__ tot, V val, B base,
                                            int* __restrict__ out, int total, int* sh) {
   const int b = blockIdx.x, tid = threadIdx.x;
   int pre = 0;
-  for (int k = tid; k < b; k += kThreads) pre += tot[k];
+  {
+    int k = tid;
+    for (; k + 3 * kThreads < b; k += 4 * kThreads)
+      pre += (tot[k] + tot[k + kThreads]) + (tot[k + 2 * kThreads] + tot[k + 3 * kThreads]);
+    for (; k < b; k += kThreads) pre += tot[k];
+  }
   int red;
   blk_excl_scan(pre, sh, red);
   const int per = (1 << shift) / kThreads;
   const int c0 = (b << shift) + tid * per, c1 = min(c0 + per, nbk);
-  int sum = 0;
-  for (int c = c0; c < c1; ++c) sum += val(c);
-  int bt;
-  int ex = red + blk_excl_scan(sum, sh, bt);
-  for (int c = c0; c < c1; ++c) {
-    out[c] = base(c) + ex;
-    ex += val(c);
+  constexpr int kPer = (1 << kMinChunkShift) / kThreads;
+  if (per == kPer) {
+    int v[kPer], bs[kPer], sum = 0;
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) {
+      const bool in = c0 + u < c1;
+      v[u] = in ? val(c0 + u) : 0;
+      bs[u] = in ? base(c0 + u) : 0;
+      sum += v[u];
+    }
+    int bt;
+    int ex = red + blk_excl_scan(sum, sh, bt);
+#pragma unroll
+    for (int u = 0; u < kPer; ++u)
+      if (c0 + u < c1) {
+        out[c0 + u] = bs[u] + ex;
+        ex += v[u];
+      }
+  } else {
+    int sum = 0;
+    for (int c = c0; c < c1; ++c) sum += val(c);
+    int bt;
+    int ex = red + blk_excl_scan(sum, sh, bt);
+    for (int c = c0; c < c1; ++c) {
+      out[c] = base(c) + ex;
+      ex += val(c);
+    }
   }
   if (b == nchunks - 1 && tid == 0) out[nbk] = total;
 }
@@ -2122,21 +2158,37 @@ __global__ __launch_bounds__(kThreads) void k_sort_s2(SortArgs a) {
   const int* __restrict__ dL = a.L.dL;
   const int* __restrict__ dA = a.L.dA;
   const int gt = blockIdx.x * kThreads + threadIdx.x, gs = gridDim.x * kThreads;
-  for (int s = gt; s < a.n; s += gs) {
-    const int p = po[s];
-    const int c = a.key[p];
-    int pos;
-    if ((dL[c] | dA[c]) == 0) {
-      pos = s + (cn[c] - co[c]);
-    } else {
-      if (a.L.mrec[p]) continue;  // placed by the mover loop below
-      int before = 0, arr = 0;
-      for (int m = a.L.headL[c]; m >= 0; m = M[kMovRec * m + 3])
-        before += M[kMovRec * m] < p;  // old segment ascends in particle index
-      for (int m = a.L.headA[c]; m >= 0; m = M[kMovRec * m + 4]) arr += M[kMovRec * m] < p;
-      pos = cn[c] + (s - co[c]) - before + arr;
+  // kResortBatch slots per thread at a time, each level of loads (slot ->
+  // particle -> cell -> the cell's counts and offsets) issued together
+  for (int sb = gt; sb < a.n; sb += kResortBatch * gs) {
+    int pv[kResortBatch], cv[kResortBatch], tv[kResortBatch], dv[kResortBatch];
+#pragma unroll
+    for (int u = 0; u < kResortBatch; ++u) {
+      const int s = sb + u * gs;
+      pv[u] = s < a.n ? po[s] : 0;
     }
-    pn[pos] = p;
+#pragma unroll
+    for (int u = 0; u < kResortBatch; ++u) cv[u] = sb + u * gs < a.n ? a.key[pv[u]] : 0;
+#pragma unroll
+    for (int u = 0; u < kResortBatch; ++u) {
+      const int c = cv[u];
+      tv[u] = dL[c] | dA[c];
+      dv[u] = cn[c] - co[c];
+    }
+#pragma unroll
+    for (int u = 0; u < kResortBatch; ++u) {
+      const int s = sb + u * gs;
+      if (s >= a.n) continue;
+      const int p = pv[u], c = cv[u];
+      int pos = s + dv[u];
+      if (tv[u]) {
+        if (a.L.mrec[p]) continue;  // placed by the mover loop below
+        // the old segment ascends in particle index
+        for (int m = a.L.headL[c]; m >= 0; m = M[kMovRec * m + 3]) pos -= M[kMovRec * m] < p;
+        for (int m = a.L.headA[c]; m >= 0; m = M[kMovRec * m + 4]) pos += M[kMovRec * m] < p;
+      }
+      pn[pos] = p;
+    }
   }
   const int nmov = a.L.flags[5];
   for (int m = gt; m < nmov; m += gs) {
