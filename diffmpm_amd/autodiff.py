@@ -188,12 +188,13 @@ class Differentiable:
             if save:
                 # a fresh node block per saved step, kept for its adjoint: the
                 # step writes mass, velocity, acceleration, momentum and the
-                # forces of every node; the rows it reads (node coordinates,
-                # the velocity a massless node keeps, damping force) come from
-                # the previous step's block
+                # forces of every node; the rows it reads (the velocity a
+                # massless node keeps, the damping force) come from the
+                # previous step's block.  Its coordinate rows are not set:
+                # neither the step nor the adjoint reads them.
                 prev = self._nodes_last if chained and self._nodes_last is not None else self._nbuf
                 nb = torch.empty_like(self._nbuf)
-                nb[_lib.N_LOC:_lib.N_A] = prev[_lib.N_LOC:_lib.N_A]
+                nb[_lib.N_V:_lib.N_A] = prev[_lib.N_V:_lib.N_A]
                 nb[_lib.N_FDAMP:_lib.N_ROWS] = prev[_lib.N_FDAMP:_lib.N_ROWS]
             call("dmpm_step_between", plan.h, ptr(self._blocks[src]), ptr(self._blocks[dst]),
                  ptr(self._eid_b), ptr(nb), self.code, self.vu, plan._gravity(gravity),
@@ -237,14 +238,21 @@ class Differentiable:
         cot[:, : self.n] = gout
         return cot
 
-    def adjoint_step_saved(self, state, log_e, out, saved, gravity, gout):
+    def _lebar(self, lebar):
+        """The dL/dlog(E_p) buffer the adjoint kernels add into: the caller's
+        (an accumulation over several steps) or a new zero one."""
+        if lebar is not None:
+            return lebar
+        return torch.zeros(self._pbuf.shape[1], dtype=self._pbuf.dtype, device=self._pbuf.device)
+
+    def adjoint_step_saved(self, state, log_e, out, saved, gravity, gout, lebar=None):
         """Like adjoint_step, from the quantities forward_step(save=True)
         returned (no recomputation of the step)."""
         nsaved, vkin, perm, cs = saved
         st_in, st_out = state.detach(), out.detach()
         ld = self._pbuf.shape[1]
         cot = self._cot_buffer(gout)
-        lebar = torch.zeros(ld, dtype=self._pbuf.dtype, device=self._pbuf.device)
+        lebar = self._lebar(lebar)
         if st_in.is_contiguous() and st_out.is_contiguous():
             # the kernels read the state tensors; mass and E from pin
             if log_e is not None:
@@ -264,28 +272,39 @@ class Differentiable:
              self.fplan._gravity(gravity), self.dt, self.fplan.s)
         return cot[:, : self.n], lebar[: self.n]
 
-    def adjoint_step(self, state, log_e, gravity, i, gout):
+    def adjoint_step(self, state, log_e, gravity, i, gout, lebar=None):
         """Cotangents (state, log E per particle) of step i's input from the
-        cotangent of its output."""
+        cotangent of its output; the log E terms are added to lebar when
+        given."""
         self.aplan.refresh_tables()
         self._write(self.pin, state.detach(), log_e)
-        ld = self._pbuf.shape[1]
         cot = self._cot_buffer(gout)
-        lebar = torch.zeros(ld, dtype=self._pbuf.dtype, device=self._pbuf.device)
+        lebar = self._lebar(lebar)
         call("dmpm_step_adjoint", self.aplan.h, ptr(self.pin), ptr(self.pwork), ptr(self.eid),
              ptr(self.nwork), ptr(cot), ptr(lebar), self.code, self.vu,
              self.aplan._gravity(gravity), self.dt, int(i), self.aplan.s)
         return cot[:, : self.n], lebar[: self.n]
 
 
+class _LogEGrad:
+    """dL/dlog(E_p) of one rollout: the adjoint kernels of all its steps add
+    into one buffer (in the order autograd would sum the per-step terms), and
+    the rollout's first step, the last one the backward pass reaches, hands
+    the sum to autograd."""
+
+    def __init__(self):
+        self.buf = None
+
+
 class _StepFn(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, state, log_e, d, gravity, i):
+    def forward(ctx, state, log_e, d, gravity, i, acc=None, emit=False):
         save = d.save_forward
         out, saved = d.forward_step(state, log_e, gravity, i, save=save)
         ctx.d, ctx.gravity, ctx.i = d, gravity, i
         ctx.has_log_e = log_e is not None
         ctx.saved_fwd = save
+        ctx.acc, ctx.emit = acc, emit
         extra = (out,) + saved if save else ()
         ctx.save_for_backward(state, log_e if log_e is not None else state.new_empty(0), *extra)
         return out
@@ -295,16 +314,25 @@ class _StepFn(torch.autograd.Function):
         state, log_e, *extra = ctx.saved_tensors
         d = ctx.d
         le = log_e if ctx.has_log_e else None
+        want_e = ctx.has_log_e and ctx.needs_input_grad[1]
+        acc = ctx.acc if want_e and log_e.dim() > 0 else None
+        lebar = None
+        if acc is not None:
+            if acc.buf is None:
+                acc.buf = d._lebar(None)
+            lebar = acc.buf
         if ctx.saved_fwd:
             gs, ge = d.adjoint_step_saved(state, le, extra[0], tuple(extra[1:]), ctx.gravity,
-                                          gout)
+                                          gout, lebar)
         else:
-            gs, ge = d.adjoint_step(state, le, ctx.gravity, ctx.i, gout)
+            gs, ge = d.adjoint_step(state, le, ctx.gravity, ctx.i, gout, lebar)
         g_log_e = None
-        if ctx.has_log_e and ctx.needs_input_grad[1]:
+        if want_e and (acc is None or ctx.emit):
+            if acc is not None:
+                acc.buf = None  # a later backward pass starts a new sum
             g_log_e = ge if log_e.dim() > 0 else ge.sum()
             g_log_e = g_log_e.to(log_e.dtype).reshape(log_e.shape)
-        return gs, g_log_e, None, None, None
+        return gs, g_log_e, None, None, None, None, None
 
 
 def step(d, state, gravity, i, log_e=None):
@@ -322,10 +350,12 @@ def rollout(d, nsteps, gravity, state=None, log_e=None, start_step=0, checkpoint
     state.  Use the returned tensors."""
     if state is None:
         state = d.get_state()
+    # per-particle log E: one gradient buffer for all steps
+    acc = _LogEGrad() if log_e is not None and log_e.dim() > 0 else None
 
     def seg(st, le, i0, k):
         for i in range(i0, i0 + k):
-            st = _StepFn.apply(st, le, d, gravity, i)
+            st = _StepFn.apply(st, le, d, gravity, i, acc, i == start_step)
         return st
 
     if checkpoint_every and checkpoint_every < nsteps:
