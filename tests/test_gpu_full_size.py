@@ -1,0 +1,110 @@
+"""The throughput workload at full size: Q4 1024x1024 cells, 4,194,304
+jittered particles, USF, FP32, gravity, bottom nodes fixed, side rollers,
+dt = 1e-5 (``workloads.block_2d(1024)``, the configuration ``bench.py``
+measures).  The oracle cannot step 4.2 M particles within a test, so these
+tests check properties that hold at any size:
+
+* P2G partition of unity: the nodal masses sum to the particle mass and the
+  nodal y external forces to g_y times it (element.py:122-152, 236-268); the
+  internal forces are self-equilibrated (element.py:842-901: the shape
+  function gradients of a cell sum to zero);
+* the one-launch step equals the per-phase launches bit for bit;
+* the incremental re-sort used by each step equals the stable counting sort
+  of the element ids, computed here from the reference's definition
+  (element.py:820-840: the largest FP32 node coordinate at or below the
+  point, per axis) with searchsorted.
+"""
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+N_CELLS = 1024
+N_PARTICLES = 4 * N_CELLS * N_CELLS
+
+
+def _block():
+    from diffmpm_amd import workloads
+    sim, gravity = workloads.block_2d(N_CELLS)
+    assert len(sim.mesh.particles[0]) == N_PARTICLES
+    return sim, gravity
+
+
+def _bits(t):
+    t = t.clone()
+    t = torch.where(torch.isnan(t), torch.full_like(t, float("nan")), t)
+    return t.view(torch.int32)
+
+
+def test_full_size_p2g_conservation_and_equilibrium():
+    from diffmpm_amd._plan import plan_for
+    sim, g = _block()
+    ps, nodes = sim.mesh.particles[0], sim.mesh.elements.nodes
+    plan = plan_for(sim.mesh.elements, ps)
+    plan.run("usf", False, g, sim.dt, 0, 6, keep_temporaries=False)
+    torch.cuda.synchronize()
+    m_p = ps.mass.double().sum().item()
+    m_n = nodes.mass.double().sum().item()
+    assert abs(m_n - m_p) <= 1e-6 * m_p, (m_n, m_p)
+    gy = float(np.float32(-9.81))
+    f_ext = nodes.f_ext.double().reshape(-1, 2).sum(0)
+    assert f_ext[0].item() == 0.0
+    assert abs(f_ext[1].item() - gy * m_p) <= 1e-6 * abs(gy) * m_p, (f_ext, gy * m_p)
+    f_int = nodes.f_int.double().reshape(-1, 2)
+    scale = f_int.abs().sum(0)
+    assert (scale > 0).all()  # the stresses are no longer zero
+    assert (f_int.sum(0).abs() <= 1e-5 * scale).all(), (f_int.sum(0), scale)
+
+
+def test_full_size_one_launch_step_equals_phase_launches():
+    from diffmpm_amd._plan import plan_for
+    a, g = _block()
+    b, _ = _block()
+    pa = plan_for(a.mesh.elements, a.mesh.particles[0])
+    pb = plan_for(b.mesh.elements, b.mesh.particles[0])
+    pa.run("usf", False, g, a.dt, 0, 4, keep_temporaries=True)
+    pb.run("usf", False, g, b.dt, 0, 4, keep_temporaries=True, per_phase=True)
+    torch.cuda.synchronize()
+    sa, sb = a.mesh.particles[0], b.mesh.particles[0]
+    n = len(sa)
+    assert torch.equal(_bits(sa._buf[:, :n]), _bits(sb._buf[:, :n]))
+    assert torch.equal(_bits(a.mesh.elements.nodes._buf), _bits(b.mesh.elements.nodes._buf))
+    assert torch.equal(sa._eid[:n], sb._eid[:n])
+
+
+def test_full_size_resort_is_the_stable_counting_sort():
+    from diffmpm_amd._plan import plan_for
+    from diffmpm_amd._lib import call, ptr
+    sim, g = _block()
+    ps, nodes = sim.mesh.particles[0], sim.mesh.elements.nodes
+    plan = plan_for(sim.mesh.elements, ps)
+    n, nc, dev = len(ps), N_CELLS * N_CELLS, ps.loc.device
+    # 100 steps first: the block falls under gravity and particles start to
+    # cross cell edges (about a thousand per step)
+    plan.run("usf", False, g, sim.dt, 0, 100, keep_temporaries=False)
+    loc_n = nodes.loc.detach().reshape(-1, 2).cpu().numpy()
+    X = loc_n[: N_CELLS + 1, 0]
+    Y = loc_n[:: N_CELLS + 1, 1]
+    assert X.dtype == np.float32 and (np.diff(X) > 0).all() and (np.diff(Y) > 0).all()
+    moved = 0
+    prev = None
+    for k in range(100, 103):
+        loc = ps.loc.detach().reshape(n, 2).cpu().numpy()
+        i = np.searchsorted(X, loc[:, 0], side="right") - 1
+        j = np.searchsorted(Y, loc[:, 1], side="right") - 1
+        assert (i >= 0).all() and (i < N_CELLS).all() and (j >= 0).all() and (j < N_CELLS).all()
+        key = (j * N_CELLS + i).astype(np.int64)
+        plan.run("usf", False, g, sim.dt, k, 1, keep_temporaries=False)
+        vkin = torch.empty((2, nodes._buf.shape[1]), dtype=nodes._buf.dtype, device=dev)
+        perm = torch.empty(n, dtype=torch.int32, device=dev)
+        cs = torch.empty(nc + 2, dtype=torch.int32, device=dev)
+        call("dmpm_plan_export_step", plan.h, ptr(vkin), ptr(perm), ptr(cs), plan.s)
+        torch.cuda.synchronize()
+        want_cs = np.concatenate([[0], np.cumsum(np.bincount(key, minlength=nc + 1))])
+        assert np.array_equal(cs.cpu().numpy()[: nc + 2], want_cs), k
+        assert np.array_equal(perm.cpu().numpy(), np.argsort(key, kind="stable")), k
+        if prev is not None:
+            moved += int((prev != key).sum())
+        prev = key
+    assert moved > 100, moved  # incremental re-sorts with cell changes
