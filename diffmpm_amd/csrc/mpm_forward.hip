@@ -894,7 +894,10 @@ int dmpm_abi_version(void) { return 100; }
 // Allocates and initialises the sort state and the simulation table of a
 // plan whose sizes are set; frees it on failure.
 static int plan_init(dmpm_plan* P, dmpm_plan** out) {
-  hipGetDevice(&P->device);
+  if (hipGetDevice(&P->device) != hipSuccess) {
+    plan_free(P);
+    return fail(DMPM_EHIP, "hipGetDevice failed");
+  }
   int64_t n = std::max<int64_t>(P->n, 1);
   int nb = P->ncells + 1;
   int nbs = (nb + kScanChunk - 1) / kScanChunk;

@@ -117,7 +117,7 @@ struct dmpm_plan {
   void release_extra() {
     for (int i = 0; i < 16; ++i)
       if (extra[i]) {
-        hipFree(extra[i]);
+        (void)hipFree(extra[i]);  // release: nothing to do on failure
         extra[i] = nullptr;
         extra_bytes[i] = 0;
       }
