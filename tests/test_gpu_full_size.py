@@ -12,7 +12,11 @@ tests check properties that hold at any size:
 * the incremental re-sort used by each step equals the stable counting sort
   of the element ids, computed here from the reference's definition
   (element.py:820-840: the largest FP32 node coordinate at or below the
-  point, per axis) with searchsorted.
+  point, per axis) with searchsorted;
+* the virial identities of the internal and body forces, which follow from
+  the linear completeness of the bilinear shape functions: the node-weighted
+  sums of the nodal forces equal the volume-weighted particle stresses and
+  the mass-weighted particle heights.
 """
 import numpy as np
 import pytest
@@ -108,3 +112,53 @@ def test_full_size_resort_is_the_stable_counting_sort():
             moved += int((prev != key).sum())
         prev = key
     assert moved > 100, moved  # incremental re-sorts with cell changes
+
+
+def test_full_size_force_virial_identities():
+    """Bilinear shape functions reproduce linear fields (sum_i X_i N_i(x) = x,
+    sum_i X_i dN_i/dx = 1, sum_i X_i dN_i/dy = 0), so the reference's
+    internal force f_int_i = -sum_p V_p sigma_p . grad N_i (element.py:842-901)
+    and body force f_ext_i = sum_p N_i m_p g (element.py:236-268) satisfy, for
+    any particle state:
+        sum_i X_i f_int_x,i = -sum_p V_p sigma_xx,p
+        sum_i Y_i f_int_y,i = -sum_p V_p sigma_yy,p
+        sum_i Y_i f_int_x,i = sum_i X_i f_int_y,i = -sum_p V_p sigma_xy,p
+        sum_i Y_i f_ext_y,i = g_y sum_p m_p y_p   (y at the start of the step)
+    USF updates sigma and V before the force P2G and leaves them unchanged
+    after it, so the particle state after a step pairs with its node block."""
+    from diffmpm_amd._plan import plan_for
+    sim, g = _block()
+    ps, nodes = sim.mesh.particles[0], sim.mesh.elements.nodes
+    plan = plan_for(sim.mesh.elements, ps)
+    plan.run("usf", False, g, sim.dt, 0, 100, keep_temporaries=False)
+    X = nodes.loc.detach().double().reshape(-1, 2)
+
+    def check(y0=None):
+        f_int = nodes.f_int.detach().double().reshape(-1, 2)
+        sig = ps.stress.detach().double().reshape(-1, 6)
+        V = ps.volume.detach().double().reshape(-1)
+        want = {"xx": -(V * sig[:, 0]).sum(), "yy": -(V * sig[:, 1]).sum(),
+                "xy": -(V * sig[:, 3]).sum()}
+        got = {"xx": (X[:, 0] * f_int[:, 0]).sum(), "yy": (X[:, 1] * f_int[:, 1]).sum(),
+               "xy": (X[:, 1] * f_int[:, 0]).sum(), "yx": (X[:, 0] * f_int[:, 1]).sum()}
+        scale = float((V * sig[:, [0, 1, 3]].abs().sum(1)).sum())
+        assert scale > 0
+        for k, w in (("xx", want["xx"]), ("yy", want["yy"]), ("xy", want["xy"]),
+                     ("yx", want["xy"])):
+            # (measured: 1e-9 .. 7e-8 of the scale in FP32; the FP64 oracle: 1e-14)
+            assert abs(float(got[k] - w)) <= 1e-6 * scale, (k, float(got[k]), float(w), scale)
+        if y0 is not None:
+            f_ext = nodes.f_ext.detach().double().reshape(-1, 2)
+            m = ps.mass.detach().double().reshape(-1)
+            gy = float(np.float32(-9.81))
+            w = gy * float((m * y0).sum())
+            got_e = float((X[:, 1] * f_ext[:, 1]).sum())
+            assert abs(got_e - w) <= 1e-6 * abs(w), (got_e, w)  # (measured: 7e-9)
+
+    y0 = ps.loc.detach().double().reshape(-1, 2)[:, 1].clone()
+    plan.run("usf", False, g, sim.dt, 100, 1, keep_temporaries=False)
+    torch.cuda.synchronize()
+    check(y0)
+    plan.run("usf", False, g, sim.dt, 101, 2, keep_temporaries=False)
+    torch.cuda.synchronize()
+    check()
