@@ -68,13 +68,19 @@ def ensemble(nsims=64, seed=0, scheme="usl", dtype=torch.float32, device="cuda")
 
 def block_2d(n=1024, rows=None, jitter=0.25, seed=0, E=1e6, nu=0.3, density=1000.0,
              dt=1e-5, scheme="usf", velocity_update=False, dtype=torch.float32,
-             device="cuda", keep_temporaries=False, fix_bottom=(0, 1), side_rollers=True):
+             device="cuda", keep_temporaries=False, fix_bottom=(0, 1), side_rollers=True,
+             round_positions=None):
     """Returns (MPMExplicit, gravity) for the uniform 2D block case.  The
-    4.2M-particle throughput configuration is ``block_2d(1024)``."""
+    4.2M-particle throughput configuration is ``block_2d(1024)``.
+    round_positions (a NumPy dtype): round the generated positions to it
+    first, so that blocks built in different precisions start from the same
+    positions, cells and particle volumes."""
     import diffmpm_amd as d
     rows = n if rows is None else rows
     h = 1.0 / n
     pts = gauss_points(n, rows, h, jitter, seed)
+    if round_positions is not None:
+        pts = pts.astype(round_positions).astype(np.float64)
     bottom = torch.arange(n + 1)
     cons = [(bottom, d.Constraint(k, 0.0)) for k in fix_bottom]
     if side_rollers:

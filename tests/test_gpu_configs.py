@@ -182,3 +182,4 @@ def test_c4_inverse_problem_reduces_the_loss():
         losses.append(float(loss))
     assert all(math.isfinite(x) for x in losses)
     assert losses[-1] < 0.9 * losses[0], losses
+

@@ -162,3 +162,83 @@ def test_full_size_force_virial_identities():
     plan.run("usf", False, g, sim.dt, 101, 2, keep_temporaries=False)
     torch.cuda.synchronize()
     check()
+
+
+def test_full_size_fp32_drift_from_fp64():
+    """The throughput workload run for 1000 steps in FP32 and in FP64 (same
+    kernels, same FP32-representable initial positions, no
+    re-synchronisation): the normwise drift of the particle displacements,
+    velocities and stresses, reported at steps 200 and 1000.
+
+    Displacements are differences of FP32 positions near 0.5 that move by
+    about 2e-7 per step (the FP32 spacing there is 6e-8), so their own
+    resolution is about 1e-3 relative.  Velocity and stress drift grows with
+    the mesh resolution: particles crossing cell edges amplify small
+    differences.  The FP32 and FP64 oracles drift apart exactly as the two GPU
+    runs do: on a 32x32 block 3.2e-5 (velocity) and 9.4e-5 (stress) after 1000
+    steps, and on 128x128 8.8e-5 and 2.3e-4 after 200 steps.  At 1024x1024 the
+    GPU measures about 0.12 and 0.3 after 1000 steps.  The bound here is a
+    sanity bound; test_jittered_block_rollout_against_fp64_oracle checks the
+    drift against the oracle where the oracle can run."""
+    from diffmpm_amd import workloads
+    from diffmpm_amd._plan import plan_for
+    # both from the FP32-rounded positions: the same cells and particle volumes
+    sims = [workloads.block_2d(N_CELLS, dtype=dt, round_positions=np.float32)[0]
+            for dt in (torch.float32, torch.float64)]
+    g = torch.tensor([0.0, -9.81])
+    plans = [plan_for(s.mesh.elements, s.mesh.particles[0]) for s in sims]
+    x0 = sims[1].mesh.particles[0].loc.detach().double().clone()
+    done, drift = 0, {}
+    for upto in (200, 1000):
+        for s, p in zip(sims, plans):
+            p.run("usf", False, g, s.dt, done, upto - done, keep_temporaries=False)
+        done = upto
+        torch.cuda.synchronize()
+        a, b = (s.mesh.particles[0] for s in sims)
+        fields = {"displacement": (a.loc.double() - x0, b.loc.double() - x0),
+                  "velocity": (a.velocity.double(), b.velocity.double()),
+                  "stress": (a.stress.double(), b.stress.double())}
+        for name, (u, v) in fields.items():
+            assert torch.isfinite(u).all() and torch.isfinite(v).all(), (upto, name)
+            drift[(upto, name)] = float((u - v).norm() / v.norm())
+    print("FP32 vs FP64 drift: " + ", ".join(f"step {k[0]} {k[1]} {v:.2e}"
+                                             for k, v in drift.items()))
+    assert drift[(200, "velocity")] < 1e-2 and drift[(200, "stress")] < 5e-2, drift
+    assert max(drift.values()) < 1.0, drift
+
+
+def test_jittered_block_rollout_against_fp64_oracle():
+    """1000 FP32 steps of the throughput workload's configuration on a 32x32
+    block (4096 jittered particles, side rollers, gravity) against the FP64
+    oracle, without re-synchronisation.  The FP32 oracle itself drifts from
+    the FP64 oracle by 3.2e-5 (velocity) and 9.4e-5 (stress) here; the GPU
+    must stay within 10 times that."""
+    from oracle import mpm as om
+    from diffmpm_amd import workloads
+    from diffmpm_amd._plan import plan_for
+    n, steps = 32, 1000
+    h = 1.0 / n
+    sim, g = workloads.block_2d(n, round_positions=np.float32)
+    ps = sim.mesh.particles[0]
+    pts = ps.loc.detach().double().reshape(-1, 2).cpu().numpy()
+    bottom, j = np.arange(n + 1), np.arange(n + 1)
+    cons = [(bottom, om.Constraint(0, 0.0)), (bottom, om.Constraint(1, 0.0)),
+            (np.concatenate([j * (n + 1), j * (n + 1) + n]), om.Constraint(0, 0.0))]
+    el = om.Elements("quad4", (n, n), (h, h), cons, [], np.float64)
+    ops = om.Particles(pts, om.LinearElastic({"density": 1000.0, "youngs_modulus": 1e6,
+                                              "poisson_ratio": 0.3}), dtype=np.float64)
+    osim = om.MPMExplicit(om.Mesh(el, [ops]), 1e-5, "usf", False, 1)
+    assert np.allclose(np.asarray(ops.volume).reshape(-1), ps.volume.double().cpu().numpy().reshape(-1), rtol=1e-6)
+    gv = np.array([0.0, -9.81])
+    for i in range(steps):
+        osim.step(i, gv)
+    plan_for(sim.mesh.elements, ps).run("usf", False, g, sim.dt, 0, steps,
+                                        keep_temporaries=False)
+    torch.cuda.synchronize()
+    drift = {}
+    for name, lim in (("velocity", 3.2e-4), ("stress", 9.4e-4)):
+        u = getattr(ps, name).detach().double().cpu().numpy().reshape(len(ps), -1)
+        v = np.asarray(getattr(ops, name), np.float64).reshape(len(ps), -1)
+        drift[name] = float(np.linalg.norm(u - v) / np.linalg.norm(v))
+        assert drift[name] < lim, (name, drift[name])
+    print(f"32x32 block, 1000 steps, GPU FP32 vs FP64 oracle: {drift}")
