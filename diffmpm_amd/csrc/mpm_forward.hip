@@ -838,7 +838,7 @@ void plan_free(dmpm_plan* P) {
   void* ptrs[] = {P->key, P->perm, P->counts, P->fill, P->cs, P->bsum, P->flags, P->zflags,
                   P->big,
                   P->perm2, P->cs2, P->mov, P->mrec, P->dL, P->dA, P->headL, P->headA,
-                  P->chunkD, P->chunkC, P->sims, P->tile_sim, P->clock,
+                  P->chunkD, P->chunkC, P->subT, P->sims, P->tile_sim, P->clock,
                   P->con_node, P->con_dir, P->con_val, P->nf_id, P->nf_dir, P->nf_off,
                   P->nf_fn, P->nf_val, P->tr_id, P->tr_dir, P->tr_off, P->tr_fn, P->tr_val,
                   P->fn_kind, P->fn_off, P->fn_cnt, P->fn_x, P->fn_f, P->cidx, P->cmask,
@@ -930,6 +930,8 @@ static int plan_init(dmpm_plan* P, dmpm_plan** out) {
   P->nchunks = (int)((nb + (1 << P->chunk_shift) - 1) >> P->chunk_shift);
   al(&P->chunkD, P->nchunks);
   al(&P->chunkC, P->nchunks);
+  P->nsub = (nb + (1 << kSubShift) - 1) >> kSubShift;
+  al(&P->subT, P->nsub);
   auto zero = [&](void* p, int64_t bytes, int v) {
     if (rc == DMPM_OK && p) rc = hip_check(hipMemset(p, v, bytes), "hipMemset");
   };

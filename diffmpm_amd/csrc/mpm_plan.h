@@ -68,6 +68,8 @@ struct dmpm_plan {
   int chunk_shift = 10;
   int* chunkD = nullptr;  // [nchunks] net arrivals per chunk (zero between sorts)
   int* chunkC = nullptr;  // [nchunks] particles per chunk in a full sort (zero between)
+  int nsub = 0;           // sub-chunks of kSubCells cells
+  int* subT = nullptr;    // [nsub] sub-chunk has recorded cell changes (incremental re-sort)
   // velocity constraints, last value per (node, dir)
   int ncon = 0;
   int* con_node = nullptr;
@@ -135,6 +137,7 @@ namespace dmpm {
 constexpr int kMovRec = 6;     // ints per recorded cell change
 constexpr int kListMax = 64;   // recorded leavers/arrivals per cell handled incrementally
 constexpr int kMinChunkShift = 10;     // at least 1024 cells (4 per thread) per scan chunk
+constexpr int kSubShift = 6;           // sub-chunks of 64 cells (one wave) in the incremental re-sort
 constexpr int kMaxChunks = 8192;       // LDS histogram bound of the full sort
 constexpr int kFlagWords = 16;
 

@@ -43,7 +43,6 @@ namespace {
 
 constexpr int kSmallCell = 64;
 constexpr int kBigCellMax = 8192;
-constexpr int kResortBatch = 8;  // slots per thread whose loads the re-sort issues together
 
 #define PR(r) (pb + (int64_t)(r) * ld)
 #define NR(r) (nb + (int64_t)(r) * nld)
@@ -1974,6 +1973,8 @@ struct SortArgs {
   MoveLog L;
   int n, ncells, nchunks;
   int* ovf;  // optional: the one-launch step's overflow counters, zeroed by stage 4
+  int* subT;  // [nsub] sub-chunk of 1 << kSubShift cells has recorded changes (INC)
+  int nsub;
 };
 
 __device__ __forceinline__ int sort_mode(const SortArgs& a) {
@@ -2094,9 +2095,57 @@ __global__ __launch_bounds__(kThreads) void k_sort_s1(Mesh2<T> Mg, const T* __re
     int* __restrict__ cn = cur ? a.cs0 : a.cs1;
     const int* __restrict__ dL = a.L.dL;
     const int* __restrict__ dA = a.L.dA;
-    chunk_scan(
-        nbk, a.nchunks, a.L.chunk_shift, a.L.chunkD, [&](int c) { return dA[c] - dL[c]; },
-        [&](int c) { return co[c]; }, cn, a.n, sh);
+    const int c0 = (int)blockIdx.x << a.L.chunk_shift;
+    if (a.L.chunk_shift == kMinChunkShift) {
+      // new offsets as chunk_scan computes them, and the chunk's sub-chunks
+      // with recorded changes (stage 2 moves the others whole), from one load
+      // of each cell's counts: 4 cells per thread, OR over the 16 threads of
+      // a sub-chunk
+      static_assert((1 << kMinChunkShift) == 4 * kThreads && (1 << kSubShift) == 64,
+                    "16 threads of 4 cells per sub-chunk");
+      const int b = blockIdx.x, tid = threadIdx.x;
+      int pre = 0;
+      {
+        int k = tid;
+        for (; k + 3 * kThreads < b; k += 4 * kThreads)
+          pre += (a.L.chunkD[k] + a.L.chunkD[k + kThreads]) +
+                 (a.L.chunkD[k + 2 * kThreads] + a.L.chunkD[k + 3 * kThreads]);
+        for (; k < b; k += kThreads) pre += a.L.chunkD[k];
+      }
+      int red;
+      blk_excl_scan(pre, sh, red);
+      const int cb = c0 + tid * 4;
+      int v[4], bs[4], sum = 0, tch = 0;
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const bool in = cb + u < nbk;
+        const int la = in ? dA[cb + u] : 0, ll = in ? dL[cb + u] : 0;
+        v[u] = la - ll;
+        bs[u] = in ? co[cb + u] : 0;
+        tch |= la | ll;
+        sum += v[u];
+      }
+      int bt;
+      int ex = red + blk_excl_scan(sum, sh, bt);
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+        if (cb + u < nbk) {
+          cn[cb + u] = bs[u] + ex;
+          ex += v[u];
+        }
+      if (b == a.nchunks - 1 && tid == 0) cn[nbk] = a.n;
+      tch = tch != 0;
+#pragma unroll
+      for (int o = 8; o >= 1; o >>= 1) tch |= __shfl_xor(tch, o, 64);
+      const int k = (c0 >> kSubShift) + (tid >> 4);
+      if ((tid & 15) == 0 && k < a.nsub) a.subT[k] = tch;
+    } else {
+      chunk_scan(
+          nbk, a.nchunks, a.L.chunk_shift, a.L.chunkD, [&](int c) { return dA[c] - dL[c]; },
+          [&](int c) { return co[c]; }, cn, a.n, sh);
+      const int k1 = min((c0 + (1 << a.L.chunk_shift)) >> kSubShift, a.nsub);
+      for (int k = (c0 >> kSubShift) + (int)threadIdx.x; k < k1; k += kThreads) a.subT[k] = 1;
+    }
     return;
   }
   const int tid = threadIdx.x;
@@ -2158,30 +2207,40 @@ __global__ __launch_bounds__(kThreads) void k_sort_s2(SortArgs a) {
   const int* __restrict__ dL = a.L.dL;
   const int* __restrict__ dA = a.L.dA;
   const int gt = blockIdx.x * kThreads + threadIdx.x, gs = gridDim.x * kThreads;
-  // kResortBatch slots per thread at a time, each level of loads (slot ->
-  // particle -> cell -> the cell's counts and offsets) issued together
-  for (int sb = gt; sb < a.n; sb += kResortBatch * gs) {
-    int pv[kResortBatch], cv[kResortBatch], tv[kResortBatch], dv[kResortBatch];
-#pragma unroll
-    for (int u = 0; u < kResortBatch; ++u) {
-      const int s = sb + u * gs;
-      pv[u] = s < a.n ? po[s] : 0;
+  // one sub-chunk of 64 cells per wave.  Without recorded changes its slots
+  // all shift by the same offset change (a coalesced copy); otherwise each
+  // slot's cell is found by binary search over the sub-chunk's old offsets,
+  // held one per lane and read by shuffles (waves take different paths, so
+  // there is no workgroup barrier here)
+  constexpr int kWaves = kThreads / 64;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  for (int k = blockIdx.x * kWaves + wv; k < a.nsub; k += gridDim.x * kWaves) {
+    const int c_lo = k << kSubShift, ncl = min(64, nbk - c_lo);
+    const int s_lo = co[c_lo], s_hi = co[c_lo + ncl];
+    if (!a.subT[k]) {
+      const int dlt = cn[c_lo] - s_lo;
+      for (int s = s_lo + lane; s < s_hi; s += 64) pn[s + dlt] = po[s];
+      continue;
     }
+    const int cl = c_lo + min(lane, ncl - 1);
+    const int my_co = co[cl], my_d = cn[cl] - co[cl], my_t = dL[cl] | dA[cl];
+    for (int sb = s_lo; sb < s_hi; sb += 64) {
+      const int s = sb + lane;
+      int lo = 0, hi = ncl - 1;
 #pragma unroll
-    for (int u = 0; u < kResortBatch; ++u) cv[u] = sb + u * gs < a.n ? a.key[pv[u]] : 0;
-#pragma unroll
-    for (int u = 0; u < kResortBatch; ++u) {
-      const int c = cv[u];
-      tv[u] = dL[c] | dA[c];
-      dv[u] = cn[c] - co[c];
-    }
-#pragma unroll
-    for (int u = 0; u < kResortBatch; ++u) {
-      const int s = sb + u * gs;
-      if (s >= a.n) continue;
-      const int p = pv[u], c = cv[u];
-      int pos = s + dv[u];
-      if (tv[u]) {
+      for (int it = 0; it < kSubShift; ++it) {
+        const int mid = (lo + hi + 1) >> 1;
+        const int v = __shfl(my_co, mid, 64);
+        if (lo < hi) {
+          if (v <= s) lo = mid;
+          else hi = mid - 1;
+        }
+      }
+      const int dlt = __shfl(my_d, lo, 64), tch = __shfl(my_t, lo, 64);
+      if (s >= s_hi) continue;
+      const int p = po[s], c = c_lo + lo;
+      int pos = s + dlt;
+      if (tch) {
         if (a.L.mrec[p]) continue;  // placed by the mover loop below
         // the old segment ascends in particle index
         for (int m = a.L.headL[c]; m >= 0; m = M[kMovRec * m + 3]) pos -= M[kMovRec * m] < p;
@@ -2341,6 +2400,8 @@ int enqueue_resort(dmpm_plan* P, const Mesh2<T>& M, const T* pb, int32_t* eid, h
   a.n = (int)P->n;
   a.ncells = P->ncells;
   a.nchunks = P->nchunks;
+  a.subT = P->subT;
+  a.nsub = P->nsub;
   const int g12 = std::max(P->nchunks, kSortGrid);
   // (positions: rows 0 and 1 of pb, row stride ld)
   k_sort_s1<T><<<g12, kThreads, 0, s>>>(M, pb, ld < 0 ? P->d.ld : ld, a);
