@@ -614,3 +614,78 @@ def test_step_between_blocks_equals_run(scheme, split):
         m = na._count
         assert torch.equal(na._buf[:, :m].view(torch.int32), nb._buf[:, :m].view(torch.int32))
         sin = sout
+
+
+@pytest.mark.parametrize("driver", ["method", "phase_kernels", "phase_launches", "step"])
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+@pytest.mark.parametrize("scheme", ["usf", "usl"])
+def test_nodal_damping_force_per_step_parity(scheme, dtype, driver):
+    """A nonzero nodal f_damp enters the total nodal force (node.py:123-125,
+    element.py:305-329) and is never reset by the step (the reference calls
+    reset_values nowhere), through each driver of the step: the scheme's
+    per-method calls, the per-phase kernels, the per-phase launches of the
+    tile kernels and the one-launch step.  Re-synchronised to the oracle
+    before every step, like test_per_step_parity.  f_damp is set on the
+    nodes the particle block covers for the whole run (a massless node under
+    a force takes nan_to_num(F / 0) = the largest float as its acceleration,
+    in the reference too)."""
+    from diffmpm_amd import _lib as L
+    from diffmpm_amd._plan import plan_for
+    spec = _moving_spec(scheme)
+    spec["nsteps"] = 6
+    npdt = np.float64 if dtype == torch.float64 else np.float32
+    rtol = RTOL_F64 if dtype == torch.float64 else RTOL_F32
+    osim = cases.build_oracle(spec, npdt)
+    gsim = cases.build_gpu(spec, dtype)
+    nx, ny = spec["nelements"]
+    jj, ii = np.meshgrid(np.arange(ny + 1), np.arange(nx + 1), indexing="ij")
+    covered = ((ii >= 6) & (ii <= nx - 10) & (jj >= 5) & (jj <= 31)).reshape(-1)
+    rng = np.random.Generator(np.random.PCG64(5))
+    fd = 0.1 * rng.standard_normal((covered.size, 2)) * covered[:, None]
+    on, gn = osim.mesh.elements.nodes, gsim.mesh.elements.nodes
+    on.f_damp = fd.reshape(on.f_damp.shape).astype(npdt)
+    gn.f_damp = torch.from_numpy(fd.reshape(gn.f_damp.shape))
+    expect_fd = gn.f_damp.clone()
+    g = torch.tensor(spec["gravity"], dtype=torch.float64)
+    plan = plan_for(gsim.mesh.elements, gsim.mesh.particles[0])
+    plan.set_option(L.DMPM_OPT_STEP_KERNEL, 1 if driver == "step" else 0)
+    for i in range(spec["nsteps"]):
+        cases.load_oracle_state_into_gpu(osim, gsim)
+        osim.step(i, np.asarray(spec["gravity"]))
+        if driver == "method":
+            gsim._step(i, g)
+        else:
+            plan.run(scheme, False, g, spec["dt"], i, 1, True,
+                     per_phase=driver == "phase_kernels")
+        _compare(osim, gsim, rtol, f"step {i}")
+        assert torch.equal(gn.f_damp, expect_fd), i
+    # the damping force changed the particles' motion
+    ref = cases.build_oracle(spec, npdt)
+    for i in range(spec["nsteps"]):
+        ref.step(i, np.asarray(spec["gravity"]))
+    assert cases.normwise(osim.mesh.particles[0].velocity,
+                          ref.mesh.particles[0].velocity) > 1e-3
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+def test_apply_force_boundary_constraints(dtype):
+    """element.py:336-340: the nodes of the first constraint get zero f_int,
+    f_ext and f_damp; every other node and field is left as it was."""
+    spec = cases.synthetic_spec(nsteps=1)
+    gsim = cases.build_gpu(spec, dtype)
+    el = gsim.mesh.elements
+    g = torch.tensor(spec["gravity"], dtype=torch.float64)
+    gsim._step(0, g)
+    n = el.nodes
+    rng = np.random.Generator(np.random.PCG64(2))
+    n.f_damp = torch.from_numpy(rng.standard_normal(tuple(n.f_damp.shape)))
+    before = n._buf.clone()
+    el.apply_force_boundary_constraints()
+    from diffmpm_amd import _lib as L
+    ids = torch.as_tensor(el.constraints[0][0], device=before.device).long()
+    expect = before.clone()
+    for r in (L.N_FINT, L.N_FEXT, L.N_FDAMP):
+        for d in range(2):
+            expect[r + d, ids] = 0
+    assert torch.equal(n._buf, expect)
+    assert torch.count_nonzero(before[L.N_FINT:L.N_ROWS][:, ids]) > 0
