@@ -840,7 +840,7 @@ __device__ __forceinline__ int xcd_pos(int b, int ntiles) {
 // Row-major tile of position g of a mesh's strip order: strips of kStrip tile
 // columns, row-major within a strip, so that the ~64 tiles an XCD runs at the
 // same time form a block of kStrip columns whose halos overlap and share its
-// L2 (16 columns measured fastest of 4, 8 and 16).
+// L2 (16 columns measured fastest of 4, 8, 16 and 32).
 constexpr int kStrip = 16;
 __device__ __forceinline__ int strip_tile(int g, int ntiles, int tiles_x) {
   const int tiles_y = ntiles / tiles_x;
