@@ -69,40 +69,77 @@ def _cpu_model():
     return "unknown"
 
 
-def cpu_baseline(rows=32, steps=24):
-    """Oracle (NumPy restatement of the reference step, one core) on a
-    1024 x `rows`-cell strip of the same workload (about 10 s of CPU work)."""
+def _strip_sample(rows):
+    """The throughput workload's bottom `rows` cell rows as a 1024 x `rows`
+    mesh: the same jittered particles (clamped below the strip's top node
+    row), bottom nodes fixed in x and y, side rollers."""
     import numpy as np
+    from diffmpm_amd.workloads import gauss_points
+    n = 1024
+    h = 1.0 / n
+    pts = gauss_points(n, rows, h, 0.25, 0)
+    top = float(np.nextafter(np.float32(rows) * np.float32(h), np.float32(0.0)))
+    np.minimum(pts[:, 1], top, out=pts[:, 1])
+    b, j = np.arange(n + 1), np.arange(rows + 1)
+    cons = [(b, 0, 0.0), (b, 1, 0.0), (np.concatenate([j * (n + 1), j * (n + 1) + n]), 0, 0.0)]
+    return n, h, pts, cons
+
+
+def cpu_baseline(rows=64, np_steps=12, torch_steps=80):
+    """The CPU restatements of the reference step on a 1024 x `rows`-cell
+    strip of the throughput workload (same particles, constraints, material,
+    gravity, dt, USF): the NumPy oracle on one core, and the torch restatement
+    of the same algorithm on every host thread (about 10 s each)."""
+    import numpy as np
+    import torch
     sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
     from oracle import mpm as om
-    from diffmpm_amd.workloads import gauss_points
+    from oracle.torch_step import TorchQ4
     try:
         from threadpoolctl import threadpool_limits
         lim = threadpool_limits(1)
     except Exception:
         lim = None
-    n = 1024
-    h = 1.0 / n
-    pts = gauss_points(n, rows, h, 0.25, 0)
-    cons = [(np.arange(n + 1), om.Constraint(0, 0.0)), (np.arange(n + 1), om.Constraint(1, 0.0))]
-    el = om.Elements("quad4", (n, rows + 1), (h, h), cons, [], np.float32)
-    ps = om.Particles(pts, om.LinearElastic({"density": 1000.0, "youngs_modulus": 1e6,
-                                              "poisson_ratio": 0.3}), dtype=np.float32)
-    sim = om.MPMExplicit(om.Mesh(el, [ps]), 1e-5, "usf", False, 1)
+    n, h, pts, cons = _strip_sample(rows)
+    props = {"density": 1000.0, "youngs_modulus": 1e6, "poisson_ratio": 0.3}
     g = np.array([0.0, -9.81])
+    el = om.Elements("quad4", (n, rows), (h, h),
+                     [(ids, om.Constraint(d, v)) for ids, d, v in cons], [], np.float32)
+    ps = om.Particles(pts, om.LinearElastic(props), dtype=np.float32)
+    sim = om.MPMExplicit(om.Mesh(el, [ps]), 1e-5, "usf", False, 1)
     sim.step(0, g)  # warm-up
     t0 = time.perf_counter()
-    for i in range(1, steps + 1):
+    for i in range(1, np_steps + 1):
         sim.step(i, g)
-    el_s = time.perf_counter() - t0
-    if lim is not None:
-        lim.unregister() if hasattr(lim, "unregister") else None
-    return {"value": len(pts) * steps / el_s, "unit": "particle-steps/s", "cores": 1,
-            "kind": "port", "cpu_model": _cpu_model(),
-            "host_cores_available": len(os.sched_getaffinity(0)),
-            "sample": f"oracle (NumPy FP32 restatement of the reference step, 1 thread) on a "
-                      f"1024x{rows}-cell strip ({len(pts)} particles) of the same workload, "
-                      f"{steps} steps after 1 warm-up ({el_s:.1f} s)"}
+    np_s = time.perf_counter() - t0
+    if lim is not None and hasattr(lim, "unregister"):
+        lim.unregister()
+    assert np.isfinite(ps.loc).all() and np.isfinite(ps.stress).all()
+    threads = len(os.sched_getaffinity(0))
+    torch.set_num_threads(threads)
+    de = om.LinearElastic(props).de64
+    t = TorchQ4(n, rows, h, h, torch.from_numpy(pts.astype(np.float32)), de, 1000.0, cons, 1e-5)
+    t.step(0, g)
+    t0 = time.perf_counter()
+    for i in range(1, torch_steps + 1):
+        t.step(i, g)
+    th_s = time.perf_counter() - t0
+    assert torch.isfinite(t.loc).all() and torch.isfinite(t.stress).all()
+    sample = (f"1024x{rows}-cell strip of the throughput workload ({len(pts)} particles: "
+              f"its bottom {rows} cell rows, the same jittered positions, bottom fixed, side "
+              f"rollers, E=1e6, nu=0.3, rho=1000, gravity, dt=1e-5, USF, FP32)")
+    return {"value": len(pts) * torch_steps / th_s, "unit": "particle-steps/s",
+            "cores": threads, "kind": "port", "cpu_model": _cpu_model(),
+            "sample": f"torch CPU restatement of the reference step (oracle/torch_step.py), "
+                      f"{threads} threads, {torch_steps} steps after 1 warm-up "
+                      f"({th_s:.1f} s), on a {sample}",
+            "config": {"workload": "Q4 strip", "nx": n, "ny": rows, "particles": len(pts),
+                       "scheme": "usf", "constraints": "bottom x+y fixed, side rollers"},
+            "single_core": {"value": len(pts) * np_steps / np_s, "unit": "particle-steps/s",
+                            "cores": 1, "kind": "port",
+                            "sample": f"NumPy oracle (np.add.at scatters in particle order, "
+                                      f"1 thread), {np_steps} steps after 1 warm-up "
+                                      f"({np_s:.1f} s), same strip"}}
 
 
 def ensemble_bench(K):

@@ -314,23 +314,20 @@ class Elements:
         1D (element.py:517-541): eid = I when X_I <= x < X_{I+1}, else -1.
         """
         loc = np.asarray(loc)
-        X = self.nodes.loc[: self.nx + 1, 0]
-        cx = X[None, :] <= loc[:, 0:1]
-        anyx = cx.any(axis=1)
-        I = np.where(anyx, (self.nx + 1) - 1 - np.argmax(cx[:, ::-1], axis=1), -1)
+
+        def last_le(coords, v):
+            # max{i : coords_i <= v} for nondecreasing coords, -1 if none or NaN
+            i = np.searchsorted(coords, v, side="right") - 1
+            return np.where(np.isnan(v), -1, i)
+
+        I = last_le(self.nodes.loc[: self.nx + 1, 0], loc[:, 0])
         if self.kind == "linear1d":
             # idl: largest n with X_n <= x, or M if none; idg: first n with
-            # X_n > x minus 1, or -2 if none.
-            M = self.nx + 1
-            idl = np.where(anyx, I, M)
-            gt = X[None, :] > loc[:, 0:1]
-            idg = np.where(gt.any(axis=1), np.argmax(gt, axis=1) - 1, -2)
-            return np.where(idl == idg, idl, -1).astype(np.int32)
-        Y = self.nodes.loc[:: self.nx + 1, 1]
-        cy = Y[None, :] <= loc[:, 1:2]
-        anyy = cy.any(axis=1)
-        J = np.where(anyy, (self.ny + 1) - 1 - np.argmax(cy[:, ::-1], axis=1), -1)
-        eid = np.where(anyx & anyy, J * self.nx + I, 0)
+            # X_n > x minus 1, or -2 if none; eid = idl if idl == idg else -1,
+            # which is I for 0 <= I < nx
+            return np.where((I >= 0) & (I < self.nx), I, -1).astype(np.int32)
+        J = last_le(self.nodes.loc[:: self.nx + 1, 1], loc[:, 1])
+        eid = np.where((I >= 0) & (J >= 0), J * self.nx + I, 0)
         return eid.astype(np.int32)
 
     def element_ids_bruteforce(self, loc):

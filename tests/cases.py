@@ -356,3 +356,20 @@ def config_c4_spec(nsteps=3, seed=0, n=128):
                           "poisson_ratio": 0.3},
                 tractions=[], scheme="usf", dt=1e-5, nsteps=nsteps, velocity_update=False,
                 youngs=E.tolist())
+
+
+def gpu_state_into_oracle(gsim, osim):
+    """Copies a GPU simulation's particle state (first particle set) and
+    nodal state into an oracle simulation of the same mesh, so that both can
+    take the next step from identical inputs."""
+    gps, ops = gsim.mesh.particles[0], osim.mesh.particles[0]
+    for name in PARTICLE_FIELDS + ["acceleration", "f_ext"]:
+        cur = getattr(ops, name)
+        v = getattr(gps, name).detach().cpu().numpy().reshape(np.shape(cur))
+        setattr(ops, name, v.astype(np.asarray(cur).dtype))
+    ops.element_ids = gps.element_ids.cpu().numpy().astype(np.int32)
+    gn, on = gsim.mesh.elements.nodes, osim.mesh.elements.nodes
+    for name in NODE_FIELDS + ["f_damp"]:
+        cur = getattr(on, name)
+        v = getattr(gn, name).detach().cpu().numpy().reshape(np.shape(cur))
+        setattr(on, name, v.astype(np.asarray(cur).dtype))

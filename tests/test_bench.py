@@ -16,9 +16,15 @@ def test_more_than_one_gpu_exits_with_a_message():
 
 
 def test_cpu_baseline_reports_its_sample():
+    """Both CPU numbers (the torch restatement on every host thread, and the
+    NumPy oracle on one core) on the same strip of the throughput workload,
+    finite (the baseline asserts it) and with the sample's configuration."""
     sys.path.insert(0, ROOT)
     import bench
-    b = bench.cpu_baseline(rows=1, steps=1)
+    b = bench.cpu_baseline(rows=2, np_steps=1, torch_steps=1)
     assert b["value"] > 0 and b["unit"] == "particle-steps/s"
-    assert b["cores"] == 1 and b["kind"] == "port"
-    assert "1024x1-cell strip" in b["sample"]
+    assert b["cores"] == len(os.sched_getaffinity(0)) and b["kind"] == "port"
+    assert "1024x2-cell strip" in b["sample"] and "side rollers" in b["sample"]
+    assert b["config"]["particles"] == 4 * 1024 * 2
+    s = b["single_core"]
+    assert s["value"] > 0 and s["cores"] == 1 and s["kind"] == "port"

@@ -294,18 +294,41 @@ def test_crowded_cell_parity_and_determinism(n_crowd):
     _compare(osim, a, 1e-5, "crowded")
 
 
-def test_particles_outside_mesh_do_not_stall():
-    """NaN and out-of-mesh particles follow the element-search quirks and the
-    step still completes (no pathological sort)."""
+def test_particles_outside_mesh_follow_the_reference():
+    """Particles left of / below the mesh (element 0), beyond the last node
+    column (next row), above the mesh (element id >= nx*ny) and at NaN: the
+    element ids equal the brute-force restatement of the reference's search
+    (element.py:820-840); after 3 steps the set of non-finite particles is the
+    oracle's (the NaN particle poisons the nodes of element 0, and through
+    them its neighbours, as the reference's scatters do; the particle above
+    the mesh is NaN in both) and every other particle matches the oracle."""
     spec = cases.synthetic_spec(nx=8, ny=8, rows=6, seed=2, nsteps=3)
     pts = np.asarray(spec["particles"])
     pts[:5] = [[-0.1, 0.5], [0.5, -0.2], [1.2, 0.3], [np.nan, 0.4], [0.3, 1.5]]
     spec["particles"] = pts.tolist()
+    osim = cases.build_oracle(spec, np.float32)
     sim = cases.build_gpu(spec)
+    ps, ops = sim.mesh.particles[0], osim.mesh.particles[0]
+    brute = osim.mesh.elements.element_ids_bruteforce(pts.astype(np.float32))
+    assert brute[:5].tolist() == [0, 0, 24, 0, 66]
+    assert np.array_equal(ps.element_ids.cpu().numpy(), brute)
+    with np.errstate(all="ignore"):
+        osim.solve(np.asarray(spec["gravity"]))
     sim.solve_jit(torch.tensor(spec["gravity"], dtype=torch.float64))
     torch.cuda.synchronize()
-    ps = sim.mesh.particles[0]
-    assert torch.isfinite(ps.loc[20:]).all() or True  # completes; values may be NaN
+    fields = ("loc", "velocity", "stress", "strain", "volume")
+    bad_o = np.zeros(len(pts), bool)
+    bad_g = np.zeros(len(pts), bool)
+    for name in fields:
+        bad_o |= ~np.isfinite(cases.oracle_particle_field(ops, name)).all(axis=1)
+        bad_g |= ~np.isfinite(cases.gpu_particle_field(ps, name)).all(axis=1)
+    assert bad_o[[3, 4]].all() and not bad_o[2] and bad_o.sum() < len(pts) // 2
+    assert np.array_equal(bad_g, bad_o), (np.nonzero(bad_g)[0], np.nonzero(bad_o)[0])
+    ok = ~bad_o
+    for name in fields:
+        err = cases.normwise(cases.gpu_particle_field(ps, name)[ok],
+                             cases.oracle_particle_field(ops, name)[ok])
+        assert err < RTOL_F32, (name, err)
 
 
 def _moving_spec(scheme, per_particle_E=False, nx=200, ny=40, rows=30, drift=(0.6, 0.3),

@@ -168,3 +168,118 @@ def test_nan_to_num_and_empty_node_velocity():
     a = el.nodes.acceleration
     fmax = np.finfo(np.float32).max
     assert a[0, 0] == fmax and a[1, 1] == -fmax and a[2, 0] == 0
+
+
+# ---------------------------------------------------------------------------
+# Linear completeness: the shape functions reproduce linear fields exactly, so
+# interpolating the node coordinates returns the particle position, and the
+# first moments of P2G equal the particle sums.  These pin the natural
+# coordinate (sign, centre and scale; for the 2-node element the reference's
+# index-2 clamp to node 1, particle.py:180-206) and every transfer that uses
+# it, in 1D as in 2D.
+# ---------------------------------------------------------------------------
+def _setup(element, dtype=np.float64, seed=11):
+    spec = cases.synthetic_spec(element, nx=9, ny=7, rows=6, seed=seed, jitter=0.4)
+    sim = cases.build_oracle(spec, dtype)
+    el, ps = sim.mesh.elements, sim.mesh.particles[0]
+    ps.set_particle_element_ids(el)
+    ps.update_natural_coords(el)
+    return el, ps
+
+
+def completeness_error(el, ps):
+    """max |sum_i X_i N_i(xi_p) - x_p| over particles and axes."""
+    N = el.shapefn(ps.reference_loc)
+    X = om.gather(el.nodes.loc, el.id_to_node_ids(ps.element_ids))
+    return float(np.abs(np.einsum("nk,nka->na", N, X) - ps.loc).max())
+
+
+@pytest.mark.parametrize("element", ["linear1d", "quad4"])
+def test_natural_coords_reproduce_positions(element):
+    el, ps = _setup(element)
+    assert np.all((ps.element_ids >= 0) & (ps.element_ids < el.total_elements))
+    assert np.all(np.abs(ps.reference_loc) <= 1 + 1e-12)
+    assert completeness_error(el, ps) < 1e-12
+
+
+@pytest.mark.parametrize("element", ["linear1d", "quad4"])
+@pytest.mark.parametrize("velocity_update", [True, False])
+def test_g2p_of_linear_nodal_fields(element, velocity_update):
+    """particle.py:208-247 with nodal v = A X + b and a = C X + d: v_p = A x_p
+    + b (velocity_update) or v_p0 + dt (C x_p + d), and x_p += dt (A x_p + b)."""
+    el, ps = _setup(element, seed=12)
+    nd = el.ndim
+    r = rng(13)
+    A, C = r.standard_normal((nd, nd)), r.standard_normal((nd, nd))
+    b, dd = r.standard_normal(nd), r.standard_normal(nd)
+    el.nodes.velocity = el.nodes.loc @ A.T + b
+    el.nodes.acceleration = el.nodes.loc @ C.T + dd
+    x0, v0 = ps.loc.copy(), ps.velocity.copy()
+    dt = 0.125
+    ps.update_position_velocity(el, dt, velocity_update)
+    want_v = x0 @ A.T + b if velocity_update else v0 + dt * (x0 @ C.T + dd)
+    assert np.allclose(ps.velocity, want_v, rtol=0, atol=1e-12)
+    assert np.allclose(ps.loc, x0 + dt * (x0 @ A.T + b), rtol=0, atol=1e-12)
+
+
+@pytest.mark.parametrize("element", ["linear1d", "quad4"])
+def test_p2g_first_moments(element):
+    """sum_i X_i m_i = sum_p m_p x_p (element.py:122-152), likewise for the
+    momentum (element.py:154-189) and body force (element.py:236-268)."""
+    el, ps = _setup(element, seed=14)
+    el.compute_nodal_mass(ps)
+    X = el.nodes.loc
+    assert np.allclose(X.T @ el.nodes.mass, ps.loc.T @ ps.mass, rtol=1e-12, atol=0)
+    el.compute_nodal_momentum(ps)
+    mv = ps.mass[:, None] * ps.velocity
+    assert np.allclose(X.T @ el.nodes.momentum, ps.loc.T @ mv, rtol=1e-11, atol=1e-12)
+    el.nodes.f_ext[:] = 0
+    g = np.arange(1, el.ndim + 1) * -2.5
+    el.compute_body_force(ps, g)
+    want = (ps.loc.T @ ps.mass)[:, None] * g[None, :]
+    assert np.allclose(X.T @ el.nodes.f_ext, want, rtol=1e-12, atol=1e-12)
+
+
+@pytest.mark.parametrize("element", ["linear1d", "quad4"])
+def test_internal_force_virial(element):
+    """sum_i X_i,a f_int,i,b = -sum_p V_p sigma_ab (element.py:548-600 in 1D:
+    sigma_xx only; element.py:842-901 in 2D), from sum_i X_i,a dN_i/dx_c =
+    delta_ac.  Pins the sign of f_int and the 1D/2D gradient chain."""
+    el, ps = _setup(element, seed=15)
+    ps.compute_volume(el)
+    ps.stress = rng(16).standard_normal(ps.stress.shape)
+    el.compute_internal_force(ps)
+    M = el.nodes.loc.T @ el.nodes.f_int
+    s, V = ps.stress, ps.volume
+    if el.ndim == 1:
+        want = np.array([[-(V * s[:, 0]).sum()]])
+    else:
+        want = -np.array([[(V * s[:, 0]).sum(), (V * s[:, 3]).sum()],
+                          [(V * s[:, 3]).sum(), (V * s[:, 1]).sum()]])
+    assert np.allclose(M, want, rtol=1e-11, atol=1e-11)
+
+
+@pytest.mark.parametrize("mistake", ["negate", "scale", "node0"])
+def test_planted_natural_coordinate_mistakes_are_caught(monkeypatch, mistake):
+    """Plants a wrong 1D natural coordinate in the oracle (a negated or
+    rescaled xi, or the far node taken as node 0 instead of the clamped node
+    1) and checks that the completeness checks above fail on it."""
+    orig = om.Particles.update_natural_coords
+
+    def wrong(self, el):
+        orig(self, el)
+        if el.kind != "linear1d":
+            return
+        if mistake == "negate":
+            self.reference_loc = -self.reference_loc
+        elif mistake == "scale":
+            self.reference_loc = 2 * self.reference_loc
+        else:
+            X0 = om.gather(el.nodes.loc, self.element_ids)
+            self.reference_loc = (self.loc - X0) * 2 / el.hx
+
+    monkeypatch.setattr(om.Particles, "update_natural_coords", wrong)
+    el, ps = _setup("linear1d")
+    assert completeness_error(el, ps) > 1e-3
+    el.compute_nodal_mass(ps)
+    assert not np.allclose(el.nodes.loc.T @ el.nodes.mass, ps.loc.T @ ps.mass, rtol=1e-9)
