@@ -104,6 +104,7 @@ SIGNATURES = {
                                 c_double, _P],
     "dmpm_step_adjoint_saved_state": [_P, _P, _P, _P, c_int64, _P, _P, _P, _P, _P, _P, c_int32,
                                       c_int32, _DP, c_double, _P],
+    "dmpm_snapshot_rows": [_P, c_int64, c_int32, c_int64, _P, _P],
 }
 
 _lib = None

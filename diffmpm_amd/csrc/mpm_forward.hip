@@ -68,7 +68,6 @@ __global__ void k_sort_zero(int* __restrict__ counts, int* __restrict__ fill, in
                             int* __restrict__ flags) {
   if (flags[0] == 0) return;
   int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i == 0) flags[2] = 0;
   if (i < nb) {
     counts[i] = 0;
     fill[i] = 0;
@@ -190,27 +189,29 @@ __global__ void k_sort_fix(const int* __restrict__ cs, int ncells, int* __restri
 }
 
 // One workgroup per queued cell: each element's slot is its rank among the
-// cell's (distinct) particle indices, O(k^2 / 256) per cell.  Cells beyond
-// kBigCellMax particles keep the atomic arrival order (sums stay correct but
-// their order is no longer reproducible); flags[2] records that this happened.
+// cell's (distinct) particle indices, O(k^2 / 256) per cell, so the cell
+// ends in ascending particle index whatever the scatter's arrival order.
+// Cells of up to kBigCellMax particles rank from a copy in LDS; bigger ones
+// from a copy in `scratch` (the second permutation buffer, which holds no
+// live order during this sort: the sort ends by selecting buffer 0 and
+// invalidating the fused path's incremental state).
 __global__ void k_sort_fix_big(const int* __restrict__ cs, int* __restrict__ perm,
-                               int* __restrict__ flags, const int* __restrict__ big) {
+                               int* __restrict__ flags, const int* __restrict__ big,
+                               int* __restrict__ scratch) {
   if (flags[0] == 0) return;
   __shared__ int sh[kBigCellMax];
   const int nbig = flags[3];
   for (int q = blockIdx.x; q < nbig; q += gridDim.x) {
     const int c = big[q];
     const int b = cs[c], n = cs[c + 1] - b;
-    if (n > kBigCellMax) {
-      if (threadIdx.x == 0) flags[2] = 1;
-      continue;
-    }
-    for (int i = threadIdx.x; i < n; i += blockDim.x) sh[i] = perm[b + i];
+    int* src = n > kBigCellMax ? scratch + b : sh;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) src[i] = perm[b + i];
+    __threadfence_block();
     __syncthreads();
     for (int i = threadIdx.x; i < n; i += blockDim.x) {
-      const int v = sh[i];
+      const int v = src[i];
       int r = 0;
-      for (int j = 0; j < n; ++j) r += sh[j] < v;
+      for (int j = 0; j < n; ++j) r += src[j] < v;
       perm[b + r] = v;
     }
     __syncthreads();
@@ -244,7 +245,7 @@ int enqueue_sort(dmpm_plan* P, hipStream_t s) {
     k_sort_scatter<<<nblocks(n), kBlock, 0, s>>>(P->key, n, P->cs, P->fill, P->perm, P->flags);
     k_sort_fix<<<nblocks(P->ncells + 1), kBlock, 0, s>>>(P->cs, P->ncells, P->perm, P->flags,
                                                      P->big, P->counts, P->fill);
-    k_sort_fix_big<<<256, kBlock, 0, s>>>(P->cs, P->perm, P->flags, P->big);
+    k_sort_fix_big<<<256, kBlock, 0, s>>>(P->cs, P->perm, P->flags, P->big, P->perm2);
   }
   k_sort_done<<<1, 64, 0, s>>>(P->flags);
   return launch_check("sort");
@@ -889,7 +890,26 @@ extern "C" {
 
 const char* dmpm_last_error(void) { return g_last_error.c_str(); }
 
-int dmpm_abi_version(void) { return 100; }
+int dmpm_abi_version(void) { return 101; }
+
+int dmpm_snapshot_rows(const void* pbuf, int64_t ld, int32_t dtype, int64_t n, void* host_dst,
+                       void* stream) {
+  if (!pbuf || !host_dst) return fail(DMPM_EINVAL, "null buffer");
+  if (dtype != DMPM_F32 && dtype != DMPM_F64) return fail(DMPM_EINVAL, "dtype");
+  if (n < 0 || ld < n) return fail(DMPM_EINVAL, "n must lie in [0, ld]");
+  if (n == 0) return DMPM_OK;
+  const size_t esz = dtype == DMPM_F32 ? 4 : 8;
+  const size_t row = (size_t)n * esz, pitch = (size_t)ld * esz;
+  const char* src = (const char*)pbuf;
+  char* dst = (char*)host_dst;
+  hipStream_t s = (hipStream_t)stream;
+  // rows 0..3 (loc, velocity), then rows 13..24 (strain, stress)
+  DMPM_TRY(hip_check(hipMemcpy2DAsync(dst, row, src, pitch, row, 4, hipMemcpyDeviceToHost, s),
+                     "hipMemcpy2DAsync"));
+  return hip_check(hipMemcpy2DAsync(dst + 4 * row, row, src + DMPM_P_STRAIN * pitch, pitch, row,
+                                    12, hipMemcpyDeviceToHost, s),
+                   "hipMemcpy2DAsync");
+}
 
 // Allocates and initialises the sort state and the simulation table of a
 // plan whose sizes are set; frees it on failure.

@@ -42,13 +42,12 @@ struct dmpm_plan {
   int* fill = nullptr;    // [ncells+1]
   int* cs = nullptr;      // [ncells+2] first slot of each cell (+ total)
   int* bsum = nullptr;    // scan block totals
-  // flags: [0] full re-sort needed, [1] perm is the identity, [2] a cell too big
-  // for the deterministic fix-up was seen, [3] big-cell count, [4] current
+  // flags: [0] full re-sort needed, [1] perm is the identity, [2] and [7]
+  // zero (a set [2] forces a full re-sort), [3] big-cell count, [4] current
   // buffer of perm/cs (0 or 1), [5] cell changes recorded since the last sort,
   // [6] a recorded change overflowed its list bound (full re-sort needed),
-  // [7] big cell left unsorted by the running full sort, [8] finished
-  // workgroups of the last full-sort stage, [10] the per-phase sort rebuilt
-  // buffer 0 since the last fused-path sort
+  // [8] finished workgroups of the last full-sort stage, [10] the per-phase
+  // sort rebuilt buffer 0 since the last fused-path sort
   int* flags = nullptr;
   int* zflags = nullptr;  // [kFlagWords] zeros (flags of a caller-held sort order)
   int* big = nullptr;     // [ncells] cells queued for the big-cell sort

@@ -2300,8 +2300,8 @@ __global__ __launch_bounds__(kThreads) void k_sort_s3(SortArgs a) {
 // count (any other workgroup then reads NONE, and INC leaves it no work).
 // FULL: ascending particle index within each cell — cells of up to
 // kSmallCell particles by one thread (insertion sort), bigger ones by one
-// workgroup (rank sort in LDS up to kBigCellMax; bigger cells keep the
-// scatter order and mark the layout unsorted); restores the zero counters;
+// workgroup (rank sort of a copy of the cell, in LDS up to kBigCellMax
+// particles, beyond that in buffer 1); restores the zero counters;
 // the last workgroup publishes the new state.
 __global__ __launch_bounds__(kThreads) void k_sort_s4(SortArgs a) {
   // the overflow counters of the step that follows (read by no stage)
@@ -2324,17 +2324,16 @@ __global__ __launch_bounds__(kThreads) void k_sort_s4(SortArgs a) {
   for (int q = blockIdx.x; q < nbig; q += gridDim.x) {
     const int c = a.big[q];
     const int s0 = cs[c], m = cs[c + 1] - s0;
-    if (m > kBigCellMax) {
-      if (threadIdx.x == 0) a.L.flags[7] = 1;
-      if (threadIdx.x == 0) a.L.flags[1] = 0;
-      continue;
-    }
-    for (int i = threadIdx.x; i < m; i += kThreads) sh[i] = perm[s0 + i];
+    // the rank sort reads a copy of the cell: in LDS, or for cells beyond
+    // kBigCellMax in buffer 1 (no live order during a full sort)
+    int* src = m > kBigCellMax ? a.perm1 + s0 : sh;
+    for (int i = threadIdx.x; i < m; i += kThreads) src[i] = perm[s0 + i];
+    __threadfence_block();
     __syncthreads();
     for (int i = threadIdx.x; i < m; i += kThreads) {
-      const int v = sh[i];
+      const int v = src[i];
       int rk = 0;
-      for (int j = 0; j < m; ++j) rk += sh[j] < v;
+      for (int j = 0; j < m; ++j) rk += src[j] < v;
       perm[s0 + rk] = v;
       if (v != s0 + rk) a.L.flags[1] = 0;
     }
@@ -2363,7 +2362,7 @@ __global__ __launch_bounds__(kThreads) void k_sort_s4(SortArgs a) {
     for (int k = threadIdx.x; k < a.nchunks; k += kThreads) a.chunkC[k] = 0;
     if (threadIdx.x == 0) {
       int* f = a.L.flags;
-      f[2] = f[7];
+      f[2] = 0;
       f[7] = 0;
       f[3] = 0;
       f[4] = 0;

@@ -26,13 +26,26 @@ from ._lib import call, ptr
 class StepGraph:
     """``nsteps`` steps of ``target`` (``MPMExplicit`` or ``Ensemble``)
     captured into one HIP graph; the first replay runs steps start_step ..
-    start_step + nsteps - 1, the next one the following nsteps, and so on."""
+    start_step + nsteps - 1, the next one the following nsteps, and so on.
 
-    def __init__(self, target, gravity, nsteps, start_step=0, keep_temporaries=None):
+    With ``out_steps`` (an ``MPMExplicit`` with a ``writer_func``) the graph
+    also takes the reference's output snapshots (after every step i with
+    i % out_steps == 0, solver.py:200-210): the captured run stops after each
+    such step and copies the output rows into pinned host memory
+    (``dmpm_snapshot_rows``), and after each replay a writer thread hands the
+    snapshots to ``writer_func`` in step order.  nsteps must then be a
+    multiple of out_steps (every replay has the same layout); a replay first
+    waits until the previous replay's snapshots are written, since it
+    overwrites their host buffers."""
+
+    def __init__(self, target, gravity, nsteps, start_step=0, keep_temporaries=None,
+                 out_steps=None):
         from .ensemble import Ensemble
         if nsteps < 1:
             raise ValueError("nsteps must be positive")
         if isinstance(target, Ensemble):
+            if out_steps:
+                raise ValueError("captured output covers single simulations")
             plan = target.plan
             pbuf, eid, nbuf = target.pbuf, target.eid, target.nbuf
             temps = bool(keep_temporaries)
@@ -48,10 +61,33 @@ class StepGraph:
             temps = target.keep_temporaries if keep_temporaries is None else keep_temporaries
             fext = ps._has_fext
         self.target, self.plan, self.nsteps = target, plan, int(nsteps)
+        self._next = int(start_step)  # first step of the next replay (host count)
         self._keep = (pbuf, eid, nbuf)  # the graph holds their addresses
         code = _lib.DMPM_USF if target.scheme == "usf" else _lib.DMPM_USL
         flags = (1 if temps else 0) | (2 if fext else 0) | _lib.DMPM_RUN_CLOCK
         grav = plan._gravity(gravity)
+        # chunks of the captured run: (steps, snapshot after them?)
+        chunks = [(self.nsteps, False)]
+        self._snaps = None
+        if out_steps:
+            out = int(out_steps)
+            if target.writer_func is None:
+                raise ValueError("out_steps needs the simulation's writer_func")
+            if self.nsteps % out:
+                raise ValueError("nsteps must be a multiple of out_steps")
+            first = (-int(start_step)) % out  # offset of the first output step
+            offs = list(range(first, self.nsteps, out))
+            chunks, done = [], 0
+            for o in offs:
+                chunks.append((o + 1 - done, True))
+                done = o + 1
+            if done < self.nsteps:
+                chunks.append((self.nsteps - done, False))
+            self._offsets = offs
+            from .output import CapturedSnapshots
+            self._snaps = CapturedSnapshots(target.mesh.particles, len(offs),
+                                            target.writer_func, target.out_dir,
+                                            target.sim_steps)
         device = pbuf.device
         with torch.cuda.device(device):
             stream = _lib.stream_of(pbuf)
@@ -60,13 +96,29 @@ class StepGraph:
             torch.cuda.synchronize(device)
             self.graph = torch.cuda.CUDAGraph()
             with torch.cuda.graph(self.graph):
-                call("dmpm_run", plan.h, ptr(pbuf), ptr(eid), ptr(nbuf), code,
-                     1 if target.velocity_update else 0, grav, float(target.dt), 0,
-                     self.nsteps, flags, _lib.stream_of(pbuf))
+                k = 0
+                for n, snap in chunks:
+                    if n:
+                        call("dmpm_run", plan.h, ptr(pbuf), ptr(eid), ptr(nbuf), code,
+                             1 if target.velocity_update else 0, grav, float(target.dt), 0,
+                             n, flags, _lib.stream_of(pbuf))
+                    if snap:
+                        self._snaps.enqueue(k)
+                        k += 1
 
     def replay(self):
         """Runs the next ``nsteps`` steps (enqueued on the current stream)."""
+        if self._snaps is not None:
+            self._snaps.wait()
         self.graph.replay()
+        if self._snaps is not None:
+            self._snaps.submit([self._next + o for o in self._offsets])
+        self._next += self.nsteps
+
+    def wait_output(self):
+        """Waits until every replay's snapshots are written."""
+        if self._snaps is not None:
+            self._snaps.wait()
 
     @property
     def step(self):

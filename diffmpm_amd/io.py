@@ -1,18 +1,31 @@
-"""TOML/JSON input front-end (io.py of the reference).
+"""TOML/JSON input front-end: ``Config`` reads a simulation file in the
+reference's format and builds this package's GPU objects from it.
 
-``Config(filepath)`` reads a simulation file in the reference's format
-(diffmpm/io.py:16-170): ``[meta]``, ``[output]``, ``[mesh]`` with
-``[[mesh.constraints]]``, ``[[materials]]``, ``[[particles]]`` (particle
-positions from a JSON file of shape (N, 1, ndim)), ``[external_loading]``
-with ``[[external_loading.concentrated_nodal_forces]]`` and
-``[[external_loading.particle_surface_traction]]``, and
-``[[math_functions]]``; node and particle sets come from the JSON file named
-by ``mesh.entity_sets``.  ``parse()`` builds the mesh of this package's GPU
-classes (the same constructors the reference calls).
+The file format is the reference's interface (diffmpm/io.py:16-170; its
+benchmark input files under benchmarks/2d/* run unmodified):
 
-Relative file names (entity sets, particle files) are opened from the
-current directory as in the reference, or, when absent there, from the
-directory of the TOML file.
+* ``[meta]`` title, type ("MPMExplicit"), dimension, scheme, dt, nsteps,
+  velocity_update; ``[output]`` format, folder, step_frequency;
+* ``[mesh]`` type ("generator"), element (class name), nelements,
+  element_length, entity_sets (a JSON file of ``node_sets`` {id: [nodes]}
+  and ``particle_sets`` [{id: [particles]}, ...]) and ``[[mesh.constraints]]``
+  (nset_ids, dir, velocity);
+* ``[[materials]]`` (type = class name, the other keys are its properties);
+* ``[[particles]]`` (file: JSON positions of shape (N, 1, ndim), material_id,
+  init_velocity);
+* ``[[math_functions]]`` (type "Linear", xvalues, fxvalues), referred to by
+  position;
+* ``[external_loading]`` gravity, ``[[...concentrated_nodal_forces]]``
+  (nset_ids, dir, force, optional math_function_id) and
+  ``[[...particle_surface_traction]]`` (pset, pset_ids, dir, traction,
+  optional math_function_id); without a function id a load is constant
+  (``Unit``).
+
+``parsed_config`` holds what the reference's ``Config`` holds (meta,
+output, materials, particles, math_functions, external_loading,
+particle_surface_traction, elements), so ``Mesh1D``/``Mesh2D`` and ``MPM``
+take it unchanged.  Relative file names are opened from the current
+directory, as the reference does, or else from the TOML file's directory.
 """
 import json
 import os
@@ -36,8 +49,20 @@ from .particle import Particles
 __all__ = ["Config"]
 
 
+def _function(index, rec):
+    kind = rec["type"]
+    if kind != "Linear":
+        raise NotImplementedError(f"math function type {kind!r}: only 'Linear' is implemented")
+    return Linear(index, np.asarray(rec["xvalues"]), np.asarray(rec["fxvalues"]))
+
+
+def _material_from(rec):
+    props = {k: v for k, v in rec.items() if k != "type"}
+    return getattr(_material, rec["type"])(props)
+
+
 class Config:
-    """io.py:16-170."""
+    """A simulation read from a TOML file; ``parse()`` returns its mesh."""
 
     def __init__(self, filepath, *, device=None, dtype=None):
         self._filepath = filepath
@@ -46,134 +71,87 @@ class Config:
         self.parsed_config = {}
         self.parse()
 
-    def _placement(self):
+    # -- files and sets ---------------------------------------------------------
+    def _locate(self, name):
+        if os.path.isabs(name) or os.path.exists(name):
+            return name
+        beside = os.path.join(os.path.dirname(os.path.abspath(self._filepath)), name)
+        return beside if os.path.exists(beside) else name
+
+    def _read_json(self, name):
+        with open(self._locate(name)) as f:
+            return json.load(f)
+
+    def _node_ids(self, set_ids):
+        """The nodes of the named node sets, set after set."""
+        table = self.entity_sets["node_sets"]
+        return torch.tensor([n for s in set_ids for n in table[str(s)]], dtype=torch.int64)
+
+    def _particle_ids(self, psets, set_ids):
+        """The particles of the named sets of each listed particle-set group."""
+        groups = self.entity_sets["particle_sets"]
+        ids = [p for g in psets for s in set_ids for p in groups[g][str(s)]]
+        return torch.tensor(ids, dtype=torch.int64)
+
+    def _on_device(self):
         from ._lib import default_device
-        kw = {"device": self._device if self._device is not None else default_device()}
+        kw = {"device": default_device() if self._device is None else self._device}
         if self._dtype is not None:
             kw["dtype"] = self._dtype
         return kw
 
-    def _path(self, name):
-        if os.path.isabs(name) or os.path.exists(name):
-            return name
-        alt = os.path.join(os.path.dirname(os.path.abspath(self._filepath)), name)
-        return alt if os.path.exists(alt) else name
+    # -- objects ---------------------------------------------------------------
+    def _particle_set(self, rec, materials):
+        loc = np.asarray(self._read_json(rec["file"]), dtype=np.float64)
+        ps = Particles(torch.from_numpy(loc), materials[rec["material_id"]],
+                       torch.zeros(loc.shape[0], dtype=torch.int32), **self._on_device())
+        v0 = torch.as_tensor(rec["init_velocity"], dtype=torch.float64).to(ps.dtype)
+        ps.velocity = v0.to(ps.device).expand(len(ps), 1, loc.shape[-1]).clone()
+        return ps
+
+    def _elements(self, mesh, nodal_forces):
+        if mesh["type"] != "generator":
+            raise NotImplementedError(
+                f"mesh type {mesh['type']!r}: only 'generator' is implemented")
+        cls = getattr(_element, mesh["element"])
+        cons = [(self._node_ids(c["nset_ids"]), Constraint(c["dir"], c["velocity"]))
+                for c in mesh["constraints"]]
+        counts, length = mesh["nelements"], mesh["element_length"]
+        if cls is _element.Linear1D:  # a one-element list names a 1D size
+            counts = int(np.ravel(counts)[0])
+            length = float(np.ravel(length)[0])
+        total = int(np.prod(np.ravel(counts)))
+        return cls(counts, total, length, cons, concentrated_nodal_forces=nodal_forces,
+                   **self._on_device())
 
     def parse(self):
-        """io.py:22-36: reads the file and returns the mesh."""
+        """Reads the file and builds the mesh (the mesh class named by
+        meta.dimension) with its particle sets, loads and constraints."""
         with open(self._filepath, "rb") as f:
-            self._fileconfig = _toml.load(f)
-        with open(self._path(self._fileconfig["mesh"]["entity_sets"])) as f:
-            self.entity_sets = json.load(f)
-        self._parse_meta(self._fileconfig)
-        self._parse_output(self._fileconfig)
-        self._parse_materials(self._fileconfig)
-        self._parse_particles(self._fileconfig)
-        if "math_functions" in self._fileconfig:
-            self._parse_math_functions(self._fileconfig)
-        self._parse_external_loading(self._fileconfig)
-        return self._parse_mesh(self._fileconfig)
+            doc = _toml.load(f)
+        self._fileconfig = doc
+        self.entity_sets = self._read_json(doc["mesh"]["entity_sets"])
+        pc = {"meta": doc["meta"], "output": doc["output"]}
+        pc["materials"] = [_material_from(rec) for rec in doc["materials"]]
+        pc["particles"] = [self._particle_set(rec, pc["materials"]) for rec in doc["particles"]]
+        fns = [_function(k, rec) for k, rec in enumerate(doc.get("math_functions", []))]
+        if "math_functions" in doc:
+            pc["math_functions"] = fns
 
-    def _get_node_set_ids(self, set_ids):
-        """io.py:38-43: concatenated node sets."""
-        ids = []
-        for s in set_ids:
-            ids.extend(self.entity_sets["node_sets"][str(s)])
-        return torch.as_tensor(ids, dtype=torch.int64)
+        def fn_of(rec):
+            return fns[rec["math_function_id"]] if "math_function_id" in rec else Unit(-1)
 
-    def _get_particle_set_ids(self, set_type, set_nos, set_ids):
-        """io.py:45-51: concatenated particle sets of particle-set numbers."""
-        ids = []
-        for n in set_nos:
-            for s in set_ids:
-                ids.extend(self.entity_sets["particle_sets"][n][str(s)])
-        return torch.as_tensor(ids, dtype=torch.int64)
-
-    def _parse_meta(self, config):
-        self.parsed_config["meta"] = config["meta"]
-
-    def _parse_output(self, config):
-        self.parsed_config["output"] = config["output"]
-
-    def _parse_materials(self, config):
-        """io.py:59-67: material classes by name, constructed from the
-        remaining keys."""
-        materials = []
-        for mc in config["materials"]:
-            mc = dict(mc)
-            cls = getattr(_material, mc.pop("type"))
-            materials.append(cls(mc))
-        self.parsed_config["materials"] = materials
-
-    def _parse_particles(self, config):
-        """io.py:69-79: positions from JSON, element ids 0, initial
-        velocity broadcast to every particle."""
-        sets = []
-        for pc in config["particles"]:
-            mat = self.parsed_config["materials"][pc["material_id"]]
-            with open(self._path(pc["file"])) as f:
-                loc = np.asarray(json.load(f), dtype=np.float64)
-            ps = Particles(torch.from_numpy(loc), mat,
-                           torch.zeros(loc.shape[0], dtype=torch.int32), **self._placement())
-            v = torch.as_tensor(pc["init_velocity"], dtype=torch.float64)
-            ps.velocity = torch.broadcast_to(v.to(ps.dtype).to(ps.device),
-                                             (len(ps), 1, loc.shape[-1])).clone()
-            sets.append(ps)
-        self.parsed_config["particles"] = sets
-
-    def _parse_math_functions(self, config):
-        """io.py:81-95: Linear functions by position; other types are not
-        supported (as in the reference)."""
-        fns = []
-        for i, fc in enumerate(config["math_functions"]):
-            if fc["type"] != "Linear":
-                raise NotImplementedError("Function type other than `Linear` not yet supported")
-            fns.append(Linear(i, np.asarray(fc["xvalues"]), np.asarray(fc["fxvalues"])))
-        self.parsed_config["math_functions"] = fns
-
-    def _parse_external_loading(self, config):
-        """io.py:97-145: gravity, concentrated nodal forces on node sets,
-        particle surface tractions on particle sets (Unit function when no
-        math_function_id is given)."""
-        el = config["external_loading"]
-        out = {"gravity": torch.as_tensor(el["gravity"], dtype=torch.float64),
-               "concentrated_nodal_forces": []}
-        for c in el.get("concentrated_nodal_forces", []):
-            fn = (self.parsed_config["math_functions"][c["math_function_id"]]
-                  if "math_function_id" in c else Unit(-1))
-            out["concentrated_nodal_forces"].append(
-                NodalForce(node_ids=self._get_node_set_ids(c["nset_ids"]), function=fn,
-                           dir=c["dir"], force=c["force"]))
-        tractions = []
-        for c in el.get("particle_surface_traction", []):
-            fn = (self.parsed_config["math_functions"][c["math_function_id"]]
-                  if "math_function_id" in c else Unit(-1))
-            tractions.append(
-                ParticleTraction(pset=c["pset"],
-                                 pids=self._get_particle_set_ids("particle", c["pset"],
-                                                                 c["pset_ids"]),
-                                 function=fn, dir=c["dir"], traction=c["traction"]))
-        self.parsed_config["external_loading"] = out
-        self.parsed_config["particle_surface_traction"] = tractions
-
-    def _parse_mesh(self, config):
-        """io.py:147-170: a generated structured mesh of the named element
-        class with the constraints and nodal forces; Mesh1D or Mesh2D by
-        meta.dimension."""
-        mc = config["mesh"]
-        if mc["type"] != "generator":
-            raise NotImplementedError("Mesh type other than `generator` not yet supported.")
-        element_cls = getattr(_element, mc["element"])
-        mesh_cls = getattr(_mesh, f"Mesh{config['meta']['dimension']}D")
-        constraints = [(self._get_node_set_ids(c["nset_ids"]), Constraint(c["dir"], c["velocity"]))
-                       for c in mc["constraints"]]
-        ne = mc["nelements"]
-        el_len = mc["element_length"]
-        if element_cls is _element.Linear1D:  # one-element lists name the 1D sizes too
-            ne = int(np.asarray(ne).reshape(-1)[0])
-            el_len = float(np.asarray(el_len).reshape(-1)[0])
-        elements = element_cls(ne, int(np.prod(np.asarray(ne))), el_len, constraints,
-                               concentrated_nodal_forces=self.parsed_config["external_loading"][
-                                   "concentrated_nodal_forces"], **self._placement())
-        self.parsed_config["elements"] = elements
-        return mesh_cls(self.parsed_config)
+        loads = doc["external_loading"]
+        nodal = [NodalForce(node_ids=self._node_ids(r["nset_ids"]), function=fn_of(r),
+                            dir=r["dir"], force=r["force"])
+                 for r in loads.get("concentrated_nodal_forces", [])]
+        pc["external_loading"] = {"gravity": torch.as_tensor(loads["gravity"],
+                                                             dtype=torch.float64),
+                                  "concentrated_nodal_forces": nodal}
+        pc["particle_surface_traction"] = [
+            ParticleTraction(pset=r["pset"], pids=self._particle_ids(r["pset"], r["pset_ids"]),
+                             function=fn_of(r), dir=r["dir"], traction=r["traction"])
+            for r in loads.get("particle_surface_traction", [])]
+        pc["elements"] = self._elements(doc["mesh"], nodal)
+        self.parsed_config = pc
+        return getattr(_mesh, f"Mesh{doc['meta']['dimension']}D")(pc)

@@ -75,31 +75,39 @@ class MPMExplicit:
             out[name] = torch.stack(vals).squeeze() if not squeeze_each else torch.stack(vals)
         return out
 
-    def _write(self, i):
-        arrays = {k: v.detach().cpu().numpy() for k, v in self._arrays(True).items()}
-        self.writer_func((arrays, i), None, out_dir=self.out_dir, max_steps=self.sim_steps)
-
     def solve_jit(self, gravity, start_step: int = 0) -> dict:
         """solver.py:159-221: runs sim_steps steps; the writer (if any) gets the
-        state after every step i with i % out_steps == 0."""
+        state after every step i with i % out_steps == 0.  Snapshots are
+        copied to pinned host memory asynchronously in stream order
+        (``output.SnapshotQueue``) and written by a writer thread; the call
+        waits for the writes only before it returns."""
+        from .output import SnapshotQueue
         single = len(self.mesh.particles) == 1
         i, end = start_step, start_step + self.sim_steps
         out = max(int(self.out_steps), 1)
-        while i < end:
-            if self.writer_func is None:
-                nxt = end
-            else:  # run through the next step index divisible by out_steps
-                k = i if i % out == 0 else (i // out + 1) * out
-                nxt = min(end, k + 1)
-            if single:
-                from ._plan import plan_for
-                plan_for(self.mesh.elements, self.mesh.particles[0]).run(
-                    self.scheme, self.velocity_update, gravity, self.dt, i, nxt - i,
-                    self.keep_temporaries)
-            else:
-                for k in range(i, nxt):
-                    self._step(k, gravity)
-            if self.writer_func is not None and (nxt - 1) % out == 0:
-                self._write(nxt - 1)
-            i = nxt
+        snaps = None
+        if self.writer_func is not None:
+            snaps = SnapshotQueue(self.mesh.particles, self.writer_func, self.out_dir,
+                                  self.sim_steps)
+        try:
+            while i < end:
+                if snaps is None:
+                    nxt = end
+                else:  # run through the next step index divisible by out_steps
+                    k = i if i % out == 0 else (i // out + 1) * out
+                    nxt = min(end, k + 1)
+                if single:
+                    from ._plan import plan_for
+                    plan_for(self.mesh.elements, self.mesh.particles[0]).run(
+                        self.scheme, self.velocity_update, gravity, self.dt, i, nxt - i,
+                        self.keep_temporaries)
+                else:
+                    for k in range(i, nxt):
+                        self._step(k, gravity)
+                if snaps is not None and (nxt - 1) % out == 0:
+                    snaps.snapshot(nxt - 1)
+                i = nxt
+        finally:
+            if snaps is not None:
+                snaps.close()
         return self._arrays(False)

@@ -94,4 +94,4 @@ def block_2d(n=1024, rows=None, jitter=0.25, seed=0, E=1e6, nu=0.3, density=1000
                      dtype=dtype)
     mesh = d.Mesh2D({"particles": [ps], "elements": el, "particle_surface_traction": []})
     sim = d.MPMExplicit(mesh, dt, scheme, velocity_update, 1, keep_temporaries=keep_temporaries)
-    return sim, torch.tensor([0.0, -9.81])
+    return sim, torch.tensor([0.0, -9.81], dtype=torch.float64)

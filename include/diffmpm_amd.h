@@ -26,9 +26,17 @@
  * of the last failure on the calling host thread.
  *
  * Threading: a plan is not re-entrant; different plans are independent.
- * Entry points only enqueue work on the given stream (no host synchronisation
- * and no allocation) except dmpm_plan_create / dmpm_plan_set_* /
- * dmpm_plan_destroy.
+ *
+ * Allocation and host synchronisation: dmpm_plan_create, dmpm_plan_set_* and
+ * dmpm_plan_destroy allocate, upload and free (and wait for the device).
+ * The step entry points (dmpm_run, dmpm_step_between, the phase and
+ * per-method entry points, the adjoint entry points) enqueue work on the
+ * given stream without waiting for it; each workspace they use is allocated
+ * (hipMalloc) by the first call that needs it, or reallocated when a call
+ * needs it larger (hipFree of the smaller one, which waits for the device),
+ * and kept until dmpm_plan_destroy.  dmpm_prepare_run allocates the
+ * workspaces of dmpm_run beforehand, so that later dmpm_run calls allocate
+ * nothing and can be captured into a HIP graph.
  */
 #ifndef DIFFMPM_AMD_H
 #define DIFFMPM_AMD_H
@@ -342,6 +350,17 @@ int dmpm_step_adjoint_saved_state(dmpm_plan* plan, const void* pconst, const voi
                                   const int32_t* cs_saved, void* cot, void* lebar,
                                   int32_t scheme, int32_t velocity_update,
                                   const double* gravity, double dt, void* stream);
+
+/* ---- output (replaces the host callback of solver.py:113-114, 187-210) ------
+ * Copies the output rows of the first n particles of a particle block
+ * (row stride ld, element type dtype) into host_dst as 16 rows of n
+ * elements: block rows 0..3 (loc, velocity; 1D uses rows 0 and 2) then rows
+ * 13..24 (strain, stress).  Two hipMemcpy2DAsync on stream: no host
+ * synchronisation and no allocation, so the copy can be captured into a HIP
+ * graph.  host_dst should be pinned host memory; it is valid once the
+ * stream has passed the copy. */
+int dmpm_snapshot_rows(const void* pbuf, int64_t ld, int32_t dtype, int64_t n, void* host_dst,
+                       void* stream);
 
 /* ---- measurement (no reference counterpart) ------------------------------- */
 /* Kernel classes for dmpm_timing_enable. */

@@ -84,6 +84,6 @@ def test_element_volume_matches_oracle(h, dtype):
 
 def test_abi_version_and_errors():
     import diffmpm_amd._lib as L
-    assert L.lib().dmpm_abi_version() == 100
+    assert L.lib().dmpm_abi_version() == 101
     with pytest.raises(ValueError, match="element"):
         L.element_volume(7, 0, 1.0, 1.0)

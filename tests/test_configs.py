@@ -37,3 +37,15 @@ def test_c4_youngs_field():
     E = np.asarray(spec["youngs"])
     assert len(E) == 128 * 64 * 4
     assert np.allclose(E, 1e6 * np.exp(0.3 * cases.bilinear(f, h, np.asarray(spec["particles"]))))
+
+
+def test_npz_file_names_follow_the_reference():
+    """writers.npz_name: particles_<step>.npz with the step zero-padded to the
+    number of digits of max_steps (diffmpm/writers.py:45-52)."""
+    import math
+    from diffmpm_amd.writers import npz_name
+    for max_steps in (1, 9, 10, 99, 100, 999, 1000, 12345):
+        digits = int(math.log10(max_steps)) + 1
+        for step in sorted({0, 1, 7, max_steps // 2, max_steps - 1}):
+            want = "particles_" + str(step).rjust(digits, "0") + ".npz"
+            assert npz_name(step, max_steps) == want, (step, max_steps)

@@ -1,12 +1,14 @@
 """Adjoint of the fused 2D step (dmpm_step_adjoint via diffmpm_amd.autodiff).
 
 The reference has no gradient test (it relies on jax.grad), so the adjoint is
-pinned by central finite differences of the FP64 forward rollout: per-particle
-log E on 16 seeded particles (relative step 1e-3, as in the inverse-problem
-configuration), and directional derivatives along seeded random directions of
-the whole initial state and of all log E.  A rollout is short and slow enough
+pinned by central finite differences of the FP64 oracle's forward rollout
+(the reference step restated in NumPy, from the same initial state and
+per-particle E): per-particle log E on 16 seeded particles (relative step
+1e-3, as in the inverse-problem configuration), directional derivatives along
+seeded random directions of the whole initial state and of all log E, and
+every state component of 4 particles.  A rollout is short and slow enough
 that no particle changes cell under the perturbations (checked), so the loss
-is smooth there.
+is smooth there.  The GPU forward itself matches the oracle's within 1e-12.
 """
 import math
 
@@ -42,6 +44,7 @@ def _setup(scheme, vu, dtype=torch.float64):
     from diffmpm_amd.autodiff import Differentiable
     sim, spec = _sim(scheme, vu, dtype)
     d = Differentiable(sim)
+    d.spec = spec  # the oracle's copy of the case
     g = torch.tensor(spec["gravity"], dtype=torch.float64)
     state0 = d.get_state()
     # give the stress and volume rows non-trivial values so that every term
@@ -54,11 +57,40 @@ def _setup(scheme, vu, dtype=torch.float64):
     return d, g, state0, log_e
 
 
+def _oracle_rollout(spec, state0, log_e):
+    """The final (8, N) state of NSTEPS FP64 oracle steps from state0 (rows
+    x, y, vx, vy, sigma_xx, sigma_yy, sigma_xy, V) with E_p = exp(log_e)."""
+    osim = cases.build_oracle(spec, np.float64)
+    ps = osim.mesh.particles[0]
+    s = state0.detach().double().cpu().numpy()
+    ps.loc = np.stack([s[0], s[1]], axis=1)
+    ps.velocity = np.stack([s[2], s[3]], axis=1)
+    ps.stress[:, 0], ps.stress[:, 1], ps.stress[:, 3] = s[4], s[5], s[6]
+    ps.volume = s[7].copy()
+    E = np.exp(log_e.detach().double().cpu().numpy())
+    ps.youngs_modulus = np.broadcast_to(E, (len(ps),)).copy()
+    g = np.asarray(spec["gravity"])
+    for i in range(NSTEPS):
+        osim.step(i, g)
+    return np.stack([ps.loc[:, 0], ps.loc[:, 1], ps.velocity[:, 0], ps.velocity[:, 1],
+                     ps.stress[:, 0], ps.stress[:, 1], ps.stress[:, 3], ps.volume])
+
+
 def _loss(d, g, state0, log_e, w):
-    from diffmpm_amd.autodiff import rollout
-    with torch.no_grad():
-        out = rollout(d, NSTEPS, g, state=state0, log_e=log_e)
-    return float((w * out).sum())
+    """The loss of the FP64 oracle's rollout (the finite differences' forward)."""
+    out = _oracle_rollout(d.spec, state0, log_e)
+    return float((w.double().cpu().numpy() * out).sum())
+
+
+def test_gpu_rollout_matches_the_oracle_rollout():
+    for scheme, vu in (("usf", False), ("usl", True)):
+        d, g, state0, log_e = _setup(scheme, vu)
+        from diffmpm_amd.autodiff import rollout
+        with torch.no_grad():
+            out = rollout(d, NSTEPS, g, state=state0, log_e=log_e).double().cpu().numpy()
+        want = _oracle_rollout(d.spec, state0, log_e)
+        for r in range(8):
+            assert cases.normwise(out[r], want[r]) < 1e-12, (scheme, r)
 
 
 def _cells(d, state):

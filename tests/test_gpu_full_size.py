@@ -258,12 +258,14 @@ def test_full_size_step_against_the_oracle():
     (element.py:820-909, particle.py:180-327, scheme.py:78-104).
 
     One exception is allowed for, and checked against the FP64 oracle from
-    the same input: the reference forms inv(J) from a 4-term FP32 dot product
-    of O(1) node coordinates (element.py:791-818), whose cancellation near
-    x = 1 on this 1024-cell mesh leaves about 2e-5 relative noise in the FP32
-    strain rate (a per-step temporary).  The HIP step uses the exact
-    diag(2/h) inverse, so for such a field its error against the FP64 oracle
-    must be within 1e-5 and below the FP32 oracle's own."""
+    the same input: the FP32 strain rate (a per-step temporary, the gradient
+    of the nodal velocities over one cell of width 2^-10) carries about 1e-5
+    relative rounding noise in any FP32 evaluation, and the reference's inv(J)
+    from a 4-term FP32 dot product of O(1) node coordinates
+    (element.py:791-818) adds more: the FP32 oracle is 1.6e-5 from the FP64
+    oracle here.  For such a field the HIP step's error against the FP64
+    oracle must be no larger than the FP32 oracle's own
+    (cases.fp32_within_reference_precision)."""
     import time
     import cases
     from oracle import mpm as om
@@ -302,10 +304,14 @@ def test_full_size_step_against_the_oracle():
                for k in cases.NODE_FIELDS]
     bad = {}
     for name, got, (w32, w64) in fields:
-        e32, e64, eo = (cases.normwise(got, w32), cases.normwise(got, w64),
-                        cases.normwise(w32, w64))
+        ok, e32, e64, eo = cases.fp32_within_reference_precision(got, w32, w64)
         print(f"{name}: vs FP32 oracle {e32:.1e}, vs FP64 oracle {e64:.1e} "
               f"(FP32 oracle vs FP64 {eo:.1e})")
-        if not (e32 <= 1e-5 or (e64 <= 1e-5 and e64 <= eo)):
+        if not ok:
             bad[name] = (e32, e64, eo)
     assert not bad, bad
+    # every persistent particle field and every nodal field within 1e-5 of
+    # the FP32 oracle outright
+    for name, got, (w32, _) in fields:
+        if name not in ("p.strain_rate", "p.dstrain", "p.dvolumetric_strain"):
+            assert cases.normwise(got, w32) <= 1e-5, name

@@ -280,18 +280,41 @@ def _crowded_spec(n_crowd, seed=11):
                 nsteps=3, velocity_update=False)
 
 
-@pytest.mark.parametrize("n_crowd", [100, 3000])
-def test_crowded_cell_parity_and_determinism(n_crowd):
+@pytest.mark.parametrize("driver", ["run", "methods"])
+@pytest.mark.parametrize("n_crowd", [100, 3000, 9000])
+def test_crowded_cell_parity_and_determinism(n_crowd, driver):
+    """A cell with n_crowd particles (9000: beyond the 8192 that the rank sort
+    orders in LDS) through the fused run and the per-method path: two runs
+    agree bit for bit (the sort puts every cell in ascending particle index,
+    so the summation order is fixed) and both match the oracle."""
     spec = _crowded_spec(n_crowd)
     osim = cases.build_oracle(spec, np.float32)
+    o64 = cases.build_oracle(spec, np.float64)
     a, b = cases.build_gpu(spec), cases.build_gpu(spec)
     g = torch.tensor(spec["gravity"], dtype=torch.float64)
     osim.solve(np.asarray(spec["gravity"]))
-    a.solve_jit(g)
-    b.solve_jit(g)
+    o64.solve(np.asarray(spec["gravity"]))
+    for sim in (a, b):
+        if driver == "run":
+            sim.solve_jit(g)
+        else:
+            for i in range(spec["nsteps"]):
+                sim._step(i, g)
     for x, y in zip(_state(a), _state(b)):
         assert torch.equal(x, y)
-    _compare(osim, a, 1e-5, "crowded")
+    if n_crowd <= 3000:
+        _compare(osim, a, 1e-5, "crowded")
+        return
+    # 9000 particles in one cell: its nodal sums have 9000 FP32 terms, and
+    # the centroid strain rate (a difference of those nodal velocities)
+    # carries their rounding; within the reference's own FP32 precision
+    ops, o6, gps = osim.mesh.particles[0], o64.mesh.particles[0], a.mesh.particles[0]
+    assert np.array_equal(ops.element_ids, gps.element_ids.cpu().numpy())
+    for name in cases.PARTICLE_FIELDS:
+        ok, *errs = cases.fp32_within_reference_precision(
+            cases.gpu_particle_field(gps, name), cases.oracle_particle_field(ops, name),
+            cases.oracle_particle_field(o6, name))
+        assert ok, (name, errs)
 
 
 def test_particles_outside_mesh_follow_the_reference():
