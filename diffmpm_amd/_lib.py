@@ -105,6 +105,8 @@ SIGNATURES = {
     "dmpm_step_adjoint_saved_state": [_P, _P, _P, _P, c_int64, _P, _P, _P, _P, _P, _P, c_int32,
                                       c_int32, _DP, c_double, _P],
     "dmpm_snapshot_rows": [_P, c_int64, c_int32, c_int64, _P, _P],
+    "dmpm_step_between_noalloc": [_P, _P, _P, _P, _P, c_int32, c_int32, _DP, c_double, c_int64,
+                                  c_int32, _P, _P, c_int64, _P],
 }
 
 _lib = None

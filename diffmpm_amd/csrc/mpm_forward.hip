@@ -918,6 +918,12 @@ static int plan_init(dmpm_plan* P, dmpm_plan** out) {
     plan_free(P);
     return fail(DMPM_EHIP, "hipGetDevice failed");
   }
+  // the device address of dmpm_step_between_noalloc's scratch array on this
+  // device (taken once per device, with it current)
+  if (noalloc_scratch_setup(P->device) != DMPM_OK) {
+    plan_free(P);
+    return DMPM_EHIP;
+  }
   int64_t n = std::max<int64_t>(P->n, 1);
   int nb = P->ncells + 1;
   int nbs = (nb + kScanChunk - 1) / kScanChunk;
@@ -1771,6 +1777,30 @@ extern "C" int dmpm_step_between(dmpm_plan* P, const void* pbuf_in, void* pbuf_o
   return tile_between(P, pbuf_in, pbuf_out, eid, nbuf, (int)scheme, (int)velocity_update,
                       gravity, dt, step, (int)(flags & 1), (int)((flags >> 1) & 1), state_in,
                       state_out, state_ld, ST(stream));
+}
+
+extern "C" int dmpm_step_between_noalloc(dmpm_plan* P, const void* pbuf_in, void* pbuf_out,
+                                         int32_t* eid, void* nbuf, int32_t scheme,
+                                         int32_t velocity_update, const double* gravity,
+                                         double dt, int64_t step, int32_t flags,
+                                         const void* state_in, void* state_out,
+                                         int64_t state_ld, void* stream) {
+  PCHK(P);
+  if (!pbuf_in || !pbuf_out || !eid || !nbuf) return fail(DMPM_EINVAL, "null buffer");
+  if (state_in && !state_out) return fail(DMPM_EINVAL, "state_in needs state_out");
+  if (state_in && state_in == state_out) return fail(DMPM_EINVAL, "state_in and state_out must differ");
+  if (state_out && state_ld < P->n) return fail(DMPM_EINVAL, "state_ld below the particle count");
+  if (pbuf_in == pbuf_out) return fail(DMPM_EINVAL, "input and output blocks must differ");
+  if (scheme != DMPM_USF && scheme != DMPM_USL)
+    return fail(DMPM_EINVAL, "scheme must be DMPM_USF or DMPM_USL");
+  if (!gravity) return fail(DMPM_EINVAL, "null gravity");
+  if (!tile_eligible(P, 0))
+    return fail(DMPM_EINVAL,
+                "dmpm_step_between_noalloc covers the fused 2D step (Quadrilateral4Node, no "
+                "particle tractions or particle external forces)");
+  return tile_between_noalloc(P, pbuf_in, pbuf_out, eid, nbuf, (int)scheme, (int)velocity_update,
+                              gravity, dt, step, (int)(flags & 1), (int)((flags >> 1) & 1),
+                              state_in, state_out, state_ld, ST(stream));
 }
 
 namespace dmpm {

@@ -110,6 +110,7 @@ struct dmpm_plan {
   int usf_split = 1;     // USF: stress update streamed (1) or in the force cell pass (0)
   int step_kernel = 1;   // single-simulation fused step: one launch (1) or one per phase (0)
   unsigned char step_aux[256];  // host copy of the one-launch step's device-side tables
+  unsigned char step_aux_ext[256];  // the same, as uploaded into caller-supplied memory
   int step_aux_bytes = 0;       // bytes of step_aux uploaded (0: none yet)
   // workspace of the fused tile kernels and the adjoint (allocated lazily)
   void* extra[16] = {nullptr};
@@ -181,6 +182,14 @@ bool tile_eligible(const dmpm_plan* P, int pfext);
 int tile_between(dmpm_plan* P, const void* pin, void* pout, int32_t* eid, void* nbuf,
                  int scheme, int vu, const double* grav, double dt, int64_t step, int temps,
                  int chained, const void* sin, void* sout, int64_t sld, hipStream_t s);
+// dmpm_step_between on workspaces in the library's static device array
+// (mpm_tile.hip); noalloc_scratch_setup takes that array's address on a
+// device (once per device, with the device current).
+int tile_between_noalloc(dmpm_plan* P, const void* pin, void* pout, int32_t* eid, void* nbuf,
+                         int scheme, int vu, const double* grav, double dt, int64_t step,
+                         int temps, int chained, const void* sin, void* sout, int64_t sld,
+                         hipStream_t s);
+int noalloc_scratch_setup(int device);
 // Kernel classes for dmpm_timing_enable.
 enum { KC_LOCATE = 0, KC_SORT = 1, KC_P2G_KIN = 2, KC_STRESS = 3, KC_P2G_FORCE = 4,
        KC_GRID = 5, KC_G2P = 6, KC_SMALL = 7, KC_TILE_KIN = 8, KC_TILE_FORCE = 9,

@@ -32,6 +32,8 @@
 
 #include <algorithm>
 #include <cstring>
+#include <mutex>
+#include <string>
 
 #include "mpm_common.h"
 #include "mpm_plan.h"
@@ -2524,12 +2526,21 @@ void step_launch(const StepArgs<T>& a, bool usf, hipStream_t s) {
 // holds the static rows (mass, E) and takes the temporaries.  clock: step
 // k's time is read on the device as (*clock + kbase + k) * dt.  prepare:
 // only allocate the workspaces and upload the tables.
+// Workspaces of step_run supplied by the caller instead of the plan
+// (dmpm_step_between_noalloc): the previous/next nodal velocities, the
+// overflow lists and the device copy of the step tables.
+template <typename T> struct StepScratch {
+  T* vbuf;       // [4][nnodes]
+  int* ovf;      // [2 + 2 ntiles] (FP32)
+  StepAux* aux;  // uploaded by every call (the memory is shared between plans)
+};
+
 template <typename T>
 int step_run(dmpm_plan* P, const Mesh2<T>& M, T* pa, T* pb2, T* pstat, int32_t* eid, T* nb,
              int scheme, int vu, const double* grav, double dt, int64_t step0, int64_t nsteps,
              int temps, hipStream_t s, T* vkin, T* sout = nullptr, int64_t sld = 0,
              const T* sin = nullptr, const int64_t* clock = nullptr, int kbase = 0,
-             bool prepare = false) {
+             bool prepare = false, const StepScratch<T>* ext = nullptr) {
   constexpr bool kFast = sizeof(T) == 4;
   const int64_t ld = P->d.ld, nld = P->d.nld;
   const int nn = P->nnodes;
@@ -2541,9 +2552,18 @@ int step_run(dmpm_plan* P, const Mesh2<T>& M, T* pa, T* pb2, T* pstat, int32_t* 
   a.nc_all = P->ncells;
   a.nn_all = nn;
   a.nsims = P->nsims;
-  T* shadow = pb2 ? pb2 : (T*)plan_workspace(P, 5, (size_t)DMPM_P_ROWS * ld * sizeof(T));
-  T* vbuf = (T*)plan_workspace(P, 6, (size_t)4 * nn * sizeof(T));
-  int* ovf = kFast ? (int*)plan_workspace(P, 7, (size_t)(2 + 2 * a.ntiles) * sizeof(int)) : nullptr;
+  T* shadow = pb2;
+  T* vbuf;
+  int* ovf;
+  if (ext) {
+    if (!pb2) return fail(DMPM_EINVAL, "caller-supplied workspaces need both particle blocks");
+    vbuf = ext->vbuf;
+    ovf = kFast ? ext->ovf : nullptr;
+  } else {
+    if (!shadow) shadow = (T*)plan_workspace(P, 5, (size_t)DMPM_P_ROWS * ld * sizeof(T));
+    vbuf = (T*)plan_workspace(P, 6, (size_t)4 * nn * sizeof(T));
+    ovf = kFast ? (int*)plan_workspace(P, 7, (size_t)(2 + 2 * a.ntiles) * sizeof(int)) : nullptr;
+  }
   if (!shadow || !vbuf || (kFast && !ovf)) return fail(DMPM_EHIP, "workspace allocation failed");
   if (!prepare) {
     // the nodal velocity a massless node keeps (the counters of ovf are
@@ -2578,12 +2598,18 @@ int step_run(dmpm_plan* P, const Mesh2<T>& M, T* pa, T* pb2, T* pstat, int32_t* 
   x.fns.fs = P->fn_f;
   a.has_nf = P->nfd != nullptr;
   static_assert(sizeof(StepAux) <= sizeof(P->step_aux), "step_aux size");
-  StepAux* aux = (StepAux*)plan_workspace(P, 8, sizeof(StepAux));
+  StepAux* aux = ext ? ext->aux : (StepAux*)plan_workspace(P, 8, sizeof(StepAux));
   if (!aux) return fail(DMPM_EHIP, "workspace allocation failed");
-  // upload only when the tables changed (a copy from host memory orders the
-  // host behind the stream)
-  if (P->step_aux_bytes != (int)sizeof(StepAux) ||
-      std::memcmp(P->step_aux, &x, sizeof(StepAux)) != 0) {
+  if (ext) {
+    // shared memory: upload on every call, from host memory the plan keeps
+    std::memcpy(P->step_aux_ext, &x, sizeof(StepAux));
+    DMPM_TRY(hip_check(
+        hipMemcpyAsync(aux, P->step_aux_ext, sizeof(StepAux), hipMemcpyHostToDevice, s),
+        "hipMemcpyAsync"));
+  } else if (P->step_aux_bytes != (int)sizeof(StepAux) ||
+             std::memcmp(P->step_aux, &x, sizeof(StepAux)) != 0) {
+    // upload only when the tables changed (a copy from host memory orders
+    // the host behind the stream)
     std::memcpy(P->step_aux, &x, sizeof(StepAux));
     P->step_aux_bytes = (int)sizeof(StepAux);
     DMPM_TRY(hip_check(hipMemcpyAsync(aux, P->step_aux, sizeof(StepAux), hipMemcpyHostToDevice, s),
@@ -2771,6 +2797,73 @@ int tile_steps_cb(dmpm_plan* P, const Mesh2<T>& M, T* pb, int32_t* eid, T* nb, i
   }
 }
 
+// ---- one step without allocation ---------------------------------------------
+//
+// dmpm_step_between_noalloc keeps its workspaces (kinematic nodal velocity,
+// previous and next nodal velocities, overflow lists, step tables) in one
+// file-scope device array instead of plan memory.  Host code must not pass
+// the array's name to a kernel: its device address on each GPU comes from
+// hipGetSymbolAddress, taken once per device at plan creation
+// (noalloc_scratch_setup) and kept in g_scratch_dev.
+constexpr size_t kNoAllocScratchBytes = (size_t)DMPM_NOALLOC_SCRATCH_BYTES;
+__device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+constexpr int kMaxDevices = 64;
+void* g_scratch_dev[kMaxDevices] = {};
+std::mutex g_scratch_mu;
+
+inline size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+// Bytes of the array that one step of plan P needs (see the layout below).
+template <typename T>
+size_t noalloc_bytes(const dmpm_plan* P) {
+  const size_t nn = (size_t)P->nnodes;
+  return align256(2 * nn * sizeof(T)) + align256(4 * nn * sizeof(T)) +
+         align256((2 + 2 * (size_t)P->ntiles) * sizeof(int)) + align256(sizeof(StepAux));
+}
+
+template <typename T>
+int tile_between_noalloc_t(dmpm_plan* P, const T* pin, T* pout, int32_t* eid, T* nb, int scheme,
+                           int vu, const double* grav, double dt, int64_t step, int temps,
+                           int chained, const T* sin, T* sout, int64_t sld, hipStream_t s) {
+  if (P->device < 0 || P->device >= kMaxDevices || !g_scratch_dev[P->device])
+    return fail(DMPM_EHIP, "no scratch address for this plan's device");
+  if (P->nsims != 1 || !step_eligible(P) ||
+      (uint64_t)8 * (uint64_t)sld * sizeof(T) >= (1ull << 32))
+    return fail(DMPM_EINVAL,
+                "dmpm_step_between_noalloc covers the one-launch step of single-simulation "
+                "plans (blocks below 4 GiB, step kernel option on)");
+  const size_t need = noalloc_bytes<T>(P);
+  if (need > kNoAllocScratchBytes)
+    return fail(DMPM_EINVAL, "dmpm_step_between_noalloc: the plan needs " +
+                                 std::to_string(need) + " bytes of scratch, above the " +
+                                 std::to_string(kNoAllocScratchBytes) +
+                                 "-byte array (24 bytes per node in FP32, 48 in FP64)");
+  // layout: vkin [2][nn] | vbuf [4][nn] | ovf [2 + 2 ntiles] | aux
+  unsigned char* base = (unsigned char*)g_scratch_dev[P->device];
+  const size_t nn = (size_t)P->nnodes;
+  T* vkin = (T*)base;
+  StepScratch<T> ext;
+  ext.vbuf = (T*)(base + align256(2 * nn * sizeof(T)));
+  ext.ovf = (int*)((unsigned char*)ext.vbuf + align256(4 * nn * sizeof(T)));
+  ext.aux = (StepAux*)((unsigned char*)ext.ovf +
+                       align256((2 + 2 * (size_t)P->ntiles) * sizeof(int)));
+  const Mesh2<T> M = mesh_of<T>(P);
+  const int64_t ld = P->d.ld;
+  const int n = (int)P->n;
+  if (!chained) {
+    const T* px = sin ? sin : pin;
+    const int64_t pld = sin ? sld : ld;
+    {
+      ProfScopeT ps_(P, KC_SORT, s);
+      DMPM_TRY(enqueue_resort<T>(P, M, px, eid, s, pld));
+    }
+    k_tile_check<T><<<nblk(n), kThreads, 0, s>>>(M, px, pld, n, eid, P->key, move_log(P));
+    DMPM_TRY(launch_check("tile_check"));
+  }
+  return step_run<T>(P, M, const_cast<T*>(pin), pout, pout, eid, nb, scheme, vu, grav, dt, step,
+                     1, temps, s, vkin, sout, sld, sin, nullptr, 0, false, &ext);
+}
+
 }  // namespace
 
 namespace dmpm {
@@ -2804,6 +2897,30 @@ int tile_run(dmpm_plan* P, void* pbuf, int32_t* eid, void* nbuf, int scheme, int
 int tile_prepare(dmpm_plan* P, int scheme, hipStream_t s) {
   if (P->d.dtype == DMPM_F32) return tile_prepare_t<float>(P, scheme, s);
   return tile_prepare_t<double>(P, scheme, s);
+}
+
+int noalloc_scratch_setup(int device) {
+  if (device < 0 || device >= kMaxDevices) return fail(DMPM_EINVAL, "device index");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_dev[device]) return DMPM_OK;
+  void* p = nullptr;
+  DMPM_TRY(hip_check(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress"));
+  if (!p) return fail(DMPM_EHIP, "hipGetSymbolAddress returned null");
+  g_scratch_dev[device] = p;
+  return DMPM_OK;
+}
+
+int tile_between_noalloc(dmpm_plan* P, const void* pin, void* pout, int32_t* eid, void* nbuf,
+                         int scheme, int vu, const double* grav, double dt, int64_t step,
+                         int temps, int chained, const void* sin, void* sout, int64_t sld,
+                         hipStream_t s) {
+  if (P->d.dtype == DMPM_F32)
+    return tile_between_noalloc_t<float>(P, (const float*)pin, (float*)pout, eid, (float*)nbuf,
+                                         scheme, vu, grav, dt, step, temps, chained,
+                                         (const float*)sin, (float*)sout, sld, s);
+  return tile_between_noalloc_t<double>(P, (const double*)pin, (double*)pout, eid, (double*)nbuf,
+                                        scheme, vu, grav, dt, step, temps, chained,
+                                        (const double*)sin, (double*)sout, sld, s);
 }
 
 }  // namespace dmpm

@@ -292,6 +292,26 @@ int dmpm_step_between(dmpm_plan* plan, const void* pbuf_in, void* pbuf_out, int3
                       const double* gravity, double dt, int64_t step, int32_t flags,
                       const void* state_in, void* state_out, int64_t state_ld, void* stream);
 
+/* dmpm_step_between for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc); no
+ * counterpart in the reference.  Same arguments, restrictions and results
+ * as dmpm_step_between for single-simulation plans whose one-launch step
+ * applies; never calls hipMalloc or hipFree.  Its workspaces (kinematic
+ * nodal velocity, previous and next nodal velocities, overflow lists, step
+ * tables: 24 bytes per node in FP32, 48 in FP64, plus 8 bytes per 16x16
+ * tile and 1 KiB) live in one static device array of
+ * DMPM_NOALLOC_SCRATCH_BYTES per GPU, which every call on that GPU shares:
+ * calls that overlap on one GPU race, and the step's kinematic nodal
+ * velocity is not kept for dmpm_plan_export_step.  A plan that needs more
+ * is rejected with DMPM_EINVAL (at 32 MiB: up to about 1.3 M nodes in FP32,
+ * 0.65 M in FP64). */
+#define DMPM_NOALLOC_SCRATCH_BYTES (32u << 20)
+int dmpm_step_between_noalloc(dmpm_plan* plan, const void* pbuf_in, void* pbuf_out,
+                              int32_t* eid, void* nbuf, int32_t scheme, int32_t velocity_update,
+                              const double* gravity, double dt, int64_t step, int32_t flags,
+                              const void* state_in, void* state_out, int64_t state_ld,
+                              void* stream);
+
 /* ---- adjoint of the fused 2D step ------------------------------------------
  * The reference differentiates a rollout with jax.grad (no hand-written
  * adjoint; diffmpm/solver.py:179-213).  This is the reverse-mode
