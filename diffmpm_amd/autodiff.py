@@ -13,9 +13,12 @@ sigma_yy, sigma_xy, volume (``STATE_ROWS``).  Optional parameter: log of
 the Young's modulus, per particle ``(N,)`` or one scalar for the whole set.
 Mass, mesh, constraints and nodal forces are constants; strain, density,
 volumetric strain and sigma_zz do not feed back into the 2D dynamics and get
-no gradient.  The covered configurations are those of the fused path: 2D
-Quadrilateral4Node, one particle set, LinearElastic or SimpleMaterial (log E
-only for LinearElastic), no particle tractions.
+no gradient.  The covered configurations: one particle set without particle
+tractions or particle external forces, LinearElastic or SimpleMaterial (log E
+for both: a per-particle E replaces the material's), on a 2D
+Quadrilateral4Node mesh (the fused step) or a Linear1D mesh (the per-phase
+step; the y and vy rows of the state are constants there, and the adjoint of
+a 1D step recomputes the step).
 
 Example::
 
@@ -57,6 +60,7 @@ class Differentiable:
         d = cls.__new__(cls)
         d.sim, d.ps, d.el = None, None, None
         d.bplan, d._blocks, d._nodes_last = None, None, None
+        d.dim = 2
         d.per_particle_E = all(s.mesh.particles[0]._has_youngs for s in ens.sims)
         d.n = ens.n
         d._pbuf, d._nbuf, d._eid_f = ens.pbuf, ens.nbuf, ens.eid
@@ -75,15 +79,27 @@ class Differentiable:
         if len(mesh.particles) != 1:
             raise ValueError("the differentiable step covers one particle set")
         ps, el = mesh.particles[0], mesh.elements
-        if el.ndim != 2:
-            raise ValueError("the differentiable step covers 2D Quadrilateral4Node meshes")
+        if el.ndim not in (1, 2):
+            raise ValueError("the differentiable step covers Linear1D and Quadrilateral4Node "
+                             "meshes")
+        if ps._has_fext:
+            raise ValueError("the differentiable step covers particle sets without particle "
+                             "external forces")
+        if getattr(ps, "_traction_specs", []):
+            raise ValueError("the differentiable step covers particle sets without particle "
+                             "tractions")
         if per_particle_E and not ps._has_youngs:
             props = ps.material.properties
-            if "youngs_modulus" not in props:
-                raise ValueError("log E gradients need a LinearElastic material")
-            ps.youngs_modulus = torch.full((len(ps),), float(props["youngs_modulus"]),
-                                           dtype=ps.dtype, device=ps.device)
+            key = next((k for k in ("youngs_modulus", "E") if k in props), None)
+            if key is None:
+                raise ValueError("log E gradients need a LinearElastic or SimpleMaterial "
+                                 "material")
+            ps.youngs_modulus = torch.full((len(ps),), float(props[key]), dtype=ps.dtype,
+                                           device=ps.device)
             ps._plans.clear()
+        self.dim = el.ndim
+        if self.dim == 1:
+            self.save_forward = False  # the 1D adjoint recomputes its step
         self.sim, self.ps, self.el = sim, ps, el
         self.per_particle_E = per_particle_E
         self.n = len(ps)
@@ -156,8 +172,10 @@ class Differentiable:
         """One forward step; with save=True also returns what its adjoint
         needs (node block, kinematic nodal velocity, sort by cell).  A single
         simulation steps between two private particle blocks (the
-        simulation's own block is not written); an ensemble steps its batch
-        block in place."""
+        simulation's own block is not written; a 1D one on one private
+        block); an ensemble steps its batch block in place."""
+        if self.dim == 1:
+            return self._forward_1d(state, log_e, gravity, i), None
         if self.bplan is None:
             self.set_state(state, log_e)
             self.fplan.run(self.scheme, self.velocity_update, gravity, self.dt, i, 1,
@@ -209,6 +227,29 @@ class Differentiable:
         nsaved = self._nbuf.clone()
         vkin, perm, cs = self._export(plan)
         return out, (nsaved, vkin, perm, cs)
+
+    def _forward_1d(self, state, log_e, gravity, i):
+        """One per-phase step of a Linear1D simulation on a private particle
+        block, node block and element ids."""
+        if self._blocks is None:
+            self._blocks = (self._pbuf.clone(),)
+            self._nodes_last = self._nbuf.clone()
+        blk = self._blocks[0]
+        last = getattr(self, "_last_1d", None)
+        if not (last is not None and last[0] is state and state._version == last[1]):
+            for a, b in _RUNS:
+                blk[b, : self.n] = state[a].detach()
+        if log_e is not None:
+            self._write_log_e_private(blk, log_e)
+        plan = self.bplan
+        plan.refresh_tables()
+        call("dmpm_run", plan.h, ptr(blk), ptr(self._eid_b), ptr(self._nodes_last), self.code,
+             self.vu, plan._gravity(gravity), self.dt, int(i), 1, 0, plan.s)
+        out = torch.empty((C_ROWS, self.n), dtype=self._pbuf.dtype, device=self._pbuf.device)
+        for a, b in _RUNS:
+            out[a] = blk[b, : self.n]
+        self._last_1d = (out, out._version)
+        return out
 
     def _export(self, plan):
         """The kinematic nodal velocity and the sort by cell of the plan's last step."""

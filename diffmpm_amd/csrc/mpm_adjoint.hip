@@ -153,7 +153,7 @@ __device__ __forceinline__ float mixed2(int k) { return (k & 1) ? -0.25f : 0.25f
 template <typename T>
 __device__ __forceinline__ void d_apply(const Mat2& m, T E_p, const T in[4], T out[4]) {
   if (m.kind == DMPM_SIMPLE_MATERIAL) {
-    const T E = T(m.de[0]);
+    const T E = m.per_E ? E_p : T(m.de[0]);
     for (int k = 0; k < 4; ++k) out[k] = in[k] * E;
     return;
   }
@@ -812,13 +812,17 @@ extern "C" int dmpm_step_adjoint(dmpm_plan* P, const void* pin, void* pwork, int
     return fail(DMPM_EINVAL, "null buffer");
   if (scheme != DMPM_USF && scheme != DMPM_USL)
     return fail(DMPM_EINVAL, "scheme must be DMPM_USF or DMPM_USL");
-  if (!tile_eligible(P, 0))
-    return fail(DMPM_EINVAL,
-                "the adjoint step covers 2D Quadrilateral4Node meshes without particle "
-                "tractions or particle external forces");
   if (P->d.material != DMPM_LINEAR_ELASTIC && P->d.material != DMPM_SIMPLE_MATERIAL)
     return fail(DMPM_EINVAL, "unknown material");
   hipStream_t s = (hipStream_t)stream;
+  if (P->dim == 1)
+    return adjoint_1d(P, pin, pwork, eid_work, nwork, cot, lebar, scheme, velocity_update,
+                      gravity, dt, step, s);
+  if (!tile_eligible(P, 0))
+    return fail(DMPM_EINVAL,
+                "the adjoint step covers Linear1D meshes without particle tractions and 2D "
+                "Quadrilateral4Node meshes without particle tractions or particle external "
+                "forces");
   if (P->d.dtype == DMPM_F32)
     return adjoint_t<float, 128>(P, (const float*)pin, (float*)pwork, eid_work, (float*)nwork,
                                  (float*)cot, (float*)lebar, scheme, velocity_update, gravity, dt,

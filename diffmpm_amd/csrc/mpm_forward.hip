@@ -351,8 +351,9 @@ template <typename T>
 __device__ __forceinline__ void stress_increment(const Mat& m, const T* __restrict__ pb,
                                                  int64_t ld, int p, const T d[6], T ds[6]) {
   if (m.kind == DMPM_SIMPLE_MATERIAL) {
-    T E = T(m.de[0]);
-    for (int k = 0; k < 6; ++k) ds[k] = d[k] * E;  // material.py:130-131
+    // material.py:130-131; a per-particle E replaces the material's
+    T E = m.per_E ? PROW(DMPM_P_YOUNGS)[p] : T(m.de[0]);
+    for (int k = 0; k < 6; ++k) ds[k] = d[k] * E;
     return;
   }
   T a1 = T(m.de[0]), a2 = T(m.de[1]), G = T(m.de[2]);
@@ -1753,6 +1754,33 @@ template <typename T, int DIM> struct RunSteps {
     return DMPM_OK;
   }
 };
+
+// One step of the per-phase path (scheme.py:22-67, as dmpm_run takes it for
+// meshes the fused step does not cover), copying the kinematic nodal velocity
+// (after the constraints, before the forces) to vkin, DIM rows of nnodes.
+template <typename T, int DIM> struct StepKeepVkin {
+  static int run(dmpm_plan* P, void* pbuf, int32_t* eid, void* nbuf, int scheme, int vu,
+                 const double* grav, double dt, int64_t step, void* vkin, hipStream_t s) {
+    DMPM_TRY((PhaseKin<T, DIM>::run(P, pbuf, eid, nbuf, s)));
+    DMPM_TRY(hip_check(hipMemcpy2DAsync(vkin, (size_t)P->nnodes * sizeof(T),
+                                        (T*)nbuf + (int64_t)DMPM_N_V * P->d.nld,
+                                        (size_t)P->d.nld * sizeof(T),
+                                        (size_t)P->nnodes * sizeof(T), DIM,
+                                        hipMemcpyDeviceToDevice, s),
+                       "hipMemcpy2DAsync"));
+    if (scheme == DMPM_USF) DMPM_TRY((PhaseStress<T, DIM>::run(P, pbuf, eid, nbuf, dt, 0, s)));
+    DMPM_TRY((PhaseForces<T, DIM>::run(P, pbuf, eid, nbuf, grav, step, dt, 0, s)));
+    DMPM_TRY((PhasePkin<T, DIM>::run(P, pbuf, eid, nbuf, dt, vu, 0, s)));
+    if (scheme == DMPM_USL) DMPM_TRY((PhaseStress<T, DIM>::run(P, pbuf, eid, nbuf, dt, 0, s)));
+    return DMPM_OK;
+  }
+};
+
+int phase_step_keep_vkin(dmpm_plan* P, void* pbuf, int32_t* eid, void* nbuf, int scheme, int vu,
+                         const double* grav, double dt, int64_t step, void* vkin,
+                         hipStream_t s) {
+  return dispatch<StepKeepVkin>(P, pbuf, eid, nbuf, scheme, vu, grav, dt, step, vkin, s);
+}
 
 }  // namespace dmpm
 

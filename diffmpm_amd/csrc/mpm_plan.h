@@ -190,6 +190,15 @@ int tile_between_noalloc(dmpm_plan* P, const void* pin, void* pout, int32_t* eid
                          int temps, int chained, const void* sin, void* sout, int64_t sld,
                          hipStream_t s);
 int noalloc_scratch_setup(int device);
+// One per-phase step (mpm_forward.hip) that also copies the kinematic nodal
+// velocity (dim rows of nnodes) to vkin.
+int phase_step_keep_vkin(dmpm_plan* P, void* pbuf, int32_t* eid, void* nbuf, int scheme, int vu,
+                         const double* grav, double dt, int64_t step, void* vkin, hipStream_t s);
+// The adjoint of one Linear1D step (mpm_adjoint_1d.hip); arguments as
+// dmpm_step_adjoint.
+int adjoint_1d(dmpm_plan* P, const void* pin, void* pwork, int32_t* eid, void* nwork, void* cot,
+               void* lebar, int scheme, int vu, const double* grav, double dt, int64_t step,
+               hipStream_t s);
 // Kernel classes for dmpm_timing_enable.
 enum { KC_LOCATE = 0, KC_SORT = 1, KC_P2G_KIN = 2, KC_STRESS = 3, KC_P2G_FORCE = 4,
        KC_GRID = 5, KC_G2P = 6, KC_SMALL = 7, KC_TILE_KIN = 8, KC_TILE_FORCE = 9,

@@ -106,6 +106,8 @@ class SimpleMaterial(Material):
         return torch.as_tensor(dstrain) * self.properties["E"]
 
     def _kernel_coefficients(self, per_particle_E=False):
+        # with a per-particle modulus (Particles.youngs_modulus) the kernels
+        # take E from the particle row instead of the material's E
         if per_particle_E:
-            raise ValueError("per-particle Young's modulus needs LinearElastic")
+            return _lib.DMPM_SIMPLE_MATERIAL, (1.0, 0.0, 0.0)
         return _lib.DMPM_SIMPLE_MATERIAL, (float(self.properties["E"]), 0.0, 0.0)

@@ -336,9 +336,12 @@ enum {
  * other rows must hold finite values); cot: in = cotangent of the state after the
  * step, out = cotangent of the state before it; lebar (may be NULL): [ld],
  * += dL/dlog(E_p) (per-particle E) or its per-particle terms (scalar E: sum
- * them).  Same restrictions as the fused path (2D Q4, no particle tractions
- * or particle f_ext).  Uses the plan's sort state: use a separate plan from
- * the forward rollout's to keep that one's incremental re-sort. */
+ * them).  Covers the fused path's plans (2D Q4, no particle tractions or
+ * particle f_ext) and single Linear1D plans (the per-phase step of
+ * scheme.py:22-67, no particle tractions or particle f_ext; in 1D the y and
+ * vy rows pass their cotangents through).  Uses the plan's sort state: use a
+ * separate plan from the forward rollout's to keep that one's incremental
+ * re-sort. */
 int dmpm_step_adjoint(dmpm_plan* plan, const void* pin, void* pwork, int32_t* eid_work,
                       void* nwork, void* cot, void* lebar, int32_t scheme,
                       int32_t velocity_update, const double* gravity, double dt, int64_t step,

@@ -579,6 +579,10 @@ class Particles:
 
     def compute_stress(self, *args):
         """particle.py:315-322 (+ per-particle E extension)."""
+        if self.youngs_modulus is not None and isinstance(self.material, SimpleMaterial):
+            # per-particle E in place of the material's E
+            self.stress = self.stress + self.dstrain * self.youngs_modulus[:, None]
+            return
         if self.youngs_modulus is not None:
             E = self.youngs_modulus
             nu = self.material.properties["poisson_ratio"]

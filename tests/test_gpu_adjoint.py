@@ -25,10 +25,12 @@ FD_H_LOGE = 1e-3
 RTOL_FD = 2e-5
 
 
-def _sim(scheme, vu, dtype=torch.float64, seed=3):
+def _sim(scheme, vu, dtype=torch.float64, seed=3, material="LinearElastic"):
     spec = cases.synthetic_spec("quad4", nx=10, ny=8, rows=5, seed=seed, scheme=scheme,
                                 velocity_update=vu, traction=False, nodal_force=True, E=1e6,
                                 dt=1e-5, nsteps=NSTEPS, per_particle_E=True, velocity_scale=0.5)
+    if material == "SimpleMaterial":
+        spec["material"] = {"type": "SimpleMaterial", "E": 1e6, "density": 1000.0}
     return cases.build_gpu(spec, dtype, keep_temporaries=False), spec
 
 
@@ -40,9 +42,9 @@ def _weights(state0, seed=5):
     return (w / scale).to(state0.dtype)
 
 
-def _setup(scheme, vu, dtype=torch.float64):
+def _setup(scheme, vu, dtype=torch.float64, material="LinearElastic"):
     from diffmpm_amd.autodiff import Differentiable
-    sim, spec = _sim(scheme, vu, dtype)
+    sim, spec = _sim(scheme, vu, dtype, material=material)
     d = Differentiable(sim)
     d.spec = spec  # the oracle's copy of the case
     g = torch.tensor(spec["gravity"], dtype=torch.float64)
@@ -107,10 +109,13 @@ def _grad(d, g, state0, log_e, w, checkpoint_every=0):
     return s.grad.detach(), le.grad.detach(), out.detach()
 
 
-@pytest.mark.parametrize("scheme,vu", [("usf", False), ("usl", False), ("usf", True),
-                                       ("usl", True)])
-def test_log_e_gradient_matches_central_differences(scheme, vu):
-    d, g, state0, log_e = _setup(scheme, vu)
+@pytest.mark.parametrize("scheme,vu,material", [
+    ("usf", False, "LinearElastic"), ("usl", False, "LinearElastic"),
+    ("usf", True, "LinearElastic"), ("usl", True, "LinearElastic"),
+    ("usf", False, "SimpleMaterial"), ("usl", True, "SimpleMaterial")])
+def test_log_e_gradient_matches_central_differences(scheme, vu, material):
+    """Per-particle log E; for SimpleMaterial E_p replaces the material's E."""
+    d, g, state0, log_e = _setup(scheme, vu, material=material)
     w = _weights(state0)
     gs, ge, out = _grad(d, g, state0, log_e, w)
     assert torch.equal(_cells(d, state0), _cells(d, out)), "a particle changed cell"

@@ -118,6 +118,36 @@ def test_linear_velocity_field_1d_quirk():
     assert np.allclose(ps.dvolumetric_strain, 0.4, atol=1e-12)
 
 
+def test_simple_material_per_particle_modulus():
+    """SimpleMaterial with a per-particle E: dsigma_p = E_p dstrain_p
+    (material.py:130-131 with E_p in place of the scalar E).  Particles
+    holding E_p = E or 2E step exactly as the scalar E or 2E material."""
+    def run(E, youngs=None):
+        el = om.Elements("linear1d", 8, 0.125, [(np.array([0]), om.Constraint(0, 0.0))],
+                         dtype=np.float64)
+        ps = om.Particles(np.linspace(0.05, 0.95, 16)[:, None],
+                          om.SimpleMaterial({"E": E, "density": 1.0}), dtype=np.float64)
+        ps.velocity[:, 0] = 0.1 * np.sin(np.pi * ps.loc[:, 0] / 2)
+        if youngs is not None:
+            ps.youngs_modulus = youngs
+        sim = om.MPMExplicit(om.Mesh(el, [ps]), 1e-3, "usf", False, 5)
+        sim.solve(np.array([0.0]))
+        return ps.stress.copy(), ps.velocity.copy()
+    for scale in (1.0, 2.0):
+        got = run(100.0, np.full(16, 100.0 * scale))
+        want = run(100.0 * scale)
+        assert np.array_equal(got[0], want[0]) and np.array_equal(got[1], want[1])
+    # mixed: each particle's increment uses its own E
+    E = np.where(np.arange(16) % 2 == 0, 100.0, 300.0)
+    el = om.Elements("linear1d", 8, 0.125, dtype=np.float64)
+    ps = om.Particles(np.linspace(0.05, 0.95, 16)[:, None],
+                      om.SimpleMaterial({"E": 7.0, "density": 1.0}), dtype=np.float64)
+    ps.youngs_modulus = E
+    ps.dstrain = rng(3).standard_normal((16, 6))
+    ps.compute_stress()
+    assert np.allclose(ps.stress, ps.dstrain * E[:, None], rtol=0, atol=0)
+
+
 @pytest.mark.parametrize("nu", [0.0, 0.25, 0.3, 0.45])
 def test_linear_elastic_closed_form(nu):
     E = 2.5e6
