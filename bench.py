@@ -251,6 +251,62 @@ def fwd_adjoint(sim, gravity, K):
                     "E; loss = sum of final positions"}
 
 
+def inverse_bench(iters, K=2000, checkpoint_every=100):
+    """The differentiable inverse problem at its configuration size
+    (workloads.inverse_2d(128): 32,768 particles, USF, dt = 1e-5): the
+    target is the final positions of a K-step rollout with the true E_p
+    field; Adam (lr 1e-2 on log E) from the uniform E = 1e6 minimises the
+    mean square position error (divided by the target's mean square
+    displacement, so that Adam's epsilon stays below the gradient scale),
+    each iteration a K-step rollout with the adjoint checkpointed every
+    checkpoint_every steps.  Reports the time per iteration, the
+    forward+adjoint step rate, and the loss and log E error trajectory."""
+    import math
+    import time
+    import torch
+    from diffmpm_amd import workloads
+    from diffmpm_amd.autodiff import Differentiable, rollout
+    sim, g, E_true = workloads.inverse_2d(128)
+    d = Differentiable(sim)
+    N = d.n
+    state0 = d.get_state().clone()
+    le_true = torch.log(E_true.to("cuda", torch.float32))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    with torch.no_grad():
+        target = rollout(d, K, g, state=state0, log_e=le_true)[0:2].clone()
+    torch.cuda.synchronize()
+    fwd_s = time.perf_counter() - t0
+    scale = ((target - state0[0:2]) ** 2).mean()
+    le = torch.full((N,), math.log(1e6), device="cuda", requires_grad=True)
+    opt = torch.optim.Adam([le], lr=1e-2)
+    dev = (le_true - le_true.mean()).norm()
+    losses, errs, secs = [], [], []
+    for _ in range(iters):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        opt.zero_grad()
+        out = rollout(d, K, g, state=state0, log_e=le, checkpoint_every=checkpoint_every)
+        loss = ((out[0:2] - target) ** 2).mean() / scale
+        loss.backward()
+        opt.step()
+        torch.cuda.synchronize()
+        secs.append(time.perf_counter() - t0)
+        losses.append(float(loss))
+        errs.append(float((le.detach() - le_true).norm() / dev))
+    it_s = sorted(secs)[len(secs) // 2]
+    return {"workload": "inverse problem: Q4 128x128, 32768 particles (lower half, Gauss "
+                        "points), E_p = 1e6 exp(0.3 g), g Gaussian field (corr. 0.1, seed 0), "
+                        "nu 0.3, rho 1000, gravity, bottom fixed, USF, dt 1e-5",
+            "particles": N, "steps": K, "checkpoint_every": checkpoint_every,
+            "iterations": iters, "s_per_iteration": it_s,
+            "fwd_adjoint_particle_steps_per_s": N * K / it_s,
+            "forward_only_s": fwd_s, "loss": losses, "log_e_rel_error": errs,
+            "note": "one iteration = forward rollout (autograd records segment starts), "
+                    "backward (each segment recomputed, then its adjoint steps), Adam step; "
+                    "median time over the iterations"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -269,6 +325,8 @@ def main():
                     help="steps of the timed 64-member uneven ensemble (0: skip)")
     ap.add_argument("--adjoint-steps", type=int, default=20,
                     help="steps of the timed forward+adjoint rollout (0: skip)")
+    ap.add_argument("--inverse-iters", type=int, default=20,
+                    help="Adam iterations of the 2000-step 128x128 inverse problem (0: skip)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="after the timed region, copy 512 MiB with a plain device copy "
@@ -361,6 +419,8 @@ def main():
         del sim, ps, plan
         torch.cuda.empty_cache()
         line["ensemble"] = ensemble_bench(args.ensemble_steps)
+    if args.inverse_iters:
+        line["inverse"] = inverse_bench(args.inverse_iters)
     if args.copy_calibration:
         a = torch.empty(128 * 1024 * 1024, dtype=torch.float32, device="cuda")
         b = torch.empty_like(a)

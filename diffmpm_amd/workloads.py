@@ -37,6 +37,45 @@ def gauss_points(nx, rows, h, jitter=0.0, seed=0):
     return pts
 
 
+def gaussian_field(n, h, corr, seed):
+    """Unit-variance Gaussian random field on the (n+1)^2 node grid (indexed
+    [j, i]): seeded white noise filtered by a Gaussian of standard deviation
+    corr (in length units, periodic), normalised to zero mean and unit
+    variance."""
+    from scipy.ndimage import gaussian_filter
+    rng = np.random.Generator(np.random.PCG64(seed))
+    w = rng.standard_normal((n + 1, n + 1))
+    f = gaussian_filter(w, sigma=corr / h, mode="wrap")
+    return (f - f.mean()) / f.std()
+
+
+def bilinear(field, h, pts):
+    """Bilinear interpolation of a node-grid field (indexed [j, i]) at points."""
+    x = pts[:, 0] / h
+    y = pts[:, 1] / h
+    i = np.clip(np.floor(x).astype(int), 0, field.shape[1] - 2)
+    j = np.clip(np.floor(y).astype(int), 0, field.shape[0] - 2)
+    fx, fy = x - i, y - j
+    return ((1 - fx) * (1 - fy) * field[j, i] + fx * (1 - fy) * field[j, i + 1]
+            + (1 - fx) * fy * field[j + 1, i] + fx * fy * field[j + 1, i + 1])
+
+
+def inverse_2d(n=128, seed=0, dtype=torch.float32, device="cuda", side_rollers=False):
+    """The differentiable inverse problem's case: block_2d(n) with particles
+    at the 2x2 Gauss points of the lower n/2 cell rows (no jitter), nu = 0.3,
+    rho = 1000, gravity, bottom nodes fixed, dt = 1e-5, USF, and the true
+    per-particle modulus E_p = 1e6 exp(0.3 g(x_p)), g = gaussian_field(n, 1/n,
+    0.1, seed) interpolated bilinearly.  The simulation starts with the
+    uniform E = 1e6.  Returns (MPMExplicit, gravity, E_true as an (N,) float64
+    CPU tensor)."""
+    sim, g = block_2d(n, rows=n // 2, jitter=0.0, E=1e6, nu=0.3, scheme="usf", dtype=dtype,
+                      device=device, side_rollers=side_rollers)
+    h = 1.0 / n
+    pts = gauss_points(n, n // 2, h)
+    E = 1e6 * np.exp(0.3 * bilinear(gaussian_field(n, h, 0.1, seed), h, pts))
+    return sim, g, torch.from_numpy(E)
+
+
 def ensemble_specs(nsims=64, seed=0):
     """Member parameters of the uneven ensemble: n = round(exp(U(ln 32,
     ln 256))) cells per side, rows = ceil(U(0.3, 1.0) n) particle rows,

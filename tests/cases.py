@@ -315,25 +315,13 @@ def config_c2_spec(nsteps=3):
 
 
 def gaussian_random_field(n, h, corr, seed):
-    """Unit-variance Gaussian random field on the (n+1)^2 node grid: seeded
-    white noise filtered by a Gaussian of standard deviation corr (in length
-    units), normalised to unit variance."""
-    from scipy.ndimage import gaussian_filter
-    rng = np.random.Generator(np.random.PCG64(seed))
-    w = rng.standard_normal((n + 1, n + 1))
-    f = gaussian_filter(w, sigma=corr / h, mode="wrap")
-    return (f - f.mean()) / f.std()
+    from diffmpm_amd.workloads import gaussian_field
+    return gaussian_field(n, h, corr, seed)
 
 
 def bilinear(field, h, pts):
-    """Bilinear interpolation of a node-grid field (indexed [j, i]) at points."""
-    x = pts[:, 0] / h
-    y = pts[:, 1] / h
-    i = np.clip(np.floor(x).astype(int), 0, field.shape[1] - 2)
-    j = np.clip(np.floor(y).astype(int), 0, field.shape[0] - 2)
-    fx, fy = x - i, y - j
-    return ((1 - fx) * (1 - fy) * field[j, i] + fx * (1 - fy) * field[j, i + 1]
-            + (1 - fx) * fy * field[j + 1, i] + fx * fy * field[j + 1, i + 1])
+    from diffmpm_amd.workloads import bilinear as b
+    return b(field, h, pts)
 
 
 def config_c4_spec(nsteps=3, seed=0, n=128):
