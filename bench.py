@@ -142,16 +142,24 @@ def cpu_baseline(rows=64, np_steps=12, torch_steps=80):
                                       f"({np_s:.1f} s), same strip"}}
 
 
-def ensemble_bench(K):
+def ensemble_bench(K, K_adj=1000, K_seq_adj=50, checkpoint_every=100):
     """64 uneven USL members (32..256 cells per side, 30-100 % of the rows
-    filled, particles at the Gauss points): K steps with one batch plan (the
-    autotuner's choice of step kernels, the K steps captured into one HIP
-    graph and replayed), against the same K steps of every member with its
-    own plan, one member after the other.  The batch plan's eager run (one
-    dmpm_run call per K steps) is reported beside the graph replay."""
+    filled, particles at the Gauss points).  Forward: K steps with one batch
+    plan (the autotuner's choice of step kernels, the K steps captured into
+    one HIP graph and replayed; the eager run of the same plan beside it),
+    against the same K steps of every member with its own plan, one member
+    after the other.  Forward+adjoint w.r.t. each member's scalar E (one log
+    E per member, mapped to its particles): K_adj batched steps with the
+    adjoint checkpointed every checkpoint_every steps, against K_seq_adj
+    steps of every member's own differentiable rollout, one after the other
+    (per-step rates; the sequential rollouts are bound by their per-step host
+    work, so fewer steps time them)."""
+    import math
+    import time
     import torch
     from diffmpm_amd import workloads
     from diffmpm_amd._plan import plan_for
+    from diffmpm_amd.autodiff import Differentiable, rollout
     from diffmpm_amd.ensemble import Ensemble
     from diffmpm_amd.graphs import StepGraph
     sims, g, specs = workloads.ensemble(64, seed=0)
@@ -177,20 +185,74 @@ def ensemble_bench(K):
     torch.cuda.synchronize()
     tg, te, ts = (ev[i].elapsed_time(ev[i + 1]) for i in range(3))
     assert torch.isfinite(ens.pbuf[0:2, :ens.n]).all()
-    return {"workload": "64 uneven Q4 members, n = 32..256 cells per side (log-uniform), "
-                        "30-100 % rows filled at the Gauss points, E 1e5..1e7, nu 0.2..0.4, "
-                        "USL, dt=1e-5, seed 0",
-            "particles": ntot, "steps": K,
-            "batched": {"value": ntot * K / (tg / 1e3), "unit": "particle-steps/s",
-                        "ms_per_step": tg / K, "run": "HIP graph of the K steps, one replay"},
-            "batched_eager": {"value": ntot * K / (te / 1e3), "unit": "particle-steps/s",
-                              "ms_per_step": te / K},
-            "sequential": {"value": ntot * K / (ts / 1e3), "unit": "particle-steps/s",
-                           "ms_per_step": ts / K},
-            "speedup": ts / tg,
-            "autotune_ms_per_step": {str(k): v for k, v in tuned.items()},
-            "batch_step": "one launch" if ens.step_kernel else
-                          "per phase, %d cells per workgroup" % ens.cells_per_block}
+    nodes = graph.kernel_nodes()
+    del graph
+    out = {"workload": "64 uneven Q4 members, n = 32..256 cells per side (log-uniform), "
+                       "30-100 % rows filled at the Gauss points, E 1e5..1e7, nu 0.2..0.4, "
+                       "USL, dt=1e-5, seed 0",
+           "particles": ntot, "steps": K,
+           "batched": {"value": ntot * K / (tg / 1e3), "unit": "particle-steps/s",
+                       "ms_per_step": tg / K, "run": "HIP graph of the K steps, one replay",
+                       "graph_kernel_nodes_per_step": nodes / K},
+           "batched_eager": {"value": ntot * K / (te / 1e3), "unit": "particle-steps/s",
+                             "ms_per_step": te / K},
+           "sequential": {"value": ntot * K / (ts / 1e3), "unit": "particle-steps/s",
+                          "ms_per_step": ts / K},
+           "speedup": ts / tg,
+           "autotune_ms_per_step": {str(k): v for k, v in tuned.items()},
+           "batch_step": "one launch" if ens.step_kernel else
+                         "per phase, %d cells per workgroup" % ens.cells_per_block}
+    if not K_adj:
+        return out
+    # forward+adjoint: fresh members, per-member scalar log E
+    del ens
+    torch.cuda.empty_cache()
+    sims, g, specs = workloads.ensemble(64, seed=0)
+    ens = Ensemble(sims, per_particle_E=True)
+    ens.autotune(g)
+    d = Differentiable.of_ensemble(ens)
+    idx = ens.sim_index()
+    le0 = torch.tensor([math.log(m["E"]) for m in specs], device="cuda")
+
+    def batched(k):
+        le = le0.clone().requires_grad_(True)
+        o = rollout(d, k, g, state=d.get_state(), log_e=le[idx],
+                    checkpoint_every=checkpoint_every if k > checkpoint_every else 0)
+        o[0:2].sum().backward()
+        return le.grad
+
+    batched(4)  # warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    grad = batched(K_adj)
+    torch.cuda.synchronize()
+    tb = time.perf_counter() - t0
+    assert torch.isfinite(grad).all()
+    del d, ens
+    torch.cuda.empty_cache()
+    sims, g, specs = workloads.ensemble(64, seed=0)
+    ds = [Differentiable(s) for s in sims]
+    for dd in ds:  # warm-up
+        le = torch.tensor(math.log(dd.ps.youngs_modulus[0].item()), device="cuda",
+                          requires_grad=True)
+        rollout(dd, 2, g, log_e=le)[0:2].sum().backward()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for dd in ds:
+        le = torch.tensor(math.log(dd.ps.youngs_modulus[0].item()), device="cuda",
+                          requires_grad=True)
+        rollout(dd, K_seq_adj, g, log_e=le)[0:2].sum().backward()
+    torch.cuda.synchronize()
+    tq = time.perf_counter() - t0
+    out["fwd_adjoint"] = {
+        "batched": {"value": ntot * K_adj / tb, "unit": "particle-steps/s", "steps": K_adj,
+                    "checkpoint_every": checkpoint_every, "ms_per_step": tb * 1e3 / K_adj},
+        "sequential": {"value": ntot * K_seq_adj / tq, "unit": "particle-steps/s",
+                       "steps": K_seq_adj, "ms_per_step": tq * 1e3 / K_seq_adj},
+        "speedup": (ntot * K_adj / tb) / (ntot * K_seq_adj / tq),
+        "parameter": "d loss / d log E of each member (one scalar per member), loss = sum of "
+                     "final positions"}
+    return out
 
 
 def fwd_adjoint(sim, gravity, K):
@@ -292,7 +354,7 @@ def inverse_bench(iters, K=2000, checkpoint_every=100):
         opt.step()
         torch.cuda.synchronize()
         secs.append(time.perf_counter() - t0)
-        losses.append(float(loss))
+        losses.append(loss.item())
         errs.append(float((le.detach() - le_true).norm() / dev))
     it_s = sorted(secs)[len(secs) // 2]
     return {"workload": "inverse problem: Q4 128x128, 32768 particles (lower half, Gauss "
@@ -321,7 +383,7 @@ def main():
                     help="phase launches: update stresses inside the force P2G cell pass")
     ap.add_argument("--phase-launches", action="store_true",
                     help="one launch per phase instead of the one-launch step")
-    ap.add_argument("--ensemble-steps", type=int, default=50,
+    ap.add_argument("--ensemble-steps", type=int, default=1000,
                     help="steps of the timed 64-member uneven ensemble (0: skip)")
     ap.add_argument("--adjoint-steps", type=int, default=20,
                     help="steps of the timed forward+adjoint rollout (0: skip)")

@@ -94,7 +94,9 @@ class StepGraph:
             call("dmpm_prepare_run", plan.h, code, flags, stream)
             call("dmpm_plan_set_clock", plan.h, int(start_step), stream)
             torch.cuda.synchronize(device)
-            self.graph = torch.cuda.CUDAGraph()
+            # the captured graph is kept (not only its executable form) so
+            # that kernel_nodes() can count its nodes
+            self.graph = torch.cuda.CUDAGraph(keep_graph=True)
             with torch.cuda.graph(self.graph):
                 k = 0
                 for n, snap in chunks:
@@ -105,6 +107,26 @@ class StepGraph:
                     if snap:
                         self._snaps.enqueue(k)
                         k += 1
+            self.graph.instantiate()
+
+    def kernel_nodes(self):
+        """The number of kernel nodes of the captured graph (all nsteps steps)."""
+        import ctypes
+        hip = ctypes.CDLL("libamdhip64.so")
+        g = ctypes.c_void_p(self.graph.raw_cuda_graph())
+        n = ctypes.c_size_t(0)
+        if hip.hipGraphGetNodes(g, None, ctypes.byref(n)) != 0:
+            raise RuntimeError("hipGraphGetNodes failed")
+        nodes = (ctypes.c_void_p * n.value)()
+        if hip.hipGraphGetNodes(g, nodes, ctypes.byref(n)) != 0:
+            raise RuntimeError("hipGraphGetNodes failed")
+        kinds = ctypes.c_int(0)
+        count = 0
+        for i in range(n.value):
+            if hip.hipGraphNodeGetType(ctypes.c_void_p(nodes[i]), ctypes.byref(kinds)) != 0:
+                raise RuntimeError("hipGraphNodeGetType failed")
+            count += kinds.value == 0  # hipGraphNodeTypeKernel
+        return count
 
     def replay(self):
         """Runs the next ``nsteps`` steps (enqueued on the current stream)."""
