@@ -153,7 +153,7 @@ __device__ __forceinline__ float mixed2(int k) { return (k & 1) ? -0.25f : 0.25f
 template <typename T>
 __device__ __forceinline__ void d_apply(const Mat2& m, T E_p, const T in[4], T out[4]) {
   if (m.kind == DMPM_SIMPLE_MATERIAL) {
-    const T E = m.per_E ? E_p : T(m.de[0]);
+    const T E = T(m.de[0]);
     for (int k = 0; k < 4; ++k) out[k] = in[k] * E;
     return;
   }

@@ -61,7 +61,7 @@ enum { C_X = 0, C_Y, C_VX, C_VY, C_SXX, C_SYY, C_SXY, C_V, C_ROWS };
 struct Mat1 {
   int kind;
   int per_E;
-  double de[3];  // (a1, a2, G) or (E, -, -); at E = 1 with a per-particle E
+  double de[3];  // (a1, a2, G) or (E, -, -); per unit E with a per-particle E
 };
 
 template <typename T> struct Adj1 {
@@ -96,8 +96,8 @@ template <typename T> struct Adj1 {
 // entries the 2D state holds), as mpm_forward.hip's stress_increment.
 template <typename T>
 __device__ __forceinline__ void inc1(const Mat1& m, T E_p, const T d[4], T ds[4]) {
-  if (m.kind == DMPM_SIMPLE_MATERIAL) {
-    const T E = m.per_E ? E_p : T(m.de[0]);
+  if (m.kind == DMPM_SIMPLE_MATERIAL) {  // (a per-particle E comes as LinearElastic)
+    const T E = T(m.de[0]);
     for (int k = 0; k < 4; ++k) ds[k] = d[k] * E;
     return;
   }
@@ -116,8 +116,8 @@ __device__ __forceinline__ void inc1(const Mat1& m, T E_p, const T d[4], T ds[4]
 // D^T applied to the stress cotangent [sxx, syy, -, sxy] (D is symmetric).
 template <typename T>
 __device__ __forceinline__ void dT1(const Mat1& m, T E_p, const T sb[4], T out[4]) {
-  if (m.kind == DMPM_SIMPLE_MATERIAL) {
-    const T E = m.per_E ? E_p : T(m.de[0]);
+  if (m.kind == DMPM_SIMPLE_MATERIAL) {  // (a per-particle E comes as LinearElastic)
+    const T E = T(m.de[0]);
     for (int k = 0; k < 4; ++k) out[k] = sb[k] * E;
     return;
   }

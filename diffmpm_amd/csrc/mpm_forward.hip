@@ -351,9 +351,8 @@ template <typename T>
 __device__ __forceinline__ void stress_increment(const Mat& m, const T* __restrict__ pb,
                                                  int64_t ld, int p, const T d[6], T ds[6]) {
   if (m.kind == DMPM_SIMPLE_MATERIAL) {
-    // material.py:130-131; a per-particle E replaces the material's
-    T E = m.per_E ? PROW(DMPM_P_YOUNGS)[p] : T(m.de[0]);
-    for (int k = 0; k < 6; ++k) ds[k] = d[k] * E;
+    T E = T(m.de[0]);
+    for (int k = 0; k < 6; ++k) ds[k] = d[k] * E;  // material.py:130-131
     return;
   }
   T a1 = T(m.de[0]), a2 = T(m.de[1]), G = T(m.de[2]);
@@ -1033,6 +1032,10 @@ int dmpm_plan_create(const dmpm_plan_desc* desc, dmpm_plan** out) {
     return fail(DMPM_EINVAL, "invalid particle count or row stride");
   if (d.material != DMPM_LINEAR_ELASTIC && d.material != DMPM_SIMPLE_MATERIAL)
     return fail(DMPM_EINVAL, "unknown material");
+  if (d.material == DMPM_SIMPLE_MATERIAL && d.per_particle_E)
+    return fail(DMPM_EINVAL,
+                "per-particle E takes DMPM_LINEAR_ELASTIC coefficients: de = (1, 0, 1) gives "
+                "SimpleMaterial's dsigma = E_p dstrain");
   dmpm_plan* P = new dmpm_plan();
   P->d = d;
   P->n = d.nparticles;

@@ -40,7 +40,7 @@ template <typename T> struct Grid2 {
 template <typename T>
 __device__ __forceinline__ void stress_inc(const Mat2& m, T E_p, const T d[6], T ds[4]) {
   if (m.kind == DMPM_SIMPLE_MATERIAL) {
-    const T E = m.per_E ? E_p : T(m.de[0]);
+    T E = T(m.de[0]);
     for (int k = 0; k < 4; ++k) ds[k] = d[k] * E;
     return;
   }

@@ -106,8 +106,10 @@ class SimpleMaterial(Material):
         return torch.as_tensor(dstrain) * self.properties["E"]
 
     def _kernel_coefficients(self, per_particle_E=False):
-        # with a per-particle modulus (Particles.youngs_modulus) the kernels
-        # take E from the particle row instead of the material's E
+        # with a per-particle modulus (Particles.youngs_modulus), dsigma =
+        # E_p dstrain is the LinearElastic update with a1 = 1, a2 = 0, G = 1
+        # scaled by E_p: (E_p d0 + 0 d1) + 0 d2 = E_p d0, E_p d1, E_p d3
+        # (equal to E_p d_k up to the sign of a zero)
         if per_particle_E:
-            return _lib.DMPM_SIMPLE_MATERIAL, (1.0, 0.0, 0.0)
+            return _lib.DMPM_LINEAR_ELASTIC, (1.0, 0.0, 1.0)
         return _lib.DMPM_SIMPLE_MATERIAL, (float(self.properties["E"]), 0.0, 0.0)

@@ -107,7 +107,10 @@ typedef struct dmpm_plan_desc {
   int32_t per_particle_E; /* 1: E_p from row DMPM_P_YOUNGS (LinearElastic only) */
   /* LinearElastic: De coefficients (a1, a2, G) computed by the caller in
    * double as material.py:60-111 does; with per_particle_E they are the
-   * coefficients per unit E.  SimpleMaterial: de[0] = E (material.py:130). */
+   * coefficients per unit E.  SimpleMaterial: de[0] = E (material.py:130);
+   * a SimpleMaterial with per-particle E is DMPM_LINEAR_ELASTIC with
+   * per_particle_E and de = (1, 0, 1) (dsigma = E_p dstrain; the plan
+   * rejects DMPM_SIMPLE_MATERIAL with per_particle_E). */
   double de[3];
 } dmpm_plan_desc;
 
