@@ -27,14 +27,19 @@ Pinning (how each function is checked, see ``tests/test_oracle_*.py``):
 * closed forms and invariants: partition of unity, gradient sums, P2G mass and
   momentum conservation, exact strain rates for linear velocity fields,
   De algebra, ``np.interp`` semantics, brute-force element search on tiny
-  meshes, FP32 vs FP64 agreement.
+  meshes, FP32 vs FP64 agreement;
+* linear completeness, which fixes the natural coordinates and G2P of both
+  elements (``TestLinear1D`` in the reference is empty): sum_i X_i N_i(xi_p)
+  = x_p, G2P of a linear nodal velocity gives a x_p exactly, the P2G first
+  moments sum_i X_i m_i = sum_p m_p x_p, and the internal-force virial; a
+  planted mistake in the 1D natural coordinate (negated, scaled, centred on
+  node 0) fails them;
+* the per-particle E of SimpleMaterial (an extension): equal to the scalar
+  material with that E.
 
 Parity unpinned (no reference test covers them and the reference cannot be
 imported here — JAX is absent):
 
-* the whole Linear1D path (``TestLinear1D`` in the reference is empty), checked
-  only by closed forms (linear-field strain rate, 1D mass/momentum
-  conservation) and its documented quirks;
 * element search for points outside the mesh (wrap / element-0 / out-of-range
   ids), checked by brute force against the reference's set definition only;
 * the exact FP32 rounding of XLA fusions (FMA contraction, LU inverse), pinned

@@ -102,6 +102,70 @@ def test_c1_bar_per_step_fp64():
     assert steps >= 1
 
 
+def _c1_stable(nsteps):
+    """The C1 bar at a stable time step, with room to move: dt = 2e-4 (c dt /
+    h = 0.256 with c = sqrt(E / rho) = 10, h = 1/128) instead of 1e-2, on a
+    mesh of 160 elements of the same length (the bar's 256 particles on the
+    first 128).  As configured, the bar's last particle (1.65e-3 from the
+    mesh end, v = 0.1) leaves the mesh at t = 0.017 whatever the time step
+    (step 1 at dt = 1e-2, step 83 at dt = 2e-4)."""
+    spec = cases.config_c1_spec(nsteps=nsteps)
+    spec["dt"] = 2e-4
+    spec["nelements"] = 160
+    return spec
+
+
+def test_c1_bar_at_a_stable_time_step_per_step_fp32():
+    """1000 USF steps of the C1 bar at dt = 2e-4, each from the FP32 oracle's
+    state: every particle and nodal field of the GPU step meets the FP32
+    criterion at 1e-5 (within 1e-5 of the FP32 or the FP64 oracle, or no
+    further from the FP64 oracle than the FP32 oracle is: the interior
+    internal forces of the bar cancel and amplify FP32 rounding) at every
+    step, and the particles stay on the mesh."""
+    spec = _c1_stable(1000)
+    o32 = cases.build_oracle(spec, np.float32)
+    o64 = cases.build_oracle(spec, np.float64)
+    gsim = cases.build_gpu(spec, torch.float32)
+    g = torch.tensor(spec["gravity"], dtype=torch.float64)
+    worst = 0.0
+    for i in range(spec["nsteps"]):
+        cases.load_oracle_state_into_gpu(o32, gsim)
+        cases.copy_oracle_state(o32, o64)
+        o32.step(i, np.asarray(spec["gravity"]))
+        o64.step(i, np.asarray(spec["gravity"]))
+        x = o32.mesh.particles[0].loc
+        assert np.all(np.isfinite(x)) and x.min() >= 0 and x.max() < 1.25, i
+        gsim._step(i, g)
+        pairs = [(cases.gpu_particle_field(gsim.mesh.particles[0], f),
+                  cases.oracle_particle_field(o32.mesh.particles[0], f),
+                  cases.oracle_particle_field(o64.mesh.particles[0], f), f)
+                 for f in ("loc", "velocity", "stress", "strain", "volume", "density")]
+        pairs += [(cases.gpu_node_field(gsim.mesh.elements.nodes, f),
+                   cases.oracle_node_field(o32.mesh.elements.nodes, f),
+                   cases.oracle_node_field(o64.mesh.elements.nodes, f), "node " + f)
+                  for f in cases.NODE_FIELDS]
+        for got, w32, w64, name in pairs:
+            ok, e32, e64, eo = cases.fp32_within_reference_precision(got, w32, w64)
+            assert ok, (i, name, e32, e64, eo)
+            worst = max(worst, e32)
+    print(f"C1 at dt = 2e-4: 1000 steps, worst normwise error to the FP32 oracle {worst:.2e}")
+
+
+def test_c1_bar_at_a_stable_time_step_rollout_fp64():
+    """The same 1000 steps in FP64 without re-synchronisation: the GPU
+    rollout matches the FP64 oracle's within 1e-10 normwise."""
+    spec = _c1_stable(1000)
+    osim = cases.build_oracle(spec, np.float64)
+    gsim = cases.build_gpu(spec, torch.float64, keep_temporaries=False)
+    osim.solve(np.asarray(spec["gravity"]))
+    gsim.solve_jit(torch.tensor(spec["gravity"], dtype=torch.float64))
+    torch.cuda.synchronize()
+    ops, gps = osim.mesh.particles[0], gsim.mesh.particles[0]
+    for f in ("loc", "velocity", "stress", "strain", "volume"):
+        err = cases.normwise(cases.gpu_particle_field(gps, f), cases.oracle_particle_field(ops, f))
+        assert err < 1e-10, (f, err)
+
+
 @pytest.mark.parametrize("fused", [False, True])
 def test_c2_usl_top_force_per_step(fused):
     assert _per_step(cases.config_c2_spec(nsteps=3), torch.float32, fused) == 3
