@@ -363,12 +363,13 @@ def gpu_state_into_oracle(gsim, osim):
         setattr(on, name, v.astype(np.asarray(cur).dtype))
 
 
-def fp32_within_reference_precision(got, want32, want64, tol=1e-5):
+def fp32_within_reference_precision(got, want32, want64, tol=1e-5, factor=1.0):
     """FP32 parity criterion for quantities whose FP32 rounding the reference
     itself amplifies (gradients of nodal velocities on fine meshes, sums over
     thousands of particles): the HIP result agrees with the FP32 oracle within
     tol, or it is within tol of the FP64 oracle, or no further from the FP64
-    oracle than the FP32 oracle (the reference's own arithmetic) is.
-    Returns (ok, err vs FP32 oracle, err vs FP64 oracle, FP32 oracle vs FP64)."""
+    oracle than factor times the FP32 oracle (the reference's own arithmetic)
+    is.  Returns (ok, err vs FP32 oracle, err vs FP64 oracle, FP32 oracle vs
+    FP64)."""
     e32, e64, eo = normwise(got, want32), normwise(got, want64), normwise(want32, want64)
-    return (e32 <= tol or e64 <= tol or e64 <= eo), e32, e64, eo
+    return (e32 <= tol or e64 <= tol or e64 <= factor * eo), e32, e64, eo
