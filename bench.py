@@ -142,14 +142,14 @@ def cpu_baseline(rows=64, np_steps=12, torch_steps=80):
                                       f"({np_s:.1f} s), same strip"}}
 
 
-def ensemble_bench(K, K_adj=1000, K_seq_adj=50, checkpoint_every=100):
+def ensemble_bench(K, K_adj=None, K_seq_adj=50, checkpoint_every=100):
     """64 uneven USL members (32..256 cells per side, 30-100 % of the rows
     filled, particles at the Gauss points).  Forward: K steps with one batch
     plan (the autotuner's choice of step kernels, the K steps captured into
     one HIP graph and replayed; the eager run of the same plan beside it),
     against the same K steps of every member with its own plan, one member
     after the other.  Forward+adjoint w.r.t. each member's scalar E (one log
-    E per member, mapped to its particles): K_adj batched steps with the
+    E per member, mapped to its particles): K_adj (default K) batched steps with the
     adjoint checkpointed every checkpoint_every steps, against K_seq_adj
     steps of every member's own differentiable rollout, one after the other
     (per-step rates; the sequential rollouts are bound by their per-step host
@@ -202,6 +202,8 @@ def ensemble_bench(K, K_adj=1000, K_seq_adj=50, checkpoint_every=100):
            "autotune_ms_per_step": {str(k): v for k, v in tuned.items()},
            "batch_step": "one launch" if ens.step_kernel else
                          "per phase, %d cells per workgroup" % ens.cells_per_block}
+    K_adj = K if K_adj is None else K_adj
+    K_seq_adj = min(K_seq_adj, K_adj)
     if not K_adj:
         return out
     # forward+adjoint: fresh members, per-member scalar log E

@@ -1324,12 +1324,16 @@ __global__ __launch_bounds__(kFT) __attribute__((amdgpu_waves_per_eu(4))) void k
       syy[j] = SR(DMPM_P_STRESS + 1, p1);
       sxy[j] = SR(DMPM_P_STRESS + 3, p1);
       V[j] = SR(DMPM_P_VOLUME, p1);
-      szz[j] = SR(DMPM_P_STRESS + 2, po);
-      ex[j] = SR(DMPM_P_STRAIN + 0, po);
-      ey[j] = SR(DMPM_P_STRAIN + 1, po);
-      gxy[j] = SR(DMPM_P_STRAIN + 3, po);
-      rho[j] = SR(DMPM_P_DENSITY, po);
-      evc[j] = SR(DMPM_P_VOLSTRAIN_C, po);
+      if constexpr (USF) {
+        // (USL loads these in its G phase, where it uses them: held from
+        // here, they spill)
+        szz[j] = SR(DMPM_P_STRESS + 2, po);
+        ex[j] = SR(DMPM_P_STRAIN + 0, po);
+        ey[j] = SR(DMPM_P_STRAIN + 1, po);
+        gxy[j] = SR(DMPM_P_STRAIN + 3, po);
+        rho[j] = SR(DMPM_P_DENSITY, po);
+        evc[j] = SR(DMPM_P_VOLSTRAIN_C, po);
+      }
     }
     if (a.mat.per_E)
 #pragma unroll
@@ -1475,6 +1479,18 @@ __global__ __launch_bounds__(kFT) __attribute__((amdgpu_waves_per_eu(4))) void k
   }
   __syncthreads();
   // ---- G ----
+  if constexpr (!USF) {
+#pragma unroll
+    for (int j = 0; j < PPT; ++j) {
+      const int po = (own & (1u << j)) ? pp[j] : 0;
+      szz[j] = SR(DMPM_P_STRESS + 2, po);
+      ex[j] = SR(DMPM_P_STRAIN + 0, po);
+      ey[j] = SR(DMPM_P_STRAIN + 1, po);
+      gxy[j] = SR(DMPM_P_STRAIN + 3, po);
+      rho[j] = SR(DMPM_P_DENSITY, po);
+      evc[j] = SR(DMPM_P_VOLSTRAIN_C, po);
+    }
+  }
 #pragma unroll
   for (int j = 0; j < PPT; ++j) {
     if (!(own & (1u << j))) continue;
