@@ -1449,12 +1449,14 @@ template <typename T, int DIM> struct PhaseStress {
                                S_STRAIN | S_VOLUME | S_STRESS | (temps ? S_TEMPS : 0), s);
   }
 };
+// sorted: the caller sorted by these element ids in this step's kinematics
+// phase (no element id changes in between), so no re-check is enqueued.
 template <typename T, int DIM> struct PhaseForces {
   static int run(dmpm_plan* P, void* pbuf, const int32_t* eid, void* nbuf, const double* grav,
-                 int64_t step, double dt, int pfext, hipStream_t s) {
+                 int64_t step, double dt, int pfext, hipStream_t s, bool sorted = false) {
     using O = Ops<T, DIM>;
     T t = O::step_time(step, dt);
-    DMPM_TRY(ensure_sorted(P, eid, s));
+    if (!sorted) DMPM_TRY(ensure_sorted(P, eid, s));
     if (P->has_tractions) DMPM_TRY(O::tractions(P, pbuf, t, s));
     int mode = F_EXT_ZERO | F_BODY | F_INT | (pfext ? F_PFEXT : 0) |
                (P->has_tractions ? F_TRAC : 0);
@@ -1750,7 +1752,7 @@ template <typename T, int DIM> struct RunSteps {
     for (int64_t i = step0; i < step0 + nsteps; ++i) {
       DMPM_TRY((PhaseKin<T, DIM>::run(P, pbuf, eid, nbuf, s)));
       if (scheme == DMPM_USF) DMPM_TRY((PhaseStress<T, DIM>::run(P, pbuf, eid, nbuf, dt, temps, s)));
-      DMPM_TRY((PhaseForces<T, DIM>::run(P, pbuf, eid, nbuf, grav, i, dt, pfext, s)));
+      DMPM_TRY((PhaseForces<T, DIM>::run(P, pbuf, eid, nbuf, grav, i, dt, pfext, s, true)));
       DMPM_TRY((PhasePkin<T, DIM>::run(P, pbuf, eid, nbuf, dt, vu, temps, s)));
       if (scheme == DMPM_USL) DMPM_TRY((PhaseStress<T, DIM>::run(P, pbuf, eid, nbuf, dt, temps, s)));
     }
@@ -1772,7 +1774,7 @@ template <typename T, int DIM> struct StepKeepVkin {
                                         hipMemcpyDeviceToDevice, s),
                        "hipMemcpy2DAsync"));
     if (scheme == DMPM_USF) DMPM_TRY((PhaseStress<T, DIM>::run(P, pbuf, eid, nbuf, dt, 0, s)));
-    DMPM_TRY((PhaseForces<T, DIM>::run(P, pbuf, eid, nbuf, grav, step, dt, 0, s)));
+    DMPM_TRY((PhaseForces<T, DIM>::run(P, pbuf, eid, nbuf, grav, step, dt, 0, s, true)));
     DMPM_TRY((PhasePkin<T, DIM>::run(P, pbuf, eid, nbuf, dt, vu, 0, s)));
     if (scheme == DMPM_USL) DMPM_TRY((PhaseStress<T, DIM>::run(P, pbuf, eid, nbuf, dt, 0, s)));
     return DMPM_OK;
