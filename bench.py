@@ -371,6 +371,29 @@ def inverse_bench(iters, K=2000, checkpoint_every=100):
                     "median time over the iterations"}
 
 
+def copy_ceiling(nbytes=1 << 30, reps=10):
+    """STREAM-copy ceiling of this GPU: a device-to-device copy of nbytes,
+    timed with HIP events over reps copies after a warm-up; GB/s of read
+    plus written bytes."""
+    import torch
+    a = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    a.fill_(1.0)
+    b.copy_(a)
+    stream = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record(stream)
+    for _ in range(reps):
+        b.copy_(a)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    del a, b
+    torch.cuda.empty_cache()
+    return 2 * nbytes / (ms / 1e3) / 1e9
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -477,6 +500,9 @@ def main():
                      "bytes_per_particle": kd["bytes_per_particle"], "launches": kd["launches"]},
         "kernels": kernels,
     }
+    ceil = copy_ceiling()
+    line["roofline"]["copy_ceiling_GBs"] = ceil
+    line["roofline"]["frac_of_copy_ceiling"] = achieved / ceil
     if args.adjoint_steps:
         line["fwd_adjoint"] = fwd_adjoint(sim, gravity, args.adjoint_steps)
     if args.ensemble_steps:
