@@ -231,3 +231,56 @@ def test_bar256_checkpointed_gradients_equal_plain_ones():
         grads.append((s0.grad.clone(), le.grad.clone()))
     assert torch.equal(grads[0][0], grads[1][0])
     assert torch.equal(grads[0][1], grads[1][1])
+
+
+# ---------------------------------------------------------------- error paths
+def test_differentiable_rejects_particle_tractions_and_external_forces():
+    import cases
+    from diffmpm_amd.autodiff import Differentiable
+    spec = _bar256("usl", "SimpleMaterial", False)
+    sim = cases.build_gpu(spec, torch.float64, keep_temporaries=False)
+    ps = sim.mesh.particles[0]
+    ps.f_ext = torch.ones((len(ps), 1, 1), dtype=torch.float64, device="cuda")
+    with pytest.raises(ValueError, match="particle external forces"):
+        Differentiable(sim)
+    spec["tractions"] = [{"pset": [0], "pids": [3], "dir": 0, "traction": 1.0,
+                          "function": None}]
+    with pytest.raises(ValueError, match="particle tractions"):
+        Differentiable(cases.build_gpu(spec, torch.float64, keep_temporaries=False))
+
+
+def test_1d_adjoint_entry_point_rejects_particle_tractions():
+    """dmpm_step_adjoint on a Linear1D plan with particle tractions: EINVAL
+    before any work."""
+    import cases
+    from diffmpm_amd import _lib
+    from diffmpm_amd._plan import Plan
+    spec = _bar256("usl", "SimpleMaterial", False)
+    spec["tractions"] = [{"pset": [0], "pids": [3], "dir": 0, "traction": 1.0,
+                          "function": None}]
+    sim = cases.build_gpu(spec, torch.float64, keep_temporaries=False)
+    ps, el = sim.mesh.particles[0], sim.mesh.elements
+    plan = Plan(el, ps)
+    pw, nw, eid = ps._buf.clone(), el.nodes._buf.clone(), ps._eid.clone()
+    cot = torch.zeros((8, ps._buf.shape[1]), dtype=torch.float64, device="cuda")
+    with pytest.raises(ValueError, match="particle tractions"):
+        _lib.call("dmpm_step_adjoint", plan.h, _lib.ptr(ps._buf), _lib.ptr(pw), _lib.ptr(eid),
+                  _lib.ptr(nw), _lib.ptr(cot), None, _lib.DMPM_USL, 0, plan._gravity([0.0]),
+                  2e-4, 0, plan.s)
+
+
+def test_plan_rejects_simple_material_with_a_per_particle_modulus():
+    """A SimpleMaterial with a per-particle E is passed as LinearElastic
+    coefficients (1, 0, 1); the raw pair is refused with that advice."""
+    from ctypes import byref, c_void_p
+    from diffmpm_amd import _lib
+    d = _lib.PlanDesc()
+    d.element, d.dtype = _lib.DMPM_LINEAR1D, _lib.dtype_code(torch.float32)
+    d.nx, d.ny, d.hx, d.hy = 4, 1, 0.25, 0.25
+    d.nparticles, d.ld, d.nld = 8, 8, 5
+    d.material, d.per_particle_E = _lib.DMPM_SIMPLE_MATERIAL, 1
+    for k, v in enumerate((1.0, 0.0, 0.0)):
+        d.de[k] = v
+    h = c_void_p()
+    with pytest.raises(ValueError, match=r"de = \(1, 0, 1\)"):
+        _lib.call("dmpm_plan_create", byref(d), byref(h))
