@@ -408,6 +408,9 @@ def main():
                     help="phase launches: update stresses inside the force P2G cell pass")
     ap.add_argument("--phase-launches", action="store_true",
                     help="one launch per phase instead of the one-launch step")
+    ap.add_argument("--step-kernel", choices=["one", "split"], default="one",
+                    help="the step's kernels: one launch per step, or the split step "
+                         "(three launches over node and cell tiles)")
     ap.add_argument("--ensemble-steps", type=int, default=1000,
                     help="steps of the timed 64-member uneven ensemble (0: skip)")
     ap.add_argument("--adjoint-steps", type=int, default=20,
@@ -439,6 +442,8 @@ def main():
         plan.set_option(_lib.DMPM_OPT_USF_SPLIT, 0)
     if args.phase_launches or args.usf_fused:
         plan.set_option(_lib.DMPM_OPT_STEP_KERNEL, 0)
+    elif args.step_kernel == "split":
+        plan.set_option(_lib.DMPM_OPT_STEP_KERNEL, 2)
     print(f"# workload built: {N} particles, {args.n}x{args.n} cells", file=sys.stderr, flush=True)
     plan.run("usf", False, gravity, sim.dt, 0, args.warmup, keep_temporaries=False)
     torch.cuda.synchronize()

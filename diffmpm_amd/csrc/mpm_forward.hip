@@ -1308,7 +1308,8 @@ int dmpm_plan_set_option(dmpm_plan* P, int32_t option, int32_t value) {
       P->usf_split = value ? 1 : 0;
       return DMPM_OK;
     case DMPM_OPT_STEP_KERNEL:
-      P->step_kernel = value ? 1 : 0;
+      if (value < 0 || value > 2) return fail(DMPM_EINVAL, "step kernel must be 0, 1 or 2");
+      P->step_kernel = value;
       return DMPM_OK;
     default:
       return fail(DMPM_EINVAL, "unknown option");

@@ -1917,6 +1917,749 @@ __global__ __launch_bounds__(kFT) void k_step_rounds(StepArgs<T> a) {
   }
 }
 
+// ---- split step: three launches per step --------------------------------------
+//
+// The step of k_step_fast in three launches.  No workgroup holds a tile's
+// whole particle state, and only the column and the row of cells left of and
+// below a tile are recomputed (1.13x the tile's particles in K, S and F,
+// against 1.56x in K and 1.27x in S and F with the two-ring halo):
+//   k_split_kin    node tile: P2G of mass and momentum over the cells of the
+//                  tile's nodes; nodal mass and kinematic velocity of the
+//                  nodes the tile owns
+//   k_split_force  node tile: USF stress update of the particles of the same
+//                  cells (the tile's own cells' results written), P2G of body
+//                  and internal forces, nodal update of the owned nodes
+//   k_split_g2p    cell tile: G2P of the tile's particles (USL: then their
+//                  stress update) with the cell changes recorded for the next
+//                  re-sort; kTailBlocks more workgroups handle the particles
+//                  outside the mesh
+// Tile (ti, tj) owns the cells R0 = [16 ti, 16 ti + 16) x [16 tj, 16 tj + 16)
+// (clipped to the mesh) and the nodes (i, j) with i in [R0.i0, R0.i1), plus
+// i = nx in the last tile column, and likewise j.  The cells of those nodes
+// are RC = R0 grown by one cell to the left and below.  Every launch reads
+// the particle state from src and writes dst, so halo reads see the step's
+// input.  Slots are staged in rounds of NT * RPT (one round unless a tile is
+// crowded).  In each round the threads that own the region's cells write
+// every slot's cell into an LDS table, so a slot finds its cell with one LDS
+// read instead of a search (the re-sort keyed the slot by the element search
+// of its position, so this is the cell the search gives).  Cell sums run in
+// slot order and node sums in node_gather's order with the expressions of
+// the other paths, so the results are bit-identical to theirs.
+
+// Half-open node region of tile R0: the nodes the tile owns.
+__device__ __forceinline__ Rgn owned_nodes(const Rgn& R0, int nx, int ny) {
+  return Rgn{R0.i0, R0.i1 + (R0.i1 == nx ? 1 : 0), R0.j0, R0.j1 + (R0.j1 == ny ? 1 : 0)};
+}
+
+// Slot segments of the rows of region R, computed by the calling wave (lane r:
+// row r): A[r] = first slot of row r's cells, P[r] = slots in the rows before
+// r (P[rows] = all).
+__device__ __forceinline__ void row_segments(const Rgn& R, int nx, int coff, const int* cs,
+                                             int* A, int* P) {
+  const int r = threadIdx.x & 63;
+  int len = 0;
+  if (r < R.j1 - R.j0) {
+    const int k = coff + (R.j0 + r) * nx + R.i0;
+    const int s0 = cs[k];
+    A[r] = s0;
+    len = cs[k + (R.i1 - R.i0)] - s0;
+  }
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int y = __shfl_up(len, o, 64);
+    if (r >= o) len += y;
+  }
+  if (r < R.j1 - R.j0) P[r + 1] = len;
+  if (r == 0) P[0] = 0;
+}
+
+// Cell centres and scales 2/(X1 - X0) of the columns and rows of region R
+// (natural coordinates and inverse Jacobians, particle.py:180-206), by the
+// calling wave.
+template <typename T>
+__device__ __forceinline__ void cell_tables(const Grid2<T>& g, const Rgn& R, T* colC, T* colS,
+                                            T* rowC, T* rowS) {
+  const int r = threadIdx.x & 63;
+  if (r < R.i1 - R.i0) {
+    const T X0 = g.X(R.i0 + r), X1 = g.X(R.i0 + r + 1);
+    colC[r] = (X0 + X1) / T(2);
+    colS[r] = T(2) / (X1 - X0);
+  }
+  if (r < R.j1 - R.j0) {
+    const T Y0 = g.Y(R.j0 + r), Y1 = g.Y(R.j0 + r + 1);
+    rowC[r] = (Y0 + Y1) / T(2);
+    rowS[r] = T(2) / (Y1 - Y0);
+  }
+}
+
+// Virtual slot range [v0, v1) of those cells from the row segments A, P.
+template <int C>
+__device__ __forceinline__ void cell_ranges_of(const int (&r)[C], const int (&s0)[C],
+                                               const int (&s1)[C], const int* A, const int* P,
+                                               int (&v0)[C], int (&v1)[C]) {
+#pragma unroll
+  for (int u = 0; u < C; ++u) {
+    v0[u] = v1[u] = 0;
+    if (r[u] >= 0) {
+      v0[u] = P[r[u]] + (s0[u] - A[r[u]]);
+      v1[u] = v0[u] + (s1[u] - s0[u]);
+    }
+  }
+}
+
+// Node sums (node_gather's order) of the nodes of the half-open node region
+// NR from the cell partials of region RC (which holds every mesh cell of those
+// nodes), through LDS buffer sl (4 * NQ * cells of RC).  Called by every
+// thread; ends with a barrier.
+template <typename T, int NQ, int C, int NPT, int NT>
+__device__ __forceinline__ void node_sums_nt(T* sl, const T (&acc)[C][4 * NQ], const Rgn& RC,
+                                             const Rgn& NR, int nx, int ny,
+                                             T (&nacc)[NPT][NQ]) {
+  const int t = threadIdx.x;
+  const int wc = RC.i1 - RC.i0, nc = wc * (RC.j1 - RC.j0);
+  const int wn = NR.i1 - NR.i0, nn = wn * (NR.j1 - NR.j0);
+  const int dI[4] = {-1, 0, -1, 0}, dJ[4] = {-1, -1, 0, 0}, corner[4] = {2, 3, 1, 0};
+  const SmallDiv dwn = small_div(wn);
+#pragma unroll
+  for (int u = 0; u < C; ++u) {
+    const int lc = t + u * NT;
+    if (lc < nc)
+#pragma unroll
+      for (int v = 0; v < 4 * NQ; ++v) sl[v * nc + lc] = acc[u][v];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int w = 0; w < NPT; ++w) {
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) nacc[w][q] = T(0);
+    const int ln = t + w * NT;
+    if (ln >= nn) continue;
+    const int jr = sdiv(dwn, ln);
+    const int i = NR.i0 + (ln - jr * wn), j = NR.j0 + jr;
+    T pv[4][NQ];
+    bool in[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      const int ci = i + dI[c], cj = j + dJ[c];
+      in[c] = ci >= 0 && ci < nx && cj >= 0 && cj < ny;
+      const int lc = in[c] ? (ci - RC.i0) + (cj - RC.j0) * wc : 0;
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) pv[c][q] = sl[(NQ * corner[c] + q) * nc + lc];
+    }
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+      if (in[c])
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) nacc[w][q] = nacc[w][q] + pv[c][q];
+  }
+  __syncthreads();
+}
+
+// Node-table values (constraint row, damping force) of this thread's nodes of
+// the half-open node region NR (w-th node: index t + w * NT).
+template <typename T, int NPT, int NT>
+__device__ __forceinline__ void node_pre_nt(const StepArgs<T>& a, const TileCtx<T>& c,
+                                            const Rgn& NR, NodePre<NPT>& q, T (*fd)[2],
+                                            T* mass = nullptr) {
+  const int t = threadIdx.x, nx = c.g.nx;
+  const int wn = NR.i1 - NR.i0, nn = wn * (NR.j1 - NR.j0);
+  const SmallDiv dwn = small_div(wn);
+#pragma unroll
+  for (int w = 0; w < NPT; ++w) {
+    const int ln = min(t + w * NT, nn - 1);
+    const int jr = sdiv(dwn, ln);
+    const int nd = c.noff + (NR.j0 + jr) * (nx + 1) + NR.i0 + (ln - jr * wn);
+    q.cr[w] = a.cidx ? a.cidx[nd] : -1;
+    if (fd) {
+      fd[w][0] = NB(DMPM_N_FDAMP, nd);
+      fd[w][1] = NB(DMPM_N_FDAMP + 1, nd);
+    }
+    if (mass) mass[w] = NB(DMPM_N_MASS, nd);
+  }
+#pragma unroll
+  for (int w = 0; w < NPT; ++w) q.mask[w] = q.cr[w] >= 0 ? a.cmask[q.cr[w]] : 0;
+}
+
+// Row (in R; -1: none), column and slot range [s0, s1) of each of this
+// thread's cells of region R (cell lc = t + u * NT, row-major), loaded ahead
+// of the row segments.
+template <int C, int NT>
+__device__ __forceinline__ void cell_slots_rc(const Rgn& R, int nx, int coff, const int* cs,
+                                              int (&r)[C], int (&col)[C], int (&s0)[C],
+                                              int (&s1)[C]) {
+  const int w = R.i1 - R.i0, nc = w * (R.j1 - R.j0);
+  const SmallDiv dw = small_div(w);
+#pragma unroll
+  for (int u = 0; u < C; ++u) {
+    const int lc = threadIdx.x + u * NT;
+    r[u] = -1;
+    col[u] = 0;
+    s0[u] = s1[u] = 0;
+    if (lc < nc) {
+      r[u] = sdiv(dw, lc);
+      col[u] = lc - r[u] * w;
+      const int k = coff + (R.j0 + r[u]) * nx + R.i0 + col[u];
+      s0[u] = cs[k];
+      s1[u] = cs[k + 1];
+    }
+  }
+}
+
+// Slot table of the round [v0, vend) of a region's slot list: entry q (slot
+// v0 + q) holds (row << 8) | column of its cell, written by the threads that
+// own the cells (virtual ranges [cv0, cv1)).
+template <int C>
+__device__ __forceinline__ void fill_slot_cells(uint16_t* tab, const int (&r)[C],
+                                                const int (&col)[C], const int (&cv0)[C],
+                                                const int (&cv1)[C], int v0, int vend) {
+#pragma unroll
+  for (int u = 0; u < C; ++u) {
+    if (r[u] < 0) continue;
+    const uint16_t code = (uint16_t)((r[u] << 8) | col[u]);
+    const int q1 = min(cv1[u], vend) - v0;
+    for (int q = max(cv0[u], v0) - v0; q < q1; ++q) tab[q] = code;
+  }
+}
+
+// Workgroup size and slots per thread and round of the split kernels.
+constexpr int kSKinT = 512, kSKinR = 3;
+constexpr int kSForT = 512, kSForR = 3;
+constexpr int kSG2pT = 256, kSG2pR = 5;
+constexpr int kSCells = (kTW + 1) * (kTH + 1);  // cells of RC (at most)
+constexpr int kSNodes = (kTW + 1) * (kTH + 1);  // owned nodes (at most)
+constexpr int kSVN = (kTW + 2) * (kTH + 2);     // nodes of RC's cells (at most)
+
+// Kinematic launch: P2G of mass and momentum over RC (k_cell_kin's
+// expressions), then nodal mass, momentum and velocity of the owned nodes
+// (k_node_kin's), written to the node block's mass row and to vkin.
+template <typename T, bool B, int NT, int RPT, int WPE = 1>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) void k_split_kin(
+    StepArgs<T> a) {
+  constexpr int CAP = NT * RPT;
+  constexpr int CC = (kSCells + NT - 1) / NT;
+  constexpr int NPT = (kSNodes + NT - 1) / NT;
+  static_assert(4 * NQ_KIN * kSCells <= 5 * CAP, "node-sum slices");
+  static_assert(NT >= 128, "two waves for the tables");
+  __shared__ T st[5 * CAP];  // xi, eta, m, m vx, m vy; the node sums alias it
+  __shared__ uint16_t tab[CAP];
+  __shared__ T colC[kTW + 1], colS[kTW + 1], rowC[kTH + 1], rowS[kTH + 1];
+  __shared__ int segA[kTH + 1], segP[kTH + 2];
+  const TileCtx<T> c = tile_ctx<T, B>(a, xcd_pos(blockIdx.x, a.ntiles), true);
+  const Grid2<T> g = c.g;
+  const int nx = g.nx, ny = g.ny, t = threadIdx.x;
+  const int* __restrict__ cs = a.sr.cs();
+  const int* __restrict__ perm = a.sr.perm();
+  const int ident = a.sr.flags[1];
+  const int ti = c.ti, tj = c.tj;
+  const Rgn R0{ti * kTW, min(ti * kTW + kTW, nx), tj * kTH, min(tj * kTH + kTH, ny)};
+  const Rgn RC{max(R0.i0 - 1, 0), R0.i1, max(R0.j0 - 1, 0), R0.j1};
+  const Rgn NR = owned_nodes(R0, nx, ny);
+  const int hC = RC.j1 - RC.j0;
+  int cr_[CC], ccol[CC], cs0[CC], cs1[CC];
+  cell_slots_rc<CC, NT>(RC, nx, c.coff, cs, cr_, ccol, cs0, cs1);
+  if (t < 64) row_segments(RC, nx, c.coff, cs, segA, segP);
+  else if (t < 128) cell_tables(g, RC, colC, colS, rowC, rowS);
+  NodePre<NPT> pre;
+  node_pre_nt<T, NPT, NT>(a, c, NR, pre, nullptr);
+  __syncthreads();
+  const int S = segP[hC];
+  int cv0[CC], cv1[CC];
+  cell_ranges_of<CC>(cr_, cs0, cs1, segA, segP, cv0, cv1);
+  T acc[CC][4 * NQ_KIN];
+#pragma unroll
+  for (int u = 0; u < CC; ++u)
+#pragma unroll
+    for (int v = 0; v < 4 * NQ_KIN; ++v) acc[u][v] = T(0);
+  for (int v0 = 0; v0 < S; v0 += CAP) {
+    const int vend = min(v0 + CAP, S);
+    fill_slot_cells<CC>(tab, cr_, ccol, cv0, cv1, v0, vend);
+    __syncthreads();
+    int pp[RPT], code[RPT];
+#pragma unroll
+    for (int j = 0; j < RPT; ++j) {
+      const int q = t + j * NT;
+      const bool live = v0 + q < vend;
+      code[j] = live ? (int)tab[q] : 0;
+      const int r = code[j] >> 8;
+      const int s = live ? segA[r] + (v0 + q - segP[r]) : 0;
+      pp[j] = ident ? s : perm[s];
+    }
+    T x[RPT], y[RPT], vx[RPT], vy[RPT], m[RPT];
+#pragma unroll
+    for (int j = 0; j < RPT; ++j) {
+      const int p = pp[j];
+      x[j] = SR(DMPM_P_X, p);
+      y[j] = SR(DMPM_P_X + 1, p);
+      vx[j] = SR(DMPM_P_V, p);
+      vy[j] = SR(DMPM_P_V + 1, p);
+      m[j] = PB(DMPM_P_MASS, p);
+    }
+#pragma unroll
+    for (int j = 0; j < RPT; ++j) {
+      const int q = t + j * NT;
+      if (v0 + q >= vend) continue;
+      // the slot's cell is the element search's result for this position
+      // (the re-sort keyed the slot by it)
+      const int r = code[j] >> 8, ci = code[j] & 255;
+      st[q] = (x[j] - colC[ci]) * colS[ci];
+      st[CAP + q] = (y[j] - rowC[r]) * rowS[r];
+      st[2 * CAP + q] = m[j];
+      st[3 * CAP + q] = m[j] * vx[j];
+      st[4 * CAP + q] = m[j] * vy[j];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < CC; ++u)
+      kin_cell(st, CAP, max(cv0[u], v0) - v0, min(cv1[u], vend) - v0, acc[u]);
+    __syncthreads();
+  }
+  T nacc[NPT][NQ_KIN];
+  node_sums_nt<T, NQ_KIN, CC, NPT, NT>(st, acc, RC, NR, nx, ny, nacc);
+  // kinematic nodal epilogue (k_node_kin / kin_epilogue)
+  const int nnodes = a.nn_all;
+  const int wn = NR.i1 - NR.i0, nn = wn * (NR.j1 - NR.j0);
+  const SmallDiv dwn = small_div(wn);
+#pragma unroll
+  for (int w = 0; w < NPT; ++w) {
+    const int ln = t + w * NT;
+    if (ln >= nn) continue;
+    const int jr = sdiv(dwn, ln);
+    const int nd = c.noff + (NR.j0 + jr) * (nx + 1) + NR.i0 + (ln - jr * wn);
+    const int cr = pre.cr[w], mask = pre.mask[w];
+    const T mm = nacc[w][0];
+    const T p[2] = {snap(nacc[w][1]), snap(nacc[w][2])};
+    NB_W(DMPM_N_MASS, nd, mm);
+#pragma unroll
+    for (int d = 0; d < 2; ++d) {
+      T v;
+      if (mm == T(0)) v = a.vold[(int64_t)d * nnodes + nd];
+      else v = p[d] / mm;
+      v = snap(v);
+      if (mask & (1 << d)) v = T(a.aux->cval[2 * cr + d]);
+      a.vkin[(int64_t)d * nnodes + nd] = v;
+    }
+  }
+}
+
+// Force launch: (USF) strain, volume and stress update of RC's particles from
+// the kinematic nodal velocities (own cells' results written), P2G of body
+// and internal forces over RC (k_cell_force's expressions), and the nodal
+// update of the owned nodes (k_node_force's): node block rows, vnew.
+template <typename T, bool USF, bool B, int NT, int RPT, int WPE = 1>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) void k_split_force(
+    StepArgs<T> a) {
+  constexpr int CAP = NT * RPT;
+  constexpr int CC = (kSCells + NT - 1) / NT;
+  constexpr int NPT = (kSNodes + NT - 1) / NT;
+  static_assert(4 * NQ_FORCE * kSCells <= 7 * CAP, "node-sum slices");
+  static_assert(NT >= 128, "two waves for the tables");
+  __shared__ T st[7 * CAP];  // xi, eta, m, -V, sxx, syy, sxy; the node sums alias it
+  __shared__ uint16_t tab[CAP];
+  __shared__ T vk[2 * kSVN];  // kinematic velocity at the nodes of RC's cells
+  __shared__ T colC[kTW + 1], colS[kTW + 1], rowC[kTH + 1], rowS[kTH + 1];
+  __shared__ int segA[kTH + 1], segP[kTH + 2];
+  const TileCtx<T> c = tile_ctx<T, B>(a, xcd_pos(blockIdx.x, a.ntiles), true);
+  const Grid2<T> g = c.g;
+  const int nx = g.nx, ny = g.ny, t = threadIdx.x;
+  const int* __restrict__ cs = a.sr.cs();
+  const int* __restrict__ perm = a.sr.perm();
+  const int ident = a.sr.flags[1];
+  const int ti = c.ti, tj = c.tj;
+  const Rgn R0{ti * kTW, min(ti * kTW + kTW, nx), tj * kTH, min(tj * kTH + kTH, ny)};
+  const Rgn RC{max(R0.i0 - 1, 0), R0.i1, max(R0.j0 - 1, 0), R0.j1};
+  const Rgn NR = owned_nodes(R0, nx, ny);
+  const int hC = RC.j1 - RC.j0, wC = RC.i1 - RC.i0;
+  const int wV = wC + 1, nV = wV * (hC + 1);  // nodes of RC's cells
+  const int oi = R0.i0 - RC.i0, oj = R0.j0 - RC.j0;  // first own column and row of RC
+  const int nnodes = a.nn_all;
+  int cr_[CC], ccol[CC], cs0[CC], cs1[CC];
+  cell_slots_rc<CC, NT>(RC, nx, c.coff, cs, cr_, ccol, cs0, cs1);
+  if (t < 64) row_segments(RC, nx, c.coff, cs, segA, segP);
+  else if (t < 128) cell_tables(g, RC, colC, colS, rowC, rowS);
+  if (USF) {
+    const SmallDiv dwv = small_div(wV);
+    for (int ln = t; ln < nV; ln += NT) {
+      const int jr = sdiv(dwv, ln);
+      const int nd = c.noff + (RC.j0 + jr) * (nx + 1) + RC.i0 + (ln - jr * wV);
+      vk[ln] = a.vkin[nd];
+      vk[kSVN + ln] = a.vkin[(int64_t)nnodes + nd];
+    }
+  }
+  NodePre<NPT> pre;
+  T fd[NPT][2], nmass[NPT];
+  node_pre_nt<T, NPT, NT>(a, c, NR, pre, fd, nmass);
+  __syncthreads();
+  const int S = segP[hC];
+  int cv0[CC], cv1[CC];
+  cell_ranges_of<CC>(cr_, cs0, cs1, segA, segP, cv0, cv1);
+  T acc[CC][4 * NQ_FORCE];
+#pragma unroll
+  for (int u = 0; u < CC; ++u)
+#pragma unroll
+    for (int v = 0; v < 4 * NQ_FORCE; ++v) acc[u][v] = T(0);
+  for (int v0 = 0; v0 < S; v0 += CAP) {
+    const int vend = min(v0 + CAP, S);
+    fill_slot_cells<CC>(tab, cr_, ccol, cv0, cv1, v0, vend);
+    __syncthreads();
+    int pp[RPT], code[RPT];
+    unsigned own = 0;
+#pragma unroll
+    for (int j = 0; j < RPT; ++j) {
+      const int q = t + j * NT;
+      const bool live = v0 + q < vend;
+      code[j] = live ? (int)tab[q] : 0;
+      const int r = code[j] >> 8, ci = code[j] & 255;
+      const int s = live ? segA[r] + (v0 + q - segP[r]) : 0;
+      if (live && r >= oj && ci >= oi) own |= 1u << j;
+      pp[j] = ident ? s : perm[s];
+    }
+    T x[RPT], y[RPT], m[RPT], sxx[RPT], syy[RPT], sxy[RPT], V[RPT], E[RPT];
+    T szz[RPT], ex[RPT], ey[RPT], gxy[RPT], rho[RPT], evc[RPT];
+#pragma unroll
+    for (int j = 0; j < RPT; ++j) {
+      const int p = pp[j];
+      x[j] = SR(DMPM_P_X, p);
+      y[j] = SR(DMPM_P_X + 1, p);
+      m[j] = PB(DMPM_P_MASS, p);
+      sxx[j] = SR(DMPM_P_STRESS + 0, p);
+      syy[j] = SR(DMPM_P_STRESS + 1, p);
+      sxy[j] = SR(DMPM_P_STRESS + 3, p);
+      V[j] = SR(DMPM_P_VOLUME, p);
+      if (USF) {
+        const int po = (own & (1u << j)) ? p : 0;
+        szz[j] = SR(DMPM_P_STRESS + 2, po);
+        ex[j] = SR(DMPM_P_STRAIN + 0, po);
+        ey[j] = SR(DMPM_P_STRAIN + 1, po);
+        gxy[j] = SR(DMPM_P_STRAIN + 3, po);
+        rho[j] = SR(DMPM_P_DENSITY, po);
+        evc[j] = SR(DMPM_P_VOLSTRAIN_C, po);
+      }
+    }
+    if (USF && a.mat.per_E)
+#pragma unroll
+      for (int j = 0; j < RPT; ++j) E[j] = PB(DMPM_P_YOUNGS, pp[j]);
+    else
+#pragma unroll
+      for (int j = 0; j < RPT; ++j) E[j] = T(0);
+#pragma unroll
+    for (int j = 0; j < RPT; ++j) {
+      const int q = t + j * NT;
+      if (v0 + q >= vend) continue;
+      const int rj = code[j] >> 8, ci = code[j] & 255;
+      const T xi = (x[j] - colC[ci]) * colS[ci];
+      const T eta = (y[j] - rowC[rj]) * rowS[rj];
+      T Vn = V[j], sx = sxx[j], sy = syy[j], sq = sxy[j];
+      if (USF) {
+        const int ll = ci + rj * wV;
+        const int ids[4] = {ll, ll + 1, ll + wV + 1, ll + wV};
+        T nvx[4], nvy[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          nvx[k] = vk[ids[k]];
+          nvy[k] = vk[kSVN + ids[k]];
+        }
+        StressUpd<T> u;
+        stress_update(c.mat, E[j], colS[ci], rowS[rj], xi, eta, nvx, nvy, a.dt, u);
+        sx = sxx[j] + u.ds[0];
+        sy = syy[j] + u.ds[1];
+        sq = sxy[j] + u.ds[3];
+        Vn = V[j] * u.f;
+        if (own & (1u << j)) {
+          const int p = pp[j];
+          DR_W(DMPM_P_STRAIN + 0, p, ex[j] + u.d[0]);
+          DR_W(DMPM_P_STRAIN + 1, p, ey[j] + u.d[1]);
+          DR_W(DMPM_P_STRAIN + 3, p, gxy[j] + u.d[3]);
+          DR_W(DMPM_P_STRESS + 0, p, sx);
+          DR_W(DMPM_P_STRESS + 1, p, sy);
+          DR_W(DMPM_P_STRESS + 2, p, szz[j] + u.ds[2]);
+          DR_W(DMPM_P_STRESS + 3, p, sq);
+          DR_W(DMPM_P_VOLUME, p, Vn);
+          DR_W(DMPM_P_DENSITY, p, rho[j] / u.f);
+          DR_W(DMPM_P_VOLSTRAIN_C, p, evc[j] + u.dvol);
+          if (a.temps) {
+            for (int k = 0; k < 6; ++k) {
+              PB_W(DMPM_P_STRAIN_RATE + k, p, u.rr[k]);
+              PB_W(DMPM_P_DSTRAIN + k, p, u.d[k]);
+            }
+            PB_W(DMPM_P_DVOLSTRAIN, p, u.dvol);
+          }
+        }
+      }
+      st[q] = xi;
+      st[CAP + q] = eta;
+      st[2 * CAP + q] = m[j];
+      st[3 * CAP + q] = -Vn;
+      st[4 * CAP + q] = sx;
+      st[5 * CAP + q] = sy;
+      st[6 * CAP + q] = sq;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < CC; ++u) {
+      if (cr_[u] < 0) continue;
+      const int I = RC.i0 + ccol[u], J = RC.j0 + cr_[u];
+      force_cell(st, CAP, max(cv0[u], v0) - v0, min(cv1[u], vend) - v0, g.X(I), g.X(I + 1),
+                 g.Y(J), g.Y(J + 1), a.gx, a.gy, acc[u]);
+    }
+    __syncthreads();
+  }
+  T nacc[NPT][NQ_FORCE];
+  node_sums_nt<T, NQ_FORCE, CC, NPT, NT>(st, acc, RC, NR, nx, ny, nacc);
+  // force nodal epilogue (k_node_force / force_epilogue) of the owned nodes
+  const int wn = NR.i1 - NR.i0, nn = wn * (NR.j1 - NR.j0);
+  const SmallDiv dwn = small_div(wn);
+#pragma unroll
+  for (int w = 0; w < NPT; ++w) {
+    const int ln = t + w * NT;
+    if (ln >= nn) continue;
+    const int jr = sdiv(dwn, ln);
+    const int i = NR.i0 + (ln - jr * wn), j = NR.j0 + jr;
+    const int nd = c.noff + j * (nx + 1) + i;
+    const int lk = (i - RC.i0) + (j - RC.j0) * wV;
+    const int cr = pre.cr[w], mask = pre.mask[w];
+    const T mm = nmass[w];
+#pragma unroll
+    for (int d = 0; d < 2; ++d) {
+      T fe = nacc[w][d];
+      if (a.has_nf) {
+        const StepAux& x = *a.aux;
+        const int qg = x.nfd[(int64_t)d * nnodes + nd];
+        if (qg >= 0)
+          for (int k = x.nf_off[qg]; k < x.nf_off[qg + 1]; ++k)
+            fe = fe + fn_value<T>(x.fns, x.nf_fn[k], step_time(a.clock, a.kofs, a.t, a.dt)) *
+                          T(x.nf_val[k]);
+      }
+      const T fi = nacc[w][2 + d];
+      const T F = (fi + fe) + fd[w][d];
+      T ac = nan_to_num(F / mm);
+      const T vkd = USF ? vk[d * kSVN + lk] : a.vkin[(int64_t)d * nnodes + nd];
+      T v = vkd + ac * a.dt;
+      T pm = mm * v;
+      v = snap(v);
+      ac = snap(ac);
+      if (mask & (1 << d)) {
+        const T cv = T(a.aux->cval[2 * cr + d]);
+        v = snap(cv);
+        pm = mm * cv;
+        ac = T(0);
+      }
+      NB_W(DMPM_N_FEXT + d, nd, fe);
+      NB_W(DMPM_N_FINT + d, nd, fi);
+      NB_W(DMPM_N_V + d, nd, v);
+      NB_W(DMPM_N_A + d, nd, ac);
+      NB_W(DMPM_N_P + d, nd, pm);
+      a.vnew[(int64_t)d * nnodes + nd] = v;
+    }
+  }
+}
+
+// G2P launch over the tile's own particles (k_step_fast's G phase), with the
+// nodal velocity and acceleration of the tile's nodes staged in LDS; USL then
+// updates strain, volume and stress from the new nodal velocities.  Records
+// the particles that change cell.  Workgroups from ntiles on handle the
+// particles outside the mesh (as k_step_rounds does).
+template <typename T, bool USF, bool B, int NT, int RPT, int WPE = 1>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) void k_split_g2p(
+    StepArgs<T> a) {
+  constexpr int NTN = (kTW + 1) * (kTH + 1);
+  constexpr int CAP = NT * RPT;
+  constexpr int CC = (kTW * kTH + NT - 1) / NT;
+  static_assert(NT >= 128, "two waves for the tables");
+  __shared__ T nv[2 * NTN], na[2 * NTN];
+  __shared__ uint16_t tab[CAP];
+  __shared__ T colC[kTW], colS[kTW], rowC[kTH], rowS[kTH], colX[kTW + 1], rowY[kTH + 1];
+  __shared__ int segA[kTH], segP[kTH + 1];
+  const int t = threadIdx.x;
+  const int* __restrict__ cs = a.sr.cs();
+  const int* __restrict__ perm = a.sr.perm();
+  const int ident = a.sr.flags[1];
+  if ((int)blockIdx.x >= a.ntiles) {
+    // particles outside the mesh: the state becomes NaN (as in k_g2p_tile)
+    const T nan = qnan<T>();
+    const int gs = kTailBlocks * NT;
+    for (int s = cs[a.nc_all] + ((int)blockIdx.x - a.ntiles) * NT + t; s < a.n; s += gs) {
+      const int p = ident ? s : perm[s];
+      for (int k = 0; k < kDynRows; ++k) DR_W(kDyn[k], p, nan);
+      for (int k = 0; k < kFixRows; ++k) PB_W(kFix[k], p, nan);
+      if (a.temps) {
+        PB_W(DMPM_P_XI, p, nan);
+        PB_W(DMPM_P_XI + 1, p, nan);
+        PB_W(DMPM_P_MOMENTUM, p, nan);
+        PB_W(DMPM_P_MOMENTUM + 1, p, nan);
+        for (int k = 0; k < 6; ++k) {
+          PB_W(DMPM_P_STRAIN_RATE + k, p, nan);
+          PB_W(DMPM_P_DSTRAIN + k, p, nan);
+        }
+        PB_W(DMPM_P_DVOLSTRAIN, p, nan);
+      }
+      // NaN positions search to element 0 (element.py:827-837) of the
+      // particle's own mesh
+      int c0 = 0;
+      if constexpr (B) {
+        int lo = 0, hi = a.nsims - 1;
+        while (lo < hi) {
+          const int mid = (lo + hi + 1) >> 1;
+          if (a.sims[mid].part_off <= p) lo = mid;
+          else hi = mid - 1;
+        }
+        c0 = a.sims[lo].cell_off;
+      }
+      record_move(a.aux->L, p, a.nc_all, 0, c0);
+    }
+    return;
+  }
+  const TileCtx<T> c = tile_ctx<T, B>(a, xcd_pos(blockIdx.x, a.ntiles), true);
+  const Grid2<T> g = c.g;
+  const int nx = g.nx;
+  const int ti = c.ti, tj = c.tj;
+  const Rgn R0{ti * kTW, min(ti * kTW + kTW, nx), tj * kTH, min(tj * kTH + kTH, g.ny)};
+  const int h0 = R0.j1 - R0.j0, w0 = R0.i1 - R0.i0;
+  const int wTN = w0 + 1, nTN = wTN * (h0 + 1);
+  const int nnodes = a.nn_all;
+  int cr_[CC], ccol[CC], cs0[CC], cs1[CC];
+  cell_slots_rc<CC, NT>(R0, nx, c.coff, cs, cr_, ccol, cs0, cs1);
+  if (t < 64) {
+    row_segments(R0, nx, c.coff, cs, segA, segP);
+  } else if (t < 128) {
+    cell_tables(g, R0, colC, colS, rowC, rowS);
+    // node coordinates of the tile's columns and rows (the cell-change test)
+    const int r = t - 64;
+    if (r <= w0) colX[r] = g.X(R0.i0 + r);
+    if (r <= h0) rowY[r] = g.Y(R0.j0 + r);
+  }
+  {
+    const SmallDiv dw = small_div(wTN);
+    for (int ln = t; ln < nTN; ln += NT) {
+      const int jr = sdiv(dw, ln);
+      const int nd = c.noff + (R0.j0 + jr) * (nx + 1) + R0.i0 + (ln - jr * wTN);
+      nv[ln] = a.vnew[nd];
+      nv[NTN + ln] = a.vnew[(int64_t)nnodes + nd];
+      na[ln] = NB(DMPM_N_A, nd);
+      na[NTN + ln] = NB(DMPM_N_A + 1, nd);
+    }
+  }
+  __syncthreads();
+  const int S = segP[h0];
+  int cv0[CC], cv1[CC];
+  cell_ranges_of<CC>(cr_, cs0, cs1, segA, segP, cv0, cv1);
+  for (int v0 = 0; v0 < S; v0 += CAP) {
+    const int vend = min(v0 + CAP, S);
+    if (v0 > 0) __syncthreads();  // the previous round's table reads
+    fill_slot_cells<CC>(tab, cr_, ccol, cv0, cv1, v0, vend);
+    __syncthreads();
+    int pp[RPT], code[RPT];
+#pragma unroll
+    for (int j = 0; j < RPT; ++j) {
+      const int q = t + j * NT;
+      const bool live = v0 + q < vend;
+      code[j] = live ? (int)tab[q] : 0;
+      const int r = code[j] >> 8;
+      const int s = live ? segA[r] + (v0 + q - segP[r]) : 0;
+      pp[j] = ident ? s : perm[s];
+    }
+    T x[RPT], y[RPT], vx[RPT], vy[RPT], m[RPT];
+    T sxx[RPT], syy[RPT], szz[RPT], sxy[RPT], V[RPT], E[RPT];
+    T ex[RPT], ey[RPT], gxy[RPT], rho[RPT], evc[RPT];
+#pragma unroll
+    for (int j = 0; j < RPT; ++j) {
+      const int p = pp[j];
+      x[j] = SR(DMPM_P_X, p);
+      y[j] = SR(DMPM_P_X + 1, p);
+      vx[j] = SR(DMPM_P_V, p);
+      vy[j] = SR(DMPM_P_V + 1, p);
+      if (!USF) {
+        sxx[j] = SR(DMPM_P_STRESS + 0, p);
+        syy[j] = SR(DMPM_P_STRESS + 1, p);
+        szz[j] = SR(DMPM_P_STRESS + 2, p);
+        sxy[j] = SR(DMPM_P_STRESS + 3, p);
+        V[j] = SR(DMPM_P_VOLUME, p);
+        ex[j] = SR(DMPM_P_STRAIN + 0, p);
+        ey[j] = SR(DMPM_P_STRAIN + 1, p);
+        gxy[j] = SR(DMPM_P_STRAIN + 3, p);
+        rho[j] = SR(DMPM_P_DENSITY, p);
+        evc[j] = SR(DMPM_P_VOLSTRAIN_C, p);
+      }
+    }
+    if (a.temps)
+#pragma unroll
+      for (int j = 0; j < RPT; ++j) m[j] = PB(DMPM_P_MASS, pp[j]);
+    if (!USF) {
+      if (a.mat.per_E)
+#pragma unroll
+        for (int j = 0; j < RPT; ++j) E[j] = PB(DMPM_P_YOUNGS, pp[j]);
+      else
+#pragma unroll
+        for (int j = 0; j < RPT; ++j) E[j] = T(0);
+    }
+#pragma unroll
+    for (int j = 0; j < RPT; ++j) {
+      if (v0 + t + j * NT >= vend) continue;
+      const int p = pp[j];
+      const int rj = code[j] >> 8, ci = code[j] & 255;
+      const T xi = (x[j] - colC[ci]) * colS[ci];
+      const T eta = (y[j] - rowC[rj]) * rowS[rj];
+      if (a.temps) {
+        PB_W(DMPM_P_XI, p, xi);
+        PB_W(DMPM_P_XI + 1, p, eta);
+      }
+      const int ll = ci + rj * wTN;
+      const int ids[4] = {ll, ll + 1, ll + wTN + 1, ll + wTN};
+      T N[4];
+      q4_N4(xi, eta, N);
+      T gvx[4], gvy[4];
+      T nvx = T(0), nvy = T(0), nax = T(0), nay = T(0);
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        gvx[k] = nv[ids[k]];
+        gvy[k] = nv[NTN + ids[k]];
+        nvx = nvx + N[k] * gvx[k];
+        nvy = nvy + N[k] * gvy[k];
+        nax = nax + N[k] * na[ids[k]];
+        nay = nay + N[k] * na[NTN + ids[k]];
+      }
+      const T pvx = a.vu ? nvx : vx[j] + nax * a.dt;
+      const T pvy = a.vu ? nvy : vy[j] + nay * a.dt;
+      const T x1 = x[j] + nvx * a.dt, y1 = y[j] + nvy * a.dt;
+      DR_W(DMPM_P_V, p, pvx);
+      DR_W(DMPM_P_V + 1, p, pvy);
+      DR_W(DMPM_P_X, p, x1);
+      DR_W(DMPM_P_X + 1, p, y1);
+      if (a.temps) {
+        PB_W(DMPM_P_MOMENTUM, p, m[j] * pvx);
+        PB_W(DMPM_P_MOMENTUM + 1, p, m[j] * pvy);
+      }
+      if (!USF) {
+        StressUpd<T> u;
+        stress_update(c.mat, E[j], colS[ci], rowS[rj], xi, eta, gvx, gvy, a.dt, u);
+        DR_W(DMPM_P_STRAIN + 0, p, ex[j] + u.d[0]);
+        DR_W(DMPM_P_STRAIN + 1, p, ey[j] + u.d[1]);
+        DR_W(DMPM_P_STRAIN + 3, p, gxy[j] + u.d[3]);
+        DR_W(DMPM_P_STRESS + 0, p, sxx[j] + u.ds[0]);
+        DR_W(DMPM_P_STRESS + 1, p, syy[j] + u.ds[1]);
+        DR_W(DMPM_P_STRESS + 2, p, szz[j] + u.ds[2]);
+        DR_W(DMPM_P_STRESS + 3, p, sxy[j] + u.ds[3]);
+        DR_W(DMPM_P_VOLUME, p, V[j] * u.f);
+        DR_W(DMPM_P_DENSITY, p, rho[j] / u.f);
+        DR_W(DMPM_P_VOLSTRAIN_C, p, evc[j] + u.dvol);
+        if (a.temps) {
+          for (int k = 0; k < 6; ++k) {
+            PB_W(DMPM_P_STRAIN_RATE + k, p, u.rr[k]);
+            PB_W(DMPM_P_DSTRAIN + k, p, u.d[k]);
+          }
+          PB_W(DMPM_P_DVOLSTRAIN, p, u.dvol);
+        }
+      }
+      // still inside its cell: no element search (the same element id)
+      if (x1 >= colX[ci] && x1 < colX[ci + 1] && y1 >= rowY[rj] && y1 < rowY[rj + 1]) continue;
+      const int I = R0.i0 + ci, J = R0.j0 + rj;
+      const int k0 = c.coff + J * nx + I;
+      const int e1 = element_id<T, 2>(g.geo(), x1, y1);
+      const int k1 = (e1 >= 0 && e1 < g.ncells) ? c.coff + e1 : a.nc_all;
+      if (k1 != k0) record_move(a.aux->L, p, k0, e1, k1);
+    }
+  }
+}
+
 #undef SR
 #undef PB
 #undef NB
@@ -2521,10 +3264,22 @@ bool step_eligible(const dmpm_plan* P) {
          (uint64_t)DMPM_N_ROWS * (uint64_t)P->d.nld * esz < (1ull << 32);
 }
 
-// The kernels of one step (B: batch plan).
+// The kernels of one step (B: batch plan): the one-launch step (kind 1) or
+// the split step (kind 2).
 template <typename T, bool B>
-void step_launch(const StepArgs<T>& a, bool usf, hipStream_t s) {
+void step_launch(const StepArgs<T>& a, bool usf, int kind, hipStream_t s) {
   constexpr bool kFast = sizeof(T) == 4;
+  if (kind == 2) {
+    k_split_kin<T, B, kSKinT, kSKinR><<<a.ntiles, kSKinT, 0, s>>>(a);
+    if (usf) {
+      k_split_force<T, true, B, kSForT, kSForR><<<a.ntiles, kSForT, 0, s>>>(a);
+      k_split_g2p<T, true, B, kSG2pT, kSG2pR><<<a.ntiles + kTailBlocks, kSG2pT, 0, s>>>(a);
+    } else {
+      k_split_force<T, false, B, kSForT, kSForR><<<a.ntiles, kSForT, 0, s>>>(a);
+      k_split_g2p<T, false, B, kSG2pT, kSG2pR><<<a.ntiles + kTailBlocks, kSG2pT, 0, s>>>(a);
+    }
+    return;
+  }
   if (usf) {
     if constexpr (kFast) k_step_fast<T, true, B><<<a.ntiles, kFT, 0, s>>>(a);
     k_step_rounds<T, true, B><<<a.nrb + kTailBlocks, kFT, 0, s>>>(a);
@@ -2669,9 +3424,9 @@ int step_run(dmpm_plan* P, const Mesh2<T>& M, T* pa, T* pb2, T* pstat, int32_t* 
     a.par = (int)(k & 1);
     ProfScopeT ps_(P, KC_STEP, s);
     if (P->nsims > 1)
-      step_launch<T, true>(a, scheme == DMPM_USF, s);
+      step_launch<T, true>(a, scheme == DMPM_USF, P->step_kernel, s);
     else
-      step_launch<T, false>(a, scheme == DMPM_USF, s);
+      step_launch<T, false>(a, scheme == DMPM_USF, P->step_kernel, s);
     DMPM_TRY(launch_check("step"));
   }
   return DMPM_OK;

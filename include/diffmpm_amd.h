@@ -164,10 +164,12 @@ int dmpm_plan_set_block(dmpm_plan* plan, int32_t cells_per_block);
  * DMPM_OPT_USF_SPLIT = 1: the fused USF step updates stresses in a streamed
  * particle kernel before the force P2G cell pass (1, default) or inside that
  * pass (0).
- * DMPM_OPT_STEP_KERNEL = 2: a single-simulation plan runs each fused 2D step
- * as one kernel launch after the re-sort (1, default; workgroups own tiles of
- * cells and recompute a halo) or as one launch per phase (0; batch plans
- * always do). */
+ * DMPM_OPT_STEP_KERNEL = 2: the fused 2D step after the re-sort runs as one
+ * kernel launch (1, default; workgroups own tiles of cells and recompute a
+ * two-cell halo), as three launches (2: kinematic P2G and the stress and
+ * force phases over tiles that own nodes and recompute one column and one row
+ * of cells, then G2P over tiles of cells), or as one launch per phase (0).
+ * Any other value is DMPM_EINVAL. */
 enum { DMPM_OPT_USF_SPLIT = 1, DMPM_OPT_STEP_KERNEL = 2 };
 int dmpm_plan_set_option(dmpm_plan* plan, int32_t option, int32_t value);
 /* Declares that the element ids changed outside the library (forces a re-sort). */

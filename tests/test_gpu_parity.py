@@ -548,17 +548,17 @@ def test_tile_path_crowded_cell_equals_per_phase_path(scheme):
         assert torch.equal(x, y)
 
 
-def _run_pair(spec, scheme, nsteps, chunk, vu=False, per_phase_b=False):
-    """Runs spec on two fresh simulations: a with the one-launch step, b with
-    one launch per phase (or the per-phase kernels); compares after every
-    chunk of steps."""
+def _run_pair(spec, scheme, nsteps, chunk, vu=False, per_phase_b=False, kind=1):
+    """Runs spec on two fresh simulations: a with the one-launch step (kind
+    1) or the split step (kind 2), b with one launch per phase (or the
+    per-phase kernels); compares after every chunk of steps."""
     from diffmpm_amd import _lib as L
     from diffmpm_amd._plan import plan_for
     a, b = cases.build_gpu(spec), cases.build_gpu(spec)
     g = torch.tensor(spec["gravity"], dtype=torch.float64)
     pa = plan_for(a.mesh.elements, a.mesh.particles[0])
     pb = plan_for(b.mesh.elements, b.mesh.particles[0])
-    pa.set_option(L.DMPM_OPT_STEP_KERNEL, 1)
+    pa.set_option(L.DMPM_OPT_STEP_KERNEL, kind)
     pb.set_option(L.DMPM_OPT_STEP_KERNEL, 0)
     for k in range(0, nsteps, chunk):
         pa.run(scheme, vu, g, spec["dt"], k, chunk, True)
@@ -568,26 +568,30 @@ def _run_pair(spec, scheme, nsteps, chunk, vu=False, per_phase_b=False):
     return a
 
 
+@pytest.mark.parametrize("kind", [1, 2])
 @pytest.mark.parametrize("chunk", [1, 3, 4])
 @pytest.mark.parametrize("scheme,per_E,drift", [("usf", False, (0.6, 0.3)),
                                                  ("usl", False, (0.6, 0.3)),
                                                  ("usf", True, None), ("usl", True, (0.6, 0.3))])
-def test_one_launch_step_equals_phase_launches(scheme, per_E, drift, chunk):
+def test_one_launch_step_equals_phase_launches(scheme, per_E, drift, chunk, kind):
     """The one-launch step (tiles of cells with halos, alternating
-    particle blocks) gives bit-identical states to one launch per phase, for
-    odd and even step counts per run (the state ends in the caller's block
-    either way), across cell changes, on a mesh of partial tiles (cases that
-    stay finite in the oracle too)."""
+    particle blocks) and the split step (three launches over node and cell
+    tiles) give bit-identical states to one launch per phase, for odd and
+    even step counts per run (the state ends in the caller's block either
+    way), across cell changes, on a mesh of partial tiles (cases that stay
+    finite in the oracle too)."""
     spec = _moving_spec(scheme, per_E, drift=drift, velocity_scale=0.05 if drift else 0.1)
-    a = _run_pair(spec, scheme, 12, chunk)
+    a = _run_pair(spec, scheme, 12, chunk, kind=kind)
     assert torch.isfinite(a.mesh.particles[0].loc).all()
 
 
+@pytest.mark.parametrize("kind", [1, 2])
 @pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
 @pytest.mark.parametrize("scheme", ["usf", "usl"])
-def test_one_launch_step_large_mesh(scheme, dtype):
+def test_one_launch_step_large_mesh(scheme, dtype, kind):
     """Several full tiles in each direction (96x70 cells), nodal force and
-    velocity update: one-launch step equals the per-phase kernels."""
+    velocity update: the one-launch and the split step equal the per-phase
+    kernels."""
     spec = _moving_spec(scheme, nx=96, ny=70, rows=60, drift=None, velocity_scale=0.1)
     spec["velocity_update"] = True
     from diffmpm_amd import _lib as L
@@ -596,15 +600,16 @@ def test_one_launch_step_large_mesh(scheme, dtype):
     g = torch.tensor(spec["gravity"], dtype=torch.float64)
     pa = plan_for(a.mesh.elements, a.mesh.particles[0])
     pb = plan_for(b.mesh.elements, b.mesh.particles[0])
-    pa.set_option(L.DMPM_OPT_STEP_KERNEL, 1)
+    pa.set_option(L.DMPM_OPT_STEP_KERNEL, kind)
     pa.run(scheme, True, g, spec["dt"], 0, 5, True)
     pb.run(scheme, True, g, spec["dt"], 0, 5, True, per_phase=True)
     for x, y in zip(_state(a), _state(b)):
         assert torch.equal(x, y)
 
 
+@pytest.mark.parametrize("kind", [1, 2])
 @pytest.mark.parametrize("scheme", ["usf", "usl"])
-def test_one_launch_step_crowded_and_outside(scheme):
+def test_one_launch_step_crowded_and_outside(scheme, kind):
     """A 3000-particle cell (several staging rounds in one tile) and
     particles outside the mesh (NaN state, handled by the tail workgroups)."""
     spec = _crowded_spec(3000)
@@ -612,7 +617,7 @@ def test_one_launch_step_crowded_and_outside(scheme):
     pts = np.asarray(spec["particles"])
     pts[:3] = [[1.2, 0.3], [np.nan, 0.4], [0.3, 1.5]]
     spec["particles"] = pts.tolist()
-    _run_pair(spec, scheme, 3, 3)
+    _run_pair(spec, scheme, 3, 3, kind=kind)
 
 
 @pytest.mark.parametrize("split", [False, True])
@@ -662,7 +667,8 @@ def test_step_between_blocks_equals_run(scheme, split):
         sin = sout
 
 
-@pytest.mark.parametrize("driver", ["method", "phase_kernels", "phase_launches", "step"])
+@pytest.mark.parametrize("driver", ["method", "phase_kernels", "phase_launches", "step",
+                                    "split"])
 @pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
 @pytest.mark.parametrize("scheme", ["usf", "usl"])
 def test_nodal_damping_force_per_step_parity(scheme, dtype, driver):
@@ -694,7 +700,7 @@ def test_nodal_damping_force_per_step_parity(scheme, dtype, driver):
     expect_fd = gn.f_damp.clone()
     g = torch.tensor(spec["gravity"], dtype=torch.float64)
     plan = plan_for(gsim.mesh.elements, gsim.mesh.particles[0])
-    plan.set_option(L.DMPM_OPT_STEP_KERNEL, 1 if driver == "step" else 0)
+    plan.set_option(L.DMPM_OPT_STEP_KERNEL, {"step": 1, "split": 2}.get(driver, 0))
     for i in range(spec["nsteps"]):
         cases.load_oracle_state_into_gpu(osim, gsim)
         osim.step(i, np.asarray(spec["gravity"]))
