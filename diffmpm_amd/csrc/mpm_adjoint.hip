@@ -12,16 +12,19 @@
 // dmpm_step_adjoint recomputes the forward step from its input state (on a
 // scratch block, keeping the nodal arrays and the kinematic nodal velocity),
 // then runs, for cotangents Sbar' of the step output:
-//   k_adj_g2p     G2P adjoint, particle-local part: cotangents of the nodal
-//                 velocity/acceleration as per-particle weights, the
-//                 natural-coordinate cotangent of G2P; USL: stress adjoint
-//   cell + node 1 scatter of those weights to the nodes (deterministic cell
-//                 pass), grid-update adjoint (a = F/m, v' = v + a dt, snaps,
-//                 constraints); USL: kinematics adjoint (v = P/m)
-//   k_adj_force   internal/body force adjoint (gather of Fbar); USF: stress
-//                 adjoint (sigma' = sigma + D deps, V' = V (1 + dvol))
-//   cell + node 2 (USF) scatter of the strain-rate cotangents to the kinematic
-//                 nodal velocities, kinematics adjoint
+//   k_adj_tile1   per node tile (mpm_tiles.h): G2P adjoint of the particles of
+//                 the tile's cells and of the column and row of cells left of
+//                 and below it (cotangents of the nodal velocity/acceleration
+//                 as per-particle weights, the natural-coordinate cotangent of
+//                 G2P; USL: stress adjoint), their scatter to the nodes the
+//                 tile owns (cell sums, node sums) and the grid-update adjoint
+//                 there (a = F/m, v' = v + a dt, snaps, constraints; USL:
+//                 kinematics adjoint, v = P/m)
+//   k_adj_tile2   (USF) per node tile: internal/body force adjoint (gather of
+//                 Fbar), stress adjoint (sigma' = sigma + D deps, V' = V (1 +
+//                 dvol)), scatter of the strain-rate cotangents to the owned
+//                 nodes, kinematics adjoint there
+//   k_adj_force   (USL) the force adjoint, per particle
 //   k_adj_kin     P2G of mass/momentum adjoint (gather of Pbar, mbar), chain
 //                 to x through the natural coordinates
 // Element ids are piecewise constant in x: no cotangent flows through them.
@@ -35,6 +38,7 @@
 #include "mpm_common.h"
 #include "mpm_plan.h"
 #include "mpm_step.h"
+#include "mpm_tiles.h"
 
 using namespace dmpm;
 
@@ -52,6 +56,10 @@ namespace {
 // particle, L00, L11 at the centroid), natural-coordinate cotangent
 enum { W_NVX = 0, W_NVY, W_NAX, W_NAY, W_L00, W_L11, W_L01, W_L10, W_C00, W_C11, W_XI, W_ETA,
        W_ROWS };
+// rows of the weight workspace that carry the updated stress and volume
+// cotangents of the particles in boundary cells between two launches (the
+// weights themselves stay in LDS)
+enum { W_BSXX = 0, W_BSYY, W_BSXY, W_BVOL };
 // rows of the nodal adjoint workspace
 enum { A_FX = 0, A_FY, A_M, A_VKX, A_VKY, A_PX, A_PY, A_ROWS };
 
@@ -146,6 +154,12 @@ __device__ __forceinline__ PGeo<T> pgeo(const Mesh2<T>& M, int p, T x, T y) {
   return q;
 }
 
+// Whether cell (I, J) of a mesh is in the last column or row of its tile
+// (a cell whose particles another tile reads).
+__device__ __forceinline__ bool boundary_cell(int I, int J) {
+  return (I % kStepTile) == kStepTile - 1 || (J % kStepTile) == kStepTile - 1;
+}
+
 // d^2 N_k / (dxi deta) of the bilinear shape functions (element.py:748-755).
 __device__ __forceinline__ float mixed2(int k) { return (k & 1) ? -0.25f : 0.25f; }
 
@@ -210,32 +224,25 @@ __device__ __forceinline__ void stress_adjoint(const Mat2& mat, T E_p, T dt, T V
   xb[1] = seta * q.ix;
 }
 
-// G2P adjoint (particle.py:208-247), particle-local part; USL also the
-// adjoint of the stress update made with the updated nodal velocities
-// (scheme.py:127-133).
-// G2P adjoint (particle.py:208-247) of particle p (inside the mesh, geometry
-// q), particle-local part: cotangent updates, the natural-coordinate
-// cotangent (rows W_XI, W_ETA of wk) and the weights w[W_NVX .. W_C11] the
-// scatter to the nodes needs; USL also the adjoint of the stress update made
-// with the updated nodal velocities (scheme.py:127-133).
+// G2P adjoint (particle.py:208-247) of one particle (geometry q) from its
+// position and velocity cotangents (xbx, xby, vbx, vby) and the velocities
+// (vx, vy) and accelerations (ax, ay) of its cell's nodes: the weights
+// w[W_NVX .. W_C11] that the scatter to the nodes needs and the
+// natural-coordinate cotangent (sxi, seta).  USL also the adjoint of the
+// stress update made with the updated nodal velocities (scheme.py:127-133):
+// sb and Vb are the stress and volume cotangents of the step output, Vb_in
+// receives the step-start volume's, *lebar accumulates dL/dlogE.
 template <typename T, bool USL>
-__device__ __forceinline__ void adj_g2p_particle(const Mesh2<T>& M, const PRows<T>& R,
-                                                 int64_t ld, const T* __restrict__ nb,
-                                                 int64_t nld, T* __restrict__ cot,
-                                                 T* __restrict__ wk, T* __restrict__ lebar,
-                                                 const Mat2& mat, T dt, int vu, int p,
-                                                 const PGeo<T>& q, T w[10]) {
-  const int* ids = q.ids;
-  const T xbx = CR(DMPM_C_X)[p], xby = CR(DMPM_C_X + 1)[p];
-  const T vbx = CR(DMPM_C_V)[p], vby = CR(DMPM_C_V + 1)[p];
-  T vx[4], vy[4];
+__device__ __forceinline__ void adj_g2p_core(const Mat2& mat, T E_p, T V_in, T dt, int vu,
+                                             const PGeo<T>& q, T xbx, T xby, T vbx, T vby,
+                                             const T vx[4], const T vy[4], const T ax[4],
+                                             const T ay[4], const T sb[3], T Vb, T* lebar,
+                                             T& Vb_in, T& sxi_out, T& seta_out, T w[10]) {
   T sxi = T(0), seta = T(0);
   for (int k = 0; k < 4; ++k) {
-    vx[k] = NR(DMPM_N_V)[ids[k]];
-    vy[k] = NR(DMPM_N_V + 1)[ids[k]];
     T s = dt * (vx[k] * xbx + vy[k] * xby);
     if (vu) s = s + (vx[k] * vbx + vy[k] * vby);
-    else s = s + dt * (NR(DMPM_N_A)[ids[k]] * vbx + NR(DMPM_N_A + 1)[ids[k]] * vby);
+    else s = s + dt * (ax[k] * vbx + ay[k] * vby);
     T gx, gy;
     q4_dNdxi(k, q.xi, q.eta, gx, gy);
     sxi = sxi + gx * s;
@@ -245,33 +252,72 @@ __device__ __forceinline__ void adj_g2p_particle(const Mesh2<T>& M, const PRows<
   w[W_NVY] = vu ? dt * xby + vby : dt * xby;
   w[W_NAX] = vu ? T(0) : dt * vbx;
   w[W_NAY] = vu ? T(0) : dt * vby;
-  if (vu) {
-    CR(DMPM_C_V)[p] = T(0);
-    CR(DMPM_C_V + 1)[p] = T(0);
-  }
   T Lb[6] = {T(0), T(0), T(0), T(0), T(0), T(0)};
   if (USL) {
-    const T sb[3] = {CR(DMPM_C_SXX)[p], CR(DMPM_C_SYY)[p], CR(DMPM_C_SXY)[p]};
-    T xb[2], Vb_in, unused = T(0);
-    stress_adjoint(M.mat(mat, q.s), mat.per_E ? R.E[p] : T(0), dt,
-                   R.V[p], q, vx, vy, sb, CR(DMPM_C_VOL)[p],
-                   lebar ? lebar + p : &unused, Lb, xb, Vb_in);
-    CR(DMPM_C_VOL)[p] = Vb_in;
+    T xb[2];
+    stress_adjoint(mat, E_p, dt, V_in, q, vx, vy, sb, Vb, lebar, Lb, xb, Vb_in);
     sxi = sxi + xb[0];
     seta = seta + xb[1];
   }
   for (int k = 0; k < 6; ++k) w[W_L00 + k] = Lb[k];
-  WR(W_XI)[p] = sxi;
-  WR(W_ETA)[p] = seta;
+  sxi_out = sxi;
+  seta_out = seta;
 }
 
-// Force P2G adjoint (element.py:236-268, 842-901): gathers the nodal force
-// cotangent; USF then the adjoint of the stress update made with the
-// kinematic nodal velocities.
-// Force P2G adjoint (element.py:236-268, 842-901) of particle p (inside the
-// mesh, geometry q): gathers the nodal force cotangent; USF then the adjoint
-// of the stress update made with the kinematic nodal velocities, whose
-// strain-rate cotangents (L00, L11, L01, L10, C00, C11) it returns in w.
+// Force P2G adjoint (element.py:236-268, 842-901) of one particle (geometry
+// q, mass m, the stress and volume the force P2G used) from the force
+// cotangents (fx, fy) of its cell's nodes and the stress and volume
+// cotangents cs[3], cvol of the step output: the new stress cotangents sb,
+// the volume cotangent vol (USF: of the step-start volume V_in, after the
+// adjoint of the stress update made with the kinematic nodal velocities vkx,
+// vky, whose strain-rate cotangents go to w), the natural-coordinate
+// cotangent (sxi, seta); *lebar accumulates dL/dlogE.
+template <typename T, bool USF>
+__device__ __forceinline__ void adj_force_core(const Mat2& mat, T E_p, T dt, T V_in, T V, T sxx,
+                                               T syy, T sxy, T m, const PGeo<T>& q, T Gx, T Gy,
+                                               const T fx[4], const T fy[4], const T vkx[4],
+                                               const T vky[4], const T cs[3], T cvol, T* lebar,
+                                               T sb[3], T& vol, T w[6], T& sxi_out,
+                                               T& seta_out) {
+  T Vb = T(0), sbxx = T(0), sbyy = T(0), sbxy = T(0), sxi = T(0), seta = T(0);
+  for (int k = 0; k < 4; ++k) {
+    T gx, gy, dx, dy;
+    q4_dNdxi(k, q.xi, q.eta, gx, gy);
+    dx = gx * q.ix;
+    dy = gy * q.iy;
+    const T body = m * (Gx * fx[k] + Gy * fy[k]);
+    sxi = sxi + gx * body;
+    seta = seta + gy * body;
+    Vb = Vb - (fx[k] * (dx * sxx + dy * sxy) + fy[k] * (dy * syy + dx * sxy));
+    sbxx = sbxx + fx[k] * dx;
+    sbyy = sbyy + fy[k] * dy;
+    sbxy = sbxy + (fx[k] * dy + fy[k] * dx);
+    const T s = T(mixed2(k));
+    sxi = sxi - V * (s * q.iy) * (fx[k] * sxy + fy[k] * syy);
+    seta = seta - V * (s * q.ix) * (fx[k] * sxx + fy[k] * sxy);
+  }
+  sb[0] = cs[0] - V * sbxx;
+  sb[1] = cs[1] - V * sbyy;
+  sb[2] = cs[2] - V * sbxy;
+  const T Vbt = cvol + Vb;
+  if (USF) {
+    T Lb[6], xb[2], Vb_in;
+    stress_adjoint(mat, E_p, dt, V_in, q, vkx, vky, sb, Vbt, lebar, Lb, xb, Vb_in);
+    vol = Vb_in;
+    for (int k = 0; k < 6; ++k) w[k] = Lb[k];
+    sxi = sxi + xb[0];
+    seta = seta + xb[1];
+  } else {
+    vol = Vbt;
+  }
+  sxi_out = sxi;
+  seta_out = seta;
+}
+
+// The same for particle p (inside the mesh, geometry q) with the nodal values
+// read from the nodal adjoint rows and vkin: cotangent updates, the
+// natural-coordinate cotangent added to rows W_XI, W_ETA of wk, the
+// strain-rate cotangents to w (USF).
 template <typename T, bool USF>
 __device__ __forceinline__ void adj_force_particle(
     const Mesh2<T>& M, const PRows<T>& R, const POut<T>& O, int64_t ld,
@@ -285,49 +331,25 @@ __device__ __forceinline__ void adj_force_particle(
   const T sxx = USF ? O.sxx[p] : R.sxx[p];
   const T syy = USF ? O.syy[p] : R.syy[p];
   const T sxy = USF ? O.sxy[p] : R.sxy[p];
-  const T m = R.m[p];
-  T Vb = T(0), sbxx = T(0), sbyy = T(0), sbxy = T(0), sxi = T(0), seta = T(0);
+  T fx[4], fy[4], vkx[4], vky[4];
   for (int k = 0; k < 4; ++k) {
-    const T fx = AR(A_FX)[ids[k]], fy = AR(A_FY)[ids[k]];
-    T gx, gy, dx, dy;
-    q4_dNdxi(k, q.xi, q.eta, gx, gy);
-    dx = gx * q.ix;
-    dy = gy * q.iy;
-    const T body = m * (Gx * fx + Gy * fy);
-    sxi = sxi + gx * body;
-    seta = seta + gy * body;
-    Vb = Vb - (fx * (dx * sxx + dy * sxy) + fy * (dy * syy + dx * sxy));
-    sbxx = sbxx + fx * dx;
-    sbyy = sbyy + fy * dy;
-    sbxy = sbxy + (fx * dy + fy * dx);
-    const T s = T(mixed2(k));
-    sxi = sxi - V * (s * q.iy) * (fx * sxy + fy * syy);
-    seta = seta - V * (s * q.ix) * (fx * sxx + fy * sxy);
+    fx[k] = AR(A_FX)[ids[k]];
+    fy[k] = AR(A_FY)[ids[k]];
+    vkx[k] = USF ? vkin[ids[k]] : T(0);
+    vky[k] = USF ? vkin[M.nnodes + ids[k]] : T(0);
   }
-  const T sb[3] = {CR(DMPM_C_SXX)[p] - V * sbxx, CR(DMPM_C_SYY)[p] - V * sbyy,
-                   CR(DMPM_C_SXY)[p] - V * sbxy};
-  const T Vbt = CR(DMPM_C_VOL)[p] + Vb;
+  const T cs[3] = {CR(DMPM_C_SXX)[p], CR(DMPM_C_SYY)[p], CR(DMPM_C_SXY)[p]};
+  // the volume cotangent after the G2P adjoint (a boundary cell's particle
+  // left it in the workspace)
+  const T cvol = boundary_cell(q.I, q.J) ? WR(W_BVOL)[p] : CR(DMPM_C_VOL)[p];
+  T sb[3], vol, sxi, seta, dummy = T(0);
+  adj_force_core<T, USF>(M.mat(mat, q.s), mat.per_E ? R.E[p] : T(0), dt, R.V[p], V, sxx, syy,
+                         sxy, R.m[p], q, Gx, Gy, fx, fy, vkx, vky, cs, cvol,
+                         lebar ? lebar + p : &dummy, sb, vol, w, sxi, seta);
   CR(DMPM_C_SXX)[p] = sb[0];
   CR(DMPM_C_SYY)[p] = sb[1];
   CR(DMPM_C_SXY)[p] = sb[2];
-  if (USF) {
-    T vx[4], vy[4];
-    for (int k = 0; k < 4; ++k) {
-      vx[k] = vkin[ids[k]];
-      vy[k] = vkin[M.nnodes + ids[k]];
-    }
-    T Lb[6], xb[2], Vb_in;
-    T dummy = T(0);
-    stress_adjoint(M.mat(mat, q.s), mat.per_E ? R.E[p] : T(0), dt,
-                   R.V[p], q, vx, vy, sb, Vbt, lebar ? lebar + p : &dummy, Lb, xb,
-                   Vb_in);
-    CR(DMPM_C_VOL)[p] = Vb_in;
-    for (int k = 0; k < 6; ++k) w[k] = Lb[k];
-    sxi = sxi + xb[0];
-    seta = seta + xb[1];
-  } else {
-    CR(DMPM_C_VOL)[p] = Vbt;
-  }
+  CR(DMPM_C_VOL)[p] = vol;
   WR(W_XI)[p] = WR(W_XI)[p] + sxi;
   WR(W_ETA)[p] = WR(W_ETA)[p] + seta;
 }
@@ -357,7 +379,8 @@ template <typename T>
 __global__ __launch_bounds__(kThreads) void k_adj_kin(Mesh2<T> M, PRows<T> R,
                                                       int64_t ld, int n,
                                                       const T* __restrict__ na,
-                                                      T* __restrict__ cot, T* __restrict__ wk) {
+                                                      T* __restrict__ cot, T* __restrict__ wk,
+                                                      int vu, int usf) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= n) return;
   const PGeo<T> q = pgeo(M, p, R.x[p], R.y[p]);
@@ -379,27 +402,57 @@ __global__ __launch_bounds__(kThreads) void k_adj_kin(Mesh2<T> M, PRows<T> R,
     sxi = sxi + gx * s;
     seta = seta + gy * s;
   }
-  CR(DMPM_C_V)[p] = CR(DMPM_C_V)[p] + m * vbx;
-  CR(DMPM_C_V + 1)[p] = CR(DMPM_C_V + 1)[p] + m * vby;
+  // (with the velocity update the G2P adjoint leaves a zero velocity
+  // cotangent)
+  CR(DMPM_C_V)[p] = (vu ? T(0) : CR(DMPM_C_V)[p]) + m * vbx;
+  CR(DMPM_C_V + 1)[p] = (vu ? T(0) : CR(DMPM_C_V + 1)[p]) + m * vby;
   CR(DMPM_C_X)[p] = CR(DMPM_C_X)[p] + sxi * q.ix;
   CR(DMPM_C_X + 1)[p] = CR(DMPM_C_X + 1)[p] + seta * q.iy;
+  if (usf && boundary_cell(q.I, q.J)) {
+    CR(DMPM_C_SXX)[p] = WR(W_BSXX)[p];
+    CR(DMPM_C_SYY)[p] = WR(W_BSYY)[p];
+    CR(DMPM_C_SXY)[p] = WR(W_BSXY)[p];
+    CR(DMPM_C_VOL)[p] = WR(W_BVOL)[p];
+  }
 }
 
-// Particle pass + cell pass of a cotangent scatter to the nodes.  The slot
-// threads run a particle-local adjoint for the particles the sort put in the
-// workgroup's cells (MODE 0, 1: G2P, adj_g2p_particle; MODE 2: force,
-// adj_force_particle) and stage their natural coordinates and weights in
-// LDS; then per corner k of a particle's cell the quantities
-//   q0 = N_k w_nvx + dNk/dx L00b + dNk/dy L10b + dNk/dx(0) C00b
-//   q1 = N_k w_nvy + dNk/dy L11b + dNk/dx L01b + dNk/dy(0) C11b
-//   q2 = N_k w_nax,  q3 = N_k w_nay
-// are summed per cell in slot order into partials[NQ * 4][ncells]
-// (node_gather then adds the four cells of a node in the canonical order).
-// MODE 0 (USL) forms all four; the USF step scatters in two passes whose
-// other weights are zero: MODE 1 the N-weighted terms, MODE 2 the
-// strain-rate terms (q0, q1 only; NQ = 2).  Workgroups from ncb on (MODE 0,
-// 1) clear the cotangents of the particles outside the mesh.
-constexpr int kSlotsA = 512;
+// Kinematics adjoint at one node and direction (element.py:191-202 with
+// constraint.py:29-44): v = snap(snap(P)/m) (or the constraint value).
+template <typename T>
+__device__ __forceinline__ void kin_adjoint(T m, T v, bool constrained, T vb, T& Pb, T& mb) {
+  Pb = T(0);
+  if (constrained || m == T(0) || v == T(0)) return;
+  Pb = vb / m;
+  mb = mb - vb * v / m;
+}
+
+// ---- node-tile adjoint ---------------------------------------------------------
+//
+// Two launches over the tiles of the forward step, each workgroup a node tile
+// as in the split forward step (mpm_tiles.h): tile (ti, tj) owns the cells R0
+// and the nodes NR; the cells of those nodes are RC = R0 grown by one cell to
+// the left and below, whose particles run the particle-local adjoint (only
+// the tile's own particles write their cotangents).  The nodal values the
+// particles read are staged in LDS for the nodes of RC's cells, the weights
+// are summed per cell in slot order and per node in node_gather's order, and
+// the node pass runs for the owned nodes in the same launch, so no cell
+// partials go through HBM:
+//   k_adj_tile1<0>  USL: G2P and stress adjoint, scatter, nodal-update and
+//                   kinematics adjoint
+//   k_adj_tile1<1>  USF: G2P adjoint, scatter, nodal-update adjoint
+//   k_adj_tile2     USF: force and stress adjoint, strain-rate scatter,
+//                   kinematics adjoint
+// Workgroups from the tile count on (k_adj_tile1) clear the cotangents of the
+// particles outside the mesh.
+//
+// A tile reads the cotangents of the particles in the column of cells left
+// of it and the row below it while the tiles that own them update them.  So
+// a particle in the last column or row of a tile (a boundary cell, the only
+// cells another tile reads) writes its updated stress and volume cotangents
+// to rows of the weight workspace (W_BSXX..W_BVOL), and the next particle
+// launch (k_adj_kin; USL: k_adj_force for the volume) moves them into place;
+// the G2P adjoint's zero velocity cotangent (velocity update) is applied by
+// k_adj_kin.
 template <typename T> struct AdjArgs {
   PRows<T> R;
   POut<T> O;
@@ -415,193 +468,531 @@ template <typename T> struct AdjArgs {
   T Gx, Gy, dt;
   int vu, n, ncb;
 };
-template <typename T, int CB, int MODE>
-__global__ __launch_bounds__(kThreads) void k_adj_scatter(Mesh2<T> M, AdjArgs<T> a, SortRef sr,
-                                                          T* __restrict__ part) {
-  constexpr int NW = MODE == 0 ? 10 : (MODE == 1 ? 4 : 6);  // staged weights
-  constexpr int W0 = MODE == 2 ? W_L00 : W_NVX;              // first staged weight
-  constexpr int NQ = MODE == 2 ? 2 : 4;
-  __shared__ T st[2 + NW][kSlotsA];
-  const int* __restrict__ perm = sr.perm();
-  const int* __restrict__ cs = sr.cs();
-  const int ident = sr.flags[1];
+
+template <typename T> struct AdjTile {
+  AdjArgs<T> a;
+  TileTab<T> tt;
+  SortRef sr;
+  const int* cidx;
+  const int* cmask;
+  int nn_all, nc_all;
+};
+
+// threads, slots per thread and round, and waves per SIMD to fit (register
+// budget) of each tile kernel
+template <int MODE> struct AdjTileShape;
+template <> struct AdjTileShape<0> { static constexpr int NT = 512, RPT = 3, WPE = 1; };
+template <> struct AdjTileShape<1> { static constexpr int NT = 512, RPT = 3, WPE = 1; };
+template <> struct AdjTileShape<2> { static constexpr int NT = 512, RPT = 3, WPE = 1; };
+constexpr int kACells = (kStepTile + 1) * (kStepTile + 1);  // cells of RC (at most)
+constexpr int kANodes = (kStepTile + 1) * (kStepTile + 1);  // owned nodes (at most)
+constexpr int kAVN = (kStepTile + 2) * (kStepTile + 2);     // nodes of RC's cells
+constexpr int kAdjOutsideBlocks = 8;
+
+// Particle geometry of a slot of RC: the slot's cell (local column ci, row r
+// of RC) from the slot table, natural coordinates and scales from the
+// tables, corner indices in the node grid of RC's cells (row length wV).
+template <typename T>
+__device__ __forceinline__ PGeo<T> tile_geo(T x, T y, int ci, int r, int wV, const T* colC,
+                                            const T* colS, const T* rowC, const T* rowS) {
+  PGeo<T> q;
+  q.in = true;
+  q.s = 0;
+  q.I = ci;
+  q.J = r;
+  q.xi = (x - colC[ci]) * colS[ci];
+  q.eta = (y - rowC[r]) * rowS[r];
+  q.ix = colS[ci];
+  q.iy = rowS[r];
+  const int ll = ci + r * wV;
+  q.ids[0] = ll;
+  q.ids[1] = ll + 1;
+  q.ids[2] = ll + wV + 1;
+  q.ids[3] = ll + wV;
+  return q;
+}
+
+template <typename T, bool B, int MODE, int NT = AdjTileShape<MODE>::NT,
+          int RPT = AdjTileShape<MODE>::RPT, int WPE = AdjTileShape<MODE>::WPE>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) void k_adj_tile1(
+    AdjTile<T> A) {
+  constexpr int CAP = NT * RPT;
+  constexpr int NW = MODE == 0 ? 10 : 4;  // staged weights, from W_NVX
+  constexpr int NQ = 4;
+  constexpr int CC = (kACells + NT - 1) / NT;
+  constexpr int NPT = (kANodes + NT - 1) / NT;
+  constexpr bool USL = MODE == 0;
+  static_assert(4 * NQ * kACells <= (2 + NW) * CAP, "node-sum slices");
+  __shared__ T st[(2 + NW) * CAP];
+  __shared__ uint16_t tab[CAP];
+  __shared__ T nvv[4 * kAVN];  // nodal vx, vy, ax, ay at the nodes of RC's cells
+  __shared__ T colC[kStepTile + 1], colS[kStepTile + 1], rowC[kStepTile + 1],
+      rowS[kStepTile + 1];
+  __shared__ int segA[kStepTile + 1], segP[kStepTile + 2];
+  const AdjArgs<T>& a = A.a;
   const PRows<T>& R = a.R;
-  const int64_t ld = a.ld;
+  const int64_t ld = a.ld, nld = a.nld;
+  const T* __restrict__ nb = a.nb;
   T* __restrict__ cot = a.cot;
-  if ((int)blockIdx.x >= a.ncb) {
-    for (int s = cs[M.ncells] + ((int)blockIdx.x - a.ncb) * kThreads + threadIdx.x; s < a.n;
-         s += (gridDim.x - a.ncb) * kThreads) {
+  T* __restrict__ wk = a.wk;
+  T* __restrict__ na = const_cast<T*>(a.na);
+  const int* __restrict__ cs = A.sr.cs();
+  const int* __restrict__ perm = A.sr.perm();
+  const int ident = A.sr.flags[1];
+  const int t = threadIdx.x;
+  const int ntiles = A.tt.ntiles;
+  if ((int)blockIdx.x >= ntiles) {
+    for (int s = cs[A.nc_all] + ((int)blockIdx.x - ntiles) * NT + t; s < a.n;
+         s += kAdjOutsideBlocks * NT) {
       const int p = ident ? s : perm[s];
       for (int r = 0; r < DMPM_C_ROWS; ++r) CR(r)[p] = T(0);
     }
     return;
   }
-  const int c0 = blockIdx.x * CB, t = threadIdx.x;
-  const int cend = min(c0 + CB, M.ncells);
-  const int mycell = c0 + t;
-  const bool celler = t < CB && mycell < cend;
-  const int ms0 = celler ? cs[mycell] : 0, ms1 = celler ? cs[mycell + 1] : 0;
-  const int s0 = cs[c0], s1 = cs[cend];
-  // the cell's inverse-Jacobian scales and shape-function gradients at the
-  // centroid
-  T ix = T(0), iy = T(0), cx[4], cy[4];
-  if (celler) {
-    const int cs_ = M.sim_c(mycell);
-    const Grid2<T> g = M.grid(cs_);
-    const int lc = mycell - M.coff(cs_);
-    const int I = lc % g.nx, J = lc / g.nx;
-    q4_invJ(T(0), T(0), g.X(I), g.X(I + 1), g.Y(J), g.Y(J + 1), ix, iy);
-    for (int k = 0; k < 4; ++k) q4_dNdx(k, T(0), T(0), ix, iy, cx[k], cy[k]);
+  const TileCtx<T> c = tile_ctx_of<T, B>(A.tt, xcd_pos(blockIdx.x, ntiles), true);
+  const Grid2<T> g = c.g;
+  const int nx = g.nx, ny = g.ny;
+  const Rgn R0{c.ti * kStepTile, min(c.ti * kStepTile + kStepTile, nx), c.tj * kStepTile,
+               min(c.tj * kStepTile + kStepTile, ny)};
+  const Rgn RC{max(R0.i0 - 1, 0), R0.i1, max(R0.j0 - 1, 0), R0.j1};
+  const Rgn NR = owned_nodes(R0, nx, ny);
+  const int hC = RC.j1 - RC.j0, wC = RC.i1 - RC.i0;
+  const int wV = wC + 1, nV = wV * (hC + 1);
+  const int oi = R0.i0 - RC.i0, oj = R0.j0 - RC.j0;
+  const int nnodes = A.nn_all;
+  int cr_[CC], ccol[CC], cs0[CC], cs1[CC];
+  cell_slots_rc<CC, NT>(RC, nx, c.coff, cs, cr_, ccol, cs0, cs1);
+  if (t < 64) row_segments(RC, nx, c.coff, cs, segA, segP);
+  else if (t < 128) cell_tables(g, RC, colC, colS, rowC, rowS);
+  {
+    const SmallDiv dwv = small_div(wV);
+    for (int ln = t; ln < nV; ln += NT) {
+      const int jr = sdiv(dwv, ln);
+      const int nd = c.noff + (RC.j0 + jr) * (nx + 1) + RC.i0 + (ln - jr * wV);
+      nvv[ln] = NR(DMPM_N_V)[nd];
+      nvv[kAVN + ln] = NR(DMPM_N_V + 1)[nd];
+      nvv[2 * kAVN + ln] = NR(DMPM_N_A)[nd];
+      nvv[3 * kAVN + ln] = NR(DMPM_N_A + 1)[nd];
+    }
   }
-  T acc[4 * NQ];
-  for (int v = 0; v < 4 * NQ; ++v) acc[v] = T(0);
-  for (int k0 = s0; k0 < s1; k0 += kSlotsA) {
-    const int kend = min(k0 + kSlotsA, s1);
-    for (int s = k0 + t; s < kend; s += kThreads) {
-      const int p = ident ? s : perm[s];
-      const int qq = s - k0;
-      const PGeo<T> q = pgeo(M, p, R.x[p], R.y[p]);
-      st[0][qq] = q.xi;
-      st[1][qq] = q.eta;
-      if (MODE == 2) {
-        T w[6];
-        adj_force_particle<T, true>(M, R, a.O, ld, a.na, a.vkin, cot, a.wk, a.lebar, a.mat,
-                                    a.Gx, a.Gy, a.dt, p, q, w);
-        for (int r = 0; r < 6; ++r) st[2 + r][qq] = w[r];
-      } else {
-        T w[10];
-        adj_g2p_particle<T, MODE == 0>(M, R, ld, a.nb, a.nld, cot, a.wk, a.lebar, a.mat, a.dt,
-                                       a.vu, p, q, w);
-        for (int r = 0; r < NW; ++r) st[2 + r][qq] = w[W0 + r];
+  // the owned nodes' rows the node pass reads, loaded ahead
+  const int wn = NR.i1 - NR.i0, nn = wn * (NR.j1 - NR.j0);
+  const SmallDiv dwn = small_div(wn);
+  int ndx[NPT], mask[NPT];
+  T nm[NPT], nF[NPT][2], nv1[NPT][2], na1[NPT][2], vk1[NPT][2];
+#pragma unroll
+  for (int w = 0; w < NPT; ++w) {
+    const int ln = min(t + w * NT, nn - 1);
+    const int jr = sdiv(dwn, ln);
+    const int nd = c.noff + (NR.j0 + jr) * (nx + 1) + NR.i0 + (ln - jr * wn);
+    ndx[w] = nd;
+    const int cr = A.cidx ? A.cidx[nd] : -1;
+    mask[w] = cr >= 0 ? A.cmask[cr] : 0;
+    nm[w] = NR(DMPM_N_MASS)[nd];
+#pragma unroll
+    for (int d = 0; d < 2; ++d) {
+      nF[w][d] = (NR(DMPM_N_FINT + d)[nd] + NR(DMPM_N_FEXT + d)[nd]) + NR(DMPM_N_FDAMP + d)[nd];
+      nv1[w][d] = NR(DMPM_N_V + d)[nd];
+      na1[w][d] = NR(DMPM_N_A + d)[nd];
+      vk1[w][d] = USL ? a.vkin[(int64_t)d * nnodes + nd] : T(0);
+    }
+  }
+  __syncthreads();
+  const int S = segP[hC];
+  int cv0[CC], cv1[CC];
+  cell_ranges_of<CC>(cr_, cs0, cs1, segA, segP, cv0, cv1);
+  // per cell: inverse-Jacobian scales and the shape-function gradients at
+  // the centroid (USL's strain-rate terms)
+  T cix[CC], ciy[CC], cx[CC][4], cy[CC][4];
+#pragma unroll
+  for (int u = 0; u < CC; ++u) {
+    cix[u] = ciy[u] = T(0);
+    for (int k = 0; k < 4; ++k) cx[u][k] = cy[u][k] = T(0);
+    if (USL && cr_[u] >= 0) {
+      cix[u] = colS[ccol[u]];
+      ciy[u] = rowS[cr_[u]];
+      for (int k = 0; k < 4; ++k) q4_dNdx(k, T(0), T(0), cix[u], ciy[u], cx[u][k], cy[u][k]);
+    }
+  }
+  T acc[CC][4 * NQ];
+#pragma unroll
+  for (int u = 0; u < CC; ++u)
+#pragma unroll
+    for (int v = 0; v < 4 * NQ; ++v) acc[u][v] = T(0);
+  for (int v0 = 0; v0 < S; v0 += CAP) {
+    const int vend = min(v0 + CAP, S);
+    if (v0 > 0) __syncthreads();
+    fill_slot_cells<CC>(tab, cr_, ccol, cv0, cv1, v0, vend);
+    __syncthreads();
+    if constexpr (USL) {
+      // one slot at a time (the G2P and stress adjoint of several slots
+      // held together spill registers)
+#pragma unroll 1
+      for (int j = 0; j < RPT; ++j) {
+        const int q = t + j * NT;
+        if (v0 + q >= vend) break;
+        const int code = tab[q];
+        const int r = code >> 8, ci = code & 255;
+        const int s = segA[r] + (v0 + q - segP[r]);
+        const int p = ident ? s : perm[s];
+        const bool mine = r >= oj && ci >= oi;
+        const PGeo<T> gq = tile_geo(R.x[p], R.y[p], ci, r, wV, colC, colS, rowC, rowS);
+        T vx[4], vy[4], ax[4], ay[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          vx[k] = nvv[gq.ids[k]];
+          vy[k] = nvv[kAVN + gq.ids[k]];
+          ax[k] = nvv[2 * kAVN + gq.ids[k]];
+          ay[k] = nvv[3 * kAVN + gq.ids[k]];
+        }
+        const T sb[3] = {CR(DMPM_C_SXX)[p], CR(DMPM_C_SYY)[p], CR(DMPM_C_SXY)[p]};
+        T w[10], Vb_in = T(0), sxi, seta, dummy = T(0);
+        adj_g2p_core<T, true>(c.mat, a.mat.per_E ? R.E[p] : T(0), R.V[p], a.dt, a.vu, gq,
+                              CR(DMPM_C_X)[p], CR(DMPM_C_X + 1)[p], CR(DMPM_C_V)[p],
+                              CR(DMPM_C_V + 1)[p], vx, vy, ax, ay, sb, CR(DMPM_C_VOL)[p],
+                              (mine && a.lebar) ? a.lebar + p : &dummy, Vb_in, sxi, seta, w);
+        if (mine) {
+          if (boundary_cell(RC.i0 + ci, RC.j0 + r)) WR(W_BVOL)[p] = Vb_in;
+          else CR(DMPM_C_VOL)[p] = Vb_in;
+          WR(W_XI)[p] = sxi;
+          WR(W_ETA)[p] = seta;
+        }
+        st[q] = gq.xi;
+        st[CAP + q] = gq.eta;
+#pragma unroll
+        for (int r2 = 0; r2 < NW; ++r2) st[(2 + r2) * CAP + q] = w[W_NVX + r2];
+      }
+    } else {
+      int pp[RPT], code[RPT];
+      unsigned own = 0;
+  #pragma unroll
+      for (int j = 0; j < RPT; ++j) {
+        const int q = t + j * NT;
+        const bool live = v0 + q < vend;
+        code[j] = live ? (int)tab[q] : 0;
+        const int r = code[j] >> 8, ci = code[j] & 255;
+        const int s = live ? segA[r] + (v0 + q - segP[r]) : 0;
+        if (live && r >= oj && ci >= oi) own |= 1u << j;
+        pp[j] = ident ? s : perm[s];
+      }
+      T x[RPT], y[RPT], xbx[RPT], xby[RPT], vbx[RPT], vby[RPT];
+      T sbx[RPT], sby[RPT], sbq[RPT], vb[RPT], Ep[RPT], Vin[RPT];
+  #pragma unroll
+      for (int j = 0; j < RPT; ++j) {
+        const int p = pp[j];
+        x[j] = R.x[p];
+        y[j] = R.y[p];
+        xbx[j] = CR(DMPM_C_X)[p];
+        xby[j] = CR(DMPM_C_X + 1)[p];
+        vbx[j] = CR(DMPM_C_V)[p];
+        vby[j] = CR(DMPM_C_V + 1)[p];
+        if (USL) {
+          sbx[j] = CR(DMPM_C_SXX)[p];
+          sby[j] = CR(DMPM_C_SYY)[p];
+          sbq[j] = CR(DMPM_C_SXY)[p];
+          vb[j] = CR(DMPM_C_VOL)[p];
+          Vin[j] = R.V[p];
+          Ep[j] = a.mat.per_E ? R.E[p] : T(0);
+        }
+      }
+  #pragma unroll
+      for (int j = 0; j < RPT; ++j) {
+        const int q = t + j * NT;
+        if (v0 + q >= vend) continue;
+        const int p = pp[j];
+        const PGeo<T> gq = tile_geo(x[j], y[j], code[j] & 255, code[j] >> 8, wV, colC, colS,
+                                    rowC, rowS);
+        T vx[4], vy[4], ax[4], ay[4];
+  #pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          vx[k] = nvv[gq.ids[k]];
+          vy[k] = nvv[kAVN + gq.ids[k]];
+          ax[k] = nvv[2 * kAVN + gq.ids[k]];
+          ay[k] = nvv[3 * kAVN + gq.ids[k]];
+        }
+        const bool mine = own & (1u << j);
+        T sb[3] = {T(0), T(0), T(0)}, Vb = T(0);
+        if (USL) {
+          sb[0] = sbx[j];
+          sb[1] = sby[j];
+          sb[2] = sbq[j];
+          Vb = vb[j];
+        }
+        T w[10], Vb_in = T(0), sxi, seta, dummy = T(0);
+        adj_g2p_core<T, USL>(c.mat, USL ? Ep[j] : T(0), USL ? Vin[j] : T(0), a.dt, a.vu, gq,
+                             xbx[j], xby[j], vbx[j], vby[j], vx, vy, ax, ay, sb, Vb,
+                             (mine && a.lebar) ? a.lebar + p : &dummy, Vb_in, sxi, seta, w);
+        if (mine) {
+          if (USL) {
+            if (boundary_cell(RC.i0 + gq.I, RC.j0 + gq.J)) WR(W_BVOL)[p] = Vb_in;
+            else CR(DMPM_C_VOL)[p] = Vb_in;
+          }
+          WR(W_XI)[p] = sxi;
+          WR(W_ETA)[p] = seta;
+        }
+        st[q] = gq.xi;
+        st[CAP + q] = gq.eta;
+  #pragma unroll
+        for (int r = 0; r < NW; ++r) st[(2 + r) * CAP + q] = w[W_NVX + r];
       }
     }
     __syncthreads();
-    if (celler) {
-      const int lo = max(ms0, k0) - k0, hi = min(ms1, kend) - k0;
+#pragma unroll
+    for (int u = 0; u < CC; ++u) {
+      if (cr_[u] < 0) continue;
+      const int lo = max(cv0[u], v0) - v0, hi = min(cv1[u], vend) - v0;
       for (int qq = lo; qq < hi; ++qq) {
-        const T xi = st[0][qq], eta = st[1][qq];
+        const T xi = st[qq], eta = st[CAP + qq];
         T N[4];
         q4_N4(xi, eta, N);
-#define WQ(r) st[2 + (r) - W0][qq]
+#define WQ(r) st[(2 + (r)) * CAP + qq]
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
           if (MODE == 1) {
-            acc[4 * k + 0] = acc[4 * k + 0] + N[k] * WQ(W_NVX);
-            acc[4 * k + 1] = acc[4 * k + 1] + N[k] * WQ(W_NVY);
-            acc[4 * k + 2] = acc[4 * k + 2] + N[k] * WQ(W_NAX);
-            acc[4 * k + 3] = acc[4 * k + 3] + N[k] * WQ(W_NAY);
+            acc[u][4 * k + 0] = acc[u][4 * k + 0] + N[k] * WQ(W_NVX);
+            acc[u][4 * k + 1] = acc[u][4 * k + 1] + N[k] * WQ(W_NVY);
+            acc[u][4 * k + 2] = acc[u][4 * k + 2] + N[k] * WQ(W_NAX);
+            acc[u][4 * k + 3] = acc[u][4 * k + 3] + N[k] * WQ(W_NAY);
             continue;
           }
           T dx, dy;
-          q4_dNdx(k, xi, eta, ix, iy, dx, dy);
-          if (MODE == 2) {
-            acc[2 * k + 0] =
-                acc[2 * k + 0] + ((dx * WQ(W_L00) + dy * WQ(W_L10)) + cx[k] * WQ(W_C00));
-            acc[2 * k + 1] =
-                acc[2 * k + 1] + ((dy * WQ(W_L11) + dx * WQ(W_L01)) + cy[k] * WQ(W_C11));
-            continue;
-          }
-          const T q0 = ((N[k] * WQ(W_NVX) + dx * WQ(W_L00)) + dy * WQ(W_L10)) + cx[k] * WQ(W_C00);
-          const T q1 = ((N[k] * WQ(W_NVY) + dy * WQ(W_L11)) + dx * WQ(W_L01)) + cy[k] * WQ(W_C11);
-          acc[4 * k + 0] = acc[4 * k + 0] + q0;
-          acc[4 * k + 1] = acc[4 * k + 1] + q1;
-          acc[4 * k + 2] = acc[4 * k + 2] + N[k] * WQ(W_NAX);
-          acc[4 * k + 3] = acc[4 * k + 3] + N[k] * WQ(W_NAY);
+          q4_dNdx(k, xi, eta, cix[u], ciy[u], dx, dy);
+          const T q0 =
+              ((N[k] * WQ(W_NVX) + dx * WQ(W_L00)) + dy * WQ(W_L10)) + cx[u][k] * WQ(W_C00);
+          const T q1 =
+              ((N[k] * WQ(W_NVY) + dy * WQ(W_L11)) + dx * WQ(W_L01)) + cy[u][k] * WQ(W_C11);
+          acc[u][4 * k + 0] = acc[u][4 * k + 0] + q0;
+          acc[u][4 * k + 1] = acc[u][4 * k + 1] + q1;
+          acc[u][4 * k + 2] = acc[u][4 * k + 2] + N[k] * WQ(W_NAX);
+          acc[u][4 * k + 3] = acc[u][4 * k + 3] + N[k] * WQ(W_NAY);
         }
 #undef WQ
       }
     }
-    __syncthreads();
   }
-  if (celler)
-    for (int v = 0; v < 4 * NQ; ++v) part[(int64_t)v * M.ncells + mycell] = acc[v];
-}
-
-// Kinematics adjoint at one node and direction (element.py:191-202 with
-// constraint.py:29-44): v = snap(snap(P)/m) (or the constraint value).
-template <typename T>
-__device__ __forceinline__ void kin_adjoint(T m, T v, bool constrained, T vb, T& Pb, T& mb) {
-  Pb = T(0);
-  if (constrained || m == T(0) || v == T(0)) return;
-  Pb = vb / m;
-  mb = mb - vb * v / m;
-}
-
-// Node pass 1: nodal-velocity/acceleration cotangents of G2P, adjoint of the
-// nodal update (element.py:305-329): F = f_int + f_ext + f_damp,
-// a0 = nan_to_num(F/m), v = snap(v_kin + a0 dt), a = snap(a0), constraints.
-// USL: the kinematic velocity's cotangent is then complete and the
-// kinematics adjoint follows.
-template <typename T, bool USL>
-__global__ __launch_bounds__(kThreads) void k_adj_node1(
-    Mesh2<T> M, const T* __restrict__ part, const T* __restrict__ nb, int64_t nld,
-    const T* __restrict__ vkin, T* __restrict__ na, const int* __restrict__ cidx,
-    const int* __restrict__ cmask, T dt) {
-  const int nd = blockIdx.x * blockDim.x + threadIdx.x;
-  if (nd >= M.nnodes) return;
-  const int sm = M.sim_n(nd);
-  const Grid2<T> g = M.grid(sm);
-  const int nl = nd - M.noff(sm);
-  const int i = nl % (g.nx + 1), j = nl / (g.nx + 1);
-  T acc[4];
-  node_gather<T, 4>(g, M.coff(sm), M.ncells, part, i, j, acc);
-  const int cr = cidx ? cidx[nd] : -1;
-  const int mask = cr >= 0 ? cmask[cr] : 0;
-  const T m = NR(DMPM_N_MASS)[nd];
-  T mb = T(0);
-  for (int d = 0; d < 2; ++d) {
-    T Fb = T(0), vkb = T(0);
-    if (!(mask & (1 << d))) {
-      const T F = (NR(DMPM_N_FINT + d)[nd] + NR(DMPM_N_FEXT + d)[nd]) + NR(DMPM_N_FDAMP + d)[nd];
-      const T r = F / m;
-      const T v1b = NR(DMPM_N_V + d)[nd] != T(0) ? acc[d] : T(0);
-      const T a0b = (NR(DMPM_N_A + d)[nd] != T(0) ? acc[2 + d] : T(0)) + dt * v1b;
-      if (isfinite(r)) {
-        Fb = a0b / m;
-        mb = mb - a0b * r / m;
+  __syncthreads();
+  T nacc[NPT][NQ];
+  node_sums_nt<T, NQ, CC, NPT, NT>(st, acc, RC, NR, nx, ny, nacc);
+  // node pass 1 of the owned nodes: adjoint of the nodal update
+#pragma unroll
+  for (int w = 0; w < NPT; ++w) {
+    if (t + w * NT >= nn) continue;
+    const int nd = ndx[w];
+    const T m = nm[w];
+    T mb = T(0);
+#pragma unroll
+    for (int d = 0; d < 2; ++d) {
+      T Fb = T(0), vkb = T(0);
+      if (!(mask[w] & (1 << d))) {
+        const T r = nF[w][d] / m;
+        const T v1b = nv1[w][d] != T(0) ? nacc[w][d] : T(0);
+        const T a0b = (na1[w][d] != T(0) ? nacc[w][2 + d] : T(0)) + a.dt * v1b;
+        if (isfinite(r)) {
+          Fb = a0b / m;
+          mb = mb - a0b * r / m;
+        }
+        vkb = v1b;
       }
-      vkb = v1b;
+      na[(int64_t)(A_FX + d) * nnodes + nd] = Fb;
+      if (USL) {
+        T Pb;
+        kin_adjoint(m, vk1[w][d], (mask[w] >> d) & 1, vkb, Pb, mb);
+        na[(int64_t)(A_PX + d) * nnodes + nd] = Pb;
+      } else {
+        na[(int64_t)(A_VKX + d) * nnodes + nd] = vkb;
+      }
     }
-    AR(A_FX + d)[nd] = Fb;
-    if (USL) {
-      T Pb;
-      kin_adjoint(m, vkin[(int64_t)d * M.nnodes + nd], (mask >> d) & 1, vkb, Pb, mb);
-      AR(A_PX + d)[nd] = Pb;
-    } else {
-      AR(A_VKX + d)[nd] = vkb;
-    }
+    na[(int64_t)A_M * nnodes + nd] = mb;
   }
-  AR(A_M)[nd] = mb;
 }
 
-// Node pass 2 (USF): the kinematic velocity's cotangent from the nodal update
-// plus the strain-rate scatter; kinematics adjoint.
-template <typename T>
-__global__ __launch_bounds__(kThreads) void k_adj_node2(Mesh2<T> M, const T* __restrict__ part,
-                                                        const T* __restrict__ nb, int64_t nld,
-                                                        const T* __restrict__ vkin,
-                                                        T* __restrict__ na,
-                                                        const int* __restrict__ cidx,
-                                                        const int* __restrict__ cmask) {
-  const int nd = blockIdx.x * blockDim.x + threadIdx.x;
-  if (nd >= M.nnodes) return;
-  const int sm = M.sim_n(nd);
-  const Grid2<T> g = M.grid(sm);
-  const int nl = nd - M.noff(sm);
-  const int i = nl % (g.nx + 1), j = nl / (g.nx + 1);
-  T acc[2];
-  node_gather<T, 2>(g, M.coff(sm), M.ncells, part, i, j, acc);
-  const int cr = cidx ? cidx[nd] : -1;
-  const int mask = cr >= 0 ? cmask[cr] : 0;
-  const T m = NR(DMPM_N_MASS)[nd];
-  T mb = AR(A_M)[nd];
-  for (int d = 0; d < 2; ++d) {
-    const T vb = AR(A_VKX + d)[nd] + acc[d];
-    T Pb;
-    kin_adjoint(m, vkin[(int64_t)d * M.nnodes + nd], (mask >> d) & 1, vb, Pb, mb);
-    AR(A_PX + d)[nd] = Pb;
+template <typename T, bool B, int NT = AdjTileShape<2>::NT, int RPT = AdjTileShape<2>::RPT,
+          int WPE = AdjTileShape<2>::WPE>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) void k_adj_tile2(
+    AdjTile<T> A) {
+  constexpr int CAP = NT * RPT;
+  constexpr int NW = 5;  // strain-rate cotangents L00, L11, L01 (= L10), C00, C11
+  constexpr int NQ = 2;
+  constexpr int CC = (kACells + NT - 1) / NT;
+  constexpr int NPT = (kANodes + NT - 1) / NT;
+  static_assert(4 * NQ * kACells <= (2 + NW) * CAP, "node-sum slices");
+  __shared__ T st[(2 + NW) * CAP];
+  __shared__ uint16_t tab[CAP];
+  __shared__ T nvv[4 * kAVN];  // force cotangents fx, fy and kinematic velocity vx, vy
+  __shared__ T colC[kStepTile + 1], colS[kStepTile + 1], rowC[kStepTile + 1],
+      rowS[kStepTile + 1];
+  __shared__ int segA[kStepTile + 1], segP[kStepTile + 2];
+  const AdjArgs<T>& a = A.a;
+  const PRows<T>& R = a.R;
+  const POut<T>& O = a.O;
+  const int64_t ld = a.ld, nld = a.nld;
+  const T* __restrict__ nb = a.nb;
+  T* __restrict__ cot = a.cot;
+  T* __restrict__ wk = a.wk;
+  T* __restrict__ na = const_cast<T*>(a.na);
+  const int* __restrict__ cs = A.sr.cs();
+  const int* __restrict__ perm = A.sr.perm();
+  const int ident = A.sr.flags[1];
+  const int t = threadIdx.x;
+  const int ntiles = A.tt.ntiles;
+  const TileCtx<T> c = tile_ctx_of<T, B>(A.tt, xcd_pos(blockIdx.x, ntiles), true);
+  const Grid2<T> g = c.g;
+  const int nx = g.nx, ny = g.ny;
+  const Rgn R0{c.ti * kStepTile, min(c.ti * kStepTile + kStepTile, nx), c.tj * kStepTile,
+               min(c.tj * kStepTile + kStepTile, ny)};
+  const Rgn RC{max(R0.i0 - 1, 0), R0.i1, max(R0.j0 - 1, 0), R0.j1};
+  const Rgn NR = owned_nodes(R0, nx, ny);
+  const int hC = RC.j1 - RC.j0, wC = RC.i1 - RC.i0;
+  const int wV = wC + 1, nV = wV * (hC + 1);
+  const int oi = R0.i0 - RC.i0, oj = R0.j0 - RC.j0;
+  const int nnodes = A.nn_all;
+  int cr_[CC], ccol[CC], cs0[CC], cs1[CC];
+  cell_slots_rc<CC, NT>(RC, nx, c.coff, cs, cr_, ccol, cs0, cs1);
+  if (t < 64) row_segments(RC, nx, c.coff, cs, segA, segP);
+  else if (t < 128) cell_tables(g, RC, colC, colS, rowC, rowS);
+  {
+    const SmallDiv dwv = small_div(wV);
+    for (int ln = t; ln < nV; ln += NT) {
+      const int jr = sdiv(dwv, ln);
+      const int nd = c.noff + (RC.j0 + jr) * (nx + 1) + RC.i0 + (ln - jr * wV);
+      nvv[ln] = na[(int64_t)A_FX * nnodes + nd];
+      nvv[kAVN + ln] = na[(int64_t)A_FY * nnodes + nd];
+      nvv[2 * kAVN + ln] = a.vkin[nd];
+      nvv[3 * kAVN + ln] = a.vkin[(int64_t)nnodes + nd];
+    }
   }
-  AR(A_M)[nd] = mb;
+  const int wn = NR.i1 - NR.i0, nn = wn * (NR.j1 - NR.j0);
+  const SmallDiv dwn = small_div(wn);
+  int ndx[NPT], mask[NPT];
+  T nm[NPT], nmb[NPT], nvb[NPT][2], vk1[NPT][2];
+#pragma unroll
+  for (int w = 0; w < NPT; ++w) {
+    const int ln = min(t + w * NT, nn - 1);
+    const int jr = sdiv(dwn, ln);
+    const int nd = c.noff + (NR.j0 + jr) * (nx + 1) + NR.i0 + (ln - jr * wn);
+    ndx[w] = nd;
+    const int cr = A.cidx ? A.cidx[nd] : -1;
+    mask[w] = cr >= 0 ? A.cmask[cr] : 0;
+    nm[w] = NR(DMPM_N_MASS)[nd];
+    nmb[w] = na[(int64_t)A_M * nnodes + nd];
+#pragma unroll
+    for (int d = 0; d < 2; ++d) {
+      nvb[w][d] = na[(int64_t)(A_VKX + d) * nnodes + nd];
+      vk1[w][d] = a.vkin[(int64_t)d * nnodes + nd];
+    }
+  }
+  __syncthreads();
+  const int S = segP[hC];
+  int cv0[CC], cv1[CC];
+  cell_ranges_of<CC>(cr_, cs0, cs1, segA, segP, cv0, cv1);
+  T cix[CC], ciy[CC], cx[CC][4], cy[CC][4];
+#pragma unroll
+  for (int u = 0; u < CC; ++u) {
+    cix[u] = ciy[u] = T(0);
+    for (int k = 0; k < 4; ++k) cx[u][k] = cy[u][k] = T(0);
+    if (cr_[u] >= 0) {
+      cix[u] = colS[ccol[u]];
+      ciy[u] = rowS[cr_[u]];
+      for (int k = 0; k < 4; ++k) q4_dNdx(k, T(0), T(0), cix[u], ciy[u], cx[u][k], cy[u][k]);
+    }
+  }
+  T acc[CC][4 * NQ];
+#pragma unroll
+  for (int u = 0; u < CC; ++u)
+#pragma unroll
+    for (int v = 0; v < 4 * NQ; ++v) acc[u][v] = T(0);
+  for (int v0 = 0; v0 < S; v0 += CAP) {
+    const int vend = min(v0 + CAP, S);
+    if (v0 > 0) __syncthreads();
+    fill_slot_cells<CC>(tab, cr_, ccol, cv0, cv1, v0, vend);
+    __syncthreads();
+    // one slot at a time (the force and stress adjoint of several slots held
+    // together spill registers)
+#pragma unroll 1
+    for (int j = 0; j < RPT; ++j) {
+      const int q = t + j * NT;
+      if (v0 + q >= vend) break;
+      const int code = tab[q];
+      const int r = code >> 8, ci = code & 255;
+      const int s = segA[r] + (v0 + q - segP[r]);
+      const int p = ident ? s : perm[s];
+      const bool mine = r >= oj && ci >= oi;
+      const PGeo<T> gq = tile_geo(R.x[p], R.y[p], ci, r, wV, colC, colS, rowC, rowS);
+      T fx[4], fy[4], vkx[4], vky[4];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        fx[k] = nvv[gq.ids[k]];
+        fy[k] = nvv[kAVN + gq.ids[k]];
+        vkx[k] = nvv[2 * kAVN + gq.ids[k]];
+        vky[k] = nvv[3 * kAVN + gq.ids[k]];
+      }
+      const T csv[3] = {CR(DMPM_C_SXX)[p], CR(DMPM_C_SYY)[p], CR(DMPM_C_SXY)[p]};
+      T sb[3], vol, w[6], sxi, seta, dummy = T(0);
+      adj_force_core<T, true>(c.mat, a.mat.per_E ? R.E[p] : T(0), a.dt, R.V[p], O.V[p],
+                              O.sxx[p], O.syy[p], O.sxy[p], R.m[p], gq, a.Gx, a.Gy, fx, fy,
+                              vkx, vky, csv, CR(DMPM_C_VOL)[p],
+                              (mine && a.lebar) ? a.lebar + p : &dummy, sb, vol, w, sxi, seta);
+      if (mine) {
+        if (boundary_cell(RC.i0 + ci, RC.j0 + r)) {
+          WR(W_BSXX)[p] = sb[0];
+          WR(W_BSYY)[p] = sb[1];
+          WR(W_BSXY)[p] = sb[2];
+          WR(W_BVOL)[p] = vol;
+        } else {
+          CR(DMPM_C_SXX)[p] = sb[0];
+          CR(DMPM_C_SYY)[p] = sb[1];
+          CR(DMPM_C_SXY)[p] = sb[2];
+          CR(DMPM_C_VOL)[p] = vol;
+        }
+        WR(W_XI)[p] = WR(W_XI)[p] + sxi;
+        WR(W_ETA)[p] = WR(W_ETA)[p] + seta;
+      }
+      // the strain-rate cotangents (L01 and L10 are the same number)
+      st[q] = gq.xi;
+      st[CAP + q] = gq.eta;
+      st[2 * CAP + q] = w[W_L00 - W_L00];
+      st[3 * CAP + q] = w[W_L11 - W_L00];
+      st[4 * CAP + q] = w[W_L01 - W_L00];
+      st[5 * CAP + q] = w[W_C00 - W_L00];
+      st[6 * CAP + q] = w[W_C11 - W_L00];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < CC; ++u) {
+      if (cr_[u] < 0) continue;
+      const int lo = max(cv0[u], v0) - v0, hi = min(cv1[u], vend) - v0;
+      for (int qq = lo; qq < hi; ++qq) {
+        const T xi = st[qq], eta = st[CAP + qq];
+        const T l00 = st[2 * CAP + qq], l11 = st[3 * CAP + qq], l01 = st[4 * CAP + qq];
+        const T c00 = st[5 * CAP + qq], c11 = st[6 * CAP + qq];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          T dx, dy;
+          q4_dNdx(k, xi, eta, cix[u], ciy[u], dx, dy);
+          acc[u][2 * k + 0] = acc[u][2 * k + 0] + ((dx * l00 + dy * l01) + cx[u][k] * c00);
+          acc[u][2 * k + 1] = acc[u][2 * k + 1] + ((dy * l11 + dx * l01) + cy[u][k] * c11);
+        }
+      }
+    }
+  }
+  __syncthreads();
+  T nacc[NPT][NQ];
+  node_sums_nt<T, NQ, CC, NPT, NT>(st, acc, RC, NR, nx, ny, nacc);
+  // node pass 2 of the owned nodes: kinematics adjoint
+#pragma unroll
+  for (int w = 0; w < NPT; ++w) {
+    if (t + w * NT >= nn) continue;
+    const int nd = ndx[w];
+    T mb = nmb[w];
+#pragma unroll
+    for (int d = 0; d < 2; ++d) {
+      const T vb = nvb[w][d] + nacc[w][d];
+      T Pb;
+      kin_adjoint(nm[w], vk1[w][d], (mask[w] >> d) & 1, vb, Pb, mb);
+      na[(int64_t)(A_PX + d) * nnodes + nd] = Pb;
+    }
+    na[(int64_t)A_M * nnodes + nd] = mb;
+  }
 }
 
 template <typename T, int CB>
@@ -644,17 +1035,15 @@ int adjoint_kernels(dmpm_plan* P, const Mesh2<T>& M, const PRows<T>& R, const PO
   const int n = (int)P->n;
   const bool usf = scheme == DMPM_USF;
   ProfScopeT ps_(P, KC_ADJ, s);
-  T* part = (T*)plan_workspace(P, 0, (size_t)16 * M.ncells * sizeof(T));
   T* wk = (T*)plan_workspace(P, 3, (size_t)W_ROWS * ld * sizeof(T));
   T* na = (T*)plan_workspace(P, 4, (size_t)A_ROWS * P->nnodes * sizeof(T));
-  if (!part || !vkin || !wk || !na) return fail(DMPM_EHIP, "workspace allocation failed");
+  if (!vkin || !wk || !na) return fail(DMPM_EHIP, "workspace allocation failed");
   Mat2 mat;
   mat.kind = P->d.material;
   mat.per_E = P->d.per_particle_E;
   for (int k = 0; k < 3; ++k) mat.de[k] = P->d.de[k];
   const T tdt = T(dt), Gx = T(grav[0]), Gy = T(grav[1]);
-  const unsigned np = nblk(n), nn = nblk(P->nnodes);
-  const unsigned ncb = (unsigned)((M.ncells + CB - 1) / CB);
+  const unsigned np = nblk(n);
   AdjArgs<T> aa;
   aa.R = R;
   aa.O = O;
@@ -672,26 +1061,39 @@ int adjoint_kernels(dmpm_plan* P, const Mesh2<T>& M, const PRows<T>& R, const PO
   aa.dt = tdt;
   aa.vu = vu;
   aa.n = n;
-  aa.ncb = (int)ncb;
-  constexpr int kOutsideBlocks = 8;
-  if (usf)
-    k_adj_scatter<T, CB, 1><<<ncb + kOutsideBlocks, kThreads, 0, s>>>(M, aa, sr, part);
-  else
-    k_adj_scatter<T, CB, 0><<<ncb + kOutsideBlocks, kThreads, 0, s>>>(M, aa, sr, part);
-  if (usf)
-    k_adj_node1<T, false><<<nn, kThreads, 0, s>>>(M, part, nb, nld, vkin, na, P->cidx, P->cmask,
-                                                   tdt);
-  else
-    k_adj_node1<T, true><<<nn, kThreads, 0, s>>>(M, part, nb, nld, vkin, na, P->cidx, P->cmask,
-                                                  tdt);
+  aa.ncb = 0;
+  AdjTile<T> at;
+  at.a = aa;
+  at.tt.sims = P->sims;
+  at.tt.tile_sim = P->tile_sim;
+  at.tt.tiles_x = P->sims_h[0].tiles_x;
+  at.tt.ntiles = P->ntiles;
+  at.tt.g = M.g0;
+  at.tt.mat = mat;
+  at.sr = sr;
+  at.cidx = P->cidx;
+  at.cmask = P->cmask;
+  at.nn_all = P->nnodes;
+  at.nc_all = P->ncells;
+  const unsigned nt = (unsigned)P->ntiles;
+  const bool batch = P->nsims > 1;
   if (usf) {
-    k_adj_scatter<T, CB, 2><<<ncb, kThreads, 0, s>>>(M, aa, sr, part);
-    k_adj_node2<T><<<nn, kThreads, 0, s>>>(M, part, nb, nld, vkin, na, P->cidx, P->cmask);
+    constexpr int NT1 = AdjTileShape<1>::NT, NT2 = AdjTileShape<2>::NT;
+    if (batch) {
+      k_adj_tile1<T, true, 1><<<nt + kAdjOutsideBlocks, NT1, 0, s>>>(at);
+      k_adj_tile2<T, true><<<nt, NT2, 0, s>>>(at);
+    } else {
+      k_adj_tile1<T, false, 1><<<nt + kAdjOutsideBlocks, NT1, 0, s>>>(at);
+      k_adj_tile2<T, false><<<nt, NT2, 0, s>>>(at);
+    }
   } else {
+    constexpr int NT0 = AdjTileShape<0>::NT;
+    if (batch) k_adj_tile1<T, true, 0><<<nt + kAdjOutsideBlocks, NT0, 0, s>>>(at);
+    else k_adj_tile1<T, false, 0><<<nt + kAdjOutsideBlocks, NT0, 0, s>>>(at);
     k_adj_force<T><<<np, kThreads, 0, s>>>(M, R, ld, n, na, vkin, cot, wk, lebar, mat, Gx, Gy,
                                           tdt);
   }
-  k_adj_kin<T><<<np, kThreads, 0, s>>>(M, R, ld, n, na, cot, wk);
+  k_adj_kin<T><<<np, kThreads, 0, s>>>(M, R, ld, n, na, cot, wk, vu, usf ? 1 : 0);
   return launch_check("step_adjoint");
 }
 
