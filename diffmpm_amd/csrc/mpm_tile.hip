@@ -1092,10 +1092,14 @@ __global__ __launch_bounds__(kFT) __attribute__((amdgpu_waves_per_eu(4))) void k
   __shared__ T st[5 * kKCAP];  // staged particle inputs; node-pass slices alias it
   static_assert(7 * kFCAP <= 5 * kKCAP, "F staging");
   __shared__ T s1f[5 * kFCAP];  // sxx, syy, sxy, V, E_p of the R_1 slots
-  __shared__ T vk[2 * NKN];
-  __shared__ T nm[NKN];
-  __shared__ T nv[2 * NTN];
-  __shared__ T na[2 * NTN];
+  // nodal arrays; until the K epilogue they hold the slot table
+  __shared__ T ndl[3 * NKN + 4 * NTN];
+  static_assert(sizeof(T) * (3 * NKN + 4 * NTN) >= sizeof(uint16_t) * kKCAP, "slot table");
+  T* const vk = ndl;
+  T* const nm = ndl + 2 * NKN;
+  T* const nv = ndl + 3 * NKN;
+  T* const na = ndl + 3 * NKN + 2 * NTN;
+  uint16_t* const tab = reinterpret_cast<uint16_t*>(ndl);
   __shared__ T colC[NCOL], colS[NCOL], rowC[NROW], rowS[NROW];
   __shared__ int segA[3][kSegRows], segP[3][kSegRows + 1];
   const TileCtx<T> c = tile_ctx<T, B>(a, xcd_pos(blockIdx.x, a.ntiles), true);
@@ -1113,6 +1117,9 @@ __global__ __launch_bounds__(kFT) __attribute__((amdgpu_waves_per_eu(4))) void k
   const Rgn RN = USF ? R1 : R0;  // cells whose nodes get kinematic velocities
   const int h0 = R0.j1 - R0.j0, h1 = R1.j1 - R1.j0, hH = RH.j1 - RH.j0;
 
+  // row, column and slot range of this thread's cells of RH (the slot table)
+  int hr[CK], hcol[CK], hs0[CK], hs1[CK];
+  cell_slots_rc<CK, kFT>(RH, nx, c.coff, cs, hr, hcol, hs0, hs1);
   // slot segments of the rows of R0 (table 0), R1 (1) and RH (2): one wave
   // per table, lane r = row r, prefix by shuffles
   {
@@ -1167,50 +1174,37 @@ __global__ __launch_bounds__(kFT) __attribute__((amdgpu_waves_per_eu(4))) void k
   T fd[NF][2];
   node_pre<T, NK>(a, c, RN, kpre, nullptr);
   node_pre<T, NF>(a, c, R0, fpre, fd);
-  // slot of each virtual index t + j*kFT of the R_H list (lanes past the
-  // end take the last slot and are flagged off): binary searches over the
-  // rows, in lockstep
-  int pp[PPT], v1i[PPT], Ii[PPT], Ji[PPT], lo[PPT], hi[PPT];
+  // the slot table of the R_H list: every slot's cell (row << 8 | column
+  // of RH), written by the threads that own the cells
+  int kv0[CK], kv1[CK];
+  cell_ranges_of<CK>(hr, hs0, hs1, segA[2], segP[2], kv0, kv1);
+  fill_slot_cells<CK>(tab, hr, hcol, kv0, kv1, 0, SH);
+  __syncthreads();
+  // slot, cell and membership of each virtual index t + j*kFT of the R_H
+  // list (lanes past the end take particle 0 and are flagged off)
+  int pp[PPT], v1i[PPT], Ii[PPT], Ji[PPT];
   unsigned own = 0, live = 0;
 #pragma unroll
   for (int j = 0; j < PPT; ++j) {
-    lo[j] = 0;
-    hi[j] = hH - 1;
-  }
-#pragma unroll
-  for (int it = 0; it < 5; ++it)  // 2^5 >= kSegRows rows
-#pragma unroll
-    for (int j = 0; j < PPT; ++j) {
-      const int v = min(t + j * kFT, SH - 1);
-      const int mid = (lo[j] + hi[j] + 1) >> 1;
-      if (segP[2][mid] <= v) lo[j] = mid;
-      else hi[j] = mid - 1;
-    }
-  static_assert(kSegRows <= 32, "search depth");
-#pragma unroll
-  for (int j = 0; j < PPT; ++j) {
-    const int v = min(t + j * kFT, SH - 1);
-    const int r = lo[j];
-    // (clamped so that an empty tile's lanes still read a valid particle)
-    const int s = min(max(segA[2][r] + (v - segP[2][r]), 0), a.n - 1);
-    if (t + j * kFT < SH) live |= 1u << j;
+    const int v = t + j * kFT;
+    const bool lv = v < SH;
+    const int code = lv ? (int)tab[v] : 0;
+    const int r = code >> 8;
+    const int s = lv ? segA[2][r] + (v - segP[2][r]) : 0;
+    if (lv) live |= 1u << j;
     pp[j] = s;
-    const int J = RH.j0 + r;
+    // the slot's cell is the element search's result for this position (the
+    // re-sort keyed the slot by it)
+    const int I = RH.i0 + (code & 255), J = RH.j0 + r;
+    Ii[j] = I;
     Ji[j] = J;
     v1i[j] = -1;
-    const int r1 = J - R1.j0;
-    if (r1 >= 0 && r1 < h1) {
-      const int a1 = segA[1][r1];
-      if (s >= a1 && s < a1 + (segP[1][r1 + 1] - segP[1][r1])) v1i[j] = segP[1][r1] + (s - a1);
+    if (lv && in_rgn(R1, I, J)) {
+      const int r1 = J - R1.j0;
+      v1i[j] = segP[1][r1] + (s - segA[1][r1]);
     }
-    const int r0 = J - R0.j0;
-    if (r0 >= 0 && r0 < h0) {
-      const int a0 = segA[0][r0];
-      if (s >= a0 && s < a0 + (segP[0][r0 + 1] - segP[0][r0])) own |= 1u << j;
-    }
-    if (!(live & (1u << j))) v1i[j] = -1;
+    if (lv && in_rgn(R0, I, J)) own |= 1u << j;
   }
-  own &= live;
   if (!ident)
 #pragma unroll
     for (int j = 0; j < PPT; ++j) pp[j] = perm[pp[j]];
@@ -1263,8 +1257,6 @@ __global__ __launch_bounds__(kFT) __attribute__((amdgpu_waves_per_eu(4))) void k
       s1f[4 * kFCAP + q] = E[j];
     }
   }
-  int kv0[CK], kv1[CK];
-  cell_ranges<CK>(RH, nx, c.coff, cs, segA[2], segP[2], kv0, kv1);
   node_pre_mask<T, NK>(a, kpre);
   node_pre_mask<T, NF>(a, fpre);
   // ---- K ----
@@ -1272,13 +1264,7 @@ __global__ __launch_bounds__(kFT) __attribute__((amdgpu_waves_per_eu(4))) void k
 #pragma unroll
   for (int j = 0; j < PPT; ++j) {
     const int v = t + j * kFT;
-    // the row is the slot's (the re-sort keyed the slot by this position's
-    // element); the column as in element_id, with its element-0 and
-    // next-row quirks
-    const int J = Ji[j];
-    int I = last_le(x[j], nx, g.hx, g.ihx);
-    if (I < 0 || !(y[j] >= T(0)) || I == nx) I = 0;
-    Ii[j] = I;
+    const int I = Ii[j], J = Ji[j];
     const int ci = min(max(I - RH.i0, 0), NCOL - 1), rj = min(max(J - RH.j0, 0), NROW - 1);
     xi[j] = (x[j] - colC[ci]) * colS[ci];
     eta[j] = (y[j] - rowC[rj]) * rowS[rj];
