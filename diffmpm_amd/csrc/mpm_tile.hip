@@ -734,6 +734,8 @@ template <typename T> struct StepArgs {
   int vu, temps, tiles_x, ntiles, n;
   int* ovf;  // [2] overflow counts by step parity, then [2][ntiles] tile lists
   int par;   // step parity
+  int last;  // the run's last step: the node block and vkin are written (earlier steps
+             // only pass the nodal velocity on through vnew)
   int nrb;   // workgroups of k_step_rounds that process tiles
   const SimRec* sims;   // batch plans: the simulation table (tile_off, tiles_x, ...)
   const int* tile_sim;  // batch plans: the simulation of each tile
@@ -966,7 +968,7 @@ __device__ __forceinline__ void kin_epilogue(const StepArgs<T>& a, const TileCtx
       v = snap(v);
       if (mask & (1 << d)) v = T(a.aux->cval[2 * cr + d]);
       vk[d * nkn + ln] = v;
-      if (own) a.vkin[(int64_t)d * nnodes + nd] = v;
+      if (own && a.last) a.vkin[(int64_t)d * nnodes + nd] = v;
     }
   }
 }
@@ -994,7 +996,7 @@ __device__ __forceinline__ void force_epilogue(const StepArgs<T>& a, const TileC
     const bool own = (i < R0.i1 || i == nx) && (j < R0.j1 || j == ny);
     const int cr = pre.cr[w], mask = pre.mask[w];
     const T m = nm[lk];
-    if (own) NB_W(DMPM_N_MASS, nd, m);
+    if (own && a.last) NB_W(DMPM_N_MASS, nd, m);
 #pragma unroll
     for (int d = 0; d < 2; ++d) {
       T fe = nacc[w][d];
@@ -1021,14 +1023,14 @@ __device__ __forceinline__ void force_epilogue(const StepArgs<T>& a, const TileC
       }
       nv[d * ntn + ln] = v;
       na[d * ntn + ln] = ac;
-      if (own) {
+      if (own && a.last) {
         NB_W(DMPM_N_FEXT + d, nd, fe);
         NB_W(DMPM_N_FINT + d, nd, fi);
         NB_W(DMPM_N_V + d, nd, v);
         NB_W(DMPM_N_A + d, nd, ac);
         NB_W(DMPM_N_P + d, nd, pm);
-        a.vnew[(int64_t)d * nnodes + nd] = v;
       }
+      if (own) a.vnew[(int64_t)d * nnodes + nd] = v;
     }
   }
 }
@@ -2189,11 +2191,14 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) void 
         pm = mm * cv;
         ac = T(0);
       }
-      NB_W(DMPM_N_FEXT + d, nd, fe);
-      NB_W(DMPM_N_FINT + d, nd, fi);
-      NB_W(DMPM_N_V + d, nd, v);
+      if (a.last) {
+        NB_W(DMPM_N_FEXT + d, nd, fe);
+        NB_W(DMPM_N_FINT + d, nd, fi);
+        NB_W(DMPM_N_V + d, nd, v);
+        NB_W(DMPM_N_P + d, nd, pm);
+      }
+      // (G2P reads the acceleration from the node block)
       NB_W(DMPM_N_A + d, nd, ac);
-      NB_W(DMPM_N_P + d, nd, pm);
       a.vnew[(int64_t)d * nnodes + nd] = v;
     }
   }
@@ -3168,6 +3173,7 @@ int step_run(dmpm_plan* P, const Mesh2<T>& M, T* pa, T* pb2, T* pstat, int32_t* 
     a.t = T(step0 + k) * a.dt;
     a.kofs = kbase + (int)k;
     a.par = (int)(k & 1);
+    a.last = k == nsteps - 1 ? 1 : 0;
     ProfScopeT ps_(P, KC_STEP, s);
     if (P->nsims > 1)
       step_launch<T, true>(a, scheme == DMPM_USF, P->step_kernel, s);
