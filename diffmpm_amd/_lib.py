@@ -97,6 +97,8 @@ SIGNATURES = {
     "dmpm_prepare_run": [_P, c_int32, c_int32, _P],
     "dmpm_step_between": [_P, _P, _P, _P, _P, c_int32, c_int32, _DP, c_double, c_int64, c_int32,
                           _P, _P, c_int64, _P],
+    "dmpm_step_between_saved": [_P, _P, _P, _P, _P, _P, c_int32, c_int32, _DP, c_double, c_int64,
+                                c_int32, _P, _P, c_int64, _P, _P, _P, _P],
     "dmpm_step_adjoint": [_P, _P, _P, _P, _P, _P, _P, c_int32, c_int32, _DP, c_double, c_int64,
                           _P],
     "dmpm_plan_export_step": [_P, _P, _P, _P, _P],

@@ -202,26 +202,30 @@ class Differentiable:
                 sin = sin.contiguous()
             out = torch.empty((C_ROWS, self.n), dtype=self._pbuf.dtype,
                               device=self._pbuf.device)
-            nb = self._nbuf
             if save:
                 # a fresh node block per saved step, kept for its adjoint: the
                 # step writes mass, velocity, acceleration, momentum and the
-                # forces of every node; the rows it reads (the velocity a
-                # massless node keeps, the damping force) come from the
-                # previous step's block.  Its coordinate rows are not set:
-                # neither the step nor the adjoint reads them.
+                # forces of every node, reads the velocity a massless node
+                # keeps from the previous step's block and copies its damping
+                # force; its coordinate rows are not set (neither the step nor
+                # the adjoint reads them).  The kinematic nodal velocity and
+                # the sort by cell go straight to the adjoint's buffers.
                 prev = self._nodes_last if chained and self._nodes_last is not None else self._nbuf
                 nb = torch.empty_like(self._nbuf)
-                nb[_lib.N_V:_lib.N_A] = prev[_lib.N_V:_lib.N_A]
-                nb[_lib.N_FDAMP:_lib.N_ROWS] = prev[_lib.N_FDAMP:_lib.N_ROWS]
+                vkin, perm, cs = self._export_buffers()
+                call("dmpm_step_between_saved", plan.h, ptr(self._blocks[src]),
+                     ptr(self._blocks[dst]), ptr(self._eid_b), ptr(prev), ptr(nb), self.code,
+                     self.vu, plan._gravity(gravity), self.dt, int(i), 2 if chained else 0,
+                     ptr(sin), ptr(out), self.n, ptr(vkin), ptr(perm), ptr(cs), plan.s)
+                self._last_b = (out, out._version, dst)
+                self._nodes_last = nb
+                return out, (nb, vkin, perm, cs)
+            nb = self._nbuf
             call("dmpm_step_between", plan.h, ptr(self._blocks[src]), ptr(self._blocks[dst]),
                  ptr(self._eid_b), ptr(nb), self.code, self.vu, plan._gravity(gravity),
                  self.dt, int(i), 2 if chained else 0, ptr(sin), ptr(out), self.n, plan.s)
             self._last_b = (out, out._version, dst)
             self._nodes_last = nb
-            if save:
-                vkin, perm, cs = self._export(plan)
-                return out, (nb, vkin, perm, cs)
         if not save:
             return out, None
         nsaved = self._nbuf.clone()
@@ -251,12 +255,18 @@ class Differentiable:
         self._last_1d = (out, out._version)
         return out
 
-    def _export(self, plan):
-        """The kinematic nodal velocity and the sort by cell of the plan's last step."""
+    def _export_buffers(self):
+        """Buffers for a step's kinematic nodal velocity ([2][nnodes] in the
+        first 2 nnodes elements) and its sort by cell."""
         vkin = torch.empty((2, self._nbuf.shape[1]), dtype=self._nbuf.dtype,
                            device=self._nbuf.device)
         perm = torch.empty(max(self.n, 1), dtype=torch.int32, device=self._nbuf.device)
         cs = torch.empty(self._ncells() + 2, dtype=torch.int32, device=self._nbuf.device)
+        return vkin, perm, cs
+
+    def _export(self, plan):
+        """The kinematic nodal velocity and the sort by cell of the plan's last step."""
+        vkin, perm, cs = self._export_buffers()
         call("dmpm_plan_export_step", plan.h, ptr(vkin), ptr(perm), ptr(cs), plan.s)
         return vkin, perm, cs
 

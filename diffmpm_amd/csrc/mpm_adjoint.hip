@@ -1110,6 +1110,25 @@ __global__ void k_export_sort(const int* __restrict__ perm0, const int* __restri
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < ncs; i += gs) cs_out[i] = cs[i];
 }
 
+// The same copy in 16-byte words (every pointer 16-byte aligned): the
+// permutation and the cell offsets of the current sort buffer.
+__global__ void k_export_sort4(const int* __restrict__ perm0, const int* __restrict__ perm1,
+                               const int* __restrict__ cs0, const int* __restrict__ cs1,
+                               const int* __restrict__ flags, int n, int ncs,
+                               int* __restrict__ perm_out, int* __restrict__ cs_out) {
+  const int cur = flags[4];
+  const int* __restrict__ perm = cur ? perm1 : perm0;
+  const int* __restrict__ cs = cur ? cs1 : cs0;
+  const int gs = gridDim.x * blockDim.x, gt = blockIdx.x * blockDim.x + threadIdx.x;
+  const int n4 = n >> 2, c4 = ncs >> 2;
+  for (int i = gt; i < n4; i += gs)
+    reinterpret_cast<int4*>(perm_out)[i] = reinterpret_cast<const int4*>(perm)[i];
+  for (int i = gt; i < c4; i += gs)
+    reinterpret_cast<int4*>(cs_out)[i] = reinterpret_cast<const int4*>(cs)[i];
+  for (int i = (n4 << 2) + gt; i < n; i += gs) perm_out[i] = perm[i];
+  for (int i = (c4 << 2) + gt; i < ncs; i += gs) cs_out[i] = cs[i];
+}
+
 #undef PR
 #undef PW
 #undef NR
@@ -1118,6 +1137,29 @@ __global__ void k_export_sort(const int* __restrict__ perm0, const int* __restri
 #undef AR
 
 }  // namespace
+
+namespace dmpm {
+
+// Copies the sort by cell of the plan's last step (permutation, cell offsets
+// of the current sort buffer) to perm_out [n] and cs_out [ncells + 2].
+int export_sort(dmpm_plan* P, int32_t* perm_out, int32_t* cs_out, hipStream_t s) {
+  const int n = (int)P->n, ncs = P->ncells + 2;
+  const bool a16 = ((uintptr_t)perm_out % 16 == 0) && ((uintptr_t)cs_out % 16 == 0) &&
+                   ((uintptr_t)P->perm % 16 == 0) && ((uintptr_t)P->perm2 % 16 == 0) &&
+                   ((uintptr_t)P->cs % 16 == 0) && ((uintptr_t)P->cs2 % 16 == 0);
+  if (a16) {
+    const int words = std::max(n, ncs) / 4 + 1;
+    const unsigned blocks = (unsigned)std::min((words + kThreads - 1) / kThreads, 4096);
+    k_export_sort4<<<blocks, kThreads, 0, s>>>(P->perm, P->perm2, P->cs, P->cs2, P->flags, n,
+                                               ncs, perm_out, cs_out);
+  } else {
+    k_export_sort<<<1024, kThreads, 0, s>>>(P->perm, P->perm2, P->cs, P->cs2, P->flags, n, ncs,
+                                            perm_out, cs_out);
+  }
+  return launch_check("export_sort");
+}
+
+}  // namespace dmpm
 
 extern "C" int dmpm_plan_export_step(dmpm_plan* P, void* vkin_out, int32_t* perm_out,
                                      int32_t* cs_out, void* stream) {
@@ -1131,9 +1173,7 @@ extern "C" int dmpm_plan_export_step(dmpm_plan* P, void* vkin_out, int32_t* perm
   DMPM_TRY(hip_check(hipMemcpyAsync(vkin_out, vkin, 2 * (size_t)P->nnodes * esz,
                                     hipMemcpyDeviceToDevice, s),
                      "hipMemcpyAsync"));
-  k_export_sort<<<1024, kThreads, 0, s>>>(P->perm, P->perm2, P->cs, P->cs2, P->flags, (int)P->n,
-                                          P->ncells + 2, perm_out, cs_out);
-  return launch_check("export_step");
+  return export_sort(P, perm_out, cs_out, s);
 }
 
 extern "C" int dmpm_step_adjoint_saved(dmpm_plan* P, const void* pin, const void* pout,

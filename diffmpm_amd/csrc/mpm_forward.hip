@@ -1813,6 +1813,34 @@ extern "C" int dmpm_step_between(dmpm_plan* P, const void* pbuf_in, void* pbuf_o
                       state_out, state_ld, ST(stream));
 }
 
+extern "C" int dmpm_step_between_saved(dmpm_plan* P, const void* pbuf_in, void* pbuf_out,
+                                       int32_t* eid, const void* nprev, void* nout,
+                                       int32_t scheme, int32_t velocity_update,
+                                       const double* gravity, double dt, int64_t step,
+                                       int32_t flags, const void* state_in, void* state_out,
+                                       int64_t state_ld, void* vkin_out, int32_t* perm_out,
+                                       int32_t* cs_out, void* stream) {
+  PCHK(P);
+  if (!pbuf_in || !pbuf_out || !eid || !nprev || !nout) return fail(DMPM_EINVAL, "null buffer");
+  if (!vkin_out || !perm_out || !cs_out) return fail(DMPM_EINVAL, "null export buffer");
+  if (state_in && !state_out) return fail(DMPM_EINVAL, "state_in needs state_out");
+  if (state_in && state_in == state_out) return fail(DMPM_EINVAL, "state_in and state_out must differ");
+  if (state_out && state_ld < P->n) return fail(DMPM_EINVAL, "state_ld below the particle count");
+  if (pbuf_in == pbuf_out) return fail(DMPM_EINVAL, "input and output blocks must differ");
+  if (scheme != DMPM_USF && scheme != DMPM_USL)
+    return fail(DMPM_EINVAL, "scheme must be DMPM_USF or DMPM_USL");
+  if (!gravity) return fail(DMPM_EINVAL, "null gravity");
+  if (flags & 1) return fail(DMPM_EINVAL, "a saved step keeps no temporaries");
+  if (!tile_eligible(P, 0))
+    return fail(DMPM_EINVAL,
+                "dmpm_step_between_saved covers the fused 2D step (Quadrilateral4Node, no "
+                "particle tractions or particle external forces)");
+  return tile_between_saved(P, pbuf_in, pbuf_out, eid, nprev, nout, (int)scheme,
+                            (int)velocity_update, gravity, dt, step, (int)((flags >> 1) & 1),
+                            state_in, state_out, state_ld, vkin_out, perm_out, cs_out,
+                            ST(stream));
+}
+
 extern "C" int dmpm_step_between_noalloc(dmpm_plan* P, const void* pbuf_in, void* pbuf_out,
                                          int32_t* eid, void* nbuf, int32_t scheme,
                                          int32_t velocity_update, const double* gravity,

@@ -178,10 +178,19 @@ int tile_run(dmpm_plan* P, void* pbuf, int32_t* eid, void* nbuf, int scheme, int
 // Allocates the workspaces of tile_run and uploads its tables (no other work).
 int tile_prepare(dmpm_plan* P, int scheme, hipStream_t s);
 bool tile_eligible(const dmpm_plan* P, int pfext);
+// Copies the sort by cell of the plan's last step to perm_out [n] and cs_out
+// [ncells + 2] (mpm_adjoint.hip).
+int export_sort(dmpm_plan* P, int32_t* perm_out, int32_t* cs_out, hipStream_t s);
 // One fused 2D step from block pin to block pout (mpm_tile.hip).
 int tile_between(dmpm_plan* P, const void* pin, void* pout, int32_t* eid, void* nbuf,
                  int scheme, int vu, const double* grav, double dt, int64_t step, int temps,
                  int chained, const void* sin, void* sout, int64_t sld, hipStream_t s);
+// tile_between of a differentiable rollout's saved step (dmpm_step_between_saved).
+int tile_between_saved(dmpm_plan* P, const void* pin, void* pout, int32_t* eid,
+                       const void* nprev, void* nout, int scheme, int vu, const double* grav,
+                       double dt, int64_t step, int chained, const void* sin, void* sout,
+                       int64_t sld, void* vkin_out, int32_t* perm_out, int32_t* cs_out,
+                       hipStream_t s);
 // dmpm_step_between on workspaces in the library's static device array
 // (mpm_tile.hip); noalloc_scratch_setup takes that array's address on a
 // device (once per device, with the device current).

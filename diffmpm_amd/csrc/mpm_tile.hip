@@ -721,7 +721,8 @@ template <typename T> struct StepArgs {
   uint32_t nld32;
   uint32_t pbytes;  // bytes of a particle block (the shadow block is as large)
   uint32_t nbytes;  // bytes of the node block
-  const T* vold;  // [2][nnodes] nodal velocity at the start of the step
+  const T* vold;  // [2][vld] nodal velocity at the start of the step
+  int64_t vld;    // row stride of vold
   T* vnew;        // [2][nnodes] nodal velocity after the step
   T* vkin;        // [2][nnodes] kinematic nodal velocity (for the adjoint's export)
   SortRef sr;
@@ -964,7 +965,7 @@ __device__ __forceinline__ void kin_epilogue(const StepArgs<T>& a, const TileCtx
     nm[ln] = m;
 #pragma unroll
     for (int d = 0; d < 2; ++d) {
-      T v = (m == T(0)) ? a.vold[(int64_t)d * nnodes + nd] : p[d] / m;
+      T v = (m == T(0)) ? a.vold[(int64_t)d * a.vld + nd] : p[d] / m;
       v = snap(v);
       if (mask & (1 << d)) v = T(a.aux->cval[2 * cr + d]);
       vk[d * nkn + ln] = v;
@@ -1980,7 +1981,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) void 
 #pragma unroll
     for (int d = 0; d < 2; ++d) {
       T v;
-      if (mm == T(0)) v = a.vold[(int64_t)d * nnodes + nd];
+      if (mm == T(0)) v = a.vold[(int64_t)d * a.vld + nd];
       else v = p[d] / mm;
       v = snap(v);
       if (mask & (1 << d)) v = T(a.aux->cval[2 * cr + d]);
@@ -3062,7 +3063,8 @@ int step_run(dmpm_plan* P, const Mesh2<T>& M, T* pa, T* pb2, T* pstat, int32_t* 
              int scheme, int vu, const double* grav, double dt, int64_t step0, int64_t nsteps,
              int temps, hipStream_t s, T* vkin, T* sout = nullptr, int64_t sld = 0,
              const T* sin = nullptr, const int64_t* clock = nullptr, int kbase = 0,
-             bool prepare = false, const StepScratch<T>* ext = nullptr) {
+             bool prepare = false, const StepScratch<T>* ext = nullptr,
+             const T* vprev = nullptr) {
   constexpr bool kFast = sizeof(T) == 4;
   const int64_t ld = P->d.ld, nld = P->d.nld;
   const int nn = P->nnodes;
@@ -3087,9 +3089,10 @@ int step_run(dmpm_plan* P, const Mesh2<T>& M, T* pa, T* pb2, T* pstat, int32_t* 
     ovf = kFast ? (int*)plan_workspace(P, 7, (size_t)(2 + 2 * a.ntiles) * sizeof(int)) : nullptr;
   }
   if (!shadow || !vbuf || (kFast && !ovf)) return fail(DMPM_EHIP, "workspace allocation failed");
-  if (!prepare) {
+  if (!prepare && !vprev) {
     // the nodal velocity a massless node keeps (the counters of ovf are
-    // zeroed by each step's re-sort)
+    // zeroed by each step's re-sort); with vprev the first step reads it
+    // from there
     k_copy_rows<T><<<dim3((unsigned)((nn + kThreads - 1) / kThreads), 2), kThreads, 0, s>>>(
         vbuf, nn, nb + DMPM_N_V * nld, nld, nn);
     DMPM_TRY(launch_check("copy_rows"));
@@ -3168,7 +3171,8 @@ int step_run(dmpm_plan* P, const Mesh2<T>& M, T* pa, T* pb2, T* pstat, int32_t* 
     }
     a.src = src;
     a.dst = (k & 1) ? pa : shadow;
-    a.vold = vbuf + (k & 1) * 2 * (int64_t)nn;
+    a.vold = (vprev && k == 0) ? vprev : vbuf + (k & 1) * 2 * (int64_t)nn;
+    a.vld = (vprev && k == 0) ? nld : nn;
     a.vnew = vbuf + ((k + 1) & 1) * 2 * (int64_t)nn;
     a.t = T(step0 + k) * a.dt;
     a.kofs = kbase + (int)k;
@@ -3251,9 +3255,10 @@ int tile_prepare_t(dmpm_plan* P, int scheme, hipStream_t s) {
 template <typename T>
 int tile_between_t(dmpm_plan* P, const T* pin, T* pout, int32_t* eid, T* nb, int scheme, int vu,
                    const double* grav, double dt, int64_t step, int temps, int chained,
-                   const T* sin, T* sout, int64_t sld, hipStream_t s) {
+                   const T* sin, T* sout, int64_t sld, hipStream_t s, const T* nprev = nullptr,
+                   T* vkin_out = nullptr) {
   const Mesh2<T> M = mesh_of<T>(P);
-  const int64_t ld = P->d.ld;
+  const int64_t ld = P->d.ld, nld = P->d.nld;
   const int n = (int)P->n;
   T* part = (T*)plan_workspace(P, 0, (size_t)4 * NQ_FORCE * M.ncells * sizeof(T));
   T* vkin = (T*)plan_workspace(P, 1, (size_t)2 * P->nnodes * sizeof(T));
@@ -3270,9 +3275,27 @@ int tile_between_t(dmpm_plan* P, const T* pin, T* pout, int32_t* eid, T* nb, int
     k_tile_check<T><<<nblk(n), kThreads, 0, s>>>(M, px, pld, n, eid, P->key, move_log(P));
     DMPM_TRY(launch_check("tile_check"));
   }
+  if (nprev && nprev != nb) {
+    // the damping force of the previous node block (the step reads it from nb)
+    DMPM_TRY(hip_check(hipMemcpy2DAsync(nb + (int64_t)DMPM_N_FDAMP * nld, nld * sizeof(T),
+                                        nprev + (int64_t)DMPM_N_FDAMP * nld, nld * sizeof(T),
+                                        (size_t)P->nnodes * sizeof(T), 2,
+                                        hipMemcpyDeviceToDevice, s),
+                       "hipMemcpy2DAsync"));
+  }
   if (step_eligible(P) && (uint64_t)8 * (uint64_t)sld * sizeof(T) < (1ull << 32))
+    // a massless node keeps the velocity of the previous node block (read
+    // there directly when it is another block)
     return step_run<T>(P, M, const_cast<T*>(pin), pout, pout, eid, nb, scheme, vu, grav, dt,
-                       step, 1, temps, s, vkin, sout, sld, sin);
+                       step, 1, temps, s, vkin_out ? vkin_out : vkin, sout, sld, sin, nullptr,
+                       0, false, nullptr,
+                       (nprev && nprev != nb) ? nprev + (int64_t)DMPM_N_V * nld : nullptr);
+  if (nprev && nprev != nb)
+    DMPM_TRY(hip_check(hipMemcpy2DAsync(nb + (int64_t)DMPM_N_V * nld, nld * sizeof(T),
+                                        nprev + (int64_t)DMPM_N_V * nld, nld * sizeof(T),
+                                        (size_t)P->nnodes * sizeof(T), 2,
+                                        hipMemcpyDeviceToDevice, s),
+                       "hipMemcpy2DAsync"));
   const int runs[6][2] = {{DMPM_P_X, 4},          {DMPM_P_VOLUME, 2}, {DMPM_P_STRAIN, 2},
                           {DMPM_P_STRAIN + 3, 1}, {DMPM_P_STRESS, 4}, {DMPM_P_VOLSTRAIN_C, 1}};
   for (const auto& r : runs)
@@ -3299,6 +3322,10 @@ int tile_between_t(dmpm_plan* P, const T* pin, T* pout, int32_t* eid, T* nb, int
                                           hipMemcpyDeviceToDevice, s),
                          "hipMemcpy2DAsync"));
   }
+  if (vkin_out)
+    DMPM_TRY(hip_check(hipMemcpyAsync(vkin_out, vkin, (size_t)2 * P->nnodes * sizeof(T),
+                                      hipMemcpyDeviceToDevice, s),
+                       "hipMemcpyAsync"));
   return DMPM_OK;
 }
 
@@ -3405,6 +3432,24 @@ int tile_between(dmpm_plan* P, const void* pin, void* pout, int32_t* eid, void* 
   return tile_between_t<double>(P, (const double*)pin, (double*)pout, eid, (double*)nbuf, scheme,
                                 vu, grav, dt, step, temps, chained, (const double*)sin,
                                 (double*)sout, sld, s);
+}
+
+int tile_between_saved(dmpm_plan* P, const void* pin, void* pout, int32_t* eid,
+                       const void* nprev, void* nout, int scheme, int vu, const double* grav,
+                       double dt, int64_t step, int chained, const void* sin, void* sout,
+                       int64_t sld, void* vkin_out, int32_t* perm_out, int32_t* cs_out,
+                       hipStream_t s) {
+  if (P->d.dtype == DMPM_F32)
+    DMPM_TRY(tile_between_t<float>(P, (const float*)pin, (float*)pout, eid, (float*)nout,
+                                   scheme, vu, grav, dt, step, 0, chained, (const float*)sin,
+                                   (float*)sout, sld, s, (const float*)nprev,
+                                   (float*)vkin_out));
+  else
+    DMPM_TRY(tile_between_t<double>(P, (const double*)pin, (double*)pout, eid, (double*)nout,
+                                    scheme, vu, grav, dt, step, 0, chained, (const double*)sin,
+                                    (double*)sout, sld, s, (const double*)nprev,
+                                    (double*)vkin_out));
+  return export_sort(P, perm_out, cs_out, s);
 }
 
 int tile_run(dmpm_plan* P, void* pbuf, int32_t* eid, void* nbuf, int scheme, int vu,

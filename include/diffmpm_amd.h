@@ -297,6 +297,26 @@ int dmpm_step_between(dmpm_plan* plan, const void* pbuf_in, void* pbuf_out, int3
                       const double* gravity, double dt, int64_t step, int32_t flags,
                       const void* state_in, void* state_out, int64_t state_ld, void* stream);
 
+/* One step of a differentiable rollout that keeps what its adjoint needs:
+ * dmpm_step_between from pbuf_in to pbuf_out (same flags, except that bit 0,
+ * temporaries, is rejected; same state tensors) that writes every node row of
+ * nout (mass, forces, velocity, acceleration, momentum; its damping-force
+ * rows are copied from nprev; its coordinate rows are not written), keeps the
+ * velocity of nprev's velocity rows at massless nodes (nprev: the previous
+ * step's node block, read only), and writes the step's kinematic nodal
+ * velocity to vkin_out [2][nnodes] (particle dtype) and its sort by cell to
+ * perm_out [nparticles] and cs_out [ncells+2], as dmpm_plan_export_step
+ * would.  Results equal dmpm_step_between on nout holding nprev's rows
+ * followed by dmpm_plan_export_step.  No reference counterpart (the reference
+ * differentiates solver.py:179-213 with jax.grad); for the saved forward
+ * steps that dmpm_step_adjoint_saved_state reverses. */
+int dmpm_step_between_saved(dmpm_plan* plan, const void* pbuf_in, void* pbuf_out, int32_t* eid,
+                            const void* nprev, void* nout, int32_t scheme,
+                            int32_t velocity_update, const double* gravity, double dt,
+                            int64_t step, int32_t flags, const void* state_in, void* state_out,
+                            int64_t state_ld, void* vkin_out, int32_t* perm_out,
+                            int32_t* cs_out, void* stream);
+
 /* dmpm_step_between for callers that cannot allocate (code under HIP graph
  * capture, tight serving loops, harnesses that forbid hipMalloc); no
  * counterpart in the reference.  Same arguments, restrictions and results
