@@ -31,7 +31,6 @@ Example::
 import math
 
 import torch
-import torch.utils.checkpoint
 
 from . import _lib
 from ._lib import call, ptr
@@ -59,12 +58,15 @@ class Differentiable:
         from .ensemble import BatchPlan
         d = cls.__new__(cls)
         d.sim, d.ps, d.el = None, None, None
-        d.bplan, d._blocks, d._nodes_last = None, None, None
+        d._blocks, d._nodes_last = None, None
         d.dim = 2
         d.per_particle_E = all(s.mesh.particles[0]._has_youngs for s in ens.sims)
         d.n = ens.n
         d._pbuf, d._nbuf, d._eid_f = ens.pbuf, ens.nbuf, ens.eid
         d.fplan, d.aplan = ens.plan, BatchPlan(ens)
+        # forward steps between two private blocks, with their own sort state
+        d.bplan = BatchPlan(ens)
+        d._eid_b = d._eid_f.clone()
         d._ens = ens
         d._finish(ens.scheme, ens.velocity_update, ens.dt)
         return d
@@ -170,67 +172,53 @@ class Differentiable:
     # -- one step ---------------------------------------------------------------
     def forward_step(self, state, log_e, gravity, i, save=False):
         """One forward step; with save=True also returns what its adjoint
-        needs (node block, kinematic nodal velocity, sort by cell).  A single
-        simulation steps between two private particle blocks (the
-        simulation's own block is not written; a 1D one on one private
-        block); an ensemble steps its batch block in place."""
+        needs (node block, kinematic nodal velocity, sort by cell).  A 2D
+        simulation or ensemble steps between two private particle blocks (its
+        own block is not written; a 1D one on one private block)."""
         if self.dim == 1:
             return self._forward_1d(state, log_e, gravity, i), None
-        if self.bplan is None:
-            self.set_state(state, log_e)
-            self.fplan.run(self.scheme, self.velocity_update, gravity, self.dt, i, 1,
-                           keep_temporaries=False)
-            self._last = None  # the block now holds the output
-            out = self.get_state()
-            plan = self.fplan
-        else:
-            if self._blocks is None:
-                self._blocks = (self._pbuf.clone(), self._pbuf.clone())
-            # the state rows are read from `state` and written to `out`; the
-            # two blocks carry the other rows (strain, density, sigma_zz,
-            # volumetric strain) and the constants
-            last = getattr(self, "_last_b", None)
-            chained = last is not None and last[0] is state and state._version == last[1]
-            src = last[2] if last is not None else 0
-            dst = 1 - src
-            if log_e is not None:
-                self._write_log_e_private(self._blocks[dst], log_e)
-            plan = self.bplan
-            plan.refresh_tables()
-            sin = state.detach()
-            if not sin.is_contiguous():
-                sin = sin.contiguous()
-            out = torch.empty((C_ROWS, self.n), dtype=self._pbuf.dtype,
-                              device=self._pbuf.device)
-            if save:
-                # a fresh node block per saved step, kept for its adjoint: the
-                # step writes mass, velocity, acceleration, momentum and the
-                # forces of every node, reads the velocity a massless node
-                # keeps from the previous step's block and copies its damping
-                # force; its coordinate rows are not set (neither the step nor
-                # the adjoint reads them).  The kinematic nodal velocity and
-                # the sort by cell go straight to the adjoint's buffers.
-                prev = self._nodes_last if chained and self._nodes_last is not None else self._nbuf
-                nb = torch.empty_like(self._nbuf)
-                vkin, perm, cs = self._export_buffers()
-                call("dmpm_step_between_saved", plan.h, ptr(self._blocks[src]),
-                     ptr(self._blocks[dst]), ptr(self._eid_b), ptr(prev), ptr(nb), self.code,
-                     self.vu, plan._gravity(gravity), self.dt, int(i), 2 if chained else 0,
-                     ptr(sin), ptr(out), self.n, ptr(vkin), ptr(perm), ptr(cs), plan.s)
-                self._last_b = (out, out._version, dst)
-                self._nodes_last = nb
-                return out, (nb, vkin, perm, cs)
-            nb = self._nbuf
-            call("dmpm_step_between", plan.h, ptr(self._blocks[src]), ptr(self._blocks[dst]),
-                 ptr(self._eid_b), ptr(nb), self.code, self.vu, plan._gravity(gravity),
-                 self.dt, int(i), 2 if chained else 0, ptr(sin), ptr(out), self.n, plan.s)
+        if self._blocks is None:
+            self._blocks = (self._pbuf.clone(), self._pbuf.clone())
+        # the state rows are read from `state` and written to `out`; the two
+        # blocks carry the other rows (strain, density, sigma_zz, volumetric
+        # strain) and the constants
+        last = getattr(self, "_last_b", None)
+        chained = last is not None and last[0] is state and state._version == last[1]
+        src = last[2] if last is not None else 0
+        dst = 1 - src
+        if log_e is not None:
+            self._write_log_e_private(self._blocks[dst], log_e)
+        plan = self.bplan
+        plan.refresh_tables()
+        sin = state.detach()
+        if not sin.is_contiguous():
+            sin = sin.contiguous()
+        out = torch.empty((C_ROWS, self.n), dtype=self._pbuf.dtype, device=self._pbuf.device)
+        if save:
+            # a fresh node block per saved step, kept for its adjoint: the step
+            # writes mass, velocity, acceleration, momentum and the forces of
+            # every node, reads the velocity a massless node keeps from the
+            # previous step's block and copies its damping force; its
+            # coordinate rows are not set (neither the step nor the adjoint
+            # reads them).  The kinematic nodal velocity and the sort by cell
+            # go straight to the adjoint's buffers.
+            prev = self._nodes_last if chained and self._nodes_last is not None else self._nbuf
+            nb = torch.empty_like(self._nbuf)
+            vkin, perm, cs = self._export_buffers()
+            call("dmpm_step_between_saved", plan.h, ptr(self._blocks[src]),
+                 ptr(self._blocks[dst]), ptr(self._eid_b), ptr(prev), ptr(nb), self.code,
+                 self.vu, plan._gravity(gravity), self.dt, int(i), 2 if chained else 0,
+                 ptr(sin), ptr(out), self.n, ptr(vkin), ptr(perm), ptr(cs), plan.s)
             self._last_b = (out, out._version, dst)
             self._nodes_last = nb
-        if not save:
-            return out, None
-        nsaved = self._nbuf.clone()
-        vkin, perm, cs = self._export(plan)
-        return out, (nsaved, vkin, perm, cs)
+            return out, (nb, vkin, perm, cs)
+        nb = self._nbuf
+        call("dmpm_step_between", plan.h, ptr(self._blocks[src]), ptr(self._blocks[dst]),
+             ptr(self._eid_b), ptr(nb), self.code, self.vu, plan._gravity(gravity), self.dt,
+             int(i), 2 if chained else 0, ptr(sin), ptr(out), self.n, plan.s)
+        self._last_b = (out, out._version, dst)
+        self._nodes_last = nb
+        return out, None
 
     def _forward_1d(self, state, log_e, gravity, i):
         """One per-phase step of a Linear1D simulation on a private particle
@@ -262,12 +250,6 @@ class Differentiable:
                            device=self._nbuf.device)
         perm = torch.empty(max(self.n, 1), dtype=torch.int32, device=self._nbuf.device)
         cs = torch.empty(self._ncells() + 2, dtype=torch.int32, device=self._nbuf.device)
-        return vkin, perm, cs
-
-    def _export(self, plan):
-        """The kinematic nodal velocity and the sort by cell of the plan's last step."""
-        vkin, perm, cs = self._export_buffers()
-        call("dmpm_plan_export_step", plan.h, ptr(vkin), ptr(perm), ptr(cs), plan.s)
         return vkin, perm, cs
 
     def _ncells(self):
@@ -366,7 +348,7 @@ class _StepFn(torch.autograd.Function):
         d = ctx.d
         le = log_e if ctx.has_log_e else None
         want_e = ctx.has_log_e and ctx.needs_input_grad[1]
-        acc = ctx.acc if want_e and log_e.dim() > 0 else None
+        acc = ctx.acc if want_e else None
         lebar = None
         if acc is not None:
             if acc.buf is None:
@@ -391,28 +373,83 @@ def step(d, state, gravity, i, log_e=None):
     return _StepFn.apply(state, log_e, d, gravity, int(i))
 
 
+class _SegmentFn(torch.autograd.Function):
+    """k steps of a checkpointed rollout as one autograd node.  The forward
+    pass runs the plain steps (nothing saved, no autograd node per step) and
+    keeps the segment's input state; the backward pass recomputes the
+    segment's steps from it with their adjoint quantities saved, then runs
+    the k adjoint steps in reverse order.  The adjoint kernels see the same
+    inputs in the same order as in an uncheckpointed rollout, so the
+    gradients are bit-identical to it."""
+
+    @staticmethod
+    def forward(ctx, state, log_e, d, gravity, i0, k, acc=None, emit=False):
+        st = state
+        for i in range(i0, i0 + k):
+            st, _ = d.forward_step(st, log_e, gravity, i, save=False)
+        ctx.d, ctx.gravity, ctx.i0, ctx.k = d, gravity, i0, k
+        ctx.has_log_e = log_e is not None
+        ctx.acc, ctx.emit = acc, emit
+        ctx.save_for_backward(state, log_e if log_e is not None else state.new_empty(0))
+        return st
+
+    @staticmethod
+    def backward(ctx, gout):
+        state, log_e = ctx.saved_tensors
+        d, gravity, i0, k = ctx.d, ctx.gravity, ctx.i0, ctx.k
+        le = log_e if ctx.has_log_e else None
+        want_e = ctx.has_log_e and ctx.needs_input_grad[1]
+        acc = ctx.acc if want_e else None
+        lebar = None
+        if acc is not None:
+            if acc.buf is None:
+                acc.buf = d._lebar(None)
+            lebar = acc.buf
+        save = d.save_forward
+        with torch.no_grad():
+            states, saved = [state.detach()], []
+            for i in range(i0, i0 + k):
+                out, sv = d.forward_step(states[-1], le, gravity, i, save=save)
+                states.append(out)
+                saved.append(sv)
+            g, ge = gout, None
+            for j in range(k - 1, -1, -1):
+                if save:
+                    g, ge = d.adjoint_step_saved(states[j], le, states[j + 1], saved[j], gravity,
+                                                 g, lebar)
+                else:
+                    g, ge = d.adjoint_step(states[j], le, gravity, i0 + j, g, lebar)
+                saved[j] = None  # free each step's node block once its adjoint ran
+        g_log_e = None
+        if want_e and (acc is None or ctx.emit):
+            if acc is not None:
+                acc.buf = None  # a later backward pass starts a new sum
+            g_log_e = ge if log_e.dim() > 0 else ge.sum()
+            g_log_e = g_log_e.to(log_e.dtype).reshape(log_e.shape)
+        return g, g_log_e, None, None, None, None, None, None
+
+
 def rollout(d, nsteps, gravity, state=None, log_e=None, start_step=0, checkpoint_every=0):
     """Differentiable rollout of ``nsteps`` steps from ``state`` (default: the
     current state of the simulation).  Returns the final (8, N) state.  With
-    ``checkpoint_every=K`` only segment-start states are kept for backward.
-    A single simulation's rollout does not write the simulation's own
-    particle block (its steps alternate between two private blocks); after a
-    backward pass an ensemble's batch block holds a recomputed intermediate
-    state.  Use the returned tensors."""
+    ``checkpoint_every=K`` only segment-start states are kept for backward:
+    the forward pass runs K plain steps per segment, and the backward pass
+    recomputes a segment's steps (saving what their adjoints read) when it
+    reaches it.  A rollout does not write the simulation's (or the
+    ensemble's) own particle block: its steps alternate between two private
+    blocks.  Use the returned tensors."""
     if state is None:
         state = d.get_state()
-    # per-particle log E: one gradient buffer for all steps
-    acc = _LogEGrad() if log_e is not None and log_e.dim() > 0 else None
-
-    def seg(st, le, i0, k):
-        for i in range(i0, i0 + k):
-            st = _StepFn.apply(st, le, d, gravity, i, acc, i == start_step)
-        return st
+    # log E (per particle or one scalar): the adjoint kernels of every step
+    # add dL/dlog(E_p) into one per-particle buffer, summed once for a scalar
+    acc = _LogEGrad() if log_e is not None else None
 
     if checkpoint_every and checkpoint_every < nsteps:
         for i0 in range(start_step, start_step + nsteps, checkpoint_every):
             k = min(checkpoint_every, start_step + nsteps - i0)
-            state = torch.utils.checkpoint.checkpoint(seg, state, log_e, i0, k,
-                                                      use_reentrant=False)
+            state = _SegmentFn.apply(state, log_e, d, gravity, i0, k, acc, i0 == start_step)
         return state
-    return seg(state, log_e, start_step, nsteps)
+    st = state
+    for i in range(start_step, start_step + nsteps):
+        st = _StepFn.apply(st, log_e, d, gravity, i, acc, i == start_step)
+    return st
