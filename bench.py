@@ -218,12 +218,14 @@ def ensemble_bench(K, K_adj=None, K_seq_adj=50, checkpoint_every=100):
 
     def batched(k):
         le = le0.clone().requires_grad_(True)
-        o = rollout(d, k, g, state=d.get_state(), log_e=le[idx],
-                    checkpoint_every=checkpoint_every if k > checkpoint_every else 0)
+        ck = checkpoint_every if k > checkpoint_every else 0
+        o = rollout(d, k, g, state=d.get_state(), log_e=le[idx], checkpoint_every=ck,
+                    graphs=bool(ck))
         o[0:2].sum().backward()
         return le.grad
 
     batched(4)  # warm-up
+    batched(2 * checkpoint_every)  # captures the segments' graphs
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     grad = batched(K_adj)
@@ -248,7 +250,9 @@ def ensemble_bench(K, K_adj=None, K_seq_adj=50, checkpoint_every=100):
     tq = time.perf_counter() - t0
     out["fwd_adjoint"] = {
         "batched": {"value": ntot * K_adj / tb, "unit": "particle-steps/s", "steps": K_adj,
-                    "checkpoint_every": checkpoint_every, "ms_per_step": tb * 1e3 / K_adj},
+                    "checkpoint_every": checkpoint_every, "ms_per_step": tb * 1e3 / K_adj,
+                    "run": "per segment: one HIP graph replay forward, one backward "
+                           "(recomputation with saved adjoint inputs, then the adjoint steps)"},
         "sequential": {"value": ntot * K_seq_adj / tq, "unit": "particle-steps/s",
                        "steps": K_seq_adj, "ms_per_step": tq * 1e3 / K_seq_adj},
         "speedup": (ntot * K_adj / tb) / (ntot * K_seq_adj / tq),
@@ -350,7 +354,8 @@ def inverse_bench(iters, K=2000, checkpoint_every=100):
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         opt.zero_grad()
-        out = rollout(d, K, g, state=state0, log_e=le, checkpoint_every=checkpoint_every)
+        out = rollout(d, K, g, state=state0, log_e=le, checkpoint_every=checkpoint_every,
+                      graphs=True)
         loss = ((out[0:2] - target) ** 2).mean() / scale
         loss.backward()
         opt.step()
@@ -367,7 +372,8 @@ def inverse_bench(iters, K=2000, checkpoint_every=100):
             "fwd_adjoint_particle_steps_per_s": N * K / it_s,
             "forward_only_s": fwd_s, "loss": losses, "log_e_rel_error": errs,
             "note": "one iteration = forward rollout (autograd records segment starts), "
-                    "backward (each segment recomputed, then its adjoint steps), Adam step; "
+                    "backward (each segment recomputed, then its adjoint steps; each "
+                    "segment's steps one HIP graph replay), Adam step; "
                     "median time over the iterations"}
 
 

@@ -252,6 +252,33 @@ class Differentiable:
         cs = torch.empty(self._ncells() + 2, dtype=torch.int32, device=self._nbuf.device)
         return vkin, perm, cs
 
+    def _new_plan(self):
+        """A new plan of the simulation or ensemble (its own sort state and
+        workspaces)."""
+        if self.sim is None:
+            from .ensemble import BatchPlan
+            return BatchPlan(self._ens)
+        return Plan(self.el, self.ps)
+
+    def _segment_graphs(self, kmax):
+        """The captured segments of up to kmax steps (kept, with their
+        buffers, while this object lives)."""
+        gr = self.__dict__.get("_graphs")
+        if gr is None or gr.kmax < kmax:
+            if gr is not None:
+                del self._graphs
+                del gr
+                torch.cuda.empty_cache()
+            gr = self._graphs = _SegmentGraphs(self, kmax)
+        return gr
+
+    def _time_dependent(self):
+        """Whether a step's result depends on its index (nodal forces with a
+        time function)."""
+        sims = self._ens.sims if self.sim is None else [self.sim]
+        return any(c.function is not None for s in sims
+                   for c in s.mesh.elements.concentrated_nodal_forces)
+
     def _ncells(self):
         if self.el is not None:
             return self.el._nx * self.el._ny
@@ -373,6 +400,119 @@ def step(d, state, gravity, i, log_e=None):
     return _StepFn.apply(state, log_e, d, gravity, int(i))
 
 
+class _SegmentGraphs:
+    """The steps of checkpointed rollout segments of a 2D ``Differentiable``
+    captured into HIP graphs (``rollout(..., graphs=True)``): per segment
+    length (and start step, when nodal forces depend on time) one graph of
+    the plain forward steps, and one of the recomputation with saved adjoint
+    inputs followed by the adjoint steps in reverse order.  The calls are
+    those of the eager segments, on plans, particle blocks and buffers that
+    only these graphs use (their workspaces stay where the graphs recorded
+    them), so the results are bit-identical to the eager segments; the host
+    work per segment is a few copies and one replay.  Memory: kmax + 1 states,
+    kmax node blocks, kinematic nodal velocities and sorts."""
+
+    def __init__(self, d, kmax):
+        if d.dim != 2:
+            raise ValueError("captured segments cover 2D meshes")
+        self.d, self.kmax = d, int(kmax)
+        self.fp, self.ap = d._new_plan(), d._new_plan()
+        pb, nbuf = d._pbuf, d._nbuf
+        self.blocks = (pb.clone(), pb.clone())
+        self.pin = pb.clone()
+        self.eid = d._eid_f.clone()
+        self.nbf = nbuf.clone()  # node block of the plain steps
+        n, ld, dev, dt = d.n, pb.shape[1], pb.device, pb.dtype
+        k = self.kmax
+        self.S = torch.empty((k + 1, C_ROWS, n), dtype=dt, device=dev)
+        self.NB = torch.empty((k,) + tuple(nbuf.shape), dtype=dt, device=dev)
+        self.VK = torch.empty((k, 2, nbuf.shape[1]), dtype=dt, device=dev)
+        self.PERM = torch.empty((k, max(n, 1)), dtype=torch.int32, device=dev)
+        self.CS = torch.empty((k, d._ncells() + 2), dtype=torch.int32, device=dev)
+        self.cot = torch.zeros((C_ROWS, ld), dtype=dt, device=dev)
+        self.lebar = torch.zeros(ld, dtype=dt, device=dev)
+        self.timed = d._time_dependent()
+        self.graphs = {}
+        self._warm = False
+
+    def set_input(self, state, log_e):
+        """Segment input state and log E into the buffers the graphs read."""
+        d = self.d
+        self.S[0].copy_(state.detach())
+        if log_e is not None:
+            for b in self.blocks + (self.pin,):
+                d._write_log_e_private(b, log_e)
+
+    def _fwd(self, j, i, grav):
+        d = self.d
+        call("dmpm_step_between", self.fp.h, ptr(self.blocks[j % 2]),
+             ptr(self.blocks[1 - j % 2]), ptr(self.eid), ptr(self.nbf), d.code, d.vu, grav, d.dt,
+             int(i), 2 if j else 0, ptr(self.S[j]), ptr(self.S[j + 1]), d.n, self.fp.s)
+
+    def _saved(self, j, i, grav):
+        d = self.d
+        prev = self.nbf if j == 0 else self.NB[j - 1]
+        call("dmpm_step_between_saved", self.fp.h, ptr(self.blocks[j % 2]),
+             ptr(self.blocks[1 - j % 2]), ptr(self.eid), ptr(prev), ptr(self.NB[j]), d.code,
+             d.vu, grav, d.dt, int(i), 2 if j else 0, ptr(self.S[j]), ptr(self.S[j + 1]), d.n,
+             ptr(self.VK[j]), ptr(self.PERM[j]), ptr(self.CS[j]), self.fp.s)
+
+    def _adj(self, j, grav):
+        d = self.d
+        call("dmpm_step_adjoint_saved_state", self.ap.h, ptr(self.pin), ptr(self.S[j]),
+             ptr(self.S[j + 1]), d.n, ptr(self.NB[j]), ptr(self.VK[j]), ptr(self.PERM[j]),
+             ptr(self.CS[j]), ptr(self.cot), ptr(self.lebar), d.code, d.vu, grav, d.dt,
+             self.ap.s)
+
+    def _graph(self, kind, i0, k, gravity):
+        grav = self.fp._gravity(gravity)
+        key = (kind, i0 if self.timed else None, k, tuple(grav))
+        g = self.graphs.get(key)
+        if g is not None:
+            return g
+        self.fp.refresh_tables()
+        self.ap.refresh_tables()
+        if not self._warm:
+            # one eager call of each kind allocates the plans' workspaces
+            # before any capture records their addresses
+            self._fwd(0, i0, grav)
+            self._saved(0, i0, grav)
+            self._adj(0, grav)
+            self.lebar.zero_()
+            self._warm = True
+        torch.cuda.synchronize(self.S.device)
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            if kind == "fwd":
+                for j in range(k):
+                    self._fwd(j, i0 + j, grav)
+            else:
+                for j in range(k):
+                    self._saved(j, i0 + j, grav)
+                for j in range(k - 1, -1, -1):
+                    self._adj(j, grav)
+        self.graphs[key] = g
+        return g
+
+    def forward(self, state, log_e, gravity, i0, k):
+        """The state after steps i0 .. i0 + k - 1 from ``state``."""
+        self.set_input(state, log_e)
+        g = self._graph("fwd", i0, k, gravity)
+        g.replay()
+        return self.S[k].clone()
+
+    def backward(self, state, log_e, gravity, i0, k, gout, first):
+        """Cotangent of the segment's input state from ``gout``; the log E
+        terms are added to ``self.lebar`` (cleared first when ``first``)."""
+        self.set_input(state, log_e)
+        g = self._graph("bwd", i0, k, gravity)
+        self.cot[:, : self.d.n].copy_(gout)
+        if first:
+            self.lebar.zero_()
+        g.replay()
+        return self.cot[:, : self.d.n].clone()
+
+
 class _SegmentFn(torch.autograd.Function):
     """k steps of a checkpointed rollout as one autograd node.  The forward
     pass runs the plain steps (nothing saved, no autograd node per step) and
@@ -383,11 +523,14 @@ class _SegmentFn(torch.autograd.Function):
     gradients are bit-identical to it."""
 
     @staticmethod
-    def forward(ctx, state, log_e, d, gravity, i0, k, acc=None, emit=False):
-        st = state
-        for i in range(i0, i0 + k):
-            st, _ = d.forward_step(st, log_e, gravity, i, save=False)
-        ctx.d, ctx.gravity, ctx.i0, ctx.k = d, gravity, i0, k
+    def forward(ctx, state, log_e, d, gravity, i0, k, acc=None, emit=False, gr=None):
+        if gr is not None:
+            st = gr.forward(state, log_e, gravity, i0, k)
+        else:
+            st = state
+            for i in range(i0, i0 + k):
+                st, _ = d.forward_step(st, log_e, gravity, i, save=False)
+        ctx.d, ctx.gravity, ctx.i0, ctx.k, ctx.gr = d, gravity, i0, k, gr
         ctx.has_log_e = log_e is not None
         ctx.acc, ctx.emit = acc, emit
         ctx.save_for_backward(state, log_e if log_e is not None else state.new_empty(0))
@@ -400,6 +543,27 @@ class _SegmentFn(torch.autograd.Function):
         le = log_e if ctx.has_log_e else None
         want_e = ctx.has_log_e and ctx.needs_input_grad[1]
         acc = ctx.acc if want_e else None
+        gr = ctx.gr
+        if gr is not None:
+            # the graphs add the log E terms into their own buffer
+            first = acc is None or acc.buf is None
+            if acc is not None:
+                acc.buf = gr.lebar
+            with torch.no_grad():
+                g = gr.backward(state, le, gravity, i0, k, gout, first)
+            ge = gr.lebar[: d.n]
+        else:
+            g, ge = _SegmentFn._eager_backward(d, state, le, gravity, i0, k, gout, acc)
+        g_log_e = None
+        if want_e and (acc is None or ctx.emit):
+            if acc is not None:
+                acc.buf = None  # a later backward pass starts a new sum
+            g_log_e = ge if log_e.dim() > 0 else ge.sum()
+            g_log_e = g_log_e.to(log_e.dtype).reshape(log_e.shape).clone()
+        return g, g_log_e, None, None, None, None, None, None, None
+
+    @staticmethod
+    def _eager_backward(d, state, le, gravity, i0, k, gout, acc):
         lebar = None
         if acc is not None:
             if acc.buf is None:
@@ -420,24 +584,22 @@ class _SegmentFn(torch.autograd.Function):
                 else:
                     g, ge = d.adjoint_step(states[j], le, gravity, i0 + j, g, lebar)
                 saved[j] = None  # free each step's node block once its adjoint ran
-        g_log_e = None
-        if want_e and (acc is None or ctx.emit):
-            if acc is not None:
-                acc.buf = None  # a later backward pass starts a new sum
-            g_log_e = ge if log_e.dim() > 0 else ge.sum()
-            g_log_e = g_log_e.to(log_e.dtype).reshape(log_e.shape)
-        return g, g_log_e, None, None, None, None, None, None
+        return g, ge
 
 
-def rollout(d, nsteps, gravity, state=None, log_e=None, start_step=0, checkpoint_every=0):
+def rollout(d, nsteps, gravity, state=None, log_e=None, start_step=0, checkpoint_every=0,
+            graphs=False):
     """Differentiable rollout of ``nsteps`` steps from ``state`` (default: the
     current state of the simulation).  Returns the final (8, N) state.  With
     ``checkpoint_every=K`` only segment-start states are kept for backward:
     the forward pass runs K plain steps per segment, and the backward pass
     recomputes a segment's steps (saving what their adjoints read) when it
-    reaches it.  A rollout does not write the simulation's (or the
-    ensemble's) own particle block: its steps alternate between two private
-    blocks.  Use the returned tensors."""
+    reaches it.  With ``graphs=True`` (2D, checkpointed) each segment's steps
+    run as HIP graph replays, captured on first use per segment length (and
+    start step, when nodal forces depend on time), with the same results.  A
+    rollout does not write the simulation's (or the ensemble's) own particle
+    block: its steps alternate between two private blocks.  Use the returned
+    tensors."""
     if state is None:
         state = d.get_state()
     # log E (per particle or one scalar): the adjoint kernels of every step
@@ -445,10 +607,13 @@ def rollout(d, nsteps, gravity, state=None, log_e=None, start_step=0, checkpoint
     acc = _LogEGrad() if log_e is not None else None
 
     if checkpoint_every and checkpoint_every < nsteps:
+        gr = d._segment_graphs(checkpoint_every) if graphs else None
         for i0 in range(start_step, start_step + nsteps, checkpoint_every):
             k = min(checkpoint_every, start_step + nsteps - i0)
-            state = _SegmentFn.apply(state, log_e, d, gravity, i0, k, acc, i0 == start_step)
+            state = _SegmentFn.apply(state, log_e, d, gravity, i0, k, acc, i0 == start_step, gr)
         return state
+    if graphs:
+        raise ValueError("graphs=True needs checkpoint_every below nsteps")
     st = state
     for i in range(start_step, start_step + nsteps):
         st = _StepFn.apply(st, log_e, d, gravity, i, acc, i == start_step)

@@ -2914,14 +2914,20 @@ int enqueue_resort(dmpm_plan* P, const Mesh2<T>& M, const T* pb, int32_t* eid, h
   a.nchunks = P->nchunks;
   a.subT = P->subT;
   a.nsub = P->nsub;
-  const int g12 = std::max(P->nchunks, kSortGrid);
+  // grid-stride stages: up to kSortGrid workgroups, fewer for small plans
+  // (a thread per particle, a wave per sub-chunk, a thread per cell), at
+  // least one per chunk where a stage takes a chunk per workgroup
+  const int np = (int)((P->n + kThreads - 1) / kThreads);
+  const int nsubw = (P->nsub + kThreads / 64 - 1) / (kThreads / 64);
+  const int ncw = (P->ncells + 1 + kThreads - 1) / kThreads;
+  const int g12 = std::max(P->nchunks, std::min(kSortGrid, std::max(np, nsubw)));
   // (positions: rows 0 and 1 of pb, row stride ld)
   k_sort_s1<T><<<g12, kThreads, 0, s>>>(M, pb, ld < 0 ? P->d.ld : ld, a);
   k_sort_s2<<<g12, kThreads, 0, s>>>(a);
   // stages 3 and 4 have little to do in the incremental mode (the movers, a
   // flag): small grids, since scheduling idle workgroups costs more
-  k_sort_s3<<<kSortGrid / 4, kThreads, 0, s>>>(a);
-  k_sort_s4<<<kSortGrid / 8, kThreads, 0, s>>>(a);
+  k_sort_s3<<<std::max(1, std::min(kSortGrid / 4, np)), kThreads, 0, s>>>(a);
+  k_sort_s4<<<std::max(1, std::min(kSortGrid / 8, ncw)), kThreads, 0, s>>>(a);
   return launch_check("resort");
 }
 

@@ -25,9 +25,9 @@ FD_H_LOGE = 1e-3
 RTOL_FD = 2e-5
 
 
-def _sim(scheme, vu, dtype=torch.float64, seed=3, material="LinearElastic"):
+def _sim(scheme, vu, dtype=torch.float64, seed=3, material="LinearElastic", nodal_force=True):
     spec = cases.synthetic_spec("quad4", nx=10, ny=8, rows=5, seed=seed, scheme=scheme,
-                                velocity_update=vu, traction=False, nodal_force=True, E=1e6,
+                                velocity_update=vu, traction=False, nodal_force=nodal_force, E=1e6,
                                 dt=1e-5, nsteps=NSTEPS, per_particle_E=True, velocity_scale=0.5)
     if material == "SimpleMaterial":
         spec["material"] = {"type": "SimpleMaterial", "E": 1e6, "density": 1000.0}
@@ -42,9 +42,9 @@ def _weights(state0, seed=5):
     return (w / scale).to(state0.dtype)
 
 
-def _setup(scheme, vu, dtype=torch.float64, material="LinearElastic"):
+def _setup(scheme, vu, dtype=torch.float64, material="LinearElastic", nodal_force=True):
     from diffmpm_amd.autodiff import Differentiable
-    sim, spec = _sim(scheme, vu, dtype, material=material)
+    sim, spec = _sim(scheme, vu, dtype, material=material, nodal_force=nodal_force)
     d = Differentiable(sim)
     d.spec = spec  # the oracle's copy of the case
     g = torch.tensor(spec["gravity"], dtype=torch.float64)
@@ -208,6 +208,37 @@ def test_fp32_adjoint_matches_fp64(scheme):
     for a, b in ((gs32, gs64), (ge32, ge64)):
         err = (a.double() - b).norm() / b.norm()
         assert err < 1e-3, float(err)
+
+
+@pytest.mark.parametrize("scheme,dtype,nodal_force", [
+    ("usf", torch.float64, True), ("usl", torch.float32, True), ("usf", torch.float32, False)])
+def test_captured_segments_equal_eager_segments(scheme, dtype, nodal_force):
+    """rollout(..., graphs=True) gives the eager checkpointed rollout's final
+    state and gradients bit for bit, on two backward passes in a row (the
+    second replays the captured graphs), with segments of two lengths (3, 3,
+    1), per-particle and scalar log E; with a time-dependent nodal force each
+    segment has graphs of its own, without one the segments share them."""
+    from diffmpm_amd.autodiff import rollout
+    d, g, state0, log_e = _setup(scheme, False, dtype=dtype, nodal_force=nodal_force)
+    w = _weights(state0)
+
+    def run(graphs, scalar):
+        s = state0.clone().requires_grad_(True)
+        le = (log_e.mean() if scalar else log_e).clone().requires_grad_(True)
+        out = rollout(d, 7, g, state=s, log_e=le, checkpoint_every=3, graphs=graphs)
+        (w * out).sum().backward()
+        return out.detach(), s.grad.detach(), le.grad.detach()
+
+    for scalar in (False, True):
+        a = run(False, scalar)
+        b = run(True, scalar)
+        c = run(True, scalar)
+        for x, y, z in zip(a, b, c):
+            assert torch.equal(x, y) and torch.equal(x, z), scalar
+        assert torch.isfinite(a[1]).all() and a[2].abs().sum() > 0
+    # per segment length: a forward and a backward graph, per segment start
+    # when the nodal force depends on time
+    assert len(d._graphs.graphs) == (2 * 3 if nodal_force else 2 * 2)
 
 
 def test_scalar_log_e_gradient_is_the_sum_of_the_per_particle_terms():
