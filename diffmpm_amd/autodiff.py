@@ -602,6 +602,19 @@ def rollout(d, nsteps, gravity, state=None, log_e=None, start_step=0, checkpoint
     tensors."""
     if state is None:
         state = d.get_state()
+    if not torch.is_grad_enabled() or not (
+            state.requires_grad or (log_e is not None and log_e.requires_grad)):
+        # nothing to differentiate: plain steps, nothing saved
+        with torch.no_grad():
+            if graphs and checkpoint_every and checkpoint_every < nsteps:
+                gr = d._segment_graphs(checkpoint_every)
+                for i0 in range(start_step, start_step + nsteps, checkpoint_every):
+                    k = min(checkpoint_every, start_step + nsteps - i0)
+                    state = gr.forward(state, log_e, gravity, i0, k)
+                return state
+            for i in range(start_step, start_step + nsteps):
+                state, _ = d.forward_step(state, log_e, gravity, i, save=False)
+            return state
     # log E (per particle or one scalar): the adjoint kernels of every step
     # add dL/dlog(E_p) into one per-particle buffer, summed once for a scalar
     acc = _LogEGrad() if log_e is not None else None

@@ -737,6 +737,9 @@ template <typename T> struct StepArgs {
   int par;   // step parity
   int last;  // the run's last step: the node block and vkin are written (earlier steps
              // only pass the nodal velocity on through vnew)
+  const T* fdsrc;  // node block whose damping-force rows the step reads (nb or another
+                   // block of the same shape)
+  int wfd;         // on the last step, also write those rows to nb
   int nrb;   // workgroups of k_step_rounds that process tiles
   const SimRec* sims;   // batch plans: the simulation table (tile_off, tiles_x, ...)
   const int* tile_sim;  // batch plans: the simulation of each tile
@@ -930,8 +933,8 @@ __device__ __forceinline__ void node_pre(const StepArgs<T>& a, const TileCtx<T>&
     q.cr[w] = a.cidx ? a.cidx[nd] : -1;
     q.mask[w] = 0;
     if (fd) {
-      fd[w][0] = NB(DMPM_N_FDAMP, nd);
-      fd[w][1] = NB(DMPM_N_FDAMP + 1, nd);
+      fd[w][0] = bld<T>(a.fdsrc, a.nbytes, (uint32_t)DMPM_N_FDAMP * a.nld32, (uint32_t)nd);
+      fd[w][1] = bld<T>(a.fdsrc, a.nbytes, (uint32_t)(DMPM_N_FDAMP + 1) * a.nld32, (uint32_t)nd);
     }
   }
 }
@@ -1030,6 +1033,7 @@ __device__ __forceinline__ void force_epilogue(const StepArgs<T>& a, const TileC
         NB_W(DMPM_N_V + d, nd, v);
         NB_W(DMPM_N_A + d, nd, ac);
         NB_W(DMPM_N_P + d, nd, pm);
+        if (a.wfd) NB_W(DMPM_N_FDAMP + d, nd, fd[w][d]);
       }
       if (own) a.vnew[(int64_t)d * nnodes + nd] = v;
     }
@@ -1861,8 +1865,8 @@ __device__ __forceinline__ void node_pre_nt(const StepArgs<T>& a, const TileCtx<
     const int nd = c.noff + (NR.j0 + jr) * (nx + 1) + NR.i0 + (ln - jr * wn);
     q.cr[w] = a.cidx ? a.cidx[nd] : -1;
     if (fd) {
-      fd[w][0] = NB(DMPM_N_FDAMP, nd);
-      fd[w][1] = NB(DMPM_N_FDAMP + 1, nd);
+      fd[w][0] = bld<T>(a.fdsrc, a.nbytes, (uint32_t)DMPM_N_FDAMP * a.nld32, (uint32_t)nd);
+      fd[w][1] = bld<T>(a.fdsrc, a.nbytes, (uint32_t)(DMPM_N_FDAMP + 1) * a.nld32, (uint32_t)nd);
     }
     if (mass) mass[w] = NB(DMPM_N_MASS, nd);
   }
@@ -2197,6 +2201,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) void 
         NB_W(DMPM_N_FINT + d, nd, fi);
         NB_W(DMPM_N_V + d, nd, v);
         NB_W(DMPM_N_P + d, nd, pm);
+        if (a.wfd) NB_W(DMPM_N_FDAMP + d, nd, fd[w][d]);
       }
       // (G2P reads the acceleration from the node block)
       NB_W(DMPM_N_A + d, nd, ac);
@@ -3070,7 +3075,7 @@ int step_run(dmpm_plan* P, const Mesh2<T>& M, T* pa, T* pb2, T* pstat, int32_t* 
              int temps, hipStream_t s, T* vkin, T* sout = nullptr, int64_t sld = 0,
              const T* sin = nullptr, const int64_t* clock = nullptr, int kbase = 0,
              bool prepare = false, const StepScratch<T>* ext = nullptr,
-             const T* vprev = nullptr) {
+             const T* vprev = nullptr, const T* fdprev = nullptr) {
   constexpr bool kFast = sizeof(T) == 4;
   const int64_t ld = P->d.ld, nld = P->d.nld;
   const int nn = P->nnodes;
@@ -3108,6 +3113,10 @@ int step_run(dmpm_plan* P, const Mesh2<T>& M, T* pa, T* pb2, T* pstat, int32_t* 
   a.ld32 = (uint32_t)ld;
   a.nb = nb;
   a.nld32 = (uint32_t)nld;
+  // damping-force rows: read from fdprev (another node block of the same
+  // shape) and written to nb on the last step, or read from nb
+  a.fdsrc = fdprev ? fdprev : nb;
+  a.wfd = fdprev ? 1 : 0;
   a.pbytes = (uint32_t)((uint64_t)DMPM_P_ROWS * ld * sizeof(T));
   a.nbytes = (uint32_t)((uint64_t)DMPM_N_ROWS * nld * sizeof(T));
   a.vkin = vkin;
@@ -3281,27 +3290,23 @@ int tile_between_t(dmpm_plan* P, const T* pin, T* pout, int32_t* eid, T* nb, int
     k_tile_check<T><<<nblk(n), kThreads, 0, s>>>(M, px, pld, n, eid, P->key, move_log(P));
     DMPM_TRY(launch_check("tile_check"));
   }
-  if (nprev && nprev != nb) {
-    // the damping force of the previous node block (the step reads it from nb)
-    DMPM_TRY(hip_check(hipMemcpy2DAsync(nb + (int64_t)DMPM_N_FDAMP * nld, nld * sizeof(T),
-                                        nprev + (int64_t)DMPM_N_FDAMP * nld, nld * sizeof(T),
-                                        (size_t)P->nnodes * sizeof(T), 2,
-                                        hipMemcpyDeviceToDevice, s),
-                       "hipMemcpy2DAsync"));
-  }
+  const bool other = nprev && nprev != nb;
   if (step_eligible(P) && (uint64_t)8 * (uint64_t)sld * sizeof(T) < (1ull << 32))
-    // a massless node keeps the velocity of the previous node block (read
-    // there directly when it is another block)
+    // a massless node keeps the velocity of the previous node block, and the
+    // damping force is that block's (read there directly when it is another
+    // block, whose damping rows the step then writes to nb)
     return step_run<T>(P, M, const_cast<T*>(pin), pout, pout, eid, nb, scheme, vu, grav, dt,
                        step, 1, temps, s, vkin_out ? vkin_out : vkin, sout, sld, sin, nullptr,
-                       0, false, nullptr,
-                       (nprev && nprev != nb) ? nprev + (int64_t)DMPM_N_V * nld : nullptr);
-  if (nprev && nprev != nb)
-    DMPM_TRY(hip_check(hipMemcpy2DAsync(nb + (int64_t)DMPM_N_V * nld, nld * sizeof(T),
-                                        nprev + (int64_t)DMPM_N_V * nld, nld * sizeof(T),
-                                        (size_t)P->nnodes * sizeof(T), 2,
-                                        hipMemcpyDeviceToDevice, s),
-                       "hipMemcpy2DAsync"));
+                       0, false, nullptr, other ? nprev + (int64_t)DMPM_N_V * nld : nullptr,
+                       other ? nprev : nullptr);
+  if (other)
+    // the rows the per-phase step reads from the node block
+    for (const int r : {(int)DMPM_N_V, (int)DMPM_N_FDAMP})
+      DMPM_TRY(hip_check(hipMemcpy2DAsync(nb + (int64_t)r * nld, nld * sizeof(T),
+                                          nprev + (int64_t)r * nld, nld * sizeof(T),
+                                          (size_t)P->nnodes * sizeof(T), 2,
+                                          hipMemcpyDeviceToDevice, s),
+                         "hipMemcpy2DAsync"));
   const int runs[6][2] = {{DMPM_P_X, 4},          {DMPM_P_VOLUME, 2}, {DMPM_P_STRAIN, 2},
                           {DMPM_P_STRAIN + 3, 1}, {DMPM_P_STRESS, 4}, {DMPM_P_VOLSTRAIN_C, 1}};
   for (const auto& r : runs)
