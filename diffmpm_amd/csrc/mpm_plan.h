@@ -240,6 +240,23 @@ inline void prof_end(dmpm_plan* P, int cls, hipStream_t s) {
   c.pending[slot] = 1;
   c.n++;
 }
+// The event pair of the next timed launch of class cls (false when the
+// class is not timed): the caller records them itself (for example as the
+// start and stop events of one kernel dispatch) and then calls prof_commit.
+inline bool prof_slot(dmpm_plan* P, int cls, hipEvent_t* e0, hipEvent_t* e1) {
+  dmpm_plan::ProfClass& c = P->prof[cls];
+  if (!c.on) return false;
+  const int slot = c.n % kProfRing;
+  prof_drain_slot(c, slot);
+  *e0 = c.ev[2 * slot];
+  *e1 = c.ev[2 * slot + 1];
+  return true;
+}
+inline void prof_commit(dmpm_plan* P, int cls) {
+  dmpm_plan::ProfClass& c = P->prof[cls];
+  c.pending[c.n % kProfRing] = 1;
+  c.n++;
+}
 struct ProfScope {
   dmpm_plan* P;
   int cls;

@@ -28,6 +28,7 @@
 // in ascending particle index, then the four cells of a node in the order
 // (i-1,j-1), (i,j-1), (i-1,j), (i,j)); every path uses the same order and the
 // same per-term arithmetic, so all of them are bit-identical.
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -3028,10 +3029,33 @@ bool step_eligible(const dmpm_plan* P) {
 }
 
 // The kernels of one step (B: batch plan): the one-launch step (kind 1) or
-// the split step (kind 2).
+// the split step (kind 2).  With events e0, e1 (timing): k_step_fast's
+// dispatch records them at its start and end (FP32, kind 1); otherwise
+// they bracket the step's launches.
 template <typename T, bool B>
-void step_launch(const StepArgs<T>& a, bool usf, int kind, hipStream_t s) {
+void step_launch(const StepArgs<T>& a, bool usf, int kind, hipStream_t s,
+                 hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr) {
   constexpr bool kFast = sizeof(T) == 4;
+  if (e0 && !(kFast && kind == 1)) {
+    (void)hipEventRecord(e0, s);
+    step_launch<T, B>(a, usf, kind, s);
+    (void)hipEventRecord(e1, s);
+    return;
+  }
+  if (e0) {
+    if constexpr (kFast) {
+      if (usf) {
+        hipExtLaunchKernelGGL(k_step_fast<T, true, B>, dim3(a.ntiles), dim3(kFT), 0, s, e0, e1,
+                              0, a);
+        k_step_rounds<T, true, B><<<a.nrb + kTailBlocks, kFT, 0, s>>>(a);
+      } else {
+        hipExtLaunchKernelGGL(k_step_fast<T, false, B>, dim3(a.ntiles), dim3(kFT), 0, s, e0, e1,
+                              0, a);
+        k_step_rounds<T, false, B><<<a.nrb + kTailBlocks, kFT, 0, s>>>(a);
+      }
+    }
+    return;
+  }
   if (kind == 2) {
     k_split_kin<T, B, kSKinT, kSKinR><<<a.ntiles, kSKinT, 0, s>>>(a);
     if (usf) {
@@ -3193,11 +3217,15 @@ int step_run(dmpm_plan* P, const Mesh2<T>& M, T* pa, T* pb2, T* pstat, int32_t* 
     a.kofs = kbase + (int)k;
     a.par = (int)(k & 1);
     a.last = k == nsteps - 1 ? 1 : 0;
-    ProfScopeT ps_(P, KC_STEP, s);
+    // timing: the events of k_step_fast's own dispatch (the class's time is
+    // that kernel's), of the step's launches otherwise
+    hipEvent_t pe0 = nullptr, pe1 = nullptr;
+    const bool timed = prof_slot(P, KC_STEP, &pe0, &pe1);
     if (P->nsims > 1)
-      step_launch<T, true>(a, scheme == DMPM_USF, P->step_kernel, s);
+      step_launch<T, true>(a, scheme == DMPM_USF, P->step_kernel, s, pe0, pe1);
     else
-      step_launch<T, false>(a, scheme == DMPM_USF, P->step_kernel, s);
+      step_launch<T, false>(a, scheme == DMPM_USF, P->step_kernel, s, pe0, pe1);
+    if (timed) prof_commit(P, KC_STEP);
     DMPM_TRY(launch_check("step"));
   }
   return DMPM_OK;
