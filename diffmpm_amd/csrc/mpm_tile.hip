@@ -1110,6 +1110,7 @@ __global__ __launch_bounds__(kFT) __attribute__((amdgpu_waves_per_eu(4))) void k
   uint16_t* const tab = reinterpret_cast<uint16_t*>(ndl);
   __shared__ T colC[NCOL], colS[NCOL], rowC[NROW], rowS[NROW];
   __shared__ int segA[3][kSegRows], segP[3][kSegRows + 1];
+  __shared__ int rowOf[kKCAP / 32];  // row of R_H holding virtual index 32 c
   const TileCtx<T> c = tile_ctx<T, B>(a, xcd_pos(blockIdx.x, a.ntiles), true);
   const Grid2<T> g = c.g;
   const int nx = g.nx, ny = g.ny;
@@ -1141,6 +1142,7 @@ __global__ __launch_bounds__(kFT) __attribute__((amdgpu_waves_per_eu(4))) void k
         segA[q][r] = s0;
         len = cs[k + (R.i1 - R.i0)] - s0;
       }
+      const int own_len = len;
 #pragma unroll
       for (int o = 1; o < 64; o <<= 1) {
         const int y = __shfl_up(len, o, 64);
@@ -1148,6 +1150,11 @@ __global__ __launch_bounds__(kFT) __attribute__((amdgpu_waves_per_eu(4))) void k
       }
       if (r < kSegRows) segP[q][r + 1] = len;
       if (r == 0) segP[q][0] = 0;
+      // R_H: the row of every 32nd virtual index (a row's indices are
+      // [len - own_len, len))
+      if (q == 2 && r < R.j1 - R.j0)
+        for (int cq = (len - own_len + 31) >> 5; cq < min((len + 31) >> 5, kKCAP / 32); ++cq)
+          rowOf[cq] = r;
     } else if (q == 3) {
       // cell centres and scales 2/(X1 - X0) of the columns and rows of RH
       // (natural coordinates and inverse Jacobians, particle.py:180-206)
@@ -1176,6 +1183,24 @@ __global__ __launch_bounds__(kFT) __attribute__((amdgpu_waves_per_eu(4))) void k
     return;
   }
   constexpr int PPT = kPPT;
+  // slot and particle of each virtual index t + j*kFT of the R_H list
+  // (lanes past the end take particle 0): the row from the row table, so
+  // that the permutation loads are in flight while the slot table is built
+  int pp[PPT], sl[PPT];
+  unsigned live = 0;
+#pragma unroll
+  for (int j = 0; j < PPT; ++j) {
+    const int v = t + j * kFT;
+    const bool lv = v < SH;
+    int r = 0;
+    if (lv) {
+      r = rowOf[v >> 5];
+      while (segP[2][r + 1] <= v) ++r;
+      live |= 1u << j;
+    }
+    sl[j] = lv ? segA[2][r] + (v - segP[2][r]) : 0;
+    pp[j] = ident ? sl[j] : perm[sl[j]];
+  }
   // node tables of the epilogues, loaded ahead
   NodePre<NK> kpre;
   NodePre<NF> fpre;
@@ -1187,38 +1212,42 @@ __global__ __launch_bounds__(kFT) __attribute__((amdgpu_waves_per_eu(4))) void k
   int kv0[CK], kv1[CK];
   cell_ranges_of<CK>(hr, hs0, hs1, segA[2], segP[2], kv0, kv1);
   fill_slot_cells<CK>(tab, hr, hcol, kv0, kv1, 0, SH);
+  // the fields every slot needs, issued before the barrier
+  T x[PPT], y[PPT], vx[PPT], vy[PPT], m[PPT];
+#pragma unroll
+  for (int j = 0; j < PPT; ++j) {
+    const int p = pp[j];
+    x[j] = SR(DMPM_P_X, p);
+    y[j] = SR(DMPM_P_X + 1, p);
+    vx[j] = SR(DMPM_P_V, p);
+    vy[j] = SR(DMPM_P_V + 1, p);
+    m[j] = PB(DMPM_P_MASS, p);
+  }
   __syncthreads();
-  // slot, cell and membership of each virtual index t + j*kFT of the R_H
-  // list (lanes past the end take particle 0 and are flagged off)
-  int pp[PPT], v1i[PPT], Ii[PPT], Ji[PPT];
-  unsigned own = 0, live = 0;
+  node_pre_mask<T, NK>(a, kpre);
+  node_pre_mask<T, NF>(a, fpre);
+  // cell and membership of each slot
+  int v1i[PPT], Ii[PPT], Ji[PPT];
+  unsigned own = 0;
 #pragma unroll
   for (int j = 0; j < PPT; ++j) {
     const int v = t + j * kFT;
-    const bool lv = v < SH;
+    const bool lv = (live >> j) & 1u;
     const int code = lv ? (int)tab[v] : 0;
-    const int r = code >> 8;
-    const int s = lv ? segA[2][r] + (v - segP[2][r]) : 0;
-    if (lv) live |= 1u << j;
-    pp[j] = s;
     // the slot's cell is the element search's result for this position (the
     // re-sort keyed the slot by it)
-    const int I = RH.i0 + (code & 255), J = RH.j0 + r;
+    const int I = RH.i0 + (code & 255), J = RH.j0 + (code >> 8);
     Ii[j] = I;
     Ji[j] = J;
     v1i[j] = -1;
     if (lv && in_rgn(R1, I, J)) {
       const int r1 = J - R1.j0;
-      v1i[j] = segP[1][r1] + (s - segA[1][r1]);
+      v1i[j] = segP[1][r1] + (sl[j] - segA[1][r1]);
     }
     if (lv && in_rgn(R0, I, J)) own |= 1u << j;
   }
-  if (!ident)
-#pragma unroll
-    for (int j = 0; j < PPT; ++j) pp[j] = perm[pp[j]];
-  // all particle loads of the step, issued together (lanes without a
+  // the other particle loads of the step, issued together (lanes without a
   // field read particle 0's)
-  T x[PPT], y[PPT], vx[PPT], vy[PPT], m[PPT];
   T szz[PPT], ex[PPT], ey[PPT], gxy[PPT], rho[PPT], evc[PPT];
   {
     T sxx[PPT], syy[PPT], sxy[PPT], V[PPT], E[PPT];
@@ -1227,11 +1256,6 @@ __global__ __launch_bounds__(kFT) __attribute__((amdgpu_waves_per_eu(4))) void k
       const int p = pp[j];
       const int p1 = v1i[j] >= 0 ? p : 0;
       const int po = (own & (1u << j)) ? p : 0;
-      x[j] = SR(DMPM_P_X, p);
-      y[j] = SR(DMPM_P_X + 1, p);
-      vx[j] = SR(DMPM_P_V, p);
-      vy[j] = SR(DMPM_P_V + 1, p);
-      m[j] = PB(DMPM_P_MASS, p);
       sxx[j] = SR(DMPM_P_STRESS + 0, p1);
       syy[j] = SR(DMPM_P_STRESS + 1, p1);
       sxy[j] = SR(DMPM_P_STRESS + 3, p1);
@@ -1265,8 +1289,6 @@ __global__ __launch_bounds__(kFT) __attribute__((amdgpu_waves_per_eu(4))) void k
       s1f[4 * kFCAP + q] = E[j];
     }
   }
-  node_pre_mask<T, NK>(a, kpre);
-  node_pre_mask<T, NF>(a, fpre);
   // ---- K ----
   T xi[PPT], eta[PPT];
 #pragma unroll
