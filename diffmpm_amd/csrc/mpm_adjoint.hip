@@ -488,6 +488,7 @@ constexpr int kACells = (kStepTile + 1) * (kStepTile + 1);  // cells of RC (at m
 constexpr int kANodes = (kStepTile + 1) * (kStepTile + 1);  // owned nodes (at most)
 constexpr int kAVN = (kStepTile + 2) * (kStepTile + 2);     // nodes of RC's cells
 constexpr int kAdjOutsideBlocks = 8;
+constexpr int kARow32 = 64;  // entries of a tile's row table (virtual indices up to 2048)
 
 // Particle geometry of a slot of RC: the slot's cell (local column ci, row r
 // of RC) from the slot table, natural coordinates and scales from the
@@ -528,7 +529,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) void 
   __shared__ T nvv[4 * kAVN];  // nodal vx, vy, ax, ay at the nodes of RC's cells
   __shared__ T colC[kStepTile + 1], colS[kStepTile + 1], rowC[kStepTile + 1],
       rowS[kStepTile + 1];
-  __shared__ int segA[kStepTile + 1], segP[kStepTile + 2];
+  __shared__ int segA[kStepTile + 1], segP[kStepTile + 2], rowOf[kARow32];
   const AdjArgs<T>& a = A.a;
   const PRows<T>& R = a.R;
   const int64_t ld = a.ld, nld = a.nld;
@@ -562,7 +563,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) void 
   const int nnodes = A.nn_all;
   int cr_[CC], ccol[CC], cs0[CC], cs1[CC];
   cell_slots_rc<CC, NT>(RC, nx, c.coff, cs, cr_, ccol, cs0, cs1);
-  if (t < 64) row_segments(RC, nx, c.coff, cs, segA, segP);
+  if (t < 64) row_segments(RC, nx, c.coff, cs, segA, segP, rowOf, kARow32);
   else if (t < 128) cell_tables(g, RC, colC, colS, rowC, rowS);
   {
     const SmallDiv dwv = small_div(wV);
@@ -622,9 +623,26 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) void 
   for (int v0 = 0; v0 < S; v0 += CAP) {
     const int vend = min(v0 + CAP, S);
     if (v0 > 0) __syncthreads();
+    // the particle of each of this thread's slots of the round (from the row
+    // table: the permutation loads are in flight while the slot table is
+    // built)
+    int pp[RPT];
+#pragma unroll
+    for (int j = 0; j < RPT; ++j) {
+      const int v = v0 + t + j * NT;
+      const int s = v < vend ? slot_of_virtual(v, rowOf, kARow32, segA, segP) : 0;
+      pp[j] = ident ? s : perm[s];
+    }
     fill_slot_cells<CC>(tab, cr_, ccol, cv0, cv1, v0, vend);
-    __syncthreads();
     if constexpr (USL) {
+      // (slot q's particle shares LDS with slot q's element of the last
+      // weight row, which its thread writes after reading the particle)
+      static_assert(sizeof(T) % sizeof(int) == 0, "slot particle in a weight element");
+      constexpr int kSp = sizeof(T) / sizeof(int);
+      int* const spp = reinterpret_cast<int*>(st + (1 + NW) * CAP);
+#pragma unroll
+      for (int j = 0; j < RPT; ++j) spp[(t + j * NT) * kSp] = pp[j];
+      __syncthreads();
       // one slot at a time (the G2P and stress adjoint of several slots
       // held together spill registers)
 #pragma unroll 1
@@ -633,8 +651,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) void 
         if (v0 + q >= vend) break;
         const int code = tab[q];
         const int r = code >> 8, ci = code & 255;
-        const int s = segA[r] + (v0 + q - segP[r]);
-        const int p = ident ? s : perm[s];
+        const int p = spp[q * kSp];
         const bool mine = r >= oj && ci >= oi;
         const PGeo<T> gq = tile_geo(R.x[p], R.y[p], ci, r, wV, colC, colS, rowC, rowS);
         T vx[4], vy[4], ax[4], ay[4];
@@ -663,18 +680,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) void 
         for (int r2 = 0; r2 < NW; ++r2) st[(2 + r2) * CAP + q] = w[W_NVX + r2];
       }
     } else {
-      int pp[RPT], code[RPT];
-      unsigned own = 0;
-  #pragma unroll
-      for (int j = 0; j < RPT; ++j) {
-        const int q = t + j * NT;
-        const bool live = v0 + q < vend;
-        code[j] = live ? (int)tab[q] : 0;
-        const int r = code[j] >> 8, ci = code[j] & 255;
-        const int s = live ? segA[r] + (v0 + q - segP[r]) : 0;
-        if (live && r >= oj && ci >= oi) own |= 1u << j;
-        pp[j] = ident ? s : perm[s];
-      }
+      // the slots' fields, issued before the barrier
       T x[RPT], y[RPT], xbx[RPT], xby[RPT], vbx[RPT], vby[RPT];
       T sbx[RPT], sby[RPT], sbq[RPT], vb[RPT], Ep[RPT], Vin[RPT];
   #pragma unroll
@@ -694,6 +700,17 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) void 
           Vin[j] = R.V[p];
           Ep[j] = a.mat.per_E ? R.E[p] : T(0);
         }
+      }
+      __syncthreads();
+      int code[RPT];
+      unsigned own = 0;
+  #pragma unroll
+      for (int j = 0; j < RPT; ++j) {
+        const int q = t + j * NT;
+        const bool live = v0 + q < vend;
+        code[j] = live ? (int)tab[q] : 0;
+        const int r = code[j] >> 8, ci = code[j] & 255;
+        if (live && r >= oj && ci >= oi) own |= 1u << j;
       }
   #pragma unroll
       for (int j = 0; j < RPT; ++j) {
@@ -821,7 +838,12 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) void 
   __shared__ T nvv[4 * kAVN];  // force cotangents fx, fy and kinematic velocity vx, vy
   __shared__ T colC[kStepTile + 1], colS[kStepTile + 1], rowC[kStepTile + 1],
       rowS[kStepTile + 1];
-  __shared__ int segA[kStepTile + 1], segP[kStepTile + 2];
+  __shared__ int segA[kStepTile + 1], segP[kStepTile + 2], rowOf[kARow32];
+  // slot q's particle shares LDS with slot q's element of the last staged
+  // row, which its thread writes after reading the particle
+  static_assert(sizeof(T) % sizeof(int) == 0, "slot particle in a staged element");
+  constexpr int kSp = sizeof(T) / sizeof(int);
+  int* const spp = reinterpret_cast<int*>(st + (1 + NW) * CAP);
   const AdjArgs<T>& a = A.a;
   const PRows<T>& R = a.R;
   const POut<T>& O = a.O;
@@ -848,7 +870,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) void 
   const int nnodes = A.nn_all;
   int cr_[CC], ccol[CC], cs0[CC], cs1[CC];
   cell_slots_rc<CC, NT>(RC, nx, c.coff, cs, cr_, ccol, cs0, cs1);
-  if (t < 64) row_segments(RC, nx, c.coff, cs, segA, segP);
+  if (t < 64) row_segments(RC, nx, c.coff, cs, segA, segP, rowOf, kARow32);
   else if (t < 128) cell_tables(g, RC, colC, colS, rowC, rowS);
   {
     const SmallDiv dwv = small_div(wV);
@@ -904,7 +926,21 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) void 
   for (int v0 = 0; v0 < S; v0 += CAP) {
     const int vend = min(v0 + CAP, S);
     if (v0 > 0) __syncthreads();
-    fill_slot_cells<CC>(tab, cr_, ccol, cv0, cv1, v0, vend);
+    // the particle of each of this thread's slots of the round (from the row
+    // table: the permutation loads are in flight while the slot table is
+    // built), kept in LDS for the loop below
+    {
+      int pp[RPT];
+#pragma unroll
+      for (int j = 0; j < RPT; ++j) {
+        const int v = v0 + t + j * NT;
+        const int s = v < vend ? slot_of_virtual(v, rowOf, kARow32, segA, segP) : 0;
+        pp[j] = ident ? s : perm[s];
+      }
+      fill_slot_cells<CC>(tab, cr_, ccol, cv0, cv1, v0, vend);
+#pragma unroll
+      for (int j = 0; j < RPT; ++j) spp[(t + j * NT) * kSp] = pp[j];
+    }
     __syncthreads();
     // one slot at a time (the force and stress adjoint of several slots held
     // together spill registers)
@@ -914,8 +950,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) void 
       if (v0 + q >= vend) break;
       const int code = tab[q];
       const int r = code >> 8, ci = code & 255;
-      const int s = segA[r] + (v0 + q - segP[r]);
-      const int p = ident ? s : perm[s];
+      const int p = spp[q * kSp];
       const bool mine = r >= oj && ci >= oi;
       const PGeo<T> gq = tile_geo(R.x[p], R.y[p], ci, r, wV, colC, colS, rowC, rowS);
       T fx[4], fy[4], vkx[4], vky[4];

@@ -127,9 +127,11 @@ __device__ __forceinline__ Rgn owned_nodes(const Rgn& R0, int nx, int ny) {
 
 // Slot segments of the rows of region R, computed by the calling wave (lane r:
 // row r): A[r] = first slot of row r's cells, P[r] = slots in the rows before
-// r (P[rows] = all).
+// r (P[rows] = all).  With rowOf (nrow32 entries): rowOf[c] = the row holding
+// virtual index 32 c, for every 32 c below min(P[rows], 32 nrow32).
 __device__ __forceinline__ void row_segments(const Rgn& R, int nx, int coff, const int* cs,
-                                             int* A, int* P) {
+                                             int* A, int* P, int* rowOf = nullptr,
+                                             int nrow32 = 0) {
   const int r = threadIdx.x & 63;
   int len = 0;
   if (r < R.j1 - R.j0) {
@@ -138,6 +140,7 @@ __device__ __forceinline__ void row_segments(const Rgn& R, int nx, int coff, con
     A[r] = s0;
     len = cs[k + (R.i1 - R.i0)] - s0;
   }
+  const int own_len = len;
 #pragma unroll
   for (int o = 1; o < 64; o <<= 1) {
     const int y = __shfl_up(len, o, 64);
@@ -145,6 +148,18 @@ __device__ __forceinline__ void row_segments(const Rgn& R, int nx, int coff, con
   }
   if (r < R.j1 - R.j0) P[r + 1] = len;
   if (r == 0) P[0] = 0;
+  // row r holds the virtual indices [len - own_len, len)
+  if (rowOf && r < R.j1 - R.j0)
+    for (int c = (len - own_len + 31) >> 5; c < min((len + 31) >> 5, nrow32); ++c) rowOf[c] = r;
+}
+
+// Slot of virtual index v (below P[rows]) from the row table (nrow32 entries)
+// and the slot segments of row_segments.
+__device__ __forceinline__ int slot_of_virtual(int v, const int* rowOf, int nrow32, const int* A,
+                                               const int* P) {
+  int r = rowOf[min(v >> 5, nrow32 - 1)];
+  while (P[r + 1] <= v) ++r;
+  return A[r] + (v - P[r]);
 }
 
 // Cell centres and scales 2/(X1 - X0) of the columns and rows of region R
