@@ -54,7 +54,7 @@ ADJOINT_BYTES = 4 * 9 + 4 * 8 * 2 + 4
 # passes of this benchmark: reads = TCC_EA0_RDREQ
 # x 128 B, writes = WRITE_SIZE (calibrated on a 512 MiB device copy; see
 # profiles/step_q4_1024_usf_hbm_counters.md)
-TRAFFIC_BYTES = {"step": 135.0, "sort": 13.5, "tile_kin": 57.7, "stress": 91.0,
+TRAFFIC_BYTES = {"step": 135.8, "sort": 13.5, "tile_kin": 57.7, "stress": 91.0,
                  "tile_force": 86.4, "tile_g2p": 38.1}
 
 
