@@ -481,7 +481,7 @@ template <typename T> struct AdjTile {
 // threads, slots per thread and round, and waves per SIMD to fit (register
 // budget) of each tile kernel
 template <int MODE> struct AdjTileShape;
-template <> struct AdjTileShape<0> { static constexpr int NT = 512, RPT = 3, WPE = 1; };
+template <> struct AdjTileShape<0> { static constexpr int NT = 512, RPT = 3, WPE = 4; };
 template <> struct AdjTileShape<1> { static constexpr int NT = 512, RPT = 3, WPE = 1; };
 template <> struct AdjTileShape<2> { static constexpr int NT = 512, RPT = 3, WPE = 1; };
 constexpr int kACells = (kStepTile + 1) * (kStepTile + 1);  // cells of RC (at most)
@@ -525,7 +525,13 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) void 
   constexpr bool USL = MODE == 0;
   static_assert(4 * NQ * kACells <= (2 + NW) * CAP, "node-sum slices");
   __shared__ T st[(2 + NW) * CAP];
-  __shared__ uint16_t tab[CAP];
+  // USL: slot q's table entry lies in slot q's element of staged row NW (the
+  // slot's thread reads it before it writes its weights there), which keeps
+  // the kernel under half the CU's LDS
+  __shared__ uint16_t tab_[USL ? 1 : CAP];
+  static_assert(sizeof(T) % sizeof(uint16_t) == 0, "slot table in a weight element");
+  constexpr int kTs = USL ? (int)(sizeof(T) / sizeof(uint16_t)) : 1;
+  uint16_t* const tab = USL ? reinterpret_cast<uint16_t*>(st + NW * CAP) : tab_;
   __shared__ T nvv[4 * kAVN];  // nodal vx, vy, ax, ay at the nodes of RC's cells
   __shared__ T colC[kStepTile + 1], colS[kStepTile + 1], rowC[kStepTile + 1],
       rowS[kStepTile + 1];
@@ -633,7 +639,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) void 
       const int s = v < vend ? slot_of_virtual(v, rowOf, kARow32, segA, segP) : 0;
       pp[j] = ident ? s : perm[s];
     }
-    fill_slot_cells<CC>(tab, cr_, ccol, cv0, cv1, v0, vend);
+    fill_slot_cells<CC>(tab, cr_, ccol, cv0, cv1, v0, vend, kTs);
     if constexpr (USL) {
       // (slot q's particle shares LDS with slot q's element of the last
       // weight row, which its thread writes after reading the particle)
@@ -649,9 +655,11 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) void 
       for (int j = 0; j < RPT; ++j) {
         const int q = t + j * NT;
         if (v0 + q >= vend) break;
-        const int code = tab[q];
+        const int code = tab[q * kTs];
         const int r = code >> 8, ci = code & 255;
         const int p = spp[q * kSp];
+        // (both entries are read before this thread's weight stores reuse them)
+        asm volatile("" ::: "memory");
         const bool mine = r >= oj && ci >= oi;
         const PGeo<T> gq = tile_geo(R.x[p], R.y[p], ci, r, wV, colC, colS, rowC, rowS);
         T vx[4], vy[4], ax[4], ay[4];
@@ -708,7 +716,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) void 
       for (int j = 0; j < RPT; ++j) {
         const int q = t + j * NT;
         const bool live = v0 + q < vend;
-        code[j] = live ? (int)tab[q] : 0;
+        code[j] = live ? (int)tab[q * kTs] : 0;
         const int r = code[j] >> 8, ci = code[j] & 255;
         if (live && r >= oj && ci >= oi) own |= 1u << j;
       }
@@ -951,6 +959,8 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) void 
       const int code = tab[q];
       const int r = code >> 8, ci = code & 255;
       const int p = spp[q * kSp];
+      // (the particle is read before this thread's staged stores reuse it)
+      asm volatile("" ::: "memory");
       const bool mine = r >= oj && ci >= oi;
       const PGeo<T> gq = tile_geo(R.x[p], R.y[p], ci, r, wV, colC, colS, rowC, rowS);
       T fx[4], fy[4], vkx[4], vky[4];

@@ -270,18 +270,19 @@ __device__ __forceinline__ void node_sums_nt(T* sl, const T (&acc)[C][4 * NQ], c
 }
 
 // Slot table of the round [v0, vend) of a region's slot list: entry q (slot
-// v0 + q) holds (row << 8) | column of its cell, written by the threads that
-// own the cells (virtual ranges [cv0, cv1)).
+// v0 + q, at tab[q * stride]) holds (row << 8) | column of its cell, written
+// by the threads that own the cells (virtual ranges [cv0, cv1)).
 template <int C>
 __device__ __forceinline__ void fill_slot_cells(uint16_t* tab, const int (&r)[C],
                                                 const int (&col)[C], const int (&cv0)[C],
-                                                const int (&cv1)[C], int v0, int vend) {
+                                                const int (&cv1)[C], int v0, int vend,
+                                                int stride = 1) {
 #pragma unroll
   for (int u = 0; u < C; ++u) {
     if (r[u] < 0) continue;
     const uint16_t code = (uint16_t)((r[u] << 8) | col[u]);
     const int q1 = min(cv1[u], vend) - v0;
-    for (int q = max(cv0[u], v0) - v0; q < q1; ++q) tab[q] = code;
+    for (int q = max(cv0[u], v0) - v0; q < q1; ++q) tab[q * stride] = code;
   }
 }
 
