@@ -170,11 +170,14 @@ class Differentiable:
         cache[key] = (log_e, log_e._version)
 
     # -- one step ---------------------------------------------------------------
-    def forward_step(self, state, log_e, gravity, i, save=False):
+    def forward_step(self, state, log_e, gravity, i, save=False, prev=None):
         """One forward step; with save=True also returns what its adjoint
         needs (node block, kinematic nodal velocity, sort by cell).  A 2D
         simulation or ensemble steps between two private particle blocks (its
-        own block is not written; a 1D one on one private block)."""
+        own block is not written; a 1D one on one private block).  prev: for
+        a saved step, the node block whose velocity rows (what a massless node
+        keeps) and damping-force rows the step reads (default: the last saved
+        step's block when chained, else the simulation's node block)."""
         if self.dim == 1:
             return self._forward_1d(state, log_e, gravity, i), None
         if self._blocks is None:
@@ -183,7 +186,7 @@ class Differentiable:
         # blocks carry the other rows (strain, density, sigma_zz, volumetric
         # strain) and the constants
         last = getattr(self, "_last_b", None)
-        chained = last is not None and last[0] is state and state._version == last[1]
+        chained = self._chained(state)
         src = last[2] if last is not None else 0
         dst = 1 - src
         if log_e is not None:
@@ -202,7 +205,8 @@ class Differentiable:
             # coordinate rows are not set (neither the step nor the adjoint
             # reads them).  The kinematic nodal velocity and the sort by cell
             # go straight to the adjoint's buffers.
-            prev = self._nodes_last if chained and self._nodes_last is not None else self._nbuf
+            if prev is None:
+                prev = self._prev_node_block(chained)
             nb = torch.empty_like(self._nbuf)
             vkin, perm, cs = self._export_buffers()
             call("dmpm_step_between_saved", plan.h, ptr(self._blocks[src]),
@@ -212,7 +216,16 @@ class Differentiable:
             self._last_b = (out, out._version, dst)
             self._nodes_last = nb
             return out, (nb, vkin, perm, cs)
-        nb = self._nbuf
+        # a plain step writes a private node block and reads the velocity a
+        # massless node keeps (and the damping force) from that block: bring
+        # in those rows of the previous step's block when it is another one
+        nb = self.__dict__.get("_nplain")
+        if nb is None:
+            nb = self._nplain = self._nbuf.clone()
+        if prev is None:
+            prev = self._prev_node_block(chained)
+        if prev is not nb:
+            self._prev_block(self._prev_rows(prev), nb)
         call("dmpm_step_between", plan.h, ptr(self._blocks[src]), ptr(self._blocks[dst]),
              ptr(self._eid_b), ptr(nb), self.code, self.vu, plan._gravity(gravity), self.dt,
              int(i), 2 if chained else 0, ptr(sin), ptr(out), self.n, plan.s)
@@ -251,6 +264,39 @@ class Differentiable:
         perm = torch.empty(max(self.n, 1), dtype=torch.int32, device=self._nbuf.device)
         cs = torch.empty(self._ncells() + 2, dtype=torch.int32, device=self._nbuf.device)
         return vkin, perm, cs
+
+    # node rows a step reads from the previous node block: the velocity a
+    # massless node keeps and the damping force
+    _PREV_ROWS = ((_lib.N_V, _lib.N_V + 2), (_lib.N_FDAMP, _lib.N_FDAMP + 2))
+
+    def _chained(self, state):
+        """Whether a step from state continues the last private-block step
+        (its output, unchanged since)."""
+        last = getattr(self, "_last_b", None)
+        return last is not None and last[0] is state and state._version == last[1]
+
+    def _prev_node_block(self, chained):
+        """The node block a step reads the massless-node velocity and damping
+        force from: the last step's when it continues it, else the
+        simulation's node block (a rollout starts from the simulation's
+        nodes)."""
+        if chained and self._nodes_last is not None:
+            return self._nodes_last
+        return self._nbuf
+
+    def _prev_rows(self, nb):
+        """The rows of node block nb that the next step reads from it."""
+        return torch.cat([nb[a:b] for a, b in self._PREV_ROWS])
+
+    def _prev_block(self, rows, out=None):
+        """A node block holding rows (from _prev_rows) where a step reads them."""
+        if out is None:
+            out = torch.empty_like(self._nbuf)
+        k = 0
+        for a, b in self._PREV_ROWS:
+            out[a:b].copy_(rows[k:k + b - a])
+            k += b - a
+        return out
 
     def _new_plan(self):
         """A new plan of the simulation or ensemble (its own sort state and
@@ -422,6 +468,7 @@ class _SegmentGraphs:
         self.pin = pb.clone()
         self.eid = d._eid_f.clone()
         self.nbf = nbuf.clone()  # node block of the plain steps
+        self.nb0 = nbuf.clone()  # the recomputation's previous node block
         n, ld, dev, dt = d.n, pb.shape[1], pb.device, pb.dtype
         k = self.kmax
         self.S = torch.empty((k + 1, C_ROWS, n), dtype=dt, device=dev)
@@ -451,7 +498,7 @@ class _SegmentGraphs:
 
     def _saved(self, j, i, grav):
         d = self.d
-        prev = self.nbf if j == 0 else self.NB[j - 1]
+        prev = self.nb0 if j == 0 else self.NB[j - 1]
         call("dmpm_step_between_saved", self.fp.h, ptr(self.blocks[j % 2]),
              ptr(self.blocks[1 - j % 2]), ptr(self.eid), ptr(prev), ptr(self.NB[j]), d.code,
              d.vu, grav, d.dt, int(i), 2 if j else 0, ptr(self.S[j]), ptr(self.S[j + 1]), d.n,
@@ -494,17 +541,25 @@ class _SegmentGraphs:
         self.graphs[key] = g
         return g
 
-    def forward(self, state, log_e, gravity, i0, k):
-        """The state after steps i0 .. i0 + k - 1 from ``state``."""
+    def forward(self, state, log_e, gravity, i0, k, first=False):
+        """The state after steps i0 .. i0 + k - 1 from ``state``, and the node
+        rows its first step read from the previous node block (_prev_rows).
+        first: the rollout's first segment, which starts from the
+        simulation's node block (later ones from the last segment's)."""
         self.set_input(state, log_e)
         g = self._graph("fwd", i0, k, gravity)
+        if first:
+            self.d._prev_block(self.d._prev_rows(self.d._nbuf), self.nbf)
+        rows = self.d._prev_rows(self.nbf)
         g.replay()
-        return self.S[k].clone()
+        return self.S[k].clone(), rows
 
-    def backward(self, state, log_e, gravity, i0, k, gout, first):
+    def backward(self, state, log_e, gravity, i0, k, gout, first, prev_rows):
         """Cotangent of the segment's input state from ``gout``; the log E
-        terms are added to ``self.lebar`` (cleared first when ``first``)."""
+        terms are added to ``self.lebar`` (cleared first when ``first``).
+        prev_rows: the node rows the segment's first step read (_prev_rows)."""
         self.set_input(state, log_e)
+        self.d._prev_block(prev_rows, self.nb0)
         g = self._graph("bwd", i0, k, gravity)
         self.cot[:, : self.d.n].copy_(gout)
         if first:
@@ -524,9 +579,12 @@ class _SegmentFn(torch.autograd.Function):
 
     @staticmethod
     def forward(ctx, state, log_e, d, gravity, i0, k, acc=None, emit=False, gr=None):
+        # the node rows the segment's first step reads from the previous node
+        # block, so that its recomputation repeats it exactly
         if gr is not None:
-            st = gr.forward(state, log_e, gravity, i0, k)
+            st, ctx.prev_rows = gr.forward(state, log_e, gravity, i0, k, first=emit)
         else:
+            ctx.prev_rows = d._prev_rows(d._prev_node_block(d._chained(state)))
             st = state
             for i in range(i0, i0 + k):
                 st, _ = d.forward_step(st, log_e, gravity, i, save=False)
@@ -550,10 +608,12 @@ class _SegmentFn(torch.autograd.Function):
             if acc is not None:
                 acc.buf = gr.lebar
             with torch.no_grad():
-                g = gr.backward(state, le, gravity, i0, k, gout, first)
+                g = gr.backward(state, le, gravity, i0, k, gout, first, ctx.prev_rows)
             ge = gr.lebar[: d.n]
         else:
-            g, ge = _SegmentFn._eager_backward(d, state, le, gravity, i0, k, gout, acc)
+            g, ge = _SegmentFn._eager_backward(d, state, le, gravity, i0, k, gout, acc,
+                                               ctx.prev_rows)
+        ctx.prev_rows = None
         g_log_e = None
         if want_e and (acc is None or ctx.emit):
             if acc is not None:
@@ -563,7 +623,7 @@ class _SegmentFn(torch.autograd.Function):
         return g, g_log_e, None, None, None, None, None, None, None
 
     @staticmethod
-    def _eager_backward(d, state, le, gravity, i0, k, gout, acc):
+    def _eager_backward(d, state, le, gravity, i0, k, gout, acc, prev_rows):
         lebar = None
         if acc is not None:
             if acc.buf is None:
@@ -572,8 +632,10 @@ class _SegmentFn(torch.autograd.Function):
         save = d.save_forward
         with torch.no_grad():
             states, saved = [state.detach()], []
+            prev = d._prev_block(prev_rows) if save else None
             for i in range(i0, i0 + k):
-                out, sv = d.forward_step(states[-1], le, gravity, i, save=save)
+                out, sv = d.forward_step(states[-1], le, gravity, i, save=save,
+                                         prev=prev if i == i0 else None)
                 states.append(out)
                 saved.append(sv)
             g, ge = gout, None
@@ -610,7 +672,7 @@ def rollout(d, nsteps, gravity, state=None, log_e=None, start_step=0, checkpoint
                 gr = d._segment_graphs(checkpoint_every)
                 for i0 in range(start_step, start_step + nsteps, checkpoint_every):
                     k = min(checkpoint_every, start_step + nsteps - i0)
-                    state = gr.forward(state, log_e, gravity, i0, k)
+                    state, _ = gr.forward(state, log_e, gravity, i0, k, first=i0 == start_step)
                 return state
             for i in range(start_step, start_step + nsteps):
                 state, _ = d.forward_step(state, log_e, gravity, i, save=False)

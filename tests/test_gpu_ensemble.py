@@ -109,6 +109,40 @@ def test_ensemble_log_e_gradients_equal_members_alone(scheme):
         assert math.isclose(float(le.grad[i]), ga, rel_tol=1e-10, abs_tol=1e-14), (i, ga)
 
 
+@pytest.mark.parametrize("scheme", ["usf", "usl"])
+def test_ensemble_checkpointed_rollout_graphs_equal_eager(scheme):
+    """An ensemble's checkpointed rollout with captured segment graphs gives
+    the eager segments' final state and per-member d/dlogE bit for bit (two
+    backward passes in a row replay the graphs), and both equal the
+    uncheckpointed rollout; the nodal force's time function gives each
+    segment start graphs of its own."""
+    from diffmpm_amd.autodiff import Differentiable, rollout
+    from diffmpm_amd.ensemble import Ensemble
+    dtype = torch.float64
+    g = _grav(_spec(*MEMBERS[0], scheme))
+    ens = Ensemble(_members(scheme, dtype), per_particle_E=True)
+    d = Differentiable.of_ensemble(ens)
+    idx = ens.sim_index()
+    s0 = d.get_state()
+    w = torch.linspace(-1, 1, s0.numel(), dtype=dtype, device="cuda").reshape(s0.shape)
+
+    def run(ck, graphs):
+        le = torch.tensor([math.log(m[4]) for m in MEMBERS], dtype=dtype, device="cuda",
+                          requires_grad=True)
+        out = rollout(d, 7, g, state=s0, log_e=le[idx], checkpoint_every=ck, graphs=graphs)
+        (w * out).sum().backward()
+        return out.detach(), le.grad.detach()
+
+    plain = run(0, False)
+    eager = run(3, False)
+    for x, y in zip(plain, eager):
+        assert torch.equal(x, y)
+    for _ in range(2):
+        for x, y in zip(plain, run(3, True)):
+            assert torch.equal(x, y)
+    assert len(d._graphs.graphs) == 2 * 3
+
+
 def _mixed_specs(scheme):
     """Members of several tiles per side (one 10 tiles wide: two strips of
     the tile order, the second narrower), a small one, and a 4 x 4 mesh with
