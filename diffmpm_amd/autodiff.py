@@ -174,10 +174,10 @@ class Differentiable:
         """One forward step; with save=True also returns what its adjoint
         needs (node block, kinematic nodal velocity, sort by cell).  A 2D
         simulation or ensemble steps between two private particle blocks (its
-        own block is not written; a 1D one on one private block).  prev: for
-        a saved step, the node block whose velocity rows (what a massless node
-        keeps) and damping-force rows the step reads (default: the last saved
-        step's block when chained, else the simulation's node block)."""
+        own block is not written; a 1D one on one private block).  prev: the
+        node block whose velocity rows (what a massless node keeps) and
+        damping-force rows the step reads (default: the last step's block when
+        the step continues it, else the simulation's node block)."""
         if self.dim == 1:
             return self._forward_1d(state, log_e, gravity, i), None
         if self._blocks is None:
@@ -659,9 +659,11 @@ def rollout(d, nsteps, gravity, state=None, log_e=None, start_step=0, checkpoint
     reaches it.  With ``graphs=True`` (2D, checkpointed) each segment's steps
     run as HIP graph replays, captured on first use per segment length (and
     start step, when nodal forces depend on time), with the same results.  A
-    rollout does not write the simulation's (or the ensemble's) own particle
-    block: its steps alternate between two private blocks.  Use the returned
-    tensors."""
+    rollout writes neither the simulation's (or the ensemble's) particle block
+    nor its node block: its steps alternate between private particle blocks
+    and write private node blocks (its first step reads the velocity a
+    massless node keeps and the damping force from the simulation's nodes).
+    Use the returned tensors."""
     if state is None:
         state = d.get_state()
     if not torch.is_grad_enabled() or not (
