@@ -300,10 +300,10 @@ int dmpm_step_between(dmpm_plan* plan, const void* pbuf_in, void* pbuf_out, int3
 /* One step of a differentiable rollout that keeps what its adjoint needs:
  * dmpm_step_between from pbuf_in to pbuf_out (same flags, except that bit 0,
  * temporaries, is rejected; same state tensors) that writes every node row of
- * nout (mass, forces, velocity, acceleration, momentum; its damping-force
- * rows are copied from nprev; its coordinate rows are not written), keeps the
- * velocity of nprev's velocity rows at massless nodes (nprev: the previous
- * step's node block, read only), and writes the step's kinematic nodal
+ * nout (mass, forces, velocity, acceleration, momentum, and the damping force,
+ * which the step reads from nprev; its coordinate rows are not written),
+ * keeps the velocity of nprev's velocity rows at massless nodes (nprev: the
+ * previous step's node block, read only), and writes the step's kinematic nodal
  * velocity to vkin_out [2][nnodes] (particle dtype) and its sort by cell to
  * perm_out [nparticles] and cs_out [ncells+2], as dmpm_plan_export_step
  * would.  Results equal dmpm_step_between on nout holding nprev's rows
