@@ -87,3 +87,42 @@ def test_abi_version_and_errors():
     assert L.lib().dmpm_abi_version() == 101
     with pytest.raises(ValueError, match="element"):
         L.element_volume(7, 0, 1.0, 1.0)
+
+
+def header_declarations():
+    """{name: [parameter type, ...]} of every function the header declares."""
+    txt = re.sub(r"/\*.*?\*/", "", header_text(), flags=re.S)
+    out = {}
+    for name, params in re.findall(r"\b(?:int|const char\*)\s+(dmpm_\w+)\s*\(([^)]*)\)\s*;",
+                                   txt):
+        params = " ".join(params.split())
+        if params in ("", "void"):
+            out[name] = []
+            continue
+        # each parameter's type: the declaration without its trailing name
+        out[name] = [re.sub(r"\s*\b\w+$", "", p.strip()) for p in params.split(",")]
+    return out
+
+
+def test_binding_signatures_match_header():
+    """The ctypes argument list of every entry point has the header's arity,
+    and each argument the header's kind: a pointer, a 32- or 64-bit integer,
+    or a double (a mismatch would pass garbage through the ABI)."""
+    import ctypes as C
+    import diffmpm_amd._lib as L
+    decls = header_declarations()
+    assert set(L.SIGNATURES) <= set(decls)
+    for name, argt in L.SIGNATURES.items():
+        types = decls[name]
+        assert len(types) == len(argt), (name, types, argt)
+        for ty, ct in zip(types, argt):
+            if "*" in ty:
+                assert ct is C.c_void_p or issubclass(ct, C._Pointer), (name, ty, ct)
+            elif ty.endswith("int32_t"):
+                assert ct is C.c_int32, (name, ty, ct)
+            elif ty.endswith("int64_t"):
+                assert ct is C.c_int64, (name, ty, ct)
+            elif ty == "double":
+                assert ct is C.c_double, (name, ty, ct)
+            else:
+                raise AssertionError((name, ty))
